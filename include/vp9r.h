@@ -1,0 +1,262 @@
+/* vp9r.h: the C ABI of the MI355X reconstruction backend for rivet-vp9's VP9
+ * decoder.
+ *
+ * Three layers, all plain C types and pointers:
+ *   1. the decoder API, mirroring the crate's public surface
+ *      (Decoder::new/set_threads/threads/set_max_pixels/decode/decode_all/
+ *      frames_decoded and Frame, src/decoder/mod.rs:150-216, src/frame.rs);
+ *   2. the per-frame seam: the CPU parser's record (blocks, transform blocks,
+ *      quantised coefficients, mode info; src/decoder/block.rs:130-152) and
+ *      the GPU reconstruction of one record into an HBM-resident frame;
+ *   3. the per-kernel seam mirroring src/dsp's scalar kernel signatures, as
+ *      batched launches over device buffers (and one-call host forms for
+ *      tests).
+ *
+ * Status codes map onto the crate's Error kinds (src/lib.rs:67-81).
+ * The library fails loudly (VP9R_ERR_DEVICE) when no GPU is usable: there is
+ * no CPU fallback behind this ABI.
+ */
+#ifndef VP9R_H
+#define VP9R_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status (src/lib.rs:67-81: Error::{Bitstream, Unsupported, InvalidInput}) */
+
+enum {
+  VP9R_OK = 0,
+  VP9R_ERR_BITSTREAM = 1,
+  VP9R_ERR_UNSUPPORTED = 2,
+  VP9R_ERR_INVALID_INPUT = 3,
+  VP9R_ERR_DEVICE = 4, /* HIP error, or no GPU */
+};
+
+/* The message of the last failure on this thread ("" if none). */
+const char *vp9r_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* The per-frame record (what the CPU parser emits, what recon consumes).   */
+
+/* Header fields reconstruction reads (src/header.rs:88-146, 16-56). */
+typedef struct vp9r_frame_hdr {
+  uint32_t width, height;             /* FrameWidth, FrameHeight */
+  uint32_t mi_cols, mi_rows;          /* 8x8 units */
+  uint32_t sb64_cols, sb64_rows;      /* 64x64 units */
+  uint32_t render_width, render_height;
+  uint8_t profile, bit_depth, ss_x, ss_y;
+  uint8_t frame_type, show_frame, show_existing_frame, frame_to_show_map_idx;
+  uint8_t intra_only, frame_is_intra, error_resilient_mode, lossless;
+  uint8_t refresh_frame_flags, legacy_uv, interp_filter, tx_mode;
+  uint8_t ref_frame_idx[3], reference_mode;
+  uint8_t color_space, color_range, tile_cols_log2, tile_rows_log2;
+  int32_t base_q_idx, delta_q_y_dc, delta_q_uv_dc, delta_q_uv_ac;
+  /* loop filter (6.2.8) */
+  uint8_t lf_level, lf_sharpness, lf_delta_enabled, _pad0;
+  int8_t lf_ref_deltas[4];
+  int8_t lf_mode_deltas[2];
+  /* segmentation (6.2.11): the parts reconstruction reads */
+  uint8_t seg_enabled, seg_abs_or_delta_update;
+  uint8_t seg_feature_enabled[8][4];
+  int16_t seg_feature_data[8][4];
+  /* sizes of the active LAST/GOLDEN/ALTREF references (inter frames) */
+  uint32_t ref_width[3], ref_height[3];
+} vp9r_frame_hdr;
+
+/* One coded block as the residual syntax sees it (block.rs:51-76). */
+typedef struct vp9r_block {
+  uint16_t mi_row, mi_col;
+  uint8_t mi_size, skip, tx_size, is_inter;
+  uint8_t y_mode, uv_mode, interp_filter, segment_id;
+  uint8_t sub_modes[4];
+  int8_t ref_frame[2];
+  uint8_t avail_u, avail_l;
+  int32_t mv[2][4][2]; /* block_mvs[list][sub-block][row, col], 1/8 sample */
+} vp9r_block;
+
+/* One transform block in reconstruction order: every transform block of an
+ * intra block that starts inside the frame, and every transform block of an
+ * inter block with eob > 0 (block.rs:1314-1364). */
+typedef struct vp9r_txb {
+  uint16_t x, y;        /* plane sample position of the top-left */
+  uint8_t plane, tx_size, tx_type, _pad;
+  uint16_t eob;         /* 0: prediction only */
+  uint16_t ncoef;       /* nonzero quantised coefficients at coef_start */
+  uint32_t coef_start;
+  uint32_t block;       /* index of the owning vp9r_block */
+} vp9r_txb;
+
+/* What the loop filter reads of each 8x8 position (block.rs:19-31). */
+typedef struct vp9r_mi {
+  uint8_t mi_size, skip, tx_size, y_mode;
+  uint8_t segment_id;
+  int8_t ref_frame0;
+  uint8_t _pad[2];
+} vp9r_mi;
+
+typedef struct vp9r_record {
+  vp9r_frame_hdr hdr;
+  const vp9r_block *blocks;
+  uint32_t n_blocks;
+  const vp9r_txb *txbs;
+  uint32_t n_txbs;
+  const uint16_t *coef_pos; /* raster position in the n x n block */
+  const int32_t *coef_val;  /* quantised value (dequantised by recon) */
+  uint32_t n_coefs;
+  const vp9r_mi *mi;        /* mi_rows * mi_cols, row-major */
+} vp9r_record;
+
+/* ------------------------------------------------------------------------ */
+/* Containers (src/superframe.rs:6-52, src/ivf.rs).                          */
+
+/* Splits a packet into frames (Annex B superframe index).  Writes up to
+ * `max` (offset, size) pairs; returns the frame count. */
+size_t vp9r_superframe_split(const uint8_t *packet, size_t len, size_t *offsets, size_t *sizes,
+                             size_t max);
+
+/* ------------------------------------------------------------------------ */
+/* CPU parser (headers, bool decoder, probabilities, block syntax, motion   */
+/* vector prediction, tokens).  Stays on the CPU; touches no GPU.           */
+
+typedef struct vp9r_parser vp9r_parser;
+
+vp9r_parser *vp9r_parser_new(void);
+void vp9r_parser_free(vp9r_parser *p);
+/* Parses one frame (not a superframe).  On VP9R_OK `out` points into the
+ * parser's buffers, valid until the next call.  A show_existing_frame frame
+ * gives hdr.show_existing_frame = 1 and no blocks. */
+int vp9r_parser_parse_frame(vp9r_parser *p, const uint8_t *data, size_t len, vp9r_record *out);
+/* decoder/mod.rs:183-188 */
+void vp9r_parser_set_max_pixels(vp9r_parser *p, uint64_t pixels);
+
+/* ------------------------------------------------------------------------ */
+/* GPU reconstruction (one device per context).                              */
+
+typedef struct vp9r_gpu vp9r_gpu;
+typedef struct vp9r_gpu_frame vp9r_gpu_frame; /* an HBM-resident frame */
+
+/* Creates a context on HIP device `device`.  Fails with VP9R_ERR_DEVICE when
+ * there is no usable GPU. */
+int vp9r_gpu_new(int device, vp9r_gpu **out);
+void vp9r_gpu_free(vp9r_gpu *g);
+
+/* A frame store of the record's geometry (planes of whole superblocks:
+ * decoder/mod.rs:27-61), u8 samples at 8 bits, u16 above. */
+int vp9r_gpu_frame_new(vp9r_gpu *g, uint32_t width, uint32_t height, uint32_t bit_depth,
+                       uint32_t ss_x, uint32_t ss_y, vp9r_gpu_frame **out);
+void vp9r_gpu_frame_free(vp9r_gpu_frame *f);
+/* Uploads / downloads whole planes (row-major, stride = plane stride, u16
+ * samples regardless of bit depth) for tests. */
+int vp9r_gpu_frame_upload_u16(vp9r_gpu_frame *f, int plane, const uint16_t *src);
+int vp9r_gpu_frame_download_u16(vp9r_gpu_frame *f, int plane, uint16_t *dst);
+void vp9r_gpu_frame_geometry(const vp9r_gpu_frame *f, int plane, uint32_t *stride, uint32_t *rows);
+
+/* Reconstructs the record into `cur` (prediction, dequantisation, inverse
+ * transforms, intra wavefront) and applies the loop filter.  refs[i] is the
+ * LAST/GOLDEN/ALTREF frame (NULL for intra frames).  Asynchronous on the
+ * context's stream; the record's host buffers may be reused on return. */
+int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *const refs[3],
+                         vp9r_gpu_frame *cur);
+/* Writes the visible frame in the crate's Frame::packed layout (Y, U, V; u8
+ * at 8 bits, little-endian u16 above; frame.rs:112-229) to `dst`, which
+ * holds vp9r_packed_size() bytes.  Waits for the frame. */
+int vp9r_gpu_download_packed(vp9r_gpu *g, vp9r_gpu_frame *f, uint8_t *dst);
+size_t vp9r_packed_size(uint32_t width, uint32_t height, uint32_t bit_depth, uint32_t ss_x,
+                        uint32_t ss_y);
+int vp9r_gpu_synchronize(vp9r_gpu *g);
+/* Per-stage device time of the last vp9r_gpu_reconstruct, in ms (HIP events
+ * on the context's stream): [0] inter prediction, [1] inter residual,
+ * [2] intra wavefront, [3] loop filter, [4] whole call.  Only filled when
+ * timing was enabled. */
+void vp9r_gpu_set_timing(vp9r_gpu *g, int enable);
+int vp9r_gpu_stage_times(vp9r_gpu *g, float out[5]);
+
+/* ------------------------------------------------------------------------ */
+/* Per-kernel seam: src/dsp signatures as batched device launches.          */
+
+/* Dequantise-free inverse transform + add (itx::inverse_transform_add,
+ * src/dsp/itx.rs:701-749), batched: block i has its n x n dequantised
+ * coefficients at coefs + coef_off[i] (row-major, int32), size n = 4 << tx[i],
+ * tx_type[i], eob[i]; its destination at dst + dst_off[i] (samples of
+ * `bit_depth`: u8 when 8, else u16) with row stride `stride`.  All pointers
+ * are device pointers; runs on the context's stream. */
+int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,
+                       const uint8_t *tx, const uint8_t *tx_type, const uint16_t *eob,
+                       const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
+                       int lossless);
+
+/* Motion compensation (inter::predict, src/dsp/inter.rs:289-395, and
+ * pixel::avg, src/dsp/pixel.rs:22-50), batched.  Block i predicts w x h
+ * samples at 1/16-sample position (x, y) with steps (xs, ys) from reference
+ * plane ref_idx of the `refs` array (device plane pointers, all with the same
+ * stride `ref_stride` and last_x/last_y clamp), and, when ref2_idx >= 0,
+ * averages with a second prediction at (x2, y2).  Writes to out + out_off[i]
+ * with stride out_stride. */
+typedef struct vp9r_mc_desc {
+  int32_t x, y, x2, y2;
+  int16_t xs, ys;
+  uint8_t w, h;
+  uint8_t filter;
+  int8_t ref_idx, ref2_idx;
+  uint8_t _pad;
+  uint64_t out_off;
+} vp9r_mc_desc;
+int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc,
+                  const void *const *refs, uint32_t n_refs, uint32_t ref_stride, int32_t last_x,
+                  int32_t last_y, void *out, uint32_t out_stride, uint32_t bit_depth);
+
+/* Loop filter sample kernels (lf::filter_edges, src/dsp/lf.rs:178-218) for a
+ * batch of superblock passes: pass i filters edges [0, n_edges[i]) of the
+ * region at (x0[i], y0[i]) with the Edges tables at edges + 1024 * i
+ * (fs[16][16] int8, then limit, blimit, thresh [16][16] uint8, unshifted).
+ * Passes run in order (each waits for the previous one), as the reference
+ * calls them. */
+int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const uint16_t *x0,
+                        const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
+                        const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t bit_depth);
+
+/* One-call host forms of the dsp kernels, for parity tests (each copies to
+ * the device, launches, and copies back). */
+int vp9r_itx_add_host(vp9r_gpu *g, const int32_t *coefs, uint32_t n, uint32_t tx_type, int lossless,
+                      uint32_t eob, uint32_t bit_depth, uint16_t *dst, uint32_t stride);
+int vp9r_intra_predict_host(vp9r_gpu *g, uint32_t mode, uint32_t log2, const int32_t *above,
+                            const int32_t *left, int have_left, int have_above, uint32_t bit_depth,
+                            uint16_t *dst, uint32_t stride);
+
+/* ------------------------------------------------------------------------ */
+/* Decoder API (src/decoder/mod.rs:150-216).                                 */
+
+typedef struct vp9r_decoder vp9r_decoder;
+
+/* A decoded, shown frame (frame.rs:112-137): packed Y, U, V. */
+typedef struct vp9r_frame {
+  uint32_t width, height, bit_depth;
+  uint32_t ss_x, ss_y;
+  uint32_t color_space, full_range, render_width, render_height;
+  uint8_t *data; /* owned by the decoder, valid until the next decode call */
+  size_t size;
+  size_t plane_offset[3];
+  uint32_t plane_width[3], plane_height[3];
+} vp9r_frame;
+
+int vp9r_decoder_new(int device, vp9r_decoder **out);
+void vp9r_decoder_free(vp9r_decoder *d);
+void vp9r_decoder_set_threads(vp9r_decoder *d, uint32_t threads);
+uint32_t vp9r_decoder_threads(const vp9r_decoder *d);
+void vp9r_decoder_set_max_pixels(vp9r_decoder *d, uint64_t pixels);
+/* decode_all: decodes one packet; returns the number of shown frames in
+ * *n_frames and their descriptions through vp9r_decoder_frame(). */
+int vp9r_decoder_decode(vp9r_decoder *d, const uint8_t *packet, size_t len, uint32_t *n_frames);
+int vp9r_decoder_frame(const vp9r_decoder *d, uint32_t i, vp9r_frame *out);
+uint64_t vp9r_decoder_frames_decoded(const vp9r_decoder *d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,62 @@
+"""Builds libvp9r.so in-tree: the C++ host side (parser, C ABI, frame
+orchestration) and the HIP kernels for gfx950, with hipcc.
+
+    python3 rivet-vp9_amd/build.py [-j N]
+
+Every source is compiled on every call (the build is also the "does it
+build" check); objects go to rivet-vp9_amd/_build/.
+"""
+import argparse
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libvp9r.so")
+
+HOST_SOURCES = [
+    "csrc/host/parser.cpp",
+    "csrc/host/capi_parser.cpp",
+]
+HIP_SOURCES = [
+]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-missing-braces", "-Wno-unused-function",
+          "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc")]
+# gfx950 device code; packed-FP32 instructions disabled.
+GPU = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+       "-munsafe-fp-atomics"]
+
+
+def compile_one(src, is_hip):
+    obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
+    cmd = [HIPCC] + COMMON + (GPU + ["-x", "hip"] if is_hip else []) + ["-c", os.path.join(HERE, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(jobs=8):
+    os.makedirs(BUILD, exist_ok=True)
+    work = [(s, False) for s in HOST_SOURCES] + [(s, True) for s in HIP_SOURCES]
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda a: compile_one(*a), work))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC"] + (GPU if HIP_SOURCES else []) + objs + ["-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=8)
+    a = ap.parse_args()
+    print(build(a.j))

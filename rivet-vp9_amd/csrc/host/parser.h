@@ -1,0 +1,205 @@
+// The CPU side of the decoder: everything that turns a VP9 frame into the
+// per-frame reconstruction record without touching samples (uncompressed
+// and compressed headers, the boolean decoder, probability contexts and
+// backward adaptation, partition/mode/motion-vector/token syntax, the
+// segmentation map and the previous frame's motion vectors).
+//
+// It follows the crate's decoder: src/header.rs, src/bool_coder.rs,
+// src/probs.rs, src/decoder/block.rs (parse half), src/decoder/mvpred.rs and
+// the frame process of src/decoder/mod.rs:218-422, minus the sample work,
+// which the record hands to the GPU.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "vp9r.h"
+
+namespace vp9r {
+
+struct ParseError {
+  int code;
+  std::string msg;
+};
+
+typedef int32_t Mv[2];
+
+// One frame context (src/probs.rs:13-86).
+struct Probs {
+  uint8_t tx[4][2][3];
+  uint8_t coef[4][2][2][6][6][3];
+  uint8_t skip[3];
+  uint8_t inter_mode[7][3];
+  uint8_t interp_filter[4][2];
+  uint8_t is_inter[4];
+  uint8_t comp_mode[5];
+  uint8_t single_ref[5][2];
+  uint8_t comp_ref[5];
+  uint8_t y_mode[4][9];
+  uint8_t uv_mode[10][9];
+  uint8_t partition[16][3];
+  uint8_t mv_joint[3];
+  uint8_t mv_sign[2];
+  uint8_t mv_class[2][10];
+  uint8_t mv_class0_bit[2];
+  uint8_t mv_bits[2][10];
+  uint8_t mv_class0_fr[2][2][3];
+  uint8_t mv_fr[2][3];
+  uint8_t mv_class0_hp[2];
+  uint8_t mv_hp[2];
+  void set_defaults();
+};
+
+// Symbol counts of section 8.3 (src/probs.rs:88-118).
+struct Counts {
+  uint32_t intra_mode[4][10];
+  uint32_t uv_mode[10][10];
+  uint32_t partition[16][4];
+  uint32_t interp_filter[4][3];
+  uint32_t inter_mode[7][4];
+  uint32_t tx[4][2][4];
+  uint32_t is_inter[4][2];
+  uint32_t comp_mode[5][2];
+  uint32_t single_ref[5][2][2];
+  uint32_t comp_ref[5][2];
+  uint32_t skip[3][2];
+  uint32_t mv_joint[4];
+  uint32_t mv_sign[2][2];
+  uint32_t mv_class[2][11];
+  uint32_t mv_class0_bit[2][2];
+  uint32_t mv_class0_fr[2][2][4];
+  uint32_t mv_class0_hp[2][2];
+  uint32_t mv_bits[2][10][2];
+  uint32_t mv_fr[2][4];
+  uint32_t mv_hp[2][2];
+  uint32_t token[4][2][2][6][6][3];
+  uint32_t more_coefs[4][2][2][6][6][2];
+};
+
+struct LoopFilterParams {
+  uint8_t level = 0, sharpness = 0;
+  bool delta_enabled = true;
+  int8_t ref_deltas[4] = {1, 0, -1, -1};
+  int8_t mode_deltas[2] = {0, 0};
+};
+
+struct SegParams {
+  bool enabled = false, update_map = false, temporal_update = false, abs_or_delta_update = false;
+  uint8_t tree_probs[7] = {255, 255, 255, 255, 255, 255, 255};
+  uint8_t pred_probs[3] = {255, 255, 255};
+  bool feature_enabled[8][4] = {};
+  int16_t feature_data[8][4] = {};
+  bool active(int seg, int feature) const { return enabled && feature_enabled[seg][feature]; }
+};
+
+// Everything the uncompressed and compressed headers say about a frame.
+struct FrameHeader {
+  uint32_t profile = 0;
+  bool show_existing_frame = false;
+  uint32_t frame_to_show_map_idx = 0;
+  uint32_t frame_type = 0, last_frame_type = 0;
+  bool show_frame = false, error_resilient_mode = false, intra_only = false;
+  uint32_t reset_frame_context = 0;
+  uint32_t bit_depth = 8, color_space = 2;
+  bool color_range = false;
+  uint32_t ss_x = 1, ss_y = 1;
+  uint32_t refresh_frame_flags = 0;
+  uint32_t ref_frame_idx[3] = {0, 0, 0};
+  bool ref_frame_sign_bias[4] = {false, false, false, false};
+  uint32_t width = 0, height = 0, render_width = 0, render_height = 0;
+  bool allow_high_precision_mv = false;
+  uint32_t interp_filter = 0;
+  bool refresh_frame_context = false, frame_parallel_decoding_mode = false;
+  uint32_t frame_context_idx = 0;
+  int32_t base_q_idx = 0, delta_q_y_dc = 0, delta_q_uv_dc = 0, delta_q_uv_ac = 0;
+  bool lossless = false;
+  uint32_t tile_cols_log2 = 0, tile_rows_log2 = 0;
+  uint32_t header_size_in_bytes = 0;
+  size_t uncompressed_size = 0;
+  bool past_independence = false;
+  bool frame_is_intra = false;
+  uint32_t mi_cols = 0, mi_rows = 0, sb64_cols = 0, sb64_rows = 0;
+  uint32_t tx_mode = 0, reference_mode = 0;
+  int32_t comp_fixed_ref = 0;
+  int32_t comp_var_ref[2] = {0, 0};
+  bool legacy_uv = false;
+  void compute_image_size() {
+    mi_cols = (width + 7) >> 3;
+    mi_rows = (height + 7) >> 3;
+    sb64_cols = (mi_cols + 7) >> 3;
+    sb64_rows = (mi_rows + 7) >> 3;
+  }
+};
+
+// What the frame remembers of each 8x8 position (block.rs:19-31).
+struct MiInfo {
+  uint8_t mi_size = 0, skip = 0, tx_size = 0, y_mode = 0;
+  uint8_t sub_modes[4] = {0, 0, 0, 0};
+  uint8_t segment_id = 0, interp_filter = 0;
+  int8_t ref_frame[2] = {0, -1};
+  int32_t mv[2][4][2] = {};
+};
+
+struct PrevMv {
+  int8_t ref_frame[2] = {0, 0};
+  int32_t mv[2][2] = {};
+};
+
+// Format of a reference slot, as the parser tracks it (no samples).
+struct SlotInfo {
+  bool valid = false;
+  uint32_t width = 0, height = 0, ss_x = 0, ss_y = 0, bit_depth = 0;
+};
+
+class Parser {
+ public:
+  Parser();
+  // Parses one frame into the record (see vp9r.h); throws ParseError.
+  void parse_frame(const uint8_t *data, size_t len, vp9r_record *out);
+  uint64_t max_pixels = 8192ull * 8192ull;
+  uint64_t frames = 0;
+
+ private:
+  // persistent state (header.rs:58-84, decoder/mod.rs:122-139)
+  LoopFilterParams lf_;
+  SegParams seg_;
+  uint32_t ref_sizes_[8][2] = {};
+  uint32_t st_last_frame_type_ = KEY_FRAME_;
+  uint32_t st_bit_depth_ = 8, st_ss_x_ = 1, st_ss_y_ = 1, st_color_space_ = 2;
+  bool st_color_range_ = false;
+  Probs contexts_[4];
+  SlotInfo slots_[8];
+  std::vector<uint8_t> prev_segment_ids_;
+  std::vector<PrevMv> prev_mvs_;
+  bool have_last_size_ = false;
+  uint32_t last_w_ = 0, last_h_ = 0;
+  bool last_show_frame_ = false;
+  bool legacy_uv_ = false;
+  static constexpr uint32_t KEY_FRAME_ = 0;
+
+  // the frame being parsed
+  FrameHeader h_;
+  Probs probs_;
+  Counts counts_;
+  std::vector<MiInfo> mi_;
+  bool use_prev_mvs_ = false;
+  uint32_t reset_idx_ = 0;
+  uint32_t ref_ok_[3] = {0, 0, 0};  // 1 if the active reference may be used
+
+  // record
+  std::vector<vp9r_block> blocks_;
+  std::vector<vp9r_txb> txbs_;
+  std::vector<uint16_t> coef_pos_;
+  std::vector<int32_t> coef_val_;
+  std::vector<vp9r_mi> lf_mi_;
+
+  void parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind);
+  void parse_compressed(const uint8_t *data, size_t len);
+  // returns true if every tile's padding was zero
+  bool decode_tiles(const uint8_t *data, size_t len);
+  void fill_record(vp9r_record *out);
+  friend class TileDecoder;
+};
+
+}  // namespace vp9r
