@@ -156,12 +156,17 @@ int vp9r_gpu_frame_upload_u16(vp9r_gpu_frame *f, int plane, const uint16_t *src)
 int vp9r_gpu_frame_download_u16(vp9r_gpu_frame *f, int plane, uint16_t *dst);
 void vp9r_gpu_frame_geometry(const vp9r_gpu_frame *f, int plane, uint32_t *stride, uint32_t *rows);
 
-/* Reconstructs the record into `cur` (prediction, dequantisation, inverse
- * transforms, intra wavefront) and applies the loop filter.  refs[i] is the
+/* Reconstructs the record into `cur`: inter prediction, dequantisation and
+ * inverse transforms, the intra wavefront (the replay of
+ * src/decoder/block.rs:1405-1498 with src/decoder/recon.rs).  refs[i] is the
  * LAST/GOLDEN/ALTREF frame (NULL for intra frames).  Asynchronous on the
  * context's stream; the record's host buffers may be reused on return. */
 int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *const refs[3],
                          vp9r_gpu_frame *cur);
+/* The loop filter of the reconstructed frame (loopfilter::filter_frame,
+ * src/decoder/loopfilter.rs:51-81) from the record's mode info and header.
+ * Asynchronous, like vp9r_gpu_reconstruct. */
+int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cur);
 /* Writes the visible frame in the crate's Frame::packed layout (Y, U, V; u8
  * at 8 bits, little-endian u16 above; frame.rs:112-229) to `dst`, which
  * holds vp9r_packed_size() bytes.  Waits for the frame. */
@@ -182,9 +187,11 @@ int vp9r_gpu_stage_times(vp9r_gpu *g, float out[5]);
 /* Dequantise-free inverse transform + add (itx::inverse_transform_add,
  * src/dsp/itx.rs:701-749), batched: block i has its n x n dequantised
  * coefficients at coefs + coef_off[i] (row-major, int32), size n = 4 << tx[i],
- * tx_type[i], eob[i]; its destination at dst + dst_off[i] (samples of
- * `bit_depth`: u8 when 8, else u16) with row stride `stride`.  All pointers
- * are device pointers; runs on the context's stream. */
+ * tx_type[i], eob[i]; its destination at dst + dst_off[i] samples (u8 when
+ * bit_depth is 8, else u16) with row stride `stride`.  `tx` is a host array
+ * (the size classes are formed on the host); every other pointer is a device
+ * pointer.  lossless selects the WHT and needs tx[i] == 0.  Runs on the
+ * context's stream. */
 int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,
                        const uint8_t *tx, const uint8_t *tx_type, const uint16_t *eob,
                        const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
@@ -192,11 +199,14 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
 
 /* Motion compensation (inter::predict, src/dsp/inter.rs:289-395, and
  * pixel::avg, src/dsp/pixel.rs:22-50), batched.  Block i predicts w x h
- * samples at 1/16-sample position (x, y) with steps (xs, ys) from reference
- * plane ref_idx of the `refs` array (device plane pointers, all with the same
- * stride `ref_stride` and last_x/last_y clamp), and, when ref2_idx >= 0,
- * averages with a second prediction at (x2, y2).  Writes to out + out_off[i]
- * with stride out_stride. */
+ * samples (w, h in 4..64) at 1/16-sample position (x, y) with steps
+ * (xs, ys) in 1..32 from reference plane ref_idx of the `refs` array (device
+ * plane pointers, all with row stride `ref_stride` and bottom-right sample
+ * (last_x, last_y)), and, when ref2_idx >= 0, averages with a second
+ * prediction at (x2, y2).  Writes to out + out_off[i] samples with stride
+ * out_stride.  `desc` is a host array (the wrapper expands it into 8 x 8
+ * tiles); `refs` is a host array of device pointers; `out` is a device
+ * pointer. */
 typedef struct vp9r_mc_desc {
   int32_t x, y, x2, y2;
   int16_t xs, ys;
@@ -211,14 +221,28 @@ int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc,
                   int32_t last_y, void *out, uint32_t out_stride, uint32_t bit_depth);
 
 /* Loop filter sample kernels (lf::filter_edges, src/dsp/lf.rs:178-218) for a
- * batch of superblock passes: pass i filters edges [0, n_edges[i]) of the
- * region at (x0[i], y0[i]) with the Edges tables at edges + 1024 * i
- * (fs[16][16] int8, then limit, blimit, thresh [16][16] uint8, unshifted).
- * Passes run in order (each waits for the previous one), as the reference
- * calls them. */
+ * sequence of superblock passes over one plane: pass i filters edges
+ * [0, n_edges[i]) (n_runs[i] runs) of the region at (x0[i], y0[i]) with the
+ * Edges tables at edges + 1024 * i (fs[16][16] int8, -1 = none, then
+ * limit, blimit, thresh [16][16] uint8, unshifted).  Passes run in order, as
+ * the reference calls them.  `buf` is a device plane of `rows` rows of
+ * `stride` samples (u8 at 8 bits, else u16); every pass's reach (8 samples
+ * before its first edge) must lie inside it.  The other arrays are host
+ * arrays. */
 int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const uint16_t *x0,
                         const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
-                        const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t bit_depth);
+                        const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t rows,
+                        uint32_t bit_depth);
+
+/* Device memory for callers of the batched forms (no counterpart in the
+ * reference: a plain-pointer ABI needs its own allocation and copies).  The
+ * copies are ordered on the context's stream and wait for completion. */
+int vp9r_gpu_malloc(vp9r_gpu *g, size_t bytes, void **out);
+void vp9r_gpu_mfree(vp9r_gpu *g, void *p);
+int vp9r_gpu_to_device(vp9r_gpu *g, void *dst, const void *src, size_t bytes);
+int vp9r_gpu_to_host(vp9r_gpu *g, void *dst, const void *src, size_t bytes);
+/* The device address of a frame store's plane (for the batched forms). */
+void *vp9r_gpu_frame_plane(vp9r_gpu_frame *f, int plane);
 
 /* One-call host forms of the dsp kernels, for parity tests (each copies to
  * the device, launches, and copies back). */

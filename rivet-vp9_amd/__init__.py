@@ -141,6 +141,41 @@ def library():
     lib.vp9r_parser_parse_frame.argtypes = [P, ctypes.c_char_p, sz, ctypes.POINTER(Record)]
     lib.vp9r_packed_size.restype = sz
     lib.vp9r_packed_size.argtypes = [u32, u32, u32, u32, u32]
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    lib.vp9r_gpu_new.argtypes = [ctypes.c_int, PP]
+    lib.vp9r_gpu_free.argtypes = [P]
+    lib.vp9r_gpu_frame_new.argtypes = [P, u32, u32, u32, u32, u32, PP]
+    lib.vp9r_gpu_frame_free.argtypes = [P]
+    lib.vp9r_gpu_frame_upload_u16.argtypes = [P, ctypes.c_int, V]
+    lib.vp9r_gpu_frame_download_u16.argtypes = [P, ctypes.c_int, V]
+    lib.vp9r_gpu_frame_geometry.argtypes = [P, ctypes.c_int, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+    lib.vp9r_gpu_reconstruct.argtypes = [P, ctypes.POINTER(Record), ctypes.c_void_p * 3, P]
+    lib.vp9r_gpu_loop_filter.argtypes = [P, ctypes.POINTER(Record), P]
+    lib.vp9r_gpu_download_packed.argtypes = [P, P, V]
+    lib.vp9r_gpu_synchronize.argtypes = [P]
+    lib.vp9r_gpu_set_timing.argtypes = [P, ctypes.c_int]
+    lib.vp9r_gpu_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+    lib.vp9r_itx_add_batch.argtypes = [P, u32, V, V, V, V, V, V, V, u32, u32, ctypes.c_int]
+    lib.vp9r_mc_batch.argtypes = [P, u32, V, V, u32, u32, i32, i32, V, u32, u32]
+    lib.vp9r_lf_edges_batch.argtypes = [P, u32, V, V, V, V, V, V, V, u32, u32, u32]
+    lib.vp9r_itx_add_host.argtypes = [P, V, u32, u32, ctypes.c_int, u32, u32, V, u32]
+    lib.vp9r_intra_predict_host.argtypes = [P, u32, u32, V, V, ctypes.c_int, ctypes.c_int, u32, V, u32]
+    lib.vp9r_gpu_malloc.argtypes = [P, sz, PP]
+    lib.vp9r_gpu_mfree.argtypes = [P, V]
+    lib.vp9r_gpu_to_device.argtypes = [P, V, V, sz]
+    lib.vp9r_gpu_to_host.argtypes = [P, V, V, sz]
+    lib.vp9r_gpu_frame_plane.restype = V
+    lib.vp9r_gpu_frame_plane.argtypes = [P, ctypes.c_int]
+    lib.vp9r_decoder_new.argtypes = [ctypes.c_int, PP]
+    lib.vp9r_decoder_free.argtypes = [P]
+    lib.vp9r_decoder_set_threads.argtypes = [P, u32]
+    lib.vp9r_decoder_threads.restype = u32
+    lib.vp9r_decoder_threads.argtypes = [P]
+    lib.vp9r_decoder_set_max_pixels.argtypes = [P, u64]
+    lib.vp9r_decoder_decode.argtypes = [P, ctypes.c_char_p, sz, ctypes.POINTER(u32)]
+    lib.vp9r_decoder_frame.argtypes = [P, u32, ctypes.POINTER(FrameDesc)]
+    lib.vp9r_decoder_frames_decoded.restype = u64
+    lib.vp9r_decoder_frames_decoded.argtypes = [P]
     _lib = lib
     return lib
 
@@ -197,3 +232,192 @@ def ivf_frames(data):
         out.append(data[pos:pos + size])
         pos += size
     return out
+
+
+class Gpu:
+    """A GPU reconstruction context on one HIP device (vp9r_gpu_*)."""
+
+    def __init__(self, device=0):
+        self._g = ctypes.c_void_p()
+        _check(library().vp9r_gpu_new(device, ctypes.byref(self._g)))
+
+    def __del__(self):
+        if getattr(self, "_g", None):
+            library().vp9r_gpu_free(self._g)
+            self._g = None
+
+    @property
+    def handle(self):
+        return self._g
+
+    def frame(self, width, height, bit_depth, ss_x, ss_y):
+        return GpuFrame(self, width, height, bit_depth, ss_x, ss_y)
+
+    def reconstruct(self, rec, refs, cur):
+        arr = (ctypes.c_void_p * 3)(*[(r.handle if r is not None else None) for r in refs])
+        _check(library().vp9r_gpu_reconstruct(self._g, ctypes.byref(rec), arr, cur.handle))
+
+    def loop_filter(self, rec, cur):
+        _check(library().vp9r_gpu_loop_filter(self._g, ctypes.byref(rec), cur.handle))
+
+    def synchronize(self):
+        _check(library().vp9r_gpu_synchronize(self._g))
+
+    def set_timing(self, on):
+        library().vp9r_gpu_set_timing(self._g, 1 if on else 0)
+
+    def stage_times(self):
+        out = (ctypes.c_float * 5)()
+        _check(library().vp9r_gpu_stage_times(self._g, out))
+        return list(out)
+
+    def buffer(self, nbytes):
+        return DevBuffer(self, nbytes)
+
+    def upload(self, arr):
+        """A device copy of a numpy array (a DevBuffer)."""
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        b = DevBuffer(self, a.nbytes)
+        _check(library().vp9r_gpu_to_device(self._g, b.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+        return b
+
+
+class DevBuffer:
+    """Device memory from vp9r_gpu_malloc."""
+
+    def __init__(self, gpu, nbytes):
+        self.gpu = gpu
+        self.nbytes = nbytes
+        p = ctypes.c_void_p()
+        _check(library().vp9r_gpu_malloc(gpu.handle, nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            library().vp9r_gpu_mfree(self.gpu.handle, self.ptr)
+            self.ptr = None
+
+    def read(self, dtype, count):
+        import numpy as np
+        out = np.empty(count, dtype=dtype)
+        _check(library().vp9r_gpu_to_host(self.gpu.handle, out.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                                          out.nbytes))
+        return out
+
+    def write(self, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        _check(library().vp9r_gpu_to_device(self.gpu.handle, self.ptr, a.ctypes.data_as(ctypes.c_void_p),
+                                            a.nbytes))
+
+
+class GpuFrame:
+    """An HBM-resident frame store."""
+
+    def __init__(self, gpu, width, height, bit_depth, ss_x, ss_y):
+        self.gpu = gpu
+        self.width, self.height, self.bit_depth, self.ss_x, self.ss_y = width, height, bit_depth, ss_x, ss_y
+        self._f = ctypes.c_void_p()
+        _check(library().vp9r_gpu_frame_new(gpu.handle, width, height, bit_depth, ss_x, ss_y,
+                                            ctypes.byref(self._f)))
+
+    def __del__(self):
+        if getattr(self, "_f", None):
+            library().vp9r_gpu_frame_free(self._f)
+            self._f = None
+
+    @property
+    def handle(self):
+        return self._f
+
+    def plane_ptr(self, plane):
+        return library().vp9r_gpu_frame_plane(self._f, plane)
+
+    def geometry(self, plane):
+        s, r = u32(), u32()
+        library().vp9r_gpu_frame_geometry(self._f, plane, ctypes.byref(s), ctypes.byref(r))
+        return s.value, r.value
+
+    def packed(self):
+        n = library().vp9r_packed_size(self.width, self.height, self.bit_depth, self.ss_x, self.ss_y)
+        buf = (ctypes.c_uint8 * n)()
+        _check(library().vp9r_gpu_download_packed(self.gpu.handle, self._f, buf))
+        return bytes(buf)
+
+    def download_plane(self, plane):
+        """The whole plane (stride x rows) as a list-like array of u16."""
+        import numpy as np
+        s, r = self.geometry(plane)
+        out = np.empty((r, s), dtype=np.uint16)
+        _check(library().vp9r_gpu_frame_download_u16(self._f, plane, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def upload_plane(self, plane, arr):
+        import numpy as np
+        s, r = self.geometry(plane)
+        a = np.ascontiguousarray(arr, dtype=np.uint16)
+        assert a.shape == (r, s)
+        _check(library().vp9r_gpu_frame_upload_u16(self._f, plane, a.ctypes.data_as(ctypes.c_void_p)))
+
+
+class Frame:
+    """A decoded frame (src/frame.rs:112-137): packed Y, U, V."""
+
+    def __init__(self, desc):
+        self.width, self.height, self.bit_depth = desc.width, desc.height, desc.bit_depth
+        self.ss_x, self.ss_y = desc.ss_x, desc.ss_y
+        self.color_space, self.full_range = desc.color_space, bool(desc.full_range)
+        self.render_width, self.render_height = desc.render_width, desc.render_height
+        self.data = ctypes.string_at(desc.data, desc.size)
+        self.planes = [(desc.plane_offset[i], desc.plane_width[i], desc.plane_height[i]) for i in range(3)]
+
+    def packed(self):
+        return self.data
+
+    def bytes_per_sample(self):
+        return 2 if self.bit_depth > 8 else 1
+
+    def plane(self, i):
+        off, w, h = self.planes[i]
+        return self.data[off:off + w * h * self.bytes_per_sample()]
+
+
+class Decoder:
+    """The crate's Decoder (src/decoder/mod.rs:150-216) on the GPU backend."""
+
+    def __init__(self, device=0):
+        self._d = ctypes.c_void_p()
+        _check(library().vp9r_decoder_new(device, ctypes.byref(self._d)))
+
+    def __del__(self):
+        if getattr(self, "_d", None):
+            library().vp9r_decoder_free(self._d)
+            self._d = None
+
+    def set_threads(self, threads):
+        library().vp9r_decoder_set_threads(self._d, threads)
+
+    def threads(self):
+        return library().vp9r_decoder_threads(self._d)
+
+    def set_max_pixels(self, pixels):
+        library().vp9r_decoder_set_max_pixels(self._d, pixels)
+
+    def decode_all(self, packet):
+        n = u32()
+        _check(library().vp9r_decoder_decode(self._d, packet, len(packet), ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            d = FrameDesc()
+            _check(library().vp9r_decoder_frame(self._d, i, ctypes.byref(d)))
+            out.append(Frame(d))
+        return out
+
+    def decode(self, packet):
+        frames = self.decode_all(packet)
+        return frames[-1] if frames else None
+
+    def frames_decoded(self):
+        return library().vp9r_decoder_frames_decoded(self._d)

@@ -21,7 +21,15 @@ HOST_SOURCES = [
     "csrc/host/parser.cpp",
     "csrc/host/capi_parser.cpp",
 ]
+# compiled as HIP (device code, or host code that launches kernels)
 HIP_SOURCES = [
+    "csrc/gpu/itx.hip",
+    "csrc/gpu/mc.hip",
+    "csrc/gpu/intra.hip",
+    "csrc/gpu/lf.hip",
+    "csrc/host/recon.cpp",
+    "csrc/host/capi_seam.cpp",
+    "csrc/host/decoder.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -36,7 +44,7 @@ def compile_one(src, is_hip):
     obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
     cmd = [HIPCC] + COMMON + (GPU + ["-x", "hip"] if is_hip else []) + ["-c", os.path.join(HERE, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
+    if r.returncode != 0 or "error" in r.stderr:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj
 

@@ -1,0 +1,246 @@
+// Intra prediction and reconstruction as a superblock wavefront.
+//
+// Each workgroup owns one (superblock, plane) with intra work: it stages the
+// superblock and the row above / column left of it in LDS, then, for every
+// intra transform block in decode order, builds the edges
+// (src/decoder/recon.rs:17-103), predicts (src/dsp/intra.rs:19-175: each
+// directional mode in its per-sample closed form), dequantises and adds the
+// inverse transform (itx_dev.h), and finally writes the superblock back.  A
+// launch runs every group of one dependency level: a superblock's intra
+// blocks read the reconstruction of its left, above and above-left
+// neighbours only (above-right reads stay inside the block,
+// recon.rs:61-69), so the host orders groups by the longest chain of such
+// dependencies between superblocks that have intra work.
+#include <type_traits>
+
+#include "itx_dev.h"
+#include "launch.h"
+
+namespace vp9g {
+
+constexpr int TS = 65;  // LDS tile pitch: column x0 - 1 .. x0 + 63
+
+DEV int r2(int x, int n) { return (x + (1 << (n - 1))) >> n; }
+
+// Sample (i, j) of the prediction for `mode` (8.5.1.2); A[-1..2*size-1] is
+// the above row, L[0..size-1] the left column.
+DEV int intra_sample(int mode, int i, int j, int size, const int32_t *A, const int32_t *L, int dc,
+                     int max) {
+  switch (mode) {
+    case V_PRED:
+      return A[j];
+    case H_PRED:
+      return L[i];
+    case D207_PRED: {
+      int k = j >> 1, jj = j & 1, ii = i + k;
+      if (ii >= size - 1) return L[size - 1];
+      if (jj == 0) return r2(L[ii] + L[ii + 1], 1);
+      if (ii == size - 2) return r2(L[size - 2] + 3 * L[size - 1], 2);
+      return r2(L[ii] + 2 * L[ii + 1] + L[ii + 2], 2);
+    }
+    case D45_PRED: {
+      int k = i + j;
+      return k + 2 < 2 * size ? r2(A[k] + 2 * A[k + 1] + A[k + 2], 2) : A[2 * size - 1];
+    }
+    case D63_PRED: {
+      int k = (i >> 1) + j;
+      return (i & 1) ? r2(A[k] + 2 * A[k + 1] + A[k + 2], 2) : r2(A[k] + A[k + 1], 1);
+    }
+    case D117_PRED: {
+      int k = min(i >> 1, j);
+      i -= 2 * k;
+      j -= k;
+      if (i == 0) return r2(A[j - 1] + A[j], 1);
+      if (i == 1) return j == 0 ? r2(L[0] + 2 * A[-1] + A[0], 2) : r2(A[j - 2] + 2 * A[j - 1] + A[j], 2);
+      if (i == 2) return r2(A[-1] + 2 * L[0] + L[1], 2);
+      return r2(L[i - 3] + 2 * L[i - 2] + L[i - 1], 2);
+    }
+    case D135_PRED: {
+      int k = min(i, j);
+      i -= k;
+      j -= k;
+      if (i == 0) return j == 0 ? r2(L[0] + 2 * A[-1] + A[0], 2) : r2(A[j - 2] + 2 * A[j - 1] + A[j], 2);
+      if (i == 1) return r2(A[-1] + 2 * L[0] + L[1], 2);
+      return r2(L[i - 2] + 2 * L[i - 1] + L[i], 2);
+    }
+    case D153_PRED: {
+      int k = min(i, j >> 1);
+      i -= k;
+      j -= 2 * k;
+      if (j == 0) return i == 0 ? r2(L[0] + A[-1], 1) : r2(L[i - 1] + L[i], 1);
+      if (j == 1) {
+        if (i == 0) return r2(L[0] + 2 * A[-1] + A[0], 2);
+        if (i == 1) return r2(A[-1] + 2 * L[0] + L[1], 2);
+        return r2(L[i - 2] + 2 * L[i - 1] + L[i], 2);
+      }
+      return r2(A[j - 3] + 2 * A[j - 2] + A[j - 1], 2);  // i == 0
+    }
+    case TM_PRED:
+      return iclamp(A[j] + L[i] - A[-1], 0, max);
+    default:
+      return dc;
+  }
+}
+
+// DC_PRED's value (intra.rs:145-168).
+DEV int dc_value(int log2, const int32_t *A, const int32_t *L, bool have_left, bool have_above, int bd) {
+  int size = 1 << log2, s = 0;
+  if (have_left && have_above) {
+    for (int k = 0; k < size; k++) s += L[k] + A[k];
+    return (s + size) >> (log2 + 1);
+  }
+  if (have_left || have_above) {
+    const int32_t *e = have_left ? L : A;
+    for (int k = 0; k < size; k++) s += e[k];
+    return (s + (1 << (log2 - 1))) >> log2;
+  }
+  return 1 << (bd - 1);
+}
+
+struct LdsDst {
+  uint16_t *base;
+  DEV void add(int i, int j, int r, int max) {
+    uint16_t *p = base + i * TS + j;
+    *p = (uint16_t)iclamp((int)*p + r, 0, max);
+  }
+};
+
+template <class L, int n>
+DEV void intra_residual(typename L::T *coef, uint16_t *tile, int lx, int ly, const vp9r_txb &t,
+                        const IntraArgs &a, int seg, bool hbd) {
+  typedef typename L::T T;
+  constexpr int n0 = 1 << n, S = n0 + 1;
+  const int lane = threadIdx.x;
+  for (int k = lane; k < n0 * S; k += 64) coef[k] = 0;
+  __syncthreads();
+  const int pt = t.plane > 0;
+  const int32_t dcq = a.q[seg][pt][0], acq = a.q[seg][pt][1];
+  const uint16_t *pos = a.coef_pos + t.coef_start;
+  const int32_t *val = a.coef_val + t.coef_start;
+  for (int k = lane; k < t.ncoef; k += 64) {
+    int p = pos[k] & (n0 * n0 - 1);
+    coef[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(val[k], p, dcq, acq, n == 5);
+  }
+  __syncthreads();
+  const bool lossless = a.g.lossless;
+  const bool dc_only = !hbd && t.eob == 1 && t.tx_type == DCT_DCT && !lossless;
+  const int32_t dc_v = dc_only ? dc_only_value((int32_t)coef[0], n) : 0;
+  LdsDst dst{tile + ly * TS + lx};
+  itx_2d<L, n>(coef, lane, t.tx_type, lossless, dc_only, dc_v, dst, (1 << a.g.bit_depth) - 1);
+}
+
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group) {
+  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
+  typedef typename L::T T;
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  __shared__ uint16_t tile[TS * TS];
+  __shared__ T coef[32 * 33];
+  __shared__ int32_t edge[65 + 32];
+  const int lane = threadIdx.x;
+  const IntraGroup gr = a.groups[first_group + blockIdx.x];
+  const FrameGeom &g = a.g;
+  const int plane = gr.plane;
+  const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+  const int SW = 64 >> sx, SH = 64 >> sy;
+  const int x0 = gr.sb_col * SW, y0 = gr.sb_row * SH;
+  const uint32_t stride = a.cur.stride[plane];
+  P *pl = (P *)a.cur.p[plane];
+  for (int k = lane; k < (SH + 1) * (SW + 1); k += 64) {
+    int r = k / (SW + 1), c = k - r * (SW + 1);
+    int py = y0 - 1 + r, px = x0 - 1 + c;
+    if (py >= 0 && px >= 0) tile[r * TS + c] = pl[(size_t)py * stride + px];
+  }
+  __syncthreads();
+  const int max_x = (int)((g.mi_cols * 8) >> sx) - 1, max_y = (int)((g.mi_rows * 8) >> sy) - 1;
+  const int base = 1 << (g.bit_depth - 1);
+  const int maxv = (1 << g.bit_depth) - 1;
+  int32_t *above = edge, *left = edge + 65;
+  const int32_t *A = above + 1;
+#define TILE(py, px) ((int)tile[((py) - y0 + 1) * TS + ((px) - x0 + 1)])
+  for (uint32_t ti = 0; ti < gr.count; ti++) {
+    const vp9r_txb t = a.txbs[a.txb_index[gr.first + ti]];
+    const vp9r_block &b = a.blocks[t.block];
+    const int tx = t.tx_size, log2 = tx + 2, size = 1 << log2;
+    const int bsize = b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size;
+    const int n4w = num4x4_wide(bsize) >> sx;
+    const int base_x = (b.mi_col * 8) >> sx, base_y = (b.mi_row * 8) >> sy;
+    const int xi = (t.x - base_x) >> 2, yi = (t.y - base_y) >> 2;
+    const int step = 1 << tx;
+    const bool have_left = b.avail_l || xi > 0, have_above = b.avail_u || yi > 0;
+    const bool not_on_right = xi + step < n4w;
+    const int block_idx = (yi / step) * ((n4w + step - 1) / step) + xi / step;
+    const int mode = plane > 0 ? b.uv_mode : (b.mi_size >= BLOCK_8X8 ? b.y_mode : b.sub_modes[block_idx]);
+    const int x = t.x, y = t.y;
+    if (lane <= 2 * size) {
+      int v;
+      if (!have_above) {
+        v = base - 1;
+      } else if (lane == 0) {
+        v = have_left ? TILE(y - 1, min(max_x, x - 1)) : base + 1;
+      } else {
+        int i = lane - 1;
+        if (i >= size && !(not_on_right && tx == TX_4X4)) i = size - 1;
+        v = TILE(y - 1, min(max_x, x + i));
+      }
+      above[lane] = v;
+    }
+    if (lane < size) left[lane] = have_left ? TILE(min(max_y, y + lane), x - 1) : base + 1;
+    __syncthreads();
+    const int dc = mode == DC_PRED ? dc_value(log2, A, left, have_left, have_above, g.bit_depth) : 0;
+    const int lx = x - x0 + 1, ly = y - y0 + 1;
+    for (int k = lane; k < size * size; k += 64) {
+      int i = k >> log2, j = k & (size - 1);
+      tile[(ly + i) * TS + lx + j] = (uint16_t)intra_sample(mode, i, j, size, A, left, dc, maxv);
+    }
+    __syncthreads();
+    if (t.eob > 0) {
+      const int seg = b.segment_id;
+      switch (tx) {
+        case 0: intra_residual<L, 2>(coef, tile, lx, ly, t, a, seg, HBD); break;
+        case 1: intra_residual<L, 3>(coef, tile, lx, ly, t, a, seg, HBD); break;
+        case 2: intra_residual<L, 4>(coef, tile, lx, ly, t, a, seg, HBD); break;
+        default: intra_residual<L, 5>(coef, tile, lx, ly, t, a, seg, HBD); break;
+      }
+      __syncthreads();
+    }
+  }
+#undef TILE
+  for (int k = lane; k < SH * SW; k += 64) {
+    int r = k / SW, c = k - r * SW;
+    pl[(size_t)(y0 + r) * stride + x0 + c] = (P)tile[(r + 1) * TS + c + 1];
+  }
+}
+
+void launch_intra(hipStream_t s, const IntraArgs &a, uint32_t first_group, uint32_t n_groups) {
+  if (!n_groups) return;
+  if (a.g.bit_depth > 8)
+    hipLaunchKernelGGL(k_intra<true>, dim3(n_groups), dim3(64), 0, s, a, first_group);
+  else
+    hipLaunchKernelGGL(k_intra<false>, dim3(n_groups), dim3(64), 0, s, a, first_group);
+}
+
+// Per-kernel seam: one intra::predict call (intra.rs:19-29) on host-given
+// edges, for the parity tests.
+__global__ void k_intra_pred(int mode, int log2, const int32_t *above, const int32_t *left, int have_left,
+                             int have_above, int bd, uint16_t *dst, uint32_t stride) {
+  __shared__ int32_t e[65 + 32];
+  const int size = 1 << log2;
+  for (int k = threadIdx.x; k <= 2 * size; k += blockDim.x) e[k] = above[k];
+  for (int k = threadIdx.x; k < size; k += blockDim.x) e[65 + k] = left[k];
+  __syncthreads();
+  const int dc = dc_value(log2, e + 1, e + 65, have_left, have_above, bd);
+  for (int k = threadIdx.x; k < size * size; k += blockDim.x) {
+    int i = k >> log2, j = k & (size - 1);
+    dst[i * stride + j] = (uint16_t)intra_sample(mode, i, j, size, e + 1, e + 65, dc, (1 << bd) - 1);
+  }
+}
+
+void launch_intra_pred(hipStream_t s, uint32_t mode, uint32_t log2, const int32_t *above,
+                       const int32_t *left, int have_left, int have_above, uint32_t bit_depth,
+                       uint16_t *dst, uint32_t stride) {
+  hipLaunchKernelGGL(k_intra_pred, dim3(1), dim3(64), 0, s, (int)mode, (int)log2, above, left, have_left,
+                     have_above, (int)bit_depth, dst, stride);
+}
+
+}  // namespace vp9g

@@ -1,0 +1,135 @@
+// Inverse transform + reconstruction add of the transform blocks of inter
+// blocks, batched by size class: a 64-lane workgroup takes 64 / n0 blocks of
+// n0 x n0, one lane per row then per column (src/dsp/itx.rs:701-749 with the
+// dequantisation of src/decoder/recon.rs:233-281).  Inter blocks' residuals
+// are independent of each other, so a whole frame's are one launch per size.
+#include <type_traits>
+
+#include "itx_dev.h"
+#include "launch.h"
+
+namespace vp9g {
+
+template <int n, bool HBD>
+__global__ __launch_bounds__(64) void k_itx_inter(ItxArgs a, uint32_t n_jobs) {
+  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
+  typedef typename L::T T;
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  constexpr int n0 = 1 << n, BPW = 64 / n0, S = n0 + 1;
+  __shared__ T tiles[BPW][n0 * S];
+  const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
+  const uint32_t job = blockIdx.x * BPW + team;
+  const bool active = job < n_jobs;
+  T *tile = tiles[team];
+#pragma unroll
+  for (int k = 0; k < S; k++) tile[lane * S + k] = 0;
+  __syncthreads();
+  vp9r_txb t = {};
+  int32_t dcq = 0, acq = 0;
+  if (active) {
+    t = a.txbs[a.jobs[job]];
+    const int seg = a.blocks[t.block].segment_id;
+    const int pt = t.plane > 0;
+    dcq = a.q[seg][pt][0];
+    acq = a.q[seg][pt][1];
+    const uint16_t *pos = a.coef_pos + t.coef_start;
+    const int32_t *val = a.coef_val + t.coef_start;
+    for (int k = lane; k < t.ncoef; k += n0) {
+      int p = pos[k] & (n0 * n0 - 1);
+      tile[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(val[k], p, dcq, acq, n == 5);
+    }
+  }
+  __syncthreads();
+  const bool lossless = a.g.lossless;
+  const bool dc_only = !HBD && active && t.eob == 1 && t.tx_type == DCT_DCT && !lossless;
+  int32_t dc_v = 0;
+  if (dc_only) dc_v = dc_only_value((int32_t)tile[0], n + 0);
+  GlobalDst<P> dst;
+  dst.stride = a.cur.stride[t.plane];
+  dst.base = (P *)a.cur.p[t.plane] + (size_t)t.y * dst.stride + t.x;
+  const int max = (1 << a.g.bit_depth) - 1;
+  itx_2d<L, n>(tile, active ? lane : n0, t.tx_type, lossless, dc_only, dc_v, dst, max);
+}
+
+// Per-kernel seam: dense dequantised coefficients per block.
+template <int n, bool HBD>
+__global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const uint32_t *coef_off,
+                                                  const uint8_t *tx_type, const uint16_t *eob,
+                                                  const uint64_t *dst_off, void *dst_base,
+                                                  uint32_t stride, int bd, int lossless,
+                                                  const uint32_t *jobs, uint32_t n_jobs) {
+  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
+  typedef typename L::T T;
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  constexpr int n0 = 1 << n, BPW = 64 / n0, S = n0 + 1;
+  __shared__ T tiles[BPW][n0 * S];
+  const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
+  const uint32_t job = blockIdx.x * BPW + team;
+  const bool active = job < n_jobs;
+  T *tile = tiles[team];
+  uint32_t i = active ? jobs[job] : 0;
+  if (active) {
+    const int32_t *c = coefs + coef_off[i];
+#pragma unroll
+    for (int k = 0; k < n0; k++) tile[lane * S + k] = (T)c[lane * n0 + k];
+  }
+  __syncthreads();
+  const int tt = active ? tx_type[i] : 0;
+  const bool dc_only = !HBD && active && eob[i] == 1 && tt == DCT_DCT && !lossless;
+  int32_t dc_v = 0;
+  if (dc_only) dc_v = dc_only_value((int32_t)tile[0], n);
+  GlobalDst<P> dst;
+  dst.stride = stride;
+  dst.base = (P *)dst_base + (active ? dst_off[i] : 0);
+  itx_2d<L, n>(tile, active ? lane : n0, tt, lossless != 0, dc_only, dc_v, dst, (1 << bd) - 1);
+}
+
+template <int n>
+static void itx_inter_n(hipStream_t s, const ItxArgs &a, uint32_t n_jobs) {
+  constexpr int BPW = 64 >> n;
+  dim3 grid((n_jobs + BPW - 1) / BPW);
+  if (a.g.bit_depth > 8)
+    hipLaunchKernelGGL((k_itx_inter<n, true>), grid, dim3(64), 0, s, a, n_jobs);
+  else
+    hipLaunchKernelGGL((k_itx_inter<n, false>), grid, dim3(64), 0, s, a, n_jobs);
+}
+
+void launch_itx_inter(hipStream_t s, const ItxArgs &a, int tx_size, uint32_t n_jobs) {
+  if (!n_jobs) return;
+  switch (tx_size) {
+    case 0: itx_inter_n<2>(s, a, n_jobs); break;
+    case 1: itx_inter_n<3>(s, a, n_jobs); break;
+    case 2: itx_inter_n<4>(s, a, n_jobs); break;
+    default: itx_inter_n<5>(s, a, n_jobs); break;
+  }
+}
+
+template <int n>
+static void itx_dense_n(hipStream_t s, const int32_t *coefs, const uint32_t *coef_off,
+                        const uint8_t *tx_type, const uint16_t *eob, const uint64_t *dst_off,
+                        void *dst, uint32_t stride, uint32_t bd, int lossless, const uint32_t *jobs,
+                        uint32_t n_jobs) {
+  constexpr int BPW = 64 >> n;
+  dim3 grid((n_jobs + BPW - 1) / BPW);
+  if (bd > 8)
+    hipLaunchKernelGGL((k_itx_dense<n, true>), grid, dim3(64), 0, s, coefs, coef_off, tx_type, eob,
+                       dst_off, dst, stride, (int)bd, lossless, jobs, n_jobs);
+  else
+    hipLaunchKernelGGL((k_itx_dense<n, false>), grid, dim3(64), 0, s, coefs, coef_off, tx_type, eob,
+                       dst_off, dst, stride, (int)bd, lossless, jobs, n_jobs);
+}
+
+void launch_itx_dense(hipStream_t s, uint32_t, const int32_t *coefs, const uint32_t *coef_off,
+                      const uint8_t *, const uint8_t *tx_type, const uint16_t *eob,
+                      const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
+                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs) {
+  if (!n_jobs) return;
+  switch (tx_size) {
+    case 0: itx_dense_n<2>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
+    case 1: itx_dense_n<3>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
+    case 2: itx_dense_n<4>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
+    default: itx_dense_n<5>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
+  }
+}
+
+}  // namespace vp9g

@@ -1,0 +1,288 @@
+// The loop filter (8.8): edge decisions per superblock (one parallel
+// launch), then the sample filters as a superblock wavefront.
+//
+// Decisions follow superblock() (src/decoder/loopfilter.rs:466-620), one
+// lane per (edge, 4-sample run) entry, with LvlLookup and the limits of
+// Ctx::new (loopfilter.rs:15-47, 135-170) precomputed by the host.  The
+// filters follow src/dsp/lf.rs:24-143.
+//
+// Order: the reference filters each plane superblock by superblock in raster
+// order, all vertical edges of a superblock (left to right) then all its
+// horizontal edges (top to bottom) (loopfilter.rs:66-80).  Superblock (r, c)
+// touches the samples from 8 left and 8 above of it to its bottom-right
+// corner, so it must follow (r, c - 1) and (r - 1, c + 1): superblocks with
+// equal c + 2r never overlap and form one launch.  Inside a superblock, the
+// vertical pass is sequential along each row and rows are independent, the
+// horizontal pass sequential along each column: one lane per row, then one
+// lane per column, on the superblock and its 8-sample halo staged in LDS.
+#include <type_traits>
+
+#include "dev_common.h"
+#include "launch.h"
+#include "vp9_consts.h"
+
+namespace vp9g {
+
+constexpr int BP = 74;  // LDS pitch of the 72 x 72 region (odd in dwords)
+
+DEV int iabs(int v) { return v < 0 ? -v : v; }
+
+// lf::filter on an LDS buffer: across the edge whose first q sample is
+// b[pos], `step` apart.
+DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit, int thresh, int bd) {
+  const int q0 = b[pos], q1 = b[pos + step], q2 = b[pos + 2 * step], q3 = b[pos + 3 * step];
+  const int p0 = b[pos - step], p1 = b[pos - 2 * step], p2 = b[pos - 3 * step], p3 = b[pos - 4 * step];
+  const int sh = bd - 8;
+  const int lim = limit << sh, blim = blimit << sh;
+  if (iabs(p3 - p2) > lim || iabs(p2 - p1) > lim || iabs(p1 - p0) > lim || iabs(q1 - q0) > lim ||
+      iabs(q2 - q1) > lim || iabs(q3 - q2) > lim || iabs(p0 - q0) * 2 + iabs(p1 - q1) / 2 > blim)
+    return;
+  const int th = thresh << sh;
+  const bool hev = iabs(p1 - p0) > th || iabs(q1 - q0) > th;
+  const int one = 1 << sh;
+  const bool flat = fs >= 1 && iabs(p1 - p0) <= one && iabs(q1 - q0) <= one && iabs(p2 - p0) <= one &&
+                    iabs(q2 - q0) <= one && iabs(p3 - p0) <= one && iabs(q3 - q0) <= one;
+  if (fs == 0 || !flat) {
+    // narrow filter (8.8.5.2)
+    const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1, off = 0x80 << sh;
+    const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
+    int f = hev ? iclamp(ps1 - qs1, lo, hi) : 0;
+    f = iclamp(f + 3 * (qs0 - ps0), lo, hi);
+    const int f1 = iclamp(f + 4, lo, hi) >> 3, f2 = iclamp(f + 3, lo, hi) >> 3;
+    b[pos] = (uint16_t)(iclamp(qs0 - f1, lo, hi) + off);
+    b[pos - step] = (uint16_t)(iclamp(ps0 + f2, lo, hi) + off);
+    if (!hev) {
+      const int g = (f1 + 1) >> 1;
+      b[pos + step] = (uint16_t)(iclamp(qs1 - g, lo, hi) + off);
+      b[pos - 2 * step] = (uint16_t)(iclamp(ps1 + g, lo, hi) + off);
+    }
+    return;
+  }
+  bool flat2 = false;
+  if (fs >= 2) {
+    flat2 = true;
+    for (int k = 4; k < 8; k++)
+      if (iabs((int)b[pos - (k + 1) * step] - p0) > one || iabs((int)b[pos + k * step] - q0) > one) {
+        flat2 = false;
+        break;
+      }
+  }
+  // wide filter (8.8.5.3): log2 3 (8 taps) or 4 (16 taps)
+  const int log2 = (fs == 1 || !flat2) ? 3 : 4;
+  const int n = (1 << (log2 - 1)) - 1;
+  int s[16], f[16];
+  for (int k = -(n + 1); k <= n; k++) s[k + 8] = b[pos + k * step];
+  for (int i = -n; i < n; i++) {
+    int t = s[i + 8];
+    for (int j = -n; j <= n; j++) t += s[iclamp(i + j, -(n + 1), n) + 8];
+    f[i + 8] = (t + (1 << (log2 - 1))) >> log2;
+  }
+  for (int i = -n; i < n; i++) b[pos + i * step] = (uint16_t)f[i + 8];
+}
+
+// ---- decisions
+
+__global__ __launch_bounds__(256) void k_lf_mask(LfArgs a) {
+  const FrameGeom &g = a.g;
+  const int sb = blockIdx.x, pass = blockIdx.y, plane = blockIdx.z;
+  const int n_sb = g.sb64_rows * g.sb64_cols;
+  const uint32_t row = (sb / g.sb64_cols) * 8, col = (sb % g.sb64_cols) * 8;
+  const int ed = threadIdx.x >> 4, rn = threadIdx.x & 15;
+  const int sub_x = plane ? g.ss_x : 0, sub_y = plane ? g.ss_y : 0;
+  const int sub = pass == 0 ? sub_x : sub_y;
+  const int edge_len = pass == 0 ? 64 >> sub_y : 64 >> sub_x;
+  const int n_edges = 16 >> sub, n_runs = edge_len / 4;
+  const uint32_t mi_cols = g.mi_cols, mi_rows = g.mi_rows;
+  int fs = -1, lvl = 0;
+  auto at_x = [&](int e, int r) -> uint32_t {
+    return pass == 0 ? col * 8 + e * (4u << sub_x) : col * 8 + ((4u * r) << sub_x);
+  };
+  auto at_y = [&](int e, int r) -> uint32_t {
+    return pass == 0 ? row * 8 + ((4u * r) << sub_y) : row * 8 + e * (4u << sub_y);
+  };
+  do {
+    if (ed >= n_edges || rn >= n_runs) break;
+    const int ca = ed >> 1, cb = rn >> 1, k = rn & 1;
+    uint32_t x = at_x(2 * ca, 2 * cb), y = at_y(2 * ca, 2 * cb);
+    if (x >= 8 * mi_cols || y >= 8 * mi_rows) break;
+    const uint32_t loop_col = ((x >> 3) >> sub_x) << sub_x, loop_row = ((y >> 3) >> sub_y) << sub_y;
+    const vp9r_mi m = a.mi[loop_row * mi_cols + loop_col];
+    const int mode_type = m.y_mode == NEARESTMV || m.y_mode == NEARMV || m.y_mode == NEWMV;
+    const int rf = m.ref_frame0 > 0 ? m.ref_frame0 : 0;
+    const int l = a.p.lvl[m.segment_id][rf][mode_type];
+    if (l == 0) break;
+    int tx_sz = m.tx_size;
+    if (plane > 0) {
+      if (m.mi_size < BLOCK_8X8) {
+        tx_sz = TX_4X4;
+      } else {
+        // get_uv_tx_size: min(tx_size, max_txsize_lookup[ss_size_lookup[...]]),
+        // i.e. the largest transform fitting the block's chroma
+        const int ux = g.legacy_uv ? 1 : g.ss_x, uy = g.legacy_uv ? 1 : g.ss_y;
+        const int wl = ((kW4P >> (3 * m.mi_size)) & 7) - ux, hl = ((kH4P >> (3 * m.mi_size)) & 7) - uy;
+        const int uv_max = min(min(wl, hl), 3);
+        tx_sz = min(tx_sz, uv_max);
+      }
+    }
+    const int sb_size = sub == 0 ? m.mi_size : max((int)m.mi_size, (int)BLOCK_16X16);
+    const uint32_t block_mask = pass == 0 ? 8u * num8x8_wide(sb_size) - 1 : 8u * num8x8_high(sb_size) - 1;
+    const bool coded = m.ref_frame0 <= INTRA_FRAME || !m.skip;
+    x = at_x(ed, 2 * cb);
+    y = at_y(ed, 2 * cb);
+    if ((pass == 0 && x == 0) || (pass == 1 && y == 0)) break;
+    const uint32_t across = pass == 0 ? x : y;
+    const bool is_block_edge = (across & block_mask) == 0;
+    const bool tx_edge = (ed & ((1 << tx_sz) - 1)) == 0;
+    if (!is_block_edge && !(tx_edge && coded)) break;
+    const bool is_32_edge = ed % 8 == 0;
+    const int base_size = (tx_sz == TX_4X4 && is_32_edge) ? TX_8X8 : min(tx_sz, (int)TX_16X16);
+    const int filter_size =
+        ((pass == 0 && sub_x == 1 && base_size == TX_16X16 && (x >> 3) == mi_cols - 1) ||
+         (pass == 1 && sub_y == 1 && base_size == TX_16X16 && (y >> 3) == mi_rows - 1))
+            ? TX_8X8
+            : base_size;
+    const uint32_t xx = at_x(ed, rn), yy = at_y(ed, rn);
+    if (k == 1 && (xx >= 8 * mi_cols || yy >= 8 * mi_rows)) break;
+    const bool is_tx_edge =
+        (pass == 1 && sub_x == 1 && (mi_cols & 1) && (ed & 1) && (xx + 8) >= mi_cols * 8) ? false : tx_edge;
+    if (!(is_block_edge || (is_tx_edge && coded))) break;
+    fs = filter_size;
+    lvl = l;
+  } while (0);
+  const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256 + threadIdx.x;
+  a.mask_fs[o] = (int8_t)fs;
+  a.mask_lvl[o] = (uint8_t)lvl;
+}
+
+// ---- one wavefront level
+
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  __shared__ uint16_t buf[72 * BP];
+  __shared__ int8_t fs[2][256];
+  __shared__ uint8_t lv[2][256];
+  const FrameGeom &g = a.g;
+  const int lane = threadIdx.x;
+  const uint32_t code = a.sbs[first + blockIdx.x];
+  const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
+  const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+  const int SW = 64 >> sx, SH = 64 >> sy;
+  const int x0 = sb_col * SW, y0 = sb_row * SH;
+  const uint32_t stride = a.cur.stride[plane];
+  P *pl = (P *)a.cur.p[plane];
+  const int RW = SW + 8, RH = SH + 8;
+  for (int k = lane; k < RW * RH; k += 64) {
+    int r = k / RW, c = k - r * RW;
+    int py = y0 - 8 + r, px = x0 - 8 + c;
+    if (py >= 0 && px >= 0) buf[r * BP + c] = pl[(size_t)py * stride + px];
+  }
+  const int n_sb = g.sb64_rows * g.sb64_cols, sb = sb_row * g.sb64_cols + sb_col;
+  for (int pass = 0; pass < 2; pass++) {
+    const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256;
+    for (int k = lane; k < 256; k += 64) {
+      fs[pass][k] = a.mask_fs[o + k];
+      lv[pass][k] = a.mask_lvl[o + k];
+    }
+  }
+  __syncthreads();
+  const int bd = g.bit_depth;
+  if (lane < SH) {
+    const int r = 8 + lane, rn = lane >> 2, n_edges = 16 >> sx;
+    for (int ed = 0; ed < n_edges; ed++) {
+      const int f = fs[0][ed * 16 + rn];
+      if (f >= 0) {
+        const int l = lv[0][ed * 16 + rn];
+        lf_filter(buf, r * BP + 8 + 4 * ed, 1, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], bd);
+      }
+    }
+  }
+  __syncthreads();
+  if (lane < SW) {
+    const int c = 8 + lane, rn = lane >> 2, n_edges = 16 >> sy;
+    for (int ed = 0; ed < n_edges; ed++) {
+      const int f = fs[1][ed * 16 + rn];
+      if (f >= 0) {
+        const int l = lv[1][ed * 16 + rn];
+        lf_filter(buf, (8 + 4 * ed) * BP + c, BP, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], bd);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = lane; k < RW * RH; k += 64) {
+    int r = k / RW, c = k - r * RW;
+    int py = y0 - 8 + r, px = x0 - 8 + c;
+    if (py >= 0 && px >= 0) pl[(size_t)py * stride + px] = (P)buf[r * BP + c];
+  }
+}
+
+void launch_lf_mask(hipStream_t s, const LfArgs &a) {
+  dim3 grid(a.g.sb64_rows * a.g.sb64_cols, 2, 3);
+  hipLaunchKernelGGL(k_lf_mask, grid, dim3(256), 0, s, a);
+}
+
+void launch_lf_level(hipStream_t s, const LfArgs &a, uint32_t first, uint32_t n) {
+  if (!n) return;
+  if (a.g.bit_depth > 8)
+    hipLaunchKernelGGL(k_lf<true>, dim3(n), dim3(64), 0, s, a, first);
+  else
+    hipLaunchKernelGGL(k_lf<false>, dim3(n), dim3(64), 0, s, a, first);
+}
+
+// Per-kernel seam: lf::filter_edges on global samples, one pass per
+// workgroup launch (passes are ordered by the stream).
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_lf_edges(const uint8_t *e, int x0, int y0, int vertical,
+                                                 int n_edges, int n_runs, void *buf_, uint32_t stride,
+                                                 int bd) {
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  // stage the pass's region (edges reach 8 back and 8 forward) in LDS
+  __shared__ uint16_t buf[80 * 80];
+  P *g = (P *)buf_;
+  const int8_t *fs = (const int8_t *)e;
+  const uint8_t *lim = e + 256, *blim = e + 512, *thr = e + 768;
+  const int ext_a = 4 * n_edges + 8, ext_r = 4 * n_runs;  // across, along
+  const int W = vertical ? ext_a + 8 : ext_r, H = vertical ? ext_r : ext_a + 8;
+  const int gx0 = vertical ? x0 - 8 : x0, gy0 = vertical ? y0 : y0 - 8;
+  for (int k = threadIdx.x; k < W * H; k += 64) {
+    int r = k / W, c = k - r * W;
+    if (gy0 + r >= 0 && gx0 + c >= 0) buf[r * 80 + c] = g[(size_t)(gy0 + r) * stride + gx0 + c];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x;
+  if (lane < ext_r) {
+    const int rn = lane >> 2;
+    for (int ed = 0; ed < n_edges; ed++) {
+      const int f = fs[ed * 16 + rn];
+      if (f < 0) continue;
+      const int i = ed * 16 + rn;
+      if (vertical)
+        lf_filter(buf, lane * 80 + 8 + 4 * ed, 1, f, lim[i], blim[i], thr[i], bd);
+      else
+        lf_filter(buf, (8 + 4 * ed) * 80 + lane, 80, f, lim[i], blim[i], thr[i], bd);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < W * H; k += 64) {
+    int r = k / W, c = k - r * W;
+    if (gy0 + r >= 0 && gx0 + c >= 0) g[(size_t)(gy0 + r) * stride + gx0 + c] = (P)buf[r * 80 + c];
+  }
+}
+
+// lf_filter's limits are unshifted values; limit / blimit / thresh tables
+// per level are not used here: the Edges tables carry the values.
+void launch_lf_edges(hipStream_t s, uint32_t count, const uint8_t *edges, const uint16_t *x0,
+                     const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
+                     const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t bit_depth) {
+  // host arrays for the geometry, device pointer for the tables
+  for (uint32_t i = 0; i < count; i++) {
+    if (bit_depth > 8)
+      hipLaunchKernelGGL(k_lf_edges<true>, dim3(1), dim3(64), 0, s, edges + 1024 * (size_t)i, x0[i], y0[i],
+                         vertical[i], n_edges[i], n_runs[i], buf, stride, (int)bit_depth);
+    else
+      hipLaunchKernelGGL(k_lf_edges<false>, dim3(1), dim3(64), 0, s, edges + 1024 * (size_t)i, x0[i], y0[i],
+                         vertical[i], n_edges[i], n_runs[i], buf, stride, (int)bit_depth);
+  }
+}
+
+}  // namespace vp9g

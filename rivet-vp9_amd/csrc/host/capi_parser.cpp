@@ -12,10 +12,6 @@ thread_local std::string g_last_error;
 void set_error(const std::string &m) { g_last_error = m; }
 }  // namespace vp9r
 
-struct vp9r_parser {
-  vp9r::Parser p;
-};
-
 extern "C" {
 
 const char *vp9r_last_error(void) { return vp9r::g_last_error.c_str(); }

@@ -1,0 +1,237 @@
+// The per-kernel seam of include/vp9r.h: src/dsp's kernels as batched GPU
+// launches (for the kernel microbenchmarks and parity tests).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "recon.h"
+
+using namespace vp9r;
+
+namespace vp9g {
+void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,
+                          const void *const *refs, uint32_t ref_stride, int32_t last_x, int32_t last_y,
+                          void *out, uint32_t out_stride, uint32_t bd);
+}
+
+static int dev_err(hipError_t e, const char *what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return VP9R_ERR_DEVICE;
+}
+#define TRY(expr, what)                         \
+  do {                                          \
+    hipError_t e_ = (expr);                     \
+    if (e_ != hipSuccess) return dev_err(e_, what); \
+  } while (0)
+
+static bool bd_ok(uint32_t bd) { return bd == 8 || bd == 10 || bd == 12; }
+
+extern "C" {
+
+int vp9r_gpu_malloc(vp9r_gpu *g, size_t bytes, void **out) {
+  *out = nullptr;
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  TRY(hipMalloc(out, bytes ? bytes : 1), "hipMalloc");
+  return VP9R_OK;
+}
+
+void vp9r_gpu_mfree(vp9r_gpu *g, void *p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(p);
+}
+
+int vp9r_gpu_to_device(vp9r_gpu *g, void *dst, const void *src, size_t bytes) {
+  TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g->stream), "copy to device");
+  TRY(hipStreamSynchronize(g->stream), "copy sync");
+  return VP9R_OK;
+}
+
+int vp9r_gpu_to_host(vp9r_gpu *g, void *dst, const void *src, size_t bytes) {
+  TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream), "copy to host");
+  TRY(hipStreamSynchronize(g->stream), "copy sync");
+  return VP9R_OK;
+}
+
+void *vp9r_gpu_frame_plane(vp9r_gpu_frame *f, int plane) {
+  return (plane >= 0 && plane < 3) ? f->planes.p[plane] : nullptr;
+}
+
+int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,
+                       const uint8_t *tx, const uint8_t *tx_type, const uint16_t *eob,
+                       const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
+                       int lossless) {
+  if (!bd_ok(bit_depth)) {
+    set_error("bit depth must be 8, 10 or 12");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  std::vector<uint32_t> jobs[4];
+  for (uint32_t i = 0; i < count; i++) {
+    if (tx[i] > 3 || (lossless && tx[i] != 0)) {
+      set_error("transform size out of range (lossless is 4x4 only)");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+    jobs[tx[i]].push_back(i);
+  }
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  TRY(g->seam_a.ensure(sizeof(uint32_t) * std::max<uint32_t>(count, 1)), "hipMalloc");
+  uint32_t *dj = (uint32_t *)g->seam_a.p;
+  uint32_t off = 0;
+  for (int k = 0; k < 4; k++) {
+    if (jobs[k].empty()) continue;
+    TRY(hipMemcpyAsync(dj + off, jobs[k].data(), 4 * jobs[k].size(), hipMemcpyHostToDevice, g->stream),
+        "upload jobs");
+    vp9g::launch_itx_dense(g->stream, count, coefs, coef_off, nullptr, tx_type, eob, dst_off, dst, stride,
+                           bit_depth, lossless, k, dj + off, (uint32_t)jobs[k].size());
+    off += (uint32_t)jobs[k].size();
+  }
+  TRY(hipGetLastError(), "itx launch");
+  TRY(hipStreamSynchronize(g->stream), "itx sync");
+  return VP9R_OK;
+}
+
+int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const void *const *refs,
+                  uint32_t n_refs, uint32_t ref_stride, int32_t last_x, int32_t last_y, void *out,
+                  uint32_t out_stride, uint32_t bit_depth) {
+  if (!bd_ok(bit_depth) || n_refs == 0 || n_refs > 64 || last_x < 0 || last_y < 0) {
+    set_error("invalid motion compensation batch");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  struct DescTile {
+    uint32_t desc;
+    uint16_t tx, ty;
+  };
+  std::vector<DescTile> tiles;
+  for (uint32_t i = 0; i < count; i++) {
+    const vp9r_mc_desc &d = desc[i];
+    bool ok = d.w >= 4 && d.w <= 64 && d.h >= 4 && d.h <= 64 && d.xs >= 1 && d.xs <= 32 && d.ys >= 1 &&
+              d.ys <= 32 && d.filter < 4 && d.ref_idx >= 0 && (uint32_t)d.ref_idx < n_refs &&
+              d.ref2_idx < (int)n_refs;
+    if (!ok) {
+      set_error("motion compensation descriptor out of range");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+    for (int ty = 0; ty < d.h; ty += 8)
+      for (int tx = 0; tx < d.w; tx += 8) tiles.push_back({i, (uint16_t)tx, (uint16_t)ty});
+  }
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  size_t dbytes = sizeof(vp9r_mc_desc) * count, tbytes = sizeof(DescTile) * tiles.size();
+  TRY(g->seam_a.ensure(dbytes + 256), "hipMalloc");
+  TRY(g->seam_b.ensure(tbytes + 256), "hipMalloc");
+  TRY(g->seam_c.ensure(sizeof(void *) * n_refs), "hipMalloc");
+  TRY(hipMemcpyAsync(g->seam_a.p, desc, dbytes, hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(hipMemcpyAsync(g->seam_b.p, tiles.data(), tbytes, hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(hipMemcpyAsync(g->seam_c.p, refs, sizeof(void *) * n_refs, hipMemcpyHostToDevice, g->stream), "upload");
+  vp9g::launch_mc_desc_tiles(g->stream, (const vp9r_mc_desc *)g->seam_a.p, g->seam_b.p, (uint32_t)tiles.size(),
+                             (const void *const *)g->seam_c.p, ref_stride, last_x, last_y, out, out_stride,
+                             bit_depth);
+  TRY(hipGetLastError(), "mc launch");
+  TRY(hipStreamSynchronize(g->stream), "mc sync");
+  return VP9R_OK;
+}
+
+int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const uint16_t *x0,
+                        const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
+                        const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t rows,
+                        uint32_t bit_depth) {
+  if (!bd_ok(bit_depth)) {
+    set_error("bit depth must be 8, 10 or 12");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  for (uint32_t i = 0; i < count; i++) {
+    uint32_t across = vertical[i] ? x0[i] : y0[i], along = vertical[i] ? y0[i] : x0[i];
+    uint32_t lim_a = vertical[i] ? stride : rows, lim_r = vertical[i] ? rows : stride;
+    bool ok = n_edges[i] >= 1 && n_edges[i] <= 16 && n_runs[i] >= 1 && n_runs[i] <= 16 &&
+              across + 4u * n_edges[i] + 8 <= lim_a && along + 4u * n_runs[i] <= lim_r;
+    if (!ok) {
+      set_error("loop filter pass out of range of the plane");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  }
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  TRY(g->seam_a.ensure(1024 * (size_t)std::max<uint32_t>(count, 1)), "hipMalloc");
+  TRY(hipMemcpyAsync(g->seam_a.p, edges, 1024 * (size_t)count, hipMemcpyHostToDevice, g->stream), "upload");
+  vp9g::launch_lf_edges(g->stream, count, (const uint8_t *)g->seam_a.p, x0, y0, vertical, n_edges, n_runs, buf,
+                        stride, bit_depth);
+  TRY(hipGetLastError(), "lf launch");
+  TRY(hipStreamSynchronize(g->stream), "lf sync");
+  return VP9R_OK;
+}
+
+int vp9r_itx_add_host(vp9r_gpu *g, const int32_t *coefs, uint32_t n, uint32_t tx_type, int lossless,
+                      uint32_t eob, uint32_t bit_depth, uint16_t *dst, uint32_t stride) {
+  if (n < 2 || n > 5 || tx_type > 3 || !bd_ok(bit_depth) || (lossless && n != 2) || stride < (1u << n)) {
+    set_error("invalid transform");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  const uint32_t n0 = 1u << n;
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  // device: coefs, then the block as samples of the bit depth, then job data
+  size_t bps = bit_depth > 8 ? 2 : 1;
+  TRY(g->seam_a.ensure(4 * n0 * n0 + bps * n0 * n0 + 64), "hipMalloc");
+  uint8_t *d = (uint8_t *)g->seam_a.p;
+  int32_t *dc = (int32_t *)d;
+  uint8_t *dpix = d + 4 * n0 * n0;
+  struct Meta {
+    uint32_t coef_off, job;
+    uint8_t tx_type, pad;
+    uint16_t eob;
+    uint64_t dst_off;
+  } meta = {0, 0, (uint8_t)tx_type, 0, (uint16_t)eob, 0};
+  uint8_t *dmeta = dpix + bps * n0 * n0;
+  dmeta = (uint8_t *)(((uintptr_t)dmeta + 15) & ~(uintptr_t)15);
+  std::vector<uint8_t> pix(bps * n0 * n0);
+  for (uint32_t i = 0; i < n0; i++)
+    for (uint32_t j = 0; j < n0; j++) {
+      uint16_t v = dst[i * stride + j];
+      if (bps == 2)
+        memcpy(&pix[2 * (i * n0 + j)], &v, 2);
+      else
+        pix[i * n0 + j] = (uint8_t)v;
+    }
+  TRY(hipMemcpyAsync(dc, coefs, 4 * n0 * n0, hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(hipMemcpyAsync(dpix, pix.data(), pix.size(), hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(hipMemcpyAsync(dmeta, &meta, sizeof meta, hipMemcpyHostToDevice, g->stream), "upload");
+  Meta *m = (Meta *)dmeta;
+  vp9g::launch_itx_dense(g->stream, 1, dc, &m->coef_off, nullptr, &m->tx_type, &m->eob, &m->dst_off, dpix, n0,
+                         bit_depth, lossless, (int)n - 2, &m->job, 1);
+  TRY(hipGetLastError(), "itx launch");
+  TRY(hipMemcpyAsync(pix.data(), dpix, pix.size(), hipMemcpyDeviceToHost, g->stream), "download");
+  TRY(hipStreamSynchronize(g->stream), "itx sync");
+  for (uint32_t i = 0; i < n0; i++)
+    for (uint32_t j = 0; j < n0; j++) {
+      uint16_t v;
+      if (bps == 2)
+        memcpy(&v, &pix[2 * (i * n0 + j)], 2);
+      else
+        v = pix[i * n0 + j];
+      dst[i * stride + j] = v;
+    }
+  return VP9R_OK;
+}
+
+int vp9r_intra_predict_host(vp9r_gpu *g, uint32_t mode, uint32_t log2, const int32_t *above,
+                            const int32_t *left, int have_left, int have_above, uint32_t bit_depth,
+                            uint16_t *dst, uint32_t stride) {
+  if (mode > 9 || log2 < 2 || log2 > 5 || !bd_ok(bit_depth) || stride < (1u << log2)) {
+    set_error("invalid intra prediction");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  const uint32_t size = 1u << log2;
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  TRY(g->seam_a.ensure(4 * (2 * size + 1) + 4 * size + 2 * size * size + 64), "hipMalloc");
+  int32_t *da = (int32_t *)g->seam_a.p, *dl = da + 2 * size + 1;
+  uint16_t *dd = (uint16_t *)(dl + size);
+  TRY(hipMemcpyAsync(da, above, 4 * (2 * size + 1), hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(hipMemcpyAsync(dl, left, 4 * size, hipMemcpyHostToDevice, g->stream), "upload");
+  vp9g::launch_intra_pred(g->stream, mode, log2, da, dl, have_left, have_above, bit_depth, dd, size);
+  TRY(hipGetLastError(), "intra launch");
+  std::vector<uint16_t> out(size * size);
+  TRY(hipMemcpyAsync(out.data(), dd, 2 * size * size, hipMemcpyDeviceToHost, g->stream), "download");
+  TRY(hipStreamSynchronize(g->stream), "intra sync");
+  for (uint32_t i = 0; i < size; i++)
+    for (uint32_t j = 0; j < size; j++) dst[i * stride + j] = out[i * size + j];
+  return VP9R_OK;
+}
+
+}  // extern "C"

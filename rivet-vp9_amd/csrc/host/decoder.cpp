@@ -1,0 +1,182 @@
+// The decoder API of include/vp9r.h: the crate's Decoder
+// (src/decoder/mod.rs:122-216) with the CPU parser and the GPU
+// reconstruction.  Reference slots hold HBM-resident frame stores shared
+// between slots (the crate's Arc<RefFrame>, decoder/mod.rs:125, 398-415);
+// frame stores are recycled through a pool once no slot holds them (the
+// crate's decoder/pool.rs, here device memory).  Shown frames are copied to
+// the host in the Frame::packed layout (decoder/mod.rs:79-106).
+#include <memory>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "parser.h"
+#include "recon.h"
+
+using namespace vp9r;
+
+namespace {
+
+struct FrameStore {
+  vp9r_gpu_frame *f = nullptr;
+  uint32_t color_space = 0, color_range = 0, render_w = 0, render_h = 0;
+  ~FrameStore() { vp9r_gpu_frame_free(f); }
+};
+
+struct OutFrame {
+  vp9r_frame desc{};
+  std::vector<uint8_t> data;
+};
+
+}  // namespace
+
+struct vp9r_decoder {
+  vp9r_gpu *g = nullptr;
+  vp9r_parser parser;
+  std::shared_ptr<FrameStore> slots[8];
+  std::vector<std::shared_ptr<FrameStore>> pool;
+  std::vector<OutFrame> out;
+  uint32_t threads = 0;
+  uint64_t frames = 0;
+
+  std::shared_ptr<FrameStore> take(uint32_t w, uint32_t h, uint32_t bd, uint32_t sx, uint32_t sy, int *st) {
+    for (size_t i = 0; i < pool.size(); i++) {
+      vp9r_gpu_frame *f = pool[i]->f;
+      if (pool[i].use_count() == 1 && f->width == w && f->height == h && f->bit_depth == bd && f->ss_x == sx &&
+          f->ss_y == sy) {
+        bool in_slot = false;
+        for (auto &s : slots) in_slot |= s == pool[i];
+        if (!in_slot) return pool[i];
+      }
+    }
+    auto fs = std::make_shared<FrameStore>();
+    *st = vp9r_gpu_frame_new(g, w, h, bd, sx, sy, &fs->f);
+    if (*st) return nullptr;
+    pool.push_back(fs);
+    // forget stores of other geometries nobody holds
+    std::vector<std::shared_ptr<FrameStore>> keep;
+    for (auto &p : pool) {
+      bool held = p.use_count() > 1;
+      for (auto &s : slots) held |= s == p;
+      if (held || (p->f->width == w && p->f->height == h)) keep.push_back(p);
+    }
+    pool.swap(keep);
+    return fs;
+  }
+
+  int emit(const std::shared_ptr<FrameStore> &fs) {
+    vp9r_gpu_frame *f = fs->f;
+    OutFrame o;
+    size_t n = vp9r_packed_size(f->width, f->height, f->bit_depth, f->ss_x, f->ss_y);
+    o.data.resize(n);
+    int st = vp9r_gpu_download_packed(g, f, o.data.data());
+    if (st) return st;
+    vp9r_frame &d = o.desc;
+    d.width = f->width;
+    d.height = f->height;
+    d.bit_depth = f->bit_depth;
+    d.ss_x = f->ss_x;
+    d.ss_y = f->ss_y;
+    d.color_space = fs->color_space;
+    d.full_range = fs->color_range;
+    d.render_width = fs->render_w;
+    d.render_height = fs->render_h;
+    d.size = n;
+    size_t bps = f->bit_depth > 8 ? 2 : 1, off = 0;
+    for (int p = 0; p < 3; p++) {
+      d.plane_width[p] = p ? (f->width + f->ss_x) >> f->ss_x : f->width;
+      d.plane_height[p] = p ? (f->height + f->ss_y) >> f->ss_y : f->height;
+      d.plane_offset[p] = off;
+      off += (size_t)d.plane_width[p] * d.plane_height[p] * bps;
+    }
+    out.push_back(std::move(o));
+    return VP9R_OK;
+  }
+
+  int decode_frame(const uint8_t *data, size_t len) {
+    vp9r_record rec;
+    int st = vp9r_parser_parse_frame(&parser, data, len, &rec);
+    if (st) return st;
+    const vp9r_frame_hdr &h = rec.hdr;
+    frames++;
+    if (h.show_existing_frame) return emit(slots[h.frame_to_show_map_idx]);
+    auto cur = take(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y, &st);
+    if (!cur) return st;
+    cur->color_space = h.color_space;
+    cur->color_range = h.color_range;
+    cur->render_w = h.render_width;
+    cur->render_h = h.render_height;
+    vp9r_gpu_frame *refs[3] = {nullptr, nullptr, nullptr};
+    if (!h.frame_is_intra)
+      for (int i = 0; i < 3; i++) refs[i] = slots[h.ref_frame_idx[i]] ? slots[h.ref_frame_idx[i]]->f : nullptr;
+    st = vp9r_gpu_reconstruct(g, &rec, refs, cur->f);
+    if (st) return st;
+    st = vp9r_gpu_loop_filter(g, &rec, cur->f);
+    if (st) return st;
+    for (int i = 0; i < 8; i++)
+      if ((h.refresh_frame_flags >> i) & 1) slots[i] = cur;
+    if (h.show_frame) return emit(cur);
+    return VP9R_OK;
+  }
+};
+
+extern "C" {
+
+int vp9r_decoder_new(int device, vp9r_decoder **out) {
+  *out = nullptr;
+  vp9r_decoder *d = new (std::nothrow) vp9r_decoder();
+  if (!d) return VP9R_ERR_INVALID_INPUT;
+  int st = vp9r_gpu_new(device, &d->g);
+  if (st) {
+    delete d;
+    return st;
+  }
+  *out = d;
+  return VP9R_OK;
+}
+
+void vp9r_decoder_free(vp9r_decoder *d) {
+  if (!d) return;
+  if (d->g) (void)hipStreamSynchronize(d->g->stream);
+  for (auto &s : d->slots) s.reset();
+  d->pool.clear();
+  vp9r_gpu_free(d->g);
+  delete d;
+}
+
+void vp9r_decoder_set_threads(vp9r_decoder *d, uint32_t threads) { d->threads = threads; }
+
+uint32_t vp9r_decoder_threads(const vp9r_decoder *d) {
+  if (d->threads) return d->threads;
+  unsigned n = std::thread::hardware_concurrency();
+  return n ? n : 1;
+}
+
+void vp9r_decoder_set_max_pixels(vp9r_decoder *d, uint64_t pixels) { d->parser.p.max_pixels = pixels; }
+
+int vp9r_decoder_decode(vp9r_decoder *d, const uint8_t *packet, size_t len, uint32_t *n_frames) {
+  d->out.clear();
+  *n_frames = 0;
+  size_t offs[8], sizes[8];
+  size_t n = vp9r_superframe_split(packet, len, offs, sizes, 8);
+  for (size_t i = 0; i < n && i < 8; i++) {
+    int st = d->decode_frame(packet + offs[i], sizes[i]);
+    if (st) return st;
+  }
+  *n_frames = (uint32_t)d->out.size();
+  return VP9R_OK;
+}
+
+int vp9r_decoder_frame(const vp9r_decoder *d, uint32_t i, vp9r_frame *out) {
+  if (i >= d->out.size()) {
+    set_error("no such frame");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  *out = d->out[i].desc;
+  out->data = const_cast<uint8_t *>(d->out[i].data.data());
+  return VP9R_OK;
+}
+
+uint64_t vp9r_decoder_frames_decoded(const vp9r_decoder *d) { return d->frames; }
+
+}  // extern "C"

@@ -203,3 +203,8 @@ class Parser {
 };
 
 }  // namespace vp9r
+
+// The C ABI's parser handle.
+struct vp9r_parser {
+  vp9r::Parser p;
+};

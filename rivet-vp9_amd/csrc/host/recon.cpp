@@ -1,0 +1,661 @@
+// GPU reconstruction, host side: frame stores, per-frame work lists and the
+// launch sequence.  One record in, on one HIP stream:
+//   1. one staged host-to-device copy of the record and its work lists;
+//   2. inter prediction of every inter block (one launch: it reads only the
+//      references);
+//   3. inter residuals, one launch per transform size;
+//   4. intra prediction + residual as a superblock wavefront, one launch per
+//      dependency level (only superblocks with intra work take part);
+// and then, in vp9r_gpu_loop_filter, the loop filter decisions (one launch)
+// and the filter wavefront, one launch per level c + 2r.
+// This is the order of the reference's replay (src/decoder/block.rs:1405-1498)
+// up to reorderings that change no sample: inter pixels depend only on
+// references, so they may all precede the intra chain, which reads
+// reconstructed neighbours only.
+#include "recon.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "vp9_consts.h"
+#include "vp9_tables.h"
+
+namespace vp9r {
+
+hipError_t DevBuf::ensure(size_t n) {
+  if (n <= cap) return hipSuccess;
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
+  size_t want = std::max<size_t>(n + n / 2, 1 << 20);
+  hipError_t e = hipMalloc(&p, want);
+  if (e == hipSuccess) cap = want;
+  return e;
+}
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+hipError_t HostBuf::ensure(size_t n) {
+  if (n <= cap) return hipSuccess;
+  if (p) hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  size_t want = std::max<size_t>(n + n / 2, 1 << 20);
+  hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+  if (e == hipSuccess) cap = want;
+  return e;
+}
+HostBuf::~HostBuf() {
+  if (p) (void)hipHostFree(p);
+}
+
+static int hip_fail(hipError_t e, const char *what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return VP9R_ERR_DEVICE;
+}
+
+#define HIP_TRY(expr, what)                 \
+  do {                                      \
+    hipError_t e_ = (expr);                 \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+static inline int iclamp(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
+
+// qindex (src/decoder/recon.rs:219-230) and the dc/ac factors of reconstruct
+// (recon.rs:240-250) for every segment and plane type.
+static void q_table(const vp9r_frame_hdr &h, int32_t q[8][2][2]) {
+  int bd_idx = (h.bit_depth - 8) >> 1;
+  for (int s = 0; s < 8; s++) {
+    int qi = h.base_q_idx;
+    if (h.seg_enabled && h.seg_feature_enabled[s][SEG_LVL_ALT_Q]) {
+      int d = h.seg_feature_data[s][SEG_LVL_ALT_Q];
+      if (!h.seg_abs_or_delta_update) d += h.base_q_idx;
+      qi = iclamp(d, 0, 255);
+    }
+    q[s][0][0] = VP9_DC_QLOOKUP[bd_idx][iclamp(qi + h.delta_q_y_dc, 0, 255)];
+    q[s][0][1] = VP9_AC_QLOOKUP[bd_idx][iclamp(qi, 0, 255)];
+    q[s][1][0] = VP9_DC_QLOOKUP[bd_idx][iclamp(qi + h.delta_q_uv_dc, 0, 255)];
+    q[s][1][1] = VP9_AC_QLOOKUP[bd_idx][iclamp(qi + h.delta_q_uv_ac, 0, 255)];
+  }
+}
+
+// lvl_lookup (src/decoder/loopfilter.rs:15-47) and the limits of Ctx::new
+// (loopfilter.rs:135-170).
+static void lf_params(const vp9r_frame_hdr &h, vp9g::LfParams &p) {
+  int n_shift = h.lf_level >> 5;
+  for (int s = 0; s < 8; s++) {
+    int lvl_seg = h.lf_level;
+    if (h.seg_enabled && h.seg_feature_enabled[s][SEG_LVL_ALT_L]) {
+      int d = h.seg_feature_data[s][SEG_LVL_ALT_L];
+      lvl_seg = iclamp(h.seg_abs_or_delta_update ? d : d + h.lf_level, 0, MAX_LOOP_FILTER);
+    }
+    if (!h.lf_delta_enabled) {
+      for (int r = 0; r < 4; r++) p.lvl[s][r][0] = p.lvl[s][r][1] = (uint8_t)lvl_seg;
+    } else {
+      int intra = lvl_seg + h.lf_ref_deltas[INTRA_FRAME] * (1 << n_shift);
+      p.lvl[s][0][0] = p.lvl[s][0][1] = (uint8_t)iclamp(intra, 0, MAX_LOOP_FILTER);
+      for (int rf = 1; rf < 4; rf++)
+        for (int m = 0; m < 2; m++)
+          p.lvl[s][rf][m] = (uint8_t)iclamp(
+              lvl_seg + h.lf_ref_deltas[rf] * (1 << n_shift) + h.lf_mode_deltas[m] * (1 << n_shift), 0,
+              MAX_LOOP_FILTER);
+    }
+  }
+  int sharp = h.lf_sharpness, shift = sharp > 4 ? 2 : sharp > 0 ? 1 : 0;
+  for (int l = 0; l < 64; l++) {
+    int limit = sharp > 0 ? iclamp(l >> shift, 1, 9 - sharp) : std::max(l >> shift, 1);
+    p.limit[l] = (uint8_t)limit;
+    p.blimit[l] = (uint8_t)(2 * (l + 2) + limit);
+    p.thresh[l] = (uint8_t)(l >> 4);
+  }
+}
+
+static vp9g::FrameGeom geom_of(const vp9r_frame_hdr &h) {
+  vp9g::FrameGeom g{};
+  g.width = h.width;
+  g.height = h.height;
+  g.mi_cols = h.mi_cols;
+  g.mi_rows = h.mi_rows;
+  g.sb64_cols = h.sb64_cols;
+  g.sb64_rows = h.sb64_rows;
+  g.bit_depth = h.bit_depth;
+  g.ss_x = h.ss_x;
+  g.ss_y = h.ss_y;
+  g.legacy_uv = h.legacy_uv;
+  g.lossless = h.lossless;
+  return g;
+}
+
+// The inter prediction tiles of the record (block.rs:1425-1448): each
+// block's planes as one prediction unit, or one 4x4 unit per sub-block
+// below 8x8; units split into tiles of at most 8 x 8.
+static void build_mc_tiles(vp9r_gpu *g, const vp9r_record *rec) {
+  const vp9r_frame_hdr &h = rec->hdr;
+  auto &tiles = g->tiles;
+  tiles.clear();
+  for (uint32_t bi = 0; bi < rec->n_blocks; bi++) {
+    const vp9r_block &b = rec->blocks[bi];
+    if (!b.is_inter) continue;
+    int bsize = b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size;
+    for (int plane = 0; plane < 3; plane++) {
+      int sx = plane ? h.ss_x : 0, sy = plane ? h.ss_y : 0;
+      int psz = VP9_SS_SIZE_LOOKUP[bsize][sx][sy];
+      int n4w = VP9_NUM_4X4_WIDE[psz], n4h = VP9_NUM_4X4_HIGH[psz];
+      int base_x = (b.mi_col * 8) >> sx, base_y = (b.mi_row * 8) >> sy;
+      auto unit = [&](int x, int y, int w, int hh, int sub) {
+        for (int ty = 0; ty < hh; ty += 8)
+          for (int tx = 0; tx < w; tx += 8) {
+            vp9g::McTile t;
+            t.block = bi;
+            t.x = (uint16_t)x;
+            t.y = (uint16_t)y;
+            t.plane = (uint8_t)plane;
+            t.sub = (uint8_t)sub;
+            t.w = (uint8_t)w;
+            t.h = (uint8_t)hh;
+            t.tx = (uint8_t)tx;
+            t.ty = (uint8_t)ty;
+            t.tw = (uint8_t)std::min(8, w - tx);
+            t.th = (uint8_t)std::min(8, hh - ty);
+            tiles.push_back(t);
+          }
+      };
+      if (b.mi_size < BLOCK_8X8) {
+        for (int y = 0; y < n4h; y++)
+          for (int x = 0; x < n4w; x++) unit(base_x + 4 * x, base_y + 4 * y, 4, 4, y * n4w + x);
+      } else {
+        unit(base_x, base_y, n4w * 4, n4h * 4, 0);
+      }
+    }
+  }
+}
+
+// Intra groups per (plane, superblock) and their dependency levels.
+static void build_intra(vp9r_gpu *g, const vp9r_record *rec) {
+  const vp9r_frame_hdr &h = rec->hdr;
+  const uint32_t n_sb = h.sb64_rows * h.sb64_cols;
+  g->intra_cnt.assign(3 * n_sb, 0);
+  g->intra_fill.assign(3 * n_sb, 0);
+  auto key_of = [&](const vp9r_txb &t) {
+    int sx = t.plane ? h.ss_x : 0, sy = t.plane ? h.ss_y : 0;
+    uint32_t c = t.x / (64u >> sx), r = t.y / (64u >> sy);
+    return t.plane * n_sb + r * h.sb64_cols + c;
+  };
+  uint32_t n_intra = 0;
+  for (uint32_t i = 0; i < rec->n_txbs; i++) {
+    const vp9r_txb &t = rec->txbs[i];
+    if (rec->blocks[t.block].is_inter) continue;
+    g->intra_cnt[key_of(t)]++;
+    n_intra++;
+  }
+  g->intra_first.resize(3 * n_sb);
+  uint32_t acc = 0;
+  for (uint32_t k = 0; k < 3 * n_sb; k++) {
+    g->intra_first[k] = acc;
+    acc += g->intra_cnt[k];
+  }
+  g->intra_index.resize(n_intra);
+  for (uint32_t i = 0; i < rec->n_txbs; i++) {
+    const vp9r_txb &t = rec->txbs[i];
+    if (rec->blocks[t.block].is_inter) continue;
+    uint32_t k = key_of(t);
+    g->intra_index[g->intra_first[k] + g->intra_fill[k]++] = i;
+  }
+  // level = 1 + the deepest of the left, above and above-left groups
+  g->intra_level.assign(3 * n_sb, -1);
+  int max_level = -1;
+  for (int p = 0; p < 3; p++)
+    for (uint32_t r = 0; r < h.sb64_rows; r++)
+      for (uint32_t c = 0; c < h.sb64_cols; c++) {
+        uint32_t k = p * n_sb + r * h.sb64_cols + c;
+        if (!g->intra_cnt[k]) continue;
+        int lv = 0;
+        auto dep = [&](int rr, int cc) {
+          if (rr < 0 || cc < 0) return;
+          uint32_t d = p * n_sb + rr * h.sb64_cols + cc;
+          if (g->intra_cnt[d]) lv = std::max(lv, g->intra_level[d] + 1);
+        };
+        dep((int)r, (int)c - 1);
+        dep((int)r - 1, (int)c);
+        dep((int)r - 1, (int)c - 1);
+        g->intra_level[k] = lv;
+        max_level = std::max(max_level, lv);
+      }
+  g->groups.clear();
+  g->group_level_start.assign(max_level + 2, 0);
+  for (uint32_t k = 0; k < 3 * n_sb; k++)
+    if (g->intra_cnt[k]) g->group_level_start[g->intra_level[k] + 1]++;
+  for (int l = 0; l <= max_level; l++) g->group_level_start[l + 1] += g->group_level_start[l];
+  g->groups.resize(g->group_level_start[max_level + 1]);
+  std::vector<uint32_t> fill(g->group_level_start.begin(), g->group_level_start.end());
+  for (uint32_t k = 0; k < 3 * n_sb; k++) {
+    if (!g->intra_cnt[k]) continue;
+    vp9g::IntraGroup gr{};
+    uint32_t sb = k % n_sb;
+    gr.plane = (uint8_t)(k / n_sb);
+    gr.sb_row = (uint16_t)(sb / h.sb64_cols);
+    gr.sb_col = (uint16_t)(sb % h.sb64_cols);
+    gr.first = g->intra_first[k];
+    gr.count = g->intra_cnt[k];
+    g->groups[fill[g->intra_level[k]]++] = gr;
+  }
+}
+
+// Loop filter wavefront levels: superblocks with equal c + 2r, all planes.
+static void build_lf_levels(vp9r_gpu *g, uint32_t rows, uint32_t cols) {
+  if (g->lf_rows == rows && g->lf_cols == cols) return;
+  g->lf_rows = rows;
+  g->lf_cols = cols;
+  g->lf_codes.clear();
+  g->lf_level_start.clear();
+  uint32_t T = (cols - 1) + 2 * (rows - 1) + 1;
+  for (uint32_t t = 0; t < T; t++) {
+    g->lf_level_start.push_back((uint32_t)g->lf_codes.size());
+    for (uint32_t r = 0; r < rows; r++) {
+      int c = (int)t - 2 * (int)r;
+      if (c < 0 || c >= (int)cols) continue;
+      for (uint32_t p = 0; p < 3; p++) g->lf_codes.push_back(p << 28 | r << 14 | (uint32_t)c);
+    }
+  }
+  g->lf_level_start.push_back((uint32_t)g->lf_codes.size());
+}
+
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace vp9r
+
+using namespace vp9r;
+
+extern "C" {
+
+int vp9r_gpu_new(int device, vp9r_gpu **out) {
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= device || device < 0) {
+    set_error("no usable GPU for the reconstruction backend");
+    return VP9R_ERR_DEVICE;
+  }
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  vp9r_gpu *g = new vp9r_gpu();
+  g->device = device;
+  e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete g;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  HIP_TRY(hipEventCreateWithFlags(&g->ev_upload, hipEventDisableTiming), "hipEventCreate");
+  HIP_TRY(hipEventCreateWithFlags(&g->ev_lf_upload, hipEventDisableTiming), "hipEventCreate");
+  for (auto &ev : g->ev) HIP_TRY(hipEventCreate(&ev), "hipEventCreate");
+  *out = g;
+  return VP9R_OK;
+}
+
+void vp9r_gpu_free(vp9r_gpu *g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipEventDestroy(g->ev_upload);
+  (void)hipEventDestroy(g->ev_lf_upload);
+  for (auto &ev : g->ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+int vp9r_gpu_frame_new(vp9r_gpu *g, uint32_t width, uint32_t height, uint32_t bd, uint32_t ss_x,
+                       uint32_t ss_y, vp9r_gpu_frame **out) {
+  *out = nullptr;
+  if (width == 0 || height == 0 || width > 65536 || height > 65536 || (bd != 8 && bd != 10 && bd != 12) ||
+      ss_x > 1 || ss_y > 1) {
+    set_error("invalid frame geometry");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  vp9r_gpu_frame *f = new vp9r_gpu_frame();
+  f->g = g;
+  f->width = width;
+  f->height = height;
+  f->bit_depth = bd;
+  f->ss_x = ss_x;
+  f->ss_y = ss_y;
+  f->mi_cols = (width + 7) >> 3;
+  f->mi_rows = (height + 7) >> 3;
+  f->sb64_cols = (f->mi_cols + 7) >> 3;
+  f->sb64_rows = (f->mi_rows + 7) >> 3;
+  size_t bps = bd > 8 ? 2 : 1, off[3], total = 0;
+  for (int p = 0; p < 3; p++) {
+    f->planes.stride[p] = (f->sb64_cols * 64) >> (p ? ss_x : 0);
+    f->planes.rows[p] = (f->sb64_rows * 64) >> (p ? ss_y : 0);
+    off[p] = total;
+    total = align256(total + (size_t)f->planes.stride[p] * f->planes.rows[p] * bps);
+  }
+  hipError_t e = hipMalloc(&f->base, total);
+  if (e != hipSuccess) {
+    delete f;
+    return hip_fail(e, "hipMalloc(frame)");
+  }
+  e = hipMemsetAsync(f->base, 0, total, g->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(f->base);
+    delete f;
+    return hip_fail(e, "hipMemset(frame)");
+  }
+  f->bytes = total;
+  for (int p = 0; p < 3; p++) f->planes.p[p] = (uint8_t *)f->base + off[p];
+  *out = f;
+  return VP9R_OK;
+}
+
+void vp9r_gpu_frame_free(vp9r_gpu_frame *f) {
+  if (!f) return;
+  if (f->g) (void)hipStreamSynchronize(f->g->stream);
+  (void)hipFree(f->base);
+  delete f;
+}
+
+void vp9r_gpu_frame_geometry(const vp9r_gpu_frame *f, int plane, uint32_t *stride, uint32_t *rows) {
+  *stride = f->planes.stride[plane];
+  *rows = f->planes.rows[plane];
+}
+
+int vp9r_gpu_frame_upload_u16(vp9r_gpu_frame *f, int plane, const uint16_t *src) {
+  size_t n = (size_t)f->planes.stride[plane] * f->planes.rows[plane];
+  hipStream_t s = f->g->stream;
+  if (f->bit_depth > 8) {
+    HIP_TRY(hipMemcpyAsync(f->planes.p[plane], src, n * 2, hipMemcpyHostToDevice, s), "upload");
+  } else {
+    std::vector<uint8_t> tmp(n);
+    for (size_t i = 0; i < n; i++) tmp[i] = (uint8_t)src[i];
+    HIP_TRY(hipMemcpyAsync(f->planes.p[plane], tmp.data(), n, hipMemcpyHostToDevice, s), "upload");
+    HIP_TRY(hipStreamSynchronize(s), "upload sync");
+  }
+  HIP_TRY(hipStreamSynchronize(s), "upload sync");
+  return VP9R_OK;
+}
+
+int vp9r_gpu_frame_download_u16(vp9r_gpu_frame *f, int plane, uint16_t *dst) {
+  size_t n = (size_t)f->planes.stride[plane] * f->planes.rows[plane];
+  hipStream_t s = f->g->stream;
+  if (f->bit_depth > 8) {
+    HIP_TRY(hipMemcpyAsync(dst, f->planes.p[plane], n * 2, hipMemcpyDeviceToHost, s), "download");
+    HIP_TRY(hipStreamSynchronize(s), "download sync");
+  } else {
+    std::vector<uint8_t> tmp(n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), f->planes.p[plane], n, hipMemcpyDeviceToHost, s), "download");
+    HIP_TRY(hipStreamSynchronize(s), "download sync");
+    for (size_t i = 0; i < n; i++) dst[i] = tmp[i];
+  }
+  return VP9R_OK;
+}
+
+int vp9r_gpu_synchronize(vp9r_gpu *g) {
+  HIP_TRY(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
+  return VP9R_OK;
+}
+
+void vp9r_gpu_set_timing(vp9r_gpu *g, int enable) { g->timing = enable != 0; }
+
+int vp9r_gpu_stage_times(vp9r_gpu *g, float out[5]) {
+  if (!g->timing) {
+    set_error("timing is not enabled");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  HIP_TRY(hipEventSynchronize(g->ev[4]), "hipEventSynchronize");
+  for (int i = 0; i < 4; i++) HIP_TRY(hipEventElapsedTime(&out[i], g->ev[i], g->ev[i + 1]), "elapsed");
+  HIP_TRY(hipEventElapsedTime(&out[4], g->ev[0], g->ev[4]), "elapsed");
+  return VP9R_OK;
+}
+
+int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *const refs[3],
+                         vp9r_gpu_frame *cur) {
+  const vp9r_frame_hdr &h = rec->hdr;
+  if (h.show_existing_frame) {
+    set_error("a show_existing_frame record has nothing to reconstruct");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  if (!cur || cur->width != h.width || cur->height != h.height || cur->bit_depth != h.bit_depth ||
+      cur->ss_x != h.ss_x || cur->ss_y != h.ss_y) {
+    set_error("the frame store does not match the record's geometry");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  if (!h.frame_is_intra) {
+    for (int i = 0; i < 3; i++) {
+      const vp9r_gpu_frame *r = refs ? refs[i] : nullptr;
+      if (!r || r->bit_depth != h.bit_depth || r->ss_x != h.ss_x || r->ss_y != h.ss_y) {
+        set_error("missing reference, or a reference of another format");
+        return VP9R_ERR_BITSTREAM;
+      }
+      if (r == cur) {
+        set_error("the frame being reconstructed is one of its references");
+        return VP9R_ERR_INVALID_INPUT;
+      }
+    }
+  }
+  if (h.mi_cols != cur->mi_cols || h.mi_rows != cur->mi_rows || h.sb64_cols != cur->sb64_cols ||
+      h.sb64_rows != cur->sb64_rows) {
+    set_error("malformed record header");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  // Every index the kernels follow is checked here, so that no record can
+  // send a kernel outside its frame stores.
+  bool ref_ok[3] = {false, false, false};
+  if (!h.frame_is_intra)
+    for (int i = 0; i < 3; i++) {
+      uint32_t rw = refs[i]->width, rh = refs[i]->height;
+      ref_ok[i] = !(2 * h.width < rw || 2 * h.height < rh || h.width > 16 * rw || h.height > 16 * rh);
+    }
+  for (uint32_t bi = 0; bi < rec->n_blocks; bi++) {
+    const vp9r_block &b = rec->blocks[bi];
+    if (b.mi_size > BLOCK_64X64 || b.mi_row >= h.mi_rows || b.mi_col >= h.mi_cols ||
+        (b.mi_row & (VP9_NUM_8X8_HIGH[b.mi_size] - 1)) || (b.mi_col & (VP9_NUM_8X8_WIDE[b.mi_size] - 1)) ||
+        b.interp_filter > 3 || b.uv_mode > TM_PRED || b.y_mode > NEWMV) {
+      set_error("malformed record block");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+    if (b.is_inter) {
+      for (int l = 0; l < 1 + (b.ref_frame[1] > INTRA_FRAME); l++) {
+        int rf = b.ref_frame[l];
+        if (rf < LAST_FRAME || rf > ALTREF_FRAME || h.frame_is_intra) {
+          set_error("malformed record block reference");
+          return VP9R_ERR_INVALID_INPUT;
+        }
+        if (!ref_ok[rf - 1]) {
+          set_error("reference frame scaled beyond 2:1 / 1:16");
+          return VP9R_ERR_BITSTREAM;
+        }
+      }
+    } else {
+      for (int k = 0; k < 4; k++)
+        if (b.sub_modes[k] > TM_PRED) {
+          set_error("malformed record block");
+          return VP9R_ERR_INVALID_INPUT;
+        }
+    }
+  }
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  // ---- host work lists
+  build_mc_tiles(g, rec);
+  for (auto &j : g->jobs) j.clear();
+  for (uint32_t i = 0; i < rec->n_txbs; i++) {
+    const vp9r_txb &t = rec->txbs[i];
+    const uint32_t n0 = 4u << t.tx_size;
+    bool ok = t.tx_size <= 3 && t.plane < 3 && t.tx_type <= 3 && t.block < rec->n_blocks &&
+              (uint64_t)t.coef_start + t.ncoef <= rec->n_coefs && t.eob <= n0 * n0 && (t.x & (n0 - 1)) == 0 &&
+              (t.y & (n0 - 1)) == 0 && t.x + n0 <= cur->planes.stride[t.plane] &&
+              t.y + n0 <= cur->planes.rows[t.plane];
+    if (ok) {
+      const vp9r_block &b = rec->blocks[t.block];
+      int sx = t.plane ? h.ss_x : 0, sy = t.plane ? h.ss_y : 0;
+      uint32_t bx = (b.mi_col * 8u) >> sx, by = (b.mi_row * 8u) >> sy;
+      ok = t.x >= bx && t.y >= by;
+    }
+    if (!ok) {
+      set_error("malformed record transform block");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+    if (rec->blocks[t.block].is_inter && t.eob > 0) g->jobs[t.tx_size].push_back(i);
+  }
+  build_intra(g, rec);
+  // ---- staging layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const size_t o_blocks = take(sizeof(vp9r_block) * rec->n_blocks);
+  const size_t o_txbs = take(sizeof(vp9r_txb) * rec->n_txbs);
+  const size_t o_cpos = take(sizeof(uint16_t) * rec->n_coefs);
+  const size_t o_cval = take(sizeof(int32_t) * rec->n_coefs);
+  const size_t o_tiles = take(sizeof(vp9g::McTile) * g->tiles.size());
+  size_t o_jobs[4];
+  for (int k = 0; k < 4; k++) o_jobs[k] = take(sizeof(uint32_t) * g->jobs[k].size());
+  const size_t o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
+  const size_t o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
+  const size_t total = off;
+  if (g->upload_pending) {
+    HIP_TRY(hipEventSynchronize(g->ev_upload), "staging reuse");
+    g->upload_pending = false;
+  }
+  HIP_TRY(g->stage.ensure(total), "hipHostMalloc(staging)");
+  HIP_TRY(g->arena.ensure(total), "hipMalloc(arena)");
+  uint8_t *st = (uint8_t *)g->stage.p;
+  auto put = [&](size_t o, const void *src, size_t bytes) {
+    if (bytes) memcpy(st + o, src, bytes);
+  };
+  put(o_blocks, rec->blocks, sizeof(vp9r_block) * rec->n_blocks);
+  put(o_txbs, rec->txbs, sizeof(vp9r_txb) * rec->n_txbs);
+  put(o_cpos, rec->coef_pos, sizeof(uint16_t) * rec->n_coefs);
+  put(o_cval, rec->coef_val, sizeof(int32_t) * rec->n_coefs);
+  put(o_tiles, g->tiles.data(), sizeof(vp9g::McTile) * g->tiles.size());
+  for (int k = 0; k < 4; k++) put(o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
+  put(o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
+  put(o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
+  hipStream_t s = g->stream;
+  if (g->timing) HIP_TRY(hipEventRecord(g->ev[0], s), "hipEventRecord");
+  HIP_TRY(hipMemcpyAsync(g->arena.p, st, total, hipMemcpyHostToDevice, s), "upload record");
+  HIP_TRY(hipEventRecord(g->ev_upload, s), "hipEventRecord");
+  g->upload_pending = true;
+  uint8_t *dv = (uint8_t *)g->arena.p;
+  const vp9g::FrameGeom geo = geom_of(h);
+  int32_t q[8][2][2];
+  q_table(h, q);
+  // ---- inter prediction
+  if (!g->tiles.empty()) {
+    vp9g::McArgs a{};
+    a.g = geo;
+    a.blocks = (const vp9r_block *)(dv + o_blocks);
+    a.tiles = (const vp9g::McTile *)(dv + o_tiles);
+    a.cur = cur->planes;
+    for (int i = 0; i < 3; i++) {
+      a.ref[i] = refs[i]->planes;
+      a.ref_w[i] = refs[i]->width;
+      a.ref_h[i] = refs[i]->height;
+    }
+    vp9g::launch_mc(s, a, (uint32_t)g->tiles.size());
+  }
+  if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
+  // ---- inter residuals
+  {
+    vp9g::ItxArgs a{};
+    a.g = geo;
+    a.blocks = (const vp9r_block *)(dv + o_blocks);
+    a.txbs = (const vp9r_txb *)(dv + o_txbs);
+    a.coef_pos = (const uint16_t *)(dv + o_cpos);
+    a.coef_val = (const int32_t *)(dv + o_cval);
+    a.cur = cur->planes;
+    memcpy(a.q, q, sizeof q);
+    for (int k = 0; k < 4; k++) {
+      a.jobs = (const uint32_t *)(dv + o_jobs[k]);
+      vp9g::launch_itx_inter(s, a, k, (uint32_t)g->jobs[k].size());
+    }
+  }
+  if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
+  // ---- intra wavefront
+  if (!g->groups.empty()) {
+    vp9g::IntraArgs a{};
+    a.g = geo;
+    a.blocks = (const vp9r_block *)(dv + o_blocks);
+    a.txbs = (const vp9r_txb *)(dv + o_txbs);
+    a.txb_index = (const uint32_t *)(dv + o_iidx);
+    a.groups = (const vp9g::IntraGroup *)(dv + o_groups);
+    a.coef_pos = (const uint16_t *)(dv + o_cpos);
+    a.coef_val = (const int32_t *)(dv + o_cval);
+    a.cur = cur->planes;
+    memcpy(a.q, q, sizeof q);
+    for (size_t l = 0; l + 1 < g->group_level_start.size(); l++) {
+      uint32_t first = g->group_level_start[l], n = g->group_level_start[l + 1] - first;
+      vp9g::launch_intra(s, a, first, n);
+    }
+  }
+  if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
+  HIP_TRY(hipGetLastError(), "kernel launch");
+  return VP9R_OK;
+}
+
+int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cur) {
+  const vp9r_frame_hdr &h = rec->hdr;
+  if (!cur || cur->width != h.width || cur->height != h.height || cur->bit_depth != h.bit_depth ||
+      cur->ss_x != h.ss_x || cur->ss_y != h.ss_y) {
+    set_error("the frame store does not match the record's geometry");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  hipStream_t s = g->stream;
+  if (h.lf_level == 0) {
+    if (g->timing) HIP_TRY(hipEventRecord(g->ev[4], s), "hipEventRecord");
+    return VP9R_OK;
+  }
+  build_lf_levels(g, h.sb64_rows, h.sb64_cols);
+  const size_t n_mi = (size_t)h.mi_rows * h.mi_cols;
+  const size_t o_mi = 0, o_codes = align256(sizeof(vp9r_mi) * n_mi);
+  const size_t total = o_codes + sizeof(uint32_t) * g->lf_codes.size();
+  if (g->lf_upload_pending) {
+    HIP_TRY(hipEventSynchronize(g->ev_lf_upload), "staging reuse");
+    g->lf_upload_pending = false;
+  }
+  HIP_TRY(g->lf_stage.ensure(total), "hipHostMalloc(staging)");
+  HIP_TRY(g->lf_arena.ensure(total), "hipMalloc(arena)");
+  uint8_t *st = (uint8_t *)g->lf_stage.p;
+  memcpy(st + o_mi, rec->mi, sizeof(vp9r_mi) * n_mi);
+  memcpy(st + o_codes, g->lf_codes.data(), sizeof(uint32_t) * g->lf_codes.size());
+  HIP_TRY(hipMemcpyAsync(g->lf_arena.p, st, total, hipMemcpyHostToDevice, s), "upload mode info");
+  HIP_TRY(hipEventRecord(g->ev_lf_upload, s), "hipEventRecord");
+  g->lf_upload_pending = true;
+  uint8_t *dv = (uint8_t *)g->lf_arena.p;
+  const size_t n_sb = (size_t)h.sb64_rows * h.sb64_cols;
+  HIP_TRY(g->lf_mask.ensure(n_sb * 6 * 256 * 2), "hipMalloc(lf masks)");
+  vp9g::LfArgs a{};
+  a.g = geom_of(h);
+  a.mi = (const vp9r_mi *)(dv + o_mi);
+  a.mask_fs = (int8_t *)g->lf_mask.p;
+  a.mask_lvl = (uint8_t *)g->lf_mask.p + n_sb * 6 * 256;
+  a.cur = cur->planes;
+  lf_params(h, a.p);
+  a.sbs = (const uint32_t *)(dv + o_codes);
+  vp9g::launch_lf_mask(s, a);
+  for (size_t l = 0; l + 1 < g->lf_level_start.size(); l++) {
+    uint32_t first = g->lf_level_start[l], n = g->lf_level_start[l + 1] - first;
+    vp9g::launch_lf_level(s, a, first, n);
+  }
+  if (g->timing) HIP_TRY(hipEventRecord(g->ev[4], s), "hipEventRecord");
+  HIP_TRY(hipGetLastError(), "kernel launch");
+  return VP9R_OK;
+}
+
+int vp9r_gpu_download_packed(vp9r_gpu *g, vp9r_gpu_frame *f, uint8_t *dst) {
+  size_t bps = f->bit_depth > 8 ? 2 : 1, off = 0;
+  for (int p = 0; p < 3; p++) {
+    size_t w = p ? (f->width + f->ss_x) >> f->ss_x : f->width;
+    size_t hh = p ? (f->height + f->ss_y) >> f->ss_y : f->height;
+    HIP_TRY(hipMemcpy2DAsync(dst + off, w * bps, f->planes.p[p], (size_t)f->planes.stride[p] * bps, w * bps,
+                             hh, hipMemcpyDeviceToHost, g->stream),
+            "download frame");
+    off += w * hh * bps;
+  }
+  HIP_TRY(hipStreamSynchronize(g->stream), "download sync");
+  return VP9R_OK;
+}
+
+}  // extern "C"
