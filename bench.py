@@ -1,0 +1,328 @@
+#!/usr/bin/env python3
+"""Single-stream GPU reconstruction benchmark: 1080p decoded frames/s.
+
+Workload: the 1920x1080 8-bit 4:2:0 profile-0 stream of 60 frames from seed 2
+(64 moving textured rectangles over a panning background, 4 tile columns,
+base_q_idx 100, NEWMV / NEARESTMV / intra), made by the committed stream
+generator (tests/streamgen).  Every frame is parsed on the CPU and its record
+(blocks, transform blocks, coefficients, mode info) plus work lists are
+copied to HBM before the timed region; a step is the GPU reconstruction and
+loop filter of one frame (vp9r_gpu_run) from those resident inputs, with the
+reference slots chained exactly as the decoder chains them.  The stream is
+replayed from its key frame once it ends.
+
+Also printed (in the same JSON line): the roofline of the dominant stage,
+measured with HIP events on the stream the kernels run on over the timed
+region; per-stage times; the PCIe-inclusive rate (every frame also copied to
+pinned host memory on the copy stream); synchronous per-frame latency
+p50/p99; and the CPU baseline: the oracle's scalar reconstruction and loop
+filter of the same records on one host core, whose per-frame MD5s the GPU's
+output is checked against (bit_exact).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu]
+"""
+import argparse
+import ctypes
+import hashlib
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "rivet-vp9_amd")
+TESTS = os.path.join(ROOT, "tests")
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+STAGES = ("inter_pred", "inter_residual", "intra", "loop_filter")
+STAGE_KERNELS = ("k_mc", "k_itx_inter", "k_intra", "k_lf_mask + k_lf")
+STAGE_BYTES = ("mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
+
+
+def _package():
+    if "rivet_vp9_amd" in sys.modules:
+        return sys.modules["rivet_vp9_amd"]
+    spec = importlib.util.spec_from_file_location("rivet_vp9_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rivet_vp9_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class Schedule:
+    """The reference slot chaining of the decoder (decoder/mod.rs:398-415)
+    over a pool of HBM frame stores, for a list of parsed frames."""
+
+    def __init__(self, vp, gpu, frames):
+        self.vp, self.gpu, self.frames = vp, gpu, frames
+        self.slots = [None] * 8
+        self.pool = []
+
+    def store(self, h):
+        busy = {id(s) for s in self.slots if s is not None}
+        for f in self.pool:
+            if id(f) not in busy and (f.width, f.height, f.bit_depth, f.ss_x, f.ss_y) == (
+                    h.width, h.height, h.bit_depth, h.ss_x, h.ss_y):
+                return f
+        f = self.gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+        self.pool.append(f)
+        return f
+
+    def step(self, i):
+        """Queues frame i; returns the frame store it shows (or None)."""
+        fr = self.frames[i]
+        h = fr["hdr"]
+        if h.show_existing_frame:
+            return self.slots[h.frame_to_show_map_idx]
+        cur = self.store(h)
+        refs = [None] * 3 if h.frame_is_intra else [self.slots[h.ref_frame_idx[k]] for k in range(3)]
+        self.gpu.run(fr["prep"], refs, cur)
+        for k in range(8):
+            if (h.refresh_frame_flags >> k) & 1:
+                self.slots[k] = cur
+        return cur if h.show_frame else None
+
+
+def cpu_baseline(frames_bytes, budget_s):
+    """The oracle's scalar reconstruction + loop filter (one core) of the
+    parsed records, best pass over the whole stream, repeated within the time
+    budget; also the per-frame MD5s of the shown frames."""
+    vp = _package()
+    if TESTS not in sys.path:
+        sys.path.insert(0, TESTS)
+    import oracle_lib as ol
+    best, md5s, passes, spent = None, None, 0, 0.0
+    while passes < 1 or (spent < budget_s and passes < 5):
+        parser = vp.Parser()
+        slots = [None] * 8
+        md5_pass, t_recon = [], 0.0
+        for data in frames_bytes:
+            rec = parser.parse(data)
+            h = rec.hdr
+            if h.show_existing_frame:
+                md5_pass.append(ol.md5(slots[h.frame_to_show_map_idx].packed()))
+                continue
+            cur = ol.Frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+            refs = (ctypes.POINTER(ol.OrcFrame) * 3)()
+            if not h.frame_is_intra:
+                for k in range(3):
+                    refs[k] = slots[h.ref_frame_idx[k]].f
+            t0 = time.perf_counter()
+            if ol.lib().orc_reconstruct(ctypes.byref(rec), refs, cur.f):
+                raise RuntimeError("oracle reconstruction failed")
+            ol.lib().orc_loop_filter(ctypes.byref(rec), cur.f)
+            t_recon += time.perf_counter() - t0
+            for k in range(8):
+                if (h.refresh_frame_flags >> k) & 1:
+                    slots[k] = cur
+            if h.show_frame:
+                md5_pass.append(ol.md5(cur.packed()))
+        passes += 1
+        spent += t_recon
+        fps = len(frames_bytes) / t_recon
+        best = fps if best is None else max(best, fps)
+        md5s = md5_pass
+    return best, md5s, passes, spent
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600, help="timed frames")
+    ap.add_argument("--warmup", type=int, default=60, help="untimed frames before the timed region")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and the MD5 check)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the PCIe and latency passes")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: the single-stream reconstruction does not shard across GPUs "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    vp = _package()
+    vp.library()
+    if TESTS not in sys.path:
+        sys.path.insert(0, TESTS)
+    import streamgen
+
+    cfg = dict(streamgen.CONFIG2)
+    t0 = time.perf_counter()
+    frames_bytes = streamgen.generate(**cfg)
+    _log(f"bench: generated {len(frames_bytes)} frames ({sum(map(len, frames_bytes))} bytes) "
+         f"in {time.perf_counter() - t0:.2f}s")
+
+    gpu = vp.Gpu(0)
+    parser = vp.Parser()
+    frames = []
+    t0 = time.perf_counter()
+    t_parse = 0.0
+    for data in frames_bytes:
+        tp = time.perf_counter()
+        rec = parser.parse(data)
+        t_parse += time.perf_counter() - tp
+        hdr = vp.FrameHdr()
+        ctypes.memmove(ctypes.byref(hdr), ctypes.byref(rec.hdr), ctypes.sizeof(vp.FrameHdr))
+        prep = None if hdr.show_existing_frame else gpu.prepare(rec)
+        frames.append({"hdr": hdr, "prep": prep, "stats": prep.stats() if prep else None})
+    n = len(frames)
+    _log(f"bench: parsed and prepared {n} frames in {time.perf_counter() - t0:.2f}s "
+         f"(parse {n / t_parse:.1f} frames/s on one core)")
+
+    sched = Schedule(vp, gpu, frames)
+
+    # bit-exactness pass: every shown frame's MD5
+    gpu_md5 = []
+    for i in range(n):
+        shown = sched.step(i)
+        if shown is not None:
+            gpu_md5.append(hashlib.md5(shown.packed()).hexdigest())
+    gpu.synchronize()
+    _log(f"bench: verification pass done ({len(gpu_md5)} shown frames)")
+
+    # warmup, then the timed region (HIP events per stage on the kernel stream)
+    pos = 0
+    for _ in range(args.warmup):
+        sched.step(pos % n)
+        pos += 1
+    gpu.synchronize()
+    gpu.set_timing(True)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    timed_idx = []
+    for _ in range(args.steps):
+        timed_idx.append(pos % n)
+        sched.step(pos % n)
+        pos += 1
+    gpu.synchronize()
+    elapsed = time.perf_counter() - t0
+    stage_ms, timed_frames = gpu.stage_times()
+    gpu.set_timing(False)
+    fps = args.steps / elapsed
+    _log(f"bench: {args.steps} frames in {elapsed:.3f}s = {fps:.1f} frames/s")
+
+    # per-stage totals over the timed frames
+    work = {k: 0 for k in STAGE_BYTES}
+    launches = [0, 0, 0, 0]
+    for i in timed_idx:
+        st = frames[i]["stats"]
+        if st is None:
+            continue
+        for k in STAGE_BYTES:
+            work[k] += st[k]
+        launches[0] += 1 if st["mc_tiles"] else 0
+        launches[1] += sum(1 for k in ("itx4", "itx8", "itx16", "itx32") if st[k])
+        launches[2] += st["intra_levels"]
+        launches[3] += (st["lf_levels"] + 1) if st["lf_levels"] else 0
+    stages = {}
+    for s, name in enumerate(STAGES):
+        ms = stage_ms[s]
+        b = work[STAGE_BYTES[s]]
+        stages[name] = {
+            "kernel": STAGE_KERNELS[s],
+            "ms_per_frame": ms / max(timed_frames, 1),
+            "launches_per_frame": launches[s] / max(timed_frames, 1),
+            "avg_launch_us": 1000.0 * ms / max(launches[s], 1),
+            "bytes_per_frame": b / max(timed_frames, 1),
+            "GB_per_s": (b / (ms * 1e-3) / 1e9) if ms > 0 else None,
+        }
+    dom = max(range(4), key=lambda s: stage_ms[s])
+    dname = STAGES[dom]
+    achieved = stages[dname]["GB_per_s"] or 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": STAGE_KERNELS[dom], "stage": dname}
+
+    extras = {}
+    if not args.no_extras:
+        # PCIe-inclusive: every shown frame also copied to pinned host memory
+        size = vp.library().vp9r_packed_size(cfg["width"], cfg["height"], 8, 1, 1)
+        hosts = [vp.HostBuffer(size) for _ in range(4)]
+        pending = [None] * 4
+        k = 0
+        gpu.synchronize()
+        t0 = time.perf_counter()
+        for j in range(args.steps):
+            shown = sched.step(pos % n)
+            pos += 1
+            if shown is not None:
+                slot = k % 4
+                if pending[slot] is not None:
+                    pending[slot].wait()
+                shown.packed_async(hosts[slot])
+                pending[slot] = shown
+                k += 1
+        for f in pending:
+            if f is not None:
+                f.wait()
+        gpu.synchronize()
+        pcie_fps = args.steps / (time.perf_counter() - t0)
+        # synchronous latency: one frame at a time, submit -> frame on the host
+        lat = []
+        for j in range(min(args.steps, 2 * n)):
+            t1 = time.perf_counter()
+            shown = sched.step(pos % n)
+            pos += 1
+            if shown is not None:
+                shown.packed_async(hosts[0])
+                shown.wait()
+            else:
+                gpu.synchronize()
+            lat.append((time.perf_counter() - t1) * 1000.0)
+        lat.sort()
+        extras = {"pcie_inclusive_fps": round(pcie_fps, 2),
+                  "latency_ms": {"p50": round(lat[len(lat) // 2], 3),
+                                 "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+                                 "frames": len(lat), "mode": "one frame in flight, incl. D2H"}}
+        _log(f"bench: PCIe-inclusive {pcie_fps:.1f} frames/s; latency p50 {extras['latency_ms']['p50']} ms "
+             f"p99 {extras['latency_ms']['p99']} ms")
+
+    cpu = None
+    bit_exact = None
+    if not args.no_cpu:
+        cpu_fps, cpu_md5, passes, spent = cpu_baseline(frames_bytes, args.cpu_seconds)
+        bit_exact = cpu_md5 == gpu_md5
+        cpu = {"value": round(cpu_fps, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"oracle scalar reconstruction + loop filter of all {n} frames of the same stream "
+                         f"(records parsed outside the timing), best of {passes} passes, {spent:.1f}s of CPU work"}
+        _log(f"bench: CPU oracle {cpu_fps:.2f} frames/s; GPU MD5s {'==' if bit_exact else '!='} oracle MD5s")
+        if not bit_exact:
+            bad = [i for i, (a, b) in enumerate(zip(gpu_md5, cpu_md5)) if a != b]
+            _log(f"bench: MD5 MISMATCH at shown frames {bad[:10]}")
+
+    out = {
+        "metric": "1080p decoded frames/s GPU vs CPU (bit-exact); p99 frame latency @32 streams",
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "1920x1080 8-bit 4:2:0 profile 0, 60 frames, seed 2, 64 moving rectangles, "
+                               "4 tile columns, base_q_idx 100 (stream generator)",
+                   "frames_in_stream": n, "records_resident_in_hbm": True},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bit_exact": bit_exact,
+        "stages": stages,
+        "parse_fps_one_core": round(n / t_parse, 2),
+    }
+    out.update(extras)
+    print(json.dumps(out), flush=True)
+    if bit_exact is False:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

@@ -165,21 +165,53 @@ int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *co
                          vp9r_gpu_frame *cur);
 /* The loop filter of the reconstructed frame (loopfilter::filter_frame,
  * src/decoder/loopfilter.rs:51-81) from the record's mode info and header.
+ * `rec` is the record last given to vp9r_gpu_reconstruct on this context.
  * Asynchronous, like vp9r_gpu_reconstruct. */
 int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cur);
+
+/* The same two steps split at the host/device boundary: vp9r_gpu_prepare
+ * checks a record, forms its work lists and copies everything to HBM (and
+ * waits); vp9r_gpu_run then reconstructs and loop filters the frame from
+ * those HBM-resident inputs alone, asynchronously.  A prepared record can
+ * be run any number of times, with reference frames of the sizes it names. */
+typedef struct vp9r_prepared vp9r_prepared;
+int vp9r_gpu_prepare(vp9r_gpu *g, const vp9r_record *rec, vp9r_prepared **out);
+void vp9r_prepared_free(vp9r_prepared *p);
+int vp9r_gpu_run(vp9r_gpu *g, const vp9r_prepared *p, vp9r_gpu_frame *const refs[3],
+                 vp9r_gpu_frame *cur);
+/* Work of a prepared record: [0] inter prediction tiles (8x8), [1..4]
+ * inter transform blocks of 4x4 .. 32x32, [5] intra groups (superblock,
+ * plane), [6] intra wavefront launches, [7] loop filter wavefront launches,
+ * [8..11] algorithmic HBM bytes of inter prediction (reference footprints
+ * read + prediction written), inter residual (n^2 x (4 + 2b) per transform
+ * block with coefficients), intra (edges + block + coefficients) and the loop
+ * filter (2 x frame bytes), b = bytes per sample. */
+int vp9r_gpu_prepared_stats(const vp9r_prepared *p, uint64_t out[12]);
 /* Writes the visible frame in the crate's Frame::packed layout (Y, U, V; u8
  * at 8 bits, little-endian u16 above; frame.rs:112-229) to `dst`, which
  * holds vp9r_packed_size() bytes.  Waits for the frame. */
 int vp9r_gpu_download_packed(vp9r_gpu *g, vp9r_gpu_frame *f, uint8_t *dst);
+/* The same download queued on the context's copy stream behind the work
+ * queued so far, so that it overlaps the reconstruction of the next frames.
+ * `dst` (best pinned: vp9r_host_alloc) must stay valid until
+ * vp9r_gpu_frame_wait(f) returns; later work that writes f waits for the
+ * copy on the device. */
+int vp9r_gpu_download_packed_async(vp9r_gpu *g, vp9r_gpu_frame *f, uint8_t *dst);
+int vp9r_gpu_frame_wait(vp9r_gpu_frame *f);
+/* Page-locked host memory for the asynchronous downloads. */
+int vp9r_host_alloc(size_t bytes, void **out);
+void vp9r_host_free(void *p);
 size_t vp9r_packed_size(uint32_t width, uint32_t height, uint32_t bit_depth, uint32_t ss_x,
                         uint32_t ss_y);
 int vp9r_gpu_synchronize(vp9r_gpu *g);
-/* Per-stage device time of the last vp9r_gpu_reconstruct, in ms (HIP events
- * on the context's stream): [0] inter prediction, [1] inter residual,
- * [2] intra wavefront, [3] loop filter, [4] whole call.  Only filled when
- * timing was enabled. */
+/* Per-stage device time, in ms (HIP events on the context's stream, where
+ * the kernels run), summed over the frames reconstructed since timing was
+ * enabled or last read: [0] inter prediction (with the record upload in
+ * vp9r_gpu_reconstruct), [1] inter residual, [2] intra wavefront, [3] loop
+ * filter, [4] whole frame.  *frames (may be NULL) gets the frame count.
+ * Waits for the last timed frame.  Read it after a frame's loop filter. */
 void vp9r_gpu_set_timing(vp9r_gpu *g, int enable);
-int vp9r_gpu_stage_times(vp9r_gpu *g, float out[5]);
+int vp9r_gpu_stage_times(vp9r_gpu *g, float out[5], uint32_t *frames);
 
 /* ------------------------------------------------------------------------ */
 /* Per-kernel seam: src/dsp signatures as batched device launches.          */

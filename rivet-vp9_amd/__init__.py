@@ -152,14 +152,22 @@ def library():
     lib.vp9r_gpu_reconstruct.argtypes = [P, ctypes.POINTER(Record), ctypes.c_void_p * 3, P]
     lib.vp9r_gpu_loop_filter.argtypes = [P, ctypes.POINTER(Record), P]
     lib.vp9r_gpu_download_packed.argtypes = [P, P, V]
+    lib.vp9r_gpu_download_packed_async.argtypes = [P, P, V]
+    lib.vp9r_gpu_frame_wait.argtypes = [P]
+    lib.vp9r_host_alloc.argtypes = [sz, PP]
+    lib.vp9r_host_free.argtypes = [V]
     lib.vp9r_gpu_synchronize.argtypes = [P]
     lib.vp9r_gpu_set_timing.argtypes = [P, ctypes.c_int]
-    lib.vp9r_gpu_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+    lib.vp9r_gpu_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32)]
     lib.vp9r_itx_add_batch.argtypes = [P, u32, V, V, V, V, V, V, V, u32, u32, ctypes.c_int]
     lib.vp9r_mc_batch.argtypes = [P, u32, V, V, u32, u32, i32, i32, V, u32, u32]
     lib.vp9r_lf_edges_batch.argtypes = [P, u32, V, V, V, V, V, V, V, u32, u32, u32]
     lib.vp9r_itx_add_host.argtypes = [P, V, u32, u32, ctypes.c_int, u32, u32, V, u32]
     lib.vp9r_intra_predict_host.argtypes = [P, u32, u32, V, V, ctypes.c_int, ctypes.c_int, u32, V, u32]
+    lib.vp9r_gpu_prepare.argtypes = [P, ctypes.POINTER(Record), PP]
+    lib.vp9r_prepared_free.argtypes = [P]
+    lib.vp9r_gpu_run.argtypes = [P, P, ctypes.c_void_p * 3, P]
+    lib.vp9r_gpu_prepared_stats.argtypes = [P, ctypes.POINTER(u64)]
     lib.vp9r_gpu_malloc.argtypes = [P, sz, PP]
     lib.vp9r_gpu_mfree.argtypes = [P, V]
     lib.vp9r_gpu_to_device.argtypes = [P, V, V, sz]
@@ -267,9 +275,20 @@ class Gpu:
         library().vp9r_gpu_set_timing(self._g, 1 if on else 0)
 
     def stage_times(self):
+        """(ms per stage summed over the frames timed since the last read:
+        inter prediction, inter residual, intra, loop filter, whole; frames)."""
         out = (ctypes.c_float * 5)()
-        _check(library().vp9r_gpu_stage_times(self._g, out))
-        return list(out)
+        n = u32()
+        _check(library().vp9r_gpu_stage_times(self._g, out, ctypes.byref(n)))
+        return list(out), n.value
+
+    def prepare(self, rec):
+        """The record's inputs copied to HBM (a Prepared), for run()."""
+        return Prepared(self, rec)
+
+    def run(self, prepared, refs, cur):
+        arr = (ctypes.c_void_p * 3)(*[(r.handle if r is not None else None) for r in refs])
+        _check(library().vp9r_gpu_run(self._g, prepared.handle, arr, cur.handle))
 
     def buffer(self, nbytes):
         return DevBuffer(self, nbytes)
@@ -281,6 +300,33 @@ class Gpu:
         b = DevBuffer(self, a.nbytes)
         _check(library().vp9r_gpu_to_device(self._g, b.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
         return b
+
+
+class Prepared:
+    """A record whose inputs (and work lists) are resident in HBM."""
+
+    def __init__(self, gpu, rec):
+        self.gpu = gpu
+        self.hdr = FrameHdr()
+        ctypes.memmove(ctypes.byref(self.hdr), ctypes.byref(rec.hdr), ctypes.sizeof(FrameHdr))
+        self._p = ctypes.c_void_p()
+        _check(library().vp9r_gpu_prepare(gpu.handle, ctypes.byref(rec), ctypes.byref(self._p)))
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            library().vp9r_prepared_free(self._p)
+            self._p = None
+
+    @property
+    def handle(self):
+        return self._p
+
+    def stats(self):
+        out = (u64 * 12)()
+        library().vp9r_gpu_prepared_stats(self._p, out)
+        keys = ("mc_tiles", "itx4", "itx8", "itx16", "itx32", "intra_groups", "intra_levels", "lf_levels",
+                "mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
+        return dict(zip(keys, list(out)))
 
 
 class DevBuffer:
@@ -311,6 +357,24 @@ class DevBuffer:
         assert a.nbytes <= self.nbytes
         _check(library().vp9r_gpu_to_device(self.gpu.handle, self.ptr, a.ctypes.data_as(ctypes.c_void_p),
                                             a.nbytes))
+
+
+class HostBuffer:
+    """Page-locked host memory (vp9r_host_alloc)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        p = ctypes.c_void_p()
+        _check(library().vp9r_host_alloc(nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            library().vp9r_host_free(self.ptr)
+            self.ptr = None
+
+    def bytes(self, n=None):
+        return ctypes.string_at(self.ptr, self.nbytes if n is None else n)
 
 
 class GpuFrame:
@@ -345,6 +409,16 @@ class GpuFrame:
         buf = (ctypes.c_uint8 * n)()
         _check(library().vp9r_gpu_download_packed(self.gpu.handle, self._f, buf))
         return bytes(buf)
+
+    def packed_async(self, host):
+        """Queues the packed download into `host` (a HostBuffer) on the copy
+        stream; wait() ends it."""
+        n = library().vp9r_packed_size(self.width, self.height, self.bit_depth, self.ss_x, self.ss_y)
+        assert host.nbytes >= n
+        _check(library().vp9r_gpu_download_packed_async(self.gpu.handle, self._f, host.ptr))
+
+    def wait(self):
+        _check(library().vp9r_gpu_frame_wait(self._f))
 
     def download_plane(self, plane):
         """The whole plane (stride x rows) as a list-like array of u16."""

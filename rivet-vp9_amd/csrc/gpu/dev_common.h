@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #define DEV __device__ __forceinline__
+#define HDEV __host__ __device__ __forceinline__
 
 namespace vp9g {
 
@@ -317,7 +318,7 @@ struct Pix<0> {
   typedef uint16_t T;
 };
 
-DEV int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+HDEV int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 DEV int64_t lclamp(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // per-frame dequantisation factors: [segment][plane > 0][dc, ac]
@@ -339,9 +340,9 @@ constexpr uint64_t pack3(const int (&v)[13]) {
 }
 constexpr uint64_t kW8P = pack3(kW8Log2), kH8P = pack3(kH8Log2), kW4P = pack3(kW4Log2),
                    kH4P = pack3(kH4Log2);
-DEV int num8x8_wide(int s) { return 1 << ((kW8P >> (3 * s)) & 7); }
-DEV int num8x8_high(int s) { return 1 << ((kH8P >> (3 * s)) & 7); }
-DEV int num4x4_wide(int s) { return 1 << ((kW4P >> (3 * s)) & 7); }
-DEV int num4x4_high(int s) { return 1 << ((kH4P >> (3 * s)) & 7); }
+HDEV int num8x8_wide(int s) { return 1 << ((kW8P >> (3 * s)) & 7); }
+HDEV int num8x8_high(int s) { return 1 << ((kH8P >> (3 * s)) & 7); }
+HDEV int num4x4_wide(int s) { return 1 << ((kW4P >> (3 * s)) & 7); }
+HDEV int num4x4_high(int s) { return 1 << ((kH4P >> (3 * s)) & 7); }
 
 }  // namespace vp9g

@@ -52,6 +52,7 @@ class BitReader {
 
 class BoolDecoder {
  public:
+  static constexpr bool kWrite = false;
   BoolDecoder(const uint8_t *d, size_t n) : p_(d), end_(d + n) {
     if (n == 0) bitstream("boolean-coded partition of size 0");
     refill();
@@ -76,14 +77,17 @@ class BoolDecoder {
     if (avail_ < 16) refill();
     return bit;
   }
-  uint32_t literal(int n) {
+  uint32_t literal(int n, uint32_t = 0) {
     uint32_t x = 0;
     for (int i = 0; i < n; i++) x = (x << 1) | (uint32_t)read(128);
     return x;
   }
+  // The syntax walk calls bit/tree with the value a writer would code; a
+  // reader ignores it.
+  inline bool bit(uint32_t prob, bool = false) { return read(prob); }
   // tree decode (9.3.3): probability of node n>>1 from prob(node)
   template <typename F>
-  inline int tree(const int8_t *t, F prob) {
+  inline int tree(const int8_t *t, F prob, int = 0) {
     int n = 0;
     do {
       n = t[n + (int)read(prob(n >> 1))];
@@ -320,7 +324,7 @@ void Parser::parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind
       if (h.profile > 0) {
         color_config(r, h);
       } else {
-        h.color_space = 2;  // BT.601
+        h.color_space = 1;  // BT.601
         h.ss_x = h.ss_y = 1;
         h.bit_depth = 8;
       }
@@ -443,6 +447,7 @@ void Parser::parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind
   }
   h.tile_rows_log2 = r.f(1);
   if (h.tile_rows_log2 == 1) h.tile_rows_log2 += r.f(1);
+  h.header_size_pos = r.position();
   h.header_size_in_bytes = r.f(16);
   r.byte_align();
   h.uncompressed_size = r.position() / 8;
@@ -451,7 +456,8 @@ void Parser::parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind
 // --------------------------------------------------------------------------
 // Compressed header (6.3; src/header.rs:470-680).
 
-static uint32_t decode_term_subexp(BoolDecoder &d) {
+template <class Coder>
+static uint32_t decode_term_subexp(Coder &d) {
   if (d.literal(1) == 0) return d.literal(4);
   if (d.literal(1) == 0) return d.literal(4) + 16;
   if (d.literal(1) == 0) return d.literal(5) + 32;
@@ -478,23 +484,33 @@ static uint8_t inv_remap_prob(uint32_t delta, uint8_t prob) {
   return (uint8_t)std::min(std::max(r, 1), 255);
 }
 
-static void diff_update_prob(BoolDecoder &d, uint8_t *p) {
-  if (d.read(252)) *p = inv_remap_prob(decode_term_subexp(d), *p);
+// (a writer codes "no update" for every probability)
+template <class Coder>
+static void diff_update_prob(Coder &d, uint8_t *p) {
+  if (d.bit(252, false)) *p = inv_remap_prob(decode_term_subexp(d), *p);
 }
 
-static void update_mv_prob(BoolDecoder &d, uint8_t *p) {
-  if (d.read(252)) *p = (uint8_t)((d.literal(7) << 1) | 1);
+template <class Coder>
+static void update_mv_prob(Coder &d, uint8_t *p) {
+  if (d.bit(252, false)) *p = (uint8_t)((d.literal(7) << 1) | 1);
 }
 
 void Parser::parse_compressed(const uint8_t *data, size_t len) {
   BoolDecoder d(data, len);
+  compressed_syntax(d);
+}
+
+template <class Coder>
+void Parser::compressed_syntax(Coder &d) {
   FrameHeader &h = h_;
+  const uint32_t want_reference_mode = h.reference_mode;
   Probs &p = probs_;
   if (h.lossless) {
     h.tx_mode = ONLY_4X4;
   } else {
-    h.tx_mode = d.literal(2);
-    if (h.tx_mode == ALLOW_32X32) h.tx_mode += d.literal(1);
+    uint32_t want = h.tx_mode;
+    h.tx_mode = d.literal(2, std::min<uint32_t>(want, ALLOW_32X32));
+    if (h.tx_mode == ALLOW_32X32) h.tx_mode += d.literal(1, want == TX_MODE_SELECT);
   }
   if (h.tx_mode == TX_MODE_SELECT) {
     for (int i = 0; i < 2; i++) diff_update_prob(d, &p.tx[1][i][0]);
@@ -505,7 +521,7 @@ void Parser::parse_compressed(const uint8_t *data, size_t len) {
   }
   int max_tx = VP9_TX_MODE_TO_BIGGEST_TX_SIZE[h.tx_mode];
   for (int tx = 0; tx <= max_tx; tx++) {
-    if (d.literal(1) == 1)
+    if (d.literal(1, 0) == 1)
       for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++)
           for (int k = 0; k < 6; k++)
@@ -525,10 +541,11 @@ void Parser::parse_compressed(const uint8_t *data, size_t len) {
   const bool *sb = h.ref_frame_sign_bias;
   bool compound_allowed = sb[2] != sb[1] || sb[3] != sb[1];
   if (compound_allowed) {
-    if (d.literal(1) == 0) {
+    uint32_t want = want_reference_mode;
+    if (d.literal(1, want != SINGLE_REFERENCE) == 0) {
       h.reference_mode = SINGLE_REFERENCE;
     } else {
-      h.reference_mode = d.literal(1) == 0 ? COMPOUND_REFERENCE : REFERENCE_MODE_SELECT;
+      h.reference_mode = d.literal(1, want == REFERENCE_MODE_SELECT) == 0 ? COMPOUND_REFERENCE : REFERENCE_MODE_SELECT;
       // setup_compound_reference_mode()
       if (sb[LAST_FRAME] == sb[GOLDEN_FRAME]) {
         h.comp_fixed_ref = ALTREF_FRAME;
@@ -685,9 +702,16 @@ struct Block {
   int32_t nearest_mv[2][2], near_mv[2][2], best_mv[2][2], mv[2][2];
 };
 
-class TileDecoder {
+// The tile syntax walk (6.4), over a reader (BoolDecoder) or, for the stream
+// generator, a writer whose values come from a SyntaxPlan.  Every context,
+// motion vector candidate list and token context is the same code either
+// way, so what is written is what is read.
+template <class Coder>
+class TileWalker {
  public:
-  TileDecoder(Parser &p) : P(p), h(p.h_), seg(p.seg_), probs(p.probs_), counts(p.counts_) {
+  static constexpr bool W = Coder::kWrite;
+  SyntaxPlan *plan = nullptr;
+  TileWalker(Parser &p) : P(p), h(p.h_), seg(p.seg_), probs(p.probs_), counts(p.counts_) {
     uint32_t n4w = h.sb64_cols * 16 + 16, n4h = h.sb64_rows * 16 + 16;
     for (int i = 0; i < 3; i++) {
       above_nz[i].assign(n4w, 0);
@@ -702,18 +726,22 @@ class TileDecoder {
     count_tokens = !h.error_resilient_mode && !h.frame_parallel_decoding_mode;
   }
 
-  void decode_tile(const uint8_t *data, size_t len, uint32_t row, uint32_t col) {
+  void walk_tile(Coder &d, uint32_t row, uint32_t col) {
     row_start = tile_offset(row, h.mi_rows, h.tile_rows_log2);
     row_end = tile_offset(row + 1, h.mi_rows, h.tile_rows_log2);
     col_start = tile_offset(col, h.mi_cols, h.tile_cols_log2);
     col_end = tile_offset(col + 1, h.mi_cols, h.tile_cols_log2);
-    BoolDecoder d(data, len);
     for (uint32_t r = row_start; r < row_end; r += 8) {
       for (int p = 0; p < 3; p++) std::fill(left_nz[p].begin(), left_nz[p].end(), 0);
       std::fill(left_part.begin(), left_part.end(), 0);
       std::fill(left_segp.begin(), left_segp.end(), 0);
       for (uint32_t c = col_start; c < col_end; c += 8) decode_partition(d, r, c, BLOCK_64X64);
     }
+  }
+
+  void decode_tile(const uint8_t *data, size_t len, uint32_t row, uint32_t col) {
+    BoolDecoder d(data, len);
+    walk_tile(d, row, col);
     padding_ok = padding_ok && d.padding_is_zero();
   }
 
@@ -728,6 +756,7 @@ class TileDecoder {
   std::vector<uint8_t> above_nz[3], left_nz[3], above_part, left_part, above_segp, left_segp;
   uint32_t row_start = 0, row_end = 0, col_start = 0, col_end = 0;
   Block b;
+  PlanBlock pb{};  // the generator's decisions for the block (writing only)
   uint32_t eob_total = 0;
   bool count_tokens;
   int32_t coefs[1024];
@@ -735,7 +764,7 @@ class TileDecoder {
 
   MiInfo &mi_at(uint32_t r, uint32_t c) { return P.mi_[(size_t)r * h.mi_cols + c]; }
 
-  void decode_partition(BoolDecoder &d, uint32_t r, uint32_t c, int bsize) {
+  void decode_partition(Coder &d, uint32_t r, uint32_t c, int bsize) {
     if (r >= h.mi_rows || c >= h.mi_cols) return;
     uint32_t num8x8 = VP9_NUM_8X8_WIDE[bsize];
     uint32_t half = num8x8 >> 1;
@@ -751,13 +780,15 @@ class TileDecoder {
     bool a = (above & (1u << boffset)) != 0, l = (left & (1u << boffset)) != 0;
     uint32_t ctx = bsl * 4 + (l ? 2 : 0) + (a ? 1 : 0);
     const uint8_t *pp = h.frame_is_intra ? VP9_KF_PARTITION_PROBS[ctx] : probs.partition[ctx];
+    int want = 0;
+    if constexpr (W) want = plan->partition(r, c, bsize);
     int partition;
     if (has_rows && has_cols)
-      partition = d.tree(VP9_PARTITION_TREE, [&](int n) { return pp[n]; });
+      partition = d.tree(VP9_PARTITION_TREE, [&](int n) { return pp[n]; }, want);
     else if (has_cols)
-      partition = d.read(pp[1]) ? PARTITION_SPLIT : PARTITION_HORZ;
+      partition = d.bit(pp[1], want == PARTITION_SPLIT) ? PARTITION_SPLIT : PARTITION_HORZ;
     else if (has_rows)
-      partition = d.read(pp[2]) ? PARTITION_SPLIT : PARTITION_VERT;
+      partition = d.bit(pp[2], want == PARTITION_SPLIT) ? PARTITION_SPLIT : PARTITION_VERT;
     else
       partition = PARTITION_SPLIT;
     if (!h.frame_is_intra) counts.partition[ctx][partition]++;
@@ -787,7 +818,7 @@ class TileDecoder {
     }
   }
 
-  void decode_block(BoolDecoder &d, uint32_t r, uint32_t c, int subsize) {
+  void decode_block(Coder &d, uint32_t r, uint32_t c, int subsize) {
     if (h.ss_x + h.ss_y > 0 && subsize >= BLOCK_8X8 &&
         VP9_SS_SIZE_LOOKUP[subsize][h.ss_x][h.ss_y] == BLOCK_INVALID)
       bitstream("block size invalid for the chroma subsampling");
@@ -801,6 +832,10 @@ class TileDecoder {
     b.left = b.avail_l ? &mi_at(r, c - 1) : nullptr;
     b.ref_frame[0] = INTRA_FRAME;
     b.ref_frame[1] = REF_NONE;
+    if constexpr (W) {
+      pb = PlanBlock{};
+      plan->block(r, c, subsize, h.frame_is_intra, &pb);
+    }
     if (h.frame_is_intra)
       intra_frame_mode_info(d);
     else
@@ -849,9 +884,9 @@ class TileDecoder {
 
   // ---- mode info (6.4.5 - 6.4.17)
 
-  void intra_frame_mode_info(BoolDecoder &d) {
+  void intra_frame_mode_info(Coder &d) {
     b.segment_id = (seg.enabled && seg.update_map)
-                       ? (uint8_t)d.tree(VP9_SEGMENT_TREE, [&](int n) { return seg.tree_probs[n]; })
+                       ? (uint8_t)d.tree(VP9_SEGMENT_TREE, [&](int n) { return seg.tree_probs[n]; }, pb.segment_id)
                        : 0;
     read_skip(d);
     read_tx_size(d, true);
@@ -862,7 +897,7 @@ class TileDecoder {
       int am = b.above ? b.above->sub_modes[2] : DC_PRED;
       int lm = b.left ? b.left->sub_modes[1] : DC_PRED;
       const uint8_t *p = VP9_KF_Y_MODE_PROBS[am][lm];
-      int mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; });
+      int mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; }, pb.y_mode);
       b.y_mode = (uint8_t)mode;
       for (int i = 0; i < 4; i++) b.sub_modes[i] = (uint8_t)mode;
     } else {
@@ -873,27 +908,27 @@ class TileDecoder {
           int am = idy > 0 ? b.sub_modes[idx] : (b.above ? b.above->sub_modes[2 + idx] : DC_PRED);
           int lm = idx > 0 ? b.sub_modes[idy * 2] : (b.left ? b.left->sub_modes[1 + idy * 2] : DC_PRED);
           const uint8_t *p = VP9_KF_Y_MODE_PROBS[am][lm];
-          mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; });
+          mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; }, pb.sub_modes[idy * 2 + idx]);
           for (int y2 = 0; y2 < n4h; y2++)
             for (int x2 = 0; x2 < n4w; x2++) b.sub_modes[(idy + y2) * 2 + idx + x2] = (uint8_t)mode;
         }
       b.y_mode = (uint8_t)mode;
     }
     const uint8_t *p = VP9_KF_UV_MODE_PROBS[b.y_mode];
-    b.uv_mode = (uint8_t)d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; });
+    b.uv_mode = (uint8_t)d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; }, pb.uv_mode);
   }
 
-  void read_skip(BoolDecoder &d) {
+  void read_skip(Coder &d) {
     if (seg.active(b.segment_id, SEG_LVL_SKIP)) {
       b.skip = true;
     } else {
       int ctx = (b.above ? b.above->skip : 0) + (b.left ? b.left->skip : 0);
-      b.skip = d.read(probs.skip[ctx]);
+      b.skip = d.bit(probs.skip[ctx], pb.skip);
       counts.skip[ctx][b.skip]++;
     }
   }
 
-  void read_tx_size(BoolDecoder &d, bool allow_select) {
+  void read_tx_size(Coder &d, bool allow_select) {
     int max_tx = VP9_MAX_TXSIZE_LOOKUP[b.mi_size];
     if (allow_select && h.tx_mode == TX_MODE_SELECT && b.mi_size >= BLOCK_8X8) {
       int above = max_tx, left = max_tx;
@@ -906,7 +941,7 @@ class TileDecoder {
       const int8_t *tree = max_tx == TX_32X32   ? VP9_TX_SIZE_32_TREE
                            : max_tx == TX_16X16 ? VP9_TX_SIZE_16_TREE
                                                 : VP9_TX_SIZE_8_TREE;
-      int v = d.tree(tree, [&](int n) { return p[n]; });
+      int v = d.tree(tree, [&](int n) { return p[n]; }, std::min<int>(pb.tx_size, max_tx));
       counts.tx[max_tx][ctx][v]++;
       b.tx_size = (uint8_t)v;
     } else {
@@ -914,7 +949,7 @@ class TileDecoder {
     }
   }
 
-  void inter_frame_mode_info(BoolDecoder &d) {
+  void inter_frame_mode_info(Coder &d) {
     inter_segment_id(d);
     read_skip(d);
     read_is_inter(d);
@@ -938,7 +973,7 @@ class TileDecoder {
     return s;
   }
 
-  void inter_segment_id(BoolDecoder &d) {
+  void inter_segment_id(Coder &d) {
     if (!seg.enabled) {
       b.segment_id = 0;
       return;
@@ -950,15 +985,15 @@ class TileDecoder {
     }
     if (seg.temporal_update) {
       int ctx = left_segp[b.mi_row] + above_segp[b.mi_col];
-      bool pred = d.read(seg.pred_probs[ctx]);
+      bool pred = d.bit(seg.pred_probs[ctx], pb.segment_id == predicted);
       b.segment_id =
           pred ? predicted
-               : (uint8_t)d.tree(VP9_SEGMENT_TREE, [&](int n) { return seg.tree_probs[n]; });
+               : (uint8_t)d.tree(VP9_SEGMENT_TREE, [&](int n) { return seg.tree_probs[n]; }, pb.segment_id);
       uint32_t bw = VP9_NUM_8X8_WIDE[b.mi_size], bh = VP9_NUM_8X8_HIGH[b.mi_size];
       for (uint32_t i = 0; i < bw; i++) above_segp[b.mi_col + i] = pred;
       for (uint32_t i = 0; i < bh; i++) left_segp[b.mi_row + i] = pred;
     } else {
-      b.segment_id = (uint8_t)d.tree(VP9_SEGMENT_TREE, [&](int n) { return seg.tree_probs[n]; });
+      b.segment_id = (uint8_t)d.tree(VP9_SEGMENT_TREE, [&](int n) { return seg.tree_probs[n]; }, pb.segment_id);
     }
   }
 
@@ -972,7 +1007,7 @@ class TileDecoder {
     o[1] = b.above ? b.above->ref_frame[1] : REF_NONE;
   }
 
-  void read_is_inter(BoolDecoder &d) {
+  void read_is_inter(Coder &d) {
     if (seg.active(b.segment_id, SEG_LVL_REF_FRAME)) {
       b.is_inter = seg.feature_data[b.segment_id][SEG_LVL_REF_FRAME] != INTRA_FRAME;
       return;
@@ -989,17 +1024,17 @@ class TileDecoder {
       ctx = 2 * (au ? ai : li);
     else
       ctx = 0;
-    b.is_inter = d.read(probs.is_inter[ctx]);
+    b.is_inter = d.bit(probs.is_inter[ctx], pb.is_inter);
     counts.is_inter[ctx][b.is_inter]++;
   }
 
-  void intra_block_mode_info(BoolDecoder &d) {
+  void intra_block_mode_info(Coder &d) {
     b.ref_frame[0] = INTRA_FRAME;
     b.ref_frame[1] = REF_NONE;
     if (b.mi_size >= BLOCK_8X8) {
       int ctx = VP9_SIZE_GROUP[b.mi_size];
       const uint8_t *p = probs.y_mode[ctx];
-      int mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; });
+      int mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; }, pb.y_mode);
       counts.intra_mode[ctx][mode]++;
       b.y_mode = (uint8_t)mode;
       for (int i = 0; i < 4; i++) b.sub_modes[i] = (uint8_t)mode;
@@ -1009,7 +1044,7 @@ class TileDecoder {
       int mode = DC_PRED;
       for (int idy = 0; idy < 2; idy += n4h)
         for (int idx = 0; idx < 2; idx += n4w) {
-          mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; });
+          mode = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; }, pb.sub_modes[idy * 2 + idx]);
           counts.intra_mode[0][mode]++;
           for (int y2 = 0; y2 < n4h; y2++)
             for (int x2 = 0; x2 < n4w; x2++) b.sub_modes[(idy + y2) * 2 + idx + x2] = (uint8_t)mode;
@@ -1017,12 +1052,12 @@ class TileDecoder {
       b.y_mode = (uint8_t)mode;
     }
     const uint8_t *p = probs.uv_mode[b.y_mode];
-    int uv = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; });
+    int uv = d.tree(VP9_INTRA_MODE_TREE, [&](int n) { return p[n]; }, pb.uv_mode);
     counts.uv_mode[b.y_mode][uv]++;
     b.uv_mode = (uint8_t)uv;
   }
 
-  void inter_block_mode_info(BoolDecoder &d) {
+  void inter_block_mode_info(Coder &d) {
     read_ref_frames(d);
     for (int j = 0; j < 2; j++)
       if (b.ref_frame[j] > INTRA_FRAME) find_best_ref_mvs(j);
@@ -1047,7 +1082,7 @@ class TileDecoder {
       else
         ctx = 3;
       const uint8_t *p = probs.interp_filter[ctx];
-      int f = d.tree(VP9_INTERP_FILTER_TREE, [&](int n) { return p[n]; });
+      int f = d.tree(VP9_INTERP_FILTER_TREE, [&](int n) { return p[n]; }, pb.interp_filter);
       counts.interp_filter[ctx][f]++;
       b.interp_filter = (uint8_t)f;
     } else {
@@ -1081,15 +1116,41 @@ class TileDecoder {
     }
   }
 
-  int read_inter_mode(BoolDecoder &d) {
+  // The inter mode a writer codes: the candidate equal to the planned motion
+  // vector, else NEWMV.
+  int planned_inter_mode() {
+    int32_t t[2][2];
+    bool compound = b.ref_frame[1] > INTRA_FRAME;
+    for (int l = 0; l < 2; l++) planned_mv(l, t[l]);
+    auto eq = [&](int32_t m[2][2]) {
+      return m[0][0] == t[0][0] && m[0][1] == t[0][1] && (!compound || (m[1][0] == t[1][0] && m[1][1] == t[1][1]));
+    };
+    int32_t zero[2][2] = {{0, 0}, {0, 0}};
+    if (eq(b.nearest_mv)) return NEARESTMV - NEARESTMV;
+    if (eq(b.near_mv)) return NEARMV - NEARESTMV;
+    if (eq(zero)) return ZEROMV - NEARESTMV;
+    return NEWMV - NEARESTMV;
+  }
+  // The planned motion vector of list l, made codable: even components when
+  // high precision is unavailable (the reader then assumes the last bit).
+  void planned_mv(int l, int32_t *out) {
+    out[0] = pb.mv[l][0];
+    out[1] = pb.mv[l][1];
+    if (!(h.allow_high_precision_mv && use_mv_hp(b.best_mv[l])))
+      for (int k = 0; k < 2; k++) out[k] = (out[k] / 2) * 2;
+  }
+
+  int read_inter_mode(Coder &d) {
     int ctx = b.mode_context[b.ref_frame[0]];
     const uint8_t *p = probs.inter_mode[ctx];
-    int v = d.tree(VP9_INTER_MODE_TREE, [&](int n) { return p[n]; });
+    int want = 0;
+    if constexpr (W) want = planned_inter_mode();
+    int v = d.tree(VP9_INTER_MODE_TREE, [&](int n) { return p[n]; }, want);
     counts.inter_mode[ctx][v]++;
     return v;
   }
 
-  void read_ref_frames(BoolDecoder &d) {
+  void read_ref_frames(Coder &d) {
     if (seg.active(b.segment_id, SEG_LVL_REF_FRAME)) {
       int rf = seg.feature_data[b.segment_id][SEG_LVL_REF_FRAME];
       if (rf < 1 || rf > 3) bitstream("segment reference feature names no inter frame");
@@ -1100,7 +1161,7 @@ class TileDecoder {
     int comp_mode;
     if (h.reference_mode == REFERENCE_MODE_SELECT) {
       int ctx = comp_mode_ctx();
-      bool v = d.read(probs.comp_mode[ctx]);
+      bool v = d.bit(probs.comp_mode[ctx], pb.ref_frame[1] > INTRA_FRAME);
       counts.comp_mode[ctx][v]++;
       comp_mode = v ? COMPOUND_REFERENCE : SINGLE_REFERENCE;
     } else {
@@ -1109,17 +1170,17 @@ class TileDecoder {
     if (comp_mode == COMPOUND_REFERENCE) {
       int idx = h.ref_frame_sign_bias[h.comp_fixed_ref] ? 1 : 0;
       int ctx = comp_ref_ctx();
-      bool v = d.read(probs.comp_ref[ctx]);
+      bool v = d.bit(probs.comp_ref[ctx], pb.ref_frame[1 - idx] == h.comp_var_ref[1]);
       counts.comp_ref[ctx][v]++;
       b.ref_frame[idx] = (int8_t)h.comp_fixed_ref;
       b.ref_frame[1 - idx] = (int8_t)h.comp_var_ref[v ? 1 : 0];
     } else {
       int ctx = single_ref_p1_ctx();
-      bool p1 = d.read(probs.single_ref[ctx][0]);
+      bool p1 = d.bit(probs.single_ref[ctx][0], pb.ref_frame[0] != LAST_FRAME);
       counts.single_ref[ctx][0][p1]++;
       if (p1) {
         int ctx2 = single_ref_p2_ctx();
-        bool p2 = d.read(probs.single_ref[ctx2][1]);
+        bool p2 = d.bit(probs.single_ref[ctx2][1], pb.ref_frame[0] == ALTREF_FRAME);
         counts.single_ref[ctx2][1][p2]++;
         b.ref_frame[0] = p2 ? ALTREF_FRAME : GOLDEN_FRAME;
       } else {
@@ -1270,7 +1331,7 @@ class TileDecoder {
     return 2;
   }
 
-  void assign_mv(BoolDecoder &d, bool is_compound) {
+  void assign_mv(Coder &d, bool is_compound) {
     b.mv[1][0] = b.mv[1][1] = 0;
     for (int i = 0; i < (is_compound ? 2 : 1); i++) {
       switch (b.y_mode) {
@@ -1291,46 +1352,61 @@ class TileDecoder {
     }
   }
 
-  void read_mv(BoolDecoder &d, int r) {
+  void read_mv(Coder &d, int r) {
     const int32_t *best = b.best_mv[r];
     bool use_hp = h.allow_high_precision_mv && use_mv_hp(best);
-    int32_t diff[2] = {0, 0};
-    int joint = d.tree(VP9_MV_JOINT_TREE, [&](int n) { return probs.mv_joint[n]; });
+    int32_t diff[2] = {0, 0}, want[2] = {0, 0};
+    if constexpr (W) {
+      int32_t t[2];
+      planned_mv(r, t);
+      want[0] = t[0] - best[0];
+      want[1] = t[1] - best[1];
+    }
+    int joint = d.tree(VP9_MV_JOINT_TREE, [&](int n) { return probs.mv_joint[n]; },
+                       (want[0] != 0 ? 2 : 0) | (want[1] != 0 ? 1 : 0));
     counts.mv_joint[joint]++;
-    if (joint == MV_JOINT_HZVNZ || joint == MV_JOINT_HNZVNZ) diff[0] = read_mv_component(d, 0, use_hp);
-    if (joint == MV_JOINT_HNZVZ || joint == MV_JOINT_HNZVNZ) diff[1] = read_mv_component(d, 1, use_hp);
+    if (joint == MV_JOINT_HZVNZ || joint == MV_JOINT_HNZVNZ) diff[0] = read_mv_component(d, 0, use_hp, want[0]);
+    if (joint == MV_JOINT_HNZVZ || joint == MV_JOINT_HNZVNZ) diff[1] = read_mv_component(d, 1, use_hp, want[1]);
     b.mv[r][0] = best[0] + diff[0];
     b.mv[r][1] = best[1] + diff[1];
   }
 
-  int32_t read_mv_component(BoolDecoder &d, int comp, bool use_hp) {
-    bool sign = d.read(probs.mv_sign[comp]);
+  // want: the component a writer codes (nonzero; even when !use_hp)
+  int32_t read_mv_component(Coder &d, int comp, bool use_hp, int32_t want) {
+    int w_cls = 0;
+    uint32_t w_off = 0;
+    if constexpr (W) {
+      w_off = (uint32_t)std::abs(want) - 1;
+      while (w_cls < 10 && w_off >= (16u << w_cls)) w_cls++;
+      if (w_cls) w_off -= 8u << w_cls;
+    }
+    bool sign = d.bit(probs.mv_sign[comp], want < 0);
     counts.mv_sign[comp][sign]++;
     const uint8_t *pc = probs.mv_class[comp];
-    int cls = d.tree(VP9_MV_CLASS_TREE, [&](int n) { return pc[n]; });
+    int cls = d.tree(VP9_MV_CLASS_TREE, [&](int n) { return pc[n]; }, w_cls);
     counts.mv_class[comp][cls]++;
     int32_t mag;
     if (cls == 0) {
-      int c0 = d.read(probs.mv_class0_bit[comp]);
+      int c0 = d.bit(probs.mv_class0_bit[comp], (w_off >> 3) & 1);
       counts.mv_class0_bit[comp][c0]++;
       const uint8_t *pf = probs.mv_class0_fr[comp][c0];
-      int fr = d.tree(VP9_MV_FR_TREE, [&](int n) { return pf[n]; });
+      int fr = d.tree(VP9_MV_FR_TREE, [&](int n) { return pf[n]; }, (w_off >> 1) & 3);
       counts.mv_class0_fr[comp][c0][fr]++;
-      int hp = use_hp ? (int)d.read(probs.mv_class0_hp[comp]) : 1;
+      int hp = use_hp ? (int)d.bit(probs.mv_class0_hp[comp], w_off & 1) : 1;
       counts.mv_class0_hp[comp][hp]++;
       mag = ((c0 << 3) | (fr << 1) | hp) + 1;
     } else {
       int32_t dd = 0;
       for (int i = 0; i < cls; i++) {
-        int bit = d.read(probs.mv_bits[comp][i]);
+        int bit = d.bit(probs.mv_bits[comp][i], (w_off >> (3 + i)) & 1);
         counts.mv_bits[comp][i][bit]++;
         dd |= bit << i;
       }
       int32_t base = 2 << (cls + 2);
       const uint8_t *pf = probs.mv_fr[comp];
-      int fr = d.tree(VP9_MV_FR_TREE, [&](int n) { return pf[n]; });
+      int fr = d.tree(VP9_MV_FR_TREE, [&](int n) { return pf[n]; }, (w_off >> 1) & 3);
       counts.mv_fr[comp][fr]++;
-      int hp = use_hp ? (int)d.read(probs.mv_hp[comp]) : 1;
+      int hp = use_hp ? (int)d.bit(probs.mv_hp[comp], w_off & 1) : 1;
       counts.mv_hp[comp][hp]++;
       mag = base + ((dd << 3) | (fr << 1) | hp) + 1;
     }
@@ -1558,7 +1634,7 @@ class TileDecoder {
     }
   }
 
-  void residual(BoolDecoder &d, uint32_t bi) {
+  void residual(Coder &d, uint32_t bi) {
     int bsize = b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size;
     for (int plane = 0; plane < 3; plane++) {
       int tx_sz = plane > 0 ? uv_tx_size() : b.tx_size;
@@ -1619,7 +1695,7 @@ class TileDecoder {
     }
   }
 
-  uint32_t tokens(BoolDecoder &d, int plane, uint32_t start_x, uint32_t start_y, int tx_sz,
+  uint32_t tokens(Coder &d, int plane, uint32_t start_x, uint32_t start_y, int tx_sz,
                   int tt) {
     uint32_t seg_eob = 16u << (tx_sz << 1);
     const uint16_t *scan = get_scan(tx_sz, tt);
@@ -1640,24 +1716,32 @@ class TileDecoder {
     bool check_eob = true;
     uint32_t c = 0;
     int bd = (int)h.bit_depth;
+    uint32_t w_eob = 0;
+    if constexpr (W) {
+      plan->coefs(plane, start_x, start_y, tx_sz, tt, coefs);
+      for (uint32_t k = 0; k < seg_eob; k++)
+        if (coefs[scan[k]]) w_eob = k + 1;
+    }
     while (c < seg_eob) {
       int pos = scan[c];
       int band = bands[c];
       if (c > 0) ctx = (1 + token_cache[nb[2 * c]] + token_cache[nb[2 * c + 1]]) >> 1;
       const uint8_t *p = cp[band][ctx];
       if (check_eob) {
-        bool more = d.read(p[0]);
+        bool more = d.bit(p[0], c < w_eob);
         if (count_tokens) counts.more_coefs[tx_sz][ptype][ref_type][band][ctx][more]++;
         if (!more) break;
       }
-      int token = d.tree(VP9_TOKEN_TREE, [&](int node) { return pareto(node, p[std::min(1 + node, 2)]); });
+      int w_tok = 0;
+      if constexpr (W) w_tok = token_of(coefs[pos]);
+      int token = d.tree(VP9_TOKEN_TREE, [&](int node) { return pareto(node, p[std::min(1 + node, 2)]); }, w_tok);
       if (count_tokens) counts.token[tx_sz][ptype][ref_type][band][ctx][std::min(token, 2)]++;
       token_cache[pos] = VP9_ENERGY_CLASS[token];
       if (token == ZERO_TOKEN) {
         check_eob = false;
       } else {
-        int32_t coef = read_coef(d, token, bd);
-        bool sign = d.read(128);
+        int32_t coef = read_coef(d, token, bd, std::abs(coefs[pos]));
+        bool sign = d.bit(128, coefs[pos] < 0);
         coefs[pos] = sign ? -coef : coef;
         check_eob = true;
       }
@@ -1667,20 +1751,34 @@ class TileDecoder {
     return c;
   }
 
-  static int32_t read_coef(BoolDecoder &d, int token, int bd) {
+  // want: the magnitude a writer codes
+  static int32_t read_coef(Coder &d, int token, int bd, int32_t want) {
     int cat = VP9_EXTRA_BITS[token][0], num_extra = VP9_EXTRA_BITS[token][1];
     int32_t coef = VP9_EXTRA_BITS[token][2];
+    const int32_t extra = want - coef;
     if (token == DCT_VAL_CAT6)
       for (int e = 0; e < bd - 8; e++) {
-        int32_t high = d.read(255);
+        int32_t high = d.bit(255, (extra >> (5 + bd - e)) & 1);
         coef += high << (5 + bd - e);
       }
     const uint8_t *p = VP9_CAT_PROBS[cat];
     for (int e = 0; e < num_extra; e++) {
-      int32_t bit = d.read(p[e]);
+      int32_t bit = d.bit(p[e], (extra >> (num_extra - 1 - e)) & 1);
       coef += bit << (num_extra - 1 - e);
     }
     return coef;
+  }
+
+  // the token of a coefficient value (6.4.24's token tree leaves)
+  static int token_of(int32_t v) {
+    int32_t a = std::abs(v);
+    if (a <= 4) return a;
+    if (a <= 6) return 5;
+    if (a <= 10) return 6;
+    if (a <= 18) return 7;
+    if (a <= 34) return 8;
+    if (a <= 66) return 9;
+    return DCT_VAL_CAT6;
   }
 };
 
@@ -1698,7 +1796,7 @@ bool Parser::decode_tiles(const uint8_t *data, size_t len) {
   coef_val_.clear();
   memset(&counts_, 0, sizeof counts_);
   mi_.assign((size_t)h_.mi_rows * h_.mi_cols, MiInfo());
-  TileDecoder td(*this);
+  TileWalker<BoolDecoder> td(*this);
   uint32_t tile_cols = 1u << h_.tile_cols_log2, tile_rows = 1u << h_.tile_rows_log2;
   size_t pos = 0;
   for (uint32_t row = 0; row < tile_rows; row++)
@@ -1743,6 +1841,22 @@ void Parser::parse_frame(const uint8_t *data, size_t len, vp9r_record *out) {
     o.ss_y = (uint8_t)s.ss_y;
     return;
   }
+  setup_frame(reset_kind);
+  size_t mi_len = (size_t)h.mi_rows * h.mi_cols;
+  size_t comp_start = h.uncompressed_size;
+  size_t comp_end = comp_start + h.header_size_in_bytes;
+  if (comp_end > len) bitstream("compressed header runs past the end of the frame");
+  parse_compressed(data + comp_start, h.header_size_in_bytes);
+  const uint8_t *tiles = data + comp_end;
+  size_t tiles_len = len - comp_end;
+  parse_tiles_and_finish(tiles, tiles_len, mi_len, out);
+}
+
+// The frame process between the uncompressed header and the compressed one:
+// format and size checks, the active references, compute_image_size()
+// semantics (7.2.6), context resets and load_probs (decoder/mod.rs:232-291).
+void Parser::setup_frame(int reset_kind) {
+  FrameHeader &h = h_;
   if ((uint64_t)h.width * h.height > max_pixels)
     fail(VP9R_ERR_UNSUPPORTED, "frame size is above the decoder's limit");
   if (h.header_size_in_bytes == 0) bitstream("compressed header of size 0");
@@ -1774,12 +1888,10 @@ void Parser::parse_frame(const uint8_t *data, size_t len, vp9r_record *out) {
   else if (reset_kind == RESET_ONE)
     contexts_[reset_idx_].set_defaults();
   probs_ = contexts_[h.frame_context_idx];
-  size_t comp_start = h.uncompressed_size;
-  size_t comp_end = comp_start + h.header_size_in_bytes;
-  if (comp_end > len) bitstream("compressed header runs past the end of the frame");
-  parse_compressed(data + comp_start, h.header_size_in_bytes);
-  const uint8_t *tiles = data + comp_end;
-  size_t tiles_len = len - comp_end;
+}
+
+void Parser::parse_tiles_and_finish(const uint8_t *tiles, size_t tiles_len, size_t mi_len, vp9r_record *out) {
+  FrameHeader &h = h_;
   // A non-4:2:0 stream may be one of the pre-final profile 1 streams that
   // size chroma transforms as 4:2:0 (decided at intra frames: the legacy
   // rules are taken if the specification's leave a tile unparsable or with
@@ -1935,6 +2047,140 @@ void Parser::parse_frame(const uint8_t *data, size_t len, vp9r_record *out) {
       ref_sizes_[i][1] = h.height;
     }
   frames++;
+}
+
+// --------------------------------------------------------------------------
+// Stream generation: the boolean encoder (the inverse of 9.2) and the frame
+// writer over the same syntax walk.
+
+class BoolWriter {
+ public:
+  static constexpr bool kWrite = true;
+  std::vector<uint8_t> out;
+  BoolWriter() { put(false, 128); }  // the marker bit
+  inline bool bit(uint32_t prob, bool v) {
+    put(v, prob);
+    return v;
+  }
+  uint32_t literal(int n, uint32_t v = 0) {
+    for (int i = n - 1; i >= 0; i--) put((v >> i) & 1, 128);
+    return v;
+  }
+  // codes leaf `value` of tree t
+  template <typename F>
+  int tree(const int8_t *t, F prob, int value) {
+    int path[32], depth = 0;
+    if (!find(t, 0, value, path, &depth)) bitstream("stream generator: value outside its tree");
+    int n = 0;
+    for (int k = 0; k < depth; k++) {
+      put(path[k], prob(n >> 1));
+      n = t[n + path[k]];
+    }
+    return value;
+  }
+  // exit: 32 zero bits flush the value; never end on a byte that looks like
+  // a superframe marker
+  void finish() {
+    for (int i = 0; i < 32; i++) put(false, 128);
+    if (!out.empty() && (out.back() & 0xe0) == 0xc0) out.push_back(0);
+  }
+
+ private:
+  static bool find(const int8_t *t, int n, int value, int *path, int *depth) {
+    for (int b = 0; b < 2; b++) {
+      int next = t[n + b];
+      path[*depth] = b;
+      (*depth)++;
+      if (next <= 0) {
+        if (-next == value) return true;
+      } else if (find(t, next, value, path, depth)) {
+        return true;
+      }
+      (*depth)--;
+    }
+    return false;
+  }
+  // low: the coded value's pending low bits (24 + carry), `bits`: how many
+  // more normalisation shifts before the next byte is complete
+  void put(bool v, uint32_t prob) {
+    uint32_t split = 1 + (((range_ - 1) * prob) >> 8);
+    if (v) {
+      low_ += split;
+      range_ -= split;
+    } else {
+      range_ = split;
+    }
+    int shift = __builtin_clz(range_) - 24;
+    range_ <<= shift;
+    bits_ += shift;
+    if (bits_ >= 0) {
+      int offset = shift - bits_;
+      if ((low_ << (offset - 1)) & 0x80000000u) {
+        // carry into the bytes already written
+        size_t k = out.size();
+        while (k > 0 && out[k - 1] == 0xff) out[--k] = 0;
+        if (k > 0) out[k - 1]++;
+      }
+      out.push_back((uint8_t)(low_ >> (24 - offset)));
+      low_ <<= offset;
+      shift = bits_;
+      low_ &= 0xffffff;
+      bits_ -= 8;
+    }
+    low_ <<= shift;
+  }
+  uint32_t low_ = 0, range_ = 255;
+  int bits_ = -24;
+};
+
+std::vector<uint8_t> Parser::write_frame(const uint8_t *uncompressed, size_t len, uint32_t tx_mode,
+                                         uint32_t reference_mode, SyntaxPlan *plan) {
+  int reset_kind;
+  parse_uncompressed(uncompressed, len, &reset_kind);
+  if (h_.show_existing_frame) bitstream("stream generator: nothing to write for show_existing_frame");
+  if (h_.uncompressed_size != len) bitstream("stream generator: uncompressed header length mismatch");
+  setup_frame(reset_kind);
+  h_.tx_mode = tx_mode;
+  h_.reference_mode = reference_mode;
+  BoolWriter cw;
+  compressed_syntax(cw);
+  cw.finish();
+  h_.legacy_uv = false;
+  blocks_.clear();
+  txbs_.clear();
+  coef_pos_.clear();
+  coef_val_.clear();
+  memset(&counts_, 0, sizeof counts_);
+  mi_.assign((size_t)h_.mi_rows * h_.mi_cols, MiInfo());
+  TileWalker<BoolWriter> tw(*this);
+  tw.plan = plan;
+  std::vector<uint8_t> frame(uncompressed, uncompressed + len);
+  // header_size_in_bytes (16 bits at header_size_pos)
+  if (cw.out.size() > 0xffff) bitstream("stream generator: compressed header too large");
+  for (int i = 0; i < 16; i++) {
+    size_t bitpos = h_.header_size_pos + i;
+    uint8_t mask = (uint8_t)(0x80 >> (bitpos & 7));
+    if ((cw.out.size() >> (15 - i)) & 1)
+      frame[bitpos >> 3] |= mask;
+    else
+      frame[bitpos >> 3] &= (uint8_t)~mask;
+  }
+  frame.insert(frame.end(), cw.out.begin(), cw.out.end());
+  uint32_t tile_cols = 1u << h_.tile_cols_log2, tile_rows = 1u << h_.tile_rows_log2;
+  for (uint32_t row = 0; row < tile_rows; row++)
+    for (uint32_t col = 0; col < tile_cols; col++) {
+      BoolWriter tw_d;
+      tw.walk_tile(tw_d, row, col);
+      tw_d.finish();
+      bool last = row == tile_rows - 1 && col == tile_cols - 1;
+      if (!last) {
+        uint32_t n = (uint32_t)tw_d.out.size();
+        uint8_t be[4] = {(uint8_t)(n >> 24), (uint8_t)(n >> 16), (uint8_t)(n >> 8), (uint8_t)n};
+        frame.insert(frame.end(), be, be + 4);
+      }
+      frame.insert(frame.end(), tw_d.out.begin(), tw_d.out.end());
+    }
+  return frame;
 }
 
 }  // namespace vp9r

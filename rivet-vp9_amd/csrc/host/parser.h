@@ -101,7 +101,7 @@ struct FrameHeader {
   uint32_t frame_type = 0, last_frame_type = 0;
   bool show_frame = false, error_resilient_mode = false, intra_only = false;
   uint32_t reset_frame_context = 0;
-  uint32_t bit_depth = 8, color_space = 2;
+  uint32_t bit_depth = 8, color_space = 1;
   bool color_range = false;
   uint32_t ss_x = 1, ss_y = 1;
   uint32_t refresh_frame_flags = 0;
@@ -116,6 +116,7 @@ struct FrameHeader {
   bool lossless = false;
   uint32_t tile_cols_log2 = 0, tile_rows_log2 = 0;
   uint32_t header_size_in_bytes = 0;
+  size_t header_size_pos = 0;  // bit position of header_size_in_bytes
   size_t uncompressed_size = 0;
   bool past_independence = false;
   bool frame_is_intra = false;
@@ -152,6 +153,26 @@ struct SlotInfo {
   uint32_t width = 0, height = 0, ss_x = 0, ss_y = 0, bit_depth = 0;
 };
 
+// The decisions a stream generator supplies to the syntax walk when it
+// writes instead of reads (see TileWalker in parser.cpp): partitions, each
+// block's modes and motion vectors, each transform block's quantised
+// coefficients.  Not used when decoding.
+struct PlanBlock {
+  uint8_t segment_id, skip, tx_size, is_inter;
+  uint8_t y_mode, sub_modes[4], uv_mode, interp_filter;
+  int8_t ref_frame[2];
+  int32_t mv[2][2];  // the motion vector of each list (1/8 sample)
+};
+
+class SyntaxPlan {
+ public:
+  virtual ~SyntaxPlan() {}
+  virtual int partition(uint32_t mi_row, uint32_t mi_col, int bsize) = 0;
+  virtual void block(uint32_t mi_row, uint32_t mi_col, int bsize, bool frame_is_intra, PlanBlock *out) = 0;
+  // fills the raster coefficients (zeroed on entry) of a transform block
+  virtual void coefs(int plane, uint32_t x, uint32_t y, int tx_size, int tx_type, int32_t *raster) = 0;
+};
+
 class Parser {
  public:
   Parser();
@@ -159,6 +180,14 @@ class Parser {
   void parse_frame(const uint8_t *data, size_t len, vp9r_record *out);
   uint64_t max_pixels = 8192ull * 8192ull;
   uint64_t frames = 0;
+  // Stream generation: given a frame's uncompressed header (its
+  // header_size_in_bytes any nonzero placeholder), writes the compressed
+  // header (tx_mode and reference_mode as given, no probability updates) and
+  // the tile data with the plan's decisions, and returns the whole frame.
+  // Changes this parser's state as parsing the header would: call it on a
+  // copy of the decoder that will read the frame.
+  std::vector<uint8_t> write_frame(const uint8_t *uncompressed, size_t len, uint32_t tx_mode,
+                                   uint32_t reference_mode, SyntaxPlan *plan);
 
  private:
   // persistent state (header.rs:58-84, decoder/mod.rs:122-139)
@@ -166,7 +195,7 @@ class Parser {
   SegParams seg_;
   uint32_t ref_sizes_[8][2] = {};
   uint32_t st_last_frame_type_ = KEY_FRAME_;
-  uint32_t st_bit_depth_ = 8, st_ss_x_ = 1, st_ss_y_ = 1, st_color_space_ = 2;
+  uint32_t st_bit_depth_ = 8, st_ss_x_ = 1, st_ss_y_ = 1, st_color_space_ = 1;
   bool st_color_range_ = false;
   Probs contexts_[4];
   SlotInfo slots_[8];
@@ -195,11 +224,16 @@ class Parser {
   std::vector<vp9r_mi> lf_mi_;
 
   void parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind);
+  void setup_frame(int reset_kind);
+  void parse_tiles_and_finish(const uint8_t *tiles, size_t tiles_len, size_t mi_len, vp9r_record *out);
   void parse_compressed(const uint8_t *data, size_t len);
   // returns true if every tile's padding was zero
   bool decode_tiles(const uint8_t *data, size_t len);
   void fill_record(vp9r_record *out);
-  friend class TileDecoder;
+  template <class Coder>
+  friend class TileWalker;
+  template <class Coder>
+  void compressed_syntax(Coder &d);
 };
 
 }  // namespace vp9r

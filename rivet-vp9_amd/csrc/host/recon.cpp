@@ -14,6 +14,8 @@
 // reconstructed neighbours only.
 #include "recon.h"
 
+#include "../gpu/mc_pos.h"
+
 #include <algorithm>
 #include <cstring>
 
@@ -264,6 +266,109 @@ static void build_lf_levels(vp9r_gpu *g, uint32_t rows, uint32_t cols) {
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// The algorithmic HBM bytes of each stage of a frame (what the roofline
+// divides by time), with b = bytes per sample:
+//   inter prediction, per prediction unit (block plane, or 4x4 sub-block
+//     below 8x8) and reference: the filter footprint read,
+//     (w + 7[x fractional]) x (h + 7[y fractional]) x b unscaled
+//     (src/dsp/inter.rs:248-281) or the scaled footprint (inter.rs:354-359),
+//     plus w x h x b written once;
+//   inter residual, per transform block with eob > 0: n^2 x 4 coefficient
+//     bytes plus the prediction read and written, 2 x n^2 x b;
+//   intra, per transform block: the edges read, (3n + 1) x b, the block
+//     written, n^2 x b, and n^2 x 4 coefficient bytes when eob > 0;
+//   loop filter: every visible sample read and written once, 2 x frame bytes.
+static void work_bytes(const vp9r_record *rec, Prepared &p) {
+  const vp9r_frame_hdr &h = rec->hdr;
+  const uint64_t b = h.bit_depth > 8 ? 2 : 1;
+  for (auto &v : p.work_bytes) v = 0;
+  for (uint32_t bi = 0; bi < rec->n_blocks; bi++) {
+    const vp9r_block &blk = rec->blocks[bi];
+    if (!blk.is_inter) continue;
+    const int bsize = blk.mi_size < BLOCK_8X8 ? BLOCK_8X8 : blk.mi_size;
+    const int nref = 1 + (blk.ref_frame[1] > INTRA_FRAME);
+    for (int plane = 0; plane < 3; plane++) {
+      int sx = plane ? h.ss_x : 0, sy = plane ? h.ss_y : 0;
+      int psz = VP9_SS_SIZE_LOOKUP[bsize][sx][sy];
+      int n4w = VP9_NUM_4X4_WIDE[psz], n4h = VP9_NUM_4X4_HIGH[psz];
+      int base_x = (blk.mi_col * 8) >> sx, base_y = (blk.mi_row * 8) >> sy;
+      auto unit = [&](int x, int y, int w, int hh, int sub) {
+        for (int rl = 0; rl < nref; rl++) {
+          vp9g::RefPos r = vp9g::mc_position(p.geo, h.ref_width, h.ref_height, blk, plane, x, y, sub, rl);
+          uint64_t rw = (r.xs == 16 && (r.x & 15) == 0) ? w : ((((w - 1) * r.xs + (r.x & 15)) >> 4) + 8);
+          uint64_t rh = (r.ys == 16 && (r.y & 15) == 0) ? hh : ((((hh - 1) * r.ys + (r.y & 15)) >> 4) + 8);
+          p.work_bytes[0] += rw * rh * b;
+        }
+        p.work_bytes[0] += (uint64_t)w * hh * b;
+      };
+      if (blk.mi_size < BLOCK_8X8) {
+        for (int y = 0; y < n4h; y++)
+          for (int x = 0; x < n4w; x++) unit(base_x + 4 * x, base_y + 4 * y, 4, 4, y * n4w + x);
+      } else {
+        unit(base_x, base_y, n4w * 4, n4h * 4, 0);
+      }
+    }
+  }
+  for (uint32_t i = 0; i < rec->n_txbs; i++) {
+    const vp9r_txb &t = rec->txbs[i];
+    const uint64_t n = 4u << t.tx_size;
+    if (rec->blocks[t.block].is_inter) {
+      if (t.eob) p.work_bytes[1] += n * n * (4 + 2 * b);
+    } else {
+      p.work_bytes[2] += (3 * n + 1) * b + n * n * b + (t.eob ? n * n * 4 : 0);
+    }
+  }
+  if (h.lf_level) {
+    uint64_t cw = (h.width + h.ss_x) >> h.ss_x, ch = (h.height + h.ss_y) >> h.ss_y;
+    p.work_bytes[3] = 2 * b * ((uint64_t)h.width * h.height + 2 * cw * ch);
+  }
+}
+
+// Adds the stage times of every frame timed since the last fold to the
+// running sums (waits for the last of them).
+hipError_t fold_stage_times(vp9r_gpu *g) {
+  if (g->ev_used == 0) return hipSuccess;
+  hipError_t e = hipEventSynchronize(g->ev_sets[g->ev_used - 1][4]);
+  for (size_t k = 0; k < g->ev_used && e == hipSuccess; k++) {
+    const auto &ev = g->ev_sets[k];
+    float ms = 0;
+    for (int i = 0; i < 4 && e == hipSuccess; i++) {
+      e = hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      g->stage_acc[i] += ms;
+    }
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[4]);
+    g->stage_acc[4] += ms;
+    g->frames_acc++;
+  }
+  g->ev_used = 0;
+  g->ev = nullptr;
+  return e;
+}
+
+// Takes the event set of the next timed frame.
+hipError_t begin_timed_frame(vp9r_gpu *g) {
+  if (g->ev_used == 1024) {
+    hipError_t e = fold_stage_times(g);
+    if (e != hipSuccess) return e;
+  }
+  if (g->ev_used == g->ev_sets.size()) {
+    std::array<hipEvent_t, 5> set{};
+    for (auto &ev : set) {
+      hipError_t e = hipEventCreate(&ev);
+      if (e != hipSuccess) return e;
+    }
+    g->ev_sets.push_back(set);
+  }
+  g->ev = g->ev_sets[g->ev_used++].data();
+  return hipSuccess;
+}
+
+// Orders work on `s` that writes frame f after f's download in flight, if any.
+hipError_t order_after_copy(hipStream_t s, vp9r_gpu_frame *f) {
+  if (!f->copy_pending) return hipSuccess;
+  return hipStreamWaitEvent(s, f->copy_done, 0);
+}
+
 }  // namespace vp9r
 
 using namespace vp9r;
@@ -282,13 +387,13 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   vp9r_gpu *g = new vp9r_gpu();
   g->device = device;
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
+    if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return hip_fail(e, "hipStreamCreate");
   }
   HIP_TRY(hipEventCreateWithFlags(&g->ev_upload, hipEventDisableTiming), "hipEventCreate");
-  HIP_TRY(hipEventCreateWithFlags(&g->ev_lf_upload, hipEventDisableTiming), "hipEventCreate");
-  for (auto &ev : g->ev) HIP_TRY(hipEventCreate(&ev), "hipEventCreate");
   *out = g;
   return VP9R_OK;
 }
@@ -297,10 +402,12 @@ void vp9r_gpu_free(vp9r_gpu *g) {
   if (!g) return;
   (void)hipSetDevice(g->device);
   (void)hipStreamSynchronize(g->stream);
+  (void)hipStreamSynchronize(g->copy_stream);
   (void)hipEventDestroy(g->ev_upload);
-  (void)hipEventDestroy(g->ev_lf_upload);
-  for (auto &ev : g->ev) (void)hipEventDestroy(ev);
+  for (auto &set : g->ev_sets)
+    for (auto ev : set) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(g->stream);
+  (void)hipStreamDestroy(g->copy_stream);
   delete g;
 }
 
@@ -336,10 +443,14 @@ int vp9r_gpu_frame_new(vp9r_gpu *g, uint32_t width, uint32_t height, uint32_t bd
     return hip_fail(e, "hipMalloc(frame)");
   }
   e = hipMemsetAsync(f->base, 0, total, g->stream);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&f->ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&f->copy_done, hipEventDisableTiming);
   if (e != hipSuccess) {
+    (void)hipStreamSynchronize(g->stream);
+    if (f->ready) (void)hipEventDestroy(f->ready);
     (void)hipFree(f->base);
     delete f;
-    return hip_fail(e, "hipMemset(frame)");
+    return hip_fail(e, "frame set-up");
   }
   f->bytes = total;
   for (int p = 0; p < 3; p++) f->planes.p[p] = (uint8_t *)f->base + off[p];
@@ -349,7 +460,12 @@ int vp9r_gpu_frame_new(vp9r_gpu *g, uint32_t width, uint32_t height, uint32_t bd
 
 void vp9r_gpu_frame_free(vp9r_gpu_frame *f) {
   if (!f) return;
-  if (f->g) (void)hipStreamSynchronize(f->g->stream);
+  if (f->g) {
+    (void)hipStreamSynchronize(f->g->stream);
+    (void)hipStreamSynchronize(f->g->copy_stream);
+  }
+  (void)hipEventDestroy(f->ready);
+  (void)hipEventDestroy(f->copy_done);
   (void)hipFree(f->base);
   delete f;
 }
@@ -362,6 +478,7 @@ void vp9r_gpu_frame_geometry(const vp9r_gpu_frame *f, int plane, uint32_t *strid
 int vp9r_gpu_frame_upload_u16(vp9r_gpu_frame *f, int plane, const uint16_t *src) {
   size_t n = (size_t)f->planes.stride[plane] * f->planes.rows[plane];
   hipStream_t s = f->g->stream;
+  HIP_TRY(vp9r::order_after_copy(s, f), "hipStreamWaitEvent");
   if (f->bit_depth > 8) {
     HIP_TRY(hipMemcpyAsync(f->planes.p[plane], src, n * 2, hipMemcpyHostToDevice, s), "upload");
   } else {
@@ -394,62 +511,65 @@ int vp9r_gpu_synchronize(vp9r_gpu *g) {
   return VP9R_OK;
 }
 
-void vp9r_gpu_set_timing(vp9r_gpu *g, int enable) { g->timing = enable != 0; }
+void vp9r_gpu_set_timing(vp9r_gpu *g, int enable) {
+  g->timing = enable != 0;
+  g->ev_used = 0;
+  g->ev = nullptr;
+  g->frames_acc = 0;
+  for (double &v : g->stage_acc) v = 0;
+}
 
-int vp9r_gpu_stage_times(vp9r_gpu *g, float out[5]) {
+int vp9r_gpu_stage_times(vp9r_gpu *g, float out[5], uint32_t *frames) {
   if (!g->timing) {
     set_error("timing is not enabled");
     return VP9R_ERR_INVALID_INPUT;
   }
-  HIP_TRY(hipEventSynchronize(g->ev[4]), "hipEventSynchronize");
-  for (int i = 0; i < 4; i++) HIP_TRY(hipEventElapsedTime(&out[i], g->ev[i], g->ev[i + 1]), "elapsed");
-  HIP_TRY(hipEventElapsedTime(&out[4], g->ev[0], g->ev[4]), "elapsed");
+  HIP_TRY(vp9r::fold_stage_times(g), "stage timing");
+  for (int i = 0; i < 5; i++) out[i] = (float)g->stage_acc[i];
+  if (frames) *frames = g->frames_acc;
+  g->frames_acc = 0;
+  for (double &v : g->stage_acc) v = 0;
   return VP9R_OK;
 }
 
-int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *const refs[3],
-                         vp9r_gpu_frame *cur) {
+}  // extern "C"
+
+namespace vp9r {
+
+// Checks the record against the frame geometry and the reference sizes it
+// names, builds the work lists, and writes everything to `stage` (host) at
+// the offsets it records in p.  Every index the kernels follow is checked
+// here, so that no record can send a kernel outside its frame stores.
+static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stage) {
   const vp9r_frame_hdr &h = rec->hdr;
   if (h.show_existing_frame) {
     set_error("a show_existing_frame record has nothing to reconstruct");
     return VP9R_ERR_INVALID_INPUT;
   }
-  if (!cur || cur->width != h.width || cur->height != h.height || cur->bit_depth != h.bit_depth ||
-      cur->ss_x != h.ss_x || cur->ss_y != h.ss_y) {
-    set_error("the frame store does not match the record's geometry");
-    return VP9R_ERR_INVALID_INPUT;
-  }
-  if (!h.frame_is_intra) {
-    for (int i = 0; i < 3; i++) {
-      const vp9r_gpu_frame *r = refs ? refs[i] : nullptr;
-      if (!r || r->bit_depth != h.bit_depth || r->ss_x != h.ss_x || r->ss_y != h.ss_y) {
-        set_error("missing reference, or a reference of another format");
-        return VP9R_ERR_BITSTREAM;
-      }
-      if (r == cur) {
-        set_error("the frame being reconstructed is one of its references");
-        return VP9R_ERR_INVALID_INPUT;
-      }
-    }
-  }
-  if (h.mi_cols != cur->mi_cols || h.mi_rows != cur->mi_rows || h.sb64_cols != cur->sb64_cols ||
-      h.sb64_rows != cur->sb64_rows) {
+  uint32_t mi_cols = (h.width + 7) >> 3, mi_rows = (h.height + 7) >> 3;
+  if (h.width == 0 || h.height == 0 || h.mi_cols != mi_cols || h.mi_rows != mi_rows ||
+      h.sb64_cols != ((mi_cols + 7) >> 3) || h.sb64_rows != ((mi_rows + 7) >> 3) ||
+      (h.bit_depth != 8 && h.bit_depth != 10 && h.bit_depth != 12) || h.ss_x > 1 || h.ss_y > 1) {
     set_error("malformed record header");
     return VP9R_ERR_INVALID_INPUT;
   }
-  // Every index the kernels follow is checked here, so that no record can
-  // send a kernel outside its frame stores.
+  p.hdr = h;
+  p.geo = geom_of(h);
+  q_table(h, p.q);
+  lf_params(h, p.lfp);
+  const uint32_t plane_w[3] = {h.sb64_cols * 64, (h.sb64_cols * 64) >> h.ss_x, (h.sb64_cols * 64) >> h.ss_x};
+  const uint32_t plane_h[3] = {h.sb64_rows * 64, (h.sb64_rows * 64) >> h.ss_y, (h.sb64_rows * 64) >> h.ss_y};
   bool ref_ok[3] = {false, false, false};
   if (!h.frame_is_intra)
     for (int i = 0; i < 3; i++) {
-      uint32_t rw = refs[i]->width, rh = refs[i]->height;
-      ref_ok[i] = !(2 * h.width < rw || 2 * h.height < rh || h.width > 16 * rw || h.height > 16 * rh);
+      uint32_t rw = h.ref_width[i], rh = h.ref_height[i];
+      ref_ok[i] = rw && rh && !(2 * h.width < rw || 2 * h.height < rh || h.width > 16 * rw || h.height > 16 * rh);
     }
   for (uint32_t bi = 0; bi < rec->n_blocks; bi++) {
     const vp9r_block &b = rec->blocks[bi];
     if (b.mi_size > BLOCK_64X64 || b.mi_row >= h.mi_rows || b.mi_col >= h.mi_cols ||
         (b.mi_row & (VP9_NUM_8X8_HIGH[b.mi_size] - 1)) || (b.mi_col & (VP9_NUM_8X8_WIDE[b.mi_size] - 1)) ||
-        b.interp_filter > 3 || b.uv_mode > TM_PRED || b.y_mode > NEWMV) {
+        b.interp_filter > 3 || b.uv_mode > TM_PRED || b.y_mode > NEWMV || b.segment_id > 7) {
       set_error("malformed record block");
       return VP9R_ERR_INVALID_INPUT;
     }
@@ -473,8 +593,6 @@ int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *co
         }
     }
   }
-  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
-  // ---- host work lists
   build_mc_tiles(g, rec);
   for (auto &j : g->jobs) j.clear();
   for (uint32_t i = 0; i < rec->n_txbs; i++) {
@@ -482,13 +600,12 @@ int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *co
     const uint32_t n0 = 4u << t.tx_size;
     bool ok = t.tx_size <= 3 && t.plane < 3 && t.tx_type <= 3 && t.block < rec->n_blocks &&
               (uint64_t)t.coef_start + t.ncoef <= rec->n_coefs && t.eob <= n0 * n0 && (t.x & (n0 - 1)) == 0 &&
-              (t.y & (n0 - 1)) == 0 && t.x + n0 <= cur->planes.stride[t.plane] &&
-              t.y + n0 <= cur->planes.rows[t.plane];
+              (t.y & (n0 - 1)) == 0 && t.x + n0 <= plane_w[t.plane] && t.y + n0 <= plane_h[t.plane];
     if (ok) {
       const vp9r_block &b = rec->blocks[t.block];
       int sx = t.plane ? h.ss_x : 0, sy = t.plane ? h.ss_y : 0;
       uint32_t bx = (b.mi_col * 8u) >> sx, by = (b.mi_row * 8u) >> sy;
-      ok = t.x >= bx && t.y >= by;
+      ok = t.x >= bx && t.y >= by && (!h.lossless || t.tx_size == 0);
     }
     if (!ok) {
       set_error("malformed record transform block");
@@ -497,95 +614,129 @@ int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *co
     if (rec->blocks[t.block].is_inter && t.eob > 0) g->jobs[t.tx_size].push_back(i);
   }
   build_intra(g, rec);
-  // ---- staging layout
+  work_bytes(rec, p);
+  const bool do_lf = h.lf_level != 0;
+  if (do_lf) build_lf_levels(g, h.sb64_rows, h.sb64_cols);
+  const size_t n_mi = (size_t)h.mi_rows * h.mi_cols;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
     off = align256(off + bytes);
     return o;
   };
-  const size_t o_blocks = take(sizeof(vp9r_block) * rec->n_blocks);
-  const size_t o_txbs = take(sizeof(vp9r_txb) * rec->n_txbs);
-  const size_t o_cpos = take(sizeof(uint16_t) * rec->n_coefs);
-  const size_t o_cval = take(sizeof(int32_t) * rec->n_coefs);
-  const size_t o_tiles = take(sizeof(vp9g::McTile) * g->tiles.size());
-  size_t o_jobs[4];
-  for (int k = 0; k < 4; k++) o_jobs[k] = take(sizeof(uint32_t) * g->jobs[k].size());
-  const size_t o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
-  const size_t o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
-  const size_t total = off;
-  if (g->upload_pending) {
-    HIP_TRY(hipEventSynchronize(g->ev_upload), "staging reuse");
-    g->upload_pending = false;
+  p.o_blocks = take(sizeof(vp9r_block) * rec->n_blocks);
+  p.o_txbs = take(sizeof(vp9r_txb) * rec->n_txbs);
+  p.o_cpos = take(sizeof(uint16_t) * rec->n_coefs);
+  p.o_cval = take(sizeof(int32_t) * rec->n_coefs);
+  p.o_mi = take(sizeof(vp9r_mi) * n_mi);
+  p.o_tiles = take(sizeof(vp9g::McTile) * g->tiles.size());
+  for (int k = 0; k < 4; k++) p.o_jobs[k] = take(sizeof(uint32_t) * g->jobs[k].size());
+  p.o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
+  p.o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
+  p.o_lf = take(sizeof(uint32_t) * (do_lf ? g->lf_codes.size() : 0));
+  p.bytes = off;
+  p.n_tiles = (uint32_t)g->tiles.size();
+  for (int k = 0; k < 4; k++) p.n_jobs[k] = (uint32_t)g->jobs[k].size();
+  p.n_groups = (uint32_t)g->groups.size();
+  p.group_level_start = g->group_level_start;
+  if (do_lf)
+    p.lf_level_start = g->lf_level_start;
+  else
+    p.lf_level_start.clear();
+  if (stage.ensure(p.bytes) != hipSuccess) {
+    set_error("hipHostMalloc(staging) failed");
+    return VP9R_ERR_DEVICE;
   }
-  HIP_TRY(g->stage.ensure(total), "hipHostMalloc(staging)");
-  HIP_TRY(g->arena.ensure(total), "hipMalloc(arena)");
-  uint8_t *st = (uint8_t *)g->stage.p;
+  uint8_t *st = (uint8_t *)stage.p;
   auto put = [&](size_t o, const void *src, size_t bytes) {
     if (bytes) memcpy(st + o, src, bytes);
   };
-  put(o_blocks, rec->blocks, sizeof(vp9r_block) * rec->n_blocks);
-  put(o_txbs, rec->txbs, sizeof(vp9r_txb) * rec->n_txbs);
-  put(o_cpos, rec->coef_pos, sizeof(uint16_t) * rec->n_coefs);
-  put(o_cval, rec->coef_val, sizeof(int32_t) * rec->n_coefs);
-  put(o_tiles, g->tiles.data(), sizeof(vp9g::McTile) * g->tiles.size());
-  for (int k = 0; k < 4; k++) put(o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
-  put(o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
-  put(o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
+  put(p.o_blocks, rec->blocks, sizeof(vp9r_block) * rec->n_blocks);
+  put(p.o_txbs, rec->txbs, sizeof(vp9r_txb) * rec->n_txbs);
+  put(p.o_cpos, rec->coef_pos, sizeof(uint16_t) * rec->n_coefs);
+  put(p.o_cval, rec->coef_val, sizeof(int32_t) * rec->n_coefs);
+  put(p.o_mi, rec->mi, sizeof(vp9r_mi) * n_mi);
+  put(p.o_tiles, g->tiles.data(), sizeof(vp9g::McTile) * g->tiles.size());
+  for (int k = 0; k < 4; k++) put(p.o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
+  put(p.o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
+  put(p.o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
+  if (do_lf) put(p.o_lf, g->lf_codes.data(), sizeof(uint32_t) * g->lf_codes.size());
+  return VP9R_OK;
+}
+
+static int check_frames(const Prepared &p, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur) {
+  const vp9r_frame_hdr &h = p.hdr;
+  if (!cur || cur->width != h.width || cur->height != h.height || cur->bit_depth != h.bit_depth ||
+      cur->ss_x != h.ss_x || cur->ss_y != h.ss_y) {
+    set_error("the frame store does not match the record's geometry");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  if (!h.frame_is_intra)
+    for (int i = 0; i < 3; i++) {
+      const vp9r_gpu_frame *r = refs ? refs[i] : nullptr;
+      if (!r || r->bit_depth != h.bit_depth || r->ss_x != h.ss_x || r->ss_y != h.ss_y) {
+        set_error("missing reference, or a reference of another format");
+        return VP9R_ERR_BITSTREAM;
+      }
+      if (r->width != h.ref_width[i] || r->height != h.ref_height[i]) {
+        set_error("a reference's size is not the one the record names");
+        return VP9R_ERR_INVALID_INPUT;
+      }
+      if (r == cur) {
+        set_error("the frame being reconstructed is one of its references");
+        return VP9R_ERR_INVALID_INPUT;
+      }
+    }
+  return VP9R_OK;
+}
+
+// Reconstruction launches from device inputs at dv (see the file comment).
+static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_gpu_frame *const refs[3],
+                        vp9r_gpu_frame *cur) {
   hipStream_t s = g->stream;
-  if (g->timing) HIP_TRY(hipEventRecord(g->ev[0], s), "hipEventRecord");
-  HIP_TRY(hipMemcpyAsync(g->arena.p, st, total, hipMemcpyHostToDevice, s), "upload record");
-  HIP_TRY(hipEventRecord(g->ev_upload, s), "hipEventRecord");
-  g->upload_pending = true;
-  uint8_t *dv = (uint8_t *)g->arena.p;
-  const vp9g::FrameGeom geo = geom_of(h);
-  int32_t q[8][2][2];
-  q_table(h, q);
-  // ---- inter prediction
-  if (!g->tiles.empty()) {
+  HIP_TRY(order_after_copy(s, cur), "hipStreamWaitEvent");
+  if (p.n_tiles) {
     vp9g::McArgs a{};
-    a.g = geo;
-    a.blocks = (const vp9r_block *)(dv + o_blocks);
-    a.tiles = (const vp9g::McTile *)(dv + o_tiles);
+    a.g = p.geo;
+    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
+    a.tiles = (const vp9g::McTile *)(dv + p.o_tiles);
     a.cur = cur->planes;
     for (int i = 0; i < 3; i++) {
       a.ref[i] = refs[i]->planes;
       a.ref_w[i] = refs[i]->width;
       a.ref_h[i] = refs[i]->height;
     }
-    vp9g::launch_mc(s, a, (uint32_t)g->tiles.size());
+    vp9g::launch_mc(s, a, p.n_tiles);
   }
   if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
-  // ---- inter residuals
   {
     vp9g::ItxArgs a{};
-    a.g = geo;
-    a.blocks = (const vp9r_block *)(dv + o_blocks);
-    a.txbs = (const vp9r_txb *)(dv + o_txbs);
-    a.coef_pos = (const uint16_t *)(dv + o_cpos);
-    a.coef_val = (const int32_t *)(dv + o_cval);
+    a.g = p.geo;
+    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
+    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
+    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
+    a.coef_val = (const int32_t *)(dv + p.o_cval);
     a.cur = cur->planes;
-    memcpy(a.q, q, sizeof q);
+    memcpy(a.q, p.q, sizeof p.q);
     for (int k = 0; k < 4; k++) {
-      a.jobs = (const uint32_t *)(dv + o_jobs[k]);
-      vp9g::launch_itx_inter(s, a, k, (uint32_t)g->jobs[k].size());
+      a.jobs = (const uint32_t *)(dv + p.o_jobs[k]);
+      vp9g::launch_itx_inter(s, a, k, p.n_jobs[k]);
     }
   }
   if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
-  // ---- intra wavefront
-  if (!g->groups.empty()) {
+  if (p.n_groups) {
     vp9g::IntraArgs a{};
-    a.g = geo;
-    a.blocks = (const vp9r_block *)(dv + o_blocks);
-    a.txbs = (const vp9r_txb *)(dv + o_txbs);
-    a.txb_index = (const uint32_t *)(dv + o_iidx);
-    a.groups = (const vp9g::IntraGroup *)(dv + o_groups);
-    a.coef_pos = (const uint16_t *)(dv + o_cpos);
-    a.coef_val = (const int32_t *)(dv + o_cval);
+    a.g = p.geo;
+    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
+    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
+    a.txb_index = (const uint32_t *)(dv + p.o_iidx);
+    a.groups = (const vp9g::IntraGroup *)(dv + p.o_groups);
+    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
+    a.coef_val = (const int32_t *)(dv + p.o_cval);
     a.cur = cur->planes;
-    memcpy(a.q, q, sizeof q);
-    for (size_t l = 0; l + 1 < g->group_level_start.size(); l++) {
-      uint32_t first = g->group_level_start[l], n = g->group_level_start[l + 1] - first;
+    memcpy(a.q, p.q, sizeof p.q);
+    for (size_t l = 0; l + 1 < p.group_level_start.size(); l++) {
+      uint32_t first = p.group_level_start[l], n = p.group_level_start[l + 1] - first;
       vp9g::launch_intra(s, a, first, n);
     }
   }
@@ -594,53 +745,133 @@ int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *co
   return VP9R_OK;
 }
 
-int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cur) {
-  const vp9r_frame_hdr &h = rec->hdr;
-  if (!cur || cur->width != h.width || cur->height != h.height || cur->bit_depth != h.bit_depth ||
-      cur->ss_x != h.ss_x || cur->ss_y != h.ss_y) {
-    set_error("the frame store does not match the record's geometry");
-    return VP9R_ERR_INVALID_INPUT;
-  }
-  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+static int launch_lf(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_gpu_frame *cur) {
   hipStream_t s = g->stream;
-  if (h.lf_level == 0) {
-    if (g->timing) HIP_TRY(hipEventRecord(g->ev[4], s), "hipEventRecord");
-    return VP9R_OK;
-  }
-  build_lf_levels(g, h.sb64_rows, h.sb64_cols);
-  const size_t n_mi = (size_t)h.mi_rows * h.mi_cols;
-  const size_t o_mi = 0, o_codes = align256(sizeof(vp9r_mi) * n_mi);
-  const size_t total = o_codes + sizeof(uint32_t) * g->lf_codes.size();
-  if (g->lf_upload_pending) {
-    HIP_TRY(hipEventSynchronize(g->ev_lf_upload), "staging reuse");
-    g->lf_upload_pending = false;
-  }
-  HIP_TRY(g->lf_stage.ensure(total), "hipHostMalloc(staging)");
-  HIP_TRY(g->lf_arena.ensure(total), "hipMalloc(arena)");
-  uint8_t *st = (uint8_t *)g->lf_stage.p;
-  memcpy(st + o_mi, rec->mi, sizeof(vp9r_mi) * n_mi);
-  memcpy(st + o_codes, g->lf_codes.data(), sizeof(uint32_t) * g->lf_codes.size());
-  HIP_TRY(hipMemcpyAsync(g->lf_arena.p, st, total, hipMemcpyHostToDevice, s), "upload mode info");
-  HIP_TRY(hipEventRecord(g->ev_lf_upload, s), "hipEventRecord");
-  g->lf_upload_pending = true;
-  uint8_t *dv = (uint8_t *)g->lf_arena.p;
-  const size_t n_sb = (size_t)h.sb64_rows * h.sb64_cols;
-  HIP_TRY(g->lf_mask.ensure(n_sb * 6 * 256 * 2), "hipMalloc(lf masks)");
-  vp9g::LfArgs a{};
-  a.g = geom_of(h);
-  a.mi = (const vp9r_mi *)(dv + o_mi);
-  a.mask_fs = (int8_t *)g->lf_mask.p;
-  a.mask_lvl = (uint8_t *)g->lf_mask.p + n_sb * 6 * 256;
-  a.cur = cur->planes;
-  lf_params(h, a.p);
-  a.sbs = (const uint32_t *)(dv + o_codes);
-  vp9g::launch_lf_mask(s, a);
-  for (size_t l = 0; l + 1 < g->lf_level_start.size(); l++) {
-    uint32_t first = g->lf_level_start[l], n = g->lf_level_start[l + 1] - first;
-    vp9g::launch_lf_level(s, a, first, n);
+  HIP_TRY(order_after_copy(s, cur), "hipStreamWaitEvent");
+  if (!p.lf_level_start.empty()) {
+    const vp9r_frame_hdr &h = p.hdr;
+    const size_t n_sb = (size_t)h.sb64_rows * h.sb64_cols;
+    HIP_TRY(g->lf_mask.ensure(n_sb * 6 * 256 * 2), "hipMalloc(lf masks)");
+    vp9g::LfArgs a{};
+    a.g = p.geo;
+    a.mi = (const vp9r_mi *)(dv + p.o_mi);
+    a.mask_fs = (int8_t *)g->lf_mask.p;
+    a.mask_lvl = (uint8_t *)g->lf_mask.p + n_sb * 6 * 256;
+    a.cur = cur->planes;
+    a.p = p.lfp;
+    a.sbs = (const uint32_t *)(dv + p.o_lf);
+    vp9g::launch_lf_mask(s, a);
+    for (size_t l = 0; l + 1 < p.lf_level_start.size(); l++) {
+      uint32_t first = p.lf_level_start[l], n = p.lf_level_start[l + 1] - first;
+      vp9g::launch_lf_level(s, a, first, n);
+    }
   }
   if (g->timing) HIP_TRY(hipEventRecord(g->ev[4], s), "hipEventRecord");
   HIP_TRY(hipGetLastError(), "kernel launch");
+  return VP9R_OK;
+}
+
+}  // namespace vp9r
+
+struct vp9r_prepared {
+  vp9r::Prepared p;
+  vp9r::DevBuf dev;
+};
+
+extern "C" {
+
+int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *const refs[3],
+                         vp9r_gpu_frame *cur) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (g->upload_pending) {
+    HIP_TRY(hipEventSynchronize(g->ev_upload), "staging reuse");
+    g->upload_pending = false;
+  }
+  Prepared &p = g->last;
+  int st = build(g, rec, p, g->stage);
+  if (st) return st;
+  st = check_frames(p, refs, cur);
+  if (st) return st;
+  HIP_TRY(g->arena.ensure(p.bytes), "hipMalloc(arena)");
+  hipStream_t s = g->stream;
+  if (g->timing) {
+    HIP_TRY(begin_timed_frame(g), "stage timing");
+    HIP_TRY(hipEventRecord(g->ev[0], s), "hipEventRecord");
+  }
+  HIP_TRY(hipMemcpyAsync(g->arena.p, g->stage.p, p.bytes, hipMemcpyHostToDevice, s), "upload record");
+  HIP_TRY(hipEventRecord(g->ev_upload, s), "hipEventRecord");
+  g->upload_pending = true;
+  g->last_valid = true;
+  return launch_recon(g, p, (const uint8_t *)g->arena.p, refs, cur);
+}
+
+int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cur) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  // the record just reconstructed: its inputs are on the device already
+  if (!g->last_valid || memcmp(&g->last.hdr, &rec->hdr, sizeof rec->hdr) != 0) {
+    set_error("vp9r_gpu_loop_filter takes the record last given to vp9r_gpu_reconstruct");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  vp9r_gpu_frame *no_refs[3] = {nullptr, nullptr, nullptr};
+  (void)no_refs;
+  if (!cur || cur->width != rec->hdr.width || cur->height != rec->hdr.height) {
+    set_error("the frame store does not match the record's geometry");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  if (g->timing && !g->ev) {
+    set_error("timing was enabled between vp9r_gpu_reconstruct and vp9r_gpu_loop_filter");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  return launch_lf(g, g->last, (const uint8_t *)g->arena.p, cur);
+}
+
+int vp9r_gpu_prepare(vp9r_gpu *g, const vp9r_record *rec, vp9r_prepared **out) {
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (g->upload_pending) {
+    HIP_TRY(hipEventSynchronize(g->ev_upload), "staging reuse");
+    g->upload_pending = false;
+  }
+  vp9r_prepared *pr = new vp9r_prepared();
+  int st = build(g, rec, pr->p, g->stage);
+  if (st) {
+    delete pr;
+    return st;
+  }
+  hipError_t e = pr->dev.ensure(pr->p.bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(pr->dev.p, g->stage.p, pr->p.bytes, hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) {
+    delete pr;
+    return hip_fail(e, "upload prepared record");
+  }
+  *out = pr;
+  return VP9R_OK;
+}
+
+void vp9r_prepared_free(vp9r_prepared *p) { delete p; }
+
+int vp9r_gpu_run(vp9r_gpu *g, const vp9r_prepared *pr, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  int st = check_frames(pr->p, refs, cur);
+  if (st) return st;
+  if (g->timing) {
+    HIP_TRY(begin_timed_frame(g), "stage timing");
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream), "hipEventRecord");
+  }
+  st = launch_recon(g, pr->p, (const uint8_t *)pr->dev.p, refs, cur);
+  if (st) return st;
+  return launch_lf(g, pr->p, (const uint8_t *)pr->dev.p, cur);
+}
+
+int vp9r_gpu_prepared_stats(const vp9r_prepared *pr, uint64_t out[12]) {
+  const vp9r::Prepared &p = pr->p;
+  for (int k = 0; k < 4; k++) out[8 + k] = p.work_bytes[k];
+  out[0] = p.n_tiles;
+  for (int k = 0; k < 4; k++) out[1 + k] = p.n_jobs[k];
+  out[5] = p.n_groups;
+  out[6] = p.group_level_start.empty() ? 0 : p.group_level_start.size() - 1;
+  out[7] = p.lf_level_start.empty() ? 0 : p.lf_level_start.size() - 1;
   return VP9R_OK;
 }
 
@@ -656,6 +887,46 @@ int vp9r_gpu_download_packed(vp9r_gpu *g, vp9r_gpu_frame *f, uint8_t *dst) {
   }
   HIP_TRY(hipStreamSynchronize(g->stream), "download sync");
   return VP9R_OK;
+}
+
+int vp9r_gpu_download_packed_async(vp9r_gpu *g, vp9r_gpu_frame *f, uint8_t *dst) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (f->g != g) {
+    set_error("the frame store belongs to another context");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  // the copy starts once the work queued so far on the compute stream is done
+  HIP_TRY(hipEventRecord(f->ready, g->stream), "hipEventRecord");
+  HIP_TRY(hipStreamWaitEvent(g->copy_stream, f->ready, 0), "hipStreamWaitEvent");
+  size_t bps = f->bit_depth > 8 ? 2 : 1, off = 0;
+  for (int p = 0; p < 3; p++) {
+    size_t w = p ? (f->width + f->ss_x) >> f->ss_x : f->width;
+    size_t hh = p ? (f->height + f->ss_y) >> f->ss_y : f->height;
+    HIP_TRY(hipMemcpy2DAsync(dst + off, w * bps, f->planes.p[p], (size_t)f->planes.stride[p] * bps, w * bps,
+                             hh, hipMemcpyDeviceToHost, g->copy_stream),
+            "download frame");
+    off += w * hh * bps;
+  }
+  HIP_TRY(hipEventRecord(f->copy_done, g->copy_stream), "hipEventRecord");
+  f->copy_pending = true;
+  return VP9R_OK;
+}
+
+int vp9r_gpu_frame_wait(vp9r_gpu_frame *f) {
+  if (!f->copy_pending) return VP9R_OK;
+  HIP_TRY(hipEventSynchronize(f->copy_done), "hipEventSynchronize");
+  f->copy_pending = false;
+  return VP9R_OK;
+}
+
+int vp9r_host_alloc(size_t bytes, void **out) {
+  *out = nullptr;
+  HIP_TRY(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+  return VP9R_OK;
+}
+
+void vp9r_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
 }
 
 }  // extern "C"

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,23 @@ struct HostBuf {
   ~HostBuf();
 };
 
+// The device inputs of one frame: the record and its work lists in one
+// device buffer (at the offsets below), with the launch geometry.
+struct Prepared {
+  vp9r_frame_hdr hdr;
+  vp9g::FrameGeom geo;
+  int32_t q[8][2][2];
+  vp9g::LfParams lfp;
+  size_t o_blocks = 0, o_txbs = 0, o_cpos = 0, o_cval = 0, o_mi = 0, o_tiles = 0, o_jobs[4] = {},
+         o_iidx = 0, o_groups = 0, o_lf = 0;
+  uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0;
+  std::vector<uint32_t> group_level_start, lf_level_start;
+  size_t bytes = 0;
+  // algorithmic HBM bytes of the frame per stage: inter prediction, inter
+  // residual, intra, loop filter
+  uint64_t work_bytes[4] = {};
+};
+
 }  // namespace vp9r
 
 struct vp9r_gpu_frame {
@@ -39,18 +57,30 @@ struct vp9r_gpu_frame {
   vp9g::Planes planes{};
   void *base = nullptr;
   size_t bytes = 0;
+  // an asynchronous download in flight on the copy stream (the compute
+  // stream waits for it before writing the frame again)
+  hipEvent_t ready = nullptr, copy_done = nullptr;
+  bool copy_pending = false;
 };
 
 struct vp9r_gpu {
   int device = 0;
-  hipStream_t stream = nullptr;
-  vp9r::DevBuf arena, lf_arena, lf_mask;
-  vp9r::HostBuf stage, lf_stage;
-  hipEvent_t ev_upload = nullptr, ev_lf_upload = nullptr;
-  bool upload_pending = false, lf_upload_pending = false;
+  hipStream_t stream = nullptr;       // kernels and uploads
+  hipStream_t copy_stream = nullptr;  // frame downloads
+  vp9r::DevBuf arena, lf_mask;
+  vp9r::HostBuf stage;
+  hipEvent_t ev_upload = nullptr;
+  bool upload_pending = false;
+  // the record of the last vp9r_gpu_reconstruct (its inputs are in `arena`)
+  vp9r::Prepared last;
+  bool last_valid = false;
+  // stage timing: one set of 5 events per frame since the last read
   bool timing = false;
-  hipEvent_t ev[5] = {};
-  float stage_ms[5] = {};
+  std::vector<std::array<hipEvent_t, 5>> ev_sets;
+  size_t ev_used = 0;
+  hipEvent_t *ev = nullptr;  // the current frame's set
+  double stage_acc[5] = {};   // folded sums (ms) of sets already read back
+  uint32_t frames_acc = 0;
   // host work lists, reused
   std::vector<vp9g::McTile> tiles;
   std::vector<uint32_t> jobs[4];
