@@ -129,14 +129,85 @@ DEV void intra_residual(typename L::T *coef, uint16_t *tile, int lx, int ly, con
   itx_2d<L, n>(coef, lane, t.tx_type, lossless, dc_only, dc_v, dst, (1 << a.g.bit_depth) - 1);
 }
 
+struct IntraShared {
+  uint16_t tile[TS * TS];  // rows y0 - 1 .., columns x0 - 1 .. of the region
+  int32_t edge[65 + 32];
+};
+
+// Predicts and reconstructs transform block t in the LDS tile whose sample
+// (0, 0) is plane position (x0 - 1, y0 - 1) (edges: recon.rs:17-103).
+template <class L, bool HBD>
+DEV void intra_txb(const IntraArgs &a, IntraShared &sh, typename L::T *coef, const vp9r_txb &t, int x0,
+                   int y0) {
+  const FrameGeom &g = a.g;
+  const int lane = threadIdx.x;
+  const vp9r_block &b = a.blocks[t.block];
+  const int plane = t.plane;
+  const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+  const int max_x = (int)((g.mi_cols * 8) >> sx) - 1, max_y = (int)((g.mi_rows * 8) >> sy) - 1;
+  const int base = 1 << (g.bit_depth - 1);
+  const int maxv = (1 << g.bit_depth) - 1;
+  uint16_t *tile = sh.tile;
+  int32_t *above = sh.edge, *left = sh.edge + 65;
+  const int32_t *A = above + 1;
+#define TILE(py, px) ((int)tile[((py) - y0 + 1) * TS + ((px) - x0 + 1)])
+  const int tx = t.tx_size, log2 = tx + 2, size = 1 << log2;
+  const int bsize = b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size;
+  const int n4w = num4x4_wide(bsize) >> sx;
+  const int base_x = (b.mi_col * 8) >> sx, base_y = (b.mi_row * 8) >> sy;
+  const int xi = (t.x - base_x) >> 2, yi = (t.y - base_y) >> 2;
+  const int step = 1 << tx;
+  const bool have_left = b.avail_l || xi > 0, have_above = b.avail_u || yi > 0;
+  const bool not_on_right = xi + step < n4w;
+  const int block_idx = (yi / step) * ((n4w + step - 1) / step) + xi / step;
+  const int mode = plane > 0 ? b.uv_mode : (b.mi_size >= BLOCK_8X8 ? b.y_mode : b.sub_modes[block_idx & 3]);
+  const int x = t.x, y = t.y;
+  // aboveRow[-1 .. 2 * size - 1]: 2 * size + 1 entries, 65 at 32x32, so
+  // lane 0 also fills the last one
+  for (int k = lane; k <= 2 * size; k += 64) {
+    int v;
+    if (!have_above) {
+      v = base - 1;
+    } else if (k == 0) {
+      v = have_left ? TILE(y - 1, min(max_x, x - 1)) : base + 1;
+    } else {
+      int i = k - 1;
+      if (i >= size && !(not_on_right && tx == TX_4X4)) i = size - 1;
+      v = TILE(y - 1, min(max_x, x + i));
+    }
+    above[k] = v;
+  }
+  if (lane < size) left[lane] = have_left ? TILE(min(max_y, y + lane), x - 1) : base + 1;
+#undef TILE
+  __syncthreads();
+  const int dc = mode == DC_PRED ? dc_value(log2, A, left, have_left, have_above, g.bit_depth) : 0;
+  const int lx = x - x0 + 1, ly = y - y0 + 1;
+  for (int k = lane; k < size * size; k += 64) {
+    int i = k >> log2, j = k & (size - 1);
+    tile[(ly + i) * TS + lx + j] = (uint16_t)intra_sample(mode, i, j, size, A, left, dc, maxv);
+  }
+  __syncthreads();
+  if (t.eob > 0) {
+    const int seg = b.segment_id & 7;
+    switch (tx) {
+      case 0: intra_residual<L, 2>(coef, tile, lx, ly, t, a, seg, HBD); break;
+      case 1: intra_residual<L, 3>(coef, tile, lx, ly, t, a, seg, HBD); break;
+      case 2: intra_residual<L, 4>(coef, tile, lx, ly, t, a, seg, HBD); break;
+      default: intra_residual<L, 5>(coef, tile, lx, ly, t, a, seg, HBD); break;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- one wavefront level of (superblock, plane) groups per launch
+
 template <bool HBD>
 __global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group) {
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ uint16_t tile[TS * TS];
+  __shared__ IntraShared sh;
   __shared__ T coef[32 * 33];
-  __shared__ int32_t edge[65 + 32];
   const int lane = threadIdx.x;
   const IntraGroup gr = a.groups[first_group + blockIdx.x];
   const FrameGeom &g = a.g;
@@ -149,67 +220,132 @@ __global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group)
   for (int k = lane; k < (SH + 1) * (SW + 1); k += 64) {
     int r = k / (SW + 1), c = k - r * (SW + 1);
     int py = y0 - 1 + r, px = x0 - 1 + c;
-    if (py >= 0 && px >= 0) tile[r * TS + c] = pl[(size_t)py * stride + px];
+    if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
   }
   __syncthreads();
-  const int max_x = (int)((g.mi_cols * 8) >> sx) - 1, max_y = (int)((g.mi_rows * 8) >> sy) - 1;
-  const int base = 1 << (g.bit_depth - 1);
-  const int maxv = (1 << g.bit_depth) - 1;
-  int32_t *above = edge, *left = edge + 65;
-  const int32_t *A = above + 1;
-#define TILE(py, px) ((int)tile[((py) - y0 + 1) * TS + ((px) - x0 + 1)])
-  for (uint32_t ti = 0; ti < gr.count; ti++) {
-    const vp9r_txb t = a.txbs[a.txb_index[gr.first + ti]];
-    const vp9r_block &b = a.blocks[t.block];
-    const int tx = t.tx_size, log2 = tx + 2, size = 1 << log2;
-    const int bsize = b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size;
-    const int n4w = num4x4_wide(bsize) >> sx;
-    const int base_x = (b.mi_col * 8) >> sx, base_y = (b.mi_row * 8) >> sy;
-    const int xi = (t.x - base_x) >> 2, yi = (t.y - base_y) >> 2;
-    const int step = 1 << tx;
-    const bool have_left = b.avail_l || xi > 0, have_above = b.avail_u || yi > 0;
-    const bool not_on_right = xi + step < n4w;
-    const int block_idx = (yi / step) * ((n4w + step - 1) / step) + xi / step;
-    const int mode = plane > 0 ? b.uv_mode : (b.mi_size >= BLOCK_8X8 ? b.y_mode : b.sub_modes[block_idx]);
-    const int x = t.x, y = t.y;
-    if (lane <= 2 * size) {
-      int v;
-      if (!have_above) {
-        v = base - 1;
-      } else if (lane == 0) {
-        v = have_left ? TILE(y - 1, min(max_x, x - 1)) : base + 1;
-      } else {
-        int i = lane - 1;
-        if (i >= size && !(not_on_right && tx == TX_4X4)) i = size - 1;
-        v = TILE(y - 1, min(max_x, x + i));
-      }
-      above[lane] = v;
-    }
-    if (lane < size) left[lane] = have_left ? TILE(min(max_y, y + lane), x - 1) : base + 1;
-    __syncthreads();
-    const int dc = mode == DC_PRED ? dc_value(log2, A, left, have_left, have_above, g.bit_depth) : 0;
-    const int lx = x - x0 + 1, ly = y - y0 + 1;
-    for (int k = lane; k < size * size; k += 64) {
-      int i = k >> log2, j = k & (size - 1);
-      tile[(ly + i) * TS + lx + j] = (uint16_t)intra_sample(mode, i, j, size, A, left, dc, maxv);
-    }
-    __syncthreads();
-    if (t.eob > 0) {
-      const int seg = b.segment_id;
-      switch (tx) {
-        case 0: intra_residual<L, 2>(coef, tile, lx, ly, t, a, seg, HBD); break;
-        case 1: intra_residual<L, 3>(coef, tile, lx, ly, t, a, seg, HBD); break;
-        case 2: intra_residual<L, 4>(coef, tile, lx, ly, t, a, seg, HBD); break;
-        default: intra_residual<L, 5>(coef, tile, lx, ly, t, a, seg, HBD); break;
-      }
-      __syncthreads();
-    }
-  }
-#undef TILE
+  for (uint32_t ti = 0; ti < gr.count; ti++)
+    intra_txb<L, HBD>(a, sh, coef, a.txbs[a.txb_index[gr.first + ti]], x0, y0);
   for (int k = lane; k < SH * SW; k += 64) {
     int r = k / SW, c = k - r * SW;
-    pl[(size_t)(y0 + r) * stride + x0 + c] = (P)tile[(r + 1) * TS + c + 1];
+    pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
   }
+}
+
+// ---- every intra block of the frame in one launch
+//
+// Workgroups take intra blocks in decode order (a topological order: a
+// block reads only samples of blocks decoded before it) through a ticket
+// counter.  A block reads the row above it (its above-left sample to its
+// top-right sample) and the column left of it; before it starts, it waits
+// until every intra block covering those 8x8 positions has published this
+// frame's epoch in the per-position flags (inter blocks are complete when
+// the launch starts).  Then it reconstructs its planes one after the other
+// in LDS, writes them back and publishes its own positions.
+
+DEV void intra_wait(const IntraArgs &a, uint32_t cell) {
+  uint32_t spins = 0;
+  while (__hip_atomic_load(a.sync.done + cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.sync.epoch) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kWaveSpinLimit) {
+      __hip_atomic_fetch_or(a.sync.status, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+  }
+}
+
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_intra_wave(IntraArgs a) {
+  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
+  typedef typename L::T T;
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  __shared__ IntraShared sh;
+  __shared__ T coef[32 * 33];
+  __shared__ uint32_t slot;
+  const FrameGeom &g = a.g;
+  const int lane = threadIdx.x;
+  for (;;) {
+    if (lane == 0)
+      slot = __hip_atomic_fetch_add(a.sync.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t t = slot;
+    __syncthreads();
+    if (t >= a.n_items) return;
+    const IntraItem it = a.items[t];
+    const vp9r_block &b = a.blocks[it.block];
+    const int mi_row = b.mi_row, mi_col = b.mi_col;
+    const int bw8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_wide(b.mi_size);
+    const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_high(b.mi_size);
+    // wait for the intra blocks above (from above-left on) and to the left
+    {
+      int r = -1, c = -1;
+      if (lane <= bw8) {
+        r = mi_row - 1;
+        c = mi_col - 1 + lane;
+        if (!b.avail_u) r = -1;
+      } else if (lane <= bw8 + bh8) {
+        r = mi_row + lane - bw8 - 1;
+        c = mi_col - 1;
+        if (!b.avail_l) r = -1;
+      }
+      if (r >= 0 && c >= 0 && r < (int)g.mi_rows && c < (int)g.mi_cols) {
+        const uint32_t cell = (uint32_t)r * g.mi_cols + c;
+        if (a.mi[cell].ref_frame0 <= INTRA_FRAME) intra_wait(a, cell);
+      }
+    }
+    __syncthreads();
+    __threadfence();
+    uint32_t k0 = it.first;
+    const uint32_t k_end = it.first + it.count;
+    for (int plane = 0; plane < 3; plane++) {
+      const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+      const int x0 = (mi_col * 8) >> sx, y0 = (mi_row * 8) >> sy;
+      const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy;
+      const uint32_t stride = a.cur.stride[plane];
+      P *pl = (P *)a.cur.p[plane];
+      // the row above (x0 - 1 .. x0 + BW - 1) and the column left
+      for (int k = lane; k < BW + 1 + BH; k += 64) {
+        int r, c;
+        if (k <= BW) {
+          r = 0;
+          c = k;
+        } else {
+          r = k - BW;
+          c = 0;
+        }
+        int py = y0 - 1 + r, px = x0 - 1 + c;
+        if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
+      }
+      __syncthreads();
+      const uint32_t k_plane = k0;
+      while (k0 < k_end && a.txbs[k0].plane == plane) {
+        intra_txb<L, HBD>(a, sh, coef, a.txbs[k0], x0, y0);
+        k0++;
+      }
+      if (k0 > k_plane)
+        for (int k = lane; k < BW * BH; k += 64) {
+          int r = k / BW, c = k - r * BW;
+          pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
+        }
+      __syncthreads();
+    }
+    if (lane < bw8 * bh8) {
+      const int r = mi_row + lane / bw8, c = mi_col + lane % bw8;
+      if (r < (int)g.mi_rows && c < (int)g.mi_cols) {
+        __threadfence();
+        __hip_atomic_store(a.sync.done + (uint32_t)r * g.mi_cols + c, a.sync.epoch, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
+void launch_intra_wave(hipStream_t s, const IntraArgs &a, uint32_t n_groups) {
+  if (!a.n_items) return;
+  const uint32_t n = min(a.n_items, n_groups);
+  if (a.g.bit_depth > 8)
+    hipLaunchKernelGGL(k_intra_wave<true>, dim3(n), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_intra_wave<false>, dim3(n), dim3(64), 0, s, a);
 }
 
 void launch_intra(hipStream_t s, const IntraArgs &a, uint32_t first_group, uint32_t n_groups) {

@@ -63,6 +63,26 @@ struct IntraGroup {
   uint32_t first, count;  // into the intra txb index list
 };
 
+// Device-side scheduling of a wavefront in one launch: tickets hand out
+// work items in a topological order, and each item's completion flag holds
+// the epoch (frame number) that completed it.
+struct WaveSync {
+  uint32_t *ticket;  // zero at launch
+  uint32_t *done;    // per item: epoch of the last frame that completed it
+  uint32_t *status;  // set to nonzero when a wait exceeds its spin limit
+  uint32_t epoch;    // nonzero
+};
+
+// Polls of a completion flag before a wait gives up (with s_sleep between
+// polls: on the order of a second).
+constexpr uint32_t kWaveSpinLimit = 1u << 22;
+
+// An intra block of the one-launch intra wavefront: its transform blocks
+// are txbs[first .. first + count) (planes in order).
+struct IntraItem {
+  uint32_t block, first, count;
+};
+
 struct IntraArgs {
   FrameGeom g;
   const vp9r_block *blocks;
@@ -73,6 +93,10 @@ struct IntraArgs {
   const int32_t *coef_val;
   Planes cur;
   int32_t q[8][2][2];
+  const vp9r_mi *mi;         // one-launch wavefront: which 8x8 positions are intra
+  const IntraItem *items;    // one-launch wavefront: intra blocks in decode order
+  uint32_t n_items;
+  WaveSync sync;             // flags per 8x8 position
 };
 
 // Loop filter: per superblock, plane and pass, 16 x 16 entries of filter
@@ -85,18 +109,22 @@ struct LfParams {
 struct LfArgs {
   FrameGeom g;
   const vp9r_mi *mi;
-  int8_t *mask_fs;   // [plane][pass][sb][16][16]
+  int8_t *mask_fs;   // [plane][pass][sb][16][16] (level launches)
   uint8_t *mask_lvl;
   Planes cur;
   LfParams p;
-  const uint32_t *sbs;  // (plane << 28 | sb_row << 14 | sb_col) per group of a level
+  const uint32_t *sbs;  // (plane << 28 | sb_row << 14 | sb_col): per level, or all in order
+  uint32_t n_items;     // one-launch wavefront: entries of sbs
+  WaveSync sync;        // one-launch wavefront: flags per (plane, superblock)
 };
 
 void launch_mc(hipStream_t s, const McArgs &a, uint32_t n_tiles);
 void launch_itx_inter(hipStream_t s, const ItxArgs &a, int tx_size, uint32_t n_jobs);
 void launch_intra(hipStream_t s, const IntraArgs &a, uint32_t first_group, uint32_t n_groups);
+void launch_intra_wave(hipStream_t s, const IntraArgs &a, uint32_t n_groups);
 void launch_lf_mask(hipStream_t s, const LfArgs &a);
 void launch_lf_level(hipStream_t s, const LfArgs &a, uint32_t first, uint32_t n);
+void launch_lf_wave(hipStream_t s, const LfArgs &a, uint32_t n_groups);
 
 // Per-kernel seam (vp9r.h batched forms).
 void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,

@@ -27,6 +27,27 @@ constexpr int BP = 74;  // LDS pitch of the 72 x 72 region (odd in dwords)
 
 DEV int iabs(int v) { return v < 0 ? -v : v; }
 
+// The wide filters (8.8.5.3) across the edge at b[pos]: output i in
+// [-n, n) is Round2(s[i] + sum_{j=-n..n} s[clamp(i + j, -n - 1, n)], LOG2)
+// with n = 2^(LOG2-1) - 1; fully unrolled, so the taps stay in registers.
+template <int LOG2>
+DEV void lf_wide(uint16_t *b, int pos, int step) {
+  constexpr int n = (1 << (LOG2 - 1)) - 1;
+  int s[2 * n + 2];
+#pragma unroll
+  for (int k = -(n + 1); k <= n; k++) s[k + n + 1] = b[pos + k * step];
+#pragma unroll
+  for (int i = -n; i < n; i++) {
+    int t = s[i + n + 1];
+#pragma unroll
+    for (int j = -n; j <= n; j++) {
+      const int k = i + j < -(n + 1) ? -(n + 1) : (i + j > n ? n : i + j);
+      t += s[k + n + 1];
+    }
+    b[pos + i * step] = (uint16_t)((t + (1 << (LOG2 - 1))) >> LOG2);
+  }
+}
+
 // lf::filter on an LDS buffer: across the edge whose first q sample is
 // b[pos], `step` apart.
 DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit, int thresh, int bd) {
@@ -58,141 +79,135 @@ DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit
     }
     return;
   }
-  bool flat2 = false;
-  if (fs >= 2) {
-    flat2 = true;
+  bool flat2 = fs >= 2;
+  if (flat2) {
+#pragma unroll
     for (int k = 4; k < 8; k++)
-      if (iabs((int)b[pos - (k + 1) * step] - p0) > one || iabs((int)b[pos + k * step] - q0) > one) {
-        flat2 = false;
-        break;
-      }
+      flat2 = flat2 && iabs((int)b[pos - (k + 1) * step] - p0) <= one && iabs((int)b[pos + k * step] - q0) <= one;
   }
   // wide filter (8.8.5.3): log2 3 (8 taps) or 4 (16 taps)
-  const int log2 = (fs == 1 || !flat2) ? 3 : 4;
-  const int n = (1 << (log2 - 1)) - 1;
-  int s[16], f[16];
-  for (int k = -(n + 1); k <= n; k++) s[k + 8] = b[pos + k * step];
-  for (int i = -n; i < n; i++) {
-    int t = s[i + 8];
-    for (int j = -n; j <= n; j++) t += s[iclamp(i + j, -(n + 1), n) + 8];
-    f[i + 8] = (t + (1 << (log2 - 1))) >> log2;
-  }
-  for (int i = -n; i < n; i++) b[pos + i * step] = (uint16_t)f[i + 8];
+  if (flat2)
+    lf_wide<4>(b, pos, step);
+  else
+    lf_wide<3>(b, pos, step);
 }
 
 // ---- decisions
 
-__global__ __launch_bounds__(256) void k_lf_mask(LfArgs a) {
+// The filter size (-1: none) and level of entry (ed, rn) of one superblock
+// pass (superblock(), loopfilter.rs:466-620): edge ed (4 samples apart
+// across), run rn (4 samples along).
+DEV void lf_decide(const LfArgs &a, int plane, int pass, uint32_t sb_row, uint32_t sb_col, int ed, int rn,
+                   int &fs_out, int &lvl_out) {
   const FrameGeom &g = a.g;
-  const int sb = blockIdx.x, pass = blockIdx.y, plane = blockIdx.z;
-  const int n_sb = g.sb64_rows * g.sb64_cols;
-  const uint32_t row = (sb / g.sb64_cols) * 8, col = (sb % g.sb64_cols) * 8;
-  const int ed = threadIdx.x >> 4, rn = threadIdx.x & 15;
+  const uint32_t row = sb_row * 8, col = sb_col * 8;
   const int sub_x = plane ? g.ss_x : 0, sub_y = plane ? g.ss_y : 0;
   const int sub = pass == 0 ? sub_x : sub_y;
   const int edge_len = pass == 0 ? 64 >> sub_y : 64 >> sub_x;
   const int n_edges = 16 >> sub, n_runs = edge_len / 4;
   const uint32_t mi_cols = g.mi_cols, mi_rows = g.mi_rows;
-  int fs = -1, lvl = 0;
+  fs_out = -1;
+  lvl_out = 0;
   auto at_x = [&](int e, int r) -> uint32_t {
     return pass == 0 ? col * 8 + e * (4u << sub_x) : col * 8 + ((4u * r) << sub_x);
   };
   auto at_y = [&](int e, int r) -> uint32_t {
     return pass == 0 ? row * 8 + ((4u * r) << sub_y) : row * 8 + e * (4u << sub_y);
   };
-  do {
-    if (ed >= n_edges || rn >= n_runs) break;
-    const int ca = ed >> 1, cb = rn >> 1, k = rn & 1;
-    uint32_t x = at_x(2 * ca, 2 * cb), y = at_y(2 * ca, 2 * cb);
-    if (x >= 8 * mi_cols || y >= 8 * mi_rows) break;
-    const uint32_t loop_col = ((x >> 3) >> sub_x) << sub_x, loop_row = ((y >> 3) >> sub_y) << sub_y;
-    const vp9r_mi m = a.mi[loop_row * mi_cols + loop_col];
-    const int mode_type = m.y_mode == NEARESTMV || m.y_mode == NEARMV || m.y_mode == NEWMV;
-    const int rf = m.ref_frame0 > 0 ? m.ref_frame0 : 0;
-    const int l = a.p.lvl[m.segment_id][rf][mode_type];
-    if (l == 0) break;
-    int tx_sz = m.tx_size;
-    if (plane > 0) {
-      if (m.mi_size < BLOCK_8X8) {
-        tx_sz = TX_4X4;
-      } else {
-        // get_uv_tx_size: min(tx_size, max_txsize_lookup[ss_size_lookup[...]]),
-        // i.e. the largest transform fitting the block's chroma
-        const int ux = g.legacy_uv ? 1 : g.ss_x, uy = g.legacy_uv ? 1 : g.ss_y;
-        const int wl = ((kW4P >> (3 * m.mi_size)) & 7) - ux, hl = ((kH4P >> (3 * m.mi_size)) & 7) - uy;
-        const int uv_max = min(min(wl, hl), 3);
-        tx_sz = min(tx_sz, uv_max);
-      }
+  if (ed >= n_edges || rn >= n_runs) return;
+  const int ca = ed >> 1, cb = rn >> 1, k = rn & 1;
+  uint32_t x = at_x(2 * ca, 2 * cb), y = at_y(2 * ca, 2 * cb);
+  if (x >= 8 * mi_cols || y >= 8 * mi_rows) return;
+  const uint32_t loop_col = ((x >> 3) >> sub_x) << sub_x, loop_row = ((y >> 3) >> sub_y) << sub_y;
+  const vp9r_mi m = a.mi[loop_row * mi_cols + loop_col];
+  const int mode_type = m.y_mode == NEARESTMV || m.y_mode == NEARMV || m.y_mode == NEWMV;
+  const int rf = m.ref_frame0 > 0 ? m.ref_frame0 : 0;
+  const int l = a.p.lvl[m.segment_id & 7][rf & 3][mode_type];
+  if (l == 0) return;
+  int tx_sz = m.tx_size;
+  if (plane > 0) {
+    if (m.mi_size < BLOCK_8X8) {
+      tx_sz = TX_4X4;
+    } else {
+      // get_uv_tx_size: min(tx_size, max_txsize_lookup[ss_size_lookup[...]]),
+      // i.e. the largest transform fitting the block's chroma
+      const int ux = g.legacy_uv ? 1 : g.ss_x, uy = g.legacy_uv ? 1 : g.ss_y;
+      const int wl = ((kW4P >> (3 * m.mi_size)) & 7) - ux, hl = ((kH4P >> (3 * m.mi_size)) & 7) - uy;
+      const int uv_max = min(min(wl, hl), 3);
+      tx_sz = min(tx_sz, uv_max);
     }
-    const int sb_size = sub == 0 ? m.mi_size : max((int)m.mi_size, (int)BLOCK_16X16);
-    const uint32_t block_mask = pass == 0 ? 8u * num8x8_wide(sb_size) - 1 : 8u * num8x8_high(sb_size) - 1;
-    const bool coded = m.ref_frame0 <= INTRA_FRAME || !m.skip;
-    x = at_x(ed, 2 * cb);
-    y = at_y(ed, 2 * cb);
-    if ((pass == 0 && x == 0) || (pass == 1 && y == 0)) break;
-    const uint32_t across = pass == 0 ? x : y;
-    const bool is_block_edge = (across & block_mask) == 0;
-    const bool tx_edge = (ed & ((1 << tx_sz) - 1)) == 0;
-    if (!is_block_edge && !(tx_edge && coded)) break;
-    const bool is_32_edge = ed % 8 == 0;
-    const int base_size = (tx_sz == TX_4X4 && is_32_edge) ? TX_8X8 : min(tx_sz, (int)TX_16X16);
-    const int filter_size =
-        ((pass == 0 && sub_x == 1 && base_size == TX_16X16 && (x >> 3) == mi_cols - 1) ||
-         (pass == 1 && sub_y == 1 && base_size == TX_16X16 && (y >> 3) == mi_rows - 1))
-            ? TX_8X8
-            : base_size;
-    const uint32_t xx = at_x(ed, rn), yy = at_y(ed, rn);
-    if (k == 1 && (xx >= 8 * mi_cols || yy >= 8 * mi_rows)) break;
-    const bool is_tx_edge =
-        (pass == 1 && sub_x == 1 && (mi_cols & 1) && (ed & 1) && (xx + 8) >= mi_cols * 8) ? false : tx_edge;
-    if (!(is_block_edge || (is_tx_edge && coded))) break;
-    fs = filter_size;
-    lvl = l;
-  } while (0);
+  }
+  const int sb_size = sub == 0 ? m.mi_size : max((int)m.mi_size, (int)BLOCK_16X16);
+  const uint32_t block_mask = pass == 0 ? 8u * num8x8_wide(sb_size) - 1 : 8u * num8x8_high(sb_size) - 1;
+  const bool coded = m.ref_frame0 <= INTRA_FRAME || !m.skip;
+  x = at_x(ed, 2 * cb);
+  y = at_y(ed, 2 * cb);
+  if ((pass == 0 && x == 0) || (pass == 1 && y == 0)) return;
+  const uint32_t across = pass == 0 ? x : y;
+  const bool is_block_edge = (across & block_mask) == 0;
+  const bool tx_edge = (ed & ((1 << tx_sz) - 1)) == 0;
+  if (!is_block_edge && !(tx_edge && coded)) return;
+  const bool is_32_edge = ed % 8 == 0;
+  const int base_size = (tx_sz == TX_4X4 && is_32_edge) ? TX_8X8 : min(tx_sz, (int)TX_16X16);
+  const int filter_size =
+      ((pass == 0 && sub_x == 1 && base_size == TX_16X16 && (x >> 3) == mi_cols - 1) ||
+       (pass == 1 && sub_y == 1 && base_size == TX_16X16 && (y >> 3) == mi_rows - 1))
+          ? TX_8X8
+          : base_size;
+  const uint32_t xx = at_x(ed, rn), yy = at_y(ed, rn);
+  if (k == 1 && (xx >= 8 * mi_cols || yy >= 8 * mi_rows)) return;
+  const bool is_tx_edge =
+      (pass == 1 && sub_x == 1 && (mi_cols & 1) && (ed & 1) && (xx + 8) >= mi_cols * 8) ? false : tx_edge;
+  if (!(is_block_edge || (is_tx_edge && coded))) return;
+  fs_out = filter_size;
+  lvl_out = l;
+}
+
+__global__ __launch_bounds__(256) void k_lf_mask(LfArgs a) {
+  const FrameGeom &g = a.g;
+  const int sb = blockIdx.x, pass = blockIdx.y, plane = blockIdx.z;
+  const int n_sb = g.sb64_rows * g.sb64_cols;
+  int fs, lvl;
+  lf_decide(a, plane, pass, sb / g.sb64_cols, sb % g.sb64_cols, threadIdx.x >> 4, threadIdx.x & 15, fs, lvl);
   const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256 + threadIdx.x;
   a.mask_fs[o] = (int8_t)fs;
   a.mask_lvl[o] = (uint8_t)lvl;
 }
 
-// ---- one wavefront level
+// ---- one superblock of one plane
 
-template <bool HBD>
-__global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
-  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ uint16_t buf[72 * BP];
-  __shared__ int8_t fs[2][256];
-  __shared__ uint8_t lv[2][256];
+struct LfShared {
+  uint16_t buf[72 * BP];  // the superblock and its 8-sample halo above and left
+  int8_t fs[2][256];
+  uint8_t lv[2][256];
+};
+
+// Loads the region, filters the vertical then the horizontal edges and
+// stores the region back; the decisions are in sh.fs / sh.lv.
+template <class P>
+DEV void lf_superblock(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
   const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
-  const uint32_t code = a.sbs[first + blockIdx.x];
-  const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
   const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
   const int SW = 64 >> sx, SH = 64 >> sy;
   const int x0 = sb_col * SW, y0 = sb_row * SH;
   const uint32_t stride = a.cur.stride[plane];
   P *pl = (P *)a.cur.p[plane];
   const int RW = SW + 8, RH = SH + 8;
+  uint16_t *buf = sh.buf;
   for (int k = lane; k < RW * RH; k += 64) {
     int r = k / RW, c = k - r * RW;
     int py = y0 - 8 + r, px = x0 - 8 + c;
     if (py >= 0 && px >= 0) buf[r * BP + c] = pl[(size_t)py * stride + px];
-  }
-  const int n_sb = g.sb64_rows * g.sb64_cols, sb = sb_row * g.sb64_cols + sb_col;
-  for (int pass = 0; pass < 2; pass++) {
-    const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256;
-    for (int k = lane; k < 256; k += 64) {
-      fs[pass][k] = a.mask_fs[o + k];
-      lv[pass][k] = a.mask_lvl[o + k];
-    }
   }
   __syncthreads();
   const int bd = g.bit_depth;
   if (lane < SH) {
     const int r = 8 + lane, rn = lane >> 2, n_edges = 16 >> sx;
     for (int ed = 0; ed < n_edges; ed++) {
-      const int f = fs[0][ed * 16 + rn];
+      const int f = sh.fs[0][ed * 16 + rn];
       if (f >= 0) {
-        const int l = lv[0][ed * 16 + rn];
+        const int l = sh.lv[0][ed * 16 + rn];
         lf_filter(buf, r * BP + 8 + 4 * ed, 1, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], bd);
       }
     }
@@ -201,9 +216,9 @@ __global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
   if (lane < SW) {
     const int c = 8 + lane, rn = lane >> 2, n_edges = 16 >> sy;
     for (int ed = 0; ed < n_edges; ed++) {
-      const int f = fs[1][ed * 16 + rn];
+      const int f = sh.fs[1][ed * 16 + rn];
       if (f >= 0) {
-        const int l = lv[1][ed * 16 + rn];
+        const int l = sh.lv[1][ed * 16 + rn];
         lf_filter(buf, (8 + 4 * ed) * BP + c, BP, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], bd);
       }
     }
@@ -213,6 +228,94 @@ __global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
     int r = k / RW, c = k - r * RW;
     int py = y0 - 8 + r, px = x0 - 8 + c;
     if (py >= 0 && px >= 0) pl[(size_t)py * stride + px] = (P)buf[r * BP + c];
+  }
+}
+
+// ---- one wavefront level per launch (decisions from k_lf_mask)
+
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  __shared__ LfShared sh;
+  const FrameGeom &g = a.g;
+  const int lane = threadIdx.x;
+  const uint32_t code = a.sbs[first + blockIdx.x];
+  const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
+  const int n_sb = g.sb64_rows * g.sb64_cols, sb = sb_row * g.sb64_cols + sb_col;
+  for (int pass = 0; pass < 2; pass++) {
+    const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256;
+    for (int k = lane; k < 256; k += 64) {
+      sh.fs[pass][k] = a.mask_fs[o + k];
+      sh.lv[pass][k] = a.mask_lvl[o + k];
+    }
+  }
+  lf_superblock<P>(a, sh, plane, sb_row, sb_col);
+}
+
+// ---- the whole frame in one launch
+//
+// Workgroups take superblocks (all planes) in the order of a.sbs, which is
+// topological for the dependencies above (c + 2r ascending), through a
+// ticket counter; each computes its decisions, waits until the superblocks
+// it depends on carry this frame's epoch in their completion flags, filters,
+// and publishes its own flag.  A workgroup only ever waits on tickets taken
+// before its own, by workgroups already running, so every wait ends; a
+// bounded spin turns a wait that does not end into an error status instead
+// of a hang.
+
+DEV void wave_wait(const WaveSync &s, uint32_t idx) {
+  uint32_t spins = 0;
+  while (__hip_atomic_load(s.done + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != s.epoch) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kWaveSpinLimit) {
+      __hip_atomic_fetch_or(s.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+  }
+}
+
+DEV void wave_publish(const WaveSync &s, uint32_t idx) {
+  __threadfence();
+  __hip_atomic_store(s.done + idx, s.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+DEV uint32_t wave_ticket(const WaveSync &s, uint32_t *slot) {
+  if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(s.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const uint32_t t = *slot;
+  __syncthreads();
+  return t;
+}
+
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_lf_wave(LfArgs a) {
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  __shared__ LfShared sh;
+  __shared__ uint32_t slot;
+  const FrameGeom &g = a.g;
+  const int lane = threadIdx.x;
+  const uint32_t n_sb = g.sb64_rows * g.sb64_cols, cols = g.sb64_cols;
+  for (;;) {
+    const uint32_t t = wave_ticket(a.sync, &slot);
+    if (t >= a.n_items) return;
+    const uint32_t code = a.sbs[t];
+    const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
+    for (int k = lane; k < 512; k += 64) {
+      int fs, lvl;
+      lf_decide(a, plane, k >> 8, sb_row, sb_col, (k >> 4) & 15, k & 15, fs, lvl);
+      sh.fs[k >> 8][k & 255] = (int8_t)fs;
+      sh.lv[k >> 8][k & 255] = (uint8_t)lvl;
+    }
+    if (lane == 0) {
+      const uint32_t base = plane * n_sb;
+      if (sb_col > 0) wave_wait(a.sync, base + sb_row * cols + sb_col - 1);
+      if (sb_row > 0) wave_wait(a.sync, base + (sb_row - 1) * cols + min(sb_col + 1, (int)cols - 1));
+    }
+    __syncthreads();
+    __threadfence();
+    lf_superblock<P>(a, sh, plane, sb_row, sb_col);
+    __syncthreads();
+    if (lane == 0) wave_publish(a.sync, plane * n_sb + sb_row * cols + sb_col);
   }
 }
 
@@ -227,6 +330,15 @@ void launch_lf_level(hipStream_t s, const LfArgs &a, uint32_t first, uint32_t n)
     hipLaunchKernelGGL(k_lf<true>, dim3(n), dim3(64), 0, s, a, first);
   else
     hipLaunchKernelGGL(k_lf<false>, dim3(n), dim3(64), 0, s, a, first);
+}
+
+void launch_lf_wave(hipStream_t s, const LfArgs &a, uint32_t n_groups) {
+  if (!a.n_items) return;
+  const uint32_t n = min(a.n_items, n_groups);
+  if (a.g.bit_depth > 8)
+    hipLaunchKernelGGL(k_lf_wave<true>, dim3(n), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_lf_wave<false>, dim3(n), dim3(64), 0, s, a);
 }
 
 // Per-kernel seam: lf::filter_edges on global samples, one pass per

@@ -17,6 +17,7 @@
 #include "../gpu/mc_pos.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "vp9_consts.h"
@@ -205,6 +206,16 @@ static void build_intra(vp9r_gpu *g, const vp9r_record *rec) {
     uint32_t k = key_of(t);
     g->intra_index[g->intra_first[k] + g->intra_fill[k]++] = i;
   }
+  // the intra blocks in decode order, for the one-launch wavefront (the
+  // record keeps a block's transform blocks together, planes in order)
+  g->intra_items.clear();
+  for (uint32_t i = 0; i < rec->n_txbs; i++) {
+    const vp9r_txb &t = rec->txbs[i];
+    if (rec->blocks[t.block].is_inter) continue;
+    if (g->intra_items.empty() || g->intra_items.back().block != t.block)
+      g->intra_items.push_back(vp9g::IntraItem{t.block, i, 0});
+    g->intra_items.back().count++;
+  }
   // level = 1 + the deepest of the left, above and above-left groups
   g->intra_level.assign(3 * n_sb, -1);
   int max_level = -1;
@@ -363,6 +374,20 @@ hipError_t begin_timed_frame(vp9r_gpu *g) {
   return hipSuccess;
 }
 
+// The one-launch wavefronts' status word: nonzero when a wait gave up
+// (which leaves that frame wrong); read after the stream is idle.
+int check_wave_status(vp9r_gpu *g) {
+  if (!g->wave.p) return VP9R_OK;
+  uint32_t st = 0;
+  HIP_TRY(hipMemcpy(&st, (uint32_t *)g->wave.p + 4, 4, hipMemcpyDeviceToHost), "read wavefront status");
+  if (st) {
+    HIP_TRY(hipMemset((uint32_t *)g->wave.p + 4, 0, 4), "clear wavefront status");
+    set_error("a device wavefront wait exceeded its spin limit (status " + std::to_string(st) + ")");
+    return VP9R_ERR_DEVICE;
+  }
+  return VP9R_OK;
+}
+
 // Orders work on `s` that writes frame f after f's download in flight, if any.
 hipError_t order_after_copy(hipStream_t s, vp9r_gpu_frame *f) {
   if (!f->copy_pending) return hipSuccess;
@@ -386,6 +411,10 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
   vp9r_gpu *g = new vp9r_gpu();
   g->device = device;
+  // VP9R_WAVEFRONT=levels: one launch per wavefront level instead of the
+  // one-launch wavefronts
+  const char *wf = getenv("VP9R_WAVEFRONT");
+  g->wave_launch = !(wf && strcmp(wf, "levels") == 0);
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -507,8 +536,9 @@ int vp9r_gpu_frame_download_u16(vp9r_gpu_frame *f, int plane, uint16_t *dst) {
 }
 
 int vp9r_gpu_synchronize(vp9r_gpu *g) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
   HIP_TRY(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
-  return VP9R_OK;
+  return vp9r::check_wave_status(g);
 }
 
 void vp9r_gpu_set_timing(vp9r_gpu *g, int enable) {
@@ -605,7 +635,10 @@ static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stag
       const vp9r_block &b = rec->blocks[t.block];
       int sx = t.plane ? h.ss_x : 0, sy = t.plane ? h.ss_y : 0;
       uint32_t bx = (b.mi_col * 8u) >> sx, by = (b.mi_row * 8u) >> sy;
-      ok = t.x >= bx && t.y >= by && (!h.lossless || t.tx_size == 0);
+      const int psz = VP9_SS_SIZE_LOOKUP[b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size][sx][sy];
+      ok = psz != BLOCK_INVALID && t.x >= bx && t.y >= by &&
+           t.x + n0 <= bx + 4u * VP9_NUM_4X4_WIDE[psz] && t.y + n0 <= by + 4u * VP9_NUM_4X4_HIGH[psz] &&
+           (!h.lossless || t.tx_size == 0);
     }
     if (!ok) {
       set_error("malformed record transform block");
@@ -614,6 +647,23 @@ static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stag
     if (rec->blocks[t.block].is_inter && t.eob > 0) g->jobs[t.tx_size].push_back(i);
   }
   build_intra(g, rec);
+  {
+    // each intra block once, its planes in order, blocks in decode order
+    uint32_t prev_block = 0;
+    for (size_t k = 0; k < g->intra_items.size(); k++) {
+      const vp9g::IntraItem &it = g->intra_items[k];
+      if (k && it.block <= prev_block) {
+        set_error("malformed record: intra transform blocks out of decode order");
+        return VP9R_ERR_INVALID_INPUT;
+      }
+      prev_block = it.block;
+      for (uint32_t i = it.first + 1; i < it.first + it.count; i++)
+        if (rec->txbs[i].plane < rec->txbs[i - 1].plane) {
+          set_error("malformed record: transform block planes out of order");
+          return VP9R_ERR_INVALID_INPUT;
+        }
+    }
+  }
   work_bytes(rec, p);
   const bool do_lf = h.lf_level != 0;
   if (do_lf) build_lf_levels(g, h.sb64_rows, h.sb64_cols);
@@ -633,11 +683,14 @@ static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stag
   for (int k = 0; k < 4; k++) p.o_jobs[k] = take(sizeof(uint32_t) * g->jobs[k].size());
   p.o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
   p.o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
+  p.o_items = take(sizeof(vp9g::IntraItem) * g->intra_items.size());
   p.o_lf = take(sizeof(uint32_t) * (do_lf ? g->lf_codes.size() : 0));
   p.bytes = off;
   p.n_tiles = (uint32_t)g->tiles.size();
   for (int k = 0; k < 4; k++) p.n_jobs[k] = (uint32_t)g->jobs[k].size();
   p.n_groups = (uint32_t)g->groups.size();
+  p.n_items = (uint32_t)g->intra_items.size();
+  p.n_lf = do_lf ? (uint32_t)g->lf_codes.size() : 0;
   p.group_level_start = g->group_level_start;
   if (do_lf)
     p.lf_level_start = g->lf_level_start;
@@ -660,6 +713,7 @@ static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stag
   for (int k = 0; k < 4; k++) put(p.o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
   put(p.o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
   put(p.o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
+  put(p.o_items, g->intra_items.data(), sizeof(vp9g::IntraItem) * g->intra_items.size());
   if (do_lf) put(p.o_lf, g->lf_codes.data(), sizeof(uint32_t) * g->lf_codes.size());
   return VP9R_OK;
 }
@@ -690,11 +744,43 @@ static int check_frames(const Prepared &p, vp9r_gpu_frame *const refs[3], vp9r_g
   return VP9R_OK;
 }
 
+// Sizes the one-launch wavefront buffer for the frame (zeroing it when it
+// grows: zero is no frame's epoch), clears the tickets and advances the epoch.
+static int begin_wave_frame(vp9r_gpu *g, const Prepared &p, hipStream_t s) {
+  const size_t cells = (size_t)p.hdr.mi_rows * p.hdr.mi_cols;
+  const size_t lf_items = 3 * (size_t)p.hdr.sb64_rows * p.hdr.sb64_cols;
+  const size_t need = 4 * (8 + cells + lf_items);
+  if (need > g->wave.cap) {
+    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    HIP_TRY(g->wave.ensure(need + need / 4), "hipMalloc(wavefront flags)");
+    HIP_TRY(hipMemsetAsync(g->wave.p, 0, g->wave.cap, s), "hipMemset(wavefront flags)");
+  } else {
+    HIP_TRY(hipMemsetAsync(g->wave.p, 0, 16, s), "hipMemset(tickets)");
+  }
+  g->epoch = g->epoch + 1 ? g->epoch + 1 : 1;
+  return VP9R_OK;
+}
+
+static vp9g::WaveSync wave_sync(vp9r_gpu *g, int which) {
+  uint32_t *w = (uint32_t *)g->wave.p;
+  vp9g::WaveSync ws{};
+  ws.ticket = w + which;
+  ws.status = w + 4;
+  ws.done = which == 0 ? w + 8 : w + 8 + (size_t)g->wave_intra_cells;
+  ws.epoch = g->epoch;
+  return ws;
+}
+
 // Reconstruction launches from device inputs at dv (see the file comment).
 static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_gpu_frame *const refs[3],
                         vp9r_gpu_frame *cur) {
   hipStream_t s = g->stream;
   HIP_TRY(order_after_copy(s, cur), "hipStreamWaitEvent");
+  if (g->wave_launch) {
+    int st = begin_wave_frame(g, p, s);
+    if (st) return st;
+    g->wave_intra_cells = (size_t)p.hdr.mi_rows * p.hdr.mi_cols;
+  }
   if (p.n_tiles) {
     vp9g::McArgs a{};
     a.g = p.geo;
@@ -724,7 +810,21 @@ static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_
     }
   }
   if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
-  if (p.n_groups) {
+  if (p.n_items && g->wave_launch) {
+    vp9g::IntraArgs a{};
+    a.g = p.geo;
+    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
+    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
+    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
+    a.coef_val = (const int32_t *)(dv + p.o_cval);
+    a.cur = cur->planes;
+    memcpy(a.q, p.q, sizeof p.q);
+    a.mi = (const vp9r_mi *)(dv + p.o_mi);
+    a.items = (const vp9g::IntraItem *)(dv + p.o_items);
+    a.n_items = p.n_items;
+    a.sync = wave_sync(g, 0);
+    vp9g::launch_intra_wave(s, a, 1024);
+  } else if (p.n_groups) {
     vp9g::IntraArgs a{};
     a.g = p.geo;
     a.blocks = (const vp9r_block *)(dv + p.o_blocks);
@@ -748,7 +848,17 @@ static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_
 static int launch_lf(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_gpu_frame *cur) {
   hipStream_t s = g->stream;
   HIP_TRY(order_after_copy(s, cur), "hipStreamWaitEvent");
-  if (!p.lf_level_start.empty()) {
+  if (p.n_lf && g->wave_launch) {
+    vp9g::LfArgs a{};
+    a.g = p.geo;
+    a.mi = (const vp9r_mi *)(dv + p.o_mi);
+    a.cur = cur->planes;
+    a.p = p.lfp;
+    a.sbs = (const uint32_t *)(dv + p.o_lf);
+    a.n_items = p.n_lf;
+    a.sync = wave_sync(g, 1);
+    vp9g::launch_lf_wave(s, a, 128);
+  } else if (!p.lf_level_start.empty()) {
     const vp9r_frame_hdr &h = p.hdr;
     const size_t n_sb = (size_t)h.sb64_rows * h.sb64_cols;
     HIP_TRY(g->lf_mask.ensure(n_sb * 6 * 256 * 2), "hipMalloc(lf masks)");

@@ -39,8 +39,8 @@ struct Prepared {
   int32_t q[8][2][2];
   vp9g::LfParams lfp;
   size_t o_blocks = 0, o_txbs = 0, o_cpos = 0, o_cval = 0, o_mi = 0, o_tiles = 0, o_jobs[4] = {},
-         o_iidx = 0, o_groups = 0, o_lf = 0;
-  uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0;
+         o_iidx = 0, o_groups = 0, o_lf = 0, o_items = 0;
+  uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0, n_items = 0, n_lf = 0;
   std::vector<uint32_t> group_level_start, lf_level_start;
   size_t bytes = 0;
   // algorithmic HBM bytes of the frame per stage: inter prediction, inter
@@ -85,11 +85,18 @@ struct vp9r_gpu {
   std::vector<vp9g::McTile> tiles;
   std::vector<uint32_t> jobs[4];
   std::vector<uint32_t> intra_cnt, intra_first, intra_fill, intra_index;
+  std::vector<vp9g::IntraItem> intra_items;
   std::vector<int32_t> intra_level;
   std::vector<vp9g::IntraGroup> groups;
   std::vector<uint32_t> group_level_start;
   std::vector<uint32_t> lf_codes, lf_level_start;
   uint32_t lf_rows = 0, lf_cols = 0;
+  // one-launch wavefronts: [0..3] tickets, [4] status, then the intra flags
+  // (per 8x8 position) and the loop filter flags (per plane and superblock)
+  bool wave_launch = true;
+  vp9r::DevBuf wave;
+  size_t wave_intra_cells = 0, wave_lf_items = 0;
+  uint32_t epoch = 0;
   // per-kernel seam scratch
   vp9r::DevBuf seam_a, seam_b, seam_c;
 };

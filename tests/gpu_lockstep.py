@@ -100,3 +100,18 @@ def run_vector(path, gpu):
                 return matched, len(exp), f"frame {shown}: md5 {gm} != expected"
             shown += 1
     return matched, len(exp), None
+
+
+def run_frames(frames, gpu):
+    """Lockstep over a list of frames (no published MD5s): returns
+    (frames compared, first error message or None)."""
+    ls = Lockstep(gpu)
+    n = 0
+    for i, f in enumerate(frames):
+        g, o, err = ls.decode_frame(f)
+        if err is not None:
+            return n, f"frame {i}: {err}"
+        if g is not None and oracle_lib.md5(g.packed()) != oracle_lib.md5(o.packed()):
+            return n, f"frame {i}: packed frame differs from the oracle's"
+        n += 1
+    return n, None

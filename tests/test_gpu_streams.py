@@ -1,0 +1,115 @@
+"""GPU parity on generated streams: features the committed streams do not
+reach (several tile columns, scaled references, 4:2:2 / 4:4:0 / 4:4:4 at
+every bit depth, the 1080p benchmark workload), every frame compared with
+the CPU oracle after reconstruction and after the loop filter; and the two
+wavefront schedules (one launch per frame, one launch per level) against
+each other."""
+import os
+
+import pytest
+
+import gpu_lockstep
+import oracle_lib
+import rivet_vp9_amd as vp
+import streamgen
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "sinusoid-352x288": dict(width=352, height=288, frames=6, seed=1, base_q_idx=120, content=0),
+    "rects-640x360-2tiles": dict(width=640, height=360, frames=5, seed=2, tile_cols_log2=1, content=1,
+                                 intra_percent=10),
+    "resize-200x120": dict(width=200, height=120, frames=8, seed=3, resize_every=3),
+    "10bit-420": dict(width=160, height=96, frames=4, seed=5, bit_depth=10),
+    "12bit-444": dict(width=160, height=96, frames=4, seed=5, bit_depth=12, ss_x=0, ss_y=0),
+    "8bit-422": dict(width=160, height=96, frames=4, seed=6, ss_x=1, ss_y=0),
+    "10bit-440": dict(width=160, height=96, frames=4, seed=7, bit_depth=10, ss_x=0, ss_y=1),
+    "lossless": dict(width=96, height=64, frames=3, seed=8, base_q_idx=0),
+    "1080p-workload": dict(streamgen.CONFIG2, frames=3),
+}
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    return vp.Gpu(0)
+
+
+@pytest.fixture(scope="module")
+def gpu_levels():
+    old = os.environ.get("VP9R_WAVEFRONT")
+    os.environ["VP9R_WAVEFRONT"] = "levels"
+    try:
+        g = vp.Gpu(0)
+    finally:
+        if old is None:
+            del os.environ["VP9R_WAVEFRONT"]
+        else:
+            os.environ["VP9R_WAVEFRONT"] = old
+    return g
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_stream_lockstep(gpu, name):
+    print(f"stream {name}", flush=True)
+    frames = streamgen.generate(**CASES[name])
+    n, err = gpu_lockstep.run_frames(frames, gpu)
+    assert err is None, f"{name}: {n} frames equal, then {err}"
+    assert n == len(frames)
+
+
+@pytest.mark.parametrize("name", ["rects-640x360-2tiles", "resize-200x120", "12bit-444"])
+def test_stream_lockstep_level_launches(gpu_levels, name):
+    print(f"stream {name} (level launches)", flush=True)
+    frames = streamgen.generate(**CASES[name])
+    n, err = gpu_lockstep.run_frames(frames, gpu_levels)
+    assert err is None, f"{name}: {n} frames equal, then {err}"
+
+
+def test_prepared_run_equals_reconstruct(gpu):
+    """vp9r_gpu_prepare + vp9r_gpu_run (the benchmark's path) gives the same
+    frames as vp9r_gpu_reconstruct + vp9r_gpu_loop_filter, run twice."""
+    frames = streamgen.generate(**dict(CASES["rects-640x360-2tiles"], frames=4))
+    p = vp.Parser()
+    recs = []
+    for f in frames:
+        rec = p.parse(f)
+        recs.append(gpu.prepare(rec))
+    ref = oracle_lib.CpuDecoder()
+    want = [oracle_lib.md5(o.packed()) for f in frames for o in ref.decode(f)]
+    for attempt in range(2):
+        slots = [None] * 8
+        got = []
+        for pr in recs:
+            h = pr.hdr
+            cur = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+            refs = [None] * 3 if h.frame_is_intra else [slots[h.ref_frame_idx[k]] for k in range(3)]
+            gpu.run(pr, refs, cur)
+            for k in range(8):
+                if (h.refresh_frame_flags >> k) & 1:
+                    slots[k] = cur
+            if h.show_frame:
+                got.append(oracle_lib.md5(cur.packed()))
+        gpu.synchronize()
+        assert got == want, f"pass {attempt}"
+
+
+def test_async_download_matches(gpu):
+    """vp9r_gpu_download_packed_async into pinned memory equals the
+    synchronous download, with the frame store rewritten afterwards."""
+    frames = streamgen.generate(**dict(CASES["sinusoid-352x288"], frames=3))
+    p = vp.Parser()
+    preps = [gpu.prepare(p.parse(f)) for f in frames]
+    h = preps[0].hdr
+    size = vp.library().vp9r_packed_size(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+    host = vp.HostBuffer(size)
+    a = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+    b = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+    gpu.run(preps[0], [None] * 3, a)
+    want = a.packed()
+    a.packed_async(host)
+    # frame 1 predicts from a into b; then a is rewritten by frame 0 again
+    gpu.run(preps[1], [a, a, a], b)
+    gpu.run(preps[0], [None] * 3, a)
+    a.wait()
+    assert host.bytes(size) == want
+    gpu.synchronize()
