@@ -179,7 +179,7 @@ int vp9r_gpu_prepare(vp9r_gpu *g, const vp9r_record *rec, vp9r_prepared **out);
 void vp9r_prepared_free(vp9r_prepared *p);
 int vp9r_gpu_run(vp9r_gpu *g, const vp9r_prepared *p, vp9r_gpu_frame *const refs[3],
                  vp9r_gpu_frame *cur);
-/* Work of a prepared record: [0] inter prediction tiles (8x8), [1..4]
+/* Work of a prepared record: [0] inter prediction tiles (<= 16x16), [1..4]
  * inter transform blocks of 4x4 .. 32x32, [5] intra groups (superblock,
  * plane), [6] intra wavefront launches, [7] loop filter wavefront launches,
  * [8..11] algorithmic HBM bytes of inter prediction (reference footprints
@@ -236,7 +236,7 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
  * plane pointers, all with row stride `ref_stride` and bottom-right sample
  * (last_x, last_y)), and, when ref2_idx >= 0, averages with a second
  * prediction at (x2, y2).  Writes to out + out_off[i] samples with stride
- * out_stride.  `desc` is a host array (the wrapper expands it into 8 x 8
+ * out_stride.  `desc` is a host array (the wrapper expands it into 16 x 16
  * tiles); `refs` is a host array of device pointers; `out` is a device
  * pointer. */
 typedef struct vp9r_mc_desc {

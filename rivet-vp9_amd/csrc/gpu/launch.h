@@ -23,7 +23,7 @@ struct FrameGeom {
   uint8_t lossless, _pad[3];
 };
 
-// An inter prediction tile: a tw x th piece (at most 8 x 8) at offset
+// An inter prediction tile: a tw x th piece (at most 16 x 16) at offset
 // (tx, ty) of one prediction unit (a block's plane, or one 4x4 sub-block)
 // of size w x h at plane position (x, y).
 struct McTile {

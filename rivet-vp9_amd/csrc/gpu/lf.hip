@@ -182,53 +182,92 @@ struct LfShared {
   uint8_t lv[2][256];
 };
 
-// Loads the region, filters the vertical then the horizontal edges and
-// stores the region back; the decisions are in sh.fs / sh.lv.
+// The steps on one superblock of one plane: the region is the superblock
+// with its 8-sample halo above and left, staged in LDS (rows [r0, r1) of
+// it by lf_load); the decisions are in sh.fs / sh.lv.
+struct LfRegion {
+  int x0, y0, SW, SH, RW, RH;
+  uint32_t stride;
+};
+
+DEV LfRegion lf_region(const LfArgs &a, int plane, int sb_row, int sb_col) {
+  const int sx = plane ? a.g.ss_x : 0, sy = plane ? a.g.ss_y : 0;
+  LfRegion g;
+  g.SW = 64 >> sx;
+  g.SH = 64 >> sy;
+  g.x0 = sb_col * g.SW;
+  g.y0 = sb_row * g.SH;
+  g.RW = g.SW + 8;
+  g.RH = g.SH + 8;
+  g.stride = a.cur.stride[plane];
+  return g;
+}
+
 template <class P>
-DEV void lf_superblock(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
-  const FrameGeom &g = a.g;
-  const int lane = threadIdx.x;
-  const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
-  const int SW = 64 >> sx, SH = 64 >> sy;
-  const int x0 = sb_col * SW, y0 = sb_row * SH;
-  const uint32_t stride = a.cur.stride[plane];
-  P *pl = (P *)a.cur.p[plane];
-  const int RW = SW + 8, RH = SH + 8;
-  uint16_t *buf = sh.buf;
-  for (int k = lane; k < RW * RH; k += 64) {
-    int r = k / RW, c = k - r * RW;
-    int py = y0 - 8 + r, px = x0 - 8 + c;
-    if (py >= 0 && px >= 0) buf[r * BP + c] = pl[(size_t)py * stride + px];
+DEV void lf_load(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g, int r0, int r1) {
+  const P *pl = (const P *)a.cur.p[plane];
+  const int n = (r1 - r0) * g.RW;
+  for (int k = threadIdx.x; k < n; k += 64) {
+    int r = r0 + k / g.RW, c = k - (k / g.RW) * g.RW;
+    int py = g.y0 - 8 + r, px = g.x0 - 8 + c;
+    if (py >= 0 && px >= 0) sh.buf[r * BP + c] = pl[(size_t)py * g.stride + px];
   }
-  __syncthreads();
-  const int bd = g.bit_depth;
-  if (lane < SH) {
+}
+
+template <class P>
+DEV void lf_store(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g) {
+  P *pl = (P *)a.cur.p[plane];
+  for (int k = threadIdx.x; k < g.RW * g.RH; k += 64) {
+    int r = k / g.RW, c = k - r * g.RW;
+    int py = g.y0 - 8 + r, px = g.x0 - 8 + c;
+    if (py >= 0 && px >= 0) pl[(size_t)py * g.stride + px] = (P)sh.buf[r * BP + c];
+  }
+}
+
+// Vertical edges, left to right, one lane per row (touches rows 8 .. of the
+// region only).
+DEV void lf_vpass(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g) {
+  const int lane = threadIdx.x;
+  const int sx = plane ? a.g.ss_x : 0;
+  if (lane < g.SH) {
     const int r = 8 + lane, rn = lane >> 2, n_edges = 16 >> sx;
     for (int ed = 0; ed < n_edges; ed++) {
       const int f = sh.fs[0][ed * 16 + rn];
       if (f >= 0) {
         const int l = sh.lv[0][ed * 16 + rn];
-        lf_filter(buf, r * BP + 8 + 4 * ed, 1, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], bd);
+        lf_filter(sh.buf, r * BP + 8 + 4 * ed, 1, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], a.g.bit_depth);
       }
     }
   }
-  __syncthreads();
-  if (lane < SW) {
+}
+
+// Horizontal edges, top to bottom, one lane per column.
+DEV void lf_hpass(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g) {
+  const int lane = threadIdx.x;
+  const int sy = plane ? a.g.ss_y : 0;
+  if (lane < g.SW) {
     const int c = 8 + lane, rn = lane >> 2, n_edges = 16 >> sy;
     for (int ed = 0; ed < n_edges; ed++) {
       const int f = sh.fs[1][ed * 16 + rn];
       if (f >= 0) {
         const int l = sh.lv[1][ed * 16 + rn];
-        lf_filter(buf, (8 + 4 * ed) * BP + c, BP, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], bd);
+        lf_filter(sh.buf, (8 + 4 * ed) * BP + c, BP, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l],
+                  a.g.bit_depth);
       }
     }
   }
+}
+
+template <class P>
+DEV void lf_superblock(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
+  const LfRegion g = lf_region(a, plane, sb_row, sb_col);
+  lf_load<P>(a, sh, plane, g, 0, g.RH);
   __syncthreads();
-  for (int k = lane; k < RW * RH; k += 64) {
-    int r = k / RW, c = k - r * RW;
-    int py = y0 - 8 + r, px = x0 - 8 + c;
-    if (py >= 0 && px >= 0) pl[(size_t)py * stride + px] = (P)buf[r * BP + c];
-  }
+  lf_vpass(a, sh, plane, g);
+  __syncthreads();
+  lf_hpass(a, sh, plane, g);
+  __syncthreads();
+  lf_store<P>(a, sh, plane, g);
 }
 
 // ---- one wavefront level per launch (decisions from k_lf_mask)
@@ -306,16 +345,28 @@ __global__ __launch_bounds__(64) void k_lf_wave(LfArgs a) {
       sh.fs[k >> 8][k & 255] = (int8_t)fs;
       sh.lv[k >> 8][k & 255] = (uint8_t)lvl;
     }
-    if (lane == 0) {
-      const uint32_t base = plane * n_sb;
-      if (sb_col > 0) wave_wait(a.sync, base + sb_row * cols + sb_col - 1);
-      if (sb_row > 0) wave_wait(a.sync, base + (sb_row - 1) * cols + min(sb_col + 1, (int)cols - 1));
-    }
+    // The vertical pass needs (r, c - 1) (and so everything up and left of
+    // it); the horizontal pass also needs (r - 1, c + 1), whose vertical
+    // edges reach 8 columns into the top halo: the halo rows are loaded
+    // again after that wait.
+    const uint32_t base = plane * n_sb;
+    if (lane == 0 && sb_col > 0) wave_wait(a.sync, base + sb_row * cols + sb_col - 1);
     __syncthreads();
     __threadfence();
-    lf_superblock<P>(a, sh, plane, sb_row, sb_col);
+    const LfRegion rg = lf_region(a, plane, sb_row, sb_col);
+    lf_load<P>(a, sh, plane, rg, 8, rg.RH);
     __syncthreads();
-    if (lane == 0) wave_publish(a.sync, plane * n_sb + sb_row * cols + sb_col);
+    lf_vpass(a, sh, plane, rg);
+    if (lane == 0 && sb_row > 0) wave_wait(a.sync, base + (sb_row - 1) * cols + min(sb_col + 1, (int)cols - 1));
+    __syncthreads();
+    __threadfence();
+    lf_load<P>(a, sh, plane, rg, 0, 8);
+    __syncthreads();
+    lf_hpass(a, sh, plane, rg);
+    __syncthreads();
+    lf_store<P>(a, sh, plane, rg);
+    __syncthreads();
+    if (lane == 0) wave_publish(a.sync, base + sb_row * cols + sb_col);
   }
 }
 

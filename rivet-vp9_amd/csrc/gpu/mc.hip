@@ -3,11 +3,11 @@
 // 8-tap filter with per-pass rounding and clamping (src/dsp/inter.rs:289-395)
 // and compound averaging (src/dsp/pixel.rs:22-50).
 //
-// One 64-lane workgroup predicts one tile of at most 8 x 8 samples of a
-// prediction unit: its reference footprint (up to 22 x 22 for a 2:1 scaled
+// One 64-lane workgroup predicts one tile of at most 16 x 16 samples of a
+// prediction unit: its reference footprint (up to 38 x 38 for a 2:1 scaled
 // reference) is staged in LDS with the reference's edge samples repeated
 // (the specification's clamped coordinates), the horizontal pass writes an
-// LDS intermediate, the vertical pass gives each lane its sample.  One code
+// LDS intermediate, the vertical pass gives each lane up to 4 samples.  One code
 // path serves every step: the specification's formula, of which the
 // reference's unscaled fast path is a special case (its own test,
 // inter.rs:486-564, checks they agree).
@@ -34,68 +34,104 @@ static void upload_filters() {
   filters_ready = true;
 }
 
-constexpr int FP_W = 24;  // footprint row pitch in LDS (>= 22)
-constexpr int FP_H = 24;
+// A tile is at most 16 x 16 samples; its footprint at most 38 x 38 (a 2:1
+// step: ((16 - 1) * 32 + 15) / 16 + 8).
+constexpr int TILE = 16;
+constexpr int FPW = 40;  // footprint row pitch in LDS
+constexpr int FPH = 38;
 
-// Predicts the 8 x 8 (or smaller) tile whose first sample is at (x0, y0)
-// (1/16 sample) with steps (xs, ys) from `ref`; returns lane (r, c)'s sample.
+struct McShared {
+  uint16_t fp[FPH * FPW];  // reference footprint, edge samples repeated
+  int16_t hp[FPH * TILE];  // after the horizontal pass
+};
+
+// Predicts the tw x th (<= 16 x 16) tile whose first sample is at (x0, y0)
+// (1/16 sample) with steps (xs, ys) from `ref`, whose bottom-right sample is
+// (last_x, last_y).  Output o = lane + 64 i (row o / tw, column o % tw) goes
+// to out[i].  The specification's formula (8.5.2.3): each pass rounds by 7
+// and clamps; a pass whose positions are all whole samples (step 16, phase
+// 0) is the identity and is skipped, with its 7 extra footprint samples.
 // All 64 lanes call it (it holds barriers).
 template <class P>
-DEV int mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int xs, int ys,
-                int th, int filter, int max, uint16_t *fp, int16_t *hp) {
+DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int xs, int ys,
+                 int tw, int th, int filter, int max, McShared &sh, int out[4]) {
   const int lane = threadIdx.x;
-  const int fx0 = (x0 >> 4) - 3, fy0 = (y0 >> 4) - 3;
-  const int ih = ((((th - 1) * ys + 15) >> 4) + 8);
-  for (int k = lane; k < ih * FP_W; k += 64) {
-    int rr = k / FP_W, cc = k - rr * FP_W;
-    int sy = iclamp(fy0 + rr, 0, last_y), sx = iclamp(fx0 + cc, 0, last_x);
-    fp[k] = ref[(size_t)sy * stride + sx];
+  const bool fx = xs != 16 || (x0 & 15), fy = ys != 16 || (y0 & 15);
+  const int fx0 = (x0 >> 4) - (fx ? 3 : 0), fy0 = (y0 >> 4) - (fy ? 3 : 0);
+  const int iw = fx ? ((((tw - 1) * xs + (x0 & 15)) >> 4) + 8) : tw;
+  const int ih = fy ? ((((th - 1) * ys + (y0 & 15)) >> 4) + 8) : th;
+  for (int k = lane; k < ih * iw; k += 64) {
+    const int rr = k / iw, cc = k - rr * iw;
+    const int sy = iclamp(fy0 + rr, 0, last_y), sx = iclamp(fx0 + cc, 0, last_x);
+    sh.fp[rr * FPW + cc] = ref[(size_t)sy * stride + sx];
   }
   __syncthreads();
-  for (int k = lane; k < ih * 8; k += 64) {
-    int rr = k >> 3, cc = k & 7;
-    int p = x0 + xs * cc;
-    const int16_t *f = c_filters[filter][p & 15];
-    const uint16_t *s = fp + rr * FP_W + (p >> 4) - 3 - fx0;
-    int sum = 0;
+  for (int k = lane; k < ih * tw; k += 64) {
+    const int rr = k / tw, cc = k - rr * tw;
+    int v;
+    if (fx) {
+      const int p = x0 + xs * cc;
+      const int16_t *f = c_filters[filter][p & 15];
+      const uint16_t *src = sh.fp + rr * FPW + (p >> 4) - 3 - fx0;
+      int sum = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) sum += f[i] * (int)s[i];
-    hp[k] = (int16_t)iclamp((sum + 64) >> 7, 0, max);
+      for (int i = 0; i < 8; i++) sum += f[i] * (int)src[i];
+      v = iclamp((sum + 64) >> 7, 0, max);
+    } else {
+      v = sh.fp[rr * FPW + cc];
+    }
+    sh.hp[rr * TILE + cc] = (int16_t)v;
   }
   __syncthreads();
-  const int r = lane >> 3, c = lane & 7;
-  int p = (y0 & 15) + ys * r;
-  const int16_t *f = c_filters[filter][p & 15];
-  const int16_t *s = hp + (p >> 4) * 8 + c;
-  int sum = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) sum += f[i] * (int)s[i * 8];
+  for (int i = 0; i < 4; i++) {
+    const int o = lane + 64 * i;
+    int v = 0;
+    if (o < th * tw) {
+      const int r = o / tw, c = o - r * tw;
+      if (fy) {
+        const int p = (y0 & 15) + ys * r;
+        const int16_t *f = c_filters[filter][p & 15];
+        const int16_t *src = sh.hp + (p >> 4) * TILE + c;
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) sum += f[k] * (int)src[k * TILE];
+        v = iclamp((sum + 64) >> 7, 0, max);
+      } else {
+        v = sh.hp[r * TILE + c];
+      }
+    }
+    out[i] = v;
+  }
   __syncthreads();
-  return iclamp((sum + 64) >> 7, 0, max);
 }
 
 template <bool HBD>
 __global__ __launch_bounds__(64) void k_mc(McArgs a) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ uint16_t fp[FP_W * FP_H];
-  __shared__ int16_t hp[FP_H * 8 + 8 * 8];
+  __shared__ McShared sh;
   const McTile t = a.tiles[blockIdx.x];
   const vp9r_block &b = a.blocks[t.block];
   const int max = (1 << a.g.bit_depth) - 1;
   const bool compound = b.ref_frame[1] > INTRA_FRAME;
-  int out[2] = {0, 0};
+  int out[2][4];
   for (int rl = 0; rl < 1 + (int)compound; rl++) {
     RefPos rp = mc_position(a.g, a.ref_w, a.ref_h, b, t.plane, t.x, t.y, t.sub, rl);
     const int ri = b.ref_frame[rl] - LAST_FRAME;
     const P *ref = (const P *)a.ref[ri].p[t.plane];
-    out[rl] = mc_tile<P>(ref, a.ref[ri].stride[t.plane], rp.last_x, rp.last_y, rp.x + rp.xs * t.tx,
-                         rp.y + rp.ys * t.ty, rp.xs, rp.ys, t.th, b.interp_filter, max, fp, hp);
+    mc_tile<P>(ref, a.ref[ri].stride[t.plane], rp.last_x, rp.last_y, rp.x + rp.xs * t.tx, rp.y + rp.ys * t.ty,
+               rp.xs, rp.ys, t.tw, t.th, b.interp_filter, max, sh, out[rl]);
   }
-  const int r = threadIdx.x >> 3, c = threadIdx.x & 7;
-  if (r < t.th && c < t.tw) {
-    int v = compound ? (out[0] + out[1] + 1) >> 1 : out[0];
-    P *dst = (P *)a.cur.p[t.plane];
-    dst[(size_t)(t.y + t.ty + r) * a.cur.stride[t.plane] + t.x + t.tx + c] = (P)v;
+  P *dst = (P *)a.cur.p[t.plane];
+  const uint32_t stride = a.cur.stride[t.plane];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int o = threadIdx.x + 64 * i;
+    if (o < t.th * t.tw) {
+      const int r = o / t.tw, c = o - r * t.tw;
+      const int v = compound ? (out[0][i] + out[1][i] + 1) >> 1 : out[0][i];
+      dst[(size_t)(t.y + t.ty + r) * stride + t.x + t.tx + c] = (P)v;
+    }
   }
 }
 
@@ -112,22 +148,27 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
                                                 int32_t last_x, int32_t last_y, void *out,
                                                 uint32_t out_stride, int bd) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ uint16_t fp[FP_W * FP_H];
-  __shared__ int16_t hp[FP_H * 8 + 8 * 8];
+  __shared__ McShared sh;
   const DescTile t = tiles[blockIdx.x];
   const vp9r_mc_desc d = descs[t.desc];
   const int max = (1 << bd) - 1;
-  const int th = d.h - t.ty < 8 ? d.h - t.ty : 8, tw = d.w - t.tx < 8 ? d.w - t.tx : 8;
-  int o0 = mc_tile<P>((const P *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + d.xs * t.tx,
-                      d.y + d.ys * t.ty, d.xs, d.ys, th, d.filter, max, fp, hp);
-  int v = o0;
-  if (d.ref2_idx >= 0) {
-    int o1 = mc_tile<P>((const P *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + d.xs * t.tx,
-                        d.y2 + d.ys * t.ty, d.xs, d.ys, th, d.filter, max, fp, hp);
-    v = (o0 + o1 + 1) >> 1;
+  const int th = d.h - t.ty < TILE ? d.h - t.ty : TILE, tw = d.w - t.tx < TILE ? d.w - t.tx : TILE;
+  int o0[4], o1[4];
+  mc_tile<P>((const P *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + d.xs * t.tx, d.y + d.ys * t.ty,
+             d.xs, d.ys, tw, th, d.filter, max, sh, o0);
+  const bool compound = d.ref2_idx >= 0;
+  if (compound)
+    mc_tile<P>((const P *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + d.xs * t.tx,
+               d.y2 + d.ys * t.ty, d.xs, d.ys, tw, th, d.filter, max, sh, o1);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int o = threadIdx.x + 64 * i;
+    if (o < th * tw) {
+      const int r = o / tw, c = o - r * tw;
+      const int v = compound ? (o0[i] + o1[i] + 1) >> 1 : o0[i];
+      ((P *)out)[d.out_off + (size_t)(t.ty + r) * out_stride + t.tx + c] = (P)v;
+    }
   }
-  const int r = threadIdx.x >> 3, c = threadIdx.x & 7;
-  if (r < th && c < tw) ((P *)out)[d.out_off + (size_t)(t.ty + r) * out_stride + t.tx + c] = (P)v;
 }
 
 void launch_mc(hipStream_t s, const McArgs &a, uint32_t n_tiles) {

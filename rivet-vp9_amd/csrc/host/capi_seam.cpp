@@ -111,8 +111,8 @@ int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const v
       set_error("motion compensation descriptor out of range");
       return VP9R_ERR_INVALID_INPUT;
     }
-    for (int ty = 0; ty < d.h; ty += 8)
-      for (int tx = 0; tx < d.w; tx += 8) tiles.push_back({i, (uint16_t)tx, (uint16_t)ty});
+    for (int ty = 0; ty < d.h; ty += 16)
+      for (int tx = 0; tx < d.w; tx += 16) tiles.push_back({i, (uint16_t)tx, (uint16_t)ty});
   }
   TRY(hipSetDevice(g->device), "hipSetDevice");
   size_t dbytes = sizeof(vp9r_mc_desc) * count, tbytes = sizeof(DescTile) * tiles.size();

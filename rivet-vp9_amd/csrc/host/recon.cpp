@@ -133,7 +133,7 @@ static vp9g::FrameGeom geom_of(const vp9r_frame_hdr &h) {
 
 // The inter prediction tiles of the record (block.rs:1425-1448): each
 // block's planes as one prediction unit, or one 4x4 unit per sub-block
-// below 8x8; units split into tiles of at most 8 x 8.
+// below 8x8; units split into tiles of at most 16 x 16.
 static void build_mc_tiles(vp9r_gpu *g, const vp9r_record *rec) {
   const vp9r_frame_hdr &h = rec->hdr;
   auto &tiles = g->tiles;
@@ -148,8 +148,8 @@ static void build_mc_tiles(vp9r_gpu *g, const vp9r_record *rec) {
       int n4w = VP9_NUM_4X4_WIDE[psz], n4h = VP9_NUM_4X4_HIGH[psz];
       int base_x = (b.mi_col * 8) >> sx, base_y = (b.mi_row * 8) >> sy;
       auto unit = [&](int x, int y, int w, int hh, int sub) {
-        for (int ty = 0; ty < hh; ty += 8)
-          for (int tx = 0; tx < w; tx += 8) {
+        for (int ty = 0; ty < hh; ty += 16)
+          for (int tx = 0; tx < w; tx += 16) {
             vp9g::McTile t;
             t.block = bi;
             t.x = (uint16_t)x;
@@ -160,8 +160,8 @@ static void build_mc_tiles(vp9r_gpu *g, const vp9r_record *rec) {
             t.h = (uint8_t)hh;
             t.tx = (uint8_t)tx;
             t.ty = (uint8_t)ty;
-            t.tw = (uint8_t)std::min(8, w - tx);
-            t.th = (uint8_t)std::min(8, hh - ty);
+            t.tw = (uint8_t)std::min(16, w - tx);
+            t.th = (uint8_t)std::min(16, hh - ty);
             tiles.push_back(t);
           }
       };
@@ -411,10 +411,10 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
   vp9r_gpu *g = new vp9r_gpu();
   g->device = device;
-  // VP9R_WAVEFRONT=levels: one launch per wavefront level instead of the
-  // one-launch wavefronts
+  // VP9R_WAVEFRONT=one: the one-launch wavefronts (device-side waits)
+  // instead of one launch per wavefront level
   const char *wf = getenv("VP9R_WAVEFRONT");
-  g->wave_launch = !(wf && strcmp(wf, "levels") == 0);
+  g->wave_launch = wf && strcmp(wf, "one") == 0;
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {

@@ -93,7 +93,7 @@ struct vp9r_gpu {
   uint32_t lf_rows = 0, lf_cols = 0;
   // one-launch wavefronts: [0..3] tickets, [4] status, then the intra flags
   // (per 8x8 position) and the loop filter flags (per plane and superblock)
-  bool wave_launch = true;
+  bool wave_launch = false;
   vp9r::DevBuf wave;
   size_t wave_intra_cells = 0, wave_lf_items = 0;
   uint32_t epoch = 0;

@@ -56,10 +56,12 @@ def generate(width, height, frames, seed, bit_depth=8, ss_x=1, ss_y=1, base_q_id
     return out
 
 
-# The benchmark configurations (sizes, formats, seeds, quantisers, tiles).
+# The benchmark configurations (sizes, formats, seeds, quantisers, tiles);
+# like the crate's encoder, no high-precision motion vectors
+# (src/encoder/mod.rs:668, 909).
 CONFIG1 = dict(width=352, height=288, frames=30, seed=1, base_q_idx=120, golden_interval=8, tile_cols_log2=0,
-               content=0, intra_percent=1)
+               content=0, intra_percent=1, allow_hp=0)
 CONFIG2 = dict(width=1920, height=1080, frames=60, seed=2, base_q_idx=100, golden_interval=8, tile_cols_log2=2,
-               content=1, intra_percent=3)
+               content=1, intra_percent=3, allow_hp=0)
 CONFIG4 = dict(width=3840, height=2160, frames=48, seed=4, bit_depth=10, base_q_idx=80, golden_interval=8,
-               tile_cols_log2=2, content=1, resize_every=12, intra_percent=3)
+               tile_cols_log2=2, content=1, resize_every=12, intra_percent=3, allow_hp=0)
