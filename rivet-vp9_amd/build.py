@@ -27,6 +27,7 @@ HIP_SOURCES = [
     "csrc/gpu/mc.hip",
     "csrc/gpu/intra.hip",
     "csrc/gpu/lf.hip",
+    "csrc/gpu/frame.hip",
     "csrc/host/recon.cpp",
     "csrc/host/capi_seam.cpp",
     "csrc/host/decoder.cpp",

@@ -118,6 +118,7 @@ struct LfArgs {
   WaveSync sync;        // one-launch wavefront: flags per (plane, superblock)
 };
 
+void launch_clear_margins(hipStream_t s, const Planes &f, const FrameGeom &g);
 void launch_mc(hipStream_t s, const McArgs &a, uint32_t n_tiles);
 void launch_itx_inter(hipStream_t s, const ItxArgs &a, int tx_size, uint32_t n_jobs);
 void launch_intra(hipStream_t s, const IntraArgs &a, uint32_t first_group, uint32_t n_groups);

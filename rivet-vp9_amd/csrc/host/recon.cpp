@@ -781,6 +781,7 @@ static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_
     if (st) return st;
     g->wave_intra_cells = (size_t)p.hdr.mi_rows * p.hdr.mi_cols;
   }
+  vp9g::launch_clear_margins(s, cur->planes, p.geo);
   if (p.n_tiles) {
     vp9g::McArgs a{};
     a.g = p.geo;

@@ -25,6 +25,8 @@ CASES = {
     "10bit-440": dict(width=160, height=96, frames=4, seed=7, bit_depth=10, ss_x=0, ss_y=1),
     "lossless": dict(width=96, height=64, frames=3, seed=8, base_q_idx=0),
     "1080p-workload": dict(streamgen.CONFIG2, frames=3),
+    # the 4K 10-bit workload, resized every other frame: 2:1 and 1:2 scaled references
+    "4k-10bit-scaled": dict(streamgen.CONFIG4, frames=4, resize_every=2),
 }
 
 
@@ -114,3 +116,50 @@ def test_async_download_matches(gpu):
     a.wait()
     assert host.bytes(size) == want
     gpu.synchronize()
+
+
+def test_reused_frame_stores_match(gpu):
+    """Frame stores reused across frames (and filled with noise first) give
+    the oracle's frames: reconstruction must not depend on what a store
+    held before.  1080p 4:2:0 has an odd number of 8x8 rows, whose last
+    chroma 4x4 edge reads past the 8x8-aligned area."""
+    import numpy as np
+    frames = streamgen.generate(**dict(streamgen.CONFIG2, frames=6))
+    ref = oracle_lib.CpuDecoder()
+    want = [oracle_lib.md5(o.packed()) for f in frames for o in ref.decode(f)]
+    p = vp.Parser()
+    preps = [gpu.prepare(p.parse(f)) for f in frames]
+    h = preps[0].hdr
+    rng = np.random.default_rng(9)
+    pool = [gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y) for _ in range(3)]
+    for f in pool:
+        for pl in range(3):
+            s, r = f.geometry(pl)
+            f.upload_plane(pl, rng.integers(0, 256, (r, s)))
+    slots = [None] * 8
+    got = []
+    for pr in preps:
+        hh = pr.hdr
+        busy = {id(x) for x in slots if x is not None}
+        cur = next(f for f in pool if id(f) not in busy)
+        refs = [None] * 3 if hh.frame_is_intra else [slots[hh.ref_frame_idx[k]] for k in range(3)]
+        gpu.run(pr, refs, cur)
+        for k in range(8):
+            if (hh.refresh_frame_flags >> k) & 1:
+                slots[k] = cur
+        if hh.show_frame:
+            got.append(oracle_lib.md5(cur.packed()))
+    gpu.synchronize()
+    assert got == want
+
+
+def test_decoder_api_on_the_1080p_workload():
+    """The decoder API (its own frame pool) on the first frames of the
+    1080p workload equals the oracle frame for frame."""
+    frames = streamgen.generate(**dict(streamgen.CONFIG2, frames=10))
+    ref = oracle_lib.CpuDecoder()
+    d = vp.Decoder(0)
+    for i, f in enumerate(frames):
+        want = [oracle_lib.md5(o.packed()) for o in ref.decode(f)]
+        got = [oracle_lib.md5(o.packed()) for o in d.decode_all(f)]
+        assert got == want, f"frame {i}"
