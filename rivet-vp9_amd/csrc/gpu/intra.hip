@@ -231,6 +231,69 @@ __global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group)
   }
 }
 
+// Reconstructs intra block `it` (all its planes, transform blocks in
+// order) in LDS: the row above it (from above-left) and the column left of
+// it come from the frame, the block goes back to the frame.
+template <class L, bool HBD>
+DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, const IntraItem &it) {
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  const FrameGeom &g = a.g;
+  const int lane = threadIdx.x;
+  const vp9r_block &b = a.blocks[it.block];
+  const int mi_row = b.mi_row, mi_col = b.mi_col;
+  const int bw8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_wide(b.mi_size);
+  const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_high(b.mi_size);
+  uint32_t k0 = it.first;
+  const uint32_t k_end = it.first + it.count;
+  for (int plane = 0; plane < 3; plane++) {
+    const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+    const int x0 = (mi_col * 8) >> sx, y0 = (mi_row * 8) >> sy;
+    const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy;
+    const uint32_t stride = a.cur.stride[plane];
+    P *pl = (P *)a.cur.p[plane];
+    // the row above (x0 - 1 .. x0 + BW - 1) and the column left; the block
+    // itself from the frame too (its samples outside the 8x8-aligned area
+    // are not reconstructed and keep the frame's)
+    for (int k = lane; k < (BH + 1) * (BW + 1); k += 64) {
+      const int r = k / (BW + 1), c = k - r * (BW + 1);
+      const int py = y0 - 1 + r, px = x0 - 1 + c;
+      if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
+    }
+    __syncthreads();
+    const uint32_t k_plane = k0;
+    while (k0 < k_end && a.txbs[k0].plane == plane) {
+      intra_txb<L, HBD>(a, sh, coef, a.txbs[k0], x0, y0);
+      k0++;
+    }
+    if (k0 > k_plane)
+      for (int k = lane; k < BW * BH; k += 64) {
+        const int r = k / BW, c = k - r * BW;
+        pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
+      }
+    __syncthreads();
+  }
+}
+
+// ---- one launch per level of intra blocks (levels from the host: a block
+// is one level above the deepest intra block it reads)
+
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_intra_blocks(IntraArgs a, uint32_t first) {
+  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
+  typedef typename L::T T;
+  __shared__ IntraShared sh;
+  __shared__ T coef[32 * 33];
+  intra_block<L, HBD>(a, sh, coef, a.items[first + blockIdx.x]);
+}
+
+void launch_intra_blocks(hipStream_t s, const IntraArgs &a, uint32_t first, uint32_t n) {
+  if (!n) return;
+  if (a.g.bit_depth > 8)
+    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(n), dim3(64), 0, s, a, first);
+  else
+    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(n), dim3(64), 0, s, a, first);
+}
+
 // ---- every intra block of the frame in one launch
 //
 // Workgroups take intra blocks in decode order (a topological order: a
@@ -294,40 +357,7 @@ __global__ __launch_bounds__(64) void k_intra_wave(IntraArgs a) {
     }
     __syncthreads();
     __threadfence();
-    uint32_t k0 = it.first;
-    const uint32_t k_end = it.first + it.count;
-    for (int plane = 0; plane < 3; plane++) {
-      const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
-      const int x0 = (mi_col * 8) >> sx, y0 = (mi_row * 8) >> sy;
-      const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy;
-      const uint32_t stride = a.cur.stride[plane];
-      P *pl = (P *)a.cur.p[plane];
-      // the row above (x0 - 1 .. x0 + BW - 1) and the column left
-      for (int k = lane; k < BW + 1 + BH; k += 64) {
-        int r, c;
-        if (k <= BW) {
-          r = 0;
-          c = k;
-        } else {
-          r = k - BW;
-          c = 0;
-        }
-        int py = y0 - 1 + r, px = x0 - 1 + c;
-        if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
-      }
-      __syncthreads();
-      const uint32_t k_plane = k0;
-      while (k0 < k_end && a.txbs[k0].plane == plane) {
-        intra_txb<L, HBD>(a, sh, coef, a.txbs[k0], x0, y0);
-        k0++;
-      }
-      if (k0 > k_plane)
-        for (int k = lane; k < BW * BH; k += 64) {
-          int r = k / BW, c = k - r * BW;
-          pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
-        }
-      __syncthreads();
-    }
+    intra_block<L, HBD>(a, sh, coef, it);
     if (lane < bw8 * bh8) {
       const int r = mi_row + lane / bw8, c = mi_col + lane % bw8;
       if (r < (int)g.mi_rows && c < (int)g.mi_cols) {

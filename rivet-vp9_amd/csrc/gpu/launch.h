@@ -123,6 +123,7 @@ void launch_mc(hipStream_t s, const McArgs &a, uint32_t n_tiles);
 void launch_itx_inter(hipStream_t s, const ItxArgs &a, int tx_size, uint32_t n_jobs);
 void launch_intra(hipStream_t s, const IntraArgs &a, uint32_t first_group, uint32_t n_groups);
 void launch_intra_wave(hipStream_t s, const IntraArgs &a, uint32_t n_groups);
+void launch_intra_blocks(hipStream_t s, const IntraArgs &a, uint32_t first, uint32_t n);
 void launch_lf_mask(hipStream_t s, const LfArgs &a);
 void launch_lf_level(hipStream_t s, const LfArgs &a, uint32_t first, uint32_t n);
 void launch_lf_wave(hipStream_t s, const LfArgs &a, uint32_t n_groups);

@@ -27,32 +27,37 @@ constexpr int BP = 74;  // LDS pitch of the 72 x 72 region (odd in dwords)
 
 DEV int iabs(int v) { return v < 0 ? -v : v; }
 
-// The wide filters (8.8.5.3) across the edge at b[pos]: output i in
-// [-n, n) is Round2(s[i] + sum_{j=-n..n} s[clamp(i + j, -n - 1, n)], LOG2)
-// with n = 2^(LOG2-1) - 1; fully unrolled, so the taps stay in registers.
-template <int LOG2>
-DEV void lf_wide(uint16_t *b, int pos, int step) {
+// The filters of one edge position, on samples reached through `s`: s(k)
+// is the sample k positions across the edge from the first q sample
+// (k = -8 .. 7), a reference into LDS or into a register array.
+
+// The wide filters (8.8.5.3): output i in [-n, n) is
+// Round2(s[i] + T[i], LOG2) with the window sum
+// T[i] = sum_{j=-n..n} s[clamp(i + j, -n - 1, n)] and n = 2^(LOG2-1) - 1.
+// T slides: T[i + 1] = T[i] - s[clamp(i - n)] + s[clamp(i + n + 1)].
+template <int LOG2, class S>
+DEV void lf_wide(S s) {
   constexpr int n = (1 << (LOG2 - 1)) - 1;
-  int s[2 * n + 2];
+  int v[2 * n + 2];
 #pragma unroll
-  for (int k = -(n + 1); k <= n; k++) s[k + n + 1] = b[pos + k * step];
+  for (int k = -(n + 1); k <= n; k++) v[k + n + 1] = s(k);
+  auto at = [&](int k) { return v[(k < -(n + 1) ? -(n + 1) : (k > n ? n : k)) + n + 1]; };
+  int t = 0;
+#pragma unroll
+  for (int j = -n; j <= n; j++) t += at(-n + j);
 #pragma unroll
   for (int i = -n; i < n; i++) {
-    int t = s[i + n + 1];
-#pragma unroll
-    for (int j = -n; j <= n; j++) {
-      const int k = i + j < -(n + 1) ? -(n + 1) : (i + j > n ? n : i + j);
-      t += s[k + n + 1];
-    }
-    b[pos + i * step] = (uint16_t)((t + (1 << (LOG2 - 1))) >> LOG2);
+    s(i) = (v[i + n + 1] + t + (1 << (LOG2 - 1))) >> LOG2;
+    t += at(i + n + 1) - at(i - n);
   }
 }
 
-// lf::filter on an LDS buffer: across the edge whose first q sample is
-// b[pos], `step` apart.
-DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit, int thresh, int bd) {
-  const int q0 = b[pos], q1 = b[pos + step], q2 = b[pos + 2 * step], q3 = b[pos + 3 * step];
-  const int p0 = b[pos - step], p1 = b[pos - 2 * step], p2 = b[pos - 3 * step], p3 = b[pos - 4 * step];
+// lf::filter (src/dsp/lf.rs:24-86): the mask, then the narrow filter, or
+// the 8- or 16-wide filter where the samples are flat.
+template <class S>
+DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
+  const int q0 = s(0), q1 = s(1), q2 = s(2), q3 = s(3);
+  const int p0 = s(-1), p1 = s(-2), p2 = s(-3), p3 = s(-4);
   const int sh = bd - 8;
   const int lim = limit << sh, blim = blimit << sh;
   if (iabs(p3 - p2) > lim || iabs(p2 - p1) > lim || iabs(p1 - p0) > lim || iabs(q1 - q0) > lim ||
@@ -70,26 +75,31 @@ DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit
     int f = hev ? iclamp(ps1 - qs1, lo, hi) : 0;
     f = iclamp(f + 3 * (qs0 - ps0), lo, hi);
     const int f1 = iclamp(f + 4, lo, hi) >> 3, f2 = iclamp(f + 3, lo, hi) >> 3;
-    b[pos] = (uint16_t)(iclamp(qs0 - f1, lo, hi) + off);
-    b[pos - step] = (uint16_t)(iclamp(ps0 + f2, lo, hi) + off);
+    s(0) = iclamp(qs0 - f1, lo, hi) + off;
+    s(-1) = iclamp(ps0 + f2, lo, hi) + off;
     if (!hev) {
       const int g = (f1 + 1) >> 1;
-      b[pos + step] = (uint16_t)(iclamp(qs1 - g, lo, hi) + off);
-      b[pos - 2 * step] = (uint16_t)(iclamp(ps1 + g, lo, hi) + off);
+      s(1) = iclamp(qs1 - g, lo, hi) + off;
+      s(-2) = iclamp(ps1 + g, lo, hi) + off;
     }
     return;
   }
   bool flat2 = fs >= 2;
   if (flat2) {
 #pragma unroll
-    for (int k = 4; k < 8; k++)
-      flat2 = flat2 && iabs((int)b[pos - (k + 1) * step] - p0) <= one && iabs((int)b[pos + k * step] - q0) <= one;
+    for (int k = 4; k < 8; k++) flat2 = flat2 && iabs(s(-(k + 1)) - p0) <= one && iabs(s(k) - q0) <= one;
   }
   // wide filter (8.8.5.3): log2 3 (8 taps) or 4 (16 taps)
   if (flat2)
-    lf_wide<4>(b, pos, step);
+    lf_wide<4>(s);
   else
-    lf_wide<3>(b, pos, step);
+    lf_wide<3>(s);
+}
+
+// lf::filter on an LDS buffer: across the edge whose first q sample is
+// b[pos], `step` apart.
+DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit, int thresh, int bd) {
+  lf_apply([&](int k) -> uint16_t & { return b[pos + k * step]; }, fs, limit, blimit, thresh, bd);
 }
 
 // ---- decisions
@@ -270,14 +280,142 @@ DEV void lf_superblock(const LfArgs &a, LfShared &sh, int plane, int sb_row, int
   lf_store<P>(a, sh, plane, g);
 }
 
+// ---- one superblock of one plane, streamed through a register window
+//
+// The region (the superblock with its 8-sample halo above and left) is
+// staged in LDS with 8- or 16-byte loads.  The vertical pass gives each lane
+// one row and slides a 16-sample register window (p7 .. q7 of the current
+// edge) along it: after edge e the 4 samples that leave the window are
+// final (no later edge of the row reaches them) and go back to LDS, 4 new
+// ones come in.  The horizontal pass does the same down each column.  The
+// loop over edges stays rolled (one copy of the filters), the window's
+// indices are constants.  Limits come from LDS: prm[l] = limit | blimit << 8
+// | thresh << 16.
+
+template <class P, int RW>
+DEV void lf_region_io(P *pl, uint32_t stride, int x0, int y0, int RH, uint16_t *buf, bool store) {
+  constexpr int V = 16 / sizeof(P) > 8 ? 8 : 16 / sizeof(P);  // samples per vector (8)
+  constexpr int NV = RW / V;
+  static_assert(RW % V == 0, "region rows of whole vectors");
+  for (int k = threadIdx.x; k < RH * NV; k += 64) {
+    const int r = k / NV, q = k - r * NV;
+    const int py = y0 - 8 + r, px = x0 - 8 + q * V;
+    if (py < 0 || px < 0) continue;
+    P *g = pl + (size_t)py * stride + px;
+    uint16_t *l = buf + r * BP + q * V;
+    if (sizeof(P) == 1) {
+      if (store) {
+        uint2 w;
+        w.x = l[0] | l[1] << 8 | l[2] << 16 | (uint32_t)l[3] << 24;
+        w.y = l[4] | l[5] << 8 | l[6] << 16 | (uint32_t)l[7] << 24;
+        *(uint2 *)g = w;
+      } else {
+        const uint2 w = *(const uint2 *)g;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          l[i] = (w.x >> (8 * i)) & 0xff;
+          l[4 + i] = (w.y >> (8 * i)) & 0xff;
+        }
+      }
+    } else {
+      if (store) {
+        uint4 w;
+        w.x = l[0] | (uint32_t)l[1] << 16;
+        w.y = l[2] | (uint32_t)l[3] << 16;
+        w.z = l[4] | (uint32_t)l[5] << 16;
+        w.w = l[6] | (uint32_t)l[7] << 16;
+        *(uint4 *)g = w;
+      } else {
+        const uint4 w = *(const uint4 *)g;
+        l[0] = w.x & 0xffff; l[1] = w.x >> 16;
+        l[2] = w.y & 0xffff; l[3] = w.y >> 16;
+        l[4] = w.z & 0xffff; l[5] = w.z >> 16;
+        l[6] = w.w & 0xffff; l[7] = w.w >> 16;
+      }
+    }
+  }
+}
+
+// One pass: lane-owned lines of `len` samples, line i at buf + i * line,
+// samples `step` apart; edges at 8 + 4 e for e < n_edges.
+DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, const int8_t *fs,
+                   const uint8_t *lv, int rn, const uint32_t *prm, int bd) {
+  int w[16];
+  uint16_t *ln = buf + line_off;
+#pragma unroll
+  for (int i = 0; i < 16; i++) w[i] = ln[i * step];
+  for (int e = 0; e < n_edges; e++) {
+    const int f = fs[e * 16 + rn];
+    if (f >= 0) {
+      const uint32_t q = prm[lv[e * 16 + rn]];
+      lf_apply([&](int k) -> int & { return w[8 + k]; }, f, q & 0xff, (q >> 8) & 0xff, q >> 16, bd);
+    }
+    const int base = 4 * e;
+#pragma unroll
+    for (int i = 0; i < 4; i++) ln[(base + i) * step] = (uint16_t)w[i];
+#pragma unroll
+    for (int i = 0; i < 12; i++) w[i] = w[i + 4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int pos = base + 16 + i;
+      w[12 + i] = pos < len ? (int)ln[pos * step] : 0;
+    }
+  }
+  const int base = 4 * n_edges;
+#pragma unroll
+  for (int i = 0; i < 16; i++)
+    if (base + i < len) ln[(base + i) * step] = (uint16_t)w[i];
+}
+
+template <class P, int SW, int SH>
+DEV void lf_sb_stream(const LfArgs &a, LfShared &sh, const uint32_t *prm, int plane, int sb_row, int sb_col) {
+  constexpr int RW = SW + 8, RH = SH + 8;
+  const int lane = threadIdx.x;
+  const int x0 = sb_col * SW, y0 = sb_row * SH;
+  const uint32_t stride = a.cur.stride[plane];
+  P *pl = (P *)a.cur.p[plane];
+  const int bd = a.g.bit_depth;
+  lf_region_io<P, RW>(pl, stride, x0, y0, RH, sh.buf, false);
+  __syncthreads();
+  if (lane < SH)
+    lf_stream(sh.buf, (8 + lane) * BP, 1, RW, SW / 4, sh.fs[0], sh.lv[0], lane >> 2, prm, bd);
+  __syncthreads();
+  if (lane < SW)
+    lf_stream(sh.buf, 8 + lane, BP, RH, SH / 4, sh.fs[1], sh.lv[1], lane >> 2, prm, bd);
+  __syncthreads();
+  lf_region_io<P, RW>(pl, stride, x0, y0, RH, sh.buf, true);
+}
+
+// Dispatches on the plane's superblock size (subsampling).
+template <class P>
+DEV void lf_sb(const LfArgs &a, LfShared &sh, const uint32_t *prm, int plane, int sb_row, int sb_col) {
+  const int sx = plane ? a.g.ss_x : 0, sy = plane ? a.g.ss_y : 0;
+  if (!sx && !sy)
+    lf_sb_stream<P, 64, 64>(a, sh, prm, plane, sb_row, sb_col);
+  else if (sx && sy)
+    lf_sb_stream<P, 32, 32>(a, sh, prm, plane, sb_row, sb_col);
+  else if (sx)
+    lf_sb_stream<P, 32, 64>(a, sh, prm, plane, sb_row, sb_col);
+  else
+    lf_sb_stream<P, 64, 32>(a, sh, prm, plane, sb_row, sb_col);
+}
+
+// The limits of every level in LDS (prm[l] = limit | blimit << 8 | thresh << 16).
+DEV void lf_load_params(const LfArgs &a, uint32_t *prm) {
+  for (int l = threadIdx.x; l < 64; l += blockDim.x)
+    prm[l] = a.p.limit[l] | (uint32_t)a.p.blimit[l] << 8 | (uint32_t)a.p.thresh[l] << 16;
+}
+
 // ---- one wavefront level per launch (decisions from k_lf_mask)
 
 template <bool HBD>
 __global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ LfShared sh;
+  __shared__ uint32_t prm[64];
   const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
+  lf_load_params(a, prm);
   const uint32_t code = a.sbs[first + blockIdx.x];
   const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
   const int n_sb = g.sb64_rows * g.sb64_cols, sb = sb_row * g.sb64_cols + sb_col;
@@ -288,7 +426,8 @@ __global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
       sh.lv[pass][k] = a.mask_lvl[o + k];
     }
   }
-  lf_superblock<P>(a, sh, plane, sb_row, sb_col);
+  __syncthreads();
+  lf_sb<P>(a, sh, prm, plane, sb_row, sb_col);
 }
 
 // ---- the whole frame in one launch

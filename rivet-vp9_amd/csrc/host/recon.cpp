@@ -216,6 +216,42 @@ static void build_intra(vp9r_gpu *g, const vp9r_record *rec) {
       g->intra_items.push_back(vp9g::IntraItem{t.block, i, 0});
     g->intra_items.back().count++;
   }
+  // Levels of the intra blocks: one above the deepest intra block covering
+  // an 8x8 position the block reads (the row above from above-left to its
+  // top-right, the column left); inter positions are complete before the
+  // intra launches.  Items are ordered by level (a topological order).
+  {
+    const uint32_t cols = h.mi_cols, rows = h.mi_rows;
+    g->cell_level.assign((size_t)rows * cols, -1);
+    g->item_level.resize(g->intra_items.size());
+    int max_lv = -1;
+    for (size_t k = 0; k < g->intra_items.size(); k++) {
+      const vp9r_block &b = rec->blocks[g->intra_items[k].block];
+      const int bw8 = b.mi_size < BLOCK_8X8 ? 1 : VP9_NUM_8X8_WIDE[b.mi_size];
+      const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : VP9_NUM_8X8_HIGH[b.mi_size];
+      int lv = 0;
+      auto dep = [&](int r, int c) {
+        if (r < 0 || c < 0 || r >= (int)rows || c >= (int)cols) return;
+        const int l = g->cell_level[(size_t)r * cols + c];
+        if (l >= 0) lv = std::max(lv, l + 1);
+      };
+      if (b.avail_u)
+        for (int i = -1; i < bw8; i++) dep((int)b.mi_row - 1, (int)b.mi_col + i);
+      if (b.avail_l)
+        for (int i = 0; i < bh8; i++) dep((int)b.mi_row + i, (int)b.mi_col - 1);
+      g->item_level[k] = lv;
+      max_lv = std::max(max_lv, lv);
+      for (int r = b.mi_row; r < (int)b.mi_row + bh8 && r < (int)rows; r++)
+        for (int c = b.mi_col; c < (int)b.mi_col + bw8 && c < (int)cols; c++)
+          g->cell_level[(size_t)r * cols + c] = lv;
+    }
+    g->item_level_start.assign(max_lv + 2, 0);
+    for (int lv : g->item_level) g->item_level_start[lv + 1]++;
+    for (int l = 0; l <= max_lv; l++) g->item_level_start[l + 1] += g->item_level_start[l];
+    std::vector<uint32_t> fill(g->item_level_start.begin(), g->item_level_start.end());
+    g->items_sorted.resize(g->intra_items.size());
+    for (size_t k = 0; k < g->intra_items.size(); k++) g->items_sorted[fill[g->item_level[k]]++] = g->intra_items[k];
+  }
   // level = 1 + the deepest of the left, above and above-left groups
   g->intra_level.assign(3 * n_sb, -1);
   int max_level = -1;
@@ -690,6 +726,7 @@ static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stag
   for (int k = 0; k < 4; k++) p.n_jobs[k] = (uint32_t)g->jobs[k].size();
   p.n_groups = (uint32_t)g->groups.size();
   p.n_items = (uint32_t)g->intra_items.size();
+  p.item_level_start = g->item_level_start;
   p.n_lf = do_lf ? (uint32_t)g->lf_codes.size() : 0;
   p.group_level_start = g->group_level_start;
   if (do_lf)
@@ -713,7 +750,7 @@ static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stag
   for (int k = 0; k < 4; k++) put(p.o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
   put(p.o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
   put(p.o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
-  put(p.o_items, g->intra_items.data(), sizeof(vp9g::IntraItem) * g->intra_items.size());
+  put(p.o_items, g->items_sorted.data(), sizeof(vp9g::IntraItem) * g->items_sorted.size());
   if (do_lf) put(p.o_lf, g->lf_codes.data(), sizeof(uint32_t) * g->lf_codes.size());
   return VP9R_OK;
 }
@@ -825,6 +862,20 @@ static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_
     a.n_items = p.n_items;
     a.sync = wave_sync(g, 0);
     vp9g::launch_intra_wave(s, a, 1024);
+  } else if (p.n_items && !p.hdr.frame_is_intra) {
+    vp9g::IntraArgs a{};
+    a.g = p.geo;
+    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
+    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
+    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
+    a.coef_val = (const int32_t *)(dv + p.o_cval);
+    a.cur = cur->planes;
+    memcpy(a.q, p.q, sizeof p.q);
+    a.items = (const vp9g::IntraItem *)(dv + p.o_items);
+    for (size_t l = 0; l + 1 < p.item_level_start.size(); l++) {
+      uint32_t first = p.item_level_start[l], n = p.item_level_start[l + 1] - first;
+      vp9g::launch_intra_blocks(s, a, first, n);
+    }
   } else if (p.n_groups) {
     vp9g::IntraArgs a{};
     a.g = p.geo;
@@ -980,8 +1031,10 @@ int vp9r_gpu_prepared_stats(const vp9r_prepared *pr, uint64_t out[12]) {
   for (int k = 0; k < 4; k++) out[8 + k] = p.work_bytes[k];
   out[0] = p.n_tiles;
   for (int k = 0; k < 4; k++) out[1 + k] = p.n_jobs[k];
-  out[5] = p.n_groups;
-  out[6] = p.group_level_start.empty() ? 0 : p.group_level_start.size() - 1;
+  const bool blocks = !p.hdr.frame_is_intra;
+  out[5] = blocks ? p.n_items : p.n_groups;
+  out[6] = blocks ? (p.item_level_start.empty() ? 0 : p.item_level_start.size() - 1)
+                  : (p.group_level_start.empty() ? 0 : p.group_level_start.size() - 1);
   out[7] = p.lf_level_start.empty() ? 0 : p.lf_level_start.size() - 1;
   return VP9R_OK;
 }

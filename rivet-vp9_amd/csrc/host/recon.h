@@ -41,7 +41,7 @@ struct Prepared {
   size_t o_blocks = 0, o_txbs = 0, o_cpos = 0, o_cval = 0, o_mi = 0, o_tiles = 0, o_jobs[4] = {},
          o_iidx = 0, o_groups = 0, o_lf = 0, o_items = 0;
   uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0, n_items = 0, n_lf = 0;
-  std::vector<uint32_t> group_level_start, lf_level_start;
+  std::vector<uint32_t> group_level_start, lf_level_start, item_level_start;
   size_t bytes = 0;
   // algorithmic HBM bytes of the frame per stage: inter prediction, inter
   // residual, intra, loop filter
@@ -85,7 +85,9 @@ struct vp9r_gpu {
   std::vector<vp9g::McTile> tiles;
   std::vector<uint32_t> jobs[4];
   std::vector<uint32_t> intra_cnt, intra_first, intra_fill, intra_index;
-  std::vector<vp9g::IntraItem> intra_items;
+  std::vector<vp9g::IntraItem> intra_items, items_sorted;
+  std::vector<int32_t> cell_level, item_level;
+  std::vector<uint32_t> item_level_start;
   std::vector<int32_t> intra_level;
   std::vector<vp9g::IntraGroup> groups;
   std::vector<uint32_t> group_level_start;
