@@ -111,13 +111,22 @@ DEV void intra_residual(typename L::T *coef, uint16_t *tile, int lx, int ly, con
   typedef typename L::T T;
   constexpr int n0 = 1 << n, S = n0 + 1;
   const int lane = threadIdx.x;
-  for (int k = lane; k < n0 * S; k += 64) coef[k] = 0;
-  __syncthreads();
   const int pt = t.plane > 0;
   const int32_t dcq = a.q[seg][pt][0], acq = a.q[seg][pt][1];
   const uint16_t *pos = a.coef_pos + t.coef_start;
   const int32_t *val = a.coef_val + t.coef_start;
-  for (int k = lane; k < t.ncoef; k += 64) {
+  // the first 64 coefficients are fetched before the tile is cleared, so
+  // their latency overlaps the clearing
+  int p0 = -1;
+  int32_t v0 = 0;
+  if (lane < t.ncoef) {
+    p0 = pos[lane] & (n0 * n0 - 1);
+    v0 = val[lane];
+  }
+  for (int k = lane; k < n0 * S; k += 64) coef[k] = 0;
+  __syncthreads();
+  if (p0 >= 0) coef[(p0 >> n) * S + (p0 & (n0 - 1))] = (T)dequant(v0, p0, dcq, acq, n == 5);
+  for (int k = lane + 64; k < t.ncoef; k += 64) {
     int p = pos[k] & (n0 * n0 - 1);
     coef[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(val[k], p, dcq, acq, n == 5);
   }
@@ -217,18 +226,19 @@ __global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group)
   const int x0 = gr.sb_col * SW, y0 = gr.sb_row * SH;
   const uint32_t stride = a.cur.stride[plane];
   P *pl = (P *)a.cur.p[plane];
-  for (int k = lane; k < (SH + 1) * (SW + 1); k += 64) {
-    int r = k / (SW + 1), c = k - r * (SW + 1);
-    int py = y0 - 1 + r, px = x0 - 1 + c;
-    if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
+  for (int r = 0; r <= SH; r++) {
+    const int py = y0 - 1 + r;
+    if (py < 0) continue;
+    for (int c = lane; c <= SW; c += 64) {
+      const int px = x0 - 1 + c;
+      if (px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
+    }
   }
   __syncthreads();
   for (uint32_t ti = 0; ti < gr.count; ti++)
     intra_txb<L, HBD>(a, sh, coef, a.txbs[a.txb_index[gr.first + ti]], x0, y0);
-  for (int k = lane; k < SH * SW; k += 64) {
-    int r = k / SW, c = k - r * SW;
-    pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
-  }
+  for (int r = 0; r < SH; r++)
+    for (int c = lane; c < SW; c += 64) pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
 }
 
 // Reconstructs intra block `it` (all its planes, transform blocks in
@@ -254,10 +264,13 @@ DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, c
     // the row above (x0 - 1 .. x0 + BW - 1) and the column left; the block
     // itself from the frame too (its samples outside the 8x8-aligned area
     // are not reconstructed and keep the frame's)
-    for (int k = lane; k < (BH + 1) * (BW + 1); k += 64) {
-      const int r = k / (BW + 1), c = k - r * (BW + 1);
-      const int py = y0 - 1 + r, px = x0 - 1 + c;
-      if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
+    for (int r = 0; r <= BH; r++) {
+      const int py = y0 - 1 + r;
+      if (py < 0) continue;
+      for (int c = lane; c <= BW; c += 64) {
+        const int px = x0 - 1 + c;
+        if (px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
+      }
     }
     __syncthreads();
     const uint32_t k_plane = k0;
@@ -266,10 +279,8 @@ DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, c
       k0++;
     }
     if (k0 > k_plane)
-      for (int k = lane; k < BW * BH; k += 64) {
-        const int r = k / BW, c = k - r * BW;
-        pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
-      }
+      for (int r = 0; r < BH; r++)
+        for (int c = lane; c < BW; c += 64) pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
     __syncthreads();
   }
 }

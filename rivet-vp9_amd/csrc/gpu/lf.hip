@@ -190,6 +190,7 @@ struct LfShared {
   uint16_t buf[72 * BP];  // the superblock and its 8-sample halo above and left
   int8_t fs[2][256];
   uint8_t lv[2][256];
+  uint32_t dq[2][256];    // packed decisions for lf_stream
 };
 
 // The steps on one superblock of one plane: the region is the superblock
@@ -337,29 +338,37 @@ DEV void lf_region_io(P *pl, uint32_t stride, int x0, int y0, int RH, uint16_t *
 }
 
 // One pass: lane-owned lines of `len` samples, line i at buf + i * line,
-// samples `step` apart; edges at 8 + 4 e for e < n_edges.
-DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, const int8_t *fs,
-                   const uint8_t *lv, int rn, const uint32_t *prm, int bd) {
+// samples `step` apart; edges at 8 + 4 e for e < n_edges.  dq[e * 16 + rn]
+// packs the edge's decision: (filter size + 1) | limit << 8 | blimit << 16 |
+// thresh << 24, 0 for no filter.  The next edge's decision and the 4
+// samples entering the window are fetched before the filter (the filter of
+// edge e writes at most up to position 4 e + 14).
+DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, const uint32_t *dq, int rn,
+                   int bd) {
   int w[16];
   uint16_t *ln = buf + line_off;
 #pragma unroll
   for (int i = 0; i < 16; i++) w[i] = ln[i * step];
+  uint32_t d_next = dq[rn];
   for (int e = 0; e < n_edges; e++) {
-    const int f = fs[e * 16 + rn];
-    if (f >= 0) {
-      const uint32_t q = prm[lv[e * 16 + rn]];
-      lf_apply([&](int k) -> int & { return w[8 + k]; }, f, q & 0xff, (q >> 8) & 0xff, q >> 16, bd);
-    }
+    const uint32_t d = d_next;
+    if (e + 1 < n_edges) d_next = dq[(e + 1) * 16 + rn];
     const int base = 4 * e;
+    int nw[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int pos = base + 16 + i;
+      nw[i] = pos < len ? (int)ln[pos * step] : 0;
+    }
+    if (d)
+      lf_apply([&](int k) -> int & { return w[8 + k]; }, (int)(d & 0xff) - 1, (d >> 8) & 0xff, (d >> 16) & 0xff,
+               d >> 24, bd);
 #pragma unroll
     for (int i = 0; i < 4; i++) ln[(base + i) * step] = (uint16_t)w[i];
 #pragma unroll
     for (int i = 0; i < 12; i++) w[i] = w[i + 4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int pos = base + 16 + i;
-      w[12 + i] = pos < len ? (int)ln[pos * step] : 0;
-    }
+    for (int i = 0; i < 4; i++) w[12 + i] = nw[i];
   }
   const int base = 4 * n_edges;
 #pragma unroll
@@ -368,7 +377,7 @@ DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, 
 }
 
 template <class P, int SW, int SH>
-DEV void lf_sb_stream(const LfArgs &a, LfShared &sh, const uint32_t *prm, int plane, int sb_row, int sb_col) {
+DEV void lf_sb_stream(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
   constexpr int RW = SW + 8, RH = SH + 8;
   const int lane = threadIdx.x;
   const int x0 = sb_col * SW, y0 = sb_row * SH;
@@ -377,27 +386,25 @@ DEV void lf_sb_stream(const LfArgs &a, LfShared &sh, const uint32_t *prm, int pl
   const int bd = a.g.bit_depth;
   lf_region_io<P, RW>(pl, stride, x0, y0, RH, sh.buf, false);
   __syncthreads();
-  if (lane < SH)
-    lf_stream(sh.buf, (8 + lane) * BP, 1, RW, SW / 4, sh.fs[0], sh.lv[0], lane >> 2, prm, bd);
+  if (lane < SH) lf_stream(sh.buf, (8 + lane) * BP, 1, RW, SW / 4, sh.dq[0], lane >> 2, bd);
   __syncthreads();
-  if (lane < SW)
-    lf_stream(sh.buf, 8 + lane, BP, RH, SH / 4, sh.fs[1], sh.lv[1], lane >> 2, prm, bd);
+  if (lane < SW) lf_stream(sh.buf, 8 + lane, BP, RH, SH / 4, sh.dq[1], lane >> 2, bd);
   __syncthreads();
   lf_region_io<P, RW>(pl, stride, x0, y0, RH, sh.buf, true);
 }
 
 // Dispatches on the plane's superblock size (subsampling).
 template <class P>
-DEV void lf_sb(const LfArgs &a, LfShared &sh, const uint32_t *prm, int plane, int sb_row, int sb_col) {
+DEV void lf_sb(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
   const int sx = plane ? a.g.ss_x : 0, sy = plane ? a.g.ss_y : 0;
   if (!sx && !sy)
-    lf_sb_stream<P, 64, 64>(a, sh, prm, plane, sb_row, sb_col);
+    lf_sb_stream<P, 64, 64>(a, sh, plane, sb_row, sb_col);
   else if (sx && sy)
-    lf_sb_stream<P, 32, 32>(a, sh, prm, plane, sb_row, sb_col);
+    lf_sb_stream<P, 32, 32>(a, sh, plane, sb_row, sb_col);
   else if (sx)
-    lf_sb_stream<P, 32, 64>(a, sh, prm, plane, sb_row, sb_col);
+    lf_sb_stream<P, 32, 64>(a, sh, plane, sb_row, sb_col);
   else
-    lf_sb_stream<P, 64, 32>(a, sh, prm, plane, sb_row, sb_col);
+    lf_sb_stream<P, 64, 32>(a, sh, plane, sb_row, sb_col);
 }
 
 // The limits of every level in LDS (prm[l] = limit | blimit << 8 | thresh << 16).
@@ -419,15 +426,16 @@ __global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
   const uint32_t code = a.sbs[first + blockIdx.x];
   const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
   const int n_sb = g.sb64_rows * g.sb64_cols, sb = sb_row * g.sb64_cols + sb_col;
+  __syncthreads();
   for (int pass = 0; pass < 2; pass++) {
     const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256;
     for (int k = lane; k < 256; k += 64) {
-      sh.fs[pass][k] = a.mask_fs[o + k];
-      sh.lv[pass][k] = a.mask_lvl[o + k];
+      const int f = a.mask_fs[o + k];
+      sh.dq[pass][k] = f < 0 ? 0u : (uint32_t)(f + 1) | prm[a.mask_lvl[o + k] & 63] << 8;
     }
   }
   __syncthreads();
-  lf_sb<P>(a, sh, prm, plane, sb_row, sb_col);
+  lf_sb<P>(a, sh, plane, sb_row, sb_col);
 }
 
 // ---- the whole frame in one launch
