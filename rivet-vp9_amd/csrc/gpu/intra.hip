@@ -226,19 +226,20 @@ __global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group)
   const int x0 = gr.sb_col * SW, y0 = gr.sb_row * SH;
   const uint32_t stride = a.cur.stride[plane];
   P *pl = (P *)a.cur.p[plane];
-  for (int r = 0; r <= SH; r++) {
-    const int py = y0 - 1 + r;
-    if (py < 0) continue;
-    for (int c = lane; c <= SW; c += 64) {
-      const int px = x0 - 1 + c;
-      if (px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
-    }
+  const uint32_t dsw = SW + 1, msw = ((1u << 20) + dsw - 1) / dsw;  // k / (SW + 1)
+  for (int k = lane; k < (SH + 1) * (SW + 1); k += 64) {
+    const int r = (int)(((uint32_t)k * msw) >> 20), c = k - r * (SW + 1);
+    const int py = y0 - 1 + r, px = x0 - 1 + c;
+    if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
   }
   __syncthreads();
   for (uint32_t ti = 0; ti < gr.count; ti++)
     intra_txb<L, HBD>(a, sh, coef, a.txbs[a.txb_index[gr.first + ti]], x0, y0);
-  for (int r = 0; r < SH; r++)
-    for (int c = lane; c < SW; c += 64) pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
+  const int lsw = 31 - __clz(SW);  // SW is 64 or 32
+  for (int k = lane; k < SH * SW; k += 64) {
+    const int r = k >> lsw, c = k & (SW - 1);
+    pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
+  }
 }
 
 // Reconstructs intra block `it` (all its planes, transform blocks in
@@ -264,13 +265,11 @@ DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, c
     // the row above (x0 - 1 .. x0 + BW - 1) and the column left; the block
     // itself from the frame too (its samples outside the 8x8-aligned area
     // are not reconstructed and keep the frame's)
-    for (int r = 0; r <= BH; r++) {
-      const int py = y0 - 1 + r;
-      if (py < 0) continue;
-      for (int c = lane; c <= BW; c += 64) {
-        const int px = x0 - 1 + c;
-        if (px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
-      }
+    const uint32_t d = BW + 1, m = ((1u << 20) + d - 1) / d;  // k / d as (k * m) >> 20
+    for (int k = lane; k < (BH + 1) * (BW + 1); k += 64) {
+      const int r = (int)(((uint32_t)k * m) >> 20), c = k - r * (BW + 1);
+      const int py = y0 - 1 + r, px = x0 - 1 + c;
+      if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
     }
     __syncthreads();
     const uint32_t k_plane = k0;
@@ -278,9 +277,13 @@ DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, c
       intra_txb<L, HBD>(a, sh, coef, a.txbs[k0], x0, y0);
       k0++;
     }
-    if (k0 > k_plane)
-      for (int r = 0; r < BH; r++)
-        for (int c = lane; c < BW; c += 64) pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
+    if (k0 > k_plane) {
+      const int lw = 31 - __clz(BW);  // BW is a power of two
+      for (int k = lane; k < BW * BH; k += 64) {
+        const int r = k >> lw, c = k & (BW - 1);
+        pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
+      }
+    }
     __syncthreads();
   }
 }

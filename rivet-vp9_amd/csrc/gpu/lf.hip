@@ -350,6 +350,9 @@ DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, 
 #pragma unroll
   for (int i = 0; i < 16; i++) w[i] = ln[i * step];
   uint32_t d_next = dq[rn];
+  // one copy of the filters: unrolled, the 16 edges would not fit the
+  // instruction cache
+#pragma unroll 1
   for (int e = 0; e < n_edges; e++) {
     const uint32_t d = d_next;
     if (e + 1 < n_edges) d_next = dq[(e + 1) * 16 + rn];
