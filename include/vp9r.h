@@ -180,9 +180,8 @@ void vp9r_prepared_free(vp9r_prepared *p);
 int vp9r_gpu_run(vp9r_gpu *g, const vp9r_prepared *p, vp9r_gpu_frame *const refs[3],
                  vp9r_gpu_frame *cur);
 /* Work of a prepared record: [0] inter prediction tiles (<= 16x16), [1..4]
- * inter transform blocks of 4x4 .. 32x32, [5] intra work items (intra
- * blocks of an inter frame; (superblock, plane) groups of an intra frame),
- * [6] intra wavefront launches, [7] loop filter wavefront launches,
+ * inter transform blocks of 4x4 .. 32x32, [5] intra blocks, [6] intra
+ * dependency levels (one launch each), [7] loop filter wavefront launches,
  * [8..11] algorithmic HBM bytes of inter prediction (reference footprints
  * read + prediction written), inter residual (n^2 x (4 + 2b) per transform
  * block with coefficients), intra (edges + block + coefficients) and the loop

@@ -107,14 +107,12 @@ struct LdsDst {
 
 template <class L, int n>
 DEV void intra_residual(typename L::T *coef, uint16_t *tile, int lx, int ly, const vp9r_txb &t,
-                        const IntraArgs &a, int seg, bool hbd) {
+                        const IntraArgs &a, int seg, bool hbd, const uint16_t *pos, const int32_t *val) {
   typedef typename L::T T;
   constexpr int n0 = 1 << n, S = n0 + 1;
   const int lane = threadIdx.x;
   const int pt = t.plane > 0;
   const int32_t dcq = a.q[seg][pt][0], acq = a.q[seg][pt][1];
-  const uint16_t *pos = a.coef_pos + t.coef_start;
-  const int32_t *val = a.coef_val + t.coef_start;
   // the first 64 coefficients are fetched before the tile is cleared, so
   // their latency overlaps the clearing
   int p0 = -1;
@@ -138,19 +136,47 @@ DEV void intra_residual(typename L::T *coef, uint16_t *tile, int lx, int ly, con
   itx_2d<L, n>(coef, lane, t.tx_type, lossless, dc_only, dc_v, dst, (1 << a.g.bit_depth) - 1);
 }
 
+// Staging of an intra block's transform block records and coefficients
+// (they are read in a dependent chain per transform block otherwise).
+constexpr int kStageTxbs = 192, kStageCoefs = 2048;
+
 struct IntraShared {
   uint16_t tile[TS * TS];  // rows y0 - 1 .., columns x0 - 1 .. of the region
   int32_t edge[65 + 32];
+  vp9r_txb txb[kStageTxbs];
+  uint16_t pos[kStageCoefs];
+  int32_t val[kStageCoefs];
 };
+
+// The fields of a block that intra reconstruction reads, in registers (the
+// sub-block modes packed, so that no array is indexed at run time).
+struct IntraBlk {
+  int mi_row, mi_col, mi_size, y_mode, uv_mode, segment_id;
+  bool avail_u, avail_l;
+  uint32_t sub_modes;  // 4 x 8 bits
+};
+
+DEV IntraBlk intra_blk(const vp9r_block *p) {
+  IntraBlk b;
+  b.mi_row = p->mi_row;
+  b.mi_col = p->mi_col;
+  b.mi_size = p->mi_size;
+  b.y_mode = p->y_mode;
+  b.uv_mode = p->uv_mode;
+  b.segment_id = p->segment_id & 7;
+  b.avail_u = p->avail_u;
+  b.avail_l = p->avail_l;
+  b.sub_modes = *(const uint32_t *)p->sub_modes;
+  return b;
+}
 
 // Predicts and reconstructs transform block t in the LDS tile whose sample
 // (0, 0) is plane position (x0 - 1, y0 - 1) (edges: recon.rs:17-103).
 template <class L, bool HBD>
-DEV void intra_txb(const IntraArgs &a, IntraShared &sh, typename L::T *coef, const vp9r_txb &t, int x0,
-                   int y0) {
+DEV void intra_txb(const IntraArgs &a, IntraShared &sh, typename L::T *coef, const vp9r_txb &t,
+                   const IntraBlk &b, int x0, int y0, const uint16_t *pos, const int32_t *val) {
   const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
-  const vp9r_block &b = a.blocks[t.block];
   const int plane = t.plane;
   const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
   const int max_x = (int)((g.mi_cols * 8) >> sx) - 1, max_y = (int)((g.mi_rows * 8) >> sy) - 1;
@@ -169,7 +195,8 @@ DEV void intra_txb(const IntraArgs &a, IntraShared &sh, typename L::T *coef, con
   const bool have_left = b.avail_l || xi > 0, have_above = b.avail_u || yi > 0;
   const bool not_on_right = xi + step < n4w;
   const int block_idx = (yi / step) * ((n4w + step - 1) / step) + xi / step;
-  const int mode = plane > 0 ? b.uv_mode : (b.mi_size >= BLOCK_8X8 ? b.y_mode : b.sub_modes[block_idx & 3]);
+  const int mode =
+      plane > 0 ? b.uv_mode : (b.mi_size >= BLOCK_8X8 ? b.y_mode : (int)((b.sub_modes >> (8 * (block_idx & 3))) & 0xff));
   const int x = t.x, y = t.y;
   // aboveRow[-1 .. 2 * size - 1]: 2 * size + 1 entries, 65 at 32x32, so
   // lane 0 also fills the last one
@@ -197,12 +224,12 @@ DEV void intra_txb(const IntraArgs &a, IntraShared &sh, typename L::T *coef, con
   }
   __syncthreads();
   if (t.eob > 0) {
-    const int seg = b.segment_id & 7;
+    const int seg = b.segment_id;
     switch (tx) {
-      case 0: intra_residual<L, 2>(coef, tile, lx, ly, t, a, seg, HBD); break;
-      case 1: intra_residual<L, 3>(coef, tile, lx, ly, t, a, seg, HBD); break;
-      case 2: intra_residual<L, 4>(coef, tile, lx, ly, t, a, seg, HBD); break;
-      default: intra_residual<L, 5>(coef, tile, lx, ly, t, a, seg, HBD); break;
+      case 0: intra_residual<L, 2>(coef, tile, lx, ly, t, a, seg, HBD, pos, val); break;
+      case 1: intra_residual<L, 3>(coef, tile, lx, ly, t, a, seg, HBD, pos, val); break;
+      case 2: intra_residual<L, 4>(coef, tile, lx, ly, t, a, seg, HBD, pos, val); break;
+      default: intra_residual<L, 5>(coef, tile, lx, ly, t, a, seg, HBD, pos, val); break;
     }
     __syncthreads();
   }
@@ -233,8 +260,11 @@ __global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group)
     if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
   }
   __syncthreads();
-  for (uint32_t ti = 0; ti < gr.count; ti++)
-    intra_txb<L, HBD>(a, sh, coef, a.txbs[a.txb_index[gr.first + ti]], x0, y0);
+  for (uint32_t ti = 0; ti < gr.count; ti++) {
+    const vp9r_txb t = a.txbs[a.txb_index[gr.first + ti]];
+    intra_txb<L, HBD>(a, sh, coef, t, intra_blk(a.blocks + t.block), x0, y0, a.coef_pos + t.coef_start,
+                      a.coef_val + t.coef_start);
+  }
   const int lsw = 31 - __clz(SW);  // SW is 64 or 32
   for (int k = lane; k < SH * SW; k += 64) {
     const int r = k >> lsw, c = k & (SW - 1);
@@ -250,12 +280,24 @@ DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, c
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
-  const vp9r_block &b = a.blocks[it.block];
+  const IntraBlk b = intra_blk(a.blocks + it.block);
   const int mi_row = b.mi_row, mi_col = b.mi_col;
   const int bw8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_wide(b.mi_size);
   const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_high(b.mi_size);
-  uint32_t k0 = it.first;
-  const uint32_t k_end = it.first + it.count;
+  // stage the transform block records and the coefficients they cover
+  const uint32_t n_st = it.count < (uint32_t)kStageTxbs ? it.count : (uint32_t)kStageTxbs;
+  for (uint32_t k = lane; k < n_st; k += 64) sh.txb[k] = a.txbs[it.first + k];
+  const uint32_t c0 = a.txbs[it.first].coef_start;
+  const vp9r_txb last = a.txbs[it.first + it.count - 1];
+  uint32_t c1 = last.coef_start + last.ncoef;
+  if (c1 < c0) c1 = c0;
+  const uint32_t n_c = c1 - c0 < (uint32_t)kStageCoefs ? c1 - c0 : (uint32_t)kStageCoefs;
+  for (uint32_t k = lane; k < n_c; k += 64) {
+    sh.pos[k] = a.coef_pos[c0 + k];
+    sh.val[k] = a.coef_val[c0 + k];
+  }
+  __syncthreads();
+  uint32_t k0 = 0;
   for (int plane = 0; plane < 3; plane++) {
     const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
     const int x0 = (mi_col * 8) >> sx, y0 = (mi_row * 8) >> sy;
@@ -273,8 +315,13 @@ DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, c
     }
     __syncthreads();
     const uint32_t k_plane = k0;
-    while (k0 < k_end && a.txbs[k0].plane == plane) {
-      intra_txb<L, HBD>(a, sh, coef, a.txbs[k0], x0, y0);
+    while (k0 < it.count) {
+      const vp9r_txb t = k0 < n_st ? sh.txb[k0] : a.txbs[it.first + k0];
+      if (t.plane != plane) break;
+      const bool staged = t.coef_start >= c0 && t.coef_start - c0 + t.ncoef <= n_c;
+      const uint16_t *pos = staged ? sh.pos + (t.coef_start - c0) : a.coef_pos + t.coef_start;
+      const int32_t *val = staged ? sh.val + (t.coef_start - c0) : a.coef_val + t.coef_start;
+      intra_txb<L, HBD>(a, sh, coef, t, b, x0, y0, pos, val);
       k0++;
     }
     if (k0 > k_plane) {

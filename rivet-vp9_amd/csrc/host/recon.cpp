@@ -451,6 +451,9 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   // instead of one launch per wavefront level
   const char *wf = getenv("VP9R_WAVEFRONT");
   g->wave_launch = wf && strcmp(wf, "one") == 0;
+  // VP9R_INTRA=groups: intra frames as (superblock, plane) groups
+  const char *im = getenv("VP9R_INTRA");
+  g->intra_blocks = !(im && strcmp(im, "groups") == 0);
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -862,7 +865,7 @@ static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_
     a.n_items = p.n_items;
     a.sync = wave_sync(g, 0);
     vp9g::launch_intra_wave(s, a, 1024);
-  } else if (p.n_items && !p.hdr.frame_is_intra) {
+  } else if (p.n_items && g->intra_blocks) {
     vp9g::IntraArgs a{};
     a.g = p.geo;
     a.blocks = (const vp9r_block *)(dv + p.o_blocks);
@@ -1031,7 +1034,7 @@ int vp9r_gpu_prepared_stats(const vp9r_prepared *pr, uint64_t out[12]) {
   for (int k = 0; k < 4; k++) out[8 + k] = p.work_bytes[k];
   out[0] = p.n_tiles;
   for (int k = 0; k < 4; k++) out[1 + k] = p.n_jobs[k];
-  const bool blocks = !p.hdr.frame_is_intra;
+  const bool blocks = true;
   out[5] = blocks ? p.n_items : p.n_groups;
   out[6] = blocks ? (p.item_level_start.empty() ? 0 : p.item_level_start.size() - 1)
                   : (p.group_level_start.empty() ? 0 : p.group_level_start.size() - 1);

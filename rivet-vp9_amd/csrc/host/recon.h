@@ -96,6 +96,7 @@ struct vp9r_gpu {
   // one-launch wavefronts: [0..3] tickets, [4] status, then the intra flags
   // (per 8x8 position) and the loop filter flags (per plane and superblock)
   bool wave_launch = false;
+  bool intra_blocks = true;  // intra as leveled blocks (else (superblock, plane) groups)
   vp9r::DevBuf wave;
   size_t wave_intra_cells = 0, wave_lf_items = 0;
   uint32_t epoch = 0;

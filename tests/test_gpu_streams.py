@@ -68,6 +68,29 @@ def test_stream_lockstep_one_launch(gpu_one_launch, name):
     assert err is None, f"{name}: {n} frames equal, then {err}"
 
 
+@pytest.fixture(scope="module")
+def gpu_groups():
+    old = os.environ.get("VP9R_INTRA")
+    os.environ["VP9R_INTRA"] = "groups"
+    try:
+        g = vp.Gpu(0)
+    finally:
+        if old is None:
+            del os.environ["VP9R_INTRA"]
+        else:
+            os.environ["VP9R_INTRA"] = old
+    return g
+
+
+@pytest.mark.parametrize("name", ["1080p-workload", "12bit-444", "8bit-422"])
+def test_stream_lockstep_intra_groups(gpu_groups, name):
+    """Intra as (superblock, plane) groups (VP9R_INTRA=groups)."""
+    print(f"stream {name} (intra groups)", flush=True)
+    frames = streamgen.generate(**CASES[name])
+    n, err = gpu_lockstep.run_frames(frames, gpu_groups)
+    assert err is None, f"{name}: {n} frames equal, then {err}"
+
+
 def test_prepared_run_equals_reconstruct(gpu):
     """vp9r_gpu_prepare + vp9r_gpu_run (the benchmark's path) gives the same
     frames as vp9r_gpu_reconstruct + vp9r_gpu_loop_filter, run twice."""
