@@ -179,6 +179,18 @@ int vp9r_gpu_prepare(vp9r_gpu *g, const vp9r_record *rec, vp9r_prepared **out);
 void vp9r_prepared_free(vp9r_prepared *p);
 int vp9r_gpu_run(vp9r_gpu *g, const vp9r_prepared *p, vp9r_gpu_frame *const refs[3],
                  vp9r_gpu_frame *cur);
+/* Runs n prepared records of independent frames (of any sizes and formats)
+ * together: each launch of the schedule covers every frame of the batch,
+ * so n frames cost one launch sequence instead of n.  refs holds 3 frame
+ * stores per record (refs[3 * i + k]; unused for intra frames), cur one.
+ * No frame of the batch may be a reference or the output of another frame
+ * in it; the results are those of n vp9r_gpu_run calls.  Frames that depend
+ * on each other go into successive batches.  This is the cross-stream
+ * batching of a multi-stream server (one ready frame per stream); the crate
+ * decodes one frame at a time (src/decoder/mod.rs:218-422) and has no
+ * counterpart. */
+int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *p,
+                       vp9r_gpu_frame *const *refs, vp9r_gpu_frame *const *cur);
 /* Work of a prepared record: [0] inter prediction tiles (<= 16x16), [1..4]
  * inter transform blocks of 4x4 .. 32x32, [5] intra blocks, [6] intra
  * dependency levels (one launch each), [7] loop filter wavefront launches,

@@ -167,6 +167,7 @@ def library():
     lib.vp9r_gpu_prepare.argtypes = [P, ctypes.POINTER(Record), PP]
     lib.vp9r_prepared_free.argtypes = [P]
     lib.vp9r_gpu_run.argtypes = [P, P, ctypes.c_void_p * 3, P]
+    lib.vp9r_gpu_run_batch.argtypes = [P, u32, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V)]
     lib.vp9r_gpu_prepared_stats.argtypes = [P, ctypes.POINTER(u64)]
     lib.vp9r_gpu_malloc.argtypes = [P, sz, PP]
     lib.vp9r_gpu_mfree.argtypes = [P, V]
@@ -289,6 +290,19 @@ class Gpu:
     def run(self, prepared, refs, cur):
         arr = (ctypes.c_void_p * 3)(*[(r.handle if r is not None else None) for r in refs])
         _check(library().vp9r_gpu_run(self._g, prepared.handle, arr, cur.handle))
+
+    def run_batch(self, jobs):
+        """Runs independent frames together (vp9r_gpu_run_batch): `jobs` is a
+        list of (prepared, refs, cur) as for run()."""
+        n = len(jobs)
+        ps = (ctypes.c_void_p * max(n, 1))(*[j[0].handle for j in jobs])
+        rs = (ctypes.c_void_p * max(3 * n, 1))()
+        for i, (_, refs, _) in enumerate(jobs):
+            for k in range(3):
+                r = refs[k] if refs else None
+                rs[3 * i + k] = r.handle if r is not None else None
+        cs = (ctypes.c_void_p * max(n, 1))(*[j[2].handle for j in jobs])
+        _check(library().vp9r_gpu_run_batch(self._g, n, ps, rs, cs))
 
     def buffer(self, nbytes):
         return DevBuffer(self, nbytes)

@@ -13,17 +13,22 @@
 
 namespace vp9g {
 
-// blockIdx.y: plane; blockIdx.x: first the rows below the 8x8-aligned area
-// (whole rows), then the rows of the area (their part right of it).
+// Per frame, workgroup j of count = 3 * rows: plane j / rows, and j % rows
+// counts first the rows below the 8x8-aligned area (whole rows), then the
+// rows of the area (their part right of it).
 template <class P>
-__global__ __launch_bounds__(256) void k_clear_margins(Planes f, FrameGeom g) {
-  const int plane = blockIdx.y;
+__global__ __launch_bounds__(256) void k_clear_margins(const ClearArgs *fa, const Range *rg, uint32_t nf) {
+  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const ClearArgs &a = fa[fi];
+  const uint32_t j = blockIdx.x - rg[fi].wg, per = rg[fi].count / 3;
+  const int plane = j / per;
+  const uint32_t r = j - plane * per;
+  const FrameGeom &g = a.g;
   const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
   const uint32_t mi_w = (g.mi_cols * 8) >> sx, mi_h = (g.mi_rows * 8) >> sy;
-  const uint32_t rows = f.rows[plane], stride = f.stride[plane];
+  const uint32_t rows = a.f.rows[plane], stride = a.f.stride[plane];
   const uint32_t n_bottom = rows - mi_h;
-  P *pl = (P *)f.p[plane];
-  const uint32_t r = blockIdx.x;
+  P *pl = (P *)a.f.p[plane];
   if (r < n_bottom) {
     P *row = pl + (size_t)(mi_h + r) * stride;
     for (uint32_t x = threadIdx.x; x < stride; x += 256) row[x] = 0;
@@ -33,7 +38,8 @@ __global__ __launch_bounds__(256) void k_clear_margins(Planes f, FrameGeom g) {
   }
 }
 
-void launch_clear_margins(hipStream_t s, const Planes &f, const FrameGeom &g) {
+// Workgroups per plane the margins of a frame need (count of its range).
+uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g) {
   uint32_t n = 0;
   for (int p = 0; p < 3; p++) {
     const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
@@ -42,11 +48,15 @@ void launch_clear_margins(hipStream_t s, const Planes &f, const FrameGeom &g) {
     if (f.stride[p] > mi_w) np += mi_h;
     n = np > n ? np : n;
   }
-  if (!n) return;
-  if (g.bit_depth > 8)
-    hipLaunchKernelGGL(k_clear_margins<uint16_t>, dim3(n, 3), dim3(256), 0, s, f, g);
+  return n;
+}
+
+void launch_clear_margins(hipStream_t s, const ClearArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
+  if (!wgs) return;
+  if (hbd)
+    hipLaunchKernelGGL(k_clear_margins<uint16_t>, dim3(wgs), dim3(256), 0, s, fa, r, n);
   else
-    hipLaunchKernelGGL(k_clear_margins<uint8_t>, dim3(n, 3), dim3(256), 0, s, f, g);
+    hipLaunchKernelGGL(k_clear_margins<uint8_t>, dim3(wgs), dim3(256), 0, s, fa, r, n);
 }
 
 }  // namespace vp9g

@@ -1,16 +1,17 @@
-// Intra prediction and reconstruction as a superblock wavefront.
+// Intra prediction and reconstruction as a dependency wavefront.
 //
-// Each workgroup owns one (superblock, plane) with intra work: it stages the
-// superblock and the row above / column left of it in LDS, then, for every
-// intra transform block in decode order, builds the edges
-// (src/decoder/recon.rs:17-103), predicts (src/dsp/intra.rs:19-175: each
-// directional mode in its per-sample closed form), dequantises and adds the
-// inverse transform (itx_dev.h), and finally writes the superblock back.  A
-// launch runs every group of one dependency level: a superblock's intra
-// blocks read the reconstruction of its left, above and above-left
-// neighbours only (above-right reads stay inside the block,
-// recon.rs:61-69), so the host orders groups by the longest chain of such
-// dependencies between superblocks that have intra work.
+// A workgroup owns one intra block (k_intra_blocks) or one (superblock,
+// plane) group (k_intra): it stages the region and the row above / column
+// left of it in LDS, then, for every transform block in decode order,
+// builds the edges (src/decoder/recon.rs:17-103), predicts
+// (src/dsp/intra.rs:19-175: each directional mode in its per-sample closed
+// form), dequantises and adds the inverse transform (itx_dev.h), and finally
+// writes the region back.  A launch runs every item of one dependency
+// level: intra blocks read the reconstruction of the blocks left, above and
+// above-left of them only (above-right reads stay inside the block,
+// recon.rs:61-69), so the host levels the items by the longest chain of
+// such dependencies between intra items (inter blocks are complete when
+// intra starts).  A launch may cover the items of several frames.
 #include <type_traits>
 
 #include "itx_dev.h"
@@ -238,14 +239,16 @@ DEV void intra_txb(const IntraArgs &a, IntraShared &sh, typename L::T *coef, con
 // ---- one wavefront level of (superblock, plane) groups per launch
 
 template <bool HBD>
-__global__ __launch_bounds__(64) void k_intra(IntraArgs a, uint32_t first_group) {
+__global__ __launch_bounds__(64) void k_intra(const IntraArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ IntraShared sh;
   __shared__ T coef[32 * 33];
   const int lane = threadIdx.x;
-  const IntraGroup gr = a.groups[first_group + blockIdx.x];
+  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const IntraArgs &a = fa[fi];
+  const IntraGroup gr = a.groups[rg[fi].first + blockIdx.x - rg[fi].wg];
   const FrameGeom &g = a.g;
   const int plane = gr.plane;
   const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
@@ -339,112 +342,30 @@ DEV void intra_block(const IntraArgs &a, IntraShared &sh, typename L::T *coef, c
 // is one level above the deepest intra block it reads)
 
 template <bool HBD>
-__global__ __launch_bounds__(64) void k_intra_blocks(IntraArgs a, uint32_t first) {
+__global__ __launch_bounds__(64) void k_intra_blocks(const IntraArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   __shared__ IntraShared sh;
   __shared__ T coef[32 * 33];
-  intra_block<L, HBD>(a, sh, coef, a.items[first + blockIdx.x]);
+  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const IntraArgs &a = fa[fi];
+  intra_block<L, HBD>(a, sh, coef, a.items[rg[fi].first + blockIdx.x - rg[fi].wg]);
 }
 
-void launch_intra_blocks(hipStream_t s, const IntraArgs &a, uint32_t first, uint32_t n) {
-  if (!n) return;
-  if (a.g.bit_depth > 8)
-    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(n), dim3(64), 0, s, a, first);
+void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
+  if (!wgs) return;
+  if (hbd)
+    hipLaunchKernelGGL(k_intra<true>, dim3(wgs), dim3(64), 0, s, fa, r, n);
   else
-    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(n), dim3(64), 0, s, a, first);
+    hipLaunchKernelGGL(k_intra<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
 }
 
-// ---- every intra block of the frame in one launch
-//
-// Workgroups take intra blocks in decode order (a topological order: a
-// block reads only samples of blocks decoded before it) through a ticket
-// counter.  A block reads the row above it (its above-left sample to its
-// top-right sample) and the column left of it; before it starts, it waits
-// until every intra block covering those 8x8 positions has published this
-// frame's epoch in the per-position flags (inter blocks are complete when
-// the launch starts).  Then it reconstructs its planes one after the other
-// in LDS, writes them back and publishes its own positions.
-
-DEV void intra_wait(const IntraArgs &a, uint32_t cell) {
-  uint32_t spins = 0;
-  while (__hip_atomic_load(a.sync.done + cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.sync.epoch) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++spins > kWaveSpinLimit) {
-      __hip_atomic_fetch_or(a.sync.status, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return;
-    }
-  }
-}
-
-template <bool HBD>
-__global__ __launch_bounds__(64) void k_intra_wave(IntraArgs a) {
-  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
-  typedef typename L::T T;
-  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ IntraShared sh;
-  __shared__ T coef[32 * 33];
-  __shared__ uint32_t slot;
-  const FrameGeom &g = a.g;
-  const int lane = threadIdx.x;
-  for (;;) {
-    if (lane == 0)
-      slot = __hip_atomic_fetch_add(a.sync.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const uint32_t t = slot;
-    __syncthreads();
-    if (t >= a.n_items) return;
-    const IntraItem it = a.items[t];
-    const vp9r_block &b = a.blocks[it.block];
-    const int mi_row = b.mi_row, mi_col = b.mi_col;
-    const int bw8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_wide(b.mi_size);
-    const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : num8x8_high(b.mi_size);
-    // wait for the intra blocks above (from above-left on) and to the left
-    {
-      int r = -1, c = -1;
-      if (lane <= bw8) {
-        r = mi_row - 1;
-        c = mi_col - 1 + lane;
-        if (!b.avail_u) r = -1;
-      } else if (lane <= bw8 + bh8) {
-        r = mi_row + lane - bw8 - 1;
-        c = mi_col - 1;
-        if (!b.avail_l) r = -1;
-      }
-      if (r >= 0 && c >= 0 && r < (int)g.mi_rows && c < (int)g.mi_cols) {
-        const uint32_t cell = (uint32_t)r * g.mi_cols + c;
-        if (a.mi[cell].ref_frame0 <= INTRA_FRAME) intra_wait(a, cell);
-      }
-    }
-    __syncthreads();
-    __threadfence();
-    intra_block<L, HBD>(a, sh, coef, it);
-    if (lane < bw8 * bh8) {
-      const int r = mi_row + lane / bw8, c = mi_col + lane % bw8;
-      if (r < (int)g.mi_rows && c < (int)g.mi_cols) {
-        __threadfence();
-        __hip_atomic_store(a.sync.done + (uint32_t)r * g.mi_cols + c, a.sync.epoch, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-  }
-}
-
-void launch_intra_wave(hipStream_t s, const IntraArgs &a, uint32_t n_groups) {
-  if (!a.n_items) return;
-  const uint32_t n = min(a.n_items, n_groups);
-  if (a.g.bit_depth > 8)
-    hipLaunchKernelGGL(k_intra_wave<true>, dim3(n), dim3(64), 0, s, a);
+void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
+  if (!wgs) return;
+  if (hbd)
+    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(wgs), dim3(64), 0, s, fa, r, n);
   else
-    hipLaunchKernelGGL(k_intra_wave<false>, dim3(n), dim3(64), 0, s, a);
-}
-
-void launch_intra(hipStream_t s, const IntraArgs &a, uint32_t first_group, uint32_t n_groups) {
-  if (!n_groups) return;
-  if (a.g.bit_depth > 8)
-    hipLaunchKernelGGL(k_intra<true>, dim3(n_groups), dim3(64), 0, s, a, first_group);
-  else
-    hipLaunchKernelGGL(k_intra<false>, dim3(n_groups), dim3(64), 0, s, a, first_group);
+    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
 }
 
 // Per-kernel seam: one intra::predict call (intra.rs:19-29) on host-given

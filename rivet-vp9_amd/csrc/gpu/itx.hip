@@ -10,16 +10,21 @@
 
 namespace vp9g {
 
+// Per frame of the batch, its range's workgroups take BPW consecutive
+// jobs each (count = the frame's jobs of this size).
 template <int n, bool HBD>
-__global__ __launch_bounds__(64) void k_itx_inter(ItxArgs a, uint32_t n_jobs) {
+__global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   constexpr int n0 = 1 << n, BPW = 64 / n0, S = n0 + 1;
   __shared__ T tiles[BPW][n0 * S];
+  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const ItxArgs &a = fa[fi];
   const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
-  const uint32_t job = blockIdx.x * BPW + team;
-  const bool active = job < n_jobs;
+  const uint32_t local = (blockIdx.x - rg[fi].wg) * BPW + team;
+  const bool active = local < rg[fi].count;
+  const uint32_t job = rg[fi].first + local;
   T *tile = tiles[team];
 #pragma unroll
   for (int k = 0; k < S; k++) tile[lane * S + k] = 0;
@@ -27,7 +32,7 @@ __global__ __launch_bounds__(64) void k_itx_inter(ItxArgs a, uint32_t n_jobs) {
   vp9r_txb t = {};
   int32_t dcq = 0, acq = 0;
   if (active) {
-    t = a.txbs[a.jobs[job]];
+    t = a.txbs[a.jobs[n - 2][job]];
     const int seg = a.blocks[t.block].segment_id;
     const int pt = t.plane > 0;
     dcq = a.q[seg][pt][0];
@@ -85,22 +90,21 @@ __global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const ui
 }
 
 template <int n>
-static void itx_inter_n(hipStream_t s, const ItxArgs &a, uint32_t n_jobs) {
-  constexpr int BPW = 64 >> n;
-  dim3 grid((n_jobs + BPW - 1) / BPW);
-  if (a.g.bit_depth > 8)
-    hipLaunchKernelGGL((k_itx_inter<n, true>), grid, dim3(64), 0, s, a, n_jobs);
+static void itx_inter_n(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t nf, uint32_t wgs, bool hbd) {
+  if (hbd)
+    hipLaunchKernelGGL((k_itx_inter<n, true>), dim3(wgs), dim3(64), 0, s, fa, r, nf);
   else
-    hipLaunchKernelGGL((k_itx_inter<n, false>), grid, dim3(64), 0, s, a, n_jobs);
+    hipLaunchKernelGGL((k_itx_inter<n, false>), dim3(wgs), dim3(64), 0, s, fa, r, nf);
 }
 
-void launch_itx_inter(hipStream_t s, const ItxArgs &a, int tx_size, uint32_t n_jobs) {
-  if (!n_jobs) return;
+void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t nf, uint32_t wgs, int tx_size,
+                      bool hbd) {
+  if (!wgs) return;
   switch (tx_size) {
-    case 0: itx_inter_n<2>(s, a, n_jobs); break;
-    case 1: itx_inter_n<3>(s, a, n_jobs); break;
-    case 2: itx_inter_n<4>(s, a, n_jobs); break;
-    default: itx_inter_n<5>(s, a, n_jobs); break;
+    case 0: itx_inter_n<2>(s, fa, r, nf, wgs, hbd); break;
+    case 1: itx_inter_n<3>(s, fa, r, nf, wgs, hbd); break;
+    case 2: itx_inter_n<4>(s, fa, r, nf, wgs, hbd); break;
+    default: itx_inter_n<5>(s, fa, r, nf, wgs, hbd); break;
   }
 }
 

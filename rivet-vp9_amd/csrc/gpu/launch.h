@@ -33,6 +33,33 @@ struct McTile {
   uint8_t tx, ty, tw, th;
 };
 
+// Cross-frame batching.  One launch covers the work items of several
+// frames (of different streams): frame i of a launch takes workgroups
+// [r[i].wg, r[i + 1].wg) and, in its own item list, the items
+// [r[i].first, r[i].first + r[i].count); r[n].wg is the launch's workgroup
+// count.  Per-frame arguments are arrays in device memory indexed by frame.
+struct Range {
+  uint32_t wg, first, count, _pad;
+};
+
+// The frame of workgroup w (r sorted by wg; n small).
+__device__ __forceinline__ uint32_t range_frame(const Range *r, uint32_t n, uint32_t w) {
+  uint32_t lo = 0, hi = n;  // r[lo].wg <= w < r[hi].wg
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (r[mid].wg <= w)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct ClearArgs {
+  Planes f;
+  FrameGeom g;
+};
+
 struct McArgs {
   FrameGeom g;
   const vp9r_block *blocks;
@@ -48,7 +75,7 @@ struct ItxArgs {
   FrameGeom g;
   const vp9r_block *blocks;
   const vp9r_txb *txbs;
-  const uint32_t *jobs;  // txb indices of this size class
+  const uint32_t *jobs[4];  // txb indices per size class
   const uint16_t *coef_pos;
   const int32_t *coef_val;
   Planes cur;
@@ -63,22 +90,8 @@ struct IntraGroup {
   uint32_t first, count;  // into the intra txb index list
 };
 
-// Device-side scheduling of a wavefront in one launch: tickets hand out
-// work items in a topological order, and each item's completion flag holds
-// the epoch (frame number) that completed it.
-struct WaveSync {
-  uint32_t *ticket;  // zero at launch
-  uint32_t *done;    // per item: epoch of the last frame that completed it
-  uint32_t *status;  // set to nonzero when a wait exceeds its spin limit
-  uint32_t epoch;    // nonzero
-};
-
-// Polls of a completion flag before a wait gives up (with s_sleep between
-// polls: on the order of a second).
-constexpr uint32_t kWaveSpinLimit = 1u << 22;
-
-// An intra block of the one-launch intra wavefront: its transform blocks
-// are txbs[first .. first + count) (planes in order).
+// An intra block: its transform blocks are txbs[first .. first + count)
+// (planes in order).
 struct IntraItem {
   uint32_t block, first, count;
 };
@@ -87,16 +100,13 @@ struct IntraArgs {
   FrameGeom g;
   const vp9r_block *blocks;
   const vp9r_txb *txbs;
-  const uint32_t *txb_index;
-  const IntraGroup *groups;
+  const uint32_t *txb_index;  // groups: the intra txbs of each group
+  const IntraGroup *groups;   // by level
+  const IntraItem *items;     // intra blocks by level
   const uint16_t *coef_pos;
   const int32_t *coef_val;
   Planes cur;
   int32_t q[8][2][2];
-  const vp9r_mi *mi;         // one-launch wavefront: which 8x8 positions are intra
-  const IntraItem *items;    // one-launch wavefront: intra blocks in decode order
-  uint32_t n_items;
-  WaveSync sync;             // flags per 8x8 position
 };
 
 // Loop filter: per superblock, plane and pass, 16 x 16 entries of filter
@@ -109,24 +119,24 @@ struct LfParams {
 struct LfArgs {
   FrameGeom g;
   const vp9r_mi *mi;
-  int8_t *mask_fs;   // [plane][pass][sb][16][16] (level launches)
+  int8_t *mask_fs;   // [plane][pass][sb][16][16]
   uint8_t *mask_lvl;
   Planes cur;
   LfParams p;
-  const uint32_t *sbs;  // (plane << 28 | sb_row << 14 | sb_col): per level, or all in order
-  uint32_t n_items;     // one-launch wavefront: entries of sbs
-  WaveSync sync;        // one-launch wavefront: flags per (plane, superblock)
+  const uint32_t *sbs;  // (plane << 28 | sb_row << 14 | sb_col), by level c + 2r
 };
 
-void launch_clear_margins(hipStream_t s, const Planes &f, const FrameGeom &g);
-void launch_mc(hipStream_t s, const McArgs &a, uint32_t n_tiles);
-void launch_itx_inter(hipStream_t s, const ItxArgs &a, int tx_size, uint32_t n_jobs);
-void launch_intra(hipStream_t s, const IntraArgs &a, uint32_t first_group, uint32_t n_groups);
-void launch_intra_wave(hipStream_t s, const IntraArgs &a, uint32_t n_groups);
-void launch_intra_blocks(hipStream_t s, const IntraArgs &a, uint32_t first, uint32_t n);
-void launch_lf_mask(hipStream_t s, const LfArgs &a);
-void launch_lf_level(hipStream_t s, const LfArgs &a, uint32_t first, uint32_t n);
-void launch_lf_wave(hipStream_t s, const LfArgs &a, uint32_t n_groups);
+// Launchers: fa = per-frame arguments, r = ranges (n + 1 entries), all in
+// device memory; wgs = r[n].wg; hbd: samples are u16.
+uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g);
+void launch_clear_margins(hipStream_t s, const ClearArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int tx_size,
+                      bool hbd);
+void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs);
+void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
 
 // Per-kernel seam (vp9r.h batched forms).
 void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,

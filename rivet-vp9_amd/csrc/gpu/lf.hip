@@ -173,10 +173,14 @@ DEV void lf_decide(const LfArgs &a, int plane, int pass, uint32_t sb_row, uint32
   lvl_out = l;
 }
 
-__global__ __launch_bounds__(256) void k_lf_mask(LfArgs a) {
+// Per frame, workgroup j of 6 * n_sb: superblock j % n_sb, pass and plane
+// from j / n_sb.
+__global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *rg, uint32_t nf) {
+  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const LfArgs &a = fa[fi];
   const FrameGeom &g = a.g;
-  const int sb = blockIdx.x, pass = blockIdx.y, plane = blockIdx.z;
   const int n_sb = g.sb64_rows * g.sb64_cols;
+  const int j = blockIdx.x - rg[fi].wg, sb = j % n_sb, pp = j / n_sb, pass = pp & 1, plane = pp >> 1;
   int fs, lvl;
   lf_decide(a, plane, pass, sb / g.sb64_cols, sb % g.sb64_cols, threadIdx.x >> 4, threadIdx.x & 15, fs, lvl);
   const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256 + threadIdx.x;
@@ -188,98 +192,8 @@ __global__ __launch_bounds__(256) void k_lf_mask(LfArgs a) {
 
 struct LfShared {
   uint16_t buf[72 * BP];  // the superblock and its 8-sample halo above and left
-  int8_t fs[2][256];
-  uint8_t lv[2][256];
   uint32_t dq[2][256];    // packed decisions for lf_stream
 };
-
-// The steps on one superblock of one plane: the region is the superblock
-// with its 8-sample halo above and left, staged in LDS (rows [r0, r1) of
-// it by lf_load); the decisions are in sh.fs / sh.lv.
-struct LfRegion {
-  int x0, y0, SW, SH, RW, RH;
-  uint32_t stride;
-};
-
-DEV LfRegion lf_region(const LfArgs &a, int plane, int sb_row, int sb_col) {
-  const int sx = plane ? a.g.ss_x : 0, sy = plane ? a.g.ss_y : 0;
-  LfRegion g;
-  g.SW = 64 >> sx;
-  g.SH = 64 >> sy;
-  g.x0 = sb_col * g.SW;
-  g.y0 = sb_row * g.SH;
-  g.RW = g.SW + 8;
-  g.RH = g.SH + 8;
-  g.stride = a.cur.stride[plane];
-  return g;
-}
-
-template <class P>
-DEV void lf_load(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g, int r0, int r1) {
-  const P *pl = (const P *)a.cur.p[plane];
-  const int n = (r1 - r0) * g.RW;
-  for (int k = threadIdx.x; k < n; k += 64) {
-    int r = r0 + k / g.RW, c = k - (k / g.RW) * g.RW;
-    int py = g.y0 - 8 + r, px = g.x0 - 8 + c;
-    if (py >= 0 && px >= 0) sh.buf[r * BP + c] = pl[(size_t)py * g.stride + px];
-  }
-}
-
-template <class P>
-DEV void lf_store(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g) {
-  P *pl = (P *)a.cur.p[plane];
-  for (int k = threadIdx.x; k < g.RW * g.RH; k += 64) {
-    int r = k / g.RW, c = k - r * g.RW;
-    int py = g.y0 - 8 + r, px = g.x0 - 8 + c;
-    if (py >= 0 && px >= 0) pl[(size_t)py * g.stride + px] = (P)sh.buf[r * BP + c];
-  }
-}
-
-// Vertical edges, left to right, one lane per row (touches rows 8 .. of the
-// region only).
-DEV void lf_vpass(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g) {
-  const int lane = threadIdx.x;
-  const int sx = plane ? a.g.ss_x : 0;
-  if (lane < g.SH) {
-    const int r = 8 + lane, rn = lane >> 2, n_edges = 16 >> sx;
-    for (int ed = 0; ed < n_edges; ed++) {
-      const int f = sh.fs[0][ed * 16 + rn];
-      if (f >= 0) {
-        const int l = sh.lv[0][ed * 16 + rn];
-        lf_filter(sh.buf, r * BP + 8 + 4 * ed, 1, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l], a.g.bit_depth);
-      }
-    }
-  }
-}
-
-// Horizontal edges, top to bottom, one lane per column.
-DEV void lf_hpass(const LfArgs &a, LfShared &sh, int plane, const LfRegion &g) {
-  const int lane = threadIdx.x;
-  const int sy = plane ? a.g.ss_y : 0;
-  if (lane < g.SW) {
-    const int c = 8 + lane, rn = lane >> 2, n_edges = 16 >> sy;
-    for (int ed = 0; ed < n_edges; ed++) {
-      const int f = sh.fs[1][ed * 16 + rn];
-      if (f >= 0) {
-        const int l = sh.lv[1][ed * 16 + rn];
-        lf_filter(sh.buf, (8 + 4 * ed) * BP + c, BP, f, a.p.limit[l], a.p.blimit[l], a.p.thresh[l],
-                  a.g.bit_depth);
-      }
-    }
-  }
-}
-
-template <class P>
-DEV void lf_superblock(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
-  const LfRegion g = lf_region(a, plane, sb_row, sb_col);
-  lf_load<P>(a, sh, plane, g, 0, g.RH);
-  __syncthreads();
-  lf_vpass(a, sh, plane, g);
-  __syncthreads();
-  lf_hpass(a, sh, plane, g);
-  __syncthreads();
-  lf_store<P>(a, sh, plane, g);
-}
 
 // ---- one superblock of one plane, streamed through a register window
 //
@@ -419,14 +333,16 @@ DEV void lf_load_params(const LfArgs &a, uint32_t *prm) {
 // ---- one wavefront level per launch (decisions from k_lf_mask)
 
 template <bool HBD>
-__global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
+__global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ LfShared sh;
   __shared__ uint32_t prm[64];
+  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const LfArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
   lf_load_params(a, prm);
-  const uint32_t code = a.sbs[first + blockIdx.x];
+  const uint32_t code = a.sbs[rg[fi].first + blockIdx.x - rg[fi].wg];
   const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
   const int n_sb = g.sb64_rows * g.sb64_cols, sb = sb_row * g.sb64_cols + sb_col;
   __syncthreads();
@@ -441,105 +357,17 @@ __global__ __launch_bounds__(64) void k_lf(LfArgs a, uint32_t first) {
   lf_sb<P>(a, sh, plane, sb_row, sb_col);
 }
 
-// ---- the whole frame in one launch
-//
-// Workgroups take superblocks (all planes) in the order of a.sbs, which is
-// topological for the dependencies above (c + 2r ascending), through a
-// ticket counter; each computes its decisions, waits until the superblocks
-// it depends on carry this frame's epoch in their completion flags, filters,
-// and publishes its own flag.  A workgroup only ever waits on tickets taken
-// before its own, by workgroups already running, so every wait ends; a
-// bounded spin turns a wait that does not end into an error status instead
-// of a hang.
-
-DEV void wave_wait(const WaveSync &s, uint32_t idx) {
-  uint32_t spins = 0;
-  while (__hip_atomic_load(s.done + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != s.epoch) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++spins > kWaveSpinLimit) {
-      __hip_atomic_fetch_or(s.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return;
-    }
-  }
+void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs) {
+  if (!wgs) return;
+  hipLaunchKernelGGL(k_lf_mask, dim3(wgs), dim3(256), 0, s, fa, r, n);
 }
 
-DEV void wave_publish(const WaveSync &s, uint32_t idx) {
-  __threadfence();
-  __hip_atomic_store(s.done + idx, s.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-DEV uint32_t wave_ticket(const WaveSync &s, uint32_t *slot) {
-  if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(s.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const uint32_t t = *slot;
-  __syncthreads();
-  return t;
-}
-
-template <bool HBD>
-__global__ __launch_bounds__(64) void k_lf_wave(LfArgs a) {
-  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ LfShared sh;
-  __shared__ uint32_t slot;
-  const FrameGeom &g = a.g;
-  const int lane = threadIdx.x;
-  const uint32_t n_sb = g.sb64_rows * g.sb64_cols, cols = g.sb64_cols;
-  for (;;) {
-    const uint32_t t = wave_ticket(a.sync, &slot);
-    if (t >= a.n_items) return;
-    const uint32_t code = a.sbs[t];
-    const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
-    for (int k = lane; k < 512; k += 64) {
-      int fs, lvl;
-      lf_decide(a, plane, k >> 8, sb_row, sb_col, (k >> 4) & 15, k & 15, fs, lvl);
-      sh.fs[k >> 8][k & 255] = (int8_t)fs;
-      sh.lv[k >> 8][k & 255] = (uint8_t)lvl;
-    }
-    // The vertical pass needs (r, c - 1) (and so everything up and left of
-    // it); the horizontal pass also needs (r - 1, c + 1), whose vertical
-    // edges reach 8 columns into the top halo: the halo rows are loaded
-    // again after that wait.
-    const uint32_t base = plane * n_sb;
-    if (lane == 0 && sb_col > 0) wave_wait(a.sync, base + sb_row * cols + sb_col - 1);
-    __syncthreads();
-    __threadfence();
-    const LfRegion rg = lf_region(a, plane, sb_row, sb_col);
-    lf_load<P>(a, sh, plane, rg, 8, rg.RH);
-    __syncthreads();
-    lf_vpass(a, sh, plane, rg);
-    if (lane == 0 && sb_row > 0) wave_wait(a.sync, base + (sb_row - 1) * cols + min(sb_col + 1, (int)cols - 1));
-    __syncthreads();
-    __threadfence();
-    lf_load<P>(a, sh, plane, rg, 0, 8);
-    __syncthreads();
-    lf_hpass(a, sh, plane, rg);
-    __syncthreads();
-    lf_store<P>(a, sh, plane, rg);
-    __syncthreads();
-    if (lane == 0) wave_publish(a.sync, base + sb_row * cols + sb_col);
-  }
-}
-
-void launch_lf_mask(hipStream_t s, const LfArgs &a) {
-  dim3 grid(a.g.sb64_rows * a.g.sb64_cols, 2, 3);
-  hipLaunchKernelGGL(k_lf_mask, grid, dim3(256), 0, s, a);
-}
-
-void launch_lf_level(hipStream_t s, const LfArgs &a, uint32_t first, uint32_t n) {
-  if (!n) return;
-  if (a.g.bit_depth > 8)
-    hipLaunchKernelGGL(k_lf<true>, dim3(n), dim3(64), 0, s, a, first);
+void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
+  if (!wgs) return;
+  if (hbd)
+    hipLaunchKernelGGL(k_lf<true>, dim3(wgs), dim3(64), 0, s, fa, r, n);
   else
-    hipLaunchKernelGGL(k_lf<false>, dim3(n), dim3(64), 0, s, a, first);
-}
-
-void launch_lf_wave(hipStream_t s, const LfArgs &a, uint32_t n_groups) {
-  if (!a.n_items) return;
-  const uint32_t n = min(a.n_items, n_groups);
-  if (a.g.bit_depth > 8)
-    hipLaunchKernelGGL(k_lf_wave<true>, dim3(n), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_lf_wave<false>, dim3(n), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_lf<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
 }
 
 // Per-kernel seam: lf::filter_edges on global samples, one pass per

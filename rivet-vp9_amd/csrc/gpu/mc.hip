@@ -107,10 +107,12 @@ DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, 
 }
 
 template <bool HBD>
-__global__ __launch_bounds__(64) void k_mc(McArgs a) {
+__global__ __launch_bounds__(64) void k_mc(const McArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ McShared sh;
-  const McTile t = a.tiles[blockIdx.x];
+  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const McArgs &a = fa[fi];
+  const McTile t = a.tiles[rg[fi].first + blockIdx.x - rg[fi].wg];
   const vp9r_block &b = a.blocks[t.block];
   const int max = (1 << a.g.bit_depth) - 1;
   const bool compound = b.ref_frame[1] > INTRA_FRAME;
@@ -171,13 +173,13 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
   }
 }
 
-void launch_mc(hipStream_t s, const McArgs &a, uint32_t n_tiles) {
+void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
   upload_filters();
-  if (!n_tiles) return;
-  if (a.g.bit_depth > 8)
-    hipLaunchKernelGGL(k_mc<true>, dim3(n_tiles), dim3(64), 0, s, a);
+  if (!wgs) return;
+  if (hbd)
+    hipLaunchKernelGGL(k_mc<true>, dim3(wgs), dim3(64), 0, s, fa, r, n);
   else
-    hipLaunchKernelGGL(k_mc<false>, dim3(n_tiles), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_mc<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
 }
 
 void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,

@@ -1,13 +1,16 @@
 // GPU reconstruction, host side: frame stores, per-frame work lists and the
-// launch sequence.  One record in, on one HIP stream:
-//   1. one staged host-to-device copy of the record and its work lists;
-//   2. inter prediction of every inter block (one launch: it reads only the
-//      references);
+// launch sequence.  A batch of records (one frame, or independent frames of
+// several streams) goes in on one HIP stream:
+//   1. one staged host-to-device copy per record of its inputs and work
+//      lists, and one copy of the batch's kernel arguments and ranges;
+//   2. margin clearing and inter prediction of every inter block (one launch
+//      each: inter prediction reads only the references);
 //   3. inter residuals, one launch per transform size;
-//   4. intra prediction + residual as a superblock wavefront, one launch per
-//      dependency level (only superblocks with intra work take part);
-// and then, in vp9r_gpu_loop_filter, the loop filter decisions (one launch)
-// and the filter wavefront, one launch per level c + 2r.
+//   4. intra prediction + residual, one launch per dependency level;
+//   5. the loop filter decisions (one launch) and the filter wavefront, one
+//      launch per superblock level c + 2r.
+// Every launch covers all frames of the batch: a workgroup finds its frame
+// in the launch's range table.
 // This is the order of the reference's replay (src/decoder/block.rs:1405-1498)
 // up to reorderings that change no sample: inter pixels depend only on
 // references, so they may all precede the intra chain, which reads
@@ -206,8 +209,8 @@ static void build_intra(vp9r_gpu *g, const vp9r_record *rec) {
     uint32_t k = key_of(t);
     g->intra_index[g->intra_first[k] + g->intra_fill[k]++] = i;
   }
-  // the intra blocks in decode order, for the one-launch wavefront (the
-  // record keeps a block's transform blocks together, planes in order)
+  // the intra blocks in decode order (the record keeps a block's transform
+  // blocks together, planes in order)
   g->intra_items.clear();
   for (uint32_t i = 0; i < rec->n_txbs; i++) {
     const vp9r_txb &t = rec->txbs[i];
@@ -385,15 +388,15 @@ hipError_t fold_stage_times(vp9r_gpu *g) {
     }
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[4]);
     g->stage_acc[4] += ms;
-    g->frames_acc++;
+    g->frames_acc += g->ev_frames[k];
   }
   g->ev_used = 0;
   g->ev = nullptr;
   return e;
 }
 
-// Takes the event set of the next timed frame.
-hipError_t begin_timed_frame(vp9r_gpu *g) {
+// Takes the event set of the next timed call, which covers `frames` frames.
+hipError_t begin_timed_frame(vp9r_gpu *g, uint32_t frames) {
   if (g->ev_used == 1024) {
     hipError_t e = fold_stage_times(g);
     if (e != hipSuccess) return e;
@@ -405,23 +408,11 @@ hipError_t begin_timed_frame(vp9r_gpu *g) {
       if (e != hipSuccess) return e;
     }
     g->ev_sets.push_back(set);
+    g->ev_frames.push_back(0);
   }
+  g->ev_frames[g->ev_used] = frames;
   g->ev = g->ev_sets[g->ev_used++].data();
   return hipSuccess;
-}
-
-// The one-launch wavefronts' status word: nonzero when a wait gave up
-// (which leaves that frame wrong); read after the stream is idle.
-int check_wave_status(vp9r_gpu *g) {
-  if (!g->wave.p) return VP9R_OK;
-  uint32_t st = 0;
-  HIP_TRY(hipMemcpy(&st, (uint32_t *)g->wave.p + 4, 4, hipMemcpyDeviceToHost), "read wavefront status");
-  if (st) {
-    HIP_TRY(hipMemset((uint32_t *)g->wave.p + 4, 0, 4), "clear wavefront status");
-    set_error("a device wavefront wait exceeded its spin limit (status " + std::to_string(st) + ")");
-    return VP9R_ERR_DEVICE;
-  }
-  return VP9R_OK;
 }
 
 // Orders work on `s` that writes frame f after f's download in flight, if any.
@@ -447,10 +438,14 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
   vp9r_gpu *g = new vp9r_gpu();
   g->device = device;
-  // VP9R_WAVEFRONT=one: the one-launch wavefronts (device-side waits)
-  // instead of one launch per wavefront level
+  // The one-launch schedule (device-side waits between wavefront levels)
+  // hung on its first key frame and was removed; asking for it is an error.
   const char *wf = getenv("VP9R_WAVEFRONT");
-  g->wave_launch = wf && strcmp(wf, "one") == 0;
+  if (wf && strcmp(wf, "one") == 0) {
+    delete g;
+    set_error("VP9R_WAVEFRONT=one: the one-launch wavefront schedule was removed");
+    return VP9R_ERR_UNSUPPORTED;
+  }
   // VP9R_INTRA=groups: intra frames as (superblock, plane) groups
   const char *im = getenv("VP9R_INTRA");
   g->intra_blocks = !(im && strcmp(im, "groups") == 0);
@@ -461,7 +456,13 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
     delete g;
     return hip_fail(e, "hipStreamCreate");
   }
-  HIP_TRY(hipEventCreateWithFlags(&g->ev_upload, hipEventDisableTiming), "hipEventCreate");
+  e = hipEventCreateWithFlags(&g->ev_upload, hipEventDisableTiming);
+  for (int i = 0; i < kBatchSlots && e == hipSuccess; i++)
+    e = hipEventCreateWithFlags(&g->slots[i].ev, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    vp9r_gpu_free(g);
+    return hip_fail(e, "hipEventCreate");
+  }
   *out = g;
   return VP9R_OK;
 }
@@ -471,7 +472,9 @@ void vp9r_gpu_free(vp9r_gpu *g) {
   (void)hipSetDevice(g->device);
   (void)hipStreamSynchronize(g->stream);
   (void)hipStreamSynchronize(g->copy_stream);
-  (void)hipEventDestroy(g->ev_upload);
+  if (g->ev_upload) (void)hipEventDestroy(g->ev_upload);
+  for (auto &s : g->slots)
+    if (s.ev) (void)hipEventDestroy(s.ev);
   for (auto &set : g->ev_sets)
     for (auto ev : set) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(g->stream);
@@ -577,7 +580,7 @@ int vp9r_gpu_frame_download_u16(vp9r_gpu_frame *f, int plane, uint16_t *dst) {
 int vp9r_gpu_synchronize(vp9r_gpu *g) {
   HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
   HIP_TRY(hipStreamSynchronize(g->stream), "hipStreamSynchronize");
-  return vp9r::check_wave_status(g);
+  return VP9R_OK;
 }
 
 void vp9r_gpu_set_timing(vp9r_gpu *g, int enable) {
@@ -784,154 +787,225 @@ static int check_frames(const Prepared &p, vp9r_gpu_frame *const refs[3], vp9r_g
   return VP9R_OK;
 }
 
-// Sizes the one-launch wavefront buffer for the frame (zeroing it when it
-// grows: zero is no frame's epoch), clears the tickets and advances the epoch.
-static int begin_wave_frame(vp9r_gpu *g, const Prepared &p, hipStream_t s) {
-  const size_t cells = (size_t)p.hdr.mi_rows * p.hdr.mi_cols;
-  const size_t lf_items = 3 * (size_t)p.hdr.sb64_rows * p.hdr.sb64_cols;
-  const size_t need = 4 * (8 + cells + lf_items);
-  if (need > g->wave.cap) {
-    HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
-    HIP_TRY(g->wave.ensure(need + need / 4), "hipMalloc(wavefront flags)");
-    HIP_TRY(hipMemsetAsync(g->wave.p, 0, g->wave.cap, s), "hipMemset(wavefront flags)");
-  } else {
-    HIP_TRY(hipMemsetAsync(g->wave.p, 0, 16, s), "hipMemset(tickets)");
+// ---- batched launches
+//
+// Per bit-depth class (8-bit and above share no kernel instance), the
+// frames' arguments and the ranges of every launch are staged in one pinned
+// buffer, copied to a device slot with one transfer, and the launches read
+// them there.  Slots form a ring; a slot is reused once its last launch
+// has finished (its event).
+
+namespace {
+
+template <class T>
+size_t stage_take(std::vector<uint8_t> &buf, size_t n) {
+  size_t o = (buf.size() + 255) & ~(size_t)255;
+  buf.resize(o + sizeof(T) * n);
+  return o;
+}
+
+// The ranges of one launch: frame i takes ceil(count_i / per_wg) workgroups.
+struct LaunchTable {
+  size_t off = 0;        // in the staging buffer
+  uint32_t wgs = 0;
+};
+
+LaunchTable add_ranges(std::vector<uint8_t> &buf, int n, const uint32_t *first, const uint32_t *count,
+                       uint32_t per_wg) {
+  LaunchTable t;
+  t.off = stage_take<vp9g::Range>(buf, n + 1);
+  vp9g::Range *r = (vp9g::Range *)(buf.data() + t.off);
+  uint32_t wg = 0;
+  for (int i = 0; i < n; i++) {
+    r[i].wg = wg;
+    r[i].first = first[i];
+    r[i].count = count[i];
+    r[i]._pad = 0;
+    wg += (count[i] + per_wg - 1) / per_wg;
   }
-  g->epoch = g->epoch + 1 ? g->epoch + 1 : 1;
-  return VP9R_OK;
+  r[n].wg = wg;
+  r[n].first = r[n].count = r[n]._pad = 0;
+  t.wgs = wg;
+  return t;
 }
 
-static vp9g::WaveSync wave_sync(vp9r_gpu *g, int which) {
-  uint32_t *w = (uint32_t *)g->wave.p;
-  vp9g::WaveSync ws{};
-  ws.ticket = w + which;
-  ws.status = w + 4;
-  ws.done = which == 0 ? w + 8 : w + 8 + (size_t)g->wave_intra_cells;
-  ws.epoch = g->epoch;
-  return ws;
-}
+}  // namespace
 
-// Reconstruction launches from device inputs at dv (see the file comment).
-static int launch_recon(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_gpu_frame *const refs[3],
-                        vp9r_gpu_frame *cur) {
+int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool lf) {
   hipStream_t s = g->stream;
-  HIP_TRY(order_after_copy(s, cur), "hipStreamWaitEvent");
-  if (g->wave_launch) {
-    int st = begin_wave_frame(g, p, s);
-    if (st) return st;
-    g->wave_intra_cells = (size_t)p.hdr.mi_rows * p.hdr.mi_cols;
-  }
-  vp9g::launch_clear_margins(s, cur->planes, p.geo);
-  if (p.n_tiles) {
-    vp9g::McArgs a{};
-    a.g = p.geo;
-    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
-    a.tiles = (const vp9g::McTile *)(dv + p.o_tiles);
-    a.cur = cur->planes;
-    for (int i = 0; i < 3; i++) {
-      a.ref[i] = refs[i]->planes;
-      a.ref_w[i] = refs[i]->width;
-      a.ref_h[i] = refs[i]->height;
+  for (int cls = 0; cls < 2; cls++) {
+    const bool hbd = cls == 1;
+    std::vector<const BatchFrame *> f;
+    for (int i = 0; i < n_all; i++)
+      if ((fr[i].p->hdr.bit_depth > 8) == hbd) f.push_back(&fr[i]);
+    const int n = (int)f.size();
+    if (!n) continue;
+    std::vector<uint8_t> &st = g->batch_stage;
+    st.clear();
+    const size_t o_clear = stage_take<vp9g::ClearArgs>(st, n), o_mc = stage_take<vp9g::McArgs>(st, n),
+                 o_itx = stage_take<vp9g::ItxArgs>(st, n), o_intra = stage_take<vp9g::IntraArgs>(st, n),
+                 o_lf = stage_take<vp9g::LfArgs>(st, n);
+    // LF decisions: per frame n_sb * 6 * 256 bytes of sizes and of levels
+    size_t mask_bytes = 0;
+    std::vector<size_t> mask_off(n);
+    for (int i = 0; i < n; i++) {
+      mask_off[i] = mask_bytes;
+      mask_bytes += (size_t)f[i]->p->hdr.sb64_rows * f[i]->p->hdr.sb64_cols * 6 * 256 * 2;
     }
-    vp9g::launch_mc(s, a, p.n_tiles);
-  }
-  if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
-  {
-    vp9g::ItxArgs a{};
-    a.g = p.geo;
-    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
-    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
-    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
-    a.coef_val = (const int32_t *)(dv + p.o_cval);
-    a.cur = cur->planes;
-    memcpy(a.q, p.q, sizeof p.q);
-    for (int k = 0; k < 4; k++) {
-      a.jobs = (const uint32_t *)(dv + p.o_jobs[k]);
-      vp9g::launch_itx_inter(s, a, k, p.n_jobs[k]);
+    if (lf && mask_bytes > g->lf_mask.cap) {
+      HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+      HIP_TRY(g->lf_mask.ensure(mask_bytes), "hipMalloc(lf masks)");
     }
-  }
-  if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
-  if (p.n_items && g->wave_launch) {
-    vp9g::IntraArgs a{};
-    a.g = p.geo;
-    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
-    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
-    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
-    a.coef_val = (const int32_t *)(dv + p.o_cval);
-    a.cur = cur->planes;
-    memcpy(a.q, p.q, sizeof p.q);
-    a.mi = (const vp9r_mi *)(dv + p.o_mi);
-    a.items = (const vp9g::IntraItem *)(dv + p.o_items);
-    a.n_items = p.n_items;
-    a.sync = wave_sync(g, 0);
-    vp9g::launch_intra_wave(s, a, 1024);
-  } else if (p.n_items && g->intra_blocks) {
-    vp9g::IntraArgs a{};
-    a.g = p.geo;
-    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
-    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
-    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
-    a.coef_val = (const int32_t *)(dv + p.o_cval);
-    a.cur = cur->planes;
-    memcpy(a.q, p.q, sizeof p.q);
-    a.items = (const vp9g::IntraItem *)(dv + p.o_items);
-    for (size_t l = 0; l + 1 < p.item_level_start.size(); l++) {
-      uint32_t first = p.item_level_start[l], n = p.item_level_start[l + 1] - first;
-      vp9g::launch_intra_blocks(s, a, first, n);
+    std::vector<uint32_t> first(n), count(n);
+    auto table = [&](auto get_first, auto get_count, uint32_t per_wg) {
+      for (int i = 0; i < n; i++) {
+        first[i] = get_first(i);
+        count[i] = get_count(i);
+      }
+      return add_ranges(st, n, first.data(), count.data(), per_wg);
+    };
+    LaunchTable t_clear, t_mc, t_itx[4], t_mask;
+    std::vector<LaunchTable> t_intra, t_lf;
+    bool groups = false;
+    if (recon) {
+      t_clear = table([](int) { return 0u; },
+                      [&](int i) { return 3 * vp9g::clear_margin_rows(f[i]->cur->planes, f[i]->p->geo); }, 1);
+      t_mc = table([](int) { return 0u; }, [&](int i) { return f[i]->p->n_tiles; }, 1);
+      for (int k = 0; k < 4; k++)
+        t_itx[k] = table([](int) { return 0u; }, [&](int i) { return f[i]->p->n_jobs[k]; }, 64u >> (2 + k));
+      // intra: leveled blocks, or (superblock, plane) groups
+      groups = !g->intra_blocks;
+      size_t levels = 0;
+      for (int i = 0; i < n; i++) {
+        const auto &ls = groups ? f[i]->p->group_level_start : f[i]->p->item_level_start;
+        if (ls.size() > 1) levels = std::max(levels, ls.size() - 1);
+      }
+      for (size_t l = 0; l < levels; l++) {
+        t_intra.push_back(table(
+            [&](int i) {
+              const auto &ls = groups ? f[i]->p->group_level_start : f[i]->p->item_level_start;
+              return l + 1 < ls.size() ? ls[l] : 0u;
+            },
+            [&](int i) {
+              const auto &ls = groups ? f[i]->p->group_level_start : f[i]->p->item_level_start;
+              return l + 1 < ls.size() ? ls[l + 1] - ls[l] : 0u;
+            },
+            1));
+      }
     }
-  } else if (p.n_groups) {
-    vp9g::IntraArgs a{};
-    a.g = p.geo;
-    a.blocks = (const vp9r_block *)(dv + p.o_blocks);
-    a.txbs = (const vp9r_txb *)(dv + p.o_txbs);
-    a.txb_index = (const uint32_t *)(dv + p.o_iidx);
-    a.groups = (const vp9g::IntraGroup *)(dv + p.o_groups);
-    a.coef_pos = (const uint16_t *)(dv + p.o_cpos);
-    a.coef_val = (const int32_t *)(dv + p.o_cval);
-    a.cur = cur->planes;
-    memcpy(a.q, p.q, sizeof p.q);
-    for (size_t l = 0; l + 1 < p.group_level_start.size(); l++) {
-      uint32_t first = p.group_level_start[l], n = p.group_level_start[l + 1] - first;
-      vp9g::launch_intra(s, a, first, n);
+    if (lf) {
+      t_mask = table([](int) { return 0u; },
+                     [&](int i) {
+                       const auto &h = f[i]->p->hdr;
+                       return f[i]->p->lf_level_start.empty() ? 0u : 6 * h.sb64_rows * h.sb64_cols;
+                     },
+                     1);
+      size_t levels = 0;
+      for (int i = 0; i < n; i++)
+        if (f[i]->p->lf_level_start.size() > 1) levels = std::max(levels, f[i]->p->lf_level_start.size() - 1);
+      for (size_t l = 0; l < levels; l++)
+        t_lf.push_back(table(
+            [&](int i) {
+              const auto &ls = f[i]->p->lf_level_start;
+              return l + 1 < ls.size() ? ls[l] : 0u;
+            },
+            [&](int i) {
+              const auto &ls = f[i]->p->lf_level_start;
+              return l + 1 < ls.size() ? ls[l + 1] - ls[l] : 0u;
+            },
+            1));
     }
-  }
-  if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
-  HIP_TRY(hipGetLastError(), "kernel launch");
-  return VP9R_OK;
-}
-
-static int launch_lf(vp9r_gpu *g, const Prepared &p, const uint8_t *dv, vp9r_gpu_frame *cur) {
-  hipStream_t s = g->stream;
-  HIP_TRY(order_after_copy(s, cur), "hipStreamWaitEvent");
-  if (p.n_lf && g->wave_launch) {
-    vp9g::LfArgs a{};
-    a.g = p.geo;
-    a.mi = (const vp9r_mi *)(dv + p.o_mi);
-    a.cur = cur->planes;
-    a.p = p.lfp;
-    a.sbs = (const uint32_t *)(dv + p.o_lf);
-    a.n_items = p.n_lf;
-    a.sync = wave_sync(g, 1);
-    vp9g::launch_lf_wave(s, a, 128);
-  } else if (!p.lf_level_start.empty()) {
-    const vp9r_frame_hdr &h = p.hdr;
-    const size_t n_sb = (size_t)h.sb64_rows * h.sb64_cols;
-    HIP_TRY(g->lf_mask.ensure(n_sb * 6 * 256 * 2), "hipMalloc(lf masks)");
-    vp9g::LfArgs a{};
-    a.g = p.geo;
-    a.mi = (const vp9r_mi *)(dv + p.o_mi);
-    a.mask_fs = (int8_t *)g->lf_mask.p;
-    a.mask_lvl = (uint8_t *)g->lf_mask.p + n_sb * 6 * 256;
-    a.cur = cur->planes;
-    a.p = p.lfp;
-    a.sbs = (const uint32_t *)(dv + p.o_lf);
-    vp9g::launch_lf_mask(s, a);
-    for (size_t l = 0; l + 1 < p.lf_level_start.size(); l++) {
-      uint32_t first = p.lf_level_start[l], n = p.lf_level_start[l + 1] - first;
-      vp9g::launch_lf_level(s, a, first, n);
+    // per-frame arguments
+    for (int i = 0; i < n; i++) {
+      const Prepared &p = *f[i]->p;
+      const uint8_t *dv = f[i]->dv;
+      vp9r_gpu_frame *cur = f[i]->cur;
+      vp9g::ClearArgs &ca = ((vp9g::ClearArgs *)(st.data() + o_clear))[i];
+      ca.f = cur->planes;
+      ca.g = p.geo;
+      vp9g::McArgs &ma = ((vp9g::McArgs *)(st.data() + o_mc))[i];
+      ma = vp9g::McArgs{};
+      ma.g = p.geo;
+      ma.blocks = (const vp9r_block *)(dv + p.o_blocks);
+      ma.tiles = (const vp9g::McTile *)(dv + p.o_tiles);
+      ma.cur = cur->planes;
+      for (int k = 0; k < 3; k++)
+        if (f[i]->refs[k]) {
+          ma.ref[k] = f[i]->refs[k]->planes;
+          ma.ref_w[k] = f[i]->refs[k]->width;
+          ma.ref_h[k] = f[i]->refs[k]->height;
+        }
+      vp9g::ItxArgs &ia = ((vp9g::ItxArgs *)(st.data() + o_itx))[i];
+      ia = vp9g::ItxArgs{};
+      ia.g = p.geo;
+      ia.blocks = ma.blocks;
+      ia.txbs = (const vp9r_txb *)(dv + p.o_txbs);
+      for (int k = 0; k < 4; k++) ia.jobs[k] = (const uint32_t *)(dv + p.o_jobs[k]);
+      ia.coef_pos = (const uint16_t *)(dv + p.o_cpos);
+      ia.coef_val = (const int32_t *)(dv + p.o_cval);
+      ia.cur = cur->planes;
+      memcpy(ia.q, p.q, sizeof p.q);
+      vp9g::IntraArgs &na = ((vp9g::IntraArgs *)(st.data() + o_intra))[i];
+      na = vp9g::IntraArgs{};
+      na.g = p.geo;
+      na.blocks = ma.blocks;
+      na.txbs = ia.txbs;
+      na.txb_index = (const uint32_t *)(dv + p.o_iidx);
+      na.groups = (const vp9g::IntraGroup *)(dv + p.o_groups);
+      na.items = (const vp9g::IntraItem *)(dv + p.o_items);
+      na.coef_pos = ia.coef_pos;
+      na.coef_val = ia.coef_val;
+      na.cur = cur->planes;
+      memcpy(na.q, p.q, sizeof p.q);
+      vp9g::LfArgs &la = ((vp9g::LfArgs *)(st.data() + o_lf))[i];
+      la = vp9g::LfArgs{};
+      la.g = p.geo;
+      la.mi = (const vp9r_mi *)(dv + p.o_mi);
+      const size_t n_sb = (size_t)p.hdr.sb64_rows * p.hdr.sb64_cols;
+      la.mask_fs = (int8_t *)g->lf_mask.p + mask_off[i];
+      la.mask_lvl = (uint8_t *)g->lf_mask.p + mask_off[i] + n_sb * 6 * 256;
+      la.cur = cur->planes;
+      la.p = p.lfp;
+      la.sbs = (const uint32_t *)(dv + p.o_lf);
     }
+    // one transfer into the next device slot
+    BatchSlot &slot = g->slots[g->slot_next];
+    g->slot_next = (g->slot_next + 1) % kBatchSlots;
+    if (slot.pending) {
+      HIP_TRY(hipEventSynchronize(slot.ev), "batch slot reuse");
+      slot.pending = false;
+    }
+    HIP_TRY(slot.host.ensure(st.size()), "hipHostMalloc(batch)");
+    HIP_TRY(slot.dev.ensure(st.size()), "hipMalloc(batch)");
+    memcpy(slot.host.p, st.data(), st.size());
+    HIP_TRY(hipMemcpyAsync(slot.dev.p, slot.host.p, st.size(), hipMemcpyHostToDevice, s), "upload batch");
+    const uint8_t *d = (const uint8_t *)slot.dev.p;
+    auto R = [&](const LaunchTable &t) { return (const vp9g::Range *)(d + t.off); };
+    if (recon) {
+      for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
+      vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
+      vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd);
+      if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
+      for (int k = 0; k < 4; k++)
+        vp9g::launch_itx_inter(s, (const vp9g::ItxArgs *)(d + o_itx), R(t_itx[k]), n, t_itx[k].wgs, k, hbd);
+      if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
+      for (const LaunchTable &t : t_intra) {
+        if (groups)
+          vp9g::launch_intra(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
+        else
+          vp9g::launch_intra_blocks(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
+      }
+      if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
+    }
+    if (lf) {
+      for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
+      vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
+      for (const LaunchTable &t : t_lf) vp9g::launch_lf_level(s, (const vp9g::LfArgs *)(d + o_lf), R(t), n, t.wgs, hbd);
+    }
+    HIP_TRY(hipEventRecord(slot.ev, s), "hipEventRecord");
+    slot.pending = true;
   }
-  if (g->timing) HIP_TRY(hipEventRecord(g->ev[4], s), "hipEventRecord");
+  if (lf && g->timing) HIP_TRY(hipEventRecord(g->ev[4], s), "hipEventRecord");
   HIP_TRY(hipGetLastError(), "kernel launch");
   return VP9R_OK;
 }
@@ -960,14 +1034,17 @@ int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *co
   HIP_TRY(g->arena.ensure(p.bytes), "hipMalloc(arena)");
   hipStream_t s = g->stream;
   if (g->timing) {
-    HIP_TRY(begin_timed_frame(g), "stage timing");
+    HIP_TRY(begin_timed_frame(g, 1), "stage timing");
     HIP_TRY(hipEventRecord(g->ev[0], s), "hipEventRecord");
   }
   HIP_TRY(hipMemcpyAsync(g->arena.p, g->stage.p, p.bytes, hipMemcpyHostToDevice, s), "upload record");
   HIP_TRY(hipEventRecord(g->ev_upload, s), "hipEventRecord");
   g->upload_pending = true;
   g->last_valid = true;
-  return launch_recon(g, p, (const uint8_t *)g->arena.p, refs, cur);
+  BatchFrame bf{&p, (const uint8_t *)g->arena.p, {nullptr, nullptr, nullptr}, cur};
+  if (!p.hdr.frame_is_intra)
+    for (int k = 0; k < 3; k++) bf.refs[k] = refs[k];
+  return launch_batch(g, &bf, 1, true, false);
 }
 
 int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cur) {
@@ -977,9 +1054,8 @@ int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cu
     set_error("vp9r_gpu_loop_filter takes the record last given to vp9r_gpu_reconstruct");
     return VP9R_ERR_INVALID_INPUT;
   }
-  vp9r_gpu_frame *no_refs[3] = {nullptr, nullptr, nullptr};
-  (void)no_refs;
-  if (!cur || cur->width != rec->hdr.width || cur->height != rec->hdr.height) {
+  if (!cur || cur->width != rec->hdr.width || cur->height != rec->hdr.height ||
+      cur->bit_depth != rec->hdr.bit_depth || cur->ss_x != rec->hdr.ss_x || cur->ss_y != rec->hdr.ss_y) {
     set_error("the frame store does not match the record's geometry");
     return VP9R_ERR_INVALID_INPUT;
   }
@@ -987,7 +1063,8 @@ int vp9r_gpu_loop_filter(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *cu
     set_error("timing was enabled between vp9r_gpu_reconstruct and vp9r_gpu_loop_filter");
     return VP9R_ERR_INVALID_INPUT;
   }
-  return launch_lf(g, g->last, (const uint8_t *)g->arena.p, cur);
+  BatchFrame bf{&g->last, (const uint8_t *)g->arena.p, {nullptr, nullptr, nullptr}, cur};
+  return launch_batch(g, &bf, 1, false, true);
 }
 
 int vp9r_gpu_prepare(vp9r_gpu *g, const vp9r_record *rec, vp9r_prepared **out) {
@@ -1018,15 +1095,51 @@ void vp9r_prepared_free(vp9r_prepared *p) { delete p; }
 
 int vp9r_gpu_run(vp9r_gpu *g, const vp9r_prepared *pr, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur) {
   HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
-  int st = check_frames(pr->p, refs, cur);
-  if (st) return st;
+  const vp9r_prepared *one[1] = {pr};
+  vp9r_gpu_frame *const cur1[1] = {cur};
+  vp9r_gpu_frame *none[3] = {nullptr, nullptr, nullptr};
+  return vp9r_gpu_run_batch(g, 1, one, refs ? refs : none, cur1);
+}
+
+int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *pr, vp9r_gpu_frame *const *refs,
+                       vp9r_gpu_frame *const *cur) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (n == 0) return VP9R_OK;
+  if (!pr || !cur || !refs) {
+    set_error("vp9r_gpu_run_batch: null argument");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  std::vector<BatchFrame> bf(n);
+  for (uint32_t i = 0; i < n; i++) {
+    if (!pr[i]) {
+      set_error("vp9r_gpu_run_batch: null prepared record");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+    const Prepared &p = pr[i]->p;
+    int st = check_frames(p, refs + 3 * i, cur[i]);
+    if (st) return st;
+    bf[i] = BatchFrame{&p, (const uint8_t *)pr[i]->dev.p, {nullptr, nullptr, nullptr}, cur[i]};
+    if (!p.hdr.frame_is_intra)
+      for (int k = 0; k < 3; k++) bf[i].refs[k] = refs[3 * i + k];
+  }
+  // frames of one batch run side by side: none may read or write another's output
+  for (uint32_t i = 0; i < n; i++)
+    for (uint32_t j = 0; j < n; j++) {
+      if (i != j && bf[i].cur == bf[j].cur) {
+        set_error("vp9r_gpu_run_batch: two frames of the batch write the same frame store");
+        return VP9R_ERR_INVALID_INPUT;
+      }
+      for (int k = 0; k < 3; k++)
+        if (i != j && bf[i].refs[k] == bf[j].cur) {
+          set_error("vp9r_gpu_run_batch: a frame of the batch references another frame's output");
+          return VP9R_ERR_INVALID_INPUT;
+        }
+    }
   if (g->timing) {
-    HIP_TRY(begin_timed_frame(g), "stage timing");
+    HIP_TRY(begin_timed_frame(g, n), "stage timing");
     HIP_TRY(hipEventRecord(g->ev[0], g->stream), "hipEventRecord");
   }
-  st = launch_recon(g, pr->p, (const uint8_t *)pr->dev.p, refs, cur);
-  if (st) return st;
-  return launch_lf(g, pr->p, (const uint8_t *)pr->dev.p, cur);
+  return launch_batch(g, bf.data(), (int)n, true, true);
 }
 
 int vp9r_gpu_prepared_stats(const vp9r_prepared *pr, uint64_t out[12]) {

@@ -48,6 +48,28 @@ struct Prepared {
   uint64_t work_bytes[4] = {};
 };
 
+// One frame of a batched launch: its prepared inputs (device copy at dv)
+// and its frame stores.
+struct BatchFrame {
+  const Prepared *p;
+  const uint8_t *dv;
+  vp9r_gpu_frame *refs[3];
+  vp9r_gpu_frame *cur;
+};
+
+// A device copy of one batch's arguments and launch ranges.
+struct BatchSlot {
+  DevBuf dev;
+  HostBuf host;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+constexpr int kBatchSlots = 4;
+
+// Queues reconstruction (recon) and/or the loop filter (lf) of n frames on
+// the context's stream, each launch covering every frame.
+int launch_batch(vp9r_gpu *g, const BatchFrame *f, int n, bool recon, bool lf);
+
 }  // namespace vp9r
 
 struct vp9r_gpu_frame {
@@ -74,11 +96,13 @@ struct vp9r_gpu {
   // the record of the last vp9r_gpu_reconstruct (its inputs are in `arena`)
   vp9r::Prepared last;
   bool last_valid = false;
-  // stage timing: one set of 5 events per frame since the last read
+  // stage timing: one set of 5 events per timed call (a frame, or a batch
+  // of ev_frames[k] frames) since the last read
   bool timing = false;
   std::vector<std::array<hipEvent_t, 5>> ev_sets;
+  std::vector<uint32_t> ev_frames;
   size_t ev_used = 0;
-  hipEvent_t *ev = nullptr;  // the current frame's set
+  hipEvent_t *ev = nullptr;  // the current call's set
   double stage_acc[5] = {};   // folded sums (ms) of sets already read back
   uint32_t frames_acc = 0;
   // host work lists, reused
@@ -93,13 +117,11 @@ struct vp9r_gpu {
   std::vector<uint32_t> group_level_start;
   std::vector<uint32_t> lf_codes, lf_level_start;
   uint32_t lf_rows = 0, lf_cols = 0;
-  // one-launch wavefronts: [0..3] tickets, [4] status, then the intra flags
-  // (per 8x8 position) and the loop filter flags (per plane and superblock)
-  bool wave_launch = false;
   bool intra_blocks = true;  // intra as leveled blocks (else (superblock, plane) groups)
-  vp9r::DevBuf wave;
-  size_t wave_intra_cells = 0, wave_lf_items = 0;
-  uint32_t epoch = 0;
+  // batched launches
+  std::vector<uint8_t> batch_stage;
+  vp9r::BatchSlot slots[vp9r::kBatchSlots];
+  int slot_next = 0;
   // per-kernel seam scratch
   vp9r::DevBuf seam_a, seam_b, seam_c;
 };

@@ -1,0 +1,111 @@
+"""Cross-stream batches (vp9r_gpu_run_batch): independent frames of several
+streams reconstructed and loop filtered by the same launches.  Each stream's
+frames must equal the oracle's, as when the stream runs alone: sizes,
+bit depths (8-bit and high-bit-depth frames share a batch), subsampling,
+scaled references and lossless frames mixed, streams of unequal length."""
+import pytest
+
+import oracle_lib
+import rivet_vp9_amd as vp
+import streamgen
+
+pytestmark = pytest.mark.gpu
+
+MIX = [
+    dict(width=352, height=288, frames=5, seed=11, base_q_idx=120, content=0),
+    dict(width=640, height=360, frames=4, seed=12, tile_cols_log2=1, content=1, intra_percent=10),
+    dict(width=200, height=120, frames=6, seed=13, resize_every=2),
+    dict(width=160, height=96, frames=3, seed=14, bit_depth=10),
+    dict(width=160, height=96, frames=4, seed=15, bit_depth=12, ss_x=0, ss_y=0),
+    dict(width=160, height=96, frames=4, seed=16, ss_x=1, ss_y=0),
+    dict(width=96, height=64, frames=2, seed=17, base_q_idx=0),
+]
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    return vp.Gpu(0)
+
+
+class StreamState:
+    def __init__(self, gpu, frames):
+        self.gpu = gpu
+        p = vp.Parser()
+        self.preps = [gpu.prepare(p.parse(f)) for f in frames]
+        ref = oracle_lib.CpuDecoder()
+        self.want = [oracle_lib.md5(o.packed()) for f in frames for o in ref.decode(f)]
+        self.slots = [None] * 8
+        self.got = []
+        self.next = 0
+
+    def job(self):
+        """The next frame's (prepared, refs, cur), or None at the end."""
+        if self.next >= len(self.preps):
+            return None
+        pr = self.preps[self.next]
+        h = pr.hdr
+        cur = self.gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+        refs = [None] * 3 if h.frame_is_intra else [self.slots[h.ref_frame_idx[k]] for k in range(3)]
+        return pr, refs, cur
+
+    def done(self, job):
+        pr, _, cur = job
+        h = pr.hdr
+        for k in range(8):
+            if (h.refresh_frame_flags >> k) & 1:
+                self.slots[k] = cur
+        if h.show_frame:
+            self.got.append(oracle_lib.md5(cur.packed()))
+        self.next += 1
+
+
+def test_mixed_streams_batched_equal_the_oracle(gpu):
+    streams = []
+    for i, cfg in enumerate(MIX):
+        print(f"generating stream {i}: {cfg}", flush=True)
+        streams.append(StreamState(gpu, streamgen.generate(**cfg)))
+    step = 0
+    while True:
+        jobs = [(s, s.job()) for s in streams]
+        jobs = [(s, j) for s, j in jobs if j is not None]
+        if not jobs:
+            break
+        print(f"batch {step}: {len(jobs)} frames", flush=True)
+        gpu.run_batch([j for _, j in jobs])
+        gpu.synchronize()
+        for s, j in jobs:
+            s.done(j)
+        step += 1
+    for i, s in enumerate(streams):
+        assert s.got == s.want, f"stream {i} ({MIX[i]})"
+
+
+def test_batch_of_one_and_repeated_batches(gpu):
+    """The same prepared frames batched twice in a row (slot ring reuse), and
+    as batches of one, give the same frames."""
+    frames = streamgen.generate(**MIX[0])
+    for attempt in range(2):
+        s = StreamState(gpu, frames)
+        while (j := s.job()) is not None:
+            gpu.run_batch([j])
+            s.done(j)
+        gpu.synchronize()
+        assert s.got == s.want, f"pass {attempt}"
+
+
+def test_batch_rejects_dependent_frames(gpu):
+    """A frame that references another frame of the same batch, or two frames
+    writing one store, are refused before anything is queued."""
+    frames = streamgen.generate(**dict(MIX[0], frames=2))
+    p = vp.Parser()
+    p0, p1 = [gpu.prepare(p.parse(f)) for f in frames]
+    h = p0.hdr
+    a = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+    b = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+    assert not p1.hdr.frame_is_intra
+    with pytest.raises(vp.InvalidInputError):
+        gpu.run_batch([(p0, [None] * 3, a), (p1, [a, a, a], b)])
+    with pytest.raises(vp.InvalidInputError):
+        gpu.run_batch([(p0, [None] * 3, a), (p0, [None] * 3, a)])
+    gpu.run_batch([(p0, [None] * 3, a)])
+    gpu.synchronize()

@@ -1,8 +1,8 @@
 """GPU parity on generated streams: features the committed streams do not
 reach (several tile columns, scaled references, 4:2:2 / 4:4:0 / 4:4:4 at
 every bit depth, the 1080p benchmark workload), every frame compared with
-the CPU oracle after reconstruction and after the loop filter, under both
-wavefront schedules (one launch per level, and one launch per frame)."""
+the CPU oracle after reconstruction and after the loop filter, with intra
+work as leveled blocks (the default) and as (superblock, plane) groups."""
 import os
 
 import pytest
@@ -59,13 +59,27 @@ def test_stream_lockstep(gpu, name):
 
 
 @pytest.mark.skip(reason="the one-launch wavefront kernels did not finish on the GPU (device-side waits "
-                         "across workgroups); off by default until their cross-workgroup protocol is fixed")
+                         "across workgroups); the schedule was removed and VP9R_WAVEFRONT=one is an error")
 @pytest.mark.parametrize("name", ["rects-640x360-2tiles", "resize-200x120", "12bit-444"])
 def test_stream_lockstep_one_launch(gpu_one_launch, name):
     print(f"stream {name} (one-launch wavefronts)", flush=True)
     frames = streamgen.generate(**CASES[name])
     n, err = gpu_lockstep.run_frames(frames, gpu_one_launch)
     assert err is None, f"{name}: {n} frames equal, then {err}"
+
+
+def test_one_launch_schedule_is_an_error():
+    """The removed one-launch schedule is refused, not silently replaced."""
+    old = os.environ.get("VP9R_WAVEFRONT")
+    os.environ["VP9R_WAVEFRONT"] = "one"
+    try:
+        with pytest.raises(vp.UnsupportedError):
+            vp.Gpu(0)
+    finally:
+        if old is None:
+            del os.environ["VP9R_WAVEFRONT"]
+        else:
+            os.environ["VP9R_WAVEFRONT"] = old
 
 
 @pytest.fixture(scope="module")
