@@ -35,6 +35,7 @@ enum {
   VP9R_ERR_UNSUPPORTED = 2,
   VP9R_ERR_INVALID_INPUT = 3,
   VP9R_ERR_DEVICE = 4, /* HIP error, or no GPU */
+  VP9R_AGAIN = 5,      /* vp9r_server_poll: no frame within the time given */
 };
 
 /* The message of the last failure on this thread ("" if none). */
@@ -322,6 +323,68 @@ void vp9r_decoder_set_max_pixels(vp9r_decoder *d, uint64_t pixels);
 int vp9r_decoder_decode(vp9r_decoder *d, const uint8_t *packet, size_t len, uint32_t *n_frames);
 int vp9r_decoder_frame(const vp9r_decoder *d, uint32_t i, vp9r_frame *out);
 uint64_t vp9r_decoder_frames_decoded(const vp9r_decoder *d);
+
+/* ------------------------------------------------------------------------ */
+/* Multi-stream server: many independent streams decoded on one GPU.        */
+/*                                                                          */
+/* A stream is admitted when its HBM footprint fits the budget: 10 frame    */
+/* stores (8 reference slots, the frame being written, one in flight) of   */
+/* its declared largest frame, superblock-aligned planes.  Streams wait in  */
+/* open order until they fit.  Parse worker threads parse submitted packets */
+/* (one worker per stream at a time) and upload each frame's inputs; a      */
+/* dispatcher thread puts the next ready frame of every stream into one     */
+/* batch (vp9r_gpu_run_batch) while earlier batches run, and copies shown   */
+/* frames to pinned host memory; a finisher thread hands them to            */
+/* vp9r_server_poll with their latency.  Every stream's frames are those of */
+/* decoding it alone.  The crate decodes one stream per Decoder             */
+/* (src/decoder/mod.rs:150-216) and has no counterpart.                     */
+
+typedef struct vp9r_server vp9r_server;
+
+typedef struct vp9r_server_config {
+  uint64_t hbm_budget;    /* bytes of frame stores; 0: 90% of the device's free memory */
+  uint32_t parse_threads; /* 0: hardware threads - 2 (at least 1) */
+  uint32_t max_batch;     /* frames per batch; 0: 64 */
+  uint32_t max_inflight;  /* batches queued on the device at once; 0: 2 */
+  uint32_t parse_ahead;   /* parsed frames per stream waiting for the device; 0: 4 */
+} vp9r_server_config;
+
+typedef struct vp9r_server_frame {
+  uint32_t stream;   /* the id vp9r_server_open gave */
+  uint32_t index;    /* shown-frame number within the stream */
+  uint32_t eos;      /* 1: the stream is over and this record holds no frame */
+  int32_t status;    /* with eos: VP9R_OK, or the error that ended the stream */
+  double latency_ms; /* submit of the frame's packet to the frame complete in host memory */
+  uint64_t handle;   /* for vp9r_server_release (0 with eos) */
+  vp9r_frame frame;  /* packed Y, U, V; data valid until released */
+} vp9r_server_frame;
+
+int vp9r_server_new(int device, const vp9r_server_config *cfg, vp9r_server **out);
+/* Stops the threads, waits for the device and frees everything (frames not
+ * yet released included). */
+void vp9r_server_free(vp9r_server *s);
+/* Bytes of one frame store (superblock-aligned planes) and a stream's
+ * footprint (10 stores). */
+uint64_t vp9r_frame_store_bytes(uint32_t width, uint32_t height, uint32_t bit_depth, uint32_t ss_x,
+                                uint32_t ss_y);
+/* Opens a stream whose frames are at most max_width x max_height, of this
+ * bit depth and subsampling (a frame outside them ends the stream with
+ * VP9R_ERR_INVALID_INPUT).  Fails when its footprint exceeds the budget. */
+int vp9r_server_open(vp9r_server *s, uint32_t max_width, uint32_t max_height, uint32_t bit_depth,
+                     uint32_t ss_x, uint32_t ss_y, uint32_t *id);
+/* Queues one packet (a frame or a superframe); the data is copied. */
+int vp9r_server_submit(vp9r_server *s, uint32_t id, const uint8_t *packet, size_t len);
+/* No more packets: once its frames are out, the stream ends (an eos record)
+ * and its footprint is released. */
+int vp9r_server_close(vp9r_server *s, uint32_t id);
+/* The next finished frame or end of stream, waiting up to timeout_ms
+ * (negative: no limit); VP9R_AGAIN when none came. */
+int vp9r_server_poll(vp9r_server *s, vp9r_server_frame *out, int timeout_ms);
+void vp9r_server_release(vp9r_server *s, uint64_t handle);
+/* [0] frames shown, [1] frames decoded, [2] batches, [3] largest batch,
+ * [4] streams admitted so far, [5] streams waiting for admission,
+ * [6] frame-store bytes allocated, [7] the budget. */
+int vp9r_server_stats(vp9r_server *s, uint64_t out[8]);
 
 #ifdef __cplusplus
 }

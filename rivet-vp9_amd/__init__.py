@@ -18,7 +18,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvp9r.so")
 
-OK, ERR_BITSTREAM, ERR_UNSUPPORTED, ERR_INVALID_INPUT, ERR_DEVICE = 0, 1, 2, 3, 4
+OK, ERR_BITSTREAM, ERR_UNSUPPORTED, ERR_INVALID_INPUT, ERR_DEVICE, AGAIN = 0, 1, 2, 3, 4, 5
 
 
 class Vp9Error(Exception):
@@ -113,6 +113,16 @@ class FrameDesc(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", u64), ("parse_threads", u32), ("max_batch", u32), ("max_inflight", u32),
+                ("parse_ahead", u32)]
+
+
+class ServerFrame(ctypes.Structure):
+    _fields_ = [("stream", u32), ("index", u32), ("eos", u32), ("status", i32), ("latency_ms", ctypes.c_double),
+                ("handle", u64), ("frame", FrameDesc)]
+
+
 class McDesc(ctypes.Structure):
     _fields_ = [("x", i32), ("y", i32), ("x2", i32), ("y2", i32), ("xs", i16), ("ys", i16),
                 ("w", u8), ("h", u8), ("filter", u8), ("ref_idx", i8), ("ref2_idx", i8),
@@ -185,6 +195,16 @@ def library():
     lib.vp9r_decoder_frame.argtypes = [P, u32, ctypes.POINTER(FrameDesc)]
     lib.vp9r_decoder_frames_decoded.restype = u64
     lib.vp9r_decoder_frames_decoded.argtypes = [P]
+    lib.vp9r_server_new.argtypes = [ctypes.c_int, ctypes.POINTER(ServerConfig), PP]
+    lib.vp9r_server_free.argtypes = [P]
+    lib.vp9r_frame_store_bytes.restype = u64
+    lib.vp9r_frame_store_bytes.argtypes = [u32] * 5
+    lib.vp9r_server_open.argtypes = [P, u32, u32, u32, u32, u32, ctypes.POINTER(u32)]
+    lib.vp9r_server_submit.argtypes = [P, u32, ctypes.c_char_p, sz]
+    lib.vp9r_server_close.argtypes = [P, u32]
+    lib.vp9r_server_poll.argtypes = [P, ctypes.POINTER(ServerFrame), ctypes.c_int]
+    lib.vp9r_server_release.argtypes = [P, u64]
+    lib.vp9r_server_stats.argtypes = [P, ctypes.POINTER(u64)]
     _lib = lib
     return lib
 
@@ -509,3 +529,78 @@ class Decoder:
 
     def frames_decoded(self):
         return library().vp9r_decoder_frames_decoded(self._d)
+
+
+class ServedFrame:
+    """A frame (or end of stream) handed out by Server.poll().  The packed
+    data stays in the server's pinned memory until release()."""
+
+    def __init__(self, server, rec):
+        self._server = server
+        self.stream, self.index, self.eos = rec.stream, rec.index, bool(rec.eos)
+        self.status, self.latency_ms, self.handle = rec.status, rec.latency_ms, rec.handle
+        self.desc = rec.frame
+        self.size = rec.frame.size if not self.eos else 0
+
+    def view(self):
+        """The packed frame as a memoryview over pinned memory (no copy)."""
+        arr = (ctypes.c_uint8 * self.size).from_address(ctypes.addressof(self.desc.data.contents))
+        return memoryview(arr).cast("B")
+
+    def frame(self):
+        """A copy as a Frame."""
+        return Frame(self.desc)
+
+    def release(self):
+        if self.handle:
+            library().vp9r_server_release(self._server._s, self.handle)
+            self.handle = 0
+
+
+class Server:
+    """Many streams on one GPU (vp9r_server_*): admission by HBM footprint,
+    parse workers, cross-stream batches, per-frame latency."""
+
+    def __init__(self, device=0, hbm_budget=0, parse_threads=0, max_batch=0, max_inflight=0, parse_ahead=0):
+        cfg = ServerConfig(hbm_budget, parse_threads, max_batch, max_inflight, parse_ahead)
+        self._s = ctypes.c_void_p()
+        _check(library().vp9r_server_new(device, ctypes.byref(cfg), ctypes.byref(self._s)))
+
+    def close_server(self):
+        if getattr(self, "_s", None):
+            library().vp9r_server_free(self._s)
+            self._s = None
+
+    def __del__(self):
+        self.close_server()
+
+    @staticmethod
+    def footprint(width, height, bit_depth=8, ss_x=1, ss_y=1):
+        return 10 * library().vp9r_frame_store_bytes(width, height, bit_depth, ss_x, ss_y)
+
+    def open(self, max_width, max_height, bit_depth=8, ss_x=1, ss_y=1):
+        sid = u32()
+        _check(library().vp9r_server_open(self._s, max_width, max_height, bit_depth, ss_x, ss_y, ctypes.byref(sid)))
+        return sid.value
+
+    def submit(self, stream, packet):
+        _check(library().vp9r_server_submit(self._s, stream, packet, len(packet)))
+
+    def close(self, stream):
+        _check(library().vp9r_server_close(self._s, stream))
+
+    def poll(self, timeout_ms=-1):
+        """The next ServedFrame, or None when none came within timeout_ms."""
+        rec = ServerFrame()
+        st = library().vp9r_server_poll(self._s, ctypes.byref(rec), timeout_ms)
+        if st == AGAIN:
+            return None
+        _check(st)
+        return ServedFrame(self, rec)
+
+    def stats(self):
+        out = (u64 * 8)()
+        _check(library().vp9r_server_stats(self._s, out))
+        keys = ["frames_shown", "frames_decoded", "batches", "max_batch", "streams_admitted", "streams_waiting",
+                "store_bytes", "hbm_budget"]
+        return dict(zip(keys, list(out)))

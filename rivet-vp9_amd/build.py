@@ -31,6 +31,7 @@ HIP_SOURCES = [
     "csrc/host/recon.cpp",
     "csrc/host/capi_seam.cpp",
     "csrc/host/decoder.cpp",
+    "csrc/host/server.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -137,7 +137,7 @@ static vp9g::FrameGeom geom_of(const vp9r_frame_hdr &h) {
 // The inter prediction tiles of the record (block.rs:1425-1448): each
 // block's planes as one prediction unit, or one 4x4 unit per sub-block
 // below 8x8; units split into tiles of at most 16 x 16.
-static void build_mc_tiles(vp9r_gpu *g, const vp9r_record *rec) {
+static void build_mc_tiles(Builder *g, const vp9r_record *rec) {
   const vp9r_frame_hdr &h = rec->hdr;
   auto &tiles = g->tiles;
   tiles.clear();
@@ -179,7 +179,7 @@ static void build_mc_tiles(vp9r_gpu *g, const vp9r_record *rec) {
 }
 
 // Intra groups per (plane, superblock) and their dependency levels.
-static void build_intra(vp9r_gpu *g, const vp9r_record *rec) {
+static void build_intra(Builder *g, const vp9r_record *rec) {
   const vp9r_frame_hdr &h = rec->hdr;
   const uint32_t n_sb = h.sb64_rows * h.sb64_cols;
   g->intra_cnt.assign(3 * n_sb, 0);
@@ -296,7 +296,7 @@ static void build_intra(vp9r_gpu *g, const vp9r_record *rec) {
 }
 
 // Loop filter wavefront levels: superblocks with equal c + 2r, all planes.
-static void build_lf_levels(vp9r_gpu *g, uint32_t rows, uint32_t cols) {
+static void build_lf_levels(Builder *g, uint32_t rows, uint32_t cols) {
   if (g->lf_rows == rows && g->lf_cols == cols) return;
   g->lf_rows = rows;
   g->lf_cols = cols;
@@ -612,7 +612,8 @@ namespace vp9r {
 // names, builds the work lists, and writes everything to `stage` (host) at
 // the offsets it records in p.  Every index the kernels follow is checked
 // here, so that no record can send a kernel outside its frame stores.
-static int build(vp9r_gpu *g, const vp9r_record *rec, Prepared &p, HostBuf &stage) {
+int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage) {
+  Builder *g = &w;
   const vp9r_frame_hdr &h = rec->hdr;
   if (h.show_existing_frame) {
     set_error("a show_existing_frame record has nothing to reconstruct");
@@ -1027,7 +1028,7 @@ int vp9r_gpu_reconstruct(vp9r_gpu *g, const vp9r_record *rec, vp9r_gpu_frame *co
     g->upload_pending = false;
   }
   Prepared &p = g->last;
-  int st = build(g, rec, p, g->stage);
+  int st = build_record(g->work, rec, p, g->stage);
   if (st) return st;
   st = check_frames(p, refs, cur);
   if (st) return st;
@@ -1075,7 +1076,7 @@ int vp9r_gpu_prepare(vp9r_gpu *g, const vp9r_record *rec, vp9r_prepared **out) {
     g->upload_pending = false;
   }
   vp9r_prepared *pr = new vp9r_prepared();
-  int st = build(g, rec, pr->p, g->stage);
+  int st = build_record(g->work, rec, pr->p, g->stage);
   if (st) {
     delete pr;
     return st;

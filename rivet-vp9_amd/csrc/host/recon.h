@@ -48,6 +48,26 @@ struct Prepared {
   uint64_t work_bytes[4] = {};
 };
 
+// Host work lists of one record, reused from record to record (one per
+// thread that prepares records).
+struct Builder {
+  std::vector<vp9g::McTile> tiles;
+  std::vector<uint32_t> jobs[4];
+  std::vector<uint32_t> intra_cnt, intra_first, intra_fill, intra_index;
+  std::vector<vp9g::IntraItem> intra_items, items_sorted;
+  std::vector<int32_t> cell_level, item_level;
+  std::vector<uint32_t> item_level_start;
+  std::vector<int32_t> intra_level;
+  std::vector<vp9g::IntraGroup> groups;
+  std::vector<uint32_t> group_level_start;
+  std::vector<uint32_t> lf_codes, lf_level_start;
+  uint32_t lf_rows = 0, lf_cols = 0;
+};
+
+// Checks a record and writes its inputs and work lists to `stage` (pinned
+// host memory) at the offsets it records in p.
+int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage);
+
 // One frame of a batched launch: its prepared inputs (device copy at dv)
 // and its frame stores.
 struct BatchFrame {
@@ -106,17 +126,7 @@ struct vp9r_gpu {
   double stage_acc[5] = {};   // folded sums (ms) of sets already read back
   uint32_t frames_acc = 0;
   // host work lists, reused
-  std::vector<vp9g::McTile> tiles;
-  std::vector<uint32_t> jobs[4];
-  std::vector<uint32_t> intra_cnt, intra_first, intra_fill, intra_index;
-  std::vector<vp9g::IntraItem> intra_items, items_sorted;
-  std::vector<int32_t> cell_level, item_level;
-  std::vector<uint32_t> item_level_start;
-  std::vector<int32_t> intra_level;
-  std::vector<vp9g::IntraGroup> groups;
-  std::vector<uint32_t> group_level_start;
-  std::vector<uint32_t> lf_codes, lf_level_start;
-  uint32_t lf_rows = 0, lf_cols = 0;
+  vp9r::Builder work;
   bool intra_blocks = true;  // intra as leveled blocks (else (superblock, plane) groups)
   // batched launches
   std::vector<uint8_t> batch_stage;
