@@ -19,7 +19,17 @@ p50/p99; and the CPU baseline: the oracle's scalar reconstruction and loop
 filter of the same records on one host core, whose per-frame MD5s the GPU's
 output is checked against (bit_exact).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu]
+Serving (in the same JSON line, "serving"): 32 streams from seed 5 arriving
+Poisson at 4/s, mixed sizes (640x360 / 1280x720 / 1920x1080) and formats
+(8-bit 4:2:0, 10-bit 4:2:0, 12-bit 4:4:4), 90-300 frames each, served by the
+multi-stream server (vp9r_server_*): paced at 30 frames/s per stream from its
+arrival (per-frame latency p50/p99 from packet submit to the frame in host
+memory, aggregate frames/s), and saturated (every packet submitted at once:
+the server's capacity).  Every served frame's MD5 is checked against the
+oracle's decode of its stream, which, one process per stream on all host
+cores, is also the serving CPU baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu] [--no-serve]
 """
 import argparse
 import ctypes
@@ -131,6 +141,170 @@ def cpu_baseline(frames_bytes, budget_s):
     return best, md5s, passes, spent
 
 
+# ---- serving workload
+
+def _gen_worker(cfg):
+    if TESTS not in sys.path:
+        sys.path.insert(0, TESTS)
+    import streamgen
+    return streamgen.generate(**cfg)
+
+
+def _oracle_worker(frames):
+    """The oracle's decode of one stream (product parser, scalar
+    reconstruction and loop filter): its shown frames' MD5s and the time."""
+    _package()
+    if TESTS not in sys.path:
+        sys.path.insert(0, TESTS)
+    import oracle_lib
+    t0 = time.perf_counter()
+    dec = oracle_lib.CpuDecoder()
+    md5s = [oracle_lib.md5(o.packed()) for f in frames for o in dec.decode(f)]
+    return md5s, time.perf_counter() - t0
+
+
+def _host_cores():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def serve_run(vp, plan, streams, paced, parse_threads, fps=30.0):
+    """Runs the streams through one server: opens each at its arrival (paced)
+    or all at once, submits its packets at `fps` from its arrival (paced) or
+    at once, closes it after its last packet.  Returns per-stream MD5 lists,
+    per-stream status, per-frame latencies, wall time and server stats."""
+    import concurrent.futures
+    import threading
+    srv = vp.Server(0, parse_threads=parse_threads)
+    events = []
+    for i, ((arrival, _), frames) in enumerate(zip(plan, streams)):
+        t0 = arrival if paced else 0.0
+        events.append((t0, 0, i, -1))
+        for j in range(len(frames)):
+            events.append((t0 + (j / fps if paced else 0.0), 1, i, j))
+        events.append((t0 + ((len(frames) - 1) / fps if paced else 0.0), 2, i, -1))
+    events.sort()
+    sid_of = {}
+    idx_of = {}
+    start = time.perf_counter()
+    err = []
+
+    def submitter():
+        try:
+            for t, kind, i, j in events:
+                d = start + t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                if kind == 0:
+                    cfg = plan[i][1]
+                    sid = srv.open(cfg["width"], cfg["height"], cfg.get("bit_depth", 8), cfg.get("ss_x", 1),
+                                   cfg.get("ss_y", 1))
+                    sid_of[i] = sid
+                    idx_of[sid] = i
+                elif kind == 1:
+                    srv.submit(sid_of[i], streams[i][j])
+                else:
+                    srv.close(sid_of[i])
+        except Exception as e:  # reported by the caller
+            err.append(e)
+
+    def hash_release(fr):
+        d = hashlib.md5(fr.view()).hexdigest()
+        fr.release()
+        return fr.stream, fr.index, d
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    pool = concurrent.futures.ThreadPoolExecutor(8)
+    futs, lat, status = [], [], {}
+    last = start
+    while len(status) < len(streams):
+        if err:
+            break
+        fr = srv.poll(500)
+        if fr is None:
+            if time.perf_counter() - start > 600:
+                raise RuntimeError(f"serving did not finish: {srv.stats()}")
+            continue
+        if fr.eos:
+            status[fr.stream] = fr.status
+            continue
+        last = time.perf_counter()
+        lat.append(fr.latency_ms)
+        futs.append(pool.submit(hash_release, fr))
+    th.join()
+    if err:
+        raise err[0]
+    md5 = [dict() for _ in streams]
+    for f in futs:
+        sid, index, d = f.result()
+        md5[idx_of[sid]][index] = d
+    pool.shutdown()
+    stats = srv.stats()
+    srv.close_server()
+    md5 = [[m[k] for k in sorted(m)] for m in md5]
+    st = [status.get(sid_of[i]) for i in range(len(streams))]
+    return md5, st, lat, last - start, stats
+
+
+def _pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(q * len(v)))] if v else None
+
+
+def serving_bench(vp, n_streams, parse_threads):
+    import concurrent.futures
+    import multiprocessing
+    import streamgen
+    plan = streamgen.config5(seed=5, streams=n_streams)
+    cores = _host_cores()
+    ctx = multiprocessing.get_context("spawn")
+    t0 = time.perf_counter()
+    with concurrent.futures.ProcessPoolExecutor(cores, mp_context=ctx) as ex:
+        streams = list(ex.map(_gen_worker, [cfg for _, cfg in plan]))
+    total = sum(len(s) for s in streams)
+    _log(f"bench: serving workload: {len(streams)} streams, {total} frames generated in "
+         f"{time.perf_counter() - t0:.1f}s on {cores} processes")
+    # CPU baseline and expected MD5s: the oracle, one decoder per stream, all cores
+    t0 = time.perf_counter()
+    with concurrent.futures.ProcessPoolExecutor(cores, mp_context=ctx) as ex:
+        # longest streams first, so the pool is not left waiting on one
+        order = sorted(range(len(streams)), key=lambda i: -len(streams[i]) * plan[i][1]["width"])
+        res = dict(zip(order, ex.map(_oracle_worker, [streams[i] for i in order])))
+    cpu_wall = time.perf_counter() - t0
+    expected = [res[i][0] for i in range(len(streams))]
+    shown_total = sum(len(e) for e in expected)
+    cpu_fps = shown_total / cpu_wall
+    _log(f"bench: serving CPU baseline (oracle, {cores} processes): {cpu_fps:.1f} frames/s")
+    out = {"workload": f"{len(streams)} streams, seed 5, Poisson arrivals 4/s; 50% 640x360, 30% 1280x720, "
+                       f"20% 1920x1080; 70% 8-bit 4:2:0, 20% 10-bit 4:2:0, 10% 12-bit 4:4:4; 90-300 frames",
+           "streams": len(streams), "frames": shown_total,
+           "cpu_baseline": {"value": round(cpu_fps, 2), "unit": "frames/s", "cores": cores, "kind": "port",
+                            "sample": "oracle decode (product parser + scalar reconstruction and loop filter) of "
+                                      "every stream, one process per stream"}}
+    exact = True
+    for mode, paced in (("paced", True), ("saturated", False)):
+        md5, st, lat, wall, stats = serve_run(vp, plan, streams, paced, parse_threads)
+        ok = all(s == vp.OK for s in st) and md5 == expected
+        exact &= ok
+        conc = 0
+        if paced:
+            iv = [(a, a + (len(s) - 1) / 30.0) for (a, _), s in zip(plan, streams)]
+            conc = max(sum(1 for a, b in iv if a <= t <= b) for t, _ in iv)
+        out[mode] = {"aggregate_fps": round(len(lat) / wall, 2),
+                     "latency_ms": {"p50": round(_pct(lat, 0.5), 3), "p99": round(_pct(lat, 0.99), 3),
+                                    "max": round(max(lat), 3)},
+                     "frames": len(lat), "wall_s": round(wall, 3), "batches": stats["batches"],
+                     "mean_batch": round(stats["frames_decoded"] / max(stats["batches"], 1), 2),
+                     "max_batch": stats["max_batch"], "bit_exact": ok}
+        if paced:
+            out[mode]["stream_fps"] = 30
+            out[mode]["max_concurrent_streams"] = conc
+        _log(f"bench: serving {mode}: {out[mode]}")
+    out["bit_exact"] = exact
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +313,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and the MD5 check)")
     ap.add_argument("--no-extras", action="store_true", help="skip the PCIe and latency passes")
+    ap.add_argument("--no-serve", action="store_true", help="skip the multi-stream serving workload")
+    ap.add_argument("--serve-streams", type=int, default=32, help="streams of the serving workload")
+    ap.add_argument("--parse-threads", type=int, default=12, help="parse workers of the server")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the reference reading of the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -309,6 +486,10 @@ def main():
             bad = [i for i, (a, b) in enumerate(zip(gpu_md5, cpu_md5)) if a != b]
             _log(f"bench: MD5 MISMATCH at shown frames {bad[:10]}")
 
+    serving = None
+    if not args.no_serve and not args.no_cpu:
+        serving = serving_bench(vp, args.serve_streams, args.parse_threads)
+
     out = {
         "metric": "1080p decoded frames/s GPU vs CPU (bit-exact); p99 frame latency @32 streams",
         "value": round(fps, 2),
@@ -332,8 +513,10 @@ def main():
         "parse_fps_one_core": round(n / t_parse, 2),
     }
     out.update(extras)
+    if serving is not None:
+        out["serving"] = serving
     print(json.dumps(out), flush=True)
-    if bit_exact is False:
+    if bit_exact is False or (serving is not None and not serving["bit_exact"]):
         sys.exit(1)
 
 

@@ -65,3 +65,26 @@ CONFIG2 = dict(width=1920, height=1080, frames=60, seed=2, base_q_idx=100, golde
                content=1, intra_percent=3, allow_hp=0)
 CONFIG4 = dict(width=3840, height=2160, frames=48, seed=4, bit_depth=10, base_q_idx=80, golden_interval=8,
                tile_cols_log2=2, content=1, resize_every=12, intra_percent=3, allow_hp=0)
+
+
+def config5(seed=5, streams=32):
+    """The serving workload: `streams` streams arriving Poisson at 4/s;
+    resolutions 50% 640x360, 30% 1280x720, 20% 1920x1080; formats 70% 8-bit
+    4:2:0 (profile 0), 20% 10-bit 4:2:0 (profile 2), 10% 12-bit 4:4:4
+    (profile 3); 90 to 300 frames of moving rectangles each.  Returns a list
+    of (arrival seconds, generator arguments), in arrival order."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [(640, 360), (1280, 720), (1920, 1080)]
+    formats = [(8, 1, 1), (10, 1, 1), (12, 0, 0)]
+    t = 0.0
+    out = []
+    for i in range(streams):
+        t += float(rng.exponential(0.25))
+        w, h = sizes[int(rng.choice(3, p=[0.5, 0.3, 0.2]))]
+        bd, sx, sy = formats[int(rng.choice(3, p=[0.7, 0.2, 0.1]))]
+        n = int(rng.integers(90, 301))
+        out.append((t, dict(width=w, height=h, frames=n, seed=seed * 1000 + i, bit_depth=bd, ss_x=sx, ss_y=sy,
+                            base_q_idx=100, golden_interval=8, tile_cols_log2=2, content=1, intra_percent=3,
+                            allow_hp=0)))
+    return out
