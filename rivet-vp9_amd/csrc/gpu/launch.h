@@ -116,14 +116,17 @@ struct LfParams {
   uint8_t limit[64], blimit[64], thresh[64];
 };
 
+// k_lf_mask's ranges cover 6 entries (plane, pass) per superblock; k_lf's
+// ranges have first = the level c + 2r and count = 3 x its superblocks
+// (item j: plane j % 3 of the level's (j / 3)-th superblock, rows ascending).
 struct LfArgs {
   FrameGeom g;
   const vp9r_mi *mi;
-  int8_t *mask_fs;   // [plane][pass][sb][16][16]
-  uint8_t *mask_lvl;
+  // [plane][pass][sb][16 edges][16 runs]: (filter size + 1) | limit << 8 |
+  // blimit << 16 | thresh << 24, 0 for no filter
+  uint32_t *dq;
   Planes cur;
   LfParams p;
-  const uint32_t *sbs;  // (plane << 28 | sb_row << 14 | sb_col), by level c + 2r
 };
 
 // Launchers: fa = per-frame arguments, r = ranges (n + 1 entries), all in

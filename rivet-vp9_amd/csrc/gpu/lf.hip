@@ -25,7 +25,9 @@ namespace vp9g {
 
 constexpr int BP = 74;  // LDS pitch of the 72 x 72 region (odd in dwords)
 
-DEV int iabs(int v) { return v < 0 ? -v : v; }
+// |a - b| of two samples (< 2^16) in one instruction (v_sad_u16).
+DEV uint32_t adiff(int a, int b) { return __builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }
+DEV int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // lo <= hi
 
 // The filters of one edge position, on samples reached through `s`: s(k)
 // is the sample k positions across the edge from the first q sample
@@ -53,44 +55,45 @@ DEV void lf_wide(S s) {
 }
 
 // lf::filter (src/dsp/lf.rs:24-86): the mask, then the narrow filter, or
-// the 8- or 16-wide filter where the samples are flat.
+// the 8- or 16-wide filter where the samples are flat.  The mask and
+// flatness tests are maxima of absolute differences against one limit
+// each (the reference's chains of comparisons, without branches).
 template <class S>
 DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
   const int q0 = s(0), q1 = s(1), q2 = s(2), q3 = s(3);
   const int p0 = s(-1), p1 = s(-2), p2 = s(-3), p3 = s(-4);
   const int sh = bd - 8;
-  const int lim = limit << sh, blim = blimit << sh;
-  if (iabs(p3 - p2) > lim || iabs(p2 - p1) > lim || iabs(p1 - p0) > lim || iabs(q1 - q0) > lim ||
-      iabs(q2 - q1) > lim || iabs(q3 - q2) > lim || iabs(p0 - q0) * 2 + iabs(p1 - q1) / 2 > blim)
-    return;
-  const int th = thresh << sh;
-  const bool hev = iabs(p1 - p0) > th || iabs(q1 - q0) > th;
-  const int one = 1 << sh;
-  const bool flat = fs >= 1 && iabs(p1 - p0) <= one && iabs(q1 - q0) <= one && iabs(p2 - p0) <= one &&
-                    iabs(q2 - q0) <= one && iabs(p3 - p0) <= one && iabs(q3 - q0) <= one;
-  if (fs == 0 || !flat) {
+  const uint32_t d10 = adiff(p1, p0), e10 = adiff(q1, q0);
+  const uint32_t m = max(max(max(adiff(p3, p2), adiff(p2, p1)), d10), max(max(e10, adiff(q2, q1)), adiff(q3, q2)));
+  const uint32_t edge = adiff(p0, q0) * 2 + (adiff(p1, q1) >> 1);
+  if (m > (uint32_t)(limit << sh) || edge > (uint32_t)(blimit << sh)) return;
+  const bool hev = max(d10, e10) > (uint32_t)(thresh << sh);
+  const uint32_t one = 1u << sh;
+  const bool flat =
+      fs >= 1 && max(max(max(d10, e10), max(adiff(p2, p0), adiff(q2, q0))), max(adiff(p3, p0), adiff(q3, q0))) <= one;
+  if (!flat) {
     // narrow filter (8.8.5.2)
     const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1, off = 0x80 << sh;
     const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
-    int f = hev ? iclamp(ps1 - qs1, lo, hi) : 0;
-    f = iclamp(f + 3 * (qs0 - ps0), lo, hi);
-    const int f1 = iclamp(f + 4, lo, hi) >> 3, f2 = iclamp(f + 3, lo, hi) >> 3;
-    s(0) = iclamp(qs0 - f1, lo, hi) + off;
-    s(-1) = iclamp(ps0 + f2, lo, hi) + off;
+    int f = hev ? clampi(ps1 - qs1, lo, hi) : 0;
+    f = clampi(f + 3 * (qs0 - ps0), lo, hi);
+    const int f1 = clampi(f + 4, lo, hi) >> 3, f2 = clampi(f + 3, lo, hi) >> 3;
+    s(0) = clampi(qs0 - f1, lo, hi) + off;
+    s(-1) = clampi(ps0 + f2, lo, hi) + off;
     if (!hev) {
       const int g = (f1 + 1) >> 1;
-      s(1) = iclamp(qs1 - g, lo, hi) + off;
-      s(-2) = iclamp(ps1 + g, lo, hi) + off;
+      s(1) = clampi(qs1 - g, lo, hi) + off;
+      s(-2) = clampi(ps1 + g, lo, hi) + off;
     }
     return;
   }
-  bool flat2 = fs >= 2;
-  if (flat2) {
+  uint32_t f2 = 0;
+  if (fs >= 2) {
 #pragma unroll
-    for (int k = 4; k < 8; k++) flat2 = flat2 && iabs(s(-(k + 1)) - p0) <= one && iabs(s(k) - q0) <= one;
+    for (int k = 4; k < 8; k++) f2 = max(f2, max(adiff(s(-(k + 1)), p0), adiff(s(k), q0)));
   }
   // wide filter (8.8.5.3): log2 3 (8 taps) or 4 (16 taps)
-  if (flat2)
+  if (fs >= 2 && f2 <= one)
     lf_wide<4>(s);
   else
     lf_wide<3>(s);
@@ -184,15 +187,18 @@ __global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *
   int fs, lvl;
   lf_decide(a, plane, pass, sb / g.sb64_cols, sb % g.sb64_cols, threadIdx.x >> 4, threadIdx.x & 15, fs, lvl);
   const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256 + threadIdx.x;
-  a.mask_fs[o] = (int8_t)fs;
-  a.mask_lvl[o] = (uint8_t)lvl;
+  a.dq[o] = fs < 0 ? 0u
+                   : (uint32_t)(fs + 1) | (uint32_t)a.p.limit[lvl] << 8 | (uint32_t)a.p.blimit[lvl] << 16 |
+                         (uint32_t)a.p.thresh[lvl] << 24;
 }
 
 // ---- one superblock of one plane
 
-struct LfShared {
-  uint16_t buf[72 * BP];  // the superblock and its 8-sample halo above and left
-  uint32_t dq[2][256];    // packed decisions for lf_stream
+struct __align__(16) LfShared {
+  // the superblock and its 8-sample halo above and left (72 x 72); the rows
+  // beyond are slack for lf_stream's reads past a line's end
+  uint16_t buf[80 * BP];
+  uint32_t dq[2][272];  // packed decisions for lf_stream (16 slack entries)
 };
 
 // ---- one superblock of one plane, streamed through a register window
@@ -202,61 +208,120 @@ struct LfShared {
 // one row and slides a 16-sample register window (p7 .. q7 of the current
 // edge) along it: after edge e the 4 samples that leave the window are
 // final (no later edge of the row reaches them) and go back to LDS, 4 new
-// ones come in.  The horizontal pass does the same down each column.  The
-// loop over edges stays rolled (one copy of the filters), the window's
-// indices are constants.  Limits come from LDS: prm[l] = limit | blimit << 8
-// | thresh << 16.
+// ones come in.  The horizontal pass does the same down each column.
 
-template <class P, int RW>
-DEV void lf_region_io(P *pl, uint32_t stride, int x0, int y0, int RH, uint16_t *buf, bool store) {
-  constexpr int V = 16 / sizeof(P) > 8 ? 8 : 16 / sizeof(P);  // samples per vector (8)
-  constexpr int NV = RW / V;
-  static_assert(RW % V == 0, "region rows of whole vectors");
-  for (int k = threadIdx.x; k < RH * NV; k += 64) {
-    const int r = k / NV, q = k - r * NV;
-    const int py = y0 - 8 + r, px = x0 - 8 + q * V;
-    if (py < 0 || px < 0) continue;
-    P *g = pl + (size_t)py * stride + px;
-    uint16_t *l = buf + r * BP + q * V;
-    if (sizeof(P) == 1) {
-      if (store) {
-        uint2 w;
-        w.x = l[0] | l[1] << 8 | l[2] << 16 | (uint32_t)l[3] << 24;
-        w.y = l[4] | l[5] << 8 | l[6] << 16 | (uint32_t)l[7] << 24;
-        *(uint2 *)g = w;
-      } else {
-        const uint2 w = *(const uint2 *)g;
+// The region's 8-sample vectors: k = r * NV + q, NV = 9 (64-wide plane
+// superblock) or 5 (32-wide), at most 11 per lane.
+template <class P>
+struct LfRegion {
+  typedef typename std::conditional<sizeof(P) == 1, uint2, uint4>::type V;  // 8 samples
+  static constexpr int kMax = 11;
+  V v[kMax];
+};
+
+template <class P>
+DEV bool lf_vec(int k, int RW, int RH, int x0, int y0, int &r, int &q, int &py, int &px) {
+  const int NV = RW / 8;
+  r = NV == 9 ? k / 9 : k / 5;
+  q = k - r * NV;
+  py = y0 - 8 + r;
+  px = x0 - 8 + q * 8;
+  return k < RH * NV && py >= 0 && px >= 0;
+}
+
+// Issues every load of the region (into registers) at once.
+template <class P>
+DEV void lf_region_load(const P *pl, uint32_t stride, int x0, int y0, int RW, int RH, LfRegion<P> &rg) {
+  typedef typename LfRegion<P>::V V;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          l[i] = (w.x >> (8 * i)) & 0xff;
-          l[4 + i] = (w.y >> (8 * i)) & 0xff;
-        }
+  for (int i = 0; i < LfRegion<P>::kMax; i++) {
+    int r, q, py, px;
+    if (lf_vec<P>(threadIdx.x + 64 * i, RW, RH, x0, y0, r, q, py, px))
+      rg.v[i] = *(const V *)(pl + (size_t)py * stride + px);
+  }
+}
+
+template <class P>
+DEV void lf_region_to_lds(int x0, int y0, int RW, int RH, const LfRegion<P> &rg, uint16_t *buf) {
+#pragma unroll
+  for (int i = 0; i < LfRegion<P>::kMax; i++) {
+    int r, q, py, px;
+    if (!lf_vec<P>(threadIdx.x + 64 * i, RW, RH, x0, y0, r, q, py, px)) continue;
+    uint16_t *l = buf + r * BP + q * 8;
+    if (sizeof(P) == 1) {
+      const uint2 w = *(const uint2 *)&rg.v[i];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        l[k] = (w.x >> (8 * k)) & 0xff;
+        l[4 + k] = (w.y >> (8 * k)) & 0xff;
       }
     } else {
-      if (store) {
-        uint4 w;
-        w.x = l[0] | (uint32_t)l[1] << 16;
-        w.y = l[2] | (uint32_t)l[3] << 16;
-        w.z = l[4] | (uint32_t)l[5] << 16;
-        w.w = l[6] | (uint32_t)l[7] << 16;
-        *(uint4 *)g = w;
-      } else {
-        const uint4 w = *(const uint4 *)g;
-        l[0] = w.x & 0xffff; l[1] = w.x >> 16;
-        l[2] = w.y & 0xffff; l[3] = w.y >> 16;
-        l[4] = w.z & 0xffff; l[5] = w.z >> 16;
-        l[6] = w.w & 0xffff; l[7] = w.w >> 16;
-      }
+      const uint4 w = *(const uint4 *)&rg.v[i];
+      l[0] = w.x & 0xffff; l[1] = w.x >> 16;
+      l[2] = w.y & 0xffff; l[3] = w.y >> 16;
+      l[4] = w.z & 0xffff; l[5] = w.z >> 16;
+      l[6] = w.w & 0xffff; l[7] = w.w >> 16;
+    }
+  }
+}
+
+template <class P>
+DEV void lf_region_store(P *pl, uint32_t stride, int x0, int y0, int RW, int RH, const uint16_t *buf) {
+#pragma unroll
+  for (int i = 0; i < LfRegion<P>::kMax; i++) {
+    int r, q, py, px;
+    if (!lf_vec<P>(threadIdx.x + 64 * i, RW, RH, x0, y0, r, q, py, px)) continue;
+    const uint16_t *l = buf + r * BP + q * 8;
+    P *g = pl + (size_t)py * stride + px;
+    if (sizeof(P) == 1) {
+      uint2 w;
+      w.x = l[0] | l[1] << 8 | l[2] << 16 | (uint32_t)l[3] << 24;
+      w.y = l[4] | l[5] << 8 | l[6] << 16 | (uint32_t)l[7] << 24;
+      *(uint2 *)g = w;
+    } else {
+      uint4 w;
+      w.x = l[0] | (uint32_t)l[1] << 16;
+      w.y = l[2] | (uint32_t)l[3] << 16;
+      w.z = l[4] | (uint32_t)l[5] << 16;
+      w.w = l[6] | (uint32_t)l[7] << 16;
+      *(uint4 *)g = w;
     }
   }
 }
 
 // One pass: lane-owned lines of `len` samples, line i at buf + i * line,
-// samples `step` apart; edges at 8 + 4 e for e < n_edges.  dq[e * 16 + rn]
-// packs the edge's decision: (filter size + 1) | limit << 8 | blimit << 16 |
-// thresh << 24, 0 for no filter.  The next edge's decision and the 4
-// samples entering the window are fetched before the filter (the filter of
-// edge e writes at most up to position 4 e + 14).
+// samples `step` apart; edges at 8 + 4 e for e < n_edges (a multiple of 4).
+// dq[e * 16 + rn] packs the edge's decision: (filter size + 1) | limit << 8
+// | blimit << 16 | thresh << 24, 0 for no filter.  The window holds the 16
+// samples p7 .. q7 of the current edge, position x of the line in w[x & 15]:
+// with the edge loop unrolled by 4 every index is a constant, and sliding
+// the window is renaming.  The next edge's decision and the 4 samples
+// entering the window are fetched before the filter (the filter of edge e
+// writes at most up to position 4 e + 14).
+// Edge e = 4 m + K of lf_stream (K a constant: every window index is one).
+template <int K>
+DEV void lf_step(int (&w)[16], uint16_t *ln, int e, int step, int n_edges, const uint32_t *dq, int rn,
+                 uint32_t &d_next, int bd) {
+  // Reads past the line's end (positions >= len, at most 4 n_edges + 11)
+  // stay inside LfShared and never reach a result: they enter the window
+  // after the last edge that could read them (the last edge is 4 apart from
+  // the line's end, a narrow filter reading q0 .. q3 only) and are not
+  // written back.  Likewise the decision after the last edge.
+  const uint32_t d = d_next;
+  d_next = dq[(e + 1) * 16 + rn];
+  int nw[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) nw[i] = ln[(4 * e + 16 + i) * step];
+  if (d)
+    lf_apply([&](int kk) -> int & { return w[(4 * K + 8 + kk) & 15]; }, (int)(d & 0xff) - 1, (d >> 8) & 0xff,
+             (d >> 16) & 0xff, d >> 24, bd);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    ln[(4 * e + i) * step] = (uint16_t)w[4 * K + i];
+    w[4 * K + i] = nw[i];
+  }
+}
+
 DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, const uint32_t *dq, int rn,
                    int bd) {
   int w[16];
@@ -264,70 +329,18 @@ DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, 
 #pragma unroll
   for (int i = 0; i < 16; i++) w[i] = ln[i * step];
   uint32_t d_next = dq[rn];
-  // one copy of the filters: unrolled, the 16 edges would not fit the
-  // instruction cache
 #pragma unroll 1
-  for (int e = 0; e < n_edges; e++) {
-    const uint32_t d = d_next;
-    if (e + 1 < n_edges) d_next = dq[(e + 1) * 16 + rn];
-    const int base = 4 * e;
-    int nw[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int pos = base + 16 + i;
-      nw[i] = pos < len ? (int)ln[pos * step] : 0;
-    }
-    if (d)
-      lf_apply([&](int k) -> int & { return w[8 + k]; }, (int)(d & 0xff) - 1, (d >> 8) & 0xff, (d >> 16) & 0xff,
-               d >> 24, bd);
-#pragma unroll
-    for (int i = 0; i < 4; i++) ln[(base + i) * step] = (uint16_t)w[i];
-#pragma unroll
-    for (int i = 0; i < 12; i++) w[i] = w[i + 4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) w[12 + i] = nw[i];
+  for (int e = 0; e < n_edges; e += 4) {
+    lf_step<0>(w, ln, e, step, n_edges, dq, rn, d_next, bd);
+    lf_step<1>(w, ln, e + 1, step, n_edges, dq, rn, d_next, bd);
+    lf_step<2>(w, ln, e + 2, step, n_edges, dq, rn, d_next, bd);
+    lf_step<3>(w, ln, e + 3, step, n_edges, dq, rn, d_next, bd);
   }
+  // the last window: positions 4 n_edges + i at w[i]
   const int base = 4 * n_edges;
 #pragma unroll
   for (int i = 0; i < 16; i++)
     if (base + i < len) ln[(base + i) * step] = (uint16_t)w[i];
-}
-
-template <class P, int SW, int SH>
-DEV void lf_sb_stream(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
-  constexpr int RW = SW + 8, RH = SH + 8;
-  const int lane = threadIdx.x;
-  const int x0 = sb_col * SW, y0 = sb_row * SH;
-  const uint32_t stride = a.cur.stride[plane];
-  P *pl = (P *)a.cur.p[plane];
-  const int bd = a.g.bit_depth;
-  lf_region_io<P, RW>(pl, stride, x0, y0, RH, sh.buf, false);
-  __syncthreads();
-  if (lane < SH) lf_stream(sh.buf, (8 + lane) * BP, 1, RW, SW / 4, sh.dq[0], lane >> 2, bd);
-  __syncthreads();
-  if (lane < SW) lf_stream(sh.buf, 8 + lane, BP, RH, SH / 4, sh.dq[1], lane >> 2, bd);
-  __syncthreads();
-  lf_region_io<P, RW>(pl, stride, x0, y0, RH, sh.buf, true);
-}
-
-// Dispatches on the plane's superblock size (subsampling).
-template <class P>
-DEV void lf_sb(const LfArgs &a, LfShared &sh, int plane, int sb_row, int sb_col) {
-  const int sx = plane ? a.g.ss_x : 0, sy = plane ? a.g.ss_y : 0;
-  if (!sx && !sy)
-    lf_sb_stream<P, 64, 64>(a, sh, plane, sb_row, sb_col);
-  else if (sx && sy)
-    lf_sb_stream<P, 32, 32>(a, sh, plane, sb_row, sb_col);
-  else if (sx)
-    lf_sb_stream<P, 32, 64>(a, sh, plane, sb_row, sb_col);
-  else
-    lf_sb_stream<P, 64, 32>(a, sh, plane, sb_row, sb_col);
-}
-
-// The limits of every level in LDS (prm[l] = limit | blimit << 8 | thresh << 16).
-DEV void lf_load_params(const LfArgs &a, uint32_t *prm) {
-  for (int l = threadIdx.x; l < 64; l += blockDim.x)
-    prm[l] = a.p.limit[l] | (uint32_t)a.p.blimit[l] << 8 | (uint32_t)a.p.thresh[l] << 16;
 }
 
 // ---- one wavefront level per launch (decisions from k_lf_mask)
@@ -336,25 +349,41 @@ template <bool HBD>
 __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ LfShared sh;
-  __shared__ uint32_t prm[64];
   const uint32_t fi = range_frame(rg, nf, blockIdx.x);
   const LfArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
-  lf_load_params(a, prm);
-  const uint32_t code = a.sbs[rg[fi].first + blockIdx.x - rg[fi].wg];
-  const int plane = code >> 28, sb_row = (code >> 14) & 0x3fff, sb_col = code & 0x3fff;
-  const int n_sb = g.sb64_rows * g.sb64_cols, sb = sb_row * g.sb64_cols + sb_col;
+  // item j of level t: plane j % 3 of superblock (r, t - 2 r), rows from
+  // the first the level reaches (t - 2 r <= sb64_cols - 1)
+  const int t = (int)rg[fi].first, j = (int)(blockIdx.x - rg[fi].wg), C = (int)g.sb64_cols;
+  const int r_min = t >= C ? (t - C + 2) >> 1 : 0;
+  const int plane = j % 3, sb_row = r_min + j / 3, sb_col = t - 2 * sb_row;
+  const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+  const int SW = 64 >> sx, SH = 64 >> sy, RW = SW + 8, RH = SH + 8;
+  const int x0 = sb_col * SW, y0 = sb_row * SH;
+  const uint32_t stride = a.cur.stride[plane];
+  P *pl = (P *)a.cur.p[plane];
+  // every global load at once: both passes' decisions, the region (the
+  // superblock of this plane and its 8-sample halo above and left)
+  const size_t n_sb = (size_t)g.sb64_rows * C, sb = (size_t)sb_row * C + sb_col;
+  const uint4 dq0 = ((const uint4 *)(a.dq + ((plane * 2 + 0) * n_sb + sb) * 256))[lane];
+  const uint4 dq1 = ((const uint4 *)(a.dq + ((plane * 2 + 1) * n_sb + sb) * 256))[lane];
+  LfRegion<P> region;
+  lf_region_load<P>(pl, stride, x0, y0, RW, RH, region);
+  ((uint4 *)sh.dq[0])[lane] = dq0;
+  ((uint4 *)sh.dq[1])[lane] = dq1;
+  lf_region_to_lds<P>(x0, y0, RW, RH, region, sh.buf);
   __syncthreads();
+  // vertical edges (a lane per row), then horizontal edges (a lane per
+  // column): one copy of the edge loop for both passes and every geometry
+#pragma unroll 1
   for (int pass = 0; pass < 2; pass++) {
-    const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256;
-    for (int k = lane; k < 256; k += 64) {
-      const int f = a.mask_fs[o + k];
-      sh.dq[pass][k] = f < 0 ? 0u : (uint32_t)(f + 1) | prm[a.mask_lvl[o + k] & 63] << 8;
-    }
+    if (lane < (pass ? SW : SH))
+      lf_stream(sh.buf, pass ? 8 + lane : (8 + lane) * BP, pass ? BP : 1, pass ? RH : RW, (pass ? SH : SW) / 4,
+                sh.dq[pass], lane >> 2, g.bit_depth);
+    __syncthreads();
   }
-  __syncthreads();
-  lf_sb<P>(a, sh, plane, sb_row, sb_col);
+  lf_region_store<P>(pl, stride, x0, y0, RW, RH, sh.buf);
 }
 
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs) {

@@ -295,23 +295,23 @@ static void build_intra(Builder *g, const vp9r_record *rec) {
   }
 }
 
-// Loop filter wavefront levels: superblocks with equal c + 2r, all planes.
+// Loop filter wavefront levels: superblocks with equal c + 2r, all planes;
+// lf_level_start[t] = 3 x the superblocks of the levels before t.
 static void build_lf_levels(Builder *g, uint32_t rows, uint32_t cols) {
   if (g->lf_rows == rows && g->lf_cols == cols) return;
   g->lf_rows = rows;
   g->lf_cols = cols;
-  g->lf_codes.clear();
   g->lf_level_start.clear();
-  uint32_t T = (cols - 1) + 2 * (rows - 1) + 1;
+  const uint32_t T = (cols - 1) + 2 * (rows - 1) + 1;
+  uint32_t n = 0;
   for (uint32_t t = 0; t < T; t++) {
-    g->lf_level_start.push_back((uint32_t)g->lf_codes.size());
+    g->lf_level_start.push_back(n);
     for (uint32_t r = 0; r < rows; r++) {
-      int c = (int)t - 2 * (int)r;
-      if (c < 0 || c >= (int)cols) continue;
-      for (uint32_t p = 0; p < 3; p++) g->lf_codes.push_back(p << 28 | r << 14 | (uint32_t)c);
+      const int c = (int)t - 2 * (int)r;
+      if (c >= 0 && c < (int)cols) n += 3;
     }
   }
-  g->lf_level_start.push_back((uint32_t)g->lf_codes.size());
+  g->lf_level_start.push_back(n);
 }
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -727,14 +727,12 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   p.o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
   p.o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
   p.o_items = take(sizeof(vp9g::IntraItem) * g->intra_items.size());
-  p.o_lf = take(sizeof(uint32_t) * (do_lf ? g->lf_codes.size() : 0));
   p.bytes = off;
   p.n_tiles = (uint32_t)g->tiles.size();
   for (int k = 0; k < 4; k++) p.n_jobs[k] = (uint32_t)g->jobs[k].size();
   p.n_groups = (uint32_t)g->groups.size();
   p.n_items = (uint32_t)g->intra_items.size();
   p.item_level_start = g->item_level_start;
-  p.n_lf = do_lf ? (uint32_t)g->lf_codes.size() : 0;
   p.group_level_start = g->group_level_start;
   if (do_lf)
     p.lf_level_start = g->lf_level_start;
@@ -758,7 +756,6 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   put(p.o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
   put(p.o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
   put(p.o_items, g->items_sorted.data(), sizeof(vp9g::IntraItem) * g->items_sorted.size());
-  if (do_lf) put(p.o_lf, g->lf_codes.data(), sizeof(uint32_t) * g->lf_codes.size());
   return VP9R_OK;
 }
 
@@ -846,12 +843,12 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     const size_t o_clear = stage_take<vp9g::ClearArgs>(st, n), o_mc = stage_take<vp9g::McArgs>(st, n),
                  o_itx = stage_take<vp9g::ItxArgs>(st, n), o_intra = stage_take<vp9g::IntraArgs>(st, n),
                  o_lf = stage_take<vp9g::LfArgs>(st, n);
-    // LF decisions: per frame n_sb * 6 * 256 bytes of sizes and of levels
+    // LF decisions: per frame n_sb * 6 * 256 packed words
     size_t mask_bytes = 0;
     std::vector<size_t> mask_off(n);
     for (int i = 0; i < n; i++) {
       mask_off[i] = mask_bytes;
-      mask_bytes += (size_t)f[i]->p->hdr.sb64_rows * f[i]->p->hdr.sb64_cols * 6 * 256 * 2;
+      mask_bytes += (size_t)f[i]->p->hdr.sb64_rows * f[i]->p->hdr.sb64_cols * 6 * 256 * 4;
     }
     if (lf && mask_bytes > g->lf_mask.cap) {
       HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -905,11 +902,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       for (int i = 0; i < n; i++)
         if (f[i]->p->lf_level_start.size() > 1) levels = std::max(levels, f[i]->p->lf_level_start.size() - 1);
       for (size_t l = 0; l < levels; l++)
-        t_lf.push_back(table(
-            [&](int i) {
-              const auto &ls = f[i]->p->lf_level_start;
-              return l + 1 < ls.size() ? ls[l] : 0u;
-            },
+        t_lf.push_back(table([&](int) { return (uint32_t)l; },
             [&](int i) {
               const auto &ls = f[i]->p->lf_level_start;
               return l + 1 < ls.size() ? ls[l + 1] - ls[l] : 0u;
@@ -962,12 +955,9 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       la = vp9g::LfArgs{};
       la.g = p.geo;
       la.mi = (const vp9r_mi *)(dv + p.o_mi);
-      const size_t n_sb = (size_t)p.hdr.sb64_rows * p.hdr.sb64_cols;
-      la.mask_fs = (int8_t *)g->lf_mask.p + mask_off[i];
-      la.mask_lvl = (uint8_t *)g->lf_mask.p + mask_off[i] + n_sb * 6 * 256;
+      la.dq = (uint32_t *)((uint8_t *)g->lf_mask.p + mask_off[i]);
       la.cur = cur->planes;
       la.p = p.lfp;
-      la.sbs = (const uint32_t *)(dv + p.o_lf);
     }
     // one transfer into the next device slot
     BatchSlot &slot = g->slots[g->slot_next];

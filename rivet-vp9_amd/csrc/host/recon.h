@@ -39,8 +39,8 @@ struct Prepared {
   int32_t q[8][2][2];
   vp9g::LfParams lfp;
   size_t o_blocks = 0, o_txbs = 0, o_cpos = 0, o_cval = 0, o_mi = 0, o_tiles = 0, o_jobs[4] = {},
-         o_iidx = 0, o_groups = 0, o_lf = 0, o_items = 0;
-  uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0, n_items = 0, n_lf = 0;
+         o_iidx = 0, o_groups = 0, o_items = 0;
+  uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0, n_items = 0;
   std::vector<uint32_t> group_level_start, lf_level_start, item_level_start;
   size_t bytes = 0;
   // algorithmic HBM bytes of the frame per stage: inter prediction, inter
@@ -60,7 +60,7 @@ struct Builder {
   std::vector<int32_t> intra_level;
   std::vector<vp9g::IntraGroup> groups;
   std::vector<uint32_t> group_level_start;
-  std::vector<uint32_t> lf_codes, lf_level_start;
+  std::vector<uint32_t> lf_level_start;
   uint32_t lf_rows = 0, lf_cols = 0;
 };
 
