@@ -96,13 +96,23 @@ struct IntraItem {
   uint32_t block, first, count;
 };
 
+// An intra block for k_intra_blocks: its transform blocks are
+// txbs[first .. first + count) (n_y luma, n_u Cb, then Cr; one transform
+// size per plane), their coefficients coef_*[coef_first .. + coef_count).
+struct IntraBlockItem {
+  uint32_t first, count, coef_first, coef_count;
+  uint16_t mi_row, mi_col, n_y, n_u;
+  uint8_t mi_size, y_mode, uv_mode, flags;  // flags: segment_id | avail_u << 3 | avail_l << 4
+  uint32_t sub_modes;                       // 4 x 8 bits
+};
+
 struct IntraArgs {
   FrameGeom g;
   const vp9r_block *blocks;
   const vp9r_txb *txbs;
   const uint32_t *txb_index;  // groups: the intra txbs of each group
   const IntraGroup *groups;   // by level
-  const IntraItem *items;     // intra blocks by level
+  const IntraBlockItem *items;  // intra blocks by level
   const uint16_t *coef_pos;
   const int32_t *coef_val;
   Planes cur;

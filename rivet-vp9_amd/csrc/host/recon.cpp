@@ -706,6 +706,46 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
           return VP9R_ERR_INVALID_INPUT;
         }
     }
+    // the kernel's view of each block (by level): one transform size per
+    // plane, each grid position of a plane's transform blocks at most once
+    g->block_items.resize(g->items_sorted.size());
+    uint8_t seen[3][256];
+    for (size_t k = 0; k < g->items_sorted.size(); k++) {
+      const vp9g::IntraItem &it = g->items_sorted[k];
+      const vp9r_block &b = rec->blocks[it.block];
+      vp9g::IntraBlockItem &o = g->block_items[k];
+      o = vp9g::IntraBlockItem{};
+      o.first = it.first;
+      o.count = it.count;
+      const vp9r_txb &t0 = rec->txbs[it.first], &t1 = rec->txbs[it.first + it.count - 1];
+      o.coef_first = t0.coef_start;
+      o.coef_count = t1.coef_start + t1.ncoef > t0.coef_start ? t1.coef_start + t1.ncoef - t0.coef_start : 0;
+      o.mi_row = b.mi_row;
+      o.mi_col = b.mi_col;
+      o.mi_size = b.mi_size;
+      o.y_mode = b.y_mode;
+      o.uv_mode = b.uv_mode;
+      o.flags = (uint8_t)((b.segment_id & 7) | (b.avail_u ? 8 : 0) | (b.avail_l ? 16 : 0));
+      memcpy(&o.sub_modes, b.sub_modes, 4);
+      int tx[3] = {-1, -1, -1};
+      uint32_t n[3] = {0, 0, 0};
+      memset(seen, 0, sizeof seen);
+      for (uint32_t i = it.first; i < it.first + it.count; i++) {
+        const vp9r_txb &t = rec->txbs[i];
+        const int sx = t.plane ? h.ss_x : 0, sy = t.plane ? h.ss_y : 0;
+        const uint32_t bx = (b.mi_col * 8u) >> sx, by = (b.mi_row * 8u) >> sy;
+        const uint32_t cell = ((t.y - by) >> (2 + t.tx_size)) * 16 + ((t.x - bx) >> (2 + t.tx_size));
+        if ((tx[t.plane] >= 0 && tx[t.plane] != t.tx_size) || cell >= 256 || seen[t.plane][cell]) {
+          set_error("malformed record: intra transform blocks of mixed sizes or repeated positions");
+          return VP9R_ERR_INVALID_INPUT;
+        }
+        tx[t.plane] = t.tx_size;
+        seen[t.plane][cell] = 1;
+        n[t.plane]++;
+      }
+      o.n_y = (uint16_t)n[0];
+      o.n_u = (uint16_t)n[1];
+    }
   }
   work_bytes(rec, p);
   const bool do_lf = h.lf_level != 0;
@@ -726,7 +766,7 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   for (int k = 0; k < 4; k++) p.o_jobs[k] = take(sizeof(uint32_t) * g->jobs[k].size());
   p.o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
   p.o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
-  p.o_items = take(sizeof(vp9g::IntraItem) * g->intra_items.size());
+  p.o_items = take(sizeof(vp9g::IntraBlockItem) * g->block_items.size());
   p.bytes = off;
   p.n_tiles = (uint32_t)g->tiles.size();
   for (int k = 0; k < 4; k++) p.n_jobs[k] = (uint32_t)g->jobs[k].size();
@@ -755,7 +795,7 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   for (int k = 0; k < 4; k++) put(p.o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
   put(p.o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
   put(p.o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
-  put(p.o_items, g->items_sorted.data(), sizeof(vp9g::IntraItem) * g->items_sorted.size());
+  put(p.o_items, g->block_items.data(), sizeof(vp9g::IntraBlockItem) * g->block_items.size());
   return VP9R_OK;
 }
 
@@ -946,7 +986,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       na.txbs = ia.txbs;
       na.txb_index = (const uint32_t *)(dv + p.o_iidx);
       na.groups = (const vp9g::IntraGroup *)(dv + p.o_groups);
-      na.items = (const vp9g::IntraItem *)(dv + p.o_items);
+      na.items = (const vp9g::IntraBlockItem *)(dv + p.o_items);
       na.coef_pos = ia.coef_pos;
       na.coef_val = ia.coef_val;
       na.cur = cur->planes;

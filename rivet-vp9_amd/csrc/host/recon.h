@@ -55,6 +55,7 @@ struct Builder {
   std::vector<uint32_t> jobs[4];
   std::vector<uint32_t> intra_cnt, intra_first, intra_fill, intra_index;
   std::vector<vp9g::IntraItem> intra_items, items_sorted;
+  std::vector<vp9g::IntraBlockItem> block_items;  // items_sorted as the kernel reads them
   std::vector<int32_t> cell_level, item_level;
   std::vector<uint32_t> item_level_start;
   std::vector<int32_t> intra_level;
