@@ -19,6 +19,11 @@ p50/p99; and the CPU baseline: the oracle's scalar reconstruction and loop
 filter of the same records on one host core, whose per-frame MD5s the GPU's
 output is checked against (bit_exact).
 
+Batched stages ("batched"): one inter frame of the stream reconstructed 64
+times in one batch (every launch covers all 64 copies), per-stage GB/s of
+algorithmic bytes against the 8 TB/s HBM peak: the kernels with enough work
+to fill the GPU.
+
 Serving (in the same JSON line, "serving"): 32 streams from seed 5 arriving
 Poisson at 4/s, mixed sizes (640x360 / 1280x720 / 1920x1080) and formats
 (8-bit 4:2:0, 10-bit 4:2:0, 12-bit 4:4:4), 90-300 frames each, served by the
@@ -139,6 +144,52 @@ def cpu_baseline(frames_bytes, budget_s):
         best = fps if best is None else max(best, fps)
         md5s = md5_pass
     return best, md5s, passes, spent
+
+
+# ---- batched stages: the kernels at full occupancy
+
+def batched_stages(vp, gpu, frames, copies=64, reps=5):
+    """One inter frame of the stream reconstructed and loop filtered `copies`
+    times in one batch (vp9r_gpu_run_batch: every launch covers all copies),
+    `reps` batches timed per stage with HIP events: each stage's kernels with
+    enough work to fill the GPU.  Every copy must equal the frame decoded
+    alone."""
+    n = len(frames)
+    k = next(i for i in range(n // 3, n) if not frames[i]["hdr"].show_existing_frame
+             and not frames[i]["hdr"].frame_is_intra)
+    sched = Schedule(vp, gpu, frames)
+    for i in range(k):
+        sched.step(i)
+    h, prep = frames[k]["hdr"], frames[k]["prep"]
+    refs = [sched.slots[h.ref_frame_idx[j]] for j in range(3)]
+    alone = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+    gpu.run(prep, refs, alone)
+    want = hashlib.md5(alone.packed()).hexdigest()
+    curs = [gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y) for _ in range(copies)]
+    jobs = [(prep, refs, c) for c in curs]
+    gpu.run_batch(jobs)
+    gpu.synchronize()
+    exact = all(hashlib.md5(curs[i].packed()).hexdigest() == want for i in (0, copies // 2, copies - 1))
+    gpu.set_timing(True)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gpu.run_batch(jobs)
+    gpu.synchronize()
+    wall = time.perf_counter() - t0
+    ms, nfr = gpu.stage_times()
+    gpu.set_timing(False)
+    st = prep.stats()
+    out = {"frame_index": k, "copies_per_batch": copies, "batches": reps, "fps": round(nfr / wall, 1),
+           "bit_exact": exact, "stages": {}}
+    for s, name in enumerate(STAGES):
+        b = st[STAGE_BYTES[s]] * nfr
+        gbs = b / (ms[s] * 1e-3) / 1e9 if ms[s] > 0 else None
+        out["stages"][name] = {"kernel": STAGE_KERNELS[s], "ms_per_frame": round(ms[s] / nfr, 4),
+                               "GB_per_s": round(gbs, 1) if gbs else None,
+                               "hbm_frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    _log(f"bench: batched x{copies}: {out}")
+    return out
 
 
 # ---- serving workload
@@ -314,6 +365,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and the MD5 check)")
     ap.add_argument("--no-extras", action="store_true", help="skip the PCIe and latency passes")
     ap.add_argument("--no-serve", action="store_true", help="skip the multi-stream serving workload")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-stage measurement")
     ap.add_argument("--serve-streams", type=int, default=32, help="streams of the serving workload")
     ap.add_argument("--parse-threads", type=int, default=12, help="parse workers of the server")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -486,6 +538,8 @@ def main():
             bad = [i for i, (a, b) in enumerate(zip(gpu_md5, cpu_md5)) if a != b]
             _log(f"bench: MD5 MISMATCH at shown frames {bad[:10]}")
 
+    batched = None if args.no_batched else batched_stages(vp, gpu, frames)
+
     serving = None
     if not args.no_serve and not args.no_cpu:
         serving = serving_bench(vp, args.serve_streams, args.parse_threads)
@@ -513,10 +567,13 @@ def main():
         "parse_fps_one_core": round(n / t_parse, 2),
     }
     out.update(extras)
+    if batched is not None:
+        out["batched"] = batched
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)
-    if bit_exact is False or (serving is not None and not serving["bit_exact"]):
+    if bit_exact is False or (serving is not None and not serving["bit_exact"]) or \
+            (batched is not None and not batched["bit_exact"]):
         sys.exit(1)
 
 
