@@ -287,52 +287,54 @@ __global__ __launch_bounds__(64) void k_intra(const IntraArgs *fa, const Range *
 
 // ---- intra blocks (k_intra_blocks): one workgroup of 256 lanes per block
 //
-// Per plane, the block's region (with the row above and the column left of
-// it) is staged in LDS.  The residuals of all its transform blocks are
-// computed first, in parallel (teams of n0 lanes, as in the inter residual
-// kernel), into an LDS residual plane: the reference adds the clamped
+// The block's three planes are independent (each reads only its own
+// plane), so they are processed together.  Each plane's region (the block
+// with the row above and the column left of it) is staged in LDS.  The
+// residuals of all transform blocks are computed first, in parallel (teams
+// of n0 lanes, as in the inter residual kernel; luma, then both chroma
+// planes), into LDS residual planes: the reference adds the clamped
 // residual to the prediction with a clip to the sample range
 // (itx.rs:606-614), so the residual does not depend on the prediction.
 // Residuals are kept as int16: beyond +-32767 the clip gives 0 or the
 // maximum whatever the prediction, as it does for the full value.  Then
-// prediction and the add run as a wavefront over the plane's grid of
+// prediction and the add run as a wavefront over each plane's grid of
 // transform blocks: block (r, c) reads the reconstruction left, above and
 // above-left of it, and at 4x4 also above-right (recon.rs:61-69), so it is
-// at level c + r (c + 2 r at 4x4) and the blocks of a level are
-// independent.  Edges are read from the LDS tile as recon.rs:17-103 builds
-// them.
-
-constexpr int kIbThreads = 256;
-constexpr int kIbStageTxbs = 256, kIbStageCoefs = 2048, kIbCoefTile = 2 * 32 * 33;
-
-struct IntraBlockShared {
-  uint16_t tile[TS * TS];  // plane region: rows y0 - 1 .., columns x0 - 1 ..
-  int16_t res[64 * 64];    // residual of sample (x, y) at (y - y0) * 64 + x - x0
-  vp9r_txb txb[kIbStageTxbs];
-  uint16_t pos[kIbStageCoefs];
-  int32_t val[kIbStageCoefs];
-  uint16_t tmap[16 * 16];  // grid cell r * 16 + c -> transform block (offset in the plane), 0xffff: none
-  int32_t dcv[16];         // DC_PRED values of the current level's blocks
-};
+// at level c + r (c + 2 r at 4x4) and the blocks of a level, in all three
+// planes, are independent.  Edges are read from the LDS tiles as
+// recon.rs:17-103 builds them.  The LDS layout depends on the frame's
+// subsampling (ib_layout); the launch passes the largest of its frames'.
 
 struct ResDst {
   int16_t *base;
-  DEV void add(int i, int j, int r, int) { base[i * 64 + j] = (int16_t)min(max(r, -32768), 32767); }
+  int pitch;
+  DEV void add(int i, int j, int r, int) { base[i * pitch + j] = (int16_t)min(max(r, -32768), 32767); }
 };
 
-// The residuals of a plane's np transform blocks (size 4 << (n - 2)),
+// Geometry of one plane of the block (in LDS).
+struct IbGeo {
+  int x0, y0, BW, BH;         // plane block origin and size
+  int log2, kk, W, H, nlv;    // transform size, level step, grid, levels
+  int tpitch, rpitch;         // LDS tile and residual pitches
+  int max_x, max_y, n4w;      // last sample of the 8x8-aligned area; width in 4x4s
+  int mode_u;                 // the plane's mode (-1: luma of a sub-8x8 block, per 4x4)
+  uint32_t tile_off, res_off; // LDS byte offsets
+};
+static_assert(sizeof(IbGeo) <= kIbGeoBytes, "IbGeo");
+
+// The residuals of np transform blocks txbs[k0 ..) (size 4 << (n - 2)),
 // `teams` blocks a pass.
 template <class L, int n, class TX>
-DEV void ib_residuals(const IntraArgs &a, IntraBlockShared &sh, typename L::T *ctile, TX txb_at, uint32_t k0,
-                      uint32_t np, int plane, int x0, int y0, int seg, uint32_t c0, uint32_t n_c, bool hbd) {
+DEV void ib_residuals(const IntraArgs &a, uint8_t *lds, const IbGeo *geo, typename L::T *ctile, TX txb_at,
+                      uint32_t k0, uint32_t np, int seg, bool hbd, const uint16_t *cpos, const int32_t *cval,
+                      uint32_t c0, uint32_t n_c) {
   typedef typename L::T T;
   constexpr int n0 = 1 << n, S = n0 + 1;
-  constexpr int teams = n == 2 ? 64 : n == 3 ? 16 : n == 4 ? 4 : 2;
+  constexpr int teams = n == 2 ? 64 : n == 3 ? 32 : n == 4 ? 16 : 4;  // coefficient tiles: 1280 .. 4352 entries
+  static_assert(teams * n0 * S <= kIbCoefTile, "coefficient tiles");
   const int tid = threadIdx.x, team = tid / n0, ln = tid % n0;
   const bool in_team = team < teams;
   T *tile = ctile + (in_team ? team : 0) * (n0 * S);
-  const int pt = plane > 0;
-  const int32_t dcq = a.q[seg][pt][0], acq = a.q[seg][pt][1];
   const bool lossless = a.g.lossless;
   for (uint32_t base = 0; base < np; base += teams) {
     const uint32_t j = base + team;
@@ -346,160 +348,288 @@ DEV void ib_residuals(const IntraArgs &a, IntraBlockShared &sh, typename L::T *c
       for (int k = ln; k < n0 * S; k += n0) tile[k] = 0;
     __syncthreads();
     if (active) {
-      const bool staged = t.coef_start >= c0 && t.coef_start - c0 + t.ncoef <= n_c;
-      const uint16_t *pos = staged ? sh.pos + (t.coef_start - c0) : a.coef_pos + t.coef_start;
-      const int32_t *val = staged ? sh.val + (t.coef_start - c0) : a.coef_val + t.coef_start;
-      for (int k = ln; k < t.ncoef; k += n0) {
-        const int p = pos[k] & (n0 * n0 - 1);
-        tile[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(val[k], p, dcq, acq, n == 5);
+      const int pt = t.plane > 0;
+      const int32_t dcq = a.q[seg][pt][0], acq = a.q[seg][pt][1];
+      if (t.coef_start >= c0 && t.coef_start - c0 + t.ncoef <= n_c) {
+        const uint32_t o = t.coef_start - c0;
+        for (int k = ln; k < t.ncoef; k += n0) {
+          const int p = cpos[o + k] & (n0 * n0 - 1);
+          tile[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(cval[o + k], p, dcq, acq, n == 5);
+        }
+      } else {
+        const uint16_t *pos = a.coef_pos + t.coef_start;
+        const int32_t *val = a.coef_val + t.coef_start;
+        for (int k = ln; k < t.ncoef; k += n0) {
+          const int p = pos[k] & (n0 * n0 - 1);
+          tile[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(val[k], p, dcq, acq, n == 5);
+        }
       }
     }
     __syncthreads();
     const bool dc_only = !hbd && active && t.eob == 1 && t.tx_type == DCT_DCT && !lossless;
     const int32_t dc_v = dc_only ? dc_only_value((int32_t)tile[0], n) : 0;
-    ResDst dst{sh.res + (active ? (t.y - y0) * 64 + (t.x - x0) : 0)};
+    ResDst dst{(int16_t *)lds, 0};
+    if (active) {
+      const IbGeo &G = geo[t.plane];
+      dst.base = (int16_t *)(lds + G.res_off) + (t.y - G.y0) * G.rpitch + (t.x - G.x0);
+      dst.pitch = G.rpitch;
+    }
     itx_2d<L, n>(tile, active ? ln : n0, t.tx_type, lossless, dc_only, dc_v, dst, (1 << a.g.bit_depth) - 1);
     __syncthreads();
   }
 }
+
+template <class L, class TX>
+DEV void ib_residuals_any(int tx, const IntraArgs &a, uint8_t *lds, const IbGeo *geo, typename L::T *ctile,
+                          TX txb_at, uint32_t k0, uint32_t np, int seg, bool hbd, const uint16_t *cpos,
+                          const int32_t *cval, uint32_t c0, uint32_t n_c) {
+  switch (tx) {  // ends with a barrier
+    case 0: ib_residuals<L, 2>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
+    case 1: ib_residuals<L, 3>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
+    case 2: ib_residuals<L, 4>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
+    default: ib_residuals<L, 5>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
+  }
+}
+
+// The intra block workgroup's LDS (laid out by ib_layout).
+extern __shared__ __align__(16) uint8_t ib_lds[];
+
+// The edges of a transform block of a plane staged in the intra block's LDS
+// (recon.rs:17-103).
+struct Edges {
+  uint32_t tile_off;  // bytes into the LDS (an offset, so that reads stay LDS reads)
+  int tp, x, y, x0, y0, size, max_x, max_y, base;
+  bool have_left, have_above, a4;  // a4: above-right real (4x4 not on the block's right)
+  DEV int T(int py, int px) const {
+    return ((const uint16_t *)(ib_lds + tile_off))[(py - y0 + 1) * tp + (px - x0 + 1)];
+  }
+  DEV int A(int k) const {
+    if (!have_above) return base - 1;
+    if (k < 0) return have_left ? T(y - 1, min(max_x, x - 1)) : base + 1;
+    return T(y - 1, min(max_x, x + ((k >= size && !a4) ? size - 1 : k)));
+  }
+  DEV int Lf(int k) const { return have_left ? T(min(max_y, y + k), x - 1) : base + 1; }
+  // DC_PRED's value (intra.rs:145-168) for size 4 << (LOG2 - 2)
+  template <int LOG2>
+  DEV int dc(int bd) const {
+    constexpr int size = 1 << LOG2;
+    int sa = 0, sl = 0;
+    if (have_above) {
+#pragma unroll
+      for (int k = 0; k < size; k++) sa += T(y - 1, min(max_x, x + k));
+    }
+    if (have_left) {
+#pragma unroll
+      for (int k = 0; k < size; k++) sl += T(min(max_y, y + k), x - 1);
+    }
+    if (have_left && have_above) return (sa + sl + size) >> (LOG2 + 1);
+    if (have_left || have_above) return (sa + sl + (size >> 1)) >> LOG2;
+    return 1 << (bd - 1);
+  }
+};
 
 template <bool HBD>
 __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ IntraBlockShared sh;
-  __shared__ T ctile[kIbCoefTile];
   const int tid = threadIdx.x;
   const uint32_t fi = range_frame(rg, nf, blockIdx.x);
   const IntraArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   const IntraBlockItem it = a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
-  // stage the transform block records and the coefficients they cover
-  const uint32_t n_st = min(it.count, (uint32_t)kIbStageTxbs), n_c = min(it.coef_count, (uint32_t)kIbStageCoefs);
-  const uint32_t c0 = it.coef_first;
-  for (uint32_t k = tid; k < n_st; k += kIbThreads) sh.txb[k] = a.txbs[it.first + k];
+  const IbLayout ly = ib_layout(g.ss_x, g.ss_y, (int)sizeof(T));
+  vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
+  uint16_t *tmap = (uint16_t *)(ib_lds + ly.tmap);
+  int32_t *dcv = (int32_t *)(ib_lds + ly.dcv);
+  IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
+  T *ctile = (T *)(ib_lds + ly.ctile);
+  // the transform block records, all in LDS (at most ib_max_txbs; the host
+  // checks each plane's grid, which bounds the count)
+  for (uint32_t k = tid; k < it.count; k += kIbThreads) stx[k] = a.txbs[it.first + k];
+  // and the coefficients they cover (the residual passes read them at once)
+  uint16_t *cpos = (uint16_t *)(ib_lds + ly.cpos);
+  int32_t *cval = (int32_t *)(ib_lds + ly.cval);
+  const uint32_t c0 = it.coef_first, n_c = min(it.coef_count, (uint32_t)kIbStageCoefs);
   for (uint32_t k = tid; k < n_c; k += kIbThreads) {
-    sh.pos[k] = a.coef_pos[c0 + k];
-    sh.val[k] = a.coef_val[c0 + k];
+    cpos[k] = a.coef_pos[c0 + k];
+    cval[k] = a.coef_val[c0 + k];
   }
-  auto txb_at = [&](uint32_t k) -> vp9r_txb { return k < n_st ? sh.txb[k] : a.txbs[it.first + k]; };
+  auto txb_at = [&](uint32_t k) -> vp9r_txb { return stx[k]; };
   const int mi_size = it.mi_size, seg = it.flags & 7;
   const bool avail_u = (it.flags & 8) != 0, avail_l = (it.flags & 16) != 0;
   const int bsize = mi_size < BLOCK_8X8 ? BLOCK_8X8 : mi_size;
   const int bw8 = num8x8_wide(bsize), bh8 = num8x8_high(bsize);
   const int bd = g.bit_depth, maxv = (1 << bd) - 1, base = 1 << (bd - 1);
-  uint32_t k0 = 0;
-  for (int plane = 0; plane < 3; plane++) {
-    const uint32_t np = plane == 0 ? it.n_y : plane == 1 ? it.n_u : it.count - it.n_y - it.n_u;
+  const uint32_t np0 = it.n_y, np1 = it.n_u, np2 = it.count - it.n_y - it.n_u;
+  // every plane's region: the block, the row above it (from above-left) and
+  // the column left of it; samples of the block outside the 8x8-aligned
+  // area are not reconstructed and keep the frame's
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    const uint32_t np = p == 0 ? np0 : p == 1 ? np1 : np2;
     if (np == 0) continue;
-    const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+    const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
     const int x0 = (it.mi_col * 8) >> sx, y0 = (it.mi_row * 8) >> sy;
-    const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy;
-    const int lw = 31 - __clz(BW);  // BW is a power of two
-    const uint32_t stride = a.cur.stride[plane];
-    P *pl = (P *)a.cur.p[plane];
-    // the region: the block, the row above it (from above-left) and the
-    // column left of it; samples of the block outside the 8x8-aligned area
-    // are not reconstructed and keep the frame's
+    const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy, lw = 31 - __clz(BW);
+    const int tp = (64 >> sx) + 1, rp = 64 >> sx;
+    uint16_t *tile = (uint16_t *)(ib_lds + ly.tile[p]);
+    int16_t *res = (int16_t *)(ib_lds + ly.res[p]);
+    const uint32_t stride = a.cur.stride[p];
+    const P *pl = (const P *)a.cur.p[p];
     const uint32_t dw = BW + 1, mw = ((1u << 20) + dw - 1) / dw;  // k / (BW + 1) as (k * mw) >> 20
     for (int k = tid; k < (BH + 1) * (BW + 1); k += kIbThreads) {
       const int r = (int)(((uint32_t)k * mw) >> 20), c = k - r * (BW + 1);
       const int py = y0 - 1 + r, px = x0 - 1 + c;
-      if (py >= 0 && px >= 0) sh.tile[r * TS + c] = pl[(size_t)py * stride + px];
+      if (py >= 0 && px >= 0) tile[r * tp + c] = pl[(size_t)py * stride + px];
     }
-    for (int k = tid; k < BW * BH; k += kIbThreads) sh.res[(k >> lw) * 64 + (k & (BW - 1))] = 0;
-    for (int k = tid; k < 256; k += kIbThreads) sh.tmap[k] = 0xffff;
-    __syncthreads();
-    const int tx = txb_at(k0).tx_size, log2 = tx + 2, size = 1 << log2;
-    for (uint32_t j = tid; j < np; j += kIbThreads) {
-      const vp9r_txb t = txb_at(k0 + j);
-      sh.tmap[((t.y - y0) >> log2) * 16 + ((t.x - x0) >> log2)] = (uint16_t)j;
-    }
-    switch (tx) {  // ends with a barrier
-      case 0: ib_residuals<L, 2>(a, sh, ctile, txb_at, k0, np, plane, x0, y0, seg, c0, n_c, HBD); break;
-      case 1: ib_residuals<L, 3>(a, sh, ctile, txb_at, k0, np, plane, x0, y0, seg, c0, n_c, HBD); break;
-      case 2: ib_residuals<L, 4>(a, sh, ctile, txb_at, k0, np, plane, x0, y0, seg, c0, n_c, HBD); break;
-      default: ib_residuals<L, 5>(a, sh, ctile, txb_at, k0, np, plane, x0, y0, seg, c0, n_c, HBD); break;
-    }
-    // the wavefront over the grid of transform blocks
-    const int W = max(BW >> log2, 1), H = max(BH >> log2, 1);
-    const int kk = tx == TX_4X4 ? 2 : 1, n_lv = (W - 1) + kk * (H - 1) + 1;
-    const int max_x = (int)((g.mi_cols * 8) >> sx) - 1, max_y = (int)((g.mi_rows * 8) >> sy) - 1;
-    const int n4w = num4x4_wide(bsize) >> sx, step4 = 1 << tx;
-    const int mode_u = plane > 0 ? it.uv_mode : (mi_size >= BLOCK_8X8 ? it.y_mode : -1);
-    bool any_dc = mode_u == DC_PRED;
-    if (mode_u < 0)
-      for (int q = 0; q < 4; q++) any_dc = any_dc || ((it.sub_modes >> (8 * q)) & 0xff) == DC_PRED;
-    auto mode_of = [&](const vp9r_txb &t) -> int {
-      if (mode_u >= 0) return mode_u;
-      const int xi = (t.x - x0) >> 2, yi = (t.y - y0) >> 2;
-      const int block_idx = (yi / step4) * ((n4w + step4 - 1) / step4) + xi / step4;
-      return (int)((it.sub_modes >> (8 * (block_idx & 3))) & 0xff);
-    };
-#define TILE(py, px) ((int)sh.tile[((py) - y0 + 1) * TS + ((px) - x0 + 1)])
-    // the edges of transform block t (recon.rs:17-103)
-    auto edges = [&](const vp9r_txb &t, bool &have_left, bool &have_above, bool &not_on_right) {
-      const int xi = (t.x - x0) >> 2, yi = (t.y - y0) >> 2;
-      have_left = avail_l || xi > 0;
-      have_above = avail_u || yi > 0;
-      not_on_right = xi + step4 < n4w;
-    };
-    auto above_of = [&](int x, int y, bool have_left, bool have_above, bool not_on_right) {
-      return [=](int k) -> int {
-        if (!have_above) return base - 1;
-        if (k < 0) return have_left ? TILE(y - 1, min(max_x, x - 1)) : base + 1;
-        const int ii = (k >= size && !(not_on_right && tx == TX_4X4)) ? size - 1 : k;
-        return TILE(y - 1, min(max_x, x + ii));
-      };
-    };
-    auto left_of = [&](int x, int y, bool have_left) {
-      return [=](int k) -> int { return have_left ? TILE(min(max_y, y + k), x - 1) : base + 1; };
-    };
-    for (int lv = 0; lv < n_lv; lv++) {
-      const int r_lo = lv > W - 1 ? (lv - (W - 1) + kk - 1) / kk : 0;
-      const int r_hi = min(H - 1, lv / kk);
-      const int n_l = r_hi - r_lo + 1;
-      if (n_l <= 0) continue;
-      if (any_dc) {
-        if (tid < n_l) {
-          const int r = r_lo + tid, c = lv - kk * r;
-          const uint16_t ix = sh.tmap[r * 16 + c];
-          if (ix != 0xffff) {
-            const vp9r_txb t = txb_at(k0 + ix);
-            if (mode_of(t) == DC_PRED) {
-              bool hl, ha, nr;
-              edges(t, hl, ha, nr);
-              sh.dcv[tid] = dc_value_at(log2, above_of(t.x, t.y, hl, ha, nr), left_of(t.x, t.y, hl), hl, ha, bd);
+    for (int k = tid; k < BW * BH; k += kIbThreads) res[(k >> lw) * rp + (k & (BW - 1))] = 0;
+  }
+  for (int k = tid; k < 3 * 256; k += kIbThreads) tmap[k] = 0xffff;
+  __syncthreads();
+  if (tid < 3) {
+    const int p = tid;
+    const uint32_t np = p == 0 ? np0 : p == 1 ? np1 : np2, k0 = p == 0 ? 0 : p == 1 ? np0 : np0 + np1;
+    const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
+    IbGeo G;
+    G.x0 = (it.mi_col * 8) >> sx;
+    G.y0 = (it.mi_row * 8) >> sy;
+    G.BW = (bw8 * 8) >> sx;
+    G.BH = (bh8 * 8) >> sy;
+    const int tx = np ? txb_at(k0).tx_size : 0;
+    G.log2 = tx + 2;
+    G.kk = tx == TX_4X4 ? 2 : 1;
+    G.W = max(G.BW >> G.log2, 1);
+    G.H = max(G.BH >> G.log2, 1);
+    G.nlv = np ? (G.W - 1) + G.kk * (G.H - 1) + 1 : 0;
+    G.tpitch = (64 >> sx) + 1;
+    G.rpitch = 64 >> sx;
+    G.max_x = (int)((g.mi_cols * 8) >> sx) - 1;
+    G.max_y = (int)((g.mi_rows * 8) >> sy) - 1;
+    G.n4w = num4x4_wide(bsize) >> sx;
+    G.mode_u = p > 0 ? it.uv_mode : (mi_size >= BLOCK_8X8 ? it.y_mode : -1);
+    G.tile_off = p == 0 ? ly.tile[0] : p == 1 ? ly.tile[1] : ly.tile[2];
+    G.res_off = p == 0 ? ly.res[0] : p == 1 ? ly.res[1] : ly.res[2];
+    geo[p] = G;
+  }
+  __syncthreads();
+  for (uint32_t j = tid; j < it.count; j += kIbThreads) {
+    const vp9r_txb t = txb_at(j);
+    const IbGeo &G = geo[t.plane];
+    tmap[t.plane * 256 + ((t.y - G.y0) >> G.log2) * 16 + ((t.x - G.x0) >> G.log2)] = (uint16_t)j;
+  }
+  // residuals: luma, then both chroma planes (one transform size); each
+  // ends with a barrier, after which the grids are visible too
+  if (np0)
+    ib_residuals_any<L>(geo[0].log2 - 2, a, ib_lds, geo, ctile, txb_at, 0, np0, seg, HBD, cpos, cval, c0, n_c);
+  if (np1 + np2)
+    ib_residuals_any<L>(txb_at(np0).tx_size, a, ib_lds, geo, ctile, txb_at, np0, np1 + np2, seg, HBD, cpos, cval,
+                        c0, n_c);
+  if (!np0 && !(np1 + np2)) __syncthreads();
+  // the wavefront over the three grids
+  const int nlv = max(max(geo[0].nlv, geo[1].nlv), geo[2].nlv);
+  bool any_dc = (np0 && geo[0].mode_u == DC_PRED) || ((np1 + np2) && it.uv_mode == DC_PRED);
+  if (np0 && geo[0].mode_u < 0)
+    for (int q = 0; q < 4; q++) any_dc = any_dc || ((it.sub_modes >> (8 * q)) & 0xff) == DC_PRED;
+  auto mode_of = [&](const vp9r_txb &t, const IbGeo &G) -> int {
+    if (G.mode_u >= 0) return G.mode_u;
+    const int step4 = 1 << (G.log2 - 2);
+    const int xi = (t.x - G.x0) >> 2, yi = (t.y - G.y0) >> 2;
+    const int block_idx = (yi / step4) * ((G.n4w + step4 - 1) / step4) + xi / step4;
+    return (int)((it.sub_modes >> (8 * (block_idx & 3))) & 0xff);
+  };
+  // level lv of plane G: rows r_lo .. of the grid, n_l blocks
+  auto level = [&](const IbGeo &G, int lv, int &r_lo) -> int {
+    if (lv >= G.nlv) return 0;
+    r_lo = lv > G.W - 1 ? (lv - (G.W - 1) + G.kk - 1) / G.kk : 0;
+    const int r_hi = min(G.H - 1, lv / G.kk);
+    return max(r_hi - r_lo + 1, 0);
+  };
+  // the edges of transform block t of plane G (Edges)
+  auto edges_of = [&](const vp9r_txb &t, const IbGeo &G) {
+    Edges e;
+    e.tile_off = G.tile_off;
+    e.tp = G.tpitch;
+    e.x = t.x;
+    e.y = t.y;
+    e.x0 = G.x0;
+    e.y0 = G.y0;
+    e.size = 1 << G.log2;
+    e.max_x = G.max_x;
+    e.max_y = G.max_y;
+    e.base = base;
+    const int xi = (t.x - G.x0) >> 2, yi = (t.y - G.y0) >> 2, step4 = 1 << (G.log2 - 2);
+    e.have_left = avail_l || xi > 0;
+    e.have_above = avail_u || yi > 0;
+    e.a4 = G.log2 == 2 && xi + step4 < G.n4w;
+    return e;
+  };
+  for (int lv = 0; lv < nlv; lv++) {
+    int r0 = 0, r1 = 0, r2 = 0;
+    const int n0 = level(geo[0], lv, r0), n1 = level(geo[1], lv, r1), n2 = level(geo[2], lv, r2);
+    if (any_dc) {
+      if (tid < n0 + n1 + n2) {
+        const int p = tid < n0 ? 0 : tid < n0 + n1 ? 1 : 2;
+        const int jt = tid - (p == 0 ? 0 : p == 1 ? n0 : n0 + n1);
+        const IbGeo &G = geo[p];
+        const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - G.kk * r;
+        const uint16_t ix = tmap[p * 256 + r * 16 + c];
+        if (ix != 0xffff) {
+          const vp9r_txb t = txb_at(ix);
+          if (mode_of(t, G) == DC_PRED) {
+            const Edges e = edges_of(t, G);
+            int v;
+            switch (G.log2) {  // every edge read in flight at once
+              case 2: v = e.dc<2>(bd); break;
+              case 3: v = e.dc<3>(bd); break;
+              case 4: v = e.dc<4>(bd); break;
+              default: v = e.dc<5>(bd); break;
             }
+            dcv[p * 16 + jt] = v;
           }
         }
-        __syncthreads();
-      }
-      for (int item = tid; item < (n_l << (2 * log2)); item += kIbThreads) {
-        const int jt = item >> (2 * log2), s = item & ((1 << (2 * log2)) - 1);
-        const int r = r_lo + jt, c = lv - kk * r;
-        const uint16_t ix = sh.tmap[r * 16 + c];
-        if (ix == 0xffff) continue;
-        const vp9r_txb t = txb_at(k0 + ix);
-        bool hl, ha, nr;
-        edges(t, hl, ha, nr);
-        const int mode = mode_of(t);
-        const int i = s >> log2, j = s & (size - 1);
-        const int pv = intra_sample_at(mode, i, j, size, above_of(t.x, t.y, hl, ha, nr), left_of(t.x, t.y, hl),
-                                    mode == DC_PRED ? sh.dcv[jt] : 0, maxv);
-        const int ry = t.y - y0 + i, rx = t.x - x0 + j;
-        sh.tile[(ry + 1) * TS + rx + 1] = (uint16_t)iclamp(pv + sh.res[ry * 64 + rx], 0, maxv);
       }
       __syncthreads();
     }
-#undef TILE
+    const int i0 = n0 << (2 * geo[0].log2), i1 = n1 << (2 * geo[1].log2), i2 = n2 << (2 * geo[2].log2);
+    for (int item = tid; item < i0 + i1 + i2; item += kIbThreads) {
+      const int p = item < i0 ? 0 : item < i0 + i1 ? 1 : 2;
+      const int local = item - (p == 0 ? 0 : p == 1 ? i0 : i0 + i1);
+      const IbGeo &G = geo[p];
+      const int jt = local >> (2 * G.log2), s = local & ((1 << (2 * G.log2)) - 1);
+      const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - G.kk * r;
+      const uint16_t ix = tmap[p * 256 + r * 16 + c];
+      if (ix == 0xffff) continue;
+      const vp9r_txb t = txb_at(ix);
+      const Edges e = edges_of(t, G);
+      const int mode = mode_of(t, G);
+      const int size = 1 << G.log2, i = s >> G.log2, j = s & (size - 1);
+      const int pv = intra_sample_at(
+          mode, i, j, size, [&](int k) { return e.A(k); }, [&](int k) { return e.Lf(k); },
+          mode == DC_PRED ? dcv[p * 16 + jt] : 0, maxv);
+      const int ry = t.y - G.y0 + i, rx = t.x - G.x0 + j;
+      const int16_t *res = (const int16_t *)(ib_lds + G.res_off);
+      uint16_t *tile = (uint16_t *)(ib_lds + G.tile_off);
+      tile[(ry + 1) * G.tpitch + rx + 1] = (uint16_t)iclamp(pv + res[ry * G.rpitch + rx], 0, maxv);
+    }
+    __syncthreads();
+  }
+  // the blocks back to the frame
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    const uint32_t np = p == 0 ? np0 : p == 1 ? np1 : np2;
+    if (np == 0) continue;
+    const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
+    const int x0 = (it.mi_col * 8) >> sx, y0 = (it.mi_row * 8) >> sy;
+    const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy, lw = 31 - __clz(BW);
+    const int tp = (64 >> sx) + 1;
+    const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
+    const uint32_t stride = a.cur.stride[p];
+    P *pl = (P *)a.cur.p[p];
     for (int k = tid; k < BW * BH; k += kIbThreads) {
       const int r = k >> lw, c = k & (BW - 1);
-      pl[(size_t)(y0 + r) * stride + x0 + c] = (P)sh.tile[(r + 1) * TS + c + 1];
+      pl[(size_t)(y0 + r) * stride + x0 + c] = (P)tile[(r + 1) * tp + c + 1];
     }
-    __syncthreads();  // the next plane's region reuses the tile
-    k0 += np;
   }
 }
 
@@ -511,12 +641,24 @@ void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n
     hipLaunchKernelGGL(k_intra<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
 }
 
-void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
+void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
+                         uint32_t lds_bytes) {
   if (!wgs) return;
+  // dynamic LDS above 64 KB (4:4:4 at high bit depth) needs the attribute
+  static bool attr_set[2] = {false, false};
+  if (!attr_set[hbd]) {
+    if (hbd)
+      (void)hipFuncSetAttribute((const void *)k_intra_blocks<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024);
+    else
+      (void)hipFuncSetAttribute((const void *)k_intra_blocks<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024);
+    attr_set[hbd] = true;
+  }
   if (hbd)
-    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(wgs), dim3(kIbThreads), 0, s, fa, r, n);
+    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n);
   else
-    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(wgs), dim3(kIbThreads), 0, s, fa, r, n);
+    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n);
 }
 
 // Per-kernel seam: one intra::predict call (intra.rs:19-29) on host-given

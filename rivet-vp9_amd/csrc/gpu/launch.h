@@ -106,6 +106,47 @@ struct IntraBlockItem {
   uint32_t sub_modes;                       // 4 x 8 bits
 };
 
+// LDS of one k_intra_blocks workgroup (kIbThreads lanes) for a frame of
+// subsampling (ss_x, ss_y) and transform lane width t_bytes (4 at 8 bits, 8
+// above): byte offsets of the transform block records (all of the block's),
+// the first kIbStageCoefs coefficients (positions, values), the three
+// transform grids, DC values, plane geometry, coefficient tiles, and per
+// plane the region tile ((64 >> sx) + 1 columns, u16) and residuals
+// ((64 >> sx) columns, int16).  The launch passes `bytes` (the largest of
+// its frames') as dynamic LDS.
+constexpr int kIbThreads = 256, kIbCoefTile = 4352, kIbGeoBytes = 80, kIbStageCoefs = 2048;
+// The most transform blocks an intra block has: 64x64 at 4x4 in every plane.
+__host__ __device__ inline uint32_t ib_max_txbs(int ss_x, int ss_y) { return 256 + 2 * (256u >> (ss_x + ss_y)); }
+struct IbLayout {
+  uint32_t txb, cpos, cval, tmap, dcv, geo, ctile, tile[3], res[3], bytes;
+};
+__host__ __device__ inline IbLayout ib_layout(int ss_x, int ss_y, int t_bytes) {
+  IbLayout l{};
+  uint32_t o = 0;
+  auto take = [&o](uint32_t n) {
+    const uint32_t r = o;
+    o = (o + n + 15u) & ~15u;
+    return r;
+  };
+  l.txb = take(ib_max_txbs(ss_x, ss_y) * (uint32_t)sizeof(vp9r_txb));
+  l.cpos = take(kIbStageCoefs * 2);
+  l.cval = take(kIbStageCoefs * 4);
+  l.tmap = take(3 * 256 * 2);
+  l.dcv = take(3 * 16 * 4);
+  l.geo = take(3 * kIbGeoBytes);
+  l.ctile = take(kIbCoefTile * (uint32_t)t_bytes);
+  for (int p = 0; p < 3; p++) {
+    const int sx = p ? ss_x : 0, sy = p ? ss_y : 0;
+    l.tile[p] = take((uint32_t)((64 >> sx) + 1) * (uint32_t)((64 >> sy) + 1) * 2);
+  }
+  for (int p = 0; p < 3; p++) {
+    const int sx = p ? ss_x : 0, sy = p ? ss_y : 0;
+    l.res[p] = take((uint32_t)(64 >> sx) * (uint32_t)(64 >> sy) * 2);
+  }
+  l.bytes = o;
+  return l;
+}
+
 struct IntraArgs {
   FrameGeom g;
   const vp9r_block *blocks;
@@ -147,7 +188,8 @@ void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int tx_size,
                       bool hbd);
 void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
-void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
+                         uint32_t lds_bytes);
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs);
 void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
 

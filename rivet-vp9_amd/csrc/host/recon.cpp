@@ -743,6 +743,14 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
         seen[t.plane][cell] = 1;
         n[t.plane]++;
       }
+      if (it.count > vp9g::ib_max_txbs(h.ss_x, h.ss_y)) {
+        set_error("malformed record: too many transform blocks in an intra block");
+        return VP9R_ERR_INVALID_INPUT;
+      }
+      if (n[1] && n[2] && tx[1] != tx[2]) {
+        set_error("malformed record: intra chroma transform blocks of two sizes");
+        return VP9R_ERR_INVALID_INPUT;
+      }
       o.n_y = (uint16_t)n[0];
       o.n_u = (uint16_t)n[1];
     }
@@ -1020,11 +1028,14 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       for (int k = 0; k < 4; k++)
         vp9g::launch_itx_inter(s, (const vp9g::ItxArgs *)(d + o_itx), R(t_itx[k]), n, t_itx[k].wgs, k, hbd);
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
+      uint32_t ib_lds = 0;
+      for (int i = 0; i < n; i++)
+        ib_lds = std::max(ib_lds, vp9g::ib_layout(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y, hbd ? 8 : 4).bytes);
       for (const LaunchTable &t : t_intra) {
         if (groups)
           vp9g::launch_intra(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
         else
-          vp9g::launch_intra_blocks(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
+          vp9g::launch_intra_blocks(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd, ib_lds);
       }
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
     }
