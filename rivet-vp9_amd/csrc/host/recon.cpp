@@ -451,12 +451,17 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   g->intra_blocks = !(im && strcmp(im, "groups") == 0);
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->lf_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->copy_stream) (void)hipStreamDestroy(g->copy_stream);
     delete g;
     return hip_fail(e, "hipStreamCreate");
   }
   e = hipEventCreateWithFlags(&g->ev_upload, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_up, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_pre, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_lf_done, hipEventDisableTiming);
   for (int i = 0; i < kBatchSlots && e == hipSuccess; i++)
     e = hipEventCreateWithFlags(&g->slots[i].ev, hipEventDisableTiming);
   if (e != hipSuccess) {
@@ -472,13 +477,18 @@ void vp9r_gpu_free(vp9r_gpu *g) {
   (void)hipSetDevice(g->device);
   (void)hipStreamSynchronize(g->stream);
   (void)hipStreamSynchronize(g->copy_stream);
+  if (g->lf_stream) (void)hipStreamSynchronize(g->lf_stream);
   if (g->ev_upload) (void)hipEventDestroy(g->ev_upload);
+  for (hipEvent_t e : {g->ev_up, g->ev_pre, g->ev_lf_done})
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : g->intra_events) (void)hipEventDestroy(e);
   for (auto &s : g->slots)
     if (s.ev) (void)hipEventDestroy(s.ev);
   for (auto &set : g->ev_sets)
     for (auto ev : set) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(g->stream);
   (void)hipStreamDestroy(g->copy_stream);
+  if (g->lf_stream) (void)hipStreamDestroy(g->lf_stream);
   delete g;
 }
 
@@ -786,6 +796,41 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
     p.lf_level_start = g->lf_level_start;
   else
     p.lf_level_start.clear();
+  // The loop filter of superblock (r, c) reads and writes samples in mi rows
+  // 8r - 2 .. 8r + 7 and columns 8c - 2 .. 8c + 7 (the chroma halo is 8
+  // samples at half resolution); an intra block writes its own area and
+  // reads the row above and the column left of it.  Since the loop filter
+  // follows the whole frame's reconstruction (decoder/mod.rs:355-358), a
+  // superblock may be filtered once every intra block meeting that region
+  // is done: lf_need[t] is the highest intra level among them, over the
+  // superblocks of levels <= t.
+  p.lf_need.clear();
+  if (do_lf) {
+    const int R = (int)h.sb64_rows, C = (int)h.sb64_cols;
+    g->sb_need.assign((size_t)R * C, -1);
+    for (size_t k = 0; k < g->intra_items.size(); k++) {
+      const vp9r_block &b = rec->blocks[g->intra_items[k].block];
+      const int bw8 = b.mi_size < BLOCK_8X8 ? 1 : VP9_NUM_8X8_WIDE[b.mi_size];
+      const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : VP9_NUM_8X8_HIGH[b.mi_size];
+      const int r_lo = std::max(0, (int)b.mi_row - 1) / 8, r_hi = std::min(R - 1, ((int)b.mi_row + bh8 + 1) / 8);
+      const int c_lo = std::max(0, (int)b.mi_col - 1) / 8, c_hi = std::min(C - 1, ((int)b.mi_col + bw8 + 1) / 8);
+      for (int r = r_lo; r <= r_hi; r++)
+        for (int c = c_lo; c <= c_hi; c++) {
+          int32_t &v = g->sb_need[(size_t)r * C + c];
+          v = std::max(v, g->item_level[k]);
+        }
+    }
+    const int T = (C - 1) + 2 * (R - 1) + 1;
+    p.lf_need.assign(T, -1);
+    int32_t run = -1;
+    for (int t = 0; t < T; t++) {
+      for (int r = 0; r < R; r++) {
+        const int c = t - 2 * r;
+        if (c >= 0 && c < C) run = std::max(run, g->sb_need[(size_t)r * C + c]);
+      }
+      p.lf_need[t] = run;
+    }
+  }
   if (stage.ensure(p.bytes) != hipSuccess) {
     set_error("hipHostMalloc(staging) failed");
     return VP9R_ERR_DEVICE;
@@ -900,6 +945,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     }
     if (lf && mask_bytes > g->lf_mask.cap) {
       HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+      HIP_TRY(hipStreamSynchronize(g->lf_stream), "hipStreamSynchronize");
       HIP_TRY(g->lf_mask.ensure(mask_bytes), "hipMalloc(lf masks)");
     }
     std::vector<uint32_t> first(n), count(n);
@@ -1018,6 +1064,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     HIP_TRY(slot.dev.ensure(st.size()), "hipMalloc(batch)");
     memcpy(slot.host.p, st.data(), st.size());
     HIP_TRY(hipMemcpyAsync(slot.dev.p, slot.host.p, st.size(), hipMemcpyHostToDevice, s), "upload batch");
+    HIP_TRY(hipEventRecord(g->ev_up, s), "hipEventRecord");
     const uint8_t *d = (const uint8_t *)slot.dev.p;
     auto R = [&](const LaunchTable &t) { return (const vp9g::Range *)(d + t.off); };
     if (recon) {
@@ -1028,21 +1075,70 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       for (int k = 0; k < 4; k++)
         vp9g::launch_itx_inter(s, (const vp9g::ItxArgs *)(d + o_itx), R(t_itx[k]), n, t_itx[k].wgs, k, hbd);
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
+      HIP_TRY(hipEventRecord(g->ev_pre, s), "hipEventRecord");
       uint32_t ib_lds = 0;
       for (int i = 0; i < n; i++)
         ib_lds = std::max(ib_lds, vp9g::ib_layout(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y, hbd ? 8 : 4).bytes);
-      for (const LaunchTable &t : t_intra) {
+      // the intra levels the loop filter waits for (an event after each)
+      std::vector<char> mark(t_intra.size(), 0);
+      if (lf && !t_intra.empty()) {
+        if (groups) {
+          mark.back() = 1;
+        } else {
+          for (size_t l = 0; l < t_lf.size(); l++)
+            for (int i = 0; i < n; i++) {
+              const auto &nd = f[i]->p->lf_need;
+              if (l < nd.size() && nd[l] >= 0) mark[nd[l]] = 1;
+            }
+        }
+      }
+      while (g->intra_events.size() < t_intra.size()) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        g->intra_events.push_back(e);
+      }
+      for (size_t l = 0; l < t_intra.size(); l++) {
+        const LaunchTable &t = t_intra[l];
         if (groups)
           vp9g::launch_intra(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
         else
           vp9g::launch_intra_blocks(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd, ib_lds);
+        if (mark[l]) HIP_TRY(hipEventRecord(g->intra_events[l], s), "hipEventRecord");
       }
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
     }
     if (lf) {
-      for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
-      vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
-      for (const LaunchTable &t : t_lf) vp9g::launch_lf_level(s, (const vp9g::LfArgs *)(d + o_lf), R(t), n, t.wgs, hbd);
+      // The loop filter runs on its own stream: the decisions once the
+      // tables are up, the superblock levels once the inter work is done and
+      // each level's intra levels (lf_need) are, overlapping the others.
+      // The main stream then waits for it (later frames read this one).
+      hipStream_t sl = g->lf_stream;
+      if (!recon) HIP_TRY(hipEventRecord(g->ev_pre, s), "hipEventRecord");
+      for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(sl, f[i]->cur), "hipStreamWaitEvent");
+      HIP_TRY(hipStreamWaitEvent(sl, g->ev_up, 0), "hipStreamWaitEvent");
+      vp9g::launch_lf_mask(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
+      HIP_TRY(hipStreamWaitEvent(sl, g->ev_pre, 0), "hipStreamWaitEvent");
+      int waited = -1;
+      for (size_t l = 0; l < t_lf.size(); l++) {
+        int need = -1;
+        if (recon && !t_intra.empty()) {
+          if (groups) {
+            need = (int)t_intra.size() - 1;
+          } else {
+            for (int i = 0; i < n; i++) {
+              const auto &nd = f[i]->p->lf_need;
+              if (l < nd.size()) need = std::max(need, (int)nd[l]);
+            }
+          }
+        }
+        if (need > waited) {
+          HIP_TRY(hipStreamWaitEvent(sl, g->intra_events[need], 0), "hipStreamWaitEvent");
+          waited = need;
+        }
+        vp9g::launch_lf_level(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_lf[l]), n, t_lf[l].wgs, hbd);
+      }
+      HIP_TRY(hipEventRecord(g->ev_lf_done, sl), "hipEventRecord");
+      HIP_TRY(hipStreamWaitEvent(s, g->ev_lf_done, 0), "hipStreamWaitEvent");
     }
     HIP_TRY(hipEventRecord(slot.ev, s), "hipEventRecord");
     slot.pending = true;

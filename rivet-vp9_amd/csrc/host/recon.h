@@ -42,6 +42,9 @@ struct Prepared {
          o_iidx = 0, o_groups = 0, o_items = 0;
   uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0, n_items = 0;
   std::vector<uint32_t> group_level_start, lf_level_start, item_level_start;
+  // per loop filter level: the last intra level (of leveled intra blocks)
+  // that must be complete before it, -1 for none
+  std::vector<int32_t> lf_need;
   size_t bytes = 0;
   // algorithmic HBM bytes of the frame per stage: inter prediction, inter
   // residual, intra, loop filter
@@ -63,6 +66,7 @@ struct Builder {
   std::vector<uint32_t> group_level_start;
   std::vector<uint32_t> lf_level_start;
   uint32_t lf_rows = 0, lf_cols = 0;
+  std::vector<int32_t> sb_need;
 };
 
 // Checks a record and writes its inputs and work lists to `stage` (pinned
@@ -110,6 +114,11 @@ struct vp9r_gpu {
   int device = 0;
   hipStream_t stream = nullptr;       // kernels and uploads
   hipStream_t copy_stream = nullptr;  // frame downloads
+  // the loop filter's stream (it overlaps the intra levels it does not
+  // need) and its events
+  hipStream_t lf_stream = nullptr;
+  hipEvent_t ev_up = nullptr, ev_pre = nullptr, ev_lf_done = nullptr;
+  std::vector<hipEvent_t> intra_events;
   vp9r::DevBuf arena, lf_mask;
   vp9r::HostBuf stage;
   hipEvent_t ev_upload = nullptr;
