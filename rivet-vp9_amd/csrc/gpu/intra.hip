@@ -391,6 +391,30 @@ DEV void ib_residuals_any(int tx, const IntraArgs &a, uint8_t *lds, const IbGeo 
   }
 }
 
+// The region of one plane of an intra block staged in LDS: the block (BW x
+// BH at (x0, y0)) and the row above it, in vectors of V = min(8, BW)
+// samples (a block is aligned to its size), then the column left of it
+// (x0 - 1), sample by sample.  Item k < nvec is vector (k >> lq) of row
+// (k >> lq) -- rows from y0 - 1 --, the rest the left column.
+struct IbRegion {
+  int x0, y0, BW, BH, V, lq;  // lq: log2 of the vectors per row
+  int nvec, n;                // vector items; all items (0 when the plane is absent)
+};
+
+DEV IbRegion ib_region(const IntraBlockItem &it, const FrameGeom &g, int p, int bw8, int bh8, bool present) {
+  IbRegion r;
+  const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
+  r.x0 = (it.mi_col * 8) >> sx;
+  r.y0 = (it.mi_row * 8) >> sy;
+  r.BW = (bw8 * 8) >> sx;
+  r.BH = (bh8 * 8) >> sy;
+  r.V = r.BW < 8 ? r.BW : 8;
+  r.lq = 31 - __clz(r.BW / r.V);
+  r.nvec = (r.BH + 1) << r.lq;
+  r.n = present ? r.nvec + r.BH + 1 : 0;
+  return r;
+}
+
 // The intra block workgroup's LDS (laid out by ib_layout).
 extern __shared__ __align__(16) uint8_t ib_lds[];
 
@@ -463,27 +487,75 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
   const int bd = g.bit_depth, maxv = (1 << bd) - 1, base = 1 << (bd - 1);
   const uint32_t np0 = it.n_y, np1 = it.n_u, np2 = it.count - it.n_y - it.n_u;
   // every plane's region: the block, the row above it (from above-left) and
-  // the column left of it; samples of the block outside the 8x8-aligned
-  // area are not reconstructed and keep the frame's
+  // the column left of it (samples of the block outside the 8x8-aligned
+  // area are not reconstructed and keep the frame's).  All loads of the
+  // three planes are in flight before the first LDS write: at most
+  // kIbRegionSlots items per lane (4:4:4 64x64: 3 x (65 x 8 + 65) items).
+  constexpr int kIbRegionSlots = 7;
+  const IbRegion rg0 = ib_region(it, g, 0, bw8, bh8, np0 > 0), rg1 = ib_region(it, g, 1, bw8, bh8, np1 > 0),
+                 rg2 = ib_region(it, g, 2, bw8, bh8, np2 > 0);
+  {
+    uint4 v[kIbRegionSlots];
+#pragma unroll
+    for (int i = 0; i < kIbRegionSlots; i++) {
+      const int k = tid + kIbThreads * i;
+      const int p = k < rg0.n ? 0 : k < rg0.n + rg1.n ? 1 : 2;
+      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
+      if (local >= R.n) continue;
+      const P *pl = (const P *)a.cur.p[p];
+      const size_t stride = a.cur.stride[p];
+      if (local < R.nvec) {
+        const int r = local >> R.lq, q = local & ((1 << R.lq) - 1), py = R.y0 - 1 + r;
+        if (py < 0) continue;
+        const P *src = pl + (size_t)py * stride + R.x0 + q * R.V;
+        const int nb = R.V * (int)sizeof(P);
+        if (nb == 16)
+          v[i] = *(const uint4 *)src;
+        else if (nb == 8)
+          v[i] = make_uint4(((const uint2 *)src)->x, ((const uint2 *)src)->y, 0, 0);
+        else
+          v[i] = make_uint4(*(const uint32_t *)src, 0, 0, 0);
+      } else {
+        const int r = local - R.nvec, py = R.y0 - 1 + r;
+        if (py < 0 || R.x0 == 0) continue;
+        v[i] = make_uint4(pl[(size_t)py * stride + R.x0 - 1], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kIbRegionSlots; i++) {
+      const int k = tid + kIbThreads * i;
+      const int p = k < rg0.n ? 0 : k < rg0.n + rg1.n ? 1 : 2;
+      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
+      if (local >= R.n) continue;
+      const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
+      uint16_t *tile = (uint16_t *)(ib_lds + (p == 0 ? ly.tile[0] : p == 1 ? ly.tile[1] : ly.tile[2]));
+      if (local < R.nvec) {
+        const int r = local >> R.lq, q = local & ((1 << R.lq) - 1);
+        if (R.y0 - 1 + r < 0) continue;
+        uint16_t *dst = tile + r * tp + 1 + q * R.V;
+        const uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          if (j >= R.V) break;
+          dst[j] = sizeof(P) == 1 ? (uint16_t)((w[j >> 2] >> (8 * (j & 3))) & 0xff)
+                                  : (uint16_t)((w[j >> 1] >> (16 * (j & 1))) & 0xffff);
+        }
+      } else {
+        const int r = local - R.nvec;
+        if (R.y0 - 1 + r < 0 || R.x0 == 0) continue;
+        tile[r * tp] = (uint16_t)v[i].x;
+      }
+    }
+  }
 #pragma unroll
   for (int p = 0; p < 3; p++) {
-    const uint32_t np = p == 0 ? np0 : p == 1 ? np1 : np2;
-    if (np == 0) continue;
-    const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
-    const int x0 = (it.mi_col * 8) >> sx, y0 = (it.mi_row * 8) >> sy;
-    const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy, lw = 31 - __clz(BW);
-    const int tp = (64 >> sx) + 1, rp = 64 >> sx;
-    uint16_t *tile = (uint16_t *)(ib_lds + ly.tile[p]);
+    const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+    if (!R.n) continue;
+    const int lw = 31 - __clz(R.BW), rp = 64 >> (p ? g.ss_x : 0);
     int16_t *res = (int16_t *)(ib_lds + ly.res[p]);
-    const uint32_t stride = a.cur.stride[p];
-    const P *pl = (const P *)a.cur.p[p];
-    const uint32_t dw = BW + 1, mw = ((1u << 20) + dw - 1) / dw;  // k / (BW + 1) as (k * mw) >> 20
-    for (int k = tid; k < (BH + 1) * (BW + 1); k += kIbThreads) {
-      const int r = (int)(((uint32_t)k * mw) >> 20), c = k - r * (BW + 1);
-      const int py = y0 - 1 + r, px = x0 - 1 + c;
-      if (py >= 0 && px >= 0) tile[r * tp + c] = pl[(size_t)py * stride + px];
-    }
-    for (int k = tid; k < BW * BH; k += kIbThreads) res[(k >> lw) * rp + (k & (BW - 1))] = 0;
+    for (int k = tid; k < R.BW * R.BH; k += kIbThreads) res[(k >> lw) * rp + (k & (R.BW - 1))] = 0;
   }
   for (int k = tid; k < 3 * 256; k += kIbThreads) tmap[k] = 0xffff;
   __syncthreads();
@@ -614,21 +686,35 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
     }
     __syncthreads();
   }
-  // the blocks back to the frame
+  // the blocks back to the frame, in vectors of V samples
 #pragma unroll
   for (int p = 0; p < 3; p++) {
-    const uint32_t np = p == 0 ? np0 : p == 1 ? np1 : np2;
-    if (np == 0) continue;
-    const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
-    const int x0 = (it.mi_col * 8) >> sx, y0 = (it.mi_row * 8) >> sy;
-    const int BW = (bw8 * 8) >> sx, BH = (bh8 * 8) >> sy, lw = 31 - __clz(BW);
-    const int tp = (64 >> sx) + 1;
+    const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+    if (!R.n) continue;
+    const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
     const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
-    const uint32_t stride = a.cur.stride[p];
+    const size_t stride = a.cur.stride[p];
     P *pl = (P *)a.cur.p[p];
-    for (int k = tid; k < BW * BH; k += kIbThreads) {
-      const int r = k >> lw, c = k & (BW - 1);
-      pl[(size_t)(y0 + r) * stride + x0 + c] = (P)tile[(r + 1) * tp + c + 1];
+    for (int k = tid; k < (R.BH << R.lq); k += kIbThreads) {
+      const int r = k >> R.lq, q = k & ((1 << R.lq) - 1);
+      const uint16_t *src = tile + (r + 1) * tp + 1 + q * R.V;
+      P *dst = pl + (size_t)(R.y0 + r) * stride + R.x0 + q * R.V;
+      uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if (j >= R.V) break;
+        if (sizeof(P) == 1)
+          w[j >> 2] |= (uint32_t)src[j] << (8 * (j & 3));
+        else
+          w[j >> 1] |= (uint32_t)src[j] << (16 * (j & 1));
+      }
+      const int nb = R.V * (int)sizeof(P);
+      if (nb == 16)
+        *(uint4 *)dst = make_uint4(w[0], w[1], w[2], w[3]);
+      else if (nb == 8)
+        *(uint2 *)dst = make_uint2(w[0], w[1]);
+      else
+        *(uint32_t *)dst = w[0];
     }
   }
 }
