@@ -8,13 +8,18 @@
 //
 // Order: the reference filters each plane superblock by superblock in raster
 // order, all vertical edges of a superblock (left to right) then all its
-// horizontal edges (top to bottom) (loopfilter.rs:66-80).  Superblock (r, c)
-// touches the samples from 8 left and 8 above of it to its bottom-right
-// corner, so it must follow (r, c - 1) and (r - 1, c + 1): superblocks with
-// equal c + 2r never overlap and form one launch.  Inside a superblock, the
-// vertical pass is sequential along each row and rows are independent, the
-// horizontal pass sequential along each column: one lane per row, then one
-// lane per column, on the superblock and its 8-sample halo staged in LDS.
+// horizontal edges (top to bottom) (loopfilter.rs:66-80).  The vertical pass
+// V(r, c) touches superblock (r, c) and the 8 samples left of it, the
+// horizontal pass H(r, c) superblock (r, c) and the 8 samples above it.  So
+// V(r, c) follows H(r, c - 1) only, and H(r, c) follows V(r, c), H(r - 1, c)
+// and V(r - 1, c + 1).  Grouping each row's sequence into units
+// U(r, c) = H(r, c) V(r, c + 1), U(r, c) follows U(r, c - 1) and U(r - 1, c)
+// only, and units with equal r + c touch disjoint samples: one launch per
+// r + c (R + C - 1 launches, against 2R + C - 2 for whole superblocks).  The
+// first vertical pass of each row, V(r, 0), depends on nothing and runs
+// alone one level early (unit (0, 0) runs V(0, 0) itself).  Inside a pass
+// the edges are sequential along each row (vertical) or column (horizontal)
+// and the lines independent: one lane per line, on the region staged in LDS.
 #include <type_traits>
 
 #include "dev_common.h"
@@ -23,7 +28,7 @@
 
 namespace vp9g {
 
-constexpr int BP = 74;  // LDS pitch of the 72 x 72 region (odd in dwords)
+constexpr int BP = 138;  // LDS pitch of a unit's region (8 + 128 columns + slack, odd in dwords)
 
 // |a - b| of two samples (< 2^16) in one instruction (v_sad_u16).
 DEV uint32_t adiff(int a, int b) { return __builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }
@@ -192,102 +197,114 @@ __global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *
                          (uint32_t)a.p.thresh[lvl] << 24;
 }
 
-// ---- one superblock of one plane
+// ---- one work item of a wavefront level: the region in LDS
+//
+// A unit (r, c) of one plane is the horizontal pass of superblock (r, c)
+// followed by the vertical pass of superblock (r, c + 1): both passes touch
+// the rows of superblock row r (the horizontal pass also the 8 rows above),
+// the first the columns of superblock c, the second those of c + 1 and the 8
+// left of them.  The region staged in LDS is that L shape: the 8 rows above
+// over superblock c's columns, and superblock row r over both superblocks.
+// LDS row 8 + y holds plane row y0 + y; LDS column 8 + x holds plane column
+// x0 + x, so that a frame-edge vertical pass (superblock column 0) has an
+// 8-sample window margin on its left.  The rows past the region are slack
+// for lf_stream's reads past a line's end.
 
 struct __align__(16) LfShared {
-  // the superblock and its 8-sample halo above and left (72 x 72); the rows
-  // beyond are slack for lf_stream's reads past a line's end
   uint16_t buf[80 * BP];
-  uint32_t dq[2][272];  // packed decisions for lf_stream (16 slack entries)
+  uint32_t dq[3][272];  // packed decisions of the item's passes (16 slack entries each)
 };
 
-// ---- one superblock of one plane, streamed through a register window
-//
-// The region (the superblock with its 8-sample halo above and left) is
-// staged in LDS with 8- or 16-byte loads.  The vertical pass gives each lane
-// one row and slides a 16-sample register window (p7 .. q7 of the current
-// edge) along it: after edge e the 4 samples that leave the window are
-// final (no later edge of the row reaches them) and go back to LDS, 4 new
-// ones come in.  The horizontal pass does the same down each column.
+// The region as 8-sample vectors: the rows above (8 << lnt vectors, none in
+// superblock row 0 and for a lone vertical pass), then rows - 8 rows of
+// 1 << lnm vectors.
+struct LfItem {
+  int x0, y0, top, lnt, lnm, rows;
+};
 
-// The region's 8-sample vectors: k = r * NV + q, NV = 9 (64-wide plane
-// superblock) or 5 (32-wide), at most 11 per lane.
+DEV bool lf_vec(const LfItem &it, int k, int &r, int &q) {
+  if (k < it.top) {
+    r = k >> it.lnt;
+    q = k & ((1 << it.lnt) - 1);
+    return true;
+  }
+  k -= it.top;
+  r = 8 + (k >> it.lnm);
+  q = k & ((1 << it.lnm) - 1);
+  return r < it.rows;
+}
+
+// At most 8 x 8 + 64 x 16 vectors (a 64-sample-wide plane superblock row).
 template <class P>
 struct LfRegion {
   typedef typename std::conditional<sizeof(P) == 1, uint2, uint4>::type V;  // 8 samples
-  static constexpr int kMax = 11;
+  static constexpr int kMax = 17;
   V v[kMax];
 };
 
-template <class P>
-DEV bool lf_vec(int k, int RW, int RH, int x0, int y0, int &r, int &q, int &py, int &px) {
-  const int NV = RW / 8;
-  r = NV == 9 ? k / 9 : k / 5;
-  q = k - r * NV;
-  py = y0 - 8 + r;
-  px = x0 - 8 + q * 8;
-  return k < RH * NV && py >= 0 && px >= 0;
-}
-
 // Issues every load of the region (into registers) at once.
 template <class P>
-DEV void lf_region_load(const P *pl, uint32_t stride, int x0, int y0, int RW, int RH, LfRegion<P> &rg) {
+DEV void lf_region_load(const P *pl, uint32_t stride, const LfItem &it, LfRegion<P> &rg) {
   typedef typename LfRegion<P>::V V;
 #pragma unroll
   for (int i = 0; i < LfRegion<P>::kMax; i++) {
-    int r, q, py, px;
-    if (lf_vec<P>(threadIdx.x + 64 * i, RW, RH, x0, y0, r, q, py, px))
-      rg.v[i] = *(const V *)(pl + (size_t)py * stride + px);
+    int r, q;
+    if (lf_vec(it, threadIdx.x + 64 * i, r, q))
+      rg.v[i] = *(const V *)(pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q);
   }
 }
 
+// Samples as 16-bit values, two to a dword store (BP and the column offsets
+// are even).
 template <class P>
-DEV void lf_region_to_lds(int x0, int y0, int RW, int RH, const LfRegion<P> &rg, uint16_t *buf) {
+DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P> &rg, uint16_t *buf) {
 #pragma unroll
   for (int i = 0; i < LfRegion<P>::kMax; i++) {
-    int r, q, py, px;
-    if (!lf_vec<P>(threadIdx.x + 64 * i, RW, RH, x0, y0, r, q, py, px)) continue;
-    uint16_t *l = buf + r * BP + q * 8;
+    int r, q;
+    if (!lf_vec(it, threadIdx.x + 64 * i, r, q)) continue;
+    uint32_t *l = (uint32_t *)(buf + r * BP + 8 + 8 * q);
     if (sizeof(P) == 1) {
       const uint2 w = *(const uint2 *)&rg.v[i];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        l[k] = (w.x >> (8 * k)) & 0xff;
-        l[4 + k] = (w.y >> (8 * k)) & 0xff;
-      }
+      l[0] = (w.x & 0xff) | (w.x & 0xff00) << 8;
+      l[1] = (w.x >> 16 & 0xff) | (w.x >> 8 & 0xff0000);
+      l[2] = (w.y & 0xff) | (w.y & 0xff00) << 8;
+      l[3] = (w.y >> 16 & 0xff) | (w.y >> 8 & 0xff0000);
     } else {
       const uint4 w = *(const uint4 *)&rg.v[i];
-      l[0] = w.x & 0xffff; l[1] = w.x >> 16;
-      l[2] = w.y & 0xffff; l[3] = w.y >> 16;
-      l[4] = w.z & 0xffff; l[5] = w.z >> 16;
-      l[6] = w.w & 0xffff; l[7] = w.w >> 16;
+      l[0] = w.x;
+      l[1] = w.y;
+      l[2] = w.z;
+      l[3] = w.w;
     }
   }
 }
 
 template <class P>
-DEV void lf_region_store(P *pl, uint32_t stride, int x0, int y0, int RW, int RH, const uint16_t *buf) {
+DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_t *buf) {
 #pragma unroll
   for (int i = 0; i < LfRegion<P>::kMax; i++) {
-    int r, q, py, px;
-    if (!lf_vec<P>(threadIdx.x + 64 * i, RW, RH, x0, y0, r, q, py, px)) continue;
-    const uint16_t *l = buf + r * BP + q * 8;
-    P *g = pl + (size_t)py * stride + px;
+    int r, q;
+    if (!lf_vec(it, threadIdx.x + 64 * i, r, q)) continue;
+    const uint32_t *l = (const uint32_t *)(buf + r * BP + 8 + 8 * q);
+    P *g = pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q;
     if (sizeof(P) == 1) {
       uint2 w;
-      w.x = l[0] | l[1] << 8 | l[2] << 16 | (uint32_t)l[3] << 24;
-      w.y = l[4] | l[5] << 8 | l[6] << 16 | (uint32_t)l[7] << 24;
+      w.x = (l[0] & 0xff) | (l[0] >> 8 & 0xff00) | (l[1] & 0xff) << 16 | (l[1] >> 16) << 24;
+      w.y = (l[2] & 0xff) | (l[2] >> 8 & 0xff00) | (l[3] & 0xff) << 16 | (l[3] >> 16) << 24;
       *(uint2 *)g = w;
     } else {
-      uint4 w;
-      w.x = l[0] | (uint32_t)l[1] << 16;
-      w.y = l[2] | (uint32_t)l[3] << 16;
-      w.z = l[4] | (uint32_t)l[5] << 16;
-      w.w = l[6] | (uint32_t)l[7] << 16;
-      *(uint4 *)g = w;
+      *(uint4 *)g = make_uint4(l[0], l[1], l[2], l[3]);
     }
   }
 }
+
+// ---- the edge filters of one pass, streamed through a register window
+//
+// The vertical pass gives each lane one row and slides a 16-sample register
+// window (p7 .. q7 of the current edge) along it: after edge e the 4 samples
+// that leave the window are final (no later edge of the row reaches them)
+// and go back to LDS, 4 new ones come in.  The horizontal pass does the same
+// down each column.
 
 // One pass: lane-owned lines of `len` samples, line i at buf + i * line,
 // samples `step` apart; edges at 8 + 4 e for e < n_edges (a multiple of 4).
@@ -353,37 +370,57 @@ __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, ui
   const LfArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
-  // item j of level t: plane j % 3 of superblock (r, t - 2 r), rows from
-  // the first the level reaches (t - 2 r <= sb64_cols - 1)
-  const int t = (int)rg[fi].first, j = (int)(blockIdx.x - rg[fi].wg), C = (int)g.sb64_cols;
-  const int r_min = t >= C ? (t - C + 2) >> 1 : 0;
-  const int plane = j % 3, sb_row = r_min + j / 3, sb_col = t - 2 * sb_row;
+  // Level t: items j / 3 (plane j % 3) below n_units are the units (r, t - r)
+  // from the first row the level reaches; item n_units, when t + 1 < R, is
+  // the vertical pass of superblock (t + 1, 0), which no later item of row
+  // t + 1 includes (unit (0, 0) does its own).
+  const int t = (int)rg[fi].first, j = (int)(blockIdx.x - rg[fi].wg);
+  const int C = (int)g.sb64_cols, R = (int)g.sb64_rows;
+  const int r_lo = t >= C ? t - C + 1 : 0, n_units = min(R - 1, t) - r_lo + 1;
+  const int plane = j % 3, item = j / 3;
+  const bool lone_v = item >= n_units;
+  const int sb_row = lone_v ? t + 1 : r_lo + item, sb_col = lone_v ? 0 : t - sb_row;
+  const bool first_v = lone_v || (sb_row == 0 && sb_col == 0);
+  const bool right_v = !lone_v && sb_col + 1 < C;
   const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
-  const int SW = 64 >> sx, SH = 64 >> sy, RW = SW + 8, RH = SH + 8;
-  const int x0 = sb_col * SW, y0 = sb_row * SH;
+  const int SW = 64 >> sx, SH = 64 >> sy, lnt = 3 - sx;
+  LfItem it;
+  it.x0 = sb_col * SW;
+  it.y0 = sb_row * SH;
+  it.top = lone_v || sb_row == 0 ? 0 : 8 << lnt;
+  it.lnt = lnt;
+  it.lnm = lnt + (right_v ? 1 : 0);
+  it.rows = 8 + SH;
   const uint32_t stride = a.cur.stride[plane];
   P *pl = (P *)a.cur.p[plane];
-  // every global load at once: both passes' decisions, the region (the
-  // superblock of this plane and its 8-sample halo above and left)
-  const size_t n_sb = (size_t)g.sb64_rows * C, sb = (size_t)sb_row * C + sb_col;
-  const uint4 dq0 = ((const uint4 *)(a.dq + ((plane * 2 + 0) * n_sb + sb) * 256))[lane];
-  const uint4 dq1 = ((const uint4 *)(a.dq + ((plane * 2 + 1) * n_sb + sb) * 256))[lane];
+  // every global load at once: the passes' decisions and the region
+  const size_t n_sb = (size_t)R * C, sb = (size_t)sb_row * C + sb_col;
+  const uint4 *dv = (const uint4 *)(a.dq + (plane * 2 + 0) * n_sb * 256);
+  const uint4 *dh = (const uint4 *)(a.dq + (plane * 2 + 1) * n_sb * 256);
+  uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0, d2 = d0;
+  if (first_v) d0 = dv[sb * 64 + lane];
+  if (!lone_v) d1 = dh[sb * 64 + lane];
+  if (right_v) d2 = dv[(sb + 1) * 64 + lane];
   LfRegion<P> region;
-  lf_region_load<P>(pl, stride, x0, y0, RW, RH, region);
-  ((uint4 *)sh.dq[0])[lane] = dq0;
-  ((uint4 *)sh.dq[1])[lane] = dq1;
-  lf_region_to_lds<P>(x0, y0, RW, RH, region, sh.buf);
+  lf_region_load<P>(pl, stride, it, region);
+  ((uint4 *)sh.dq[0])[lane] = d0;
+  ((uint4 *)sh.dq[1])[lane] = d1;
+  ((uint4 *)sh.dq[2])[lane] = d2;
+  lf_region_to_lds<P>(it, region, sh.buf);
   __syncthreads();
-  // vertical edges (a lane per row), then horizontal edges (a lane per
-  // column): one copy of the edge loop for both passes and every geometry
+  // pass 0: the vertical edges of superblock (r, c) (c = 0 only); pass 1 its
+  // horizontal edges; pass 2 the vertical edges of (r, c + 1).  One copy of
+  // the edge loop for every pass and geometry.
 #pragma unroll 1
-  for (int pass = 0; pass < 2; pass++) {
-    if (lane < (pass ? SW : SH))
-      lf_stream(sh.buf, pass ? 8 + lane : (8 + lane) * BP, pass ? BP : 1, pass ? RH : RW, (pass ? SH : SW) / 4,
-                sh.dq[pass], lane >> 2, g.bit_depth);
+  for (int pass = 0; pass < 3; pass++) {
+    if (pass == 0 ? !first_v : pass == 1 ? lone_v : !right_v) continue;
+    const bool vert = pass != 1;
+    if (lane < (vert ? SH : SW))
+      lf_stream(sh.buf, vert ? (8 + lane) * BP + (pass == 2 ? SW : 0) : 8 + lane, vert ? 1 : BP,
+                (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, g.bit_depth);
     __syncthreads();
   }
-  lf_region_store<P>(pl, stride, x0, y0, RW, RH, sh.buf);
+  lf_region_store<P>(pl, stride, it, sh.buf);
 }
 
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs) {

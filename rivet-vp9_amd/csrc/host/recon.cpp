@@ -8,7 +8,7 @@
 //   3. inter residuals, one launch per transform size;
 //   4. intra prediction + residual, one launch per dependency level;
 //   5. the loop filter decisions (one launch) and the filter wavefront, one
-//      launch per superblock level c + 2r.
+//      launch per level r + c of (horizontal pass, next vertical pass) units.
 // Every launch covers all frames of the batch: a workgroup finds its frame
 // in the launch's range table.
 // This is the order of the reference's replay (src/decoder/block.rs:1405-1498)
@@ -295,21 +295,21 @@ static void build_intra(Builder *g, const vp9r_record *rec) {
   }
 }
 
-// Loop filter wavefront levels: superblocks with equal c + 2r, all planes;
-// lf_level_start[t] = 3 x the superblocks of the levels before t.
+// The loop filter's wavefront levels (lf.hip): level t holds the units
+// (r, t - r) and, while t + 1 < rows, the lone vertical pass of superblock
+// (t + 1, 0); lf_level_start[t] = 3 x the items of the levels before t (one
+// workgroup per item and plane).
 static void build_lf_levels(Builder *g, uint32_t rows, uint32_t cols) {
   if (g->lf_rows == rows && g->lf_cols == cols) return;
   g->lf_rows = rows;
   g->lf_cols = cols;
   g->lf_level_start.clear();
-  const uint32_t T = (cols - 1) + 2 * (rows - 1) + 1;
+  const int R = (int)rows, C = (int)cols, T = R + C - 1;
   uint32_t n = 0;
-  for (uint32_t t = 0; t < T; t++) {
+  for (int t = 0; t < T; t++) {
     g->lf_level_start.push_back(n);
-    for (uint32_t r = 0; r < rows; r++) {
-      const int c = (int)t - 2 * (int)r;
-      if (c >= 0 && c < (int)cols) n += 3;
-    }
+    const int units = std::min(R - 1, t) - std::max(0, t - C + 1) + 1;
+    n += 3 * (uint32_t)(units + (t + 1 < R ? 1 : 0));
   }
   g->lf_level_start.push_back(n);
 }
@@ -796,38 +796,46 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
     p.lf_level_start = g->lf_level_start;
   else
     p.lf_level_start.clear();
-  // The loop filter of superblock (r, c) reads and writes samples in mi rows
-  // 8r - 2 .. 8r + 7 and columns 8c - 2 .. 8c + 7 (the chroma halo is 8
-  // samples at half resolution); an intra block writes its own area and
-  // reads the row above and the column left of it.  Since the loop filter
-  // follows the whole frame's reconstruction (decoder/mod.rs:355-358), a
-  // superblock may be filtered once every intra block meeting that region
-  // is done: lf_need[t] is the highest intra level among them, over the
-  // superblocks of levels <= t.
+  // The loop filter's passes on superblock (r, c) read and write samples in
+  // mi rows 8r - 2 .. 8r + 7 and columns 8c - 2 .. 8c + 7 (the chroma halo
+  // is 8 samples at half resolution).  An intra block writes its own area
+  // and reads the row above it (with the above-right samples, up to twice
+  // its width) and the column left of it.  Since the loop filter follows
+  // the whole frame's reconstruction (decoder/mod.rs:355-358), a level may
+  // run once every intra block meeting one of its superblocks is done:
+  // lf_need[t] is the highest intra level among them, over the levels <= t.
   p.lf_need.clear();
   if (do_lf) {
     const int R = (int)h.sb64_rows, C = (int)h.sb64_cols;
     g->sb_need.assign((size_t)R * C, -1);
+    auto meet = [&](int row0, int row1, int col0, int col1, int32_t level) {
+      const int r_lo = std::max(0, row0) / 8, r_hi = std::min(R - 1, (row1 + 2) / 8);
+      const int c_lo = std::max(0, col0) / 8, c_hi = std::min(C - 1, (col1 + 2) / 8);
+      for (int r = r_lo; r <= r_hi; r++)
+        for (int c = c_lo; c <= c_hi; c++) {
+          int32_t &v = g->sb_need[(size_t)r * C + c];
+          v = std::max(v, level);
+        }
+    };
     for (size_t k = 0; k < g->intra_items.size(); k++) {
       const vp9r_block &b = rec->blocks[g->intra_items[k].block];
       const int bw8 = b.mi_size < BLOCK_8X8 ? 1 : VP9_NUM_8X8_WIDE[b.mi_size];
       const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : VP9_NUM_8X8_HIGH[b.mi_size];
-      const int r_lo = std::max(0, (int)b.mi_row - 1) / 8, r_hi = std::min(R - 1, ((int)b.mi_row + bh8 + 1) / 8);
-      const int c_lo = std::max(0, (int)b.mi_col - 1) / 8, c_hi = std::min(C - 1, ((int)b.mi_col + bw8 + 1) / 8);
-      for (int r = r_lo; r <= r_hi; r++)
-        for (int c = c_lo; c <= c_hi; c++) {
-          int32_t &v = g->sb_need[(size_t)r * C + c];
-          v = std::max(v, g->item_level[k]);
-        }
+      const int row = (int)b.mi_row, col = (int)b.mi_col;
+      meet(row - 1, row + bh8 - 1, col - 1, col + bw8 - 1, g->item_level[k]);
+      if (row > 0) meet(row - 1, row - 1, col + bw8, col + 2 * bw8 - 1, g->item_level[k]);
     }
-    const int T = (C - 1) + 2 * (R - 1) + 1;
+    auto need = [&](int r, int c) { return g->sb_need[(size_t)r * C + c]; };
+    const int T = R + C - 1;
     p.lf_need.assign(T, -1);
     int32_t run = -1;
     for (int t = 0; t < T; t++) {
-      for (int r = 0; r < R; r++) {
-        const int c = t - 2 * r;
-        if (c >= 0 && c < C) run = std::max(run, g->sb_need[(size_t)r * C + c]);
+      for (int r = std::max(0, t - C + 1); r <= std::min(R - 1, t); r++) {
+        const int c = t - r;
+        run = std::max(run, need(r, c));
+        if (c + 1 < C) run = std::max(run, need(r, c + 1));
       }
+      if (t + 1 < R) run = std::max(run, need(t + 1, 0));
       p.lf_need[t] = run;
     }
   }
