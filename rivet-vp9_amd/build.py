@@ -42,9 +42,19 @@ GPU = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packe
        "-munsafe-fp-atomics"]
 
 
+# Per-file flags.  lf.hip: the loop filter keeps a 16-sample window of each
+# line in a register array that the filters update in place under per-lane
+# branches; promoted to one vector register, every partial update costs a
+# copy of the whole window (16 moves per edge), so the promotion is off.
+FILE_FLAGS = {
+    "csrc/gpu/lf.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
+}
+
+
 def compile_one(src, is_hip):
     obj = os.path.join(BUILD, src.replace("/", "_") + ".o")
-    cmd = [HIPCC] + COMMON + (GPU + ["-x", "hip"] if is_hip else []) + ["-c", os.path.join(HERE, src), "-o", obj]
+    cmd = [HIPCC] + COMMON + (GPU + ["-x", "hip"] if is_hip else []) + FILE_FLAGS.get(src, []) + \
+        ["-c", os.path.join(HERE, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0 or "error" in r.stderr:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -41,28 +41,33 @@ DEV int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // lo <= 
 // The wide filters (8.8.5.3): output i in [-n, n) is
 // Round2(s[i] + T[i], LOG2) with the window sum
 // T[i] = sum_{j=-n..n} s[clamp(i + j, -n - 1, n)] and n = 2^(LOG2-1) - 1.
-// T slides: T[i + 1] = T[i] - s[clamp(i - n)] + s[clamp(i + n + 1)].
+// T slides: T[i + 1] = T[i] - s[clamp(i - n)] + s[clamp(i + n + 1)].  Every
+// T is formed from the inputs before any output is written, so each output
+// replaces its input in place.
 template <int LOG2, class S>
 DEV void lf_wide(S s) {
   constexpr int n = (1 << (LOG2 - 1)) - 1;
-  int v[2 * n + 2];
+  auto at = [&](int k) -> int { return s(k < -(n + 1) ? -(n + 1) : (k > n ? n : k)); };
+  int t[2 * n];
+  int tt = 0;
 #pragma unroll
-  for (int k = -(n + 1); k <= n; k++) v[k + n + 1] = s(k);
-  auto at = [&](int k) { return v[(k < -(n + 1) ? -(n + 1) : (k > n ? n : k)) + n + 1]; };
-  int t = 0;
-#pragma unroll
-  for (int j = -n; j <= n; j++) t += at(-n + j);
+  for (int j = -n; j <= n; j++) tt += at(-n + j);
 #pragma unroll
   for (int i = -n; i < n; i++) {
-    s(i) = (v[i + n + 1] + t + (1 << (LOG2 - 1))) >> LOG2;
-    t += at(i + n + 1) - at(i - n);
+    t[i + n] = tt;
+    tt += at(i + n + 1) - at(i - n);
   }
+#pragma unroll
+  for (int i = -n; i < n; i++) s(i) = (s(i) + t[i + n] + (1 << (LOG2 - 1))) >> LOG2;
 }
 
 // lf::filter (src/dsp/lf.rs:24-86): the mask, then the narrow filter, or
 // the 8- or 16-wide filter where the samples are flat.  The mask and
 // flatness tests are maxima of absolute differences against one limit
-// each (the reference's chains of comparisons, without branches).
+// each (the reference's chains of comparisons, without branches).  The
+// three filters are exclusive per lane and each rewrites its samples in
+// place: three independent conditional blocks, so that no sample is copied
+// to merge the paths.
 template <class S>
 DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
   const int q0 = s(0), q1 = s(1), q2 = s(2), q3 = s(3);
@@ -71,12 +76,19 @@ DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
   const uint32_t d10 = adiff(p1, p0), e10 = adiff(q1, q0);
   const uint32_t m = max(max(max(adiff(p3, p2), adiff(p2, p1)), d10), max(max(e10, adiff(q2, q1)), adiff(q3, q2)));
   const uint32_t edge = adiff(p0, q0) * 2 + (adiff(p1, q1) >> 1);
-  if (m > (uint32_t)(limit << sh) || edge > (uint32_t)(blimit << sh)) return;
+  const bool on = fs >= 0 && m <= (uint32_t)(limit << sh) && edge <= (uint32_t)(blimit << sh);
   const bool hev = max(d10, e10) > (uint32_t)(thresh << sh);
   const uint32_t one = 1u << sh;
-  const bool flat =
-      fs >= 1 && max(max(max(d10, e10), max(adiff(p2, p0), adiff(q2, q0))), max(adiff(p3, p0), adiff(q3, q0))) <= one;
-  if (!flat) {
+  const bool flat = on && fs >= 1 &&
+                    max(max(max(d10, e10), max(adiff(p2, p0), adiff(q2, q0))), max(adiff(p3, p0), adiff(q3, q0))) <= one;
+  bool flat2 = false;
+  if (flat && fs >= 2) {
+    uint32_t f2 = 0;
+#pragma unroll
+    for (int k = 4; k < 8; k++) f2 = max(f2, max(adiff(s(-(k + 1)), p0), adiff(s(k), q0)));
+    flat2 = f2 <= one;
+  }
+  if (on && !flat) {
     // narrow filter (8.8.5.2)
     const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1, off = 0x80 << sh;
     const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
@@ -90,18 +102,10 @@ DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
       s(1) = clampi(qs1 - g, lo, hi) + off;
       s(-2) = clampi(ps1 + g, lo, hi) + off;
     }
-    return;
-  }
-  uint32_t f2 = 0;
-  if (fs >= 2) {
-#pragma unroll
-    for (int k = 4; k < 8; k++) f2 = max(f2, max(adiff(s(-(k + 1)), p0), adiff(s(k), q0)));
   }
   // wide filter (8.8.5.3): log2 3 (8 taps) or 4 (16 taps)
-  if (fs >= 2 && f2 <= one)
-    lf_wide<4>(s);
-  else
-    lf_wide<3>(s);
+  if (flat && !flat2) lf_wide<3>(s);
+  if (flat2) lf_wide<4>(s);
 }
 
 // lf::filter on an LDS buffer: across the edge whose first q sample is
@@ -417,7 +421,7 @@ __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, ui
     const bool vert = pass != 1;
     if (lane < (vert ? SH : SW))
       lf_stream(sh.buf, vert ? (8 + lane) * BP + (pass == 2 ? SW : 0) : 8 + lane, vert ? 1 : BP,
-                (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, g.bit_depth);
+                (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, HBD ? (int)g.bit_depth : 8);
     __syncthreads();
   }
   lf_region_store<P>(pl, stride, it, sh.buf);

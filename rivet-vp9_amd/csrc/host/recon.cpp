@@ -452,9 +452,11 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->lf_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->up_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     if (g->stream) (void)hipStreamDestroy(g->stream);
     if (g->copy_stream) (void)hipStreamDestroy(g->copy_stream);
+    if (g->lf_stream) (void)hipStreamDestroy(g->lf_stream);
     delete g;
     return hip_fail(e, "hipStreamCreate");
   }
@@ -478,6 +480,7 @@ void vp9r_gpu_free(vp9r_gpu *g) {
   (void)hipStreamSynchronize(g->stream);
   (void)hipStreamSynchronize(g->copy_stream);
   if (g->lf_stream) (void)hipStreamSynchronize(g->lf_stream);
+  if (g->up_stream) (void)hipStreamSynchronize(g->up_stream);
   if (g->ev_upload) (void)hipEventDestroy(g->ev_upload);
   for (hipEvent_t e : {g->ev_up, g->ev_pre, g->ev_lf_done})
     if (e) (void)hipEventDestroy(e);
@@ -489,6 +492,7 @@ void vp9r_gpu_free(vp9r_gpu *g) {
   (void)hipStreamDestroy(g->stream);
   (void)hipStreamDestroy(g->copy_stream);
   if (g->lf_stream) (void)hipStreamDestroy(g->lf_stream);
+  if (g->up_stream) (void)hipStreamDestroy(g->up_stream);
   delete g;
 }
 
@@ -1071,8 +1075,11 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     HIP_TRY(slot.host.ensure(st.size()), "hipHostMalloc(batch)");
     HIP_TRY(slot.dev.ensure(st.size()), "hipMalloc(batch)");
     memcpy(slot.host.p, st.data(), st.size());
-    HIP_TRY(hipMemcpyAsync(slot.dev.p, slot.host.p, st.size(), hipMemcpyHostToDevice, s), "upload batch");
-    HIP_TRY(hipEventRecord(g->ev_up, s), "hipEventRecord");
+    // on its own stream: the copy runs while earlier batches still filter
+    HIP_TRY(hipMemcpyAsync(slot.dev.p, slot.host.p, st.size(), hipMemcpyHostToDevice, g->up_stream),
+            "upload batch");
+    HIP_TRY(hipEventRecord(g->ev_up, g->up_stream), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(s, g->ev_up, 0), "hipStreamWaitEvent");
     const uint8_t *d = (const uint8_t *)slot.dev.p;
     auto R = [&](const LaunchTable &t) { return (const vp9g::Range *)(d + t.off); };
     if (recon) {
@@ -1116,16 +1123,17 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
     }
     if (lf) {
-      // The loop filter runs on its own stream: the decisions once the
-      // tables are up, the superblock levels once the inter work is done and
-      // each level's intra levels (lf_need) are, overlapping the others.
+      // The loop filter runs on its own stream: the decisions and levels
+      // once the inter work is done, each level once its intra levels
+      // (lf_need) are, overlapping the others.
       // The main stream then waits for it (later frames read this one).
       hipStream_t sl = g->lf_stream;
       if (!recon) HIP_TRY(hipEventRecord(g->ev_pre, s), "hipEventRecord");
       for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(sl, f[i]->cur), "hipStreamWaitEvent");
-      HIP_TRY(hipStreamWaitEvent(sl, g->ev_up, 0), "hipStreamWaitEvent");
-      vp9g::launch_lf_mask(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
+      // ev_pre follows, on the kernel stream, the batch tables, the frames'
+      // records (uploaded on that stream) and the inter stages
       HIP_TRY(hipStreamWaitEvent(sl, g->ev_pre, 0), "hipStreamWaitEvent");
+      vp9g::launch_lf_mask(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
       int waited = -1;
       for (size_t l = 0; l < t_lf.size(); l++) {
         int need = -1;

@@ -112,8 +112,9 @@ struct vp9r_gpu_frame {
 
 struct vp9r_gpu {
   int device = 0;
-  hipStream_t stream = nullptr;       // kernels and uploads
+  hipStream_t stream = nullptr;       // kernels and record uploads
   hipStream_t copy_stream = nullptr;  // frame downloads
+  hipStream_t up_stream = nullptr;    // batch table uploads (ahead of the kernels)
   // the loop filter's stream (it overlaps the intra levels it does not
   // need) and its events
   hipStream_t lf_stream = nullptr;
