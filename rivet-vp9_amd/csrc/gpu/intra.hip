@@ -289,14 +289,13 @@ __global__ __launch_bounds__(64) void k_intra(const IntraArgs *fa, const Range *
 //
 // The block's three planes are independent (each reads only its own
 // plane), so they are processed together.  Each plane's region (the block
-// with the row above and the column left of it) is staged in LDS.  The
-// residuals of all transform blocks are computed first, in parallel (teams
-// of n0 lanes, as in the inter residual kernel; luma, then both chroma
-// planes), into LDS residual planes: the reference adds the clamped
+// with the row above and the column left of it) is staged in LDS, with the
+// residuals of its transform blocks: the reference adds the clamped
 // residual to the prediction with a clip to the sample range
-// (itx.rs:606-614), so the residual does not depend on the prediction.
-// Residuals are kept as int16: beyond +-32767 the clip gives 0 or the
-// maximum whatever the prediction, as it does for the full value.  Then
+// (itx.rs:606-614), so the residual does not depend on the prediction and
+// the inter residual launches compute it beforehand (k_itx_inter, jobs
+// flagged kItxIntra, int16 planes: beyond +-32767 the clip gives 0 or the
+// maximum whatever the prediction, as it does for the full value).  Then
 // prediction and the add run as a wavefront over each plane's grid of
 // transform blocks: block (r, c) reads the reconstruction left, above and
 // above-left of it, and at 4x4 also above-right (recon.rs:61-69), so it is
@@ -304,12 +303,6 @@ __global__ __launch_bounds__(64) void k_intra(const IntraArgs *fa, const Range *
 // planes, are independent.  Edges are read from the LDS tiles as
 // recon.rs:17-103 builds them.  The LDS layout depends on the frame's
 // subsampling (ib_layout); the launch passes the largest of its frames'.
-
-struct ResDst {
-  int16_t *base;
-  int pitch;
-  DEV void add(int i, int j, int r, int) { base[i * pitch + j] = (int16_t)min(max(r, -32768), 32767); }
-};
 
 // Geometry of one plane of the block (in LDS).
 struct IbGeo {
@@ -321,75 +314,6 @@ struct IbGeo {
   uint32_t tile_off, res_off; // LDS byte offsets
 };
 static_assert(sizeof(IbGeo) <= kIbGeoBytes, "IbGeo");
-
-// The residuals of np transform blocks txbs[k0 ..) (size 4 << (n - 2)),
-// `teams` blocks a pass.
-template <class L, int n, class TX>
-DEV void ib_residuals(const IntraArgs &a, uint8_t *lds, const IbGeo *geo, typename L::T *ctile, TX txb_at,
-                      uint32_t k0, uint32_t np, int seg, bool hbd, const uint16_t *cpos, const int32_t *cval,
-                      uint32_t c0, uint32_t n_c) {
-  typedef typename L::T T;
-  constexpr int n0 = 1 << n, S = n0 + 1;
-  constexpr int teams = n == 2 ? 64 : n == 3 ? 32 : n == 4 ? 16 : 4;  // coefficient tiles: 1280 .. 4352 entries
-  static_assert(teams * n0 * S <= kIbCoefTile, "coefficient tiles");
-  const int tid = threadIdx.x, team = tid / n0, ln = tid % n0;
-  const bool in_team = team < teams;
-  T *tile = ctile + (in_team ? team : 0) * (n0 * S);
-  const bool lossless = a.g.lossless;
-  for (uint32_t base = 0; base < np; base += teams) {
-    const uint32_t j = base + team;
-    vp9r_txb t = {};
-    bool active = false;
-    if (in_team && j < np) {
-      t = txb_at(k0 + j);
-      active = t.eob > 0;
-    }
-    if (in_team)
-      for (int k = ln; k < n0 * S; k += n0) tile[k] = 0;
-    __syncthreads();
-    if (active) {
-      const int pt = t.plane > 0;
-      const int32_t dcq = a.q[seg][pt][0], acq = a.q[seg][pt][1];
-      if (t.coef_start >= c0 && t.coef_start - c0 + t.ncoef <= n_c) {
-        const uint32_t o = t.coef_start - c0;
-        for (int k = ln; k < t.ncoef; k += n0) {
-          const int p = cpos[o + k] & (n0 * n0 - 1);
-          tile[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(cval[o + k], p, dcq, acq, n == 5);
-        }
-      } else {
-        const uint16_t *pos = a.coef_pos + t.coef_start;
-        const int32_t *val = a.coef_val + t.coef_start;
-        for (int k = ln; k < t.ncoef; k += n0) {
-          const int p = pos[k] & (n0 * n0 - 1);
-          tile[(p >> n) * S + (p & (n0 - 1))] = (T)dequant(val[k], p, dcq, acq, n == 5);
-        }
-      }
-    }
-    __syncthreads();
-    const bool dc_only = !hbd && active && t.eob == 1 && t.tx_type == DCT_DCT && !lossless;
-    const int32_t dc_v = dc_only ? dc_only_value((int32_t)tile[0], n) : 0;
-    ResDst dst{(int16_t *)lds, 0};
-    if (active) {
-      const IbGeo &G = geo[t.plane];
-      dst.base = (int16_t *)(lds + G.res_off) + (t.y - G.y0) * G.rpitch + (t.x - G.x0);
-      dst.pitch = G.rpitch;
-    }
-    itx_2d<L, n>(tile, active ? ln : n0, t.tx_type, lossless, dc_only, dc_v, dst, (1 << a.g.bit_depth) - 1);
-    __syncthreads();
-  }
-}
-
-template <class L, class TX>
-DEV void ib_residuals_any(int tx, const IntraArgs &a, uint8_t *lds, const IbGeo *geo, typename L::T *ctile,
-                          TX txb_at, uint32_t k0, uint32_t np, int seg, bool hbd, const uint16_t *cpos,
-                          const int32_t *cval, uint32_t c0, uint32_t n_c) {
-  switch (tx) {  // ends with a barrier
-    case 0: ib_residuals<L, 2>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
-    case 1: ib_residuals<L, 3>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
-    case 2: ib_residuals<L, 4>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
-    default: ib_residuals<L, 5>(a, lds, geo, ctile, txb_at, k0, np, seg, hbd, cpos, cval, c0, n_c); break;
-  }
-}
 
 // The region of one plane of an intra block staged in LDS: the block (BW x
 // BH at (x0, y0)) and the row above it, in vectors of V = min(8, BW)
@@ -454,33 +378,22 @@ struct Edges {
 
 template <bool HBD>
 __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa, const Range *rg, uint32_t nf) {
-  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
-  typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   const int tid = threadIdx.x;
   const uint32_t fi = range_frame(rg, nf, blockIdx.x);
   const IntraArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   const IntraBlockItem it = a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
-  const IbLayout ly = ib_layout(g.ss_x, g.ss_y, (int)sizeof(T));
+  const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
   vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
   uint16_t *tmap = (uint16_t *)(ib_lds + ly.tmap);
   int32_t *dcv = (int32_t *)(ib_lds + ly.dcv);
   IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
-  T *ctile = (T *)(ib_lds + ly.ctile);
   // the transform block records, all in LDS (at most ib_max_txbs; the host
   // checks each plane's grid, which bounds the count)
   for (uint32_t k = tid; k < it.count; k += kIbThreads) stx[k] = a.txbs[it.first + k];
-  // and the coefficients they cover (the residual passes read them at once)
-  uint16_t *cpos = (uint16_t *)(ib_lds + ly.cpos);
-  int32_t *cval = (int32_t *)(ib_lds + ly.cval);
-  const uint32_t c0 = it.coef_first, n_c = min(it.coef_count, (uint32_t)kIbStageCoefs);
-  for (uint32_t k = tid; k < n_c; k += kIbThreads) {
-    cpos[k] = a.coef_pos[c0 + k];
-    cval[k] = a.coef_val[c0 + k];
-  }
   auto txb_at = [&](uint32_t k) -> vp9r_txb { return stx[k]; };
-  const int mi_size = it.mi_size, seg = it.flags & 7;
+  const int mi_size = it.mi_size;
   const bool avail_u = (it.flags & 8) != 0, avail_l = (it.flags & 16) != 0;
   const int bsize = mi_size < BLOCK_8X8 ? BLOCK_8X8 : mi_size;
   const int bw8 = num8x8_wide(bsize), bh8 = num8x8_high(bsize);
@@ -494,8 +407,10 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
   constexpr int kIbRegionSlots = 7;
   const IbRegion rg0 = ib_region(it, g, 0, bw8, bh8, np0 > 0), rg1 = ib_region(it, g, 1, bw8, bh8, np1 > 0),
                  rg2 = ib_region(it, g, 2, bw8, bh8, np2 > 0);
+  constexpr int kIbResSlots = 6;
+  const int nr[3] = {rg0.n ? rg0.BH << rg0.lq : 0, rg1.n ? rg1.BH << rg1.lq : 0, rg2.n ? rg2.BH << rg2.lq : 0};
   {
-    uint4 v[kIbRegionSlots];
+    uint4 v[kIbRegionSlots], rv[kIbResSlots];
 #pragma unroll
     for (int i = 0; i < kIbRegionSlots; i++) {
       const int k = tid + kIbThreads * i;
@@ -521,6 +436,37 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
         if (py < 0 || R.x0 == 0) continue;
         v[i] = make_uint4(pl[(size_t)py * stride + R.x0 - 1], 0, 0, 0);
       }
+    }
+    // the residuals of the block (k_itx_inter wrote every transform block's;
+    // samples outside the transform blocks are never read), in vectors of
+    // V samples: at most kIbResSlots per lane (4:4:4 64x64: 3 x 512)
+#pragma unroll
+    for (int i = 0; i < kIbResSlots; i++) {
+      const int k = tid + kIbThreads * i;
+      const int p = k < nr[0] ? 0 : k < nr[0] + nr[1] ? 1 : 2;
+      const int local = k - (p == 0 ? 0 : p == 1 ? nr[0] : nr[0] + nr[1]);
+      if (local >= nr[p]) continue;
+      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const int r = local >> R.lq, q = local & ((1 << R.lq) - 1);
+      const int16_t *src = a.res[p] + (size_t)(R.y0 + r) * a.cur.stride[p] + R.x0 + q * R.V;
+      if (R.V == 8)
+        rv[i] = *(const uint4 *)src;
+      else
+        rv[i] = make_uint4(((const uint2 *)src)->x, ((const uint2 *)src)->y, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < kIbResSlots; i++) {
+      const int k = tid + kIbThreads * i;
+      const int p = k < nr[0] ? 0 : k < nr[0] + nr[1] ? 1 : 2;
+      const int local = k - (p == 0 ? 0 : p == 1 ? nr[0] : nr[0] + nr[1]);
+      if (local >= nr[p]) continue;
+      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const int r = local >> R.lq, q = local & ((1 << R.lq) - 1), rp = 64 >> (p ? g.ss_x : 0);
+      uint8_t *dst = ib_lds + (p == 0 ? ly.res[0] : p == 1 ? ly.res[1] : ly.res[2]) + (r * rp + q * R.V) * 2;
+      if (R.V == 8)
+        *(uint4 *)dst = rv[i];
+      else
+        *(uint2 *)dst = make_uint2(rv[i].x, rv[i].y);
     }
 #pragma unroll
     for (int i = 0; i < kIbRegionSlots; i++) {
@@ -548,14 +494,6 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
         tile[r * tp] = (uint16_t)v[i].x;
       }
     }
-  }
-#pragma unroll
-  for (int p = 0; p < 3; p++) {
-    const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
-    if (!R.n) continue;
-    const int lw = 31 - __clz(R.BW), rp = 64 >> (p ? g.ss_x : 0);
-    int16_t *res = (int16_t *)(ib_lds + ly.res[p]);
-    for (int k = tid; k < R.BW * R.BH; k += kIbThreads) res[(k >> lw) * rp + (k & (R.BW - 1))] = 0;
   }
   for (int k = tid; k < 3 * 256; k += kIbThreads) tmap[k] = 0xffff;
   __syncthreads();
@@ -590,14 +528,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
     const IbGeo &G = geo[t.plane];
     tmap[t.plane * 256 + ((t.y - G.y0) >> G.log2) * 16 + ((t.x - G.x0) >> G.log2)] = (uint16_t)j;
   }
-  // residuals: luma, then both chroma planes (one transform size); each
-  // ends with a barrier, after which the grids are visible too
-  if (np0)
-    ib_residuals_any<L>(geo[0].log2 - 2, a, ib_lds, geo, ctile, txb_at, 0, np0, seg, HBD, cpos, cval, c0, n_c);
-  if (np1 + np2)
-    ib_residuals_any<L>(txb_at(np0).tx_size, a, ib_lds, geo, ctile, txb_at, np0, np1 + np2, seg, HBD, cpos, cval,
-                        c0, n_c);
-  if (!np0 && !(np1 + np2)) __syncthreads();
+  __syncthreads();
   // the wavefront over the three grids
   const int nlv = max(max(geo[0].nlv, geo[1].nlv), geo[2].nlv);
   bool any_dc = (np0 && geo[0].mode_u == DC_PRED) || ((np1 + np2) && it.uv_mode == DC_PRED);

@@ -31,8 +31,11 @@ __global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range
   __syncthreads();
   vp9r_txb t = {};
   int32_t dcq = 0, acq = 0;
+  bool intra = false;
   if (active) {
-    t = a.txbs[a.jobs[n - 2][job]];
+    const uint32_t jv = a.jobs[n - 2][job];
+    intra = (jv & kItxIntra) != 0;
+    t = a.txbs[jv & ~kItxIntra];
     const int seg = a.blocks[t.block].segment_id;
     const int pt = t.plane > 0;
     dcq = a.q[seg][pt][0];
@@ -46,12 +49,14 @@ __global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range
   }
   __syncthreads();
   const bool lossless = a.g.lossless;
-  const bool dc_only = !HBD && active && t.eob == 1 && t.tx_type == DCT_DCT && !lossless;
+  // an all-zero block's residual is zero (only intra blocks' are listed)
+  const bool dc_only = active && (t.eob == 0 || (!HBD && t.eob == 1 && t.tx_type == DCT_DCT && !lossless));
   int32_t dc_v = 0;
-  if (dc_only) dc_v = dc_only_value((int32_t)tile[0], n + 0);
-  GlobalDst<P> dst;
+  if (dc_only && t.eob) dc_v = dc_only_value((int32_t)tile[0], n + 0);
+  ItxDst<P> dst;
   dst.stride = a.cur.stride[t.plane];
   dst.base = (P *)a.cur.p[t.plane] + (size_t)t.y * dst.stride + t.x;
+  dst.res = intra ? a.res[t.plane] + (size_t)t.y * dst.stride + t.x : nullptr;
   const int max = (1 << a.g.bit_depth) - 1;
   itx_2d<L, n>(tile, active ? lane : n0, t.tx_type, lossless, dc_only, dc_v, dst, max);
 }

@@ -41,6 +41,25 @@ struct GlobalDst {
   }
 };
 
+// The inter residual kernel's destination: added into the frame, or, for a
+// transform block of an intra block (res set), stored as int16.  Beyond
+// +-32767 the clip to the sample range after the prediction is added gives 0
+// or the maximum whatever the prediction, as it does for the full value.
+template <class P>
+struct ItxDst {
+  P *base;
+  int16_t *res;
+  uint32_t stride;
+  DEV void add(int i, int j, int r, int max) {
+    if (res) {
+      res[(size_t)i * stride + j] = (int16_t)iclamp(r, -32768, 32767);
+    } else {
+      P *p = base + (size_t)i * stride + j;
+      *p = (P)iclamp((int)*p + r, 0, max);
+    }
+  }
+};
+
 // Rows, then columns, then Round2 by min(n + 2, 6) (none when lossless),
 // residual clamp +-2^16, add.  `tile` holds the dequantised n0 x n0 block
 // (row stride n0 + 1).  Every lane of the workgroup calls this (lanes with

@@ -80,7 +80,12 @@ struct ItxArgs {
   const int32_t *coef_val;
   Planes cur;
   int32_t q[8][2][2];  // [segment][plane > 0][dc, ac]
+  // Jobs flagged kItxIntra are transform blocks of intra blocks: their
+  // residuals go to res (int16 planes with cur's strides), not into cur,
+  // since the prediction they add to comes later (k_intra_blocks).
+  int16_t *res[3];
 };
+constexpr uint32_t kItxIntra = 0x80000000u;
 
 // The intra wavefront: one group per (superblock, plane) with intra work;
 // a launch runs the groups of one dependency level.
@@ -98,7 +103,8 @@ struct IntraItem {
 
 // An intra block for k_intra_blocks: its transform blocks are
 // txbs[first .. first + count) (n_y luma, n_u Cb, then Cr; one transform
-// size per plane), their coefficients coef_*[coef_first .. + coef_count).
+// size per plane), their coefficients coef_*[coef_first .. + coef_count);
+// their residuals are in IntraArgs::res (written by k_itx_inter).
 struct IntraBlockItem {
   uint32_t first, count, coef_first, coef_count;
   uint16_t mi_row, mi_col, n_y, n_u;
@@ -107,20 +113,18 @@ struct IntraBlockItem {
 };
 
 // LDS of one k_intra_blocks workgroup (kIbThreads lanes) for a frame of
-// subsampling (ss_x, ss_y) and transform lane width t_bytes (4 at 8 bits, 8
-// above): byte offsets of the transform block records (all of the block's),
-// the first kIbStageCoefs coefficients (positions, values), the three
-// transform grids, DC values, plane geometry, coefficient tiles, and per
-// plane the region tile ((64 >> sx) + 1 columns, u16) and residuals
-// ((64 >> sx) columns, int16).  The launch passes `bytes` (the largest of
-// its frames') as dynamic LDS.
-constexpr int kIbThreads = 256, kIbCoefTile = 4352, kIbGeoBytes = 80, kIbStageCoefs = 2048;
+// subsampling (ss_x, ss_y): byte offsets of the transform block records
+// (all of the block's), the three transform grids, DC values, plane
+// geometry, and per plane the region tile ((64 >> sx) + 1 columns, u16) and
+// residuals ((64 >> sx) columns, int16).  The launch passes `bytes` (the
+// largest of its frames') as dynamic LDS.
+constexpr int kIbThreads = 256, kIbGeoBytes = 80;
 // The most transform blocks an intra block has: 64x64 at 4x4 in every plane.
 __host__ __device__ inline uint32_t ib_max_txbs(int ss_x, int ss_y) { return 256 + 2 * (256u >> (ss_x + ss_y)); }
 struct IbLayout {
-  uint32_t txb, cpos, cval, tmap, dcv, geo, ctile, tile[3], res[3], bytes;
+  uint32_t txb, tmap, dcv, geo, tile[3], res[3], bytes;
 };
-__host__ __device__ inline IbLayout ib_layout(int ss_x, int ss_y, int t_bytes) {
+__host__ __device__ inline IbLayout ib_layout(int ss_x, int ss_y) {
   IbLayout l{};
   uint32_t o = 0;
   auto take = [&o](uint32_t n) {
@@ -129,12 +133,9 @@ __host__ __device__ inline IbLayout ib_layout(int ss_x, int ss_y, int t_bytes) {
     return r;
   };
   l.txb = take(ib_max_txbs(ss_x, ss_y) * (uint32_t)sizeof(vp9r_txb));
-  l.cpos = take(kIbStageCoefs * 2);
-  l.cval = take(kIbStageCoefs * 4);
   l.tmap = take(3 * 256 * 2);
   l.dcv = take(3 * 16 * 4);
   l.geo = take(3 * kIbGeoBytes);
-  l.ctile = take(kIbCoefTile * (uint32_t)t_bytes);
   for (int p = 0; p < 3; p++) {
     const int sx = p ? ss_x : 0, sy = p ? ss_y : 0;
     l.tile[p] = take((uint32_t)((64 >> sx) + 1) * (uint32_t)((64 >> sy) + 1) * 2);
@@ -158,6 +159,7 @@ struct IntraArgs {
   const int32_t *coef_val;
   Planes cur;
   int32_t q[8][2][2];
+  const int16_t *res[3];  // the intra transform blocks' residuals (k_itx_inter)
 };
 
 // Loop filter: per superblock, plane and pass, 16 x 16 entries of filter
@@ -168,8 +170,9 @@ struct LfParams {
 };
 
 // k_lf_mask's ranges cover 6 entries (plane, pass) per superblock; k_lf's
-// ranges have first = the level c + 2r and count = 3 x its superblocks
-// (item j: plane j % 3 of the level's (j / 3)-th superblock, rows ascending).
+// ranges have first = the level t and count = 3 x its items (item j: plane
+// j % 3 of the level's (j / 3)-th unit (r, t - r), rows ascending, then the
+// lone vertical pass of superblock (t + 1, 0)).
 struct LfArgs {
   FrameGeom g;
   const vp9r_mi *mi;

@@ -682,6 +682,7 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   }
   build_mc_tiles(g, rec);
   for (auto &j : g->jobs) j.clear();
+  for (auto &j : g->ijobs) j.clear();
   for (uint32_t i = 0; i < rec->n_txbs; i++) {
     const vp9r_txb &t = rec->txbs[i];
     const uint32_t n0 = 4u << t.tx_size;
@@ -701,7 +702,10 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
       set_error("malformed record transform block");
       return VP9R_ERR_INVALID_INPUT;
     }
-    if (rec->blocks[t.block].is_inter && t.eob > 0) g->jobs[t.tx_size].push_back(i);
+    if (!rec->blocks[t.block].is_inter)
+      g->ijobs[t.tx_size].push_back(i | vp9g::kItxIntra);
+    else if (t.eob > 0)
+      g->jobs[t.tx_size].push_back(i);
   }
   build_intra(g, rec);
   {
@@ -785,13 +789,16 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   p.o_cval = take(sizeof(int32_t) * rec->n_coefs);
   p.o_mi = take(sizeof(vp9r_mi) * n_mi);
   p.o_tiles = take(sizeof(vp9g::McTile) * g->tiles.size());
-  for (int k = 0; k < 4; k++) p.o_jobs[k] = take(sizeof(uint32_t) * g->jobs[k].size());
+  for (int k = 0; k < 4; k++) p.o_jobs[k] = take(sizeof(uint32_t) * (g->jobs[k].size() + g->ijobs[k].size()));
   p.o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
   p.o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
   p.o_items = take(sizeof(vp9g::IntraBlockItem) * g->block_items.size());
   p.bytes = off;
   p.n_tiles = (uint32_t)g->tiles.size();
-  for (int k = 0; k < 4; k++) p.n_jobs[k] = (uint32_t)g->jobs[k].size();
+  for (int k = 0; k < 4; k++) {
+    p.n_jobs[k] = (uint32_t)g->jobs[k].size();
+    p.n_ijobs[k] = (uint32_t)g->ijobs[k].size();
+  }
   p.n_groups = (uint32_t)g->groups.size();
   p.n_items = (uint32_t)g->intra_items.size();
   p.item_level_start = g->item_level_start;
@@ -803,8 +810,8 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   // The loop filter's passes on superblock (r, c) read and write samples in
   // mi rows 8r - 2 .. 8r + 7 and columns 8c - 2 .. 8c + 7 (the chroma halo
   // is 8 samples at half resolution).  An intra block writes its own area
-  // and reads the row above it (with the above-right samples, up to twice
-  // its width) and the column left of it.  Since the loop filter follows
+  // and reads the row above it (above-right reads stay inside the block's
+  // columns, recon.rs:61-69) and the column left of it.  Since the loop filter follows
   // the whole frame's reconstruction (decoder/mod.rs:355-358), a level may
   // run once every intra block meeting one of its superblocks is done:
   // lf_need[t] is the highest intra level among them, over the levels <= t.
@@ -827,7 +834,6 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
       const int bh8 = b.mi_size < BLOCK_8X8 ? 1 : VP9_NUM_8X8_HIGH[b.mi_size];
       const int row = (int)b.mi_row, col = (int)b.mi_col;
       meet(row - 1, row + bh8 - 1, col - 1, col + bw8 - 1, g->item_level[k]);
-      if (row > 0) meet(row - 1, row - 1, col + bw8, col + 2 * bw8 - 1, g->item_level[k]);
     }
     auto need = [&](int r, int c) { return g->sb_need[(size_t)r * C + c]; };
     const int T = R + C - 1;
@@ -857,7 +863,11 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   put(p.o_cval, rec->coef_val, sizeof(int32_t) * rec->n_coefs);
   put(p.o_mi, rec->mi, sizeof(vp9r_mi) * n_mi);
   put(p.o_tiles, g->tiles.data(), sizeof(vp9g::McTile) * g->tiles.size());
-  for (int k = 0; k < 4; k++) put(p.o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
+  for (int k = 0; k < 4; k++) {
+    put(p.o_jobs[k], g->jobs[k].data(), sizeof(uint32_t) * g->jobs[k].size());
+    put(p.o_jobs[k] + sizeof(uint32_t) * g->jobs[k].size(), g->ijobs[k].data(),
+        sizeof(uint32_t) * g->ijobs[k].size());
+  }
   put(p.o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
   put(p.o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
   put(p.o_items, g->block_items.data(), sizeof(vp9g::IntraBlockItem) * g->block_items.size());
@@ -960,6 +970,22 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       HIP_TRY(hipStreamSynchronize(g->lf_stream), "hipStreamSynchronize");
       HIP_TRY(g->lf_mask.ensure(mask_bytes), "hipMalloc(lf masks)");
     }
+    // intra residual planes (int16, the frame store's strides) of the
+    // frames with intra blocks; read and written on the kernel stream only
+    size_t res_bytes = 0;
+    std::vector<size_t> res_off(3 * n, 0);
+    if (recon && g->intra_blocks)
+      for (int i = 0; i < n; i++) {
+        if (!f[i]->p->n_items) continue;
+        for (int pl = 0; pl < 3; pl++) {
+          res_off[3 * i + pl] = res_bytes;
+          res_bytes = align256(res_bytes + (size_t)f[i]->cur->planes.stride[pl] * f[i]->cur->planes.rows[pl] * 2);
+        }
+      }
+    if (res_bytes > g->intra_res.cap) {
+      HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+      HIP_TRY(g->intra_res.ensure(res_bytes), "hipMalloc(intra residuals)");
+    }
     std::vector<uint32_t> first(n), count(n);
     auto table = [&](auto get_first, auto get_count, uint32_t per_wg) {
       for (int i = 0; i < n; i++) {
@@ -975,10 +1001,13 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       t_clear = table([](int) { return 0u; },
                       [&](int i) { return 3 * vp9g::clear_margin_rows(f[i]->cur->planes, f[i]->p->geo); }, 1);
       t_mc = table([](int) { return 0u; }, [&](int i) { return f[i]->p->n_tiles; }, 1);
-      for (int k = 0; k < 4; k++)
-        t_itx[k] = table([](int) { return 0u; }, [&](int i) { return f[i]->p->n_jobs[k]; }, 64u >> (2 + k));
-      // intra: leveled blocks, or (superblock, plane) groups
+      // intra: leveled blocks, whose residuals the inter residual launches
+      // compute, or (superblock, plane) groups, which compute their own
       groups = !g->intra_blocks;
+      for (int k = 0; k < 4; k++)
+        t_itx[k] = table([](int) { return 0u; },
+                         [&](int i) { return f[i]->p->n_jobs[k] + (groups ? 0u : f[i]->p->n_ijobs[k]); },
+                         64u >> (2 + k));
       size_t levels = 0;
       for (int i = 0; i < n; i++) {
         const auto &ls = groups ? f[i]->p->group_level_start : f[i]->p->item_level_start;
@@ -1045,6 +1074,8 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       ia.coef_val = (const int32_t *)(dv + p.o_cval);
       ia.cur = cur->planes;
       memcpy(ia.q, p.q, sizeof p.q);
+      if (res_bytes && p.n_items)
+        for (int pl = 0; pl < 3; pl++) ia.res[pl] = (int16_t *)((uint8_t *)g->intra_res.p + res_off[3 * i + pl]);
       vp9g::IntraArgs &na = ((vp9g::IntraArgs *)(st.data() + o_intra))[i];
       na = vp9g::IntraArgs{};
       na.g = p.geo;
@@ -1056,6 +1087,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       na.coef_pos = ia.coef_pos;
       na.coef_val = ia.coef_val;
       na.cur = cur->planes;
+      for (int pl = 0; pl < 3; pl++) na.res[pl] = ia.res[pl];
       memcpy(na.q, p.q, sizeof p.q);
       vp9g::LfArgs &la = ((vp9g::LfArgs *)(st.data() + o_lf))[i];
       la = vp9g::LfArgs{};
@@ -1093,7 +1125,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       HIP_TRY(hipEventRecord(g->ev_pre, s), "hipEventRecord");
       uint32_t ib_lds = 0;
       for (int i = 0; i < n; i++)
-        ib_lds = std::max(ib_lds, vp9g::ib_layout(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y, hbd ? 8 : 4).bytes);
+        ib_lds = std::max(ib_lds, vp9g::ib_layout(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y).bytes);
       // the intra levels the loop filter waits for (an event after each)
       std::vector<char> mark(t_intra.size(), 0);
       if (lf && !t_intra.empty()) {

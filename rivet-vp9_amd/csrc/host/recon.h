@@ -40,7 +40,10 @@ struct Prepared {
   vp9g::LfParams lfp;
   size_t o_blocks = 0, o_txbs = 0, o_cpos = 0, o_cval = 0, o_mi = 0, o_tiles = 0, o_jobs[4] = {},
          o_iidx = 0, o_groups = 0, o_items = 0;
-  uint32_t n_tiles = 0, n_jobs[4] = {}, n_groups = 0, n_items = 0;
+  // jobs[k] lists the inter transform blocks with coefficients (n_jobs[k]),
+  // then every transform block of the intra blocks (n_ijobs[k], flagged
+  // kItxIntra), whose residuals the intra block kernel reads
+  uint32_t n_tiles = 0, n_jobs[4] = {}, n_ijobs[4] = {}, n_groups = 0, n_items = 0;
   std::vector<uint32_t> group_level_start, lf_level_start, item_level_start;
   // per loop filter level: the last intra level (of leveled intra blocks)
   // that must be complete before it, -1 for none
@@ -55,7 +58,7 @@ struct Prepared {
 // thread that prepares records).
 struct Builder {
   std::vector<vp9g::McTile> tiles;
-  std::vector<uint32_t> jobs[4];
+  std::vector<uint32_t> jobs[4], ijobs[4];
   std::vector<uint32_t> intra_cnt, intra_first, intra_fill, intra_index;
   std::vector<vp9g::IntraItem> intra_items, items_sorted;
   std::vector<vp9g::IntraBlockItem> block_items;  // items_sorted as the kernel reads them
@@ -120,7 +123,7 @@ struct vp9r_gpu {
   hipStream_t lf_stream = nullptr;
   hipEvent_t ev_up = nullptr, ev_pre = nullptr, ev_lf_done = nullptr;
   std::vector<hipEvent_t> intra_events;
-  vp9r::DevBuf arena, lf_mask;
+  vp9r::DevBuf arena, lf_mask, intra_res;
   vp9r::HostBuf stage;
   hipEvent_t ev_upload = nullptr;
   bool upload_pending = false;
