@@ -219,53 +219,70 @@ struct __align__(16) LfShared {
   uint32_t dq[3][272];  // packed decisions of the item's passes (16 slack entries each)
 };
 
-// The region as 8-sample vectors: the rows above (8 << lnt vectors, none in
-// superblock row 0 and for a lone vertical pass), then rows - 8 rows of
-// 1 << lnm vectors.
+// The region in parts of 8-sample vectors (nt = 1 << lnt vectors per
+// superblock row): part 0 is what the first vertical and the horizontal
+// pass read, the 8 rows above (8 nt vectors, none in superblock row 0 and
+// for a lone vertical pass) and then superblock c's sh rows; part 1 is
+// superblock c + 1's rows, which only the second vertical pass reads (it is
+// written to LDS after the horizontal pass, its loads in flight meanwhile);
+// part 2 is superblock c's last vector of each row, the 8 samples left of
+// superblock c + 1 that the second vertical pass rewrites (the rest of part
+// 0 goes back to the frame before that pass runs).
 struct LfItem {
-  int x0, y0, top, lnt, lnm, rows;
+  int x0, y0, top, lnt, sh;
 };
 
+template <int PART>
 DEV bool lf_vec(const LfItem &it, int k, int &r, int &q) {
-  if (k < it.top) {
-    r = k >> it.lnt;
-    q = k & ((1 << it.lnt) - 1);
-    return true;
+  const int nt = 1 << it.lnt;
+  if (PART == 0) {
+    if (k < it.top) {
+      r = k >> it.lnt;
+      q = k & (nt - 1);
+      return true;
+    }
+    k -= it.top;
+    r = 8 + (k >> it.lnt);
+    q = k & (nt - 1);
+    return k < (it.sh << it.lnt);
   }
-  k -= it.top;
-  r = 8 + (k >> it.lnm);
-  q = k & ((1 << it.lnm) - 1);
-  return r < it.rows;
+  if (PART == 1) {
+    r = 8 + (k >> it.lnt);
+    q = nt + (k & (nt - 1));
+    return k < (it.sh << it.lnt);
+  }
+  r = 8 + k;
+  q = nt - 1;
+  return k < it.sh;
 }
 
-// At most 8 x 8 + 64 x 16 vectors (a 64-sample-wide plane superblock row).
-template <class P>
+// Part 0: at most 8 x 8 + 64 x 8 vectors; part 1: 64 x 8; part 2: 64.
+template <class P, int N>
 struct LfRegion {
   typedef typename std::conditional<sizeof(P) == 1, uint2, uint4>::type V;  // 8 samples
-  static constexpr int kMax = 17;
-  V v[kMax];
+  V v[N];
 };
 
-// Issues every load of the region (into registers) at once.
-template <class P>
-DEV void lf_region_load(const P *pl, uint32_t stride, const LfItem &it, LfRegion<P> &rg) {
-  typedef typename LfRegion<P>::V V;
+// Issues every load of a part (into registers) at once.
+template <class P, int PART, int N>
+DEV void lf_region_load(const P *pl, uint32_t stride, const LfItem &it, LfRegion<P, N> &rg) {
+  typedef typename LfRegion<P, N>::V V;
 #pragma unroll
-  for (int i = 0; i < LfRegion<P>::kMax; i++) {
+  for (int i = 0; i < N; i++) {
     int r, q;
-    if (lf_vec(it, threadIdx.x + 64 * i, r, q))
+    if (lf_vec<PART>(it, threadIdx.x + 64 * i, r, q))
       rg.v[i] = *(const V *)(pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q);
   }
 }
 
 // Samples as 16-bit values, two to a dword store (BP and the column offsets
 // are even).
-template <class P>
-DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P> &rg, uint16_t *buf) {
+template <class P, int PART, int N>
+DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P, N> &rg, uint16_t *buf) {
 #pragma unroll
-  for (int i = 0; i < LfRegion<P>::kMax; i++) {
+  for (int i = 0; i < N; i++) {
     int r, q;
-    if (!lf_vec(it, threadIdx.x + 64 * i, r, q)) continue;
+    if (!lf_vec<PART>(it, threadIdx.x + 64 * i, r, q)) continue;
     uint32_t *l = (uint32_t *)(buf + r * BP + 8 + 8 * q);
     if (sizeof(P) == 1) {
       const uint2 w = *(const uint2 *)&rg.v[i];
@@ -283,12 +300,15 @@ DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P> &rg, uint16_t *buf
   }
 }
 
-template <class P>
-DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_t *buf) {
+// Stores the vectors of a part; skip_last leaves out the last vector of
+// each superblock row (part 0 before the second vertical pass).
+template <class P, int PART, int N>
+DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_t *buf, bool skip_last) {
 #pragma unroll
-  for (int i = 0; i < LfRegion<P>::kMax; i++) {
+  for (int i = 0; i < N; i++) {
     int r, q;
-    if (!lf_vec(it, threadIdx.x + 64 * i, r, q)) continue;
+    if (!lf_vec<PART>(it, threadIdx.x + 64 * i, r, q)) continue;
+    if (skip_last && r >= 8 && q == (1 << it.lnt) - 1) continue;
     const uint32_t *l = (const uint32_t *)(buf + r * BP + 8 + 8 * q);
     P *g = pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q;
     if (sizeof(P) == 1) {
@@ -393,8 +413,7 @@ __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, ui
   it.y0 = sb_row * SH;
   it.top = lone_v || sb_row == 0 ? 0 : 8 << lnt;
   it.lnt = lnt;
-  it.lnm = lnt + (right_v ? 1 : 0);
-  it.rows = 8 + SH;
+  it.sh = SH;
   const uint32_t stride = a.cur.stride[plane];
   P *pl = (P *)a.cur.p[plane];
   // every global load at once: the passes' decisions and the region
@@ -405,12 +424,14 @@ __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, ui
   if (first_v) d0 = dv[sb * 64 + lane];
   if (!lone_v) d1 = dh[sb * 64 + lane];
   if (right_v) d2 = dv[(sb + 1) * 64 + lane];
-  LfRegion<P> region;
-  lf_region_load<P>(pl, stride, it, region);
+  LfRegion<P, 9> part0;
+  LfRegion<P, 8> part1;
+  lf_region_load<P, 0>(pl, stride, it, part0);
+  if (right_v) lf_region_load<P, 1>(pl, stride, it, part1);
   ((uint4 *)sh.dq[0])[lane] = d0;
   ((uint4 *)sh.dq[1])[lane] = d1;
   ((uint4 *)sh.dq[2])[lane] = d2;
-  lf_region_to_lds<P>(it, region, sh.buf);
+  lf_region_to_lds<P, 0>(it, part0, sh.buf);
   __syncthreads();
   // pass 0: the vertical edges of superblock (r, c) (c = 0 only); pass 1 its
   // horizontal edges; pass 2 the vertical edges of (r, c + 1).  One copy of
@@ -418,13 +439,24 @@ __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, ui
 #pragma unroll 1
   for (int pass = 0; pass < 3; pass++) {
     if (pass == 0 ? !first_v : pass == 1 ? lone_v : !right_v) continue;
+    if (pass == 2) {
+      // superblock c + 1 into LDS, superblock c's finished samples out
+      lf_region_to_lds<P, 1>(it, part1, sh.buf);
+      lf_region_store<P, 0, 9>(pl, stride, it, sh.buf, true);
+      __syncthreads();
+    }
     const bool vert = pass != 1;
     if (lane < (vert ? SH : SW))
       lf_stream(sh.buf, vert ? (8 + lane) * BP + (pass == 2 ? SW : 0) : 8 + lane, vert ? 1 : BP,
                 (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, HBD ? (int)g.bit_depth : 8);
     __syncthreads();
   }
-  lf_region_store<P>(pl, stride, it, sh.buf);
+  if (right_v) {
+    lf_region_store<P, 1, 8>(pl, stride, it, sh.buf, false);
+    lf_region_store<P, 2, 1>(pl, stride, it, sh.buf, false);
+  } else {
+    lf_region_store<P, 0, 9>(pl, stride, it, sh.buf, false);
+  }
 }
 
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs) {
