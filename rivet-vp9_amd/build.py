@@ -37,9 +37,11 @@ HIP_SOURCES = [
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-missing-braces", "-Wno-unused-function",
           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc")]
-# gfx950 device code; packed-FP32 instructions disabled.
+# gfx950 device code; packed-FP32 instructions disabled; the first 16 dwords
+# of kernel arguments preloaded into SGPRs (the code keeps the loading
+# preamble for a dispatcher that does not preload).
 GPU = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-       "-munsafe-fp-atomics"]
+       "-munsafe-fp-atomics", "-mllvm", "-amdgpu-kernarg-preload-count=16"]
 
 
 # Per-file flags.  lf.hip: the loop filter keeps a 16-sample window of each
