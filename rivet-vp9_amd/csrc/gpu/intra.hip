@@ -342,33 +342,39 @@ DEV IbRegion ib_region(const IntraBlockItem &it, const FrameGeom &g, int p, int 
 // The intra block workgroup's LDS (laid out by ib_layout).
 extern __shared__ __align__(16) uint8_t ib_lds[];
 
-// The edges of a transform block of a plane staged in the intra block's LDS
-// (recon.rs:17-103).
-struct Edges {
-  uint32_t tile_off;  // bytes into the LDS (an offset, so that reads stay LDS reads)
-  int tp, x, y, x0, y0, size, max_x, max_y, base;
-  bool have_left, have_above, a4;  // a4: above-right real (4x4 not on the block's right)
-  DEV int T(int py, int px) const {
-    return ((const uint16_t *)(ib_lds + tile_off))[(py - y0 + 1) * tp + (px - x0 + 1)];
+// One transform block of the intra block, set up before the wavefront: its
+// place in the LDS tile and residual plane (16-bit element indices into the
+// workgroup's LDS), its mode and edge availability, and the clamps of its
+// edge reads (recon.rs:17-103): the above row is read up to max_x - x
+// samples right of x (and, past the block's width, repeats its last sample
+// unless it is a 4x4 not on the block's right: a4), the left column up to
+// max_y - y rows down.  Grid cell (plane, r, c) of the block holds it.
+struct IbTx {
+  int tile, tp, res, rp;  // tile[tile + i tp + j] = sample (i, j); res likewise
+  int bits;               // mode | log2 << 4 | above << 8 | left << 9 | a4 << 10 | valid << 11
+  int lim_a, lim_l, _pad;
+  DEV int mode() const { return bits & 15; }
+  DEV bool valid() const { return (bits >> 11) & 1; }
+  DEV int T(int idx) const { return ((const uint16_t *)ib_lds)[idx]; }
+  DEV int A(int k, int size, int base) const {
+    if (!((bits >> 8) & 1)) return base - 1;
+    if (k < 0) return ((bits >> 9) & 1) ? T(tile - tp - 1) : base + 1;
+    return T(tile - tp + min((k >= size && !((bits >> 10) & 1)) ? size - 1 : k, lim_a));
   }
-  DEV int A(int k) const {
-    if (!have_above) return base - 1;
-    if (k < 0) return have_left ? T(y - 1, min(max_x, x - 1)) : base + 1;
-    return T(y - 1, min(max_x, x + ((k >= size && !a4) ? size - 1 : k)));
-  }
-  DEV int Lf(int k) const { return have_left ? T(min(max_y, y + k), x - 1) : base + 1; }
+  DEV int L(int k, int base) const { return ((bits >> 9) & 1) ? T(tile - 1 + min(k, lim_l) * tp) : base + 1; }
   // DC_PRED's value (intra.rs:145-168) for size 4 << (LOG2 - 2)
   template <int LOG2>
   DEV int dc(int bd) const {
     constexpr int size = 1 << LOG2;
+    const bool have_above = (bits >> 8) & 1, have_left = (bits >> 9) & 1;
     int sa = 0, sl = 0;
     if (have_above) {
 #pragma unroll
-      for (int k = 0; k < size; k++) sa += T(y - 1, min(max_x, x + k));
+      for (int k = 0; k < size; k++) sa += T(tile - tp + min(k, lim_a));
     }
     if (have_left) {
 #pragma unroll
-      for (int k = 0; k < size; k++) sl += T(min(max_y, y + k), x - 1);
+      for (int k = 0; k < size; k++) sl += T(tile - 1 + min(k, lim_l) * tp);
     }
     if (have_left && have_above) return (sa + sl + size) >> (LOG2 + 1);
     if (have_left || have_above) return (sa + sl + (size >> 1)) >> LOG2;
@@ -386,7 +392,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
   const IntraBlockItem it = a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
   const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
   vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
-  uint16_t *tmap = (uint16_t *)(ib_lds + ly.tmap);
+  IbTx *txs = (IbTx *)(ib_lds + ly.txs);
   int32_t *dcv = (int32_t *)(ib_lds + ly.dcv);
   IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
   // the transform block records, all in LDS (at most ib_max_txbs; the host
@@ -495,7 +501,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
       }
     }
   }
-  for (int k = tid; k < 3 * 256; k += kIbThreads) tmap[k] = 0xffff;
+  for (int k = tid; k < 3 * 256; k += kIbThreads) txs[k].bits = 0;
   __syncthreads();
   if (tid < 3) {
     const int p = tid;
@@ -523,97 +529,89 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
     geo[p] = G;
   }
   __syncthreads();
+  // every transform block's descriptor in its grid cell
   for (uint32_t j = tid; j < it.count; j += kIbThreads) {
     const vp9r_txb t = txb_at(j);
     const IbGeo &G = geo[t.plane];
-    tmap[t.plane * 256 + ((t.y - G.y0) >> G.log2) * 16 + ((t.x - G.x0) >> G.log2)] = (uint16_t)j;
+    const int tx = t.x - G.x0, ty = t.y - G.y0, xi = tx >> 2, yi = ty >> 2, step4 = 1 << (G.log2 - 2);
+    int mode = G.mode_u;
+    if (mode < 0) {
+      const int block_idx = (yi / step4) * ((G.n4w + step4 - 1) / step4) + xi / step4;
+      mode = (int)((it.sub_modes >> (8 * (block_idx & 3))) & 0xff);
+    }
+    IbTx d;
+    d.tp = G.tpitch;
+    d.tile = (int)(G.tile_off >> 1) + (ty + 1) * G.tpitch + tx + 1;
+    d.rp = G.rpitch;
+    d.res = (int)(G.res_off >> 1) + ty * G.rpitch + tx;
+    const bool have_left = avail_l || xi > 0, have_above = avail_u || yi > 0;
+    const bool a4 = G.log2 == 2 && xi + step4 < G.n4w;
+    d.bits = mode | G.log2 << 4 | (int)have_above << 8 | (int)have_left << 9 | (int)a4 << 10 | 1 << 11;
+    d.lim_a = G.max_x - (int)t.x;
+    d.lim_l = G.max_y - (int)t.y;
+    d._pad = 0;
+    txs[t.plane * 256 + (ty >> G.log2) * 16 + (tx >> G.log2)] = d;
   }
   __syncthreads();
-  // the wavefront over the three grids
-  const int nlv = max(max(geo[0].nlv, geo[1].nlv), geo[2].nlv);
+  // the wavefront over the three grids; the planes' grid shapes in registers
+  int lg[3], kk[3], gw[3], gh[3], gn[3];
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    lg[p] = geo[p].log2;
+    kk[p] = geo[p].kk;
+    gw[p] = geo[p].W;
+    gh[p] = geo[p].H;
+    gn[p] = geo[p].nlv;
+  }
+  const int nlv = max(max(gn[0], gn[1]), gn[2]);
   bool any_dc = (np0 && geo[0].mode_u == DC_PRED) || ((np1 + np2) && it.uv_mode == DC_PRED);
   if (np0 && geo[0].mode_u < 0)
     for (int q = 0; q < 4; q++) any_dc = any_dc || ((it.sub_modes >> (8 * q)) & 0xff) == DC_PRED;
-  auto mode_of = [&](const vp9r_txb &t, const IbGeo &G) -> int {
-    if (G.mode_u >= 0) return G.mode_u;
-    const int step4 = 1 << (G.log2 - 2);
-    const int xi = (t.x - G.x0) >> 2, yi = (t.y - G.y0) >> 2;
-    const int block_idx = (yi / step4) * ((G.n4w + step4 - 1) / step4) + xi / step4;
-    return (int)((it.sub_modes >> (8 * (block_idx & 3))) & 0xff);
-  };
-  // level lv of plane G: rows r_lo .. of the grid, n_l blocks
-  auto level = [&](const IbGeo &G, int lv, int &r_lo) -> int {
-    if (lv >= G.nlv) return 0;
-    r_lo = lv > G.W - 1 ? (lv - (G.W - 1) + G.kk - 1) / G.kk : 0;
-    const int r_hi = min(G.H - 1, lv / G.kk);
+  // level lv of plane p: rows r_lo .. of the grid, n_l blocks
+  auto level = [&](int p, int lv, int &r_lo) -> int {
+    if (lv >= gn[p]) return 0;
+    r_lo = lv > gw[p] - 1 ? (lv - (gw[p] - 1) + kk[p] - 1) / kk[p] : 0;
+    const int r_hi = min(gh[p] - 1, lv / kk[p]);
     return max(r_hi - r_lo + 1, 0);
-  };
-  // the edges of transform block t of plane G (Edges)
-  auto edges_of = [&](const vp9r_txb &t, const IbGeo &G) {
-    Edges e;
-    e.tile_off = G.tile_off;
-    e.tp = G.tpitch;
-    e.x = t.x;
-    e.y = t.y;
-    e.x0 = G.x0;
-    e.y0 = G.y0;
-    e.size = 1 << G.log2;
-    e.max_x = G.max_x;
-    e.max_y = G.max_y;
-    e.base = base;
-    const int xi = (t.x - G.x0) >> 2, yi = (t.y - G.y0) >> 2, step4 = 1 << (G.log2 - 2);
-    e.have_left = avail_l || xi > 0;
-    e.have_above = avail_u || yi > 0;
-    e.a4 = G.log2 == 2 && xi + step4 < G.n4w;
-    return e;
   };
   for (int lv = 0; lv < nlv; lv++) {
     int r0 = 0, r1 = 0, r2 = 0;
-    const int n0 = level(geo[0], lv, r0), n1 = level(geo[1], lv, r1), n2 = level(geo[2], lv, r2);
+    const int n0 = level(0, lv, r0), n1 = level(1, lv, r1), n2 = level(2, lv, r2);
     if (any_dc) {
       if (tid < n0 + n1 + n2) {
         const int p = tid < n0 ? 0 : tid < n0 + n1 ? 1 : 2;
         const int jt = tid - (p == 0 ? 0 : p == 1 ? n0 : n0 + n1);
-        const IbGeo &G = geo[p];
-        const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - G.kk * r;
-        const uint16_t ix = tmap[p * 256 + r * 16 + c];
-        if (ix != 0xffff) {
-          const vp9r_txb t = txb_at(ix);
-          if (mode_of(t, G) == DC_PRED) {
-            const Edges e = edges_of(t, G);
-            int v;
-            switch (G.log2) {  // every edge read in flight at once
-              case 2: v = e.dc<2>(bd); break;
-              case 3: v = e.dc<3>(bd); break;
-              case 4: v = e.dc<4>(bd); break;
-              default: v = e.dc<5>(bd); break;
-            }
-            dcv[p * 16 + jt] = v;
+        const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - kk[p] * r;
+        const IbTx d = txs[p * 256 + r * 16 + c];
+        if (d.valid() && d.mode() == DC_PRED) {
+          int v;
+          switch (lg[p]) {  // every edge read in flight at once
+            case 2: v = d.dc<2>(bd); break;
+            case 3: v = d.dc<3>(bd); break;
+            case 4: v = d.dc<4>(bd); break;
+            default: v = d.dc<5>(bd); break;
           }
+          dcv[p * 16 + jt] = v;
         }
       }
       __syncthreads();
     }
-    const int i0 = n0 << (2 * geo[0].log2), i1 = n1 << (2 * geo[1].log2), i2 = n2 << (2 * geo[2].log2);
+    const int i0 = n0 << (2 * lg[0]), i1 = n1 << (2 * lg[1]), i2 = n2 << (2 * lg[2]);
     for (int item = tid; item < i0 + i1 + i2; item += kIbThreads) {
       const int p = item < i0 ? 0 : item < i0 + i1 ? 1 : 2;
       const int local = item - (p == 0 ? 0 : p == 1 ? i0 : i0 + i1);
-      const IbGeo &G = geo[p];
-      const int jt = local >> (2 * G.log2), s = local & ((1 << (2 * G.log2)) - 1);
-      const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - G.kk * r;
-      const uint16_t ix = tmap[p * 256 + r * 16 + c];
-      if (ix == 0xffff) continue;
-      const vp9r_txb t = txb_at(ix);
-      const Edges e = edges_of(t, G);
-      const int mode = mode_of(t, G);
-      const int size = 1 << G.log2, i = s >> G.log2, j = s & (size - 1);
+      const int l2 = lg[p], jt = local >> (2 * l2), s = local & ((1 << (2 * l2)) - 1);
+      const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - kk[p] * r;
+      const IbTx d = txs[p * 256 + r * 16 + c];
+      if (!d.valid()) continue;
+      const int mode = d.mode();
+      const int size = 1 << l2, i = s >> l2, j = s & (size - 1);
       const int pv = intra_sample_at(
-          mode, i, j, size, [&](int k) { return e.A(k); }, [&](int k) { return e.Lf(k); },
+          mode, i, j, size, [&](int k) { return d.A(k, size, base); }, [&](int k) { return d.L(k, base); },
           mode == DC_PRED ? dcv[p * 16 + jt] : 0, maxv);
-      const int ry = t.y - G.y0 + i, rx = t.x - G.x0 + j;
-      const int16_t *res = (const int16_t *)(ib_lds + G.res_off);
-      uint16_t *tile = (uint16_t *)(ib_lds + G.tile_off);
-      tile[(ry + 1) * G.tpitch + rx + 1] = (uint16_t)iclamp(pv + res[ry * G.rpitch + rx], 0, maxv);
+      const int16_t *res = (const int16_t *)ib_lds;
+      uint16_t *tile = (uint16_t *)ib_lds;
+      tile[d.tile + i * d.tp + j] = (uint16_t)iclamp(pv + res[d.res + i * d.rp + j], 0, maxv);
     }
     __syncthreads();
   }

@@ -114,15 +114,15 @@ struct IntraBlockItem {
 
 // LDS of one k_intra_blocks workgroup (kIbThreads lanes) for a frame of
 // subsampling (ss_x, ss_y): byte offsets of the transform block records
-// (all of the block's), the three transform grids, DC values, plane
-// geometry, and per plane the region tile ((64 >> sx) + 1 columns, u16) and
+// (all of the block's), the three transform grids' descriptors (32 B per
+// cell), DC values, plane geometry, and per plane the region tile ((64 >> sx) + 1 columns, u16) and
 // residuals ((64 >> sx) columns, int16).  The launch passes `bytes` (the
 // largest of its frames') as dynamic LDS.
 constexpr int kIbThreads = 256, kIbGeoBytes = 80;
 // The most transform blocks an intra block has: 64x64 at 4x4 in every plane.
 __host__ __device__ inline uint32_t ib_max_txbs(int ss_x, int ss_y) { return 256 + 2 * (256u >> (ss_x + ss_y)); }
 struct IbLayout {
-  uint32_t txb, tmap, dcv, geo, tile[3], res[3], bytes;
+  uint32_t txb, txs, dcv, geo, tile[3], res[3], bytes;
 };
 __host__ __device__ inline IbLayout ib_layout(int ss_x, int ss_y) {
   IbLayout l{};
@@ -133,7 +133,7 @@ __host__ __device__ inline IbLayout ib_layout(int ss_x, int ss_y) {
     return r;
   };
   l.txb = take(ib_max_txbs(ss_x, ss_y) * (uint32_t)sizeof(vp9r_txb));
-  l.tmap = take(3 * 256 * 2);
+  l.txs = take(3 * 256 * 32);
   l.dcv = take(3 * 16 * 4);
   l.geo = take(3 * kIbGeoBytes);
   for (int p = 0; p < 3; p++) {
