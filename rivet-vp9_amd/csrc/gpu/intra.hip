@@ -383,13 +383,16 @@ struct IbTx {
 };
 
 template <bool HBD>
-__global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa, const Range *rg, uint32_t nf) {
+__global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa, const Range *rg, uint32_t nf,
+                                                             const IntraBlockItem *items0) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   const int tid = threadIdx.x;
   const uint32_t fi = range_frame(rg, nf, blockIdx.x);
   const IntraArgs &a = fa[fi];
   const FrameGeom &g = a.g;
-  const IntraBlockItem it = a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
+  // a launch of one frame passes its items (the item loads with the frame's
+  // arguments instead of after them)
+  const IntraBlockItem it = items0 ? items0[blockIdx.x] : a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
   const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
   vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
   IbTx *txs = (IbTx *)(ib_lds + ly.txs);
@@ -657,7 +660,7 @@ void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n
 }
 
 void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-                         uint32_t lds_bytes) {
+                         uint32_t lds_bytes, const IntraBlockItem *items0) {
   if (!wgs) return;
   // dynamic LDS above 64 KB (4:4:4 at high bit depth) needs the attribute
   static bool attr_set[2] = {false, false};
@@ -671,9 +674,9 @@ void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uin
     attr_set[hbd] = true;
   }
   if (hbd)
-    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n);
+    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0);
   else
-    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n);
+    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0);
 }
 
 // Per-kernel seam: one intra::predict call (intra.rs:19-29) on host-given

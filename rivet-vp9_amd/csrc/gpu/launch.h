@@ -183,6 +183,28 @@ struct LfArgs {
   LfParams p;
 };
 
+// What a k_lf workgroup needs of its frame: the planes, the decisions, the
+// superblock grid, subsampling and bit depth (geo: cols | rows << 12 |
+// ss_x << 24 | ss_y << 25 | bit_depth << 26), and the level t.  13 dwords,
+// first among the kernel's arguments, so that they arrive preloaded.
+struct LfOne {
+  uint8_t *p[3];
+  uint32_t stride[3];
+  uint32_t *dq;
+  uint32_t geo, t;
+};
+__host__ __device__ inline LfOne lf_one(const LfArgs &a, uint32_t t) {
+  LfOne o;
+  for (int k = 0; k < 3; k++) {
+    o.p[k] = (uint8_t *)a.cur.p[k];
+    o.stride[k] = a.cur.stride[k];
+  }
+  o.dq = a.dq;
+  o.geo = a.g.sb64_cols | a.g.sb64_rows << 12 | a.g.ss_x << 24 | a.g.ss_y << 25 | a.g.bit_depth << 26;
+  o.t = t;
+  return o;
+}
+
 // Launchers: fa = per-frame arguments, r = ranges (n + 1 entries), all in
 // device memory; wgs = r[n].wg; hbd: samples are u16.
 uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g);
@@ -191,10 +213,15 @@ void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int tx_size,
                       bool hbd);
 void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+// items0: for a launch of one frame, that frame's items of the level (else
+// null: each workgroup finds its item through the range table)
 void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-                         uint32_t lds_bytes);
+                         uint32_t lds_bytes, const IntraBlockItem *items0);
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs);
-void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+// one: for a launch of one frame, lf_one of its arguments and the level
+// (else p[0] null: each workgroup finds its frame through the range table)
+void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
+                     const LfOne &one);
 
 // Per-kernel seam (vp9r.h batched forms).
 void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,

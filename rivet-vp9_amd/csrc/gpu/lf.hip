@@ -387,26 +387,45 @@ DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, 
 // ---- one wavefront level per launch (decisions from k_lf_mask)
 
 template <bool HBD>
-__global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, uint32_t nf) {
+__global__ __launch_bounds__(64) void k_lf(uint8_t *p0, uint8_t *p1, uint8_t *p2, uint32_t s0, uint32_t s1,
+                                           uint32_t s2, uint32_t *dq, uint32_t geo, uint32_t t1, const LfArgs *fa,
+                                           const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ LfShared sh;
-  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
-  const LfArgs &a = fa[fi];
-  const FrameGeom &g = a.g;
   const int lane = threadIdx.x;
+  // a launch of one frame passes what the workgroups need as arguments
+  // (preloaded, so the region loads need no load before them: LfOne); a
+  // batch finds each frame's through the range table
+  int j = (int)blockIdx.x;
+  if (!p0) {
+    const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+    const LfOne o = lf_one(fa[fi], rg[fi].first);
+    p0 = o.p[0];
+    p1 = o.p[1];
+    p2 = o.p[2];
+    s0 = o.stride[0];
+    s1 = o.stride[1];
+    s2 = o.stride[2];
+    dq = o.dq;
+    geo = o.geo;
+    t1 = o.t;
+    j = (int)(blockIdx.x - rg[fi].wg);
+  }
+  const LfOne f = {{p0, p1, p2}, {s0, s1, s2}, dq, geo, t1};
   // Level t: items j / 3 (plane j % 3) below n_units are the units (r, t - r)
   // from the first row the level reaches; item n_units, when t + 1 < R, is
   // the vertical pass of superblock (t + 1, 0), which no later item of row
   // t + 1 includes (unit (0, 0) does its own).
-  const int t = (int)rg[fi].first, j = (int)(blockIdx.x - rg[fi].wg);
-  const int C = (int)g.sb64_cols, R = (int)g.sb64_rows;
+  const int t = (int)f.t;
+  const int C = (int)(f.geo & 0xfff), R = (int)((f.geo >> 12) & 0xfff);
   const int r_lo = t >= C ? t - C + 1 : 0, n_units = min(R - 1, t) - r_lo + 1;
   const int plane = j % 3, item = j / 3;
   const bool lone_v = item >= n_units;
   const int sb_row = lone_v ? t + 1 : r_lo + item, sb_col = lone_v ? 0 : t - sb_row;
   const bool first_v = lone_v || (sb_row == 0 && sb_col == 0);
   const bool right_v = !lone_v && sb_col + 1 < C;
-  const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
+  const int sx = plane ? (int)((f.geo >> 24) & 1) : 0, sy = plane ? (int)((f.geo >> 25) & 1) : 0;
+  const int bit_depth = (int)(f.geo >> 26);
   const int SW = 64 >> sx, SH = 64 >> sy, lnt = 3 - sx;
   LfItem it;
   it.x0 = sb_col * SW;
@@ -414,12 +433,12 @@ __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, ui
   it.top = lone_v || sb_row == 0 ? 0 : 8 << lnt;
   it.lnt = lnt;
   it.sh = SH;
-  const uint32_t stride = a.cur.stride[plane];
-  P *pl = (P *)a.cur.p[plane];
+  const uint32_t stride = plane == 0 ? s0 : plane == 1 ? s1 : s2;
+  P *pl = (P *)(plane == 0 ? p0 : plane == 1 ? p1 : p2);
   // every global load at once: the passes' decisions and the region
   const size_t n_sb = (size_t)R * C, sb = (size_t)sb_row * C + sb_col;
-  const uint4 *dv = (const uint4 *)(a.dq + (plane * 2 + 0) * n_sb * 256);
-  const uint4 *dh = (const uint4 *)(a.dq + (plane * 2 + 1) * n_sb * 256);
+  const uint4 *dv = (const uint4 *)(f.dq + (plane * 2 + 0) * n_sb * 256);
+  const uint4 *dh = (const uint4 *)(f.dq + (plane * 2 + 1) * n_sb * 256);
   uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0, d2 = d0;
   if (first_v) d0 = dv[sb * 64 + lane];
   if (!lone_v) d1 = dh[sb * 64 + lane];
@@ -448,7 +467,7 @@ __global__ __launch_bounds__(64) void k_lf(const LfArgs *fa, const Range *rg, ui
     const bool vert = pass != 1;
     if (lane < (vert ? SH : SW))
       lf_stream(sh.buf, vert ? (8 + lane) * BP + (pass == 2 ? SW : 0) : 8 + lane, vert ? 1 : BP,
-                (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, HBD ? (int)g.bit_depth : 8);
+                (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, HBD ? bit_depth : 8);
     __syncthreads();
   }
   if (right_v) {
@@ -464,12 +483,15 @@ void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n,
   hipLaunchKernelGGL(k_lf_mask, dim3(wgs), dim3(256), 0, s, fa, r, n);
 }
 
-void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
+void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
+                     const LfOne &one) {
   if (!wgs) return;
   if (hbd)
-    hipLaunchKernelGGL(k_lf<true>, dim3(wgs), dim3(64), 0, s, fa, r, n);
+    hipLaunchKernelGGL(k_lf<true>, dim3(wgs), dim3(64), 0, s, one.p[0], one.p[1], one.p[2], one.stride[0],
+                       one.stride[1], one.stride[2], one.dq, one.geo, one.t, fa, r, n);
   else
-    hipLaunchKernelGGL(k_lf<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
+    hipLaunchKernelGGL(k_lf<false>, dim3(wgs), dim3(64), 0, s, one.p[0], one.p[1], one.p[2], one.stride[0],
+                       one.stride[1], one.stride[2], one.dq, one.geo, one.t, fa, r, n);
 }
 
 // Per-kernel seam: lf::filter_edges on global samples, one pass per

@@ -1149,7 +1149,10 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
         if (groups)
           vp9g::launch_intra(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
         else
-          vp9g::launch_intra_blocks(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd, ib_lds);
+          vp9g::launch_intra_blocks(
+              s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd, ib_lds,
+              n == 1 ? (const vp9g::IntraBlockItem *)(f[0]->dv + f[0]->p->o_items) + f[0]->p->item_level_start[l]
+                     : nullptr);
         if (mark[l]) HIP_TRY(hipEventRecord(g->intra_events[l], s), "hipEventRecord");
       }
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
@@ -1183,7 +1186,9 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
           HIP_TRY(hipStreamWaitEvent(sl, g->intra_events[need], 0), "hipStreamWaitEvent");
           waited = need;
         }
-        vp9g::launch_lf_level(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_lf[l]), n, t_lf[l].wgs, hbd);
+        vp9g::launch_lf_level(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_lf[l]), n, t_lf[l].wgs, hbd,
+                              n == 1 ? vp9g::lf_one(((const vp9g::LfArgs *)(st.data() + o_lf))[0], (uint32_t)l)
+                                     : vp9g::LfOne{});
       }
       HIP_TRY(hipEventRecord(g->ev_lf_done, sl), "hipEventRecord");
       HIP_TRY(hipStreamWaitEvent(s, g->ev_lf_done, 0), "hipStreamWaitEvent");
