@@ -15,6 +15,41 @@
 #define DEV __device__ __forceinline__
 #define HDEV __host__ __device__ __forceinline__
 
+// Loads and stores through global-memory pointers.  Pointers that reach a
+// kernel inside argument structs are generic, and generic (flat) accesses
+// count against the LDS wait counter too: an LDS wait would also wait for
+// every frame load or store in flight.
+// (through a native type of the same size: 4, 8 or 16 bytes)
+template <int N>
+struct GWord;
+template <>
+struct GWord<4> {
+  typedef unsigned T;
+};
+template <>
+struct GWord<8> {
+  typedef unsigned __attribute__((ext_vector_type(2))) T;
+};
+template <>
+struct GWord<16> {
+  typedef unsigned __attribute__((ext_vector_type(4))) T;
+};
+template <class T>
+DEV T gld(const T *p) {
+  typedef typename GWord<sizeof(T)>::T W;
+  const W w = *(const __attribute__((address_space(1))) W *)p;
+  T r;
+  __builtin_memcpy(&r, &w, sizeof(T));
+  return r;
+}
+template <class T>
+DEV void gst(T *p, const T &v) {
+  typedef typename GWord<sizeof(T)>::T W;
+  W w;
+  __builtin_memcpy(&w, &v, sizeof(T));
+  *(__attribute__((address_space(1))) W *)p = w;
+}
+
 namespace vp9g {
 
 // ---- integer lanes

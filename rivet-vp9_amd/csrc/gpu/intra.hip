@@ -382,17 +382,27 @@ struct IbTx {
   }
 };
 
-template <bool HBD>
+template <bool HBD, bool ONE>
 __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa, const Range *rg, uint32_t nf,
-                                                             const IntraBlockItem *items0) {
+                                                             const IntraBlockItem *items0, uint8_t *cp0,
+                                                             uint8_t *cp1, uint8_t *cp2, const int16_t *rp0,
+                                                             const int16_t *rp1, const int16_t *rp2,
+                                                             const vp9r_txb *txbs1) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   const int tid = threadIdx.x;
   const uint32_t fi = range_frame(rg, nf, blockIdx.x);
   const IntraArgs &a = fa[fi];
   const FrameGeom &g = a.g;
-  // a launch of one frame passes its items (the item loads with the frame's
-  // arguments instead of after them)
-  const IntraBlockItem it = items0 ? items0[blockIdx.x] : a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
+  // a launch of one frame (ONE) passes its items (the item loads with the
+  // frame's arguments instead of after them) and its planes, residual
+  // planes and transform block records (kernel arguments: known to be
+  // global memory)
+  const IntraBlockItem it = ONE ? items0[blockIdx.x] : a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
+  auto cur_plane = [&](int p) -> uint8_t * {
+    return ONE ? (p == 0 ? cp0 : p == 1 ? cp1 : cp2) : (uint8_t *)a.cur.p[p];
+  };
+  auto res_plane = [&](int p) -> const int16_t * { return ONE ? (p == 0 ? rp0 : p == 1 ? rp1 : rp2) : a.res[p]; };
+  const vp9r_txb *txbs = ONE ? txbs1 : a.txbs;
   const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
   vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
   IbTx *txs = (IbTx *)(ib_lds + ly.txs);
@@ -400,7 +410,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
   IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
   // the transform block records, all in LDS (at most ib_max_txbs; the host
   // checks each plane's grid, which bounds the count)
-  for (uint32_t k = tid; k < it.count; k += kIbThreads) stx[k] = a.txbs[it.first + k];
+  for (uint32_t k = tid; k < it.count; k += kIbThreads) stx[k] = txbs[it.first + k];
   auto txb_at = [&](uint32_t k) -> vp9r_txb { return stx[k]; };
   const int mi_size = it.mi_size;
   const bool avail_u = (it.flags & 8) != 0, avail_l = (it.flags & 16) != 0;
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
       const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
       if (local >= R.n) continue;
-      const P *pl = (const P *)a.cur.p[p];
+      const P *pl = (const P *)cur_plane(p);
       const size_t stride = a.cur.stride[p];
       if (local < R.nvec) {
         const int r = local >> R.lq, q = local & ((1 << R.lq) - 1), py = R.y0 - 1 + r;
@@ -457,7 +467,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
       if (local >= nr[p]) continue;
       const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int r = local >> R.lq, q = local & ((1 << R.lq) - 1);
-      const int16_t *src = a.res[p] + (size_t)(R.y0 + r) * a.cur.stride[p] + R.x0 + q * R.V;
+      const int16_t *src = res_plane(p) + (size_t)(R.y0 + r) * a.cur.stride[p] + R.x0 + q * R.V;
       if (R.V == 8)
         rv[i] = *(const uint4 *)src;
       else
@@ -626,7 +636,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
     const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
     const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
     const size_t stride = a.cur.stride[p];
-    P *pl = (P *)a.cur.p[p];
+    P *pl = (P *)cur_plane(p);
     for (int k = tid; k < (R.BH << R.lq); k += kIbThreads) {
       const int r = k >> R.lq, q = k & ((1 << R.lq) - 1);
       const uint16_t *src = tile + (r + 1) * tp + 1 + q * R.V;
@@ -660,23 +670,35 @@ void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n
 }
 
 void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-                         uint32_t lds_bytes, const IntraBlockItem *items0) {
+                         uint32_t lds_bytes, const IntraBlockItem *items0, const IntraArgs *one) {
   if (!wgs) return;
   // dynamic LDS above 64 KB (4:4:4 at high bit depth) needs the attribute
-  static bool attr_set[2] = {false, false};
-  if (!attr_set[hbd]) {
-    if (hbd)
-      (void)hipFuncSetAttribute((const void *)k_intra_blocks<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                160 * 1024);
+  static bool attr_set[2][2] = {{false, false}, {false, false}};
+  auto go = [&](auto kernel, bool h, bool o) {
+    if (!attr_set[h][o]) {
+      (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr_set[h][o] = true;
+    }
+    if (one)
+      hipLaunchKernelGGL(kernel, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0,
+                         (uint8_t *)one->cur.p[0], (uint8_t *)one->cur.p[1], (uint8_t *)one->cur.p[2], one->res[0],
+                         one->res[1], one->res[2], one->txbs);
     else
-      (void)hipFuncSetAttribute((const void *)k_intra_blocks<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                160 * 1024);
-    attr_set[hbd] = true;
+      hipLaunchKernelGGL(kernel, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0, (uint8_t *)nullptr,
+                         (uint8_t *)nullptr, (uint8_t *)nullptr, (const int16_t *)nullptr, (const int16_t *)nullptr,
+                         (const int16_t *)nullptr, (const vp9r_txb *)nullptr);
+  };
+  if (one) {
+    if (hbd)
+      go(k_intra_blocks<true, true>, true, true);
+    else
+      go(k_intra_blocks<false, true>, false, true);
+  } else {
+    if (hbd)
+      go(k_intra_blocks<true, false>, true, false);
+    else
+      go(k_intra_blocks<false, false>, false, false);
   }
-  if (hbd)
-    hipLaunchKernelGGL(k_intra_blocks<true>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0);
-  else
-    hipLaunchKernelGGL(k_intra_blocks<false>, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0);
 }
 
 // Per-kernel seam: one intra::predict call (intra.rs:19-29) on host-given

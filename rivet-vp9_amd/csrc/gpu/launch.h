@@ -213,10 +213,12 @@ void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int tx_size,
                       bool hbd);
 void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
-// items0: for a launch of one frame, that frame's items of the level (else
-// null: each workgroup finds its item through the range table)
+// items0, one: for a launch of one frame, that frame's items of the level
+// and (a host copy of) its arguments, whose pointers become kernel
+// arguments (else null: each workgroup finds its frame and item through the
+// range table)
 void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-                         uint32_t lds_bytes, const IntraBlockItem *items0);
+                         uint32_t lds_bytes, const IntraBlockItem *items0, const IntraArgs *one);
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs);
 // one: for a launch of one frame, lf_one of its arguments and the level
 // (else p[0] null: each workgroup finds its frame through the range table)

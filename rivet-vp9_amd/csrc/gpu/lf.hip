@@ -271,7 +271,7 @@ DEV void lf_region_load(const P *pl, uint32_t stride, const LfItem &it, LfRegion
   for (int i = 0; i < N; i++) {
     int r, q;
     if (lf_vec<PART>(it, threadIdx.x + 64 * i, r, q))
-      rg.v[i] = *(const V *)(pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q);
+      rg.v[i] = gld((const V *)(pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q));
   }
 }
 
@@ -315,9 +315,9 @@ DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_
       uint2 w;
       w.x = (l[0] & 0xff) | (l[0] >> 8 & 0xff00) | (l[1] & 0xff) << 16 | (l[1] >> 16) << 24;
       w.y = (l[2] & 0xff) | (l[2] >> 8 & 0xff00) | (l[3] & 0xff) << 16 | (l[3] >> 16) << 24;
-      *(uint2 *)g = w;
+      gst((uint2 *)g, w);
     } else {
-      *(uint4 *)g = make_uint4(l[0], l[1], l[2], l[3]);
+      gst((uint4 *)g, make_uint4(l[0], l[1], l[2], l[3]));
     }
   }
 }
@@ -386,18 +386,19 @@ DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, 
 
 // ---- one wavefront level per launch (decisions from k_lf_mask)
 
-template <bool HBD>
+template <bool HBD, bool ONE>
 __global__ __launch_bounds__(64) void k_lf(uint8_t *p0, uint8_t *p1, uint8_t *p2, uint32_t s0, uint32_t s1,
                                            uint32_t s2, uint32_t *dq, uint32_t geo, uint32_t t1, const LfArgs *fa,
                                            const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ LfShared sh;
   const int lane = threadIdx.x;
-  // a launch of one frame passes what the workgroups need as arguments
-  // (preloaded, so the region loads need no load before them: LfOne); a
-  // batch finds each frame's through the range table
+  // a launch of one frame (ONE) passes what the workgroups need as
+  // arguments (preloaded, so the region loads need no load before them, and
+  // known to be global memory: LfOne); a batch finds each frame's through
+  // the range table
   int j = (int)blockIdx.x;
-  if (!p0) {
+  if (!ONE) {
     const uint32_t fi = range_frame(rg, nf, blockIdx.x);
     const LfOne o = lf_one(fa[fi], rg[fi].first);
     p0 = o.p[0];
@@ -440,9 +441,9 @@ __global__ __launch_bounds__(64) void k_lf(uint8_t *p0, uint8_t *p1, uint8_t *p2
   const uint4 *dv = (const uint4 *)(f.dq + (plane * 2 + 0) * n_sb * 256);
   const uint4 *dh = (const uint4 *)(f.dq + (plane * 2 + 1) * n_sb * 256);
   uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0, d2 = d0;
-  if (first_v) d0 = dv[sb * 64 + lane];
-  if (!lone_v) d1 = dh[sb * 64 + lane];
-  if (right_v) d2 = dv[(sb + 1) * 64 + lane];
+  if (first_v) d0 = gld(dv + sb * 64 + lane);
+  if (!lone_v) d1 = gld(dh + sb * 64 + lane);
+  if (right_v) d2 = gld(dv + (sb + 1) * 64 + lane);
   LfRegion<P, 9> part0;
   LfRegion<P, 8> part1;
   lf_region_load<P, 0>(pl, stride, it, part0);
@@ -486,12 +487,21 @@ void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n,
 void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
                      const LfOne &one) {
   if (!wgs) return;
-  if (hbd)
-    hipLaunchKernelGGL(k_lf<true>, dim3(wgs), dim3(64), 0, s, one.p[0], one.p[1], one.p[2], one.stride[0],
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(wgs), dim3(64), 0, s, one.p[0], one.p[1], one.p[2], one.stride[0],
                        one.stride[1], one.stride[2], one.dq, one.geo, one.t, fa, r, n);
-  else
-    hipLaunchKernelGGL(k_lf<false>, dim3(wgs), dim3(64), 0, s, one.p[0], one.p[1], one.p[2], one.stride[0],
-                       one.stride[1], one.stride[2], one.dq, one.geo, one.t, fa, r, n);
+  };
+  if (one.p[0]) {
+    if (hbd)
+      go(k_lf<true, true>);
+    else
+      go(k_lf<false, true>);
+  } else {
+    if (hbd)
+      go(k_lf<true, false>);
+    else
+      go(k_lf<false, false>);
+  }
 }
 
 // Per-kernel seam: lf::filter_edges on global samples, one pass per

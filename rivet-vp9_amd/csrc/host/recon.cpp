@@ -1152,7 +1152,8 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
           vp9g::launch_intra_blocks(
               s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd, ib_lds,
               n == 1 ? (const vp9g::IntraBlockItem *)(f[0]->dv + f[0]->p->o_items) + f[0]->p->item_level_start[l]
-                     : nullptr);
+                     : nullptr,
+              n == 1 ? (const vp9g::IntraArgs *)(st.data() + o_intra) : nullptr);
         if (mark[l]) HIP_TRY(hipEventRecord(g->intra_events[l], s), "hipEventRecord");
       }
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
