@@ -44,12 +44,13 @@ GPU = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packe
        "-munsafe-fp-atomics", "-mllvm", "-amdgpu-kernarg-preload-count=16"]
 
 
-# Per-file flags.  lf.hip: the loop filter keeps a 16-sample window of each
+# Per-file flags.  lf.hip and intra.hip (whose k_wave inlines it): the loop filter keeps a 16-sample window of each
 # line in a register array that the filters update in place under per-lane
 # branches; promoted to one vector register, every partial update costs a
 # copy of the whole window (16 moves per edge), so the promotion is off.
 FILE_FLAGS = {
     "csrc/gpu/lf.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
+    "csrc/gpu/intra.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],  # k_wave runs the loop filter too
 }
 
 

@@ -16,6 +16,7 @@
 
 #include "itx_dev.h"
 #include "launch.h"
+#include "lf_dev.h"
 
 namespace vp9g {
 
@@ -382,22 +383,21 @@ struct IbTx {
   }
 };
 
+// Intra block wg of the launch (kIbThreads lanes).
 template <bool HBD, bool ONE>
-__global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa, const Range *rg, uint32_t nf,
-                                                             const IntraBlockItem *items0, uint8_t *cp0,
-                                                             uint8_t *cp1, uint8_t *cp2, const int16_t *rp0,
-                                                             const int16_t *rp1, const int16_t *rp2,
-                                                             const vp9r_txb *txbs1) {
+DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const IntraBlockItem *items0, uint8_t *cp0,
+                     uint8_t *cp1, uint8_t *cp2, const int16_t *rp0, const int16_t *rp1, const int16_t *rp2,
+                     const vp9r_txb *txbs1, uint32_t wg) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   const int tid = threadIdx.x;
-  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const uint32_t fi = range_frame(rg, nf, wg);
   const IntraArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   // a launch of one frame (ONE) passes its items (the item loads with the
   // frame's arguments instead of after them) and its planes, residual
   // planes and transform block records (kernel arguments: known to be
   // global memory)
-  const IntraBlockItem it = ONE ? items0[blockIdx.x] : a.items[rg[fi].first + blockIdx.x - rg[fi].wg];
+  const IntraBlockItem it = ONE ? items0[wg] : a.items[rg[fi].first + wg - rg[fi].wg];
   auto cur_plane = [&](int p) -> uint8_t * {
     return ONE ? (p == 0 ? cp0 : p == 1 ? cp1 : cp2) : (uint8_t *)a.cur.p[p];
   };
@@ -427,14 +427,14 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
   const IbRegion rg0 = ib_region(it, g, 0, bw8, bh8, np0 > 0), rg1 = ib_region(it, g, 1, bw8, bh8, np1 > 0),
                  rg2 = ib_region(it, g, 2, bw8, bh8, np2 > 0);
   constexpr int kIbResSlots = 6;
-  const int nr[3] = {rg0.n ? rg0.BH << rg0.lq : 0, rg1.n ? rg1.BH << rg1.lq : 0, rg2.n ? rg2.BH << rg2.lq : 0};
+  const int nr0 = rg0.n ? rg0.BH << rg0.lq : 0, nr1 = rg1.n ? rg1.BH << rg1.lq : 0, nr2 = rg2.n ? rg2.BH << rg2.lq : 0;
   {
     uint4 v[kIbRegionSlots], rv[kIbResSlots];
 #pragma unroll
     for (int i = 0; i < kIbRegionSlots; i++) {
       const int k = tid + kIbThreads * i;
       const int p = k < rg0.n ? 0 : k < rg0.n + rg1.n ? 1 : 2;
-      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
       if (local >= R.n) continue;
       const P *pl = (const P *)cur_plane(p);
@@ -462,10 +462,10 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
 #pragma unroll
     for (int i = 0; i < kIbResSlots; i++) {
       const int k = tid + kIbThreads * i;
-      const int p = k < nr[0] ? 0 : k < nr[0] + nr[1] ? 1 : 2;
-      const int local = k - (p == 0 ? 0 : p == 1 ? nr[0] : nr[0] + nr[1]);
-      if (local >= nr[p]) continue;
-      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const int p = k < nr0 ? 0 : k < nr0 + nr1 ? 1 : 2;
+      const int local = k - (p == 0 ? 0 : p == 1 ? nr0 : nr0 + nr1);
+      if (local >= (p == 0 ? nr0 : p == 1 ? nr1 : nr2)) continue;
+      const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int r = local >> R.lq, q = local & ((1 << R.lq) - 1);
       const int16_t *src = res_plane(p) + (size_t)(R.y0 + r) * a.cur.stride[p] + R.x0 + q * R.V;
       if (R.V == 8)
@@ -476,10 +476,10 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
 #pragma unroll
     for (int i = 0; i < kIbResSlots; i++) {
       const int k = tid + kIbThreads * i;
-      const int p = k < nr[0] ? 0 : k < nr[0] + nr[1] ? 1 : 2;
-      const int local = k - (p == 0 ? 0 : p == 1 ? nr[0] : nr[0] + nr[1]);
-      if (local >= nr[p]) continue;
-      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const int p = k < nr0 ? 0 : k < nr0 + nr1 ? 1 : 2;
+      const int local = k - (p == 0 ? 0 : p == 1 ? nr0 : nr0 + nr1);
+      if (local >= (p == 0 ? nr0 : p == 1 ? nr1 : nr2)) continue;
+      const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int r = local >> R.lq, q = local & ((1 << R.lq) - 1), rp = 64 >> (p ? g.ss_x : 0);
       uint8_t *dst = ib_lds + (p == 0 ? ly.res[0] : p == 1 ? ly.res[1] : ly.res[2]) + (r * rp + q * R.V) * 2;
       if (R.V == 8)
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
     for (int i = 0; i < kIbRegionSlots; i++) {
       const int k = tid + kIbThreads * i;
       const int p = k < rg0.n ? 0 : k < rg0.n + rg1.n ? 1 : 2;
-      const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+      const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
       if (local >= R.n) continue;
       const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
         const uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          if (j >= R.V) break;
+          if (j >= R.V) continue;
           dst[j] = sizeof(P) == 1 ? (uint16_t)((w[j >> 2] >> (8 * (j & 3))) & 0xff)
                                   : (uint16_t)((w[j >> 1] >> (16 * (j & 1))) & 0xffff);
         }
@@ -567,24 +567,23 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
   }
   __syncthreads();
   // the wavefront over the three grids; the planes' grid shapes in registers
-  int lg[3], kk[3], gw[3], gh[3], gn[3];
-#pragma unroll
-  for (int p = 0; p < 3; p++) {
-    lg[p] = geo[p].log2;
-    kk[p] = geo[p].kk;
-    gw[p] = geo[p].W;
-    gh[p] = geo[p].H;
-    gn[p] = geo[p].nlv;
-  }
-  const int nlv = max(max(gn[0], gn[1]), gn[2]);
+  // (selected per plane, not indexed: an indexed private array is memory)
+  auto sel3 = [](int p, int a, int b, int c) { return p == 0 ? a : p == 1 ? b : c; };
+  const int lg0 = geo[0].log2, lg1 = geo[1].log2, lg2 = geo[2].log2;
+  const int kk0 = geo[0].kk, kk1 = geo[1].kk, kk2 = geo[2].kk;
+  const int gw0 = geo[0].W, gw1 = geo[1].W, gw2 = geo[2].W;
+  const int gh0 = geo[0].H, gh1 = geo[1].H, gh2 = geo[2].H;
+  const int gn0 = geo[0].nlv, gn1 = geo[1].nlv, gn2 = geo[2].nlv;
+  const int nlv = max(max(gn0, gn1), gn2);
   bool any_dc = (np0 && geo[0].mode_u == DC_PRED) || ((np1 + np2) && it.uv_mode == DC_PRED);
   if (np0 && geo[0].mode_u < 0)
     for (int q = 0; q < 4; q++) any_dc = any_dc || ((it.sub_modes >> (8 * q)) & 0xff) == DC_PRED;
   // level lv of plane p: rows r_lo .. of the grid, n_l blocks
   auto level = [&](int p, int lv, int &r_lo) -> int {
-    if (lv >= gn[p]) return 0;
-    r_lo = lv > gw[p] - 1 ? (lv - (gw[p] - 1) + kk[p] - 1) / kk[p] : 0;
-    const int r_hi = min(gh[p] - 1, lv / kk[p]);
+    const int W = sel3(p, gw0, gw1, gw2), H = sel3(p, gh0, gh1, gh2), k = sel3(p, kk0, kk1, kk2);
+    if (lv >= sel3(p, gn0, gn1, gn2)) return 0;
+    r_lo = lv > W - 1 ? (lv - (W - 1) + k - 1) / k : 0;
+    const int r_hi = min(H - 1, lv / k);
     return max(r_hi - r_lo + 1, 0);
   };
   for (int lv = 0; lv < nlv; lv++) {
@@ -594,11 +593,11 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
       if (tid < n0 + n1 + n2) {
         const int p = tid < n0 ? 0 : tid < n0 + n1 ? 1 : 2;
         const int jt = tid - (p == 0 ? 0 : p == 1 ? n0 : n0 + n1);
-        const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - kk[p] * r;
+        const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - sel3(p, kk0, kk1, kk2) * r;
         const IbTx d = txs[p * 256 + r * 16 + c];
         if (d.valid() && d.mode() == DC_PRED) {
           int v;
-          switch (lg[p]) {  // every edge read in flight at once
+          switch (sel3(p, lg0, lg1, lg2)) {  // every edge read in flight at once
             case 2: v = d.dc<2>(bd); break;
             case 3: v = d.dc<3>(bd); break;
             case 4: v = d.dc<4>(bd); break;
@@ -609,12 +608,12 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
       }
       __syncthreads();
     }
-    const int i0 = n0 << (2 * lg[0]), i1 = n1 << (2 * lg[1]), i2 = n2 << (2 * lg[2]);
+    const int i0 = n0 << (2 * lg0), i1 = n1 << (2 * lg1), i2 = n2 << (2 * lg2);
     for (int item = tid; item < i0 + i1 + i2; item += kIbThreads) {
       const int p = item < i0 ? 0 : item < i0 + i1 ? 1 : 2;
       const int local = item - (p == 0 ? 0 : p == 1 ? i0 : i0 + i1);
-      const int l2 = lg[p], jt = local >> (2 * l2), s = local & ((1 << (2 * l2)) - 1);
-      const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - kk[p] * r;
+      const int l2 = sel3(p, lg0, lg1, lg2), jt = local >> (2 * l2), s = local & ((1 << (2 * l2)) - 1);
+      const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - sel3(p, kk0, kk1, kk2) * r;
       const IbTx d = txs[p * 256 + r * 16 + c];
       if (!d.valid()) continue;
       const int mode = d.mode();
@@ -631,7 +630,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
   // the blocks back to the frame, in vectors of V samples
 #pragma unroll
   for (int p = 0; p < 3; p++) {
-    const IbRegion &R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+    const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
     if (!R.n) continue;
     const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
     const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
@@ -644,7 +643,7 @@ __global__ __launch_bounds__(kIbThreads) void k_intra_blocks(const IntraArgs *fa
       uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int j = 0; j < 8; j++) {
-        if (j >= R.V) break;
+        if (j >= R.V) continue;
         if (sizeof(P) == 1)
           w[j >> 2] |= (uint32_t)src[j] << (8 * (j & 3));
         else
@@ -669,35 +668,85 @@ void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n
     hipLaunchKernelGGL(k_intra<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
 }
 
-void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-                         uint32_t lds_bytes, const IntraBlockItem *items0, const IntraArgs *one) {
-  if (!wgs) return;
-  // dynamic LDS above 64 KB (4:4:4 at high bit depth) needs the attribute
+// ---- the fused wavefront launch (k_wave)
+//
+// One launch holds one intra dependency level and one loop filter level,
+// both on the kernel stream: workgroups below n_iw are intra blocks of the
+// level (intra_block), the rest are loop filter work items of the level
+// (lf_unit: waves 0..2 filter the three planes, wave 3 only joins the
+// barriers).  The host places loop filter level j in a launch after the
+// intra levels it needs (lf_need) and after level j - 1, so the two chains
+// overlap without cross-stream events.  A launch of one frame (ONE) passes
+// the frame's planes, decisions, level, items, residual planes and records
+// as arguments (the first 16 dwords preloaded, all known to be global
+// memory); a batch goes through the range tables of both parts.
+template <bool HBD, bool ONE>
+__global__ __launch_bounds__(kIbThreads) void k_wave(uint32_t n_iw, uint32_t geo, uint32_t t1, uint8_t *p0,
+                                                     uint8_t *p1, uint8_t *p2, uint32_t *dq, uint32_t s0, uint32_t s1,
+                                                     uint32_t s2, const IntraBlockItem *items0, const int16_t *rp0,
+                                                     const int16_t *rp1, const int16_t *rp2, const vp9r_txb *txbs1,
+                                                     const IntraArgs *fa_i, const Range *rg_i, const LfArgs *fa_l,
+                                                     const Range *rg_l, uint32_t nf) {
+  if (blockIdx.x < n_iw) {
+    intra_block<HBD, ONE>(fa_i, rg_i, nf, items0, p0, p1, p2, rp0, rp1, rp2, txbs1, blockIdx.x);
+    return;
+  }
+  const uint32_t w = blockIdx.x - n_iw;
+  // the wave's plane, uniform across the wave
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), plane = wave < 3 ? wave : 0;
+  LfShared &sh = ((LfShared *)ib_lds)[plane];
+  const int lane = (int)(threadIdx.x & 63);
+  if (!ONE) {
+    const uint32_t fi = range_frame(rg_l, nf, w);
+    const LfArgs &a = fa_l[fi];
+    const LfOne o = lf_one(a, rg_l[fi].first);
+    lf_unit<HBD>((uint8_t *)a.cur.p[plane], a.cur.stride[plane], o.dq, o.geo, o.t, (int)(w - rg_l[fi].wg), plane,
+                 wave < 3, lane, sh);
+    return;
+  }
+  lf_unit<HBD>(plane == 0 ? p0 : plane == 1 ? p1 : p2, plane == 0 ? s0 : plane == 1 ? s1 : s2, dq, geo, t1, (int)w,
+               plane, wave < 3, lane, sh);
+}
+
+uint32_t wave_lds_bytes(int ss_x, int ss_y) {
+  return max(ib_layout(ss_x, ss_y).bytes, (uint32_t)(3 * sizeof(LfShared)));
+}
+
+void launch_wave(hipStream_t s, const WaveLaunch &w, bool hbd) {
+  if (!w.n_iw && !w.n_lw) return;
+  // dynamic LDS above 64 KB needs the attribute
   static bool attr_set[2][2] = {{false, false}, {false, false}};
-  auto go = [&](auto kernel, bool h, bool o) {
-    if (!attr_set[h][o]) {
+  const bool one = w.one_i || w.one_l;
+  auto go = [&](auto kernel, bool h) {
+    if (!attr_set[h][one]) {
       (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      attr_set[h][o] = true;
+      attr_set[h][one] = true;
     }
-    if (one)
-      hipLaunchKernelGGL(kernel, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0,
-                         (uint8_t *)one->cur.p[0], (uint8_t *)one->cur.p[1], (uint8_t *)one->cur.p[2], one->res[0],
-                         one->res[1], one->res[2], one->txbs);
-    else
-      hipLaunchKernelGGL(kernel, dim3(wgs), dim3(kIbThreads), lds_bytes, s, fa, r, n, items0, (uint8_t *)nullptr,
-                         (uint8_t *)nullptr, (uint8_t *)nullptr, (const int16_t *)nullptr, (const int16_t *)nullptr,
-                         (const int16_t *)nullptr, (const vp9r_txb *)nullptr);
+    LfOne lo = w.one_l ? lf_one(*w.one_l, w.lf_level) : LfOne{};
+    uint8_t *p[3] = {lo.p[0], lo.p[1], lo.p[2]};
+    const int16_t *rp[3] = {nullptr, nullptr, nullptr};
+    const vp9r_txb *txbs = nullptr;
+    if (w.one_i) {
+      for (int k = 0; k < 3; k++) {
+        p[k] = (uint8_t *)w.one_i->cur.p[k];
+        rp[k] = w.one_i->res[k];
+      }
+      txbs = w.one_i->txbs;
+    }
+    hipLaunchKernelGGL(kernel, dim3(w.n_iw + w.n_lw), dim3(kIbThreads), w.lds_bytes, s, w.n_iw, lo.geo, lo.t, p[0],
+                       p[1], p[2], lo.dq, lo.stride[0], lo.stride[1], lo.stride[2], w.items0, rp[0], rp[1], rp[2],
+                       txbs, w.fa_i, w.rg_i, w.fa_l, w.rg_l, w.nf);
   };
   if (one) {
     if (hbd)
-      go(k_intra_blocks<true, true>, true, true);
+      go(k_wave<true, true>, true);
     else
-      go(k_intra_blocks<false, true>, false, true);
+      go(k_wave<false, true>, false);
   } else {
     if (hbd)
-      go(k_intra_blocks<true, false>, true, false);
+      go(k_wave<true, false>, true);
     else
-      go(k_intra_blocks<false, false>, false, false);
+      go(k_wave<false, false>, false);
   }
 }
 

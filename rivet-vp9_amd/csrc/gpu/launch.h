@@ -213,12 +213,25 @@ void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int tx_size,
                       bool hbd);
 void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
-// items0, one: for a launch of one frame, that frame's items of the level
-// and (a host copy of) its arguments, whose pointers become kernel
-// arguments (else null: each workgroup finds its frame and item through the
-// range table)
-void launch_intra_blocks(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-                         uint32_t lds_bytes, const IntraBlockItem *items0, const IntraArgs *one);
+// One fused wavefront launch (k_wave): n_iw intra block workgroups (ranges
+// rg_i over fa_i) and n_lw loop filter work items (ranges rg_l over fa_l,
+// first = the frame's level).  For a launch of one frame, one_i / one_l are
+// host copies of its arguments (whose pointers become kernel arguments),
+// items0 its items of the intra level and lf_level its loop filter level;
+// else they are null and every workgroup goes through the ranges.
+struct WaveLaunch {
+  uint32_t n_iw = 0, n_lw = 0, nf = 0, lds_bytes = 0, lf_level = 0;
+  const IntraArgs *fa_i = nullptr;
+  const Range *rg_i = nullptr;
+  const LfArgs *fa_l = nullptr;
+  const Range *rg_l = nullptr;
+  const IntraArgs *one_i = nullptr;
+  const LfArgs *one_l = nullptr;
+  const IntraBlockItem *items0 = nullptr;
+};
+void launch_wave(hipStream_t s, const WaveLaunch &w, bool hbd);
+// dynamic LDS of a k_wave workgroup for a frame of subsampling (ss_x, ss_y)
+uint32_t wave_lds_bytes(int ss_x, int ss_y);
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs);
 // one: for a launch of one frame, lf_one of its arguments and the level
 // (else p[0] null: each workgroup finds its frame through the range table)

@@ -25,94 +25,9 @@
 #include "dev_common.h"
 #include "launch.h"
 #include "vp9_consts.h"
+#include "lf_dev.h"
 
 namespace vp9g {
-
-constexpr int BP = 138;  // LDS pitch of a unit's region (8 + 128 columns + slack, odd in dwords)
-
-// |a - b| of two samples (< 2^16) in one instruction (v_sad_u16).
-DEV uint32_t adiff(int a, int b) { return __builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }
-DEV int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // lo <= hi
-
-// The filters of one edge position, on samples reached through `s`: s(k)
-// is the sample k positions across the edge from the first q sample
-// (k = -8 .. 7), a reference into LDS or into a register array.
-
-// The wide filters (8.8.5.3): output i in [-n, n) is
-// Round2(s[i] + T[i], LOG2) with the window sum
-// T[i] = sum_{j=-n..n} s[clamp(i + j, -n - 1, n)] and n = 2^(LOG2-1) - 1.
-// T slides: T[i + 1] = T[i] - s[clamp(i - n)] + s[clamp(i + n + 1)].  Every
-// T is formed from the inputs before any output is written, so each output
-// replaces its input in place.
-template <int LOG2, class S>
-DEV void lf_wide(S s) {
-  constexpr int n = (1 << (LOG2 - 1)) - 1;
-  auto at = [&](int k) -> int { return s(k < -(n + 1) ? -(n + 1) : (k > n ? n : k)); };
-  int t[2 * n];
-  int tt = 0;
-#pragma unroll
-  for (int j = -n; j <= n; j++) tt += at(-n + j);
-#pragma unroll
-  for (int i = -n; i < n; i++) {
-    t[i + n] = tt;
-    tt += at(i + n + 1) - at(i - n);
-  }
-#pragma unroll
-  for (int i = -n; i < n; i++) s(i) = (s(i) + t[i + n] + (1 << (LOG2 - 1))) >> LOG2;
-}
-
-// lf::filter (src/dsp/lf.rs:24-86): the mask, then the narrow filter, or
-// the 8- or 16-wide filter where the samples are flat.  The mask and
-// flatness tests are maxima of absolute differences against one limit
-// each (the reference's chains of comparisons, without branches).  The
-// three filters are exclusive per lane and each rewrites its samples in
-// place: three independent conditional blocks, so that no sample is copied
-// to merge the paths.
-template <class S>
-DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
-  const int q0 = s(0), q1 = s(1), q2 = s(2), q3 = s(3);
-  const int p0 = s(-1), p1 = s(-2), p2 = s(-3), p3 = s(-4);
-  const int sh = bd - 8;
-  const uint32_t d10 = adiff(p1, p0), e10 = adiff(q1, q0);
-  const uint32_t m = max(max(max(adiff(p3, p2), adiff(p2, p1)), d10), max(max(e10, adiff(q2, q1)), adiff(q3, q2)));
-  const uint32_t edge = adiff(p0, q0) * 2 + (adiff(p1, q1) >> 1);
-  const bool on = fs >= 0 && m <= (uint32_t)(limit << sh) && edge <= (uint32_t)(blimit << sh);
-  const bool hev = max(d10, e10) > (uint32_t)(thresh << sh);
-  const uint32_t one = 1u << sh;
-  const bool flat = on && fs >= 1 &&
-                    max(max(max(d10, e10), max(adiff(p2, p0), adiff(q2, q0))), max(adiff(p3, p0), adiff(q3, q0))) <= one;
-  bool flat2 = false;
-  if (flat && fs >= 2) {
-    uint32_t f2 = 0;
-#pragma unroll
-    for (int k = 4; k < 8; k++) f2 = max(f2, max(adiff(s(-(k + 1)), p0), adiff(s(k), q0)));
-    flat2 = f2 <= one;
-  }
-  if (on && !flat) {
-    // narrow filter (8.8.5.2)
-    const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1, off = 0x80 << sh;
-    const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
-    int f = hev ? clampi(ps1 - qs1, lo, hi) : 0;
-    f = clampi(f + 3 * (qs0 - ps0), lo, hi);
-    const int f1 = clampi(f + 4, lo, hi) >> 3, f2 = clampi(f + 3, lo, hi) >> 3;
-    s(0) = clampi(qs0 - f1, lo, hi) + off;
-    s(-1) = clampi(ps0 + f2, lo, hi) + off;
-    if (!hev) {
-      const int g = (f1 + 1) >> 1;
-      s(1) = clampi(qs1 - g, lo, hi) + off;
-      s(-2) = clampi(ps1 + g, lo, hi) + off;
-    }
-  }
-  // wide filter (8.8.5.3): log2 3 (8 taps) or 4 (16 taps)
-  if (flat && !flat2) lf_wide<3>(s);
-  if (flat2) lf_wide<4>(s);
-}
-
-// lf::filter on an LDS buffer: across the edge whose first q sample is
-// b[pos], `step` apart.
-DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit, int thresh, int bd) {
-  lf_apply([&](int k) -> uint16_t & { return b[pos + k * step]; }, fs, limit, blimit, thresh, bd);
-}
 
 // ---- decisions
 
@@ -201,282 +116,31 @@ __global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *
                          (uint32_t)a.p.thresh[lvl] << 24;
 }
 
-// ---- one work item of a wavefront level: the region in LDS
-//
-// A unit (r, c) of one plane is the horizontal pass of superblock (r, c)
-// followed by the vertical pass of superblock (r, c + 1): both passes touch
-// the rows of superblock row r (the horizontal pass also the 8 rows above),
-// the first the columns of superblock c, the second those of c + 1 and the 8
-// left of them.  The region staged in LDS is that L shape: the 8 rows above
-// over superblock c's columns, and superblock row r over both superblocks.
-// LDS row 8 + y holds plane row y0 + y; LDS column 8 + x holds plane column
-// x0 + x, so that a frame-edge vertical pass (superblock column 0) has an
-// 8-sample window margin on its left.  The rows past the region are slack
-// for lf_stream's reads past a line's end.
-
-struct __align__(16) LfShared {
-  uint16_t buf[80 * BP];
-  uint32_t dq[3][272];  // packed decisions of the item's passes (16 slack entries each)
-};
-
-// The region in parts of 8-sample vectors (nt = 1 << lnt vectors per
-// superblock row): part 0 is what the first vertical and the horizontal
-// pass read, the 8 rows above (8 nt vectors, none in superblock row 0 and
-// for a lone vertical pass) and then superblock c's sh rows; part 1 is
-// superblock c + 1's rows, which only the second vertical pass reads (it is
-// written to LDS after the horizontal pass, its loads in flight meanwhile);
-// part 2 is superblock c's last vector of each row, the 8 samples left of
-// superblock c + 1 that the second vertical pass rewrites (the rest of part
-// 0 goes back to the frame before that pass runs).
-struct LfItem {
-  int x0, y0, top, lnt, sh;
-};
-
-template <int PART>
-DEV bool lf_vec(const LfItem &it, int k, int &r, int &q) {
-  const int nt = 1 << it.lnt;
-  if (PART == 0) {
-    if (k < it.top) {
-      r = k >> it.lnt;
-      q = k & (nt - 1);
-      return true;
-    }
-    k -= it.top;
-    r = 8 + (k >> it.lnt);
-    q = k & (nt - 1);
-    return k < (it.sh << it.lnt);
-  }
-  if (PART == 1) {
-    r = 8 + (k >> it.lnt);
-    q = nt + (k & (nt - 1));
-    return k < (it.sh << it.lnt);
-  }
-  r = 8 + k;
-  q = nt - 1;
-  return k < it.sh;
-}
-
-// Part 0: at most 8 x 8 + 64 x 8 vectors; part 1: 64 x 8; part 2: 64.
-template <class P, int N>
-struct LfRegion {
-  typedef typename std::conditional<sizeof(P) == 1, uint2, uint4>::type V;  // 8 samples
-  V v[N];
-};
-
-// Issues every load of a part (into registers) at once.
-template <class P, int PART, int N>
-DEV void lf_region_load(const P *pl, uint32_t stride, const LfItem &it, LfRegion<P, N> &rg) {
-  typedef typename LfRegion<P, N>::V V;
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    int r, q;
-    if (lf_vec<PART>(it, threadIdx.x + 64 * i, r, q))
-      rg.v[i] = gld((const V *)(pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q));
-  }
-}
-
-// Samples as 16-bit values, two to a dword store (BP and the column offsets
-// are even).
-template <class P, int PART, int N>
-DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P, N> &rg, uint16_t *buf) {
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    int r, q;
-    if (!lf_vec<PART>(it, threadIdx.x + 64 * i, r, q)) continue;
-    uint32_t *l = (uint32_t *)(buf + r * BP + 8 + 8 * q);
-    if (sizeof(P) == 1) {
-      const uint2 w = *(const uint2 *)&rg.v[i];
-      l[0] = (w.x & 0xff) | (w.x & 0xff00) << 8;
-      l[1] = (w.x >> 16 & 0xff) | (w.x >> 8 & 0xff0000);
-      l[2] = (w.y & 0xff) | (w.y & 0xff00) << 8;
-      l[3] = (w.y >> 16 & 0xff) | (w.y >> 8 & 0xff0000);
-    } else {
-      const uint4 w = *(const uint4 *)&rg.v[i];
-      l[0] = w.x;
-      l[1] = w.y;
-      l[2] = w.z;
-      l[3] = w.w;
-    }
-  }
-}
-
-// Stores the vectors of a part; skip_last leaves out the last vector of
-// each superblock row (part 0 before the second vertical pass).
-template <class P, int PART, int N>
-DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_t *buf, bool skip_last) {
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    int r, q;
-    if (!lf_vec<PART>(it, threadIdx.x + 64 * i, r, q)) continue;
-    if (skip_last && r >= 8 && q == (1 << it.lnt) - 1) continue;
-    const uint32_t *l = (const uint32_t *)(buf + r * BP + 8 + 8 * q);
-    P *g = pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q;
-    if (sizeof(P) == 1) {
-      uint2 w;
-      w.x = (l[0] & 0xff) | (l[0] >> 8 & 0xff00) | (l[1] & 0xff) << 16 | (l[1] >> 16) << 24;
-      w.y = (l[2] & 0xff) | (l[2] >> 8 & 0xff00) | (l[3] & 0xff) << 16 | (l[3] >> 16) << 24;
-      gst((uint2 *)g, w);
-    } else {
-      gst((uint4 *)g, make_uint4(l[0], l[1], l[2], l[3]));
-    }
-  }
-}
-
-// ---- the edge filters of one pass, streamed through a register window
-//
-// The vertical pass gives each lane one row and slides a 16-sample register
-// window (p7 .. q7 of the current edge) along it: after edge e the 4 samples
-// that leave the window are final (no later edge of the row reaches them)
-// and go back to LDS, 4 new ones come in.  The horizontal pass does the same
-// down each column.
-
-// One pass: lane-owned lines of `len` samples, line i at buf + i * line,
-// samples `step` apart; edges at 8 + 4 e for e < n_edges (a multiple of 4).
-// dq[e * 16 + rn] packs the edge's decision: (filter size + 1) | limit << 8
-// | blimit << 16 | thresh << 24, 0 for no filter.  The window holds the 16
-// samples p7 .. q7 of the current edge, position x of the line in w[x & 15]:
-// with the edge loop unrolled by 4 every index is a constant, and sliding
-// the window is renaming.  The next edge's decision and the 4 samples
-// entering the window are fetched before the filter (the filter of edge e
-// writes at most up to position 4 e + 14).
-// Edge e = 4 m + K of lf_stream (K a constant: every window index is one).
-template <int K>
-DEV void lf_step(int (&w)[16], uint16_t *ln, int e, int step, int n_edges, const uint32_t *dq, int rn,
-                 uint32_t &d_next, int bd) {
-  // Reads past the line's end (positions >= len, at most 4 n_edges + 11)
-  // stay inside LfShared and never reach a result: they enter the window
-  // after the last edge that could read them (the last edge is 4 apart from
-  // the line's end, a narrow filter reading q0 .. q3 only) and are not
-  // written back.  Likewise the decision after the last edge.
-  const uint32_t d = d_next;
-  d_next = dq[(e + 1) * 16 + rn];
-  int nw[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) nw[i] = ln[(4 * e + 16 + i) * step];
-  if (d)
-    lf_apply([&](int kk) -> int & { return w[(4 * K + 8 + kk) & 15]; }, (int)(d & 0xff) - 1, (d >> 8) & 0xff,
-             (d >> 16) & 0xff, d >> 24, bd);
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    ln[(4 * e + i) * step] = (uint16_t)w[4 * K + i];
-    w[4 * K + i] = nw[i];
-  }
-}
-
-DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, const uint32_t *dq, int rn,
-                   int bd) {
-  int w[16];
-  uint16_t *ln = buf + line_off;
-#pragma unroll
-  for (int i = 0; i < 16; i++) w[i] = ln[i * step];
-  uint32_t d_next = dq[rn];
-#pragma unroll 1
-  for (int e = 0; e < n_edges; e += 4) {
-    lf_step<0>(w, ln, e, step, n_edges, dq, rn, d_next, bd);
-    lf_step<1>(w, ln, e + 1, step, n_edges, dq, rn, d_next, bd);
-    lf_step<2>(w, ln, e + 2, step, n_edges, dq, rn, d_next, bd);
-    lf_step<3>(w, ln, e + 3, step, n_edges, dq, rn, d_next, bd);
-  }
-  // the last window: positions 4 n_edges + i at w[i]
-  const int base = 4 * n_edges;
-#pragma unroll
-  for (int i = 0; i < 16; i++)
-    if (base + i < len) ln[(base + i) * step] = (uint16_t)w[i];
-}
-
 // ---- one wavefront level per launch (decisions from k_lf_mask)
 
 template <bool HBD, bool ONE>
 __global__ __launch_bounds__(64) void k_lf(uint8_t *p0, uint8_t *p1, uint8_t *p2, uint32_t s0, uint32_t s1,
                                            uint32_t s2, uint32_t *dq, uint32_t geo, uint32_t t1, const LfArgs *fa,
                                            const Range *rg, uint32_t nf) {
-  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ LfShared sh;
-  const int lane = threadIdx.x;
   // a launch of one frame (ONE) passes what the workgroups need as
   // arguments (preloaded, so the region loads need no load before them, and
   // known to be global memory: LfOne); a batch finds each frame's through
-  // the range table
+  // the range table.  Workgroup j: plane j % 3 of item j / 3.
   int j = (int)blockIdx.x;
   if (!ONE) {
     const uint32_t fi = range_frame(rg, nf, blockIdx.x);
-    const LfOne o = lf_one(fa[fi], rg[fi].first);
-    p0 = o.p[0];
-    p1 = o.p[1];
-    p2 = o.p[2];
-    s0 = o.stride[0];
-    s1 = o.stride[1];
-    s2 = o.stride[2];
-    dq = o.dq;
-    geo = o.geo;
-    t1 = o.t;
+    const LfArgs &a = fa[fi];
     j = (int)(blockIdx.x - rg[fi].wg);
+    const int plane = j % 3;
+    const LfOne o = lf_one(a, rg[fi].first);
+    lf_unit<HBD>((uint8_t *)a.cur.p[plane], a.cur.stride[plane], o.dq, o.geo, o.t, j / 3, plane, true,
+                 (int)threadIdx.x, sh);
+    return;
   }
-  const LfOne f = {{p0, p1, p2}, {s0, s1, s2}, dq, geo, t1};
-  // Level t: items j / 3 (plane j % 3) below n_units are the units (r, t - r)
-  // from the first row the level reaches; item n_units, when t + 1 < R, is
-  // the vertical pass of superblock (t + 1, 0), which no later item of row
-  // t + 1 includes (unit (0, 0) does its own).
-  const int t = (int)f.t;
-  const int C = (int)(f.geo & 0xfff), R = (int)((f.geo >> 12) & 0xfff);
-  const int r_lo = t >= C ? t - C + 1 : 0, n_units = min(R - 1, t) - r_lo + 1;
-  const int plane = j % 3, item = j / 3;
-  const bool lone_v = item >= n_units;
-  const int sb_row = lone_v ? t + 1 : r_lo + item, sb_col = lone_v ? 0 : t - sb_row;
-  const bool first_v = lone_v || (sb_row == 0 && sb_col == 0);
-  const bool right_v = !lone_v && sb_col + 1 < C;
-  const int sx = plane ? (int)((f.geo >> 24) & 1) : 0, sy = plane ? (int)((f.geo >> 25) & 1) : 0;
-  const int bit_depth = (int)(f.geo >> 26);
-  const int SW = 64 >> sx, SH = 64 >> sy, lnt = 3 - sx;
-  LfItem it;
-  it.x0 = sb_col * SW;
-  it.y0 = sb_row * SH;
-  it.top = lone_v || sb_row == 0 ? 0 : 8 << lnt;
-  it.lnt = lnt;
-  it.sh = SH;
-  const uint32_t stride = plane == 0 ? s0 : plane == 1 ? s1 : s2;
-  P *pl = (P *)(plane == 0 ? p0 : plane == 1 ? p1 : p2);
-  // every global load at once: the passes' decisions and the region
-  const size_t n_sb = (size_t)R * C, sb = (size_t)sb_row * C + sb_col;
-  const uint4 *dv = (const uint4 *)(f.dq + (plane * 2 + 0) * n_sb * 256);
-  const uint4 *dh = (const uint4 *)(f.dq + (plane * 2 + 1) * n_sb * 256);
-  uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0, d2 = d0;
-  if (first_v) d0 = gld(dv + sb * 64 + lane);
-  if (!lone_v) d1 = gld(dh + sb * 64 + lane);
-  if (right_v) d2 = gld(dv + (sb + 1) * 64 + lane);
-  LfRegion<P, 9> part0;
-  LfRegion<P, 8> part1;
-  lf_region_load<P, 0>(pl, stride, it, part0);
-  if (right_v) lf_region_load<P, 1>(pl, stride, it, part1);
-  ((uint4 *)sh.dq[0])[lane] = d0;
-  ((uint4 *)sh.dq[1])[lane] = d1;
-  ((uint4 *)sh.dq[2])[lane] = d2;
-  lf_region_to_lds<P, 0>(it, part0, sh.buf);
-  __syncthreads();
-  // pass 0: the vertical edges of superblock (r, c) (c = 0 only); pass 1 its
-  // horizontal edges; pass 2 the vertical edges of (r, c + 1).  One copy of
-  // the edge loop for every pass and geometry.
-#pragma unroll 1
-  for (int pass = 0; pass < 3; pass++) {
-    if (pass == 0 ? !first_v : pass == 1 ? lone_v : !right_v) continue;
-    if (pass == 2) {
-      // superblock c + 1 into LDS, superblock c's finished samples out
-      lf_region_to_lds<P, 1>(it, part1, sh.buf);
-      lf_region_store<P, 0, 9>(pl, stride, it, sh.buf, true);
-      __syncthreads();
-    }
-    const bool vert = pass != 1;
-    if (lane < (vert ? SH : SW))
-      lf_stream(sh.buf, vert ? (8 + lane) * BP + (pass == 2 ? SW : 0) : 8 + lane, vert ? 1 : BP,
-                (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, HBD ? bit_depth : 8);
-    __syncthreads();
-  }
-  if (right_v) {
-    lf_region_store<P, 1, 8>(pl, stride, it, sh.buf, false);
-    lf_region_store<P, 2, 1>(pl, stride, it, sh.buf, false);
-  } else {
-    lf_region_store<P, 0, 9>(pl, stride, it, sh.buf, false);
-  }
+  const int plane = j % 3;
+  lf_unit<HBD>(plane == 0 ? p0 : plane == 1 ? p1 : p2, plane == 0 ? s0 : plane == 1 ? s1 : s2, dq, geo, t1, j / 3,
+               plane, true, (int)threadIdx.x, sh);
 }
 
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs) {

@@ -997,6 +997,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     LaunchTable t_clear, t_mc, t_itx[4], t_mask;
     std::vector<LaunchTable> t_intra, t_lf;
     bool groups = false;
+    const bool fused = g->intra_blocks;
     if (recon) {
       t_clear = table([](int) { return 0u; },
                       [&](int i) { return 3 * vp9g::clear_margin_rows(f[i]->cur->planes, f[i]->p->geo); }, 1);
@@ -1013,6 +1014,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
         const auto &ls = groups ? f[i]->p->group_level_start : f[i]->p->item_level_start;
         if (ls.size() > 1) levels = std::max(levels, ls.size() - 1);
       }
+      if (fused) levels = 0;
       for (size_t l = 0; l < levels; l++) {
         t_intra.push_back(table(
             [&](int i) {
@@ -1026,6 +1028,54 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
             1));
       }
     }
+    // Fused wavefront launches (leveled intra blocks): launch k holds intra
+    // level k of every frame and, per frame, the loop filter level placed at
+    // k: level j goes after level j - 1 and after the intra levels it needs
+    // (lf_need), all on the kernel stream.  Each filter work item is one
+    // workgroup (three planes).
+    std::vector<LaunchTable> tw_i, tw_l;
+    std::vector<std::vector<int>> lf_lvl;  // [frame][launch]: its loop filter level there, or -1
+    size_t n_wave = 0, n_wave_intra = 0;
+    if (fused) {
+      std::vector<size_t> n_int(n), n_lf(n);
+      std::vector<std::vector<int>> lf_at(n);
+      for (int i = 0; i < n; i++) {
+        const Prepared &p = *f[i]->p;
+        n_int[i] = recon && p.item_level_start.size() > 1 ? p.item_level_start.size() - 1 : 0;
+        n_lf[i] = lf && p.lf_level_start.size() > 1 ? p.lf_level_start.size() - 1 : 0;
+        int prev = -1;
+        for (size_t j = 0; j < n_lf[i]; j++) {
+          const int need = recon && j < p.lf_need.size() ? p.lf_need[j] : -1;
+          prev = std::max(prev + 1, need + 1);
+          lf_at[i].push_back(prev);
+        }
+        n_wave_intra = std::max(n_wave_intra, n_int[i]);
+        n_wave = std::max(n_wave, std::max(n_int[i], n_lf[i] ? (size_t)lf_at[i].back() + 1 : 0));
+      }
+      for (int i = 0; i < n; i++) {
+        lf_lvl.emplace_back(n_wave, -1);
+        for (size_t j = 0; j < n_lf[i]; j++) lf_lvl[i][lf_at[i][j]] = (int)j;
+      }
+      for (size_t k = 0; k < n_wave; k++) {
+        tw_i.push_back(table(
+            [&](int i) { return k < n_int[i] ? f[i]->p->item_level_start[k] : 0u; },
+            [&](int i) {
+              const auto &ls = f[i]->p->item_level_start;
+              return k < n_int[i] ? ls[k + 1] - ls[k] : 0u;
+            },
+            1));
+        // with intra work: one k_wave workgroup per item; after the last
+        // intra level: k_lf, one workgroup per item and plane
+        const bool with_intra = k < n_wave_intra;
+        tw_l.push_back(table([&](int i) { return lf_lvl[i][k] >= 0 ? (uint32_t)lf_lvl[i][k] : 0u; },
+                             [&](int i) {
+                               const int j = lf_lvl[i][k];
+                               const auto &ls = f[i]->p->lf_level_start;
+                               return j >= 0 ? (ls[j + 1] - ls[j]) / (with_intra ? 3 : 1) : 0u;
+                             },
+                             1));
+      }
+    }
     if (lf) {
       t_mask = table([](int) { return 0u; },
                      [&](int i) {
@@ -1034,7 +1084,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
                      },
                      1);
       size_t levels = 0;
-      for (int i = 0; i < n; i++)
+      for (int i = 0; i < n && !fused; i++)
         if (f[i]->p->lf_level_start.size() > 1) levels = std::max(levels, f[i]->p->lf_level_start.size() - 1);
       for (size_t l = 0; l < levels; l++)
         t_lf.push_back(table([&](int) { return (uint32_t)l; },
@@ -1122,10 +1172,9 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       for (int k = 0; k < 4; k++)
         vp9g::launch_itx_inter(s, (const vp9g::ItxArgs *)(d + o_itx), R(t_itx[k]), n, t_itx[k].wgs, k, hbd);
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
+    }
+    if (recon && !fused) {
       HIP_TRY(hipEventRecord(g->ev_pre, s), "hipEventRecord");
-      uint32_t ib_lds = 0;
-      for (int i = 0; i < n; i++)
-        ib_lds = std::max(ib_lds, vp9g::ib_layout(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y).bytes);
       // the intra levels the loop filter waits for (an event after each)
       std::vector<char> mark(t_intra.size(), 0);
       if (lf && !t_intra.empty()) {
@@ -1146,19 +1195,48 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       }
       for (size_t l = 0; l < t_intra.size(); l++) {
         const LaunchTable &t = t_intra[l];
-        if (groups)
-          vp9g::launch_intra(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
-        else
-          vp9g::launch_intra_blocks(
-              s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd, ib_lds,
-              n == 1 ? (const vp9g::IntraBlockItem *)(f[0]->dv + f[0]->p->o_items) + f[0]->p->item_level_start[l]
-                     : nullptr,
-              n == 1 ? (const vp9g::IntraArgs *)(st.data() + o_intra) : nullptr);
+        vp9g::launch_intra(s, (const vp9g::IntraArgs *)(d + o_intra), R(t), n, t.wgs, hbd);
         if (mark[l]) HIP_TRY(hipEventRecord(g->intra_events[l], s), "hipEventRecord");
       }
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
     }
-    if (lf) {
+    if (fused) {
+      // everything on the kernel stream: the decisions, then the fused levels
+      if (!recon)
+        for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
+      if (lf) vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
+      if (recon && g->timing && !n_wave_intra) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
+      uint32_t lds = 0;
+      for (int i = 0; i < n; i++) lds = std::max(lds, vp9g::wave_lds_bytes(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y));
+      for (size_t k = 0; k < n_wave; k++) {
+        if (k >= n_wave_intra) {
+          vp9g::launch_lf_level(s, (const vp9g::LfArgs *)(d + o_lf), R(tw_l[k]), n, tw_l[k].wgs, hbd,
+                                n == 1 && lf_lvl[0][k] >= 0
+                                    ? vp9g::lf_one(((const vp9g::LfArgs *)(st.data() + o_lf))[0], (uint32_t)lf_lvl[0][k])
+                                    : vp9g::LfOne{});
+          continue;
+        }
+        vp9g::WaveLaunch w;
+        w.n_iw = tw_i[k].wgs;
+        w.n_lw = tw_l[k].wgs;
+        w.nf = (uint32_t)n;
+        w.lds_bytes = lds;
+        w.fa_i = (const vp9g::IntraArgs *)(d + o_intra);
+        w.rg_i = R(tw_i[k]);
+        w.fa_l = (const vp9g::LfArgs *)(d + o_lf);
+        w.rg_l = R(tw_l[k]);
+        if (n == 1) {
+          // host copies of the frame's arguments: pointers as kernel arguments
+          w.one_i = (const vp9g::IntraArgs *)(st.data() + o_intra);
+          w.one_l = (const vp9g::LfArgs *)(st.data() + o_lf);
+          if (w.n_iw)
+            w.items0 = (const vp9g::IntraBlockItem *)(f[0]->dv + f[0]->p->o_items) + f[0]->p->item_level_start[k];
+          w.lf_level = lf_lvl[0][k] >= 0 ? (uint32_t)lf_lvl[0][k] : 0u;
+        }
+        vp9g::launch_wave(s, w, hbd);
+        if (recon && g->timing && k + 1 == n_wave_intra) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
+      }
+    } else if (lf) {
       // The loop filter runs on its own stream: the decisions and levels
       // once the inter work is done, each level once its intra levels
       // (lf_need) are, overlapping the others.
