@@ -53,7 +53,10 @@ TESTS = os.path.join(ROOT, "tests")
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 STAGES = ("inter_pred", "inter_residual", "intra", "loop_filter")
-STAGE_KERNELS = ("k_mc", "k_itx_inter", "k_intra", "k_lf_mask + k_lf")
+# intra: the loop filter decisions and the fused launches holding an intra
+# level (the loop filter levels placed beside them run there too);
+# loop_filter: the filter levels after the last intra level
+STAGE_KERNELS = ("k_mc", "k_itx_inter", "k_lf_mask + k_wave", "k_lf")
 STAGE_BYTES = ("mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
 
 
@@ -463,18 +466,23 @@ def main():
     _log(f"bench: stage pass {staged_fps:.1f} frames/s")
 
     # per-stage totals over the timed frames
+    # (the loop filter's bytes split between the two by their share of its
+    # levels)
     work = {k: 0 for k in STAGE_BYTES}
     launches = [0, 0, 0, 0]
     for i in timed_idx:
         st = frames[i]["stats"]
         if st is None:
             continue
-        for k in STAGE_BYTES:
-            work[k] += st[k]
+        tail = st["lf_tail_launches"] / st["lf_levels"] if st["lf_levels"] else 0.0
+        work["mc_bytes"] += st["mc_bytes"]
+        work["itx_bytes"] += st["itx_bytes"]
+        work["intra_bytes"] += st["intra_bytes"] + (1.0 - tail) * st["lf_bytes"]
+        work["lf_bytes"] += tail * st["lf_bytes"]
         launches[0] += 1 if st["mc_tiles"] else 0
-        launches[1] += sum(1 for k in ("itx4", "itx8", "itx16", "itx32") if st[k])
-        launches[2] += st["intra_levels"]
-        launches[3] += (st["lf_levels"] + 1) if st["lf_levels"] else 0
+        launches[1] += 1
+        launches[2] += st["wave_launches"] + (1 if st["lf_levels"] else 0)
+        launches[3] += st["lf_tail_launches"]
     stages = {}
     for s, name in enumerate(STAGES):
         ms = stage_ms[s]

@@ -200,6 +200,12 @@ int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *p,
  * block with coefficients), intra (edges + block + coefficients) and the loop
  * filter (2 x frame bytes), b = bytes per sample. */
 int vp9r_gpu_prepared_stats(const vp9r_prepared *p, uint64_t out[12]);
+/* The launch schedule of a prepared record run alone (vp9r_gpu_run): [0]
+ * fused wavefront launches holding an intra level (with the loop filter
+ * levels placed beside them), [1] loop filter launches after the last intra
+ * level, [2] the loop filter levels.  Measurement aid; no counterpart in the
+ * reference (its filter runs after reconstruction, decoder/mod.rs:355-358). */
+int vp9r_gpu_prepared_schedule(const vp9r_prepared *p, uint32_t out[3]);
 /* Writes the visible frame in the crate's Frame::packed layout (Y, U, V; u8
  * at 8 bits, little-endian u16 above; frame.rs:112-229) to `dst`, which
  * holds vp9r_packed_size() bytes.  Waits for the frame. */

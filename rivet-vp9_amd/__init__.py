@@ -179,6 +179,7 @@ def library():
     lib.vp9r_gpu_run.argtypes = [P, P, ctypes.c_void_p * 3, P]
     lib.vp9r_gpu_run_batch.argtypes = [P, u32, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V)]
     lib.vp9r_gpu_prepared_stats.argtypes = [P, ctypes.POINTER(u64)]
+    lib.vp9r_gpu_prepared_schedule.argtypes = [P, ctypes.POINTER(ctypes.c_uint32)]
     lib.vp9r_gpu_malloc.argtypes = [P, sz, PP]
     lib.vp9r_gpu_mfree.argtypes = [P, V]
     lib.vp9r_gpu_to_device.argtypes = [P, V, V, sz]
@@ -360,7 +361,11 @@ class Prepared:
         library().vp9r_gpu_prepared_stats(self._p, out)
         keys = ("mc_tiles", "itx4", "itx8", "itx16", "itx32", "intra_groups", "intra_levels", "lf_levels",
                 "mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
-        return dict(zip(keys, list(out)))
+        d = dict(zip(keys, list(out)))
+        sched = (ctypes.c_uint32 * 3)()
+        library().vp9r_gpu_prepared_schedule(self._p, sched)
+        d["wave_launches"], d["lf_tail_launches"] = sched[0], sched[1]
+        return d
 
 
 class DevBuffer:

@@ -10,22 +10,23 @@
 
 namespace vp9g {
 
-// Per frame of the batch, its range's workgroups take BPW consecutive
-// jobs each (count = the frame's jobs of this size).
+// Workgroup wg of size class n: per frame of the batch, its range's
+// workgroups take BPW consecutive jobs each (count = the frame's jobs of
+// this size); tiles: BPW * n0 * (n0 + 1) elements of LDS.
 template <int n, bool HBD>
-__global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range *rg, uint32_t nf) {
+DEV void itx_inter_wg(const ItxArgs *fa, const Range *rg, uint32_t nf, uint32_t wg,
+                      typename std::conditional<HBD, Lane64, Lane32>::type::T *tiles) {
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   constexpr int n0 = 1 << n, BPW = 64 / n0, S = n0 + 1;
-  __shared__ T tiles[BPW][n0 * S];
-  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const uint32_t fi = range_frame(rg, nf, wg);
   const ItxArgs &a = fa[fi];
   const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
-  const uint32_t local = (blockIdx.x - rg[fi].wg) * BPW + team;
+  const uint32_t local = (wg - rg[fi].wg) * BPW + team;
   const bool active = local < rg[fi].count;
   const uint32_t job = rg[fi].first + local;
-  T *tile = tiles[team];
+  T *tile = tiles + team * (n0 * S);
 #pragma unroll
   for (int k = 0; k < S; k++) tile[lane * S + k] = 0;
   __syncthreads();
@@ -61,6 +62,26 @@ __global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range
   itx_2d<L, n>(tile, active ? lane : n0, t.tx_type, lossless, dc_only, dc_v, dst, max);
 }
 
+// All four size classes in one launch (they are independent): workgroups
+// [0, b4) take 32x32 blocks, [b4, b8) 16x16, [b8, b16) 8x8, the rest 4x4
+// (the largest, slowest, first).  LDS: the largest class's tiles.
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range *r32, const Range *r16,
+                                                  const Range *r8, const Range *r4, uint32_t nf, uint32_t b4,
+                                                  uint32_t b8, uint32_t b16) {
+  typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
+  __shared__ typename L::T tiles[2 * 32 * 33];
+  const uint32_t w = blockIdx.x;
+  if (w < b4)
+    itx_inter_wg<5, HBD>(fa, r32, nf, w, tiles);
+  else if (w < b8)
+    itx_inter_wg<4, HBD>(fa, r16, nf, w - b4, tiles);
+  else if (w < b16)
+    itx_inter_wg<3, HBD>(fa, r8, nf, w - b8, tiles);
+  else
+    itx_inter_wg<2, HBD>(fa, r4, nf, w - b16, tiles);
+}
+
 // Per-kernel seam: dense dequantised coefficients per block.
 template <int n, bool HBD>
 __global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const uint32_t *coef_off,
@@ -94,23 +115,14 @@ __global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const ui
   itx_2d<L, n>(tile, active ? lane : n0, tt, lossless != 0, dc_only, dc_v, dst, (1 << bd) - 1);
 }
 
-template <int n>
-static void itx_inter_n(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t nf, uint32_t wgs, bool hbd) {
+void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4], uint32_t nf,
+                      const uint32_t wgs[4], bool hbd) {
+  const uint32_t b4 = wgs[3], b8 = b4 + wgs[2], b16 = b8 + wgs[1], total = b16 + wgs[0];
+  if (!total) return;
   if (hbd)
-    hipLaunchKernelGGL((k_itx_inter<n, true>), dim3(wgs), dim3(64), 0, s, fa, r, nf);
+    hipLaunchKernelGGL(k_itx_inter<true>, dim3(total), dim3(64), 0, s, fa, r[3], r[2], r[1], r[0], nf, b4, b8, b16);
   else
-    hipLaunchKernelGGL((k_itx_inter<n, false>), dim3(wgs), dim3(64), 0, s, fa, r, nf);
-}
-
-void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t nf, uint32_t wgs, int tx_size,
-                      bool hbd) {
-  if (!wgs) return;
-  switch (tx_size) {
-    case 0: itx_inter_n<2>(s, fa, r, nf, wgs, hbd); break;
-    case 1: itx_inter_n<3>(s, fa, r, nf, wgs, hbd); break;
-    case 2: itx_inter_n<4>(s, fa, r, nf, wgs, hbd); break;
-    default: itx_inter_n<5>(s, fa, r, nf, wgs, hbd); break;
-  }
+    hipLaunchKernelGGL(k_itx_inter<false>, dim3(total), dim3(64), 0, s, fa, r[3], r[2], r[1], r[0], nf, b4, b8, b16);
 }
 
 template <int n>

@@ -210,7 +210,8 @@ __host__ __device__ inline LfOne lf_one(const LfArgs &a, uint32_t t) {
 uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g);
 void launch_clear_margins(hipStream_t s, const ClearArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
-void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int tx_size,
+// r[k], wgs[k]: the ranges and workgroups of transform size k (4x4 .. 32x32)
+void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4], uint32_t n, const uint32_t wgs[4],
                       bool hbd);
 void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
 // One fused wavefront launch (k_wave): n_iw intra block workgroups (ranges

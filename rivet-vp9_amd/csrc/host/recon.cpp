@@ -944,6 +944,23 @@ LaunchTable add_ranges(std::vector<uint8_t> &buf, int n, const uint32_t *first, 
 
 }  // namespace
 
+// The fused wavefront schedule of one frame: its intra levels (launches 0
+// .. n_int - 1), and per loop filter level j the launch lf_at[j]: after
+// level j - 1 and after the intra levels it needs (lf_need).  Returns the
+// number of launches.
+size_t wave_schedule(const Prepared &p, bool recon, bool lf, size_t &n_int, std::vector<int> &lf_at) {
+  n_int = recon && p.item_level_start.size() > 1 ? p.item_level_start.size() - 1 : 0;
+  const size_t n_lf = lf && p.lf_level_start.size() > 1 ? p.lf_level_start.size() - 1 : 0;
+  lf_at.clear();
+  int prev = -1;
+  for (size_t j = 0; j < n_lf; j++) {
+    const int need = recon && j < p.lf_need.size() ? p.lf_need[j] : -1;
+    prev = std::max(prev + 1, need + 1);
+    lf_at.push_back(prev);
+  }
+  return std::max(n_int, n_lf ? (size_t)lf_at.back() + 1 : 0);
+}
+
 int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool lf) {
   hipStream_t s = g->stream;
   for (int cls = 0; cls < 2; cls++) {
@@ -1037,24 +1054,15 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     std::vector<std::vector<int>> lf_lvl;  // [frame][launch]: its loop filter level there, or -1
     size_t n_wave = 0, n_wave_intra = 0;
     if (fused) {
-      std::vector<size_t> n_int(n), n_lf(n);
+      std::vector<size_t> n_int(n);
       std::vector<std::vector<int>> lf_at(n);
       for (int i = 0; i < n; i++) {
-        const Prepared &p = *f[i]->p;
-        n_int[i] = recon && p.item_level_start.size() > 1 ? p.item_level_start.size() - 1 : 0;
-        n_lf[i] = lf && p.lf_level_start.size() > 1 ? p.lf_level_start.size() - 1 : 0;
-        int prev = -1;
-        for (size_t j = 0; j < n_lf[i]; j++) {
-          const int need = recon && j < p.lf_need.size() ? p.lf_need[j] : -1;
-          prev = std::max(prev + 1, need + 1);
-          lf_at[i].push_back(prev);
-        }
+        n_wave = std::max(n_wave, wave_schedule(*f[i]->p, recon, lf, n_int[i], lf_at[i]));
         n_wave_intra = std::max(n_wave_intra, n_int[i]);
-        n_wave = std::max(n_wave, std::max(n_int[i], n_lf[i] ? (size_t)lf_at[i].back() + 1 : 0));
       }
       for (int i = 0; i < n; i++) {
         lf_lvl.emplace_back(n_wave, -1);
-        for (size_t j = 0; j < n_lf[i]; j++) lf_lvl[i][lf_at[i][j]] = (int)j;
+        for (size_t j = 0; j < lf_at[i].size(); j++) lf_lvl[i][lf_at[i][j]] = (int)j;
       }
       for (size_t k = 0; k < n_wave; k++) {
         tw_i.push_back(table(
@@ -1169,8 +1177,11 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
       vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd);
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
-      for (int k = 0; k < 4; k++)
-        vp9g::launch_itx_inter(s, (const vp9g::ItxArgs *)(d + o_itx), R(t_itx[k]), n, t_itx[k].wgs, k, hbd);
+      {
+        const vp9g::Range *rk[4] = {R(t_itx[0]), R(t_itx[1]), R(t_itx[2]), R(t_itx[3])};
+        const uint32_t wk[4] = {t_itx[0].wgs, t_itx[1].wgs, t_itx[2].wgs, t_itx[3].wgs};
+        vp9g::launch_itx_inter(s, (const vp9g::ItxArgs *)(d + o_itx), rk, n, wk, hbd);
+      }
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[2], s), "hipEventRecord");
     }
     if (recon && !fused) {
@@ -1410,6 +1421,16 @@ int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *pr, 
     HIP_TRY(hipEventRecord(g->ev[0], g->stream), "hipEventRecord");
   }
   return launch_batch(g, bf.data(), (int)n, true, true);
+}
+
+int vp9r_gpu_prepared_schedule(const vp9r_prepared *pr, uint32_t out[3]) {
+  size_t n_int = 0;
+  std::vector<int> lf_at;
+  const size_t n_wave = vp9r::wave_schedule(pr->p, true, pr->p.hdr.lf_level != 0, n_int, lf_at);
+  out[0] = (uint32_t)n_int;
+  out[1] = (uint32_t)(n_wave - n_int);
+  out[2] = (uint32_t)lf_at.size();
+  return VP9R_OK;
 }
 
 int vp9r_gpu_prepared_stats(const vp9r_prepared *pr, uint64_t out[12]) {
