@@ -26,6 +26,10 @@ times in one batch (every launch covers all 64 copies), per-stage GB/s of
 algorithmic bytes against the 8 TB/s HBM peak: the kernels with enough work
 to fill the GPU.
 
+4K 10-bit ("uhd_10bit"): the 3840x2160 10-bit 4:2:0 stream of 48 frames from
+seed 4 whose every 12th frame is coded at 1920x1080 (2:1 and 1:2 scaled
+references), frames/s from resident records and its MD5s against the oracle.
+
 Serving (in the same JSON line, "serving"): 32 streams from seed 5 arriving
 Poisson at 4/s, mixed sizes (640x360 / 1280x720 / 1920x1080) and formats
 (8-bit 4:2:0, 10-bit 4:2:0, 12-bit 4:4:4), 90-300 frames each, served by the
@@ -37,6 +41,7 @@ oracle's decode of its stream, which, one process per stream on all host
 cores, is also the serving CPU baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu] [--no-serve]
+                    [--no-batched] [--no-uhd] [--no-extras]
 """
 import argparse
 import ctypes
@@ -149,6 +154,58 @@ def cpu_baseline(frames_bytes, budget_s):
         best = fps if best is None else max(best, fps)
         md5s = md5_pass
     return best, md5s, passes, spent
+
+
+# ---- the 4K 10-bit stream with scaled references
+
+def uhd_bench(vp, gpu, check, passes=3):
+    """The 3840x2160 10-bit 4:2:0 profile-2 stream of 48 frames from seed 4
+    (moving rectangles at 10-bit range, 4 tile columns, q 80; every 12th
+    frame coded at 1920x1080, so its neighbours predict through a 2:1 or 1:2
+    scaled reference): frames/s of vp9r_gpu_run from resident records over
+    `passes` passes of the stream, and its shown frames' MD5s against the
+    oracle's decode (check)."""
+    if TESTS not in sys.path:
+        sys.path.insert(0, TESTS)
+    import streamgen
+    t0 = time.perf_counter()
+    frames_bytes = streamgen.generate(**streamgen.CONFIG4)
+    parser = vp.Parser()
+    frames = []
+    for data in frames_bytes:
+        rec = parser.parse(data)
+        hdr = vp.FrameHdr()
+        ctypes.memmove(ctypes.byref(hdr), ctypes.byref(rec.hdr), ctypes.sizeof(vp.FrameHdr))
+        frames.append({"hdr": hdr, "prep": None if hdr.show_existing_frame else gpu.prepare(rec)})
+    n = len(frames)
+    _log(f"bench: 4K 10-bit stream: {n} frames generated and prepared in {time.perf_counter() - t0:.1f}s")
+    sched = Schedule(vp, gpu, frames)
+    md5 = []
+    for i in range(n):
+        shown = sched.step(i)
+        if shown is not None:
+            md5.append(hashlib.md5(shown.packed()).hexdigest())
+    for i in range(n):
+        sched.step(i)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    for k in range(passes * n):
+        sched.step(k % n)
+    gpu.synchronize()
+    fps = passes * n / (time.perf_counter() - t0)
+    out = {"workload": "3840x2160 10-bit 4:2:0 profile 2, 48 frames, seed 4, every 12th frame at 1920x1080 "
+                       "(2:1 / 1:2 scaled references), 4 tile columns, base_q_idx 80 (stream generator)",
+           "frames_per_s": round(fps, 2), "ms_per_frame": round(1000.0 / fps, 4), "timed_frames": passes * n,
+           "bit_exact": None}
+    if check:
+        cpu_fps, cpu_md5, _, spent = cpu_baseline(frames_bytes, 0.0)
+        out["bit_exact"] = cpu_md5 == md5
+        out["cpu_baseline"] = {"value": round(cpu_fps, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle reconstruction + loop filter of the {n} frames, one pass, "
+                                         f"{spent:.1f}s"}
+    _log(f"bench: 4K 10-bit {fps:.1f} frames/s, bit-exact {out['bit_exact']}")
+    del sched, frames
+    return out
 
 
 # ---- batched stages: the kernels at full occupancy
@@ -371,6 +428,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the PCIe and latency passes")
     ap.add_argument("--no-serve", action="store_true", help="skip the multi-stream serving workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-stage measurement")
+    ap.add_argument("--no-uhd", action="store_true", help="skip the 4K 10-bit scaled-reference stream")
     ap.add_argument("--serve-streams", type=int, default=32, help="streams of the serving workload")
     ap.add_argument("--parse-threads", type=int, default=12, help="parse workers of the server")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -560,6 +618,7 @@ def main():
             _log(f"bench: MD5 MISMATCH at shown frames {bad[:10]}")
 
     batched = None if args.no_batched else batched_stages(vp, gpu, frames)
+    uhd = None if args.no_uhd else uhd_bench(vp, gpu, not args.no_cpu)
 
     serving = None
     if not args.no_serve and not args.no_cpu:
@@ -591,11 +650,13 @@ def main():
     out.update(extras)
     if batched is not None:
         out["batched"] = batched
+    if uhd is not None:
+        out["uhd_10bit"] = uhd
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)
     if bit_exact is False or (serving is not None and not serving["bit_exact"]) or \
-            (batched is not None and not batched["bit_exact"]):
+            (batched is not None and not batched["bit_exact"]) or (uhd is not None and uhd["bit_exact"] is False):
         sys.exit(1)
 
 
