@@ -40,8 +40,13 @@ the server's capacity).  Every served frame's MD5 is checked against the
 oracle's decode of its stream, which, one process per stream on all host
 cores, is also the serving CPU baseline.
 
+Kernel microbenchmark ("microbench"): 2M inverse-transform blocks and 1M
+motion-compensation blocks from seed 3 (tests/microbench.py), one batched
+call each; kernel device time, algorithmic GB/s, bit-exactness against the
+oracle over every block, and the oracle on one core as CPU baseline.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu] [--no-serve]
-                    [--no-batched] [--no-uhd] [--no-extras]
+                    [--no-batched] [--no-uhd] [--no-micro] [--micro-only] [--no-extras]
 """
 import argparse
 import ctypes
@@ -205,6 +210,97 @@ def uhd_bench(vp, gpu, check, passes=3):
                                          f"{spent:.1f}s"}
     _log(f"bench: 4K 10-bit {fps:.1f} frames/s, bit-exact {out['bit_exact']}")
     del sched, frames
+    return out
+
+
+# ---- kernel microbenchmark: transforms and motion compensation, no bitstream
+
+def _micro_cpu(kind, oracle_fn, obj, sample, threads):
+    """The oracle's batched form on one core over the first `sample` blocks
+    (blocks/s), then its result over every block on `threads` cores."""
+    t0 = time.perf_counter()
+    oracle_fn(obj, 1, sample)
+    spent = time.perf_counter() - t0
+    cpu = {"value": round(sample / spent, 1), "unit": "blocks/s", "cores": 1, "kind": "port",
+           "sample": f"oracle {kind} batch over the first {sample} blocks, one core, {spent:.1f}s"}
+    return cpu, oracle_fn(obj, threads)
+
+
+def micro_bench(vp, gpu, check, reps=5):
+    """Kernel microbenchmark (tests/microbench.py): 2M inverse-transform
+    blocks and 1M motion-compensation blocks from seed 3, 8-bit, each batch
+    one vp9r_itx_add_batch / vp9r_mc_batch call with its inputs resident in
+    HBM; the kernels' device time from events around their launches
+    (vp9r_gpu_seam_ms), median of `reps` calls after one untimed call, as
+    algorithmic GB/s against the 8 TB/s HBM peak.  With check, the first
+    call's output is compared with the oracle's over every block, and the
+    oracle's batched forms on one core over a sample are the CPU baseline."""
+    import numpy as np
+    if TESTS not in sys.path:
+        sys.path.insert(0, TESTS)
+    import microbench as mb
+    lib = vp.library()
+    threads = _host_cores()
+    out = {"workload": "2M inverse-transform blocks (sizes 4/8/16/32 at 40/30/20/10%, Laplace(0,40) coefficients, "
+                       "85% zeros, eob U[1,n^2]) and 1M motion-compensation blocks (VP9 shapes 8..64, MVs +-64 px, "
+                       "30% compound, four 1920x1080 references), seed 3, 8-bit", "bit_exact": None}
+
+    def roof(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(gbs / 8000.0, 4), "algorithmic_bytes": nbytes}
+
+    t0 = time.perf_counter()
+    s = mb.ItxSet(2_000_000, 3)
+    bufs = [gpu.upload(a) for a in (s.coefs, s.coef_off, s.tx_type, s.eob, s.dst_off)]
+    d_plane = gpu.upload(s.plane.astype(np.uint8))
+    _log(f"bench: microbench transforms generated and uploaded in {time.perf_counter() - t0:.1f}s")
+    times, first = [], None
+    for r in range(reps + 1):
+        vp._check(lib.vp9r_itx_add_batch(gpu.handle, s.count, bufs[0].ptr, bufs[1].ptr, mb.ptr(s.tx), bufs[2].ptr,
+                                         bufs[3].ptr, bufs[4].ptr, d_plane.ptr, mb.PLANE_STRIDE, 8, 0))
+        if r == 0:
+            first = d_plane.read(np.uint8, s.rows * mb.PLANE_STRIDE) if check else None
+        else:
+            times.append(gpu.seam_ms())
+    ms = sorted(times)[len(times) // 2]
+    itx = {"blocks": s.count, "kernel_ms": round(ms, 4), "blocks_per_s": round(s.count / (ms * 1e-3), 1),
+           "roofline": roof(s.bytes(), ms), "bit_exact": None}
+    del bufs, d_plane
+    if check:
+        itx["cpu_baseline"], want = _micro_cpu("inverse transform", mb.oracle_itx, s, 300_000, threads)
+        itx["bit_exact"] = bool(np.array_equal(first.reshape(want.shape), want))
+        del want
+    _log(f"bench: microbench itx {ms:.3f} ms ({itx['roofline']['achieved']} GB/s), bit-exact {itx['bit_exact']}")
+    del s, first
+
+    t0 = time.perf_counter()
+    m = mb.McSet(1_000_000, 3)
+    d_refs = [gpu.upload(r.astype(np.uint8)) for r in m.refs]
+    ref_ptrs = (ctypes.c_void_p * mb.N_REFS)(*[d.ptr.value for d in d_refs])
+    d_out = gpu.upload(np.zeros(m.rows * mb.PLANE_STRIDE, dtype=np.uint8))
+    _log(f"bench: microbench MC generated and uploaded in {time.perf_counter() - t0:.1f}s")
+    times = []
+    for r in range(reps + 1):
+        vp._check(lib.vp9r_mc_batch(gpu.handle, m.count, mb.ptr(m.desc), ref_ptrs, mb.N_REFS, mb.REF_W, mb.REF_W - 1,
+                                    mb.REF_H - 1, d_out.ptr, mb.PLANE_STRIDE, 8))
+        if r:
+            times.append(gpu.seam_ms())
+    ms = sorted(times)[len(times) // 2]
+    mc = {"blocks": m.count, "kernel_ms": round(ms, 4), "blocks_per_s": round(m.count / (ms * 1e-3), 1),
+          "roofline": roof(m.bytes(), ms), "bit_exact": None,
+          "note": "algorithmic bytes count every block's filter footprint; the four 2 MB reference planes "
+                  "stay in cache, so HBM traffic is mostly the output"}
+    if check:
+        got = d_out.read(np.uint8, m.rows * mb.PLANE_STRIDE).reshape(m.rows, mb.PLANE_STRIDE)
+        mc["cpu_baseline"], want = _micro_cpu("motion compensation", mb.oracle_mc, m, 60_000, threads)
+        mc["bit_exact"] = bool(np.array_equal(got, want))
+        del got, want
+    _log(f"bench: microbench MC {ms:.3f} ms ({mc['roofline']['achieved']} GB/s), bit-exact {mc['bit_exact']}")
+    del d_refs, d_out, m
+    out["itx"], out["mc"] = itx, mc
+    if check:
+        out["bit_exact"] = itx["bit_exact"] and mc["bit_exact"]
     return out
 
 
@@ -429,6 +525,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the multi-stream serving workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-stage measurement")
     ap.add_argument("--no-uhd", action="store_true", help="skip the 4K 10-bit scaled-reference stream")
+    ap.add_argument("--no-micro", action="store_true", help="skip the transform / MC kernel microbenchmark")
+    ap.add_argument("--micro-only", action="store_true",
+                    help="only run the kernel microbenchmark and print its JSON (a profiling aid)")
     ap.add_argument("--serve-streams", type=int, default=32, help="streams of the serving workload")
     ap.add_argument("--parse-threads", type=int, default=12, help="parse workers of the server")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -455,6 +554,10 @@ def main():
     vp.library()
     if TESTS not in sys.path:
         sys.path.insert(0, TESTS)
+    if args.micro_only:
+        micro = micro_bench(vp, vp.Gpu(0), not args.no_cpu)
+        print(json.dumps({"microbench": micro}), flush=True)
+        sys.exit(1 if micro["bit_exact"] is False else 0)
     import streamgen
 
     cfg = dict(streamgen.CONFIG2)
@@ -619,6 +722,7 @@ def main():
 
     batched = None if args.no_batched else batched_stages(vp, gpu, frames)
     uhd = None if args.no_uhd else uhd_bench(vp, gpu, not args.no_cpu)
+    micro = None if args.no_micro else micro_bench(vp, gpu, not args.no_cpu)
 
     serving = None
     if not args.no_serve and not args.no_cpu:
@@ -652,11 +756,14 @@ def main():
         out["batched"] = batched
     if uhd is not None:
         out["uhd_10bit"] = uhd
+    if micro is not None:
+        out["microbench"] = micro
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)
     if bit_exact is False or (serving is not None and not serving["bit_exact"]) or \
-            (batched is not None and not batched["bit_exact"]) or (uhd is not None and uhd["bit_exact"] is False):
+            (batched is not None and not batched["bit_exact"]) or (uhd is not None and uhd["bit_exact"] is False) or \
+            (micro is not None and micro["bit_exact"] is False):
         sys.exit(1)
 
 

@@ -292,6 +292,12 @@ int vp9r_gpu_malloc(vp9r_gpu *g, size_t bytes, void **out);
 void vp9r_gpu_mfree(vp9r_gpu *g, void *p);
 int vp9r_gpu_to_device(vp9r_gpu *g, void *dst, const void *src, size_t bytes);
 int vp9r_gpu_to_host(vp9r_gpu *g, void *dst, const void *src, size_t bytes);
+/* The elapsed device time (ms) of the kernels of the last successful
+ * vp9r_itx_add_batch or vp9r_mc_batch call, between events recorded on the
+ * context's stream around its launches (uploads excluded).  No counterpart in
+ * the reference (src/dsp/bench.rs:9-20 times the scalar calls with the host
+ * clock): a measurement aid for the kernel microbenchmarks. */
+int vp9r_gpu_seam_ms(vp9r_gpu *g, float *ms);
 /* The device address of a frame store's plane (for the batched forms). */
 void *vp9r_gpu_frame_plane(vp9r_gpu_frame *f, int plane);
 

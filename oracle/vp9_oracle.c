@@ -487,6 +487,32 @@ void orc_avg(const uint16_t *a, const uint16_t *b, int w, int h, uint16_t *dst, 
       dst[(size_t)i * stride + j] = (uint16_t)(((uint32_t)a[i * w + j] + b[i * w + j] + 1) >> 1);
 }
 
+void orc_itx_add_batch(uint32_t count, const int32_t *coefs, const uint32_t *coef_off, const uint8_t *tx,
+                       const uint8_t *tx_type, const uint16_t *eob, const uint64_t *dst_off, uint16_t *plane,
+                       size_t stride, int bd, int lossless) {
+  for (uint32_t i = 0; i < count; i++)
+    orc_itx_add(coefs + coef_off[i], 2 + tx[i], tx_type[i], lossless, eob[i], bd, plane + dst_off[i], stride);
+}
+
+void orc_mc_batch(uint32_t count, const orc_mc_desc *desc, const uint16_t *const *refs, size_t ref_stride,
+                  int last_x, int last_y, uint16_t *out, size_t out_stride, int bd) {
+  uint16_t a[64 * 64], b[64 * 64];
+  for (uint32_t i = 0; i < count; i++) {
+    const orc_mc_desc *d = &desc[i];
+    uint16_t *o = out + d->out_off;
+    if (d->ref2_idx < 0) {
+      orc_inter_predict(refs[d->ref_idx], ref_stride, last_x, last_y, d->x, d->y, d->xs, d->ys, d->w, d->h,
+                        d->filter, bd, o, out_stride);
+      continue;
+    }
+    orc_inter_predict(refs[d->ref_idx], ref_stride, last_x, last_y, d->x, d->y, d->xs, d->ys, d->w, d->h,
+                      d->filter, bd, a, d->w);
+    orc_inter_predict(refs[d->ref2_idx], ref_stride, last_x, last_y, d->x2, d->y2, d->xs, d->ys, d->w, d->h,
+                      d->filter, bd, b, d->w);
+    orc_avg(a, b, d->w, d->h, o, out_stride);
+  }
+}
+
 /* ======================================================================== */
 /* Intra prediction (8.5.1; src/dsp/intra.rs:19-175).                        */
 

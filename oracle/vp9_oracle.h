@@ -44,6 +44,26 @@ void orc_inter_predict(const uint16_t *ref, size_t ref_stride, int last_x, int l
 void orc_avg(const uint16_t *a, const uint16_t *b, int w, int h, uint16_t *dst, size_t stride);
 void orc_add_const(uint16_t *dst, size_t stride, int n, int v);
 
+/* Batched forms for the kernel microbenchmark (the loops of
+ * src/dsp/bench.rs:26-185 over many blocks).  orc_itx_add_batch: block i is
+ * the dense n x n coefficients at coefs + coef_off[i], n = 4 << tx[i], added
+ * to plane + dst_off[i].  orc_mc_batch: descriptor i (the layout of
+ * vp9r_mc_desc) predicts into out + out_off, averaging with a second
+ * prediction when ref2_idx >= 0. */
+void orc_itx_add_batch(uint32_t count, const int32_t *coefs, const uint32_t *coef_off, const uint8_t *tx,
+                       const uint8_t *tx_type, const uint16_t *eob, const uint64_t *dst_off, uint16_t *plane,
+                       size_t stride, int bit_depth, int lossless);
+typedef struct orc_mc_desc {
+  int32_t x, y, x2, y2;
+  int16_t xs, ys;
+  uint8_t w, h, filter;
+  int8_t ref_idx, ref2_idx;
+  uint8_t pad;
+  uint64_t out_off;
+} orc_mc_desc;
+void orc_mc_batch(uint32_t count, const orc_mc_desc *desc, const uint16_t *const *refs, size_t ref_stride,
+                  int last_x, int last_y, uint16_t *out, size_t out_stride, int bit_depth);
+
 /* intra::predict (src/dsp/intra.rs:19-175): above[0] is aboveRow[-1]. */
 void orc_intra_predict(int mode, int log2, const int32_t *above, const int32_t *left,
                        int have_left, int have_above, int bit_depth, uint16_t *dst, size_t stride);

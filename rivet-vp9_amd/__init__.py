@@ -184,6 +184,7 @@ def library():
     lib.vp9r_gpu_mfree.argtypes = [P, V]
     lib.vp9r_gpu_to_device.argtypes = [P, V, V, sz]
     lib.vp9r_gpu_to_host.argtypes = [P, V, V, sz]
+    lib.vp9r_gpu_seam_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
     lib.vp9r_gpu_frame_plane.restype = V
     lib.vp9r_gpu_frame_plane.argtypes = [P, ctypes.c_int]
     lib.vp9r_decoder_new.argtypes = [ctypes.c_int, PP]
@@ -335,6 +336,12 @@ class Gpu:
         b = DevBuffer(self, a.nbytes)
         _check(library().vp9r_gpu_to_device(self._g, b.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
         return b
+
+    def seam_ms(self):
+        """Device time (ms) of the last batched seam call's kernels."""
+        ms = ctypes.c_float()
+        _check(library().vp9r_gpu_seam_ms(self._g, ctypes.byref(ms)))
+        return ms.value
 
 
 class Prepared:

@@ -26,6 +26,22 @@ static int dev_err(hipError_t e, const char *what) {
 
 static bool bd_ok(uint32_t bd) { return bd == 8 || bd == 10 || bd == 12; }
 
+// Events around a batched seam call's kernels (created on first use).
+static hipError_t seam_begin(vp9r_gpu *g) {
+  g->seam_ms = -1.0f;
+  for (hipEvent_t &e : g->seam_ev)
+    if (!e) {
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+    }
+  return hipEventRecord(g->seam_ev[0], g->stream);
+}
+
+// After the stream has synchronised: the kernels' elapsed time.
+static hipError_t seam_end(vp9r_gpu *g) {
+  return hipEventElapsedTime(&g->seam_ms, g->seam_ev[0], g->seam_ev[1]);
+}
+
 extern "C" {
 
 int vp9r_gpu_malloc(vp9r_gpu *g, size_t bytes, void **out) {
@@ -53,6 +69,15 @@ int vp9r_gpu_to_host(vp9r_gpu *g, void *dst, const void *src, size_t bytes) {
   return VP9R_OK;
 }
 
+int vp9r_gpu_seam_ms(vp9r_gpu *g, float *ms) {
+  *ms = g->seam_ms;
+  if (g->seam_ms < 0) {
+    set_error("no timed batched call");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  return VP9R_OK;
+}
+
 void *vp9r_gpu_frame_plane(vp9r_gpu_frame *f, int plane) {
   return (plane >= 0 && plane < 3) ? f->planes.p[plane] : nullptr;
 }
@@ -74,19 +99,26 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
     jobs[tx[i]].push_back(i);
   }
   TRY(hipSetDevice(g->device), "hipSetDevice");
+  // the job lists of the four sizes, largest first, in one upload; then the
+  // launches back to back
+  std::vector<uint32_t> all;
+  all.reserve(count);
+  uint32_t start[4];
+  for (int k = 3; k >= 0; k--) {
+    start[k] = (uint32_t)all.size();
+    all.insert(all.end(), jobs[k].begin(), jobs[k].end());
+  }
   TRY(g->seam_a.ensure(sizeof(uint32_t) * std::max<uint32_t>(count, 1)), "hipMalloc");
   uint32_t *dj = (uint32_t *)g->seam_a.p;
-  uint32_t off = 0;
-  for (int k = 0; k < 4; k++) {
-    if (jobs[k].empty()) continue;
-    TRY(hipMemcpyAsync(dj + off, jobs[k].data(), 4 * jobs[k].size(), hipMemcpyHostToDevice, g->stream),
-        "upload jobs");
+  if (count) TRY(hipMemcpyAsync(dj, all.data(), 4 * (size_t)count, hipMemcpyHostToDevice, g->stream), "upload jobs");
+  TRY(seam_begin(g), "hipEventRecord");
+  for (int k = 3; k >= 0; k--)
     vp9g::launch_itx_dense(g->stream, count, coefs, coef_off, nullptr, tx_type, eob, dst_off, dst, stride,
-                           bit_depth, lossless, k, dj + off, (uint32_t)jobs[k].size());
-    off += (uint32_t)jobs[k].size();
-  }
+                           bit_depth, lossless, k, dj + start[k], (uint32_t)jobs[k].size());
   TRY(hipGetLastError(), "itx launch");
+  TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "itx sync");
+  TRY(seam_end(g), "hipEventElapsedTime");
   return VP9R_OK;
 }
 
@@ -122,11 +154,14 @@ int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const v
   TRY(hipMemcpyAsync(g->seam_a.p, desc, dbytes, hipMemcpyHostToDevice, g->stream), "upload");
   TRY(hipMemcpyAsync(g->seam_b.p, tiles.data(), tbytes, hipMemcpyHostToDevice, g->stream), "upload");
   TRY(hipMemcpyAsync(g->seam_c.p, refs, sizeof(void *) * n_refs, hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(seam_begin(g), "hipEventRecord");
   vp9g::launch_mc_desc_tiles(g->stream, (const vp9r_mc_desc *)g->seam_a.p, g->seam_b.p, (uint32_t)tiles.size(),
                              (const void *const *)g->seam_c.p, ref_stride, last_x, last_y, out, out_stride,
                              bit_depth);
   TRY(hipGetLastError(), "mc launch");
+  TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "mc sync");
+  TRY(seam_end(g), "hipEventElapsedTime");
   return VP9R_OK;
 }
 

@@ -489,6 +489,8 @@ void vp9r_gpu_free(vp9r_gpu *g) {
     if (s.ev) (void)hipEventDestroy(s.ev);
   for (auto &set : g->ev_sets)
     for (auto ev : set) (void)hipEventDestroy(ev);
+  for (hipEvent_t e : g->seam_ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(g->stream);
   (void)hipStreamDestroy(g->copy_stream);
   if (g->lf_stream) (void)hipStreamDestroy(g->lf_stream);

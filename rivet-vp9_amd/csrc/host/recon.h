@@ -146,6 +146,9 @@ struct vp9r_gpu {
   std::vector<uint8_t> batch_stage;
   vp9r::BatchSlot slots[vp9r::kBatchSlots];
   int slot_next = 0;
-  // per-kernel seam scratch
+  // per-kernel seam scratch, and events around the last batched seam call's
+  // kernels (seam_ms: their elapsed time)
   vp9r::DevBuf seam_a, seam_b, seam_c;
+  hipEvent_t seam_ev[2] = {};
+  float seam_ms = -1.0f;
 };

@@ -53,6 +53,9 @@ def lib():
     L.orc_spec_iadst.argtypes = [P, ctypes.c_int]
     L.orc_inter_predict.argtypes = [P, ctypes.c_size_t] + [ctypes.c_int] * 10 + [P, ctypes.c_size_t]
     L.orc_avg.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_size_t]
+    L.orc_itx_add_batch.argtypes = [ctypes.c_uint32] + [P] * 7 + [ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+    L.orc_mc_batch.argtypes = [ctypes.c_uint32, P, P, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, P,
+                               ctypes.c_size_t, ctypes.c_int]
     L.orc_add_const.argtypes = [P, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
     L.orc_intra_predict.argtypes = [ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, P, ctypes.c_size_t]

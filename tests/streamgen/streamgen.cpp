@@ -32,6 +32,14 @@
 
 using namespace vp9r;
 
+// The coefficient scan of a transform size and type (the decoder's tables).
+static const uint16_t *scan_of(int tx_size, int tx_type) {
+  return tx_size == TX_4X4     ? (tx_type == ADST_DCT ? VP9_ROW_SCAN_4X4 : tx_type == DCT_ADST ? VP9_COL_SCAN_4X4 : VP9_DEFAULT_SCAN_4X4)
+         : tx_size == TX_8X8   ? (tx_type == ADST_DCT ? VP9_ROW_SCAN_8X8 : tx_type == DCT_ADST ? VP9_COL_SCAN_8X8 : VP9_DEFAULT_SCAN_8X8)
+         : tx_size == TX_16X16 ? (tx_type == ADST_DCT ? VP9_ROW_SCAN_16X16 : tx_type == DCT_ADST ? VP9_COL_SCAN_16X16 : VP9_DEFAULT_SCAN_16X16)
+                               : VP9_DEFAULT_SCAN_32X32;
+}
+
 extern "C" {
 typedef struct sg_config {
   uint32_t width, height, frames, seed;
@@ -198,10 +206,7 @@ class Plan : public SyntaxPlan {
     double scale = cur_intra_ ? 6.0 : 1.6;
     scale *= 100.0 / std::max(40u, c_.base_q_idx);
     std::exponential_distribution<double> lap(1.0 / scale);
-    const uint16_t *scan = tx_size == TX_4X4   ? (tx_type == ADST_DCT ? VP9_ROW_SCAN_4X4 : tx_type == DCT_ADST ? VP9_COL_SCAN_4X4 : VP9_DEFAULT_SCAN_4X4)
-                           : tx_size == TX_8X8 ? (tx_type == ADST_DCT ? VP9_ROW_SCAN_8X8 : tx_type == DCT_ADST ? VP9_COL_SCAN_8X8 : VP9_DEFAULT_SCAN_8X8)
-                           : tx_size == TX_16X16 ? (tx_type == ADST_DCT ? VP9_ROW_SCAN_16X16 : tx_type == DCT_ADST ? VP9_COL_SCAN_16X16 : VP9_DEFAULT_SCAN_16X16)
-                                                 : VP9_DEFAULT_SCAN_32X32;
+    const uint16_t *scan = scan_of(tx_size, tx_type);
     // positions: the first scan indices mostly, spread over about 3k
     int span = std::min(nn, 3 * k + 2);
     std::uniform_int_distribution<int> pos(0, span - 1);
@@ -381,6 +386,13 @@ extern "C" {
 void *sg_new(const sg_config *cfg) { return new Gen(*cfg); }
 void sg_free(void *g) { delete (Gen *)g; }
 const char *sg_error(void) { return g_err.c_str(); }
+
+// The scan order of transform size tx_size (0..3) and type tx_type: out
+// receives (4 << tx_size)^2 raster positions in scan order.
+void sg_scan(int tx_size, int tx_type, uint16_t *out) {
+  const int n = 4 << tx_size;
+  memcpy(out, scan_of(tx_size, tx_type), sizeof(uint16_t) * n * n);
+}
 
 // Writes the next frame; returns its length (0 at the end, -1 on error) and
 // points *data at it (valid until the next call).

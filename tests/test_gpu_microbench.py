@@ -1,0 +1,68 @@
+"""GPU parity of the kernel microbenchmark's batches (tests/microbench.py)
+against the oracle, every block, through the C ABI: 2M inverse-transform
+blocks (vp9r_itx_add_batch, itx.rs:701-749) and 1M motion-compensation
+blocks (vp9r_mc_batch, inter.rs:289-395 and pixel.rs:22-50) from seed 3 at
+8 bits, and a smaller 10-bit batch of each.  Bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+import microbench as mb
+import rivet_vp9_amd as vp
+
+pytestmark = pytest.mark.gpu
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    return vp.Gpu(0)
+
+
+def run_itx(gpu, s):
+    dt = np.uint8 if s.bit_depth == 8 else np.uint16
+    d_plane = gpu.upload(s.plane.astype(dt))
+    bufs = [gpu.upload(a) for a in (s.coefs, s.coef_off, s.tx_type, s.eob, s.dst_off)]
+    st = vp.library().vp9r_itx_add_batch(gpu.handle, s.count, bufs[0].ptr, bufs[1].ptr, mb.ptr(s.tx),
+                                         bufs[2].ptr, bufs[3].ptr, bufs[4].ptr, d_plane.ptr, mb.PLANE_STRIDE,
+                                         s.bit_depth, 0)
+    assert st == 0, vp.library().vp9r_last_error()
+    assert gpu.seam_ms() > 0
+    return d_plane.read(dt, s.rows * mb.PLANE_STRIDE).reshape(s.rows, mb.PLANE_STRIDE)
+
+
+def run_mc(gpu, m):
+    import ctypes
+    dt = np.uint8 if m.bit_depth == 8 else np.uint16
+    d_refs = [gpu.upload(r.astype(dt)) for r in m.refs]
+    ref_ptrs = (ctypes.c_void_p * mb.N_REFS)(*[d.ptr.value for d in d_refs])
+    d_out = gpu.upload(np.zeros(m.rows * mb.PLANE_STRIDE, dtype=dt))
+    st = vp.library().vp9r_mc_batch(gpu.handle, m.count, mb.ptr(m.desc), ref_ptrs, mb.N_REFS, mb.REF_W,
+                                    mb.REF_W - 1, mb.REF_H - 1, d_out.ptr, mb.PLANE_STRIDE, m.bit_depth)
+    assert st == 0, vp.library().vp9r_last_error()
+    assert gpu.seam_ms() > 0
+    return d_out.read(dt, m.rows * mb.PLANE_STRIDE).reshape(m.rows, mb.PLANE_STRIDE)
+
+
+def first_diff(got, want):
+    r, c = np.argwhere(got.astype(np.uint16) != want)[0]
+    return f"first difference at row {r} col {c}: {got[r, c]} != {want[r, c]}"
+
+
+@pytest.mark.parametrize("n,bd", [(2_000_000, 8), (100_000, 10)])
+def test_itx_batch_every_block(gpu, n, bd):
+    print(f"itx batch: {n} blocks, seed 3, {bd}-bit", flush=True)
+    s = mb.ItxSet(n, 3, bd)
+    got = run_itx(gpu, s)
+    want = mb.oracle_itx(s, THREADS)
+    assert np.array_equal(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("n,bd", [(1_000_000, 8), (50_000, 10)])
+def test_mc_batch_every_block(gpu, n, bd):
+    print(f"mc batch: {n} blocks, seed 3, {bd}-bit", flush=True)
+    m = mb.McSet(n, 3, bd)
+    got = run_mc(gpu, m)
+    want = mb.oracle_mc(m, THREADS)
+    assert np.array_equal(got, want), first_diff(got, want)
