@@ -23,6 +23,14 @@
 template <int N>
 struct GWord;
 template <>
+struct GWord<1> {
+  typedef unsigned char T;
+};
+template <>
+struct GWord<2> {
+  typedef unsigned short T;
+};
+template <>
 struct GWord<4> {
   typedef unsigned T;
 };

@@ -11,6 +11,15 @@
 // path serves every step: the specification's formula, of which the
 // reference's unscaled fast path is a special case (its own test,
 // inter.rs:486-564, checks they agree).
+//
+// 8-bit tiles at steps of 16 (every unscaled reference) whose sides are
+// multiples of 4 take a second path: the footprint is staged as dwords, and
+// each 8-tap sum is two v_dot4_i32_i8 on samples biased by -128 (they then
+// fit signed bytes; sum f (p - 128) + 128 * 128 = sum f p, the taps of every
+// fractional phase fitting signed bytes and summing to 128), a lane producing
+// four samples per pass: four rows of one column in the horizontal pass
+// (stored column-major), four columns of one row in the vertical pass (one
+// dword of the output).
 #include <type_traits>
 
 #include "dev_common.h"
@@ -22,15 +31,22 @@
 namespace vp9g {
 
 __constant__ int16_t c_filters[4][16][8];
+// the taps as signed bytes, taps 0..3 then 4..7 (phase 0 unused)
+__constant__ uint32_t c_taps8[4][16][2];
 
 static bool filters_ready = false;
 static void upload_filters() {
   if (filters_ready) return;
   int16_t f[4][16][8];
+  uint32_t t8[4][16][2] = {};
   for (int a = 0; a < 4; a++)
     for (int b = 0; b < 16; b++)
-      for (int c = 0; c < 8; c++) f[a][b][c] = (int16_t)VP9_SUBPEL_FILTERS[a][b][c];
+      for (int c = 0; c < 8; c++) {
+        f[a][b][c] = (int16_t)VP9_SUBPEL_FILTERS[a][b][c];
+        t8[a][b][c >> 2] |= (uint32_t)(uint8_t)(int8_t)VP9_SUBPEL_FILTERS[a][b][c] << (8 * (c & 3));
+      }
   hipMemcpyToSymbol(HIP_SYMBOL(c_filters), f, sizeof f);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_taps8), t8, sizeof t8);
   filters_ready = true;
 }
 
@@ -44,6 +60,102 @@ struct McShared {
   uint16_t fp[FPH * FPW];  // reference footprint, edge samples repeated
   int16_t hp[FPH * TILE];  // after the horizontal pass
 };
+
+// The 8-bit unscaled path's staging: footprint rows of 8 dwords (bytes from
+// a dword-aligned column), and the horizontal pass column-major (column c
+// holds rows 0..23 in 6 dwords).
+struct Mc8Shared {
+  uint32_t fp[24 * 8];
+  uint32_t hp[16 * 6];
+};
+
+union McLds {
+  McShared g;
+  Mc8Shared f;
+};
+
+// Byte-wise rounding average of four packed samples: (a + b + 1) >> 1
+// (pixel.rs:22-50).
+DEV uint32_t avg4(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu); }
+
+// One 8-tap sum of the biased window of bytes s .. s + 7 of (w0, w1, w2),
+// rounded by 7 and clamped to 8 bits.
+DEV uint32_t tap8(uint32_t w0, uint32_t w1, uint32_t w2, int s, uint32_t f0, uint32_t f1) {
+  const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, s);
+  const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, s);
+  const int sum = __builtin_amdgcn_sdot4((int)lo, (int)f0,
+                                         __builtin_amdgcn_sdot4((int)hi, (int)f1, 128 * 128 + 64, false), false);
+  return (uint32_t)iclamp(sum >> 7, 0, 255);
+}
+
+// The 8-bit tile of tw x th samples (multiples of 4, at most 16) whose first
+// sample is at (x0, y0) (1/16 sample, steps 16) from `ref` (bottom-right
+// sample (last_x, last_y)).  Returns the lane's output dword: row lane / 4,
+// columns 4 (lane % 4) .. + 3 (meaningful when inside the tile).  A footprint
+// wholly inside the plane is read as dwords; otherwise sample by sample at
+// clamped coordinates.  All 64 lanes call it (it holds barriers).
+DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int tw, int th,
+                      int filter, Mc8Shared &sh) {
+  const int lane = threadIdx.x;
+  const int px = x0 & 15, py = y0 & 15;
+  const bool fx = px != 0, fy = py != 0;
+  const int fx0 = (x0 >> 4) - (fx ? 3 : 0), fy0 = (y0 >> 4) - (fy ? 3 : 0);
+  const int iw = tw + (fx ? 7 : 0), ih = th + (fy ? 7 : 0);
+  const int ax = fx0 & ~3, off = fx0 - ax, nd = (off + iw + 3) >> 2;
+  const bool inside = fx0 >= 0 && fy0 >= 0 && ax + 4 * nd - 1 <= last_x && fy0 + ih - 1 <= last_y &&
+                      ((stride | (uint32_t)(uintptr_t)ref) & 3) == 0;
+  // the horizontal pass's input is biased by -128 (bytes flipped)
+  const uint32_t flip = fx ? 0x80808080u : 0u;
+  const int dd = lane & 7;
+  for (int rr = lane >> 3; rr < ih; rr += 8) {
+    if (dd < nd) {
+      uint32_t w = 0;
+      if (inside) {
+        w = gld((const uint32_t *)(ref + (size_t)(fy0 + rr) * stride + ax + 4 * dd));
+      } else {
+        const uint8_t *row = ref + (size_t)iclamp(fy0 + rr, 0, last_y) * stride;
+#pragma unroll
+        for (int b = 0; b < 4; b++) w |= (uint32_t)gld(row + iclamp(ax + 4 * dd + b, 0, last_x)) << (8 * b);
+      }
+      sh.fp[rr * 8 + dd] = w ^ flip;
+    }
+  }
+  __syncthreads();
+  // horizontal: lane task (column c, rows 4 rq .. 4 rq + 3); the vertical
+  // pass's input is biased too
+  const uint32_t f0 = c_taps8[filter][px][0], f1 = c_taps8[filter][px][1];
+  const uint32_t vflip = fy ? 0x80808080u : 0u;
+  for (int t = lane; t < 16 * 6; t += 64) {
+    const int c = t & 15, rq = t >> 4;
+    if (c < tw && 4 * rq < ih) {
+      const int o = off + c, k = o >> 2, s = o & 3;
+      uint32_t packed = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t *w = sh.fp + (4 * rq + j) * 8 + k;
+        const uint32_t v = fx ? tap8(w[0], w[1], w[2], s, f0, f1) : (w[0] >> (8 * s)) & 0xffu;
+        packed |= v << (8 * j);
+      }
+      sh.hp[c * 6 + rq] = packed ^ vflip;
+    }
+  }
+  __syncthreads();
+  // vertical: lane task (row r, columns 4 q .. 4 q + 3)
+  const int r = lane >> 2, q = lane & 3;
+  uint32_t res = 0;
+  if (r < th && 4 * q < tw) {
+    const uint32_t g0 = c_taps8[filter][py][0], g1 = c_taps8[filter][py][1];
+    const int s = r & 3;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t *w = sh.hp + (4 * q + j) * 6 + (r >> 2);
+      const uint32_t v = fy ? tap8(w[0], w[1], w[2], s, g0, g1) : (w[0] >> (8 * s)) & 0xffu;
+      res |= v << (8 * j);
+    }
+  }
+  __syncthreads();
+  return res;
+}
 
 // Predicts the tw x th (<= 16 x 16) tile whose first sample is at (x0, y0)
 // (1/16 sample) with steps (xs, ys) from `ref`, whose bottom-right sample is
@@ -109,23 +221,38 @@ DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, 
 template <bool HBD>
 __global__ __launch_bounds__(64) void k_mc(const McArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ McShared sh;
+  __shared__ McLds sh;
   const uint32_t fi = range_frame(rg, nf, blockIdx.x);
   const McArgs &a = fa[fi];
   const McTile t = a.tiles[rg[fi].first + blockIdx.x - rg[fi].wg];
   const vp9r_block &b = a.blocks[t.block];
   const int max = (1 << a.g.bit_depth) - 1;
   const bool compound = b.ref_frame[1] > INTRA_FRAME;
-  int out[2][4];
-  for (int rl = 0; rl < 1 + (int)compound; rl++) {
-    RefPos rp = mc_position(a.g, a.ref_w, a.ref_h, b, t.plane, t.x, t.y, t.sub, rl);
-    const int ri = b.ref_frame[rl] - LAST_FRAME;
-    const P *ref = (const P *)a.ref[ri].p[t.plane];
-    mc_tile<P>(ref, a.ref[ri].stride[t.plane], rp.last_x, rp.last_y, rp.x + rp.xs * t.tx, rp.y + rp.ys * t.ty,
-               rp.xs, rp.ys, t.tw, t.th, b.interp_filter, max, sh, out[rl]);
-  }
   P *dst = (P *)a.cur.p[t.plane];
   const uint32_t stride = a.cur.stride[t.plane];
+  const RefPos rp0 = mc_position(a.g, a.ref_w, a.ref_h, b, t.plane, t.x, t.y, t.sub, 0);
+  const RefPos rp1 = compound ? mc_position(a.g, a.ref_w, a.ref_h, b, t.plane, t.x, t.y, t.sub, 1) : rp0;
+  const int r0 = b.ref_frame[0] - LAST_FRAME, r1 = compound ? b.ref_frame[1] - LAST_FRAME : r0;
+  if (!HBD && ((t.tw | t.th | stride | (t.x + t.tx)) & 3) == 0 && rp0.xs == 16 && rp0.ys == 16 &&
+      rp1.xs == 16 && rp1.ys == 16) {
+    uint32_t v = mc_tile8((const uint8_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
+                          rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f);
+    if (compound)
+      v = avg4(v, mc_tile8((const uint8_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x,
+                           rp1.last_y, rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f));
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    if (r < t.th && 4 * q < t.tw)
+      gst((uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q), v);
+    return;
+  }
+  int out[2][4];
+  mc_tile<P>((const P *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
+             rp0.x + rp0.xs * t.tx, rp0.y + rp0.ys * t.ty, rp0.xs, rp0.ys, t.tw, t.th, b.interp_filter, max, sh.g,
+             out[0]);
+  if (compound)
+    mc_tile<P>((const P *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
+               rp1.x + rp1.xs * t.tx, rp1.y + rp1.ys * t.ty, rp1.xs, rp1.ys, t.tw, t.th, b.interp_filter, max,
+               sh.g, out[1]);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int o = threadIdx.x + 64 * i;
@@ -150,18 +277,29 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
                                                 int32_t last_x, int32_t last_y, void *out,
                                                 uint32_t out_stride, int bd) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ McShared sh;
+  __shared__ McLds sh;
   const DescTile t = tiles[blockIdx.x];
   const vp9r_mc_desc d = descs[t.desc];
   const int max = (1 << bd) - 1;
   const int th = d.h - t.ty < TILE ? d.h - t.ty : TILE, tw = d.w - t.tx < TILE ? d.w - t.tx : TILE;
+  const bool compound = d.ref2_idx >= 0;
+  P *dst = (P *)out + d.out_off + (size_t)t.ty * out_stride + t.tx;
+  if (!HBD && d.xs == 16 && d.ys == 16 && ((tw | th | out_stride | (uint32_t)(uintptr_t)dst) & 3) == 0) {
+    uint32_t v = mc_tile8((const uint8_t *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + 16 * t.tx,
+                          d.y + 16 * t.ty, tw, th, d.filter, sh.f);
+    if (compound)
+      v = avg4(v, mc_tile8((const uint8_t *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + 16 * t.tx,
+                           d.y2 + 16 * t.ty, tw, th, d.filter, sh.f));
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    if (r < th && 4 * q < tw) gst((uint32_t *)(dst + (size_t)r * out_stride + 4 * q), v);
+    return;
+  }
   int o0[4], o1[4];
   mc_tile<P>((const P *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + d.xs * t.tx, d.y + d.ys * t.ty,
-             d.xs, d.ys, tw, th, d.filter, max, sh, o0);
-  const bool compound = d.ref2_idx >= 0;
+             d.xs, d.ys, tw, th, d.filter, max, sh.g, o0);
   if (compound)
     mc_tile<P>((const P *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + d.xs * t.tx,
-               d.y2 + d.ys * t.ty, d.xs, d.ys, tw, th, d.filter, max, sh, o1);
+               d.y2 + d.ys * t.ty, d.xs, d.ys, tw, th, d.filter, max, sh.g, o1);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int o = threadIdx.x + 64 * i;

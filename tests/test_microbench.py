@@ -107,3 +107,20 @@ def test_oracle_batches_equal_single_calls():
             a = c
         r, c0 = divmod(int(d["out_off"]), mb.PLANE_STRIDE)
         assert np.array_equal(got[r:r + h, c0:c0 + w], a), f"descriptor {i}"
+
+
+def test_subpel_taps_fit_signed_bytes():
+    """The 8-bit unscaled MC path sums taps as signed bytes over samples
+    biased by -128: every fractional phase's taps must fit [-128, 127] and
+    every phase sum to 128."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rivet-vp9_amd", "csrc",
+                        "vp9_tables.h")
+    text = open(path).read()
+    body = text[text.index("VP9_SUBPEL_FILTERS[4][16][8]"):]
+    body = body[body.index("{") + 1:body.index("};")]
+    taps = np.array([int(v) for v in re.findall(r"-?\d+", body)]).reshape(4, 16, 8)
+    assert (taps.sum(axis=2) == 128).all()
+    assert (taps[:, 1:] >= -128).all() and (taps[:, 1:] <= 127).all()
+    assert (taps[:, 0] == [0, 0, 0, 128, 0, 0, 0, 0]).all()
