@@ -144,6 +144,9 @@ typedef struct vp9r_gpu_frame vp9r_gpu_frame; /* an HBM-resident frame */
 /* Creates a context on HIP device `device`.  Fails with VP9R_ERR_DEVICE when
  * there is no usable GPU. */
 int vp9r_gpu_new(int device, vp9r_gpu **out);
+/* Releases the caller's context.  Frame stores and device buffers made from
+ * it keep it alive until they are freed too, so objects may be freed in any
+ * order (the context is destroyed with the last of them). */
 void vp9r_gpu_free(vp9r_gpu *g);
 
 /* A frame store of the record's geometry (planes of whole superblocks:

@@ -384,10 +384,12 @@ class DevBuffer:
         p = ctypes.c_void_p()
         _check(library().vp9r_gpu_malloc(gpu.handle, nbytes, ctypes.byref(p)))
         self.ptr = p
+        # the context stays valid while this buffer holds its reference
+        self._g = ctypes.c_void_p(gpu.handle.value)
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            library().vp9r_gpu_mfree(self.gpu.handle, self.ptr)
+            library().vp9r_gpu_mfree(self._g, self.ptr)
             self.ptr = None
 
     def read(self, dtype, count):

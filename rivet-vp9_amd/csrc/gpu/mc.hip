@@ -61,11 +61,13 @@ struct McShared {
   int16_t hp[FPH * TILE];  // after the horizontal pass
 };
 
-// The 8-bit unscaled path's staging: footprint rows of 8 dwords (bytes from
-// a dword-aligned column), and the horizontal pass column-major (column c
-// holds rows 0..23 in 6 dwords).
+// The 8-bit unscaled path's staging: footprint rows of FP8 dwords (bytes
+// from a dword-aligned column; the pitch of 10 puts the four row groups a
+// read touches in distinct banks), and the horizontal pass column-major
+// (column c holds rows 0..23 in 6 dwords).
+constexpr int FP8 = 10;
 struct Mc8Shared {
-  uint32_t fp[24 * 8];
+  uint32_t fp[24 * FP8];
   uint32_t hp[16 * 6];
 };
 
@@ -79,13 +81,41 @@ union McLds {
 DEV uint32_t avg4(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu); }
 
 // One 8-tap sum of the biased window of bytes s .. s + 7 of (w0, w1, w2),
-// rounded by 7 and clamped to 8 bits.
+// rounded by 7 and clamped to 8 bits.  The clamp is an explicit v_med3_i32:
+// left to itself the compiler fuses two such clamps and their packing into
+// v_ashr_pk_u8_i32 and takes the result's upper half as zero, but the
+// instruction keeps its destination's previous upper half.
 DEV uint32_t tap8(uint32_t w0, uint32_t w1, uint32_t w2, int s, uint32_t f0, uint32_t f1) {
   const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, s);
   const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, s);
   const int sum = __builtin_amdgcn_sdot4((int)lo, (int)f0,
                                          __builtin_amdgcn_sdot4((int)hi, (int)f1, 128 * 128 + 64, false), false);
-  return (uint32_t)iclamp(sum >> 7, 0, 255);
+  uint32_t v;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(v) : "v"(sum >> 7), "v"(255));
+  return v;
+}
+
+// Four samples, the filtered (taps f0, f1) or plain bytes at byte offset s
+// of rows w, w + pitch, w + 2 pitch, w + 3 pitch, packed.
+template <int PITCH>
+DEV uint32_t four_samples(const uint32_t *w, int s, bool filter, uint32_t f0, uint32_t f1) {
+  uint32_t packed = 0;
+  if (filter) {
+    uint32_t a[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int m = 0; m < 3; m++) a[j][m] = w[j * PITCH + m];
+#pragma unroll
+    for (int j = 0; j < 4; j++) packed |= tap8(a[j][0], a[j][1], a[j][2], s, f0, f1) << (8 * j);
+  } else {
+    uint32_t a[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) a[j] = w[j * PITCH];
+#pragma unroll
+    for (int j = 0; j < 4; j++) packed |= ((a[j] >> (8 * s)) & 0xffu) << (8 * j);
+  }
+  return packed;
 }
 
 // The 8-bit tile of tw x th samples (multiples of 4, at most 16) whose first
@@ -104,39 +134,39 @@ DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_
   const int ax = fx0 & ~3, off = fx0 - ax, nd = (off + iw + 3) >> 2;
   const bool inside = fx0 >= 0 && fy0 >= 0 && ax + 4 * nd - 1 <= last_x && fy0 + ih - 1 <= last_y &&
                       ((stride | (uint32_t)(uintptr_t)ref) & 3) == 0;
-  // the horizontal pass's input is biased by -128 (bytes flipped)
-  const uint32_t flip = fx ? 0x80808080u : 0u;
-  const int dd = lane & 7;
-  for (int rr = lane >> 3; rr < ih; rr += 8) {
-    if (dd < nd) {
-      uint32_t w = 0;
-      if (inside) {
-        w = gld((const uint32_t *)(ref + (size_t)(fy0 + rr) * stride + ax + 4 * dd));
-      } else {
-        const uint8_t *row = ref + (size_t)iclamp(fy0 + rr, 0, last_y) * stride;
+  // footprint: lane (row lane / 8 + 8 i, dword lane % 8), all loads issued
+  // before the first store; the horizontal pass's input is biased by -128
+  const int dd = lane & 7, r0 = lane >> 3;
+  uint32_t w[3] = {0, 0, 0};
+  if (inside) {
 #pragma unroll
-        for (int b = 0; b < 4; b++) w |= (uint32_t)gld(row + iclamp(ax + 4 * dd + b, 0, last_x)) << (8 * b);
+    for (int i = 0; i < 3; i++)
+      if (dd < nd && r0 + 8 * i < ih)
+        w[i] = gld((const uint32_t *)(ref + (size_t)(fy0 + r0 + 8 * i) * stride + ax + 4 * dd));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+      if (dd < nd && r0 + 8 * i < ih) {
+        const uint8_t *row = ref + (size_t)iclamp(fy0 + r0 + 8 * i, 0, last_y) * stride;
+#pragma unroll
+        for (int b = 0; b < 4; b++) w[i] |= (uint32_t)gld(row + iclamp(ax + 4 * dd + b, 0, last_x)) << (8 * b);
       }
-      sh.fp[rr * 8 + dd] = w ^ flip;
-    }
   }
+  const uint32_t flip = fx ? 0x80808080u : 0u;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+    if (dd < nd && r0 + 8 * i < ih) sh.fp[(r0 + 8 * i) * FP8 + dd] = w[i] ^ flip;
   __syncthreads();
-  // horizontal: lane task (column c, rows 4 rq .. 4 rq + 3); the vertical
-  // pass's input is biased too
+  // horizontal: lane task t (column t % 16, rows 4 (t / 16) .. + 3) for
+  // t = lane, lane + 64; the vertical pass's input is biased too
   const uint32_t f0 = c_taps8[filter][px][0], f1 = c_taps8[filter][px][1];
   const uint32_t vflip = fy ? 0x80808080u : 0u;
-  for (int t = lane; t < 16 * 6; t += 64) {
-    const int c = t & 15, rq = t >> 4;
-    if (c < tw && 4 * rq < ih) {
-      const int o = off + c, k = o >> 2, s = o & 3;
-      uint32_t packed = 0;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const uint32_t *w = sh.fp + (4 * rq + j) * 8 + k;
-        const uint32_t v = fx ? tap8(w[0], w[1], w[2], s, f0, f1) : (w[0] >> (8 * s)) & 0xffu;
-        packed |= v << (8 * j);
-      }
-      sh.hp[c * 6 + rq] = packed ^ vflip;
+  for (int h = 0; h < 2; h++) {
+    const int t = lane + 64 * h, c = t & 15, rq = t >> 4;
+    if (t < 96 && c < tw && 4 * rq < ih) {
+      const int o = off + c;
+      sh.hp[c * 6 + rq] = four_samples<FP8>(sh.fp + 4 * rq * FP8 + (o >> 2), o & 3, fx, f0, f1) ^ vflip;
     }
   }
   __syncthreads();
@@ -145,13 +175,8 @@ DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_
   uint32_t res = 0;
   if (r < th && 4 * q < tw) {
     const uint32_t g0 = c_taps8[filter][py][0], g1 = c_taps8[filter][py][1];
-    const int s = r & 3;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t *w = sh.hp + (4 * q + j) * 6 + (r >> 2);
-      const uint32_t v = fy ? tap8(w[0], w[1], w[2], s, g0, g1) : (w[0] >> (8 * s)) & 0xffu;
-      res |= v << (8 * j);
-    }
+    // four_samples reads columns 4 q .. 4 q + 3 as its "rows" (pitch 6)
+    res = four_samples<6>(sh.hp + 4 * q * 6 + (r >> 2), r & 3, fy, g0, g1);
   }
   __syncthreads();
   return res;

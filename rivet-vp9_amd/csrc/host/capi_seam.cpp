@@ -48,6 +48,7 @@ int vp9r_gpu_malloc(vp9r_gpu *g, size_t bytes, void **out) {
   *out = nullptr;
   TRY(hipSetDevice(g->device), "hipSetDevice");
   TRY(hipMalloc(out, bytes ? bytes : 1), "hipMalloc");
+  gpu_retain(g);
   return VP9R_OK;
 }
 
@@ -55,6 +56,7 @@ void vp9r_gpu_mfree(vp9r_gpu *g, void *p) {
   if (!p) return;
   (void)hipStreamSynchronize(g->stream);
   (void)hipFree(p);
+  gpu_release(g);
 }
 
 int vp9r_gpu_to_device(vp9r_gpu *g, void *dst, const void *src, size_t bytes) {

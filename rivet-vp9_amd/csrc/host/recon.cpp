@@ -475,7 +475,15 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
 }
 
 void vp9r_gpu_free(vp9r_gpu *g) {
-  if (!g) return;
+  if (g) vp9r::gpu_release(g);
+}
+
+}  // extern "C"
+
+void vp9r::gpu_retain(vp9r_gpu *g) { g->refs.fetch_add(1); }
+
+void vp9r::gpu_release(vp9r_gpu *g) {
+  if (g->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(g->device);
   (void)hipStreamSynchronize(g->stream);
   (void)hipStreamSynchronize(g->copy_stream);
@@ -497,6 +505,8 @@ void vp9r_gpu_free(vp9r_gpu *g) {
   if (g->up_stream) (void)hipStreamDestroy(g->up_stream);
   delete g;
 }
+
+extern "C" {
 
 int vp9r_gpu_frame_new(vp9r_gpu *g, uint32_t width, uint32_t height, uint32_t bd, uint32_t ss_x,
                        uint32_t ss_y, vp9r_gpu_frame **out) {
@@ -541,6 +551,7 @@ int vp9r_gpu_frame_new(vp9r_gpu *g, uint32_t width, uint32_t height, uint32_t bd
   }
   f->bytes = total;
   for (int p = 0; p < 3; p++) f->planes.p[p] = (uint8_t *)f->base + off[p];
+  gpu_retain(g);
   *out = f;
   return VP9R_OK;
 }
@@ -554,6 +565,7 @@ void vp9r_gpu_frame_free(vp9r_gpu_frame *f) {
   (void)hipEventDestroy(f->ready);
   (void)hipEventDestroy(f->copy_done);
   (void)hipFree(f->base);
+  if (f->g) gpu_release(f->g);
   delete f;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -98,6 +99,10 @@ constexpr int kBatchSlots = 4;
 // the context's stream, each launch covering every frame.
 int launch_batch(vp9r_gpu *g, const BatchFrame *f, int n, bool recon, bool lf);
 
+// Takes / releases a reference to a context (the last release destroys it).
+void gpu_retain(vp9r_gpu *g);
+void gpu_release(vp9r_gpu *g);
+
 }  // namespace vp9r
 
 struct vp9r_gpu_frame {
@@ -114,6 +119,10 @@ struct vp9r_gpu_frame {
 };
 
 struct vp9r_gpu {
+  // the context and each frame store or device buffer made from it hold a
+  // reference: vp9r_gpu_free releases the context's own, and the last
+  // release destroys it, so the objects may be freed in any order
+  std::atomic<int> refs{1};
   int device = 0;
   hipStream_t stream = nullptr;       // kernels and record uploads
   hipStream_t copy_stream = nullptr;  // frame downloads
