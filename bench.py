@@ -43,7 +43,9 @@ cores, is also the serving CPU baseline.
 Kernel microbenchmark ("microbench"): 2M inverse-transform blocks and 1M
 motion-compensation blocks from seed 3 (tests/microbench.py), one batched
 call each; kernel device time, algorithmic GB/s, bit-exactness against the
-oracle over every block, and the oracle on one core as CPU baseline.
+oracle over every block, and the oracle on one core as CPU baseline.  Its
+"lf" part: 64 1080p key frames with random filter levels and sharpness,
+filtered in one batch (vp9r_gpu_filter_batch), MD5s against the oracle.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--no-cpu] [--no-serve]
                     [--no-batched] [--no-uhd] [--no-micro] [--micro-only] [--no-extras]
@@ -304,6 +306,68 @@ def micro_bench(vp, gpu, check, reps=5):
     return out
 
 
+# ---- loop filter microbenchmark: random levels and sharpness, 64 frames a launch
+
+def lf_bench(vp, gpu, check, copies=64, reps=5):
+    """64 one-frame 1920x1080 8-bit 4:2:0 streams (key frames of moving
+    rectangles, 4 tile columns, q 100, seeds 300..363, so 64 partitions),
+    each with its own loop filter level U[0, 63] and sharpness U[0, 7] (seed
+    3).  Each of `reps` + 1 rounds reconstructs all 64 with the filter off
+    (vp9r_gpu_run_batch of the records at level 0), then filters them in one
+    batch (vp9r_gpu_filter_batch: every launch covers all 64) timed by events
+    around its launches; median of the timed rounds, against 2 x the visible
+    frame bytes per frame.  With check, every filtered frame's MD5 (first
+    round) against the oracle's decode of its stream."""
+    import concurrent.futures
+    import multiprocessing
+    import numpy as np
+    import streamgen
+    rng = np.random.default_rng(3)
+    levels, sharp = rng.integers(0, 64, size=copies), rng.integers(0, 8, size=copies)
+    cfgs = [dict(streamgen.CONFIG2, frames=1, seed=300 + i, lf_level=int(levels[i]), lf_sharpness=int(sharp[i]))
+            for i in range(copies)]
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(_host_cores()) as ex:
+        streams = list(ex.map(lambda c: streamgen.generate(**c), cfgs))
+    p_lf, p_off, curs = [], [], []
+    for fr in streams:
+        parser = vp.Parser()
+        rec = parser.parse(fr[0])
+        p_lf.append(gpu.prepare(rec))
+        rec.hdr.lf_level = 0
+        p_off.append(gpu.prepare(rec))
+        curs.append(gpu.frame(1920, 1080, 8, 1, 1))
+    _log(f"bench: loop filter microbench: {copies} key frames generated and prepared in "
+         f"{time.perf_counter() - t0:.1f}s")
+    times, md5 = [], None
+    for r in range(reps + 1):
+        gpu.run_batch([(p, None, c) for p, c in zip(p_off, curs)])
+        gpu.synchronize()
+        ms = gpu.filter_batch(list(zip(p_lf, curs)))
+        if r == 0:
+            md5 = [hashlib.md5(c.packed()).hexdigest() for c in curs]
+        else:
+            times.append(ms)
+    ms = sorted(times)[len(times) // 2]
+    nbytes = sum(p.stats()["lf_bytes"] for p in p_lf)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    out = {"workload": f"{copies} 1920x1080 8-bit 4:2:0 key frames (seeds 300..{299 + copies}), filter level "
+                       f"U[0,63] and sharpness U[0,7] per frame (seed 3), one vp9r_gpu_filter_batch",
+           "frames": copies, "kernel_ms": round(ms, 4), "frames_per_s": round(copies / (ms * 1e-3), 1),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes},
+           "sharpness_levels": sorted({int(v) for v in sharp}), "bit_exact": None}
+    if check:
+        ctx = multiprocessing.get_context("spawn")
+        with concurrent.futures.ProcessPoolExecutor(_host_cores(), mp_context=ctx) as ex:
+            want = [m[0][0] for m in ex.map(_oracle_worker, streams)]
+        out["bit_exact"] = want == md5
+    _log(f"bench: loop filter microbench {ms:.3f} ms for {copies} frames ({gbs:.1f} GB/s), "
+         f"bit-exact {out['bit_exact']}")
+    del curs, p_lf, p_off
+    return out
+
+
 # ---- batched stages: the kernels at full occupancy
 
 def batched_stages(vp, gpu, frames, copies=64, reps=5):
@@ -555,7 +619,11 @@ def main():
     if TESTS not in sys.path:
         sys.path.insert(0, TESTS)
     if args.micro_only:
-        micro = micro_bench(vp, vp.Gpu(0), not args.no_cpu)
+        g = vp.Gpu(0)
+        micro = micro_bench(vp, g, not args.no_cpu)
+        micro["lf"] = lf_bench(vp, g, not args.no_cpu)
+        if micro["lf"]["bit_exact"] is False:
+            micro["bit_exact"] = False
         print(json.dumps({"microbench": micro}), flush=True)
         sys.exit(1 if micro["bit_exact"] is False else 0)
     import streamgen
@@ -723,6 +791,10 @@ def main():
     batched = None if args.no_batched else batched_stages(vp, gpu, frames)
     uhd = None if args.no_uhd else uhd_bench(vp, gpu, not args.no_cpu)
     micro = None if args.no_micro else micro_bench(vp, gpu, not args.no_cpu)
+    if micro is not None:
+        micro["lf"] = lf_bench(vp, gpu, not args.no_cpu)
+        if micro["lf"]["bit_exact"] is False:
+            micro["bit_exact"] = False
 
     serving = None
     if not args.no_serve and not args.no_cpu:

@@ -195,6 +195,14 @@ int vp9r_gpu_run(vp9r_gpu *g, const vp9r_prepared *p, vp9r_gpu_frame *const refs
  * counterpart. */
 int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *p,
                        vp9r_gpu_frame *const *refs, vp9r_gpu_frame *const *cur);
+/* The loop filter alone (loopfilter::filter_frame, loopfilter.rs:51-81) of n
+ * frames, each launch covering every frame: cur[i] must hold the
+ * reconstruction of record p[i] (by vp9r_gpu_run with a filter level of 0,
+ * or an earlier copy).  Waits for completion; the device time of its
+ * launches is read with vp9r_gpu_seam_ms.  Not while stage timing is on.
+ * Batches the single-frame vp9r_gpu_loop_filter. */
+int vp9r_gpu_filter_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *p,
+                          vp9r_gpu_frame *const *cur);
 /* Work of a prepared record: [0] inter prediction tiles (<= 16x16), [1..4]
  * inter transform blocks of 4x4 .. 32x32, [5] intra blocks, [6] intra
  * dependency levels (one launch each), [7] loop filter wavefront launches,
@@ -296,7 +304,7 @@ void vp9r_gpu_mfree(vp9r_gpu *g, void *p);
 int vp9r_gpu_to_device(vp9r_gpu *g, void *dst, const void *src, size_t bytes);
 int vp9r_gpu_to_host(vp9r_gpu *g, void *dst, const void *src, size_t bytes);
 /* The elapsed device time (ms) of the kernels of the last successful
- * vp9r_itx_add_batch or vp9r_mc_batch call, between events recorded on the
+ * vp9r_itx_add_batch, vp9r_mc_batch or vp9r_gpu_filter_batch call, between events recorded on the
  * context's stream around its launches (uploads excluded).  No counterpart in
  * the reference (src/dsp/bench.rs:9-20 times the scalar calls with the host
  * clock): a measurement aid for the kernel microbenchmarks. */

@@ -178,6 +178,7 @@ def library():
     lib.vp9r_prepared_free.argtypes = [P]
     lib.vp9r_gpu_run.argtypes = [P, P, ctypes.c_void_p * 3, P]
     lib.vp9r_gpu_run_batch.argtypes = [P, u32, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V)]
+    lib.vp9r_gpu_filter_batch.argtypes = [P, u32, ctypes.POINTER(V), ctypes.POINTER(V)]
     lib.vp9r_gpu_prepared_stats.argtypes = [P, ctypes.POINTER(u64)]
     lib.vp9r_gpu_prepared_schedule.argtypes = [P, ctypes.POINTER(ctypes.c_uint32)]
     lib.vp9r_gpu_malloc.argtypes = [P, sz, PP]
@@ -325,6 +326,15 @@ class Gpu:
                 rs[3 * i + k] = r.handle if r is not None else None
         cs = (ctypes.c_void_p * max(n, 1))(*[j[2].handle for j in jobs])
         _check(library().vp9r_gpu_run_batch(self._g, n, ps, rs, cs))
+
+    def filter_batch(self, jobs):
+        """The loop filter alone of reconstructed frames (vp9r_gpu_filter_batch):
+        `jobs` is a list of (prepared, cur); returns the launches' device ms."""
+        n = len(jobs)
+        ps = (ctypes.c_void_p * max(n, 1))(*[j[0].handle for j in jobs])
+        cs = (ctypes.c_void_p * max(n, 1))(*[j[1].handle for j in jobs])
+        _check(library().vp9r_gpu_filter_batch(self._g, n, ps, cs))
+        return self.seam_ms()
 
     def buffer(self, nbytes):
         return DevBuffer(self, nbytes)

@@ -26,21 +26,6 @@ static int dev_err(hipError_t e, const char *what) {
 
 static bool bd_ok(uint32_t bd) { return bd == 8 || bd == 10 || bd == 12; }
 
-// Events around a batched seam call's kernels (created on first use).
-static hipError_t seam_begin(vp9r_gpu *g) {
-  g->seam_ms = -1.0f;
-  for (hipEvent_t &e : g->seam_ev)
-    if (!e) {
-      hipError_t r = hipEventCreate(&e);
-      if (r != hipSuccess) return r;
-    }
-  return hipEventRecord(g->seam_ev[0], g->stream);
-}
-
-// After the stream has synchronised: the kernels' elapsed time.
-static hipError_t seam_end(vp9r_gpu *g) {
-  return hipEventElapsedTime(&g->seam_ms, g->seam_ev[0], g->seam_ev[1]);
-}
 
 extern "C" {
 

@@ -482,6 +482,18 @@ void vp9r_gpu_free(vp9r_gpu *g) {
 
 void vp9r::gpu_retain(vp9r_gpu *g) { g->refs.fetch_add(1); }
 
+hipError_t vp9r::seam_begin(vp9r_gpu *g) {
+  g->seam_ms = -1.0f;
+  for (hipEvent_t &e : g->seam_ev)
+    if (!e) {
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+    }
+  return hipEventRecord(g->seam_ev[0], g->stream);
+}
+
+hipError_t vp9r::seam_end(vp9r_gpu *g) { return hipEventElapsedTime(&g->seam_ms, g->seam_ev[0], g->seam_ev[1]); }
+
 void vp9r::gpu_release(vp9r_gpu *g) {
   if (g->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(g->device);
@@ -888,13 +900,19 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   return VP9R_OK;
 }
 
-static int check_frames(const Prepared &p, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur) {
+static int check_cur(const Prepared &p, const vp9r_gpu_frame *cur) {
   const vp9r_frame_hdr &h = p.hdr;
   if (!cur || cur->width != h.width || cur->height != h.height || cur->bit_depth != h.bit_depth ||
       cur->ss_x != h.ss_x || cur->ss_y != h.ss_y) {
     set_error("the frame store does not match the record's geometry");
     return VP9R_ERR_INVALID_INPUT;
   }
+  return VP9R_OK;
+}
+
+static int check_frames(const Prepared &p, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur) {
+  const vp9r_frame_hdr &h = p.hdr;
+  if (int st = check_cur(p, cur)) return st;
   if (!h.frame_is_intra)
     for (int i = 0; i < 3; i++) {
       const vp9r_gpu_frame *r = refs ? refs[i] : nullptr;
@@ -1435,6 +1453,41 @@ int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *pr, 
     HIP_TRY(hipEventRecord(g->ev[0], g->stream), "hipEventRecord");
   }
   return launch_batch(g, bf.data(), (int)n, true, true);
+}
+
+int vp9r_gpu_filter_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *pr, vp9r_gpu_frame *const *cur) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (n == 0) return VP9R_OK;
+  if (!pr || !cur) {
+    set_error("vp9r_gpu_filter_batch: null argument");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  if (g->timing) {
+    set_error("vp9r_gpu_filter_batch: stage timing is on");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  std::vector<BatchFrame> bf(n);
+  for (uint32_t i = 0; i < n; i++) {
+    if (!pr[i]) {
+      set_error("vp9r_gpu_filter_batch: null prepared record");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+    const Prepared &p = pr[i]->p;
+    if (int st = check_cur(p, cur[i])) return st;
+    bf[i] = BatchFrame{&p, (const uint8_t *)pr[i]->dev.p, {nullptr, nullptr, nullptr}, cur[i]};
+    for (uint32_t j = 0; j < i; j++)
+      if (bf[j].cur == cur[i]) {
+        set_error("vp9r_gpu_filter_batch: two frames of the batch filter the same frame store");
+        return VP9R_ERR_INVALID_INPUT;
+      }
+  }
+  HIP_TRY(seam_begin(g), "hipEventRecord");
+  int st = launch_batch(g, bf.data(), (int)n, false, true);
+  if (st) return st;
+  HIP_TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
+  HIP_TRY(hipStreamSynchronize(g->stream), "filter sync");
+  HIP_TRY(seam_end(g), "hipEventElapsedTime");
+  return VP9R_OK;
 }
 
 int vp9r_gpu_prepared_schedule(const vp9r_prepared *pr, uint32_t out[3]) {

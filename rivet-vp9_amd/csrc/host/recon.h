@@ -103,6 +103,12 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *f, int n, bool recon, bool lf);
 void gpu_retain(vp9r_gpu *g);
 void gpu_release(vp9r_gpu *g);
 
+// Events around a timed call's launches (created on first use) on the
+// context's stream; after the stream has synchronised, seam_end reads their
+// elapsed time into g->seam_ms.
+hipError_t seam_begin(vp9r_gpu *g);
+hipError_t seam_end(vp9r_gpu *g);
+
 }  // namespace vp9r
 
 struct vp9r_gpu_frame {

@@ -12,7 +12,7 @@ GEN_LIB = os.path.join(GEN_DIR, "_build", "libstreamgen.so")
 class Config(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "width", "height", "frames", "seed", "bit_depth", "ss_x", "ss_y", "base_q_idx", "tile_cols_log2",
-        "golden_interval", "content", "resize_every", "intra_percent", "allow_hp")]
+        "golden_interval", "content", "resize_every", "intra_percent", "allow_hp", "lf_level", "lf_sharpness")]
 
 
 _lib = None
@@ -35,11 +35,16 @@ def lib():
 
 
 def generate(width, height, frames, seed, bit_depth=8, ss_x=1, ss_y=1, base_q_idx=100, tile_cols_log2=0,
-             golden_interval=8, content=1, resize_every=0, intra_percent=3, allow_hp=1):
-    """The frames (bytes) of a generated stream."""
+             golden_interval=8, content=1, resize_every=0, intra_percent=3, allow_hp=1, lf_level=None,
+             lf_sharpness=0):
+    """The frames (bytes) of a generated stream.  lf_level: the loop filter
+    level (0..63; None derives it from the quantiser); lf_sharpness 0..7."""
     L = lib()
+    if lf_level is not None and not 0 <= lf_level <= 63 or not 0 <= lf_sharpness <= 7:
+        raise ValueError("loop filter level 0..63, sharpness 0..7")
     cfg = Config(width, height, frames, seed, bit_depth, ss_x, ss_y, base_q_idx, tile_cols_log2,
-                 golden_interval, content, resize_every, intra_percent, allow_hp)
+                 golden_interval, content, resize_every, intra_percent, allow_hp,
+                 0 if lf_level is None else lf_level + 1, lf_sharpness)
     g = L.sg_new(ctypes.byref(cfg))
     out = []
     try:

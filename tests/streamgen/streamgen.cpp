@@ -18,7 +18,7 @@
 // Frame headers mirror the crate's encoder (src/encoder/mod.rs:558-697):
 // key frames, then error-resilient, frame-parallel inter frames without
 // context refresh, EIGHTTAP, single reference, TX_MODE_SELECT, loop filter
-// level (q * 10 + 32) / 64, sharpness 0, no segmentation, LAST refreshed
+// level (q * 10 + 32) / 64 unless set, sharpness as set (0 by default), no segmentation, LAST refreshed
 // every frame and GOLDEN every golden_interval frames.  Optional resizing
 // codes every Nth frame at half size (scaled references both ways).
 #include <cmath>
@@ -49,6 +49,8 @@ typedef struct sg_config {
   uint32_t resize_every;  // 0 never, else every Nth frame at half size
   uint32_t intra_percent; // share of inter-frame blocks coded intra
   uint32_t allow_hp;
+  uint32_t lf_level;      // 0 derived from the quantiser, else the filter level + 1
+  uint32_t lf_sharpness;  // 0..7
 } sg_config;
 }
 
@@ -331,7 +333,7 @@ struct Gen {
     }
     b.f(2, 0);  // frame_context_idx
     b.f(6, lf_level);
-    b.f(3, 0);  // sharpness
+    b.f(3, c.lf_sharpness);
     b.f(1, 1);  // mode_ref_delta_enabled
     b.f(1, 0);  // mode_ref_delta_update
     b.f(8, c.base_q_idx);
@@ -360,7 +362,7 @@ struct Gen {
     uint32_t w, h;
     frame_size(t, &w, &h);
     uint32_t refresh = key ? 0xff : (1u | ((c.golden_interval && t % c.golden_interval == 0) ? 2u : 0u));
-    uint32_t lf_level = std::min(63u, (c.base_q_idx * 10 + 32) / 64);
+    uint32_t lf_level = c.lf_level ? std::min(63u, c.lf_level - 1) : std::min(63u, (c.base_q_idx * 10 + 32) / 64);
     std::vector<uint8_t> hdr = header(key, w, h, refresh, lf_level);
     Plan plan(c, w, h, (int)t, key, rects, bgx, bgy, rng);
     Parser sim = dec;

@@ -109,3 +109,45 @@ def test_batch_rejects_dependent_frames(gpu):
         gpu.run_batch([(p0, [None] * 3, a), (p0, [None] * 3, a)])
     gpu.run_batch([(p0, [None] * 3, a)])
     gpu.synchronize()
+
+
+def test_filter_batch_equals_the_oracle(gpu):
+    """vp9r_gpu_filter_batch: frames reconstructed with the filter off (their
+    records at level 0), then filtered together with their own levels and
+    sharpness (intra and inter frames, 8 and 10 bits, 4:4:4), equal the
+    oracle's decode; refused while stage timing is on."""
+    cases = [dict(width=352, height=288, frames=2, seed=31, lf_level=40, lf_sharpness=5),
+             dict(width=200, height=120, frames=2, seed=32, lf_level=63, lf_sharpness=7, bit_depth=10),
+             dict(width=160, height=96, frames=2, seed=33, lf_level=20, lf_sharpness=0, ss_x=0, ss_y=0)]
+    jobs_lf, want = [], []
+    keep = []
+    for cfg in cases:
+        frames = streamgen.generate(**cfg)
+        ref = oracle_lib.CpuDecoder()
+        md5s = [oracle_lib.md5(o.packed()) for f in frames for o in ref.decode(f)]
+        # frame 0 (intra) decoded alone; frame 1 (inter) predicts from it
+        p = vp.Parser()
+        rec0 = p.parse(frames[0])
+        h = rec0.hdr
+        f0 = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+        gpu.run(gpu.prepare(rec0), [None] * 3, f0)
+        rec1 = p.parse(frames[1])
+        assert not rec1.hdr.frame_is_intra
+        p_lf = gpu.prepare(rec1)
+        rec1.hdr.lf_level = 0
+        p_off = gpu.prepare(rec1)
+        cur = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+        gpu.run(p_off, [f0] * 3, cur)
+        jobs_lf.append((p_lf, cur))
+        want.append(md5s[1])
+        keep += [f0, p_off]
+    gpu.synchronize()
+    ms = gpu.filter_batch(jobs_lf)
+    assert ms > 0
+    assert [oracle_lib.md5(c.packed()) for _, c in jobs_lf] == want
+    gpu.set_timing(True)
+    try:
+        with pytest.raises(vp.InvalidInputError):
+            gpu.filter_batch(jobs_lf)
+    finally:
+        gpu.set_timing(False)

@@ -24,6 +24,14 @@ CASES = {
     "8bit-422": dict(width=160, height=96, frames=4, seed=6, ss_x=1, ss_y=0),
     "10bit-440": dict(width=160, height=96, frames=4, seed=7, bit_depth=10, ss_x=0, ss_y=1),
     "lossless": dict(width=96, height=64, frames=3, seed=8, base_q_idx=0),
+    # loop filter sharpness > 0 (limits shifted and capped, lf.rs / 8.8.1) and
+    # the extreme levels
+    "sharpness5-level40": dict(width=352, height=288, frames=3, seed=9, content=0, lf_level=40, lf_sharpness=5),
+    "sharpness7-level63-10bit": dict(width=320, height=192, frames=3, seed=10, bit_depth=10, lf_level=63,
+                                     lf_sharpness=7),
+    "sharpness3-level1-444": dict(width=160, height=96, frames=3, seed=11, ss_x=0, ss_y=0, lf_level=1,
+                                  lf_sharpness=3),
+    "level0-filter-off": dict(width=160, height=96, frames=3, seed=12, lf_level=0),
     "1080p-workload": dict(streamgen.CONFIG2, frames=3),
     # the 4K 10-bit workload, resized every other frame: 2:1 and 1:2 scaled references
     "4k-10bit-scaled": dict(streamgen.CONFIG4, frames=4, resize_every=2),

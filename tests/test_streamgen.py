@@ -66,6 +66,17 @@ def test_formats_decode(bd, ssx, ssy):
     assert all(r[4:7] == (bd, ssx, ssy) for r in recs)
 
 
+def test_loop_filter_level_and_sharpness_as_set():
+    for level, sharp in ((0, 0), (1, 3), (40, 5), (63, 7)):
+        frames = streamgen.generate(160, 96, 2, seed=5, lf_level=level, lf_sharpness=sharp)
+        p = vp.Parser()
+        for f in frames:
+            h = p.parse(f).hdr
+            assert (h.lf_level, h.lf_sharpness) == (level, sharp)
+    h = vp.Parser().parse(streamgen.generate(160, 96, 1, seed=5, base_q_idx=100)[0]).hdr
+    assert (h.lf_level, h.lf_sharpness) == ((100 * 10 + 32) // 64, 0)
+
+
 def test_benchmark_workload_headers():
     """The 1080p workload's first frames: size, tiles, quantiser and a mix
     of intra and inter blocks."""
