@@ -80,26 +80,33 @@ union McLds {
 // (pixel.rs:22-50).
 DEV uint32_t avg4(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu); }
 
-// One 8-tap sum of the biased window of bytes s .. s + 7 of (w0, w1, w2),
-// rounded by 7 and clamped to 8 bits.  The clamp is an explicit v_med3_i32:
-// left to itself the compiler fuses two such clamps and their packing into
-// v_ashr_pk_u8_i32 and takes the result's upper half as zero, but the
-// instruction keeps its destination's previous upper half.
+// One 8-tap sum over the biased window of bytes s .. s + 7 of (w0, w1, w2)
+// (samples p - 128): since the taps sum to 128,
+// Round2(sum f p, 7) - 128 = (sum f (p - 128) + 64) >> 7, returned clamped to
+// [-128, 127] (the 8-bit range, still biased).  The clamp is an explicit
+// v_med3_i32: left to itself the compiler fuses two such clamps and their
+// packing into v_ashr_pk_u8_i32 and takes the result's upper half as zero,
+// but the instruction keeps its destination's previous upper half.
 DEV uint32_t tap8(uint32_t w0, uint32_t w1, uint32_t w2, int s, uint32_t f0, uint32_t f1) {
   const uint32_t lo = __builtin_amdgcn_alignbyte(w1, w0, s);
   const uint32_t hi = __builtin_amdgcn_alignbyte(w2, w1, s);
-  const int sum = __builtin_amdgcn_sdot4((int)lo, (int)f0,
-                                         __builtin_amdgcn_sdot4((int)hi, (int)f1, 128 * 128 + 64, false), false);
+  const int sum = __builtin_amdgcn_sdot4((int)lo, (int)f0, __builtin_amdgcn_sdot4((int)hi, (int)f1, 64, false), false);
   uint32_t v;
-  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(v) : "v"(sum >> 7), "v"(255));
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(v) : "v"(sum >> 7), "v"(-128), "v"(127));
   return v;
 }
 
-// Four samples, the filtered (taps f0, f1) or plain bytes at byte offset s
-// of rows w, w + pitch, w + 2 pitch, w + 3 pitch, packed.
+// The low bytes of four values packed, a first (three byte permutes).
+DEV uint32_t pack4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  const uint32_t ab = __builtin_amdgcn_perm(b, a, 0x0c0c0400u), cd = __builtin_amdgcn_perm(d, c, 0x0c0c0400u);
+  return __builtin_amdgcn_perm(cd, ab, 0x05040100u);
+}
+
+// Four samples packed: filtered (taps f0, f1; biased) or the plain bytes at
+// byte offset s of rows w, w + pitch, w + 2 pitch, w + 3 pitch.
 template <int PITCH>
 DEV uint32_t four_samples(const uint32_t *w, int s, bool filter, uint32_t f0, uint32_t f1) {
-  uint32_t packed = 0;
+  uint32_t v[4];
   if (filter) {
     uint32_t a[4][3];
 #pragma unroll
@@ -107,15 +114,12 @@ DEV uint32_t four_samples(const uint32_t *w, int s, bool filter, uint32_t f0, ui
 #pragma unroll
       for (int m = 0; m < 3; m++) a[j][m] = w[j * PITCH + m];
 #pragma unroll
-    for (int j = 0; j < 4; j++) packed |= tap8(a[j][0], a[j][1], a[j][2], s, f0, f1) << (8 * j);
+    for (int j = 0; j < 4; j++) v[j] = tap8(a[j][0], a[j][1], a[j][2], s, f0, f1);
   } else {
-    uint32_t a[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) a[j] = w[j * PITCH];
-#pragma unroll
-    for (int j = 0; j < 4; j++) packed |= ((a[j] >> (8 * s)) & 0xffu) << (8 * j);
+    for (int j = 0; j < 4; j++) v[j] = w[j * PITCH] >> (8 * s);
   }
-  return packed;
+  return pack4(v[0], v[1], v[2], v[3]);
 }
 
 // The 8-bit tile of tw x th samples (multiples of 4, at most 16) whose first
@@ -158,9 +162,10 @@ DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_
     if (dd < nd && r0 + 8 * i < ih) sh.fp[(r0 + 8 * i) * FP8 + dd] = w[i] ^ flip;
   __syncthreads();
   // horizontal: lane task t (column t % 16, rows 4 (t / 16) .. + 3) for
-  // t = lane, lane + 64; the vertical pass's input is biased too
+  // t = lane, lane + 64.  Its results are biased when filtered (fx); the
+  // vertical pass wants them biased when it filters (fy)
   const uint32_t f0 = c_taps8[filter][px][0], f1 = c_taps8[filter][px][1];
-  const uint32_t vflip = fy ? 0x80808080u : 0u;
+  const uint32_t vflip = fx != fy ? 0x80808080u : 0u;
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int t = lane + 64 * h, c = t & 15, rq = t >> 4;
@@ -175,8 +180,9 @@ DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_
   uint32_t res = 0;
   if (r < th && 4 * q < tw) {
     const uint32_t g0 = c_taps8[filter][py][0], g1 = c_taps8[filter][py][1];
-    // four_samples reads columns 4 q .. 4 q + 3 as its "rows" (pitch 6)
-    res = four_samples<6>(sh.hp + 4 * q * 6 + (r >> 2), r & 3, fy, g0, g1);
+    // four_samples reads columns 4 q .. 4 q + 3 as its "rows" (pitch 6);
+    // filtered results are biased
+    res = four_samples<6>(sh.hp + 4 * q * 6 + (r >> 2), r & 3, fy, g0, g1) ^ (fy ? 0x80808080u : 0u);
   }
   __syncthreads();
   return res;
