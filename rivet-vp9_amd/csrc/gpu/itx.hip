@@ -82,7 +82,37 @@ __global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range
     itx_inter_wg<2, HBD>(fa, r4, nf, w - b16, tiles);
 }
 
-// Per-kernel seam: dense dequantised coefficients per block.
+// Adds residuals r[0..n0) to the n0 samples at `row`, clamped to [0, max]:
+// as dwords when the row is 4-byte aligned, else sample by sample.
+template <class P, int n0>
+DEV void add_row(P *row, const int32_t *r, int max) {
+  if (((uintptr_t)row & 3) == 0) {
+    constexpr int PER = 4 / (int)sizeof(P), ND = n0 / PER;
+    uint32_t w[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) w[d] = gld((const uint32_t *)row + d);
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      uint32_t o = 0;
+#pragma unroll
+      for (int e = 0; e < PER; e++) {
+        const int sh = 32 / PER * e;
+        const int v = iclamp((int)((w[d] >> sh) & (0xffffffffu >> (32 - 32 / PER))) + r[d * PER + e], 0, max);
+        o |= (uint32_t)v << sh;
+      }
+      gst((uint32_t *)row + d, o);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < n0; k++) row[k] = (P)iclamp((int)row[k] + r[k], 0, max);
+  }
+}
+
+// Per-kernel seam: dense dequantised coefficients per block.  A team of n0
+// lanes takes a block: lane k transforms row k straight from memory, the
+// columns go through the LDS tile, the column residuals go back into the
+// tile, and lane k adds row k of them to the destination (one dword access
+// per 4 bytes of the row).
 template <int n, bool HBD>
 __global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const uint32_t *coef_off,
                                                   const uint8_t *tx_type, const uint16_t *eob,
@@ -98,21 +128,50 @@ __global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const ui
   const uint32_t job = blockIdx.x * BPW + team;
   const bool active = job < n_jobs;
   T *tile = tiles[team];
-  uint32_t i = active ? jobs[job] : 0;
-  if (active) {
-    const int32_t *c = coefs + coef_off[i];
+  const uint32_t i = active ? jobs[job] : 0;
+  const int tt = active ? tx_type[i] : 0;
+  const int32_t *c = coefs + (active ? coef_off[i] : 0);
+  const bool dc_only = !HBD && active && eob[i] == 1 && tt == DCT_DCT && !lossless;
+  T t[n0];
+  if (active && !dc_only) {
+    bool zero = true;
 #pragma unroll
-    for (int k = 0; k < n0; k++) tile[lane * S + k] = (T)c[lane * n0 + k];
+    for (int k = 0; k < n0; k++) {
+      t[k] = (T)c[lane * n0 + k];
+      zero = zero && t[k] == 0;
+    }
+    if (!zero || lossless) tx1d<L, n>(t, (tt & 2) == 0, lossless != 0, 2);
+#pragma unroll
+    for (int k = 0; k < n0; k++) tile[lane * S + k] = t[k];
   }
   __syncthreads();
-  const int tt = active ? tx_type[i] : 0;
-  const bool dc_only = !HBD && active && eob[i] == 1 && tt == DCT_DCT && !lossless;
-  int32_t dc_v = 0;
-  if (dc_only) dc_v = dc_only_value((int32_t)tile[0], n);
-  GlobalDst<P> dst;
-  dst.stride = stride;
-  dst.base = (P *)dst_base + (active ? dst_off[i] : 0);
-  itx_2d<L, n>(tile, active ? lane : n0, tt, lossless != 0, dc_only, dc_v, dst, (1 << bd) - 1);
+  const int max = (1 << bd) - 1;
+  if (active && !dc_only) {
+    // column `lane`; only this lane reads or writes it until the barrier
+#pragma unroll
+    for (int k = 0; k < n0; k++) t[k] = tile[k * S + lane];
+    tx1d<L, n>(t, (tt & 1) == 0, lossless != 0, 0);
+    constexpr int shift = n + 2 < 6 ? n + 2 : 6;
+#pragma unroll
+    for (int k = 0; k < n0; k++) {
+      const T v = lossless ? t[k] : L::rnd(t[k], shift);
+      tile[k * S + lane] = (T)lclamp((int64_t)v, -65536, 65536);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    int32_t r[n0];
+    if (dc_only) {
+      const int32_t v = iclamp(dc_only_value(c[0], n), -65536, 65536);
+#pragma unroll
+      for (int k = 0; k < n0; k++) r[k] = v;
+    } else {
+#pragma unroll
+      for (int k = 0; k < n0; k++) r[k] = (int32_t)tile[lane * S + k];
+    }
+    P *row = (P *)dst_base + dst_off[i] + (size_t)lane * stride;
+    add_row<P, n0>(row, r, HBD ? max : 255);
+  }
 }
 
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4], uint32_t nf,
