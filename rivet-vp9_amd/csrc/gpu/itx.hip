@@ -10,6 +10,98 @@
 
 namespace vp9g {
 
+// Adds residuals r[0..n0) to the n0 samples at `row`, clamped to [0, max]:
+// as dwords when the row is 4-byte aligned, else sample by sample.
+template <class P, int n0>
+DEV void add_row(P *row, const int32_t *r, int max) {
+  if (((uintptr_t)row & 3) == 0) {
+    constexpr int PER = 4 / (int)sizeof(P), ND = n0 / PER;
+    uint32_t w[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) w[d] = gld((const uint32_t *)row + d);
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      uint32_t o = 0;
+#pragma unroll
+      for (int e = 0; e < PER; e++) {
+        const int sh = 32 / PER * e;
+        const int v = iclamp((int)((w[d] >> sh) & (0xffffffffu >> (32 - 32 / PER))) + r[d * PER + e], 0, max);
+        o |= (uint32_t)v << sh;
+      }
+      gst((uint32_t *)row + d, o);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < n0; k++) row[k] = (P)iclamp((int)row[k] + r[k], 0, max);
+  }
+}
+
+// Stores residuals r[0..n0) as int16 at `row` (clamped): as dwords when the
+// row is 4-byte aligned.
+template <int n0>
+DEV void store_res_row(int16_t *row, const int32_t *r) {
+  if (((uintptr_t)row & 3) == 0) {
+#pragma unroll
+    for (int d = 0; d < n0 / 2; d++) {
+      const uint32_t lo = (uint16_t)iclamp(r[2 * d], -32768, 32767), hi = (uint16_t)iclamp(r[2 * d + 1], -32768, 32767);
+      gst((uint32_t *)row + d, lo | hi << 16);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < n0; k++) row[k] = (int16_t)iclamp(r[k], -32768, 32767);
+  }
+}
+
+// The 2D transform of the block in `tile` (row stride n0 + 1) by its team's
+// lanes, ending row-wise: rows, then columns (residuals back into the tile,
+// each lane its own column), then lane k writes row k of the residuals:
+// added into the frame at base (clamped to [0, max]) or, with res set,
+// stored as int16.  dc_only: every residual is dc_v.  Every lane of the
+// workgroup calls this (it holds barriers).
+template <class L, int n, class P>
+DEV void itx_2d_rows(typename L::T *tile, int lane, bool active, int tx_type, bool lossless, bool dc_only,
+                     int32_t dc_v, P *base, int16_t *res, uint32_t stride, int max) {
+  typedef typename L::T T;
+  constexpr int n0 = 1 << n, S = n0 + 1;
+  T t[n0];
+  if (active && !dc_only) {
+    T *row = tile + lane * S;
+    bool zero = true;
+#pragma unroll
+    for (int k = 0; k < n0; k++) {
+      t[k] = row[k];
+      zero = zero && t[k] == 0;
+    }
+    if (!zero || lossless) {
+      tx1d<L, n>(t, (tx_type & 2) == 0, lossless, 2);
+#pragma unroll
+      for (int k = 0; k < n0; k++) row[k] = t[k];
+    }
+  }
+  __syncthreads();
+  if (active && !dc_only) {
+#pragma unroll
+    for (int k = 0; k < n0; k++) t[k] = tile[k * S + lane];
+    tx1d<L, n>(t, (tx_type & 1) == 0, lossless, 0);
+    constexpr int shift = n + 2 < 6 ? n + 2 : 6;
+#pragma unroll
+    for (int k = 0; k < n0; k++) {
+      const T v = lossless ? t[k] : L::rnd(t[k], shift);
+      tile[k * S + lane] = (T)lclamp((int64_t)v, -65536, 65536);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    int32_t r[n0];
+#pragma unroll
+    for (int k = 0; k < n0; k++) r[k] = dc_only ? iclamp(dc_v, -65536, 65536) : (int32_t)tile[lane * S + k];
+    if (res)
+      store_res_row<n0>(res + (size_t)lane * stride, r);
+    else
+      add_row<P, n0>(base + (size_t)lane * stride, r, max);
+  }
+}
+
 // Workgroup wg of size class n: per frame of the batch, its range's
 // workgroups take BPW consecutive jobs each (count = the frame's jobs of
 // this size); tiles: BPW * n0 * (n0 + 1) elements of LDS.
@@ -54,12 +146,12 @@ DEV void itx_inter_wg(const ItxArgs *fa, const Range *rg, uint32_t nf, uint32_t 
   const bool dc_only = active && (t.eob == 0 || (!HBD && t.eob == 1 && t.tx_type == DCT_DCT && !lossless));
   int32_t dc_v = 0;
   if (dc_only && t.eob) dc_v = dc_only_value((int32_t)tile[0], n + 0);
-  ItxDst<P> dst;
-  dst.stride = a.cur.stride[t.plane];
-  dst.base = (P *)a.cur.p[t.plane] + (size_t)t.y * dst.stride + t.x;
-  dst.res = intra ? a.res[t.plane] + (size_t)t.y * dst.stride + t.x : nullptr;
+  const uint32_t stride = a.cur.stride[t.plane];
+  P *base = (P *)a.cur.p[t.plane] + (size_t)t.y * stride + t.x;
+  int16_t *res = intra ? a.res[t.plane] + (size_t)t.y * stride + t.x : nullptr;
   const int max = (1 << a.g.bit_depth) - 1;
-  itx_2d<L, n>(tile, active ? lane : n0, t.tx_type, lossless, dc_only, dc_v, dst, max);
+  itx_2d_rows<L, n, P>(tile, lane, active, t.tx_type, lossless, dc_only, dc_v, base, res, stride,
+                       dc_only && !HBD ? 255 : max);
 }
 
 // All four size classes in one launch (they are independent): workgroups
@@ -80,32 +172,6 @@ __global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range
     itx_inter_wg<3, HBD>(fa, r8, nf, w - b8, tiles);
   else
     itx_inter_wg<2, HBD>(fa, r4, nf, w - b16, tiles);
-}
-
-// Adds residuals r[0..n0) to the n0 samples at `row`, clamped to [0, max]:
-// as dwords when the row is 4-byte aligned, else sample by sample.
-template <class P, int n0>
-DEV void add_row(P *row, const int32_t *r, int max) {
-  if (((uintptr_t)row & 3) == 0) {
-    constexpr int PER = 4 / (int)sizeof(P), ND = n0 / PER;
-    uint32_t w[ND];
-#pragma unroll
-    for (int d = 0; d < ND; d++) w[d] = gld((const uint32_t *)row + d);
-#pragma unroll
-    for (int d = 0; d < ND; d++) {
-      uint32_t o = 0;
-#pragma unroll
-      for (int e = 0; e < PER; e++) {
-        const int sh = 32 / PER * e;
-        const int v = iclamp((int)((w[d] >> sh) & (0xffffffffu >> (32 - 32 / PER))) + r[d * PER + e], 0, max);
-        o |= (uint32_t)v << sh;
-      }
-      gst((uint32_t *)row + d, o);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < n0; k++) row[k] = (P)iclamp((int)row[k] + r[k], 0, max);
-  }
 }
 
 // Per-kernel seam: dense dequantised coefficients per block.  A team of n0
