@@ -266,9 +266,9 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
  * plane pointers, all with row stride `ref_stride` and bottom-right sample
  * (last_x, last_y)), and, when ref2_idx >= 0, averages with a second
  * prediction at (x2, y2).  Writes to out + out_off[i] samples with stride
- * out_stride.  `desc` is a host array (the wrapper expands it into 16 x 16
- * tiles); `refs` is a host array of device pointers; `out` is a device
- * pointer. */
+ * out_stride.  `desc` is a host array (the wrapper expands it into tiles of
+ * 16 columns by up to 32 rows); `refs` is a host array of device pointers;
+ * `out` is a device pointer. */
 typedef struct vp9r_mc_desc {
   int32_t x, y, x2, y2;
   int16_t xs, ys;

@@ -61,19 +61,22 @@ struct McShared {
   int16_t hp[FPH * TILE];  // after the horizontal pass
 };
 
-// The 8-bit unscaled path's staging: footprint rows of FP8 dwords (bytes
-// from a dword-aligned column; the pitch of 10 puts the four row groups a
-// read touches in distinct banks), and the horizontal pass column-major
-// (column c holds rows 0..23 in 6 dwords).
+// The 8-bit unscaled path's staging for tiles of up to MAXH rows: footprint
+// rows of FP8 dwords (bytes from a dword-aligned column; the pitch of 10
+// puts the row groups a read touches in distinct banks), and the horizontal
+// pass column-major (column c holds rows 0 .. MAXH + 7 in HP dwords).
 constexpr int FP8 = 10;
+template <int MAXH>
 struct Mc8Shared {
-  uint32_t fp[24 * FP8];
-  uint32_t hp[16 * 6];
+  static constexpr int ROWS = MAXH + 8, HP = ROWS / 4;
+  uint32_t fp[ROWS * FP8];
+  uint32_t hp[16 * HP];
 };
 
 union McLds {
   McShared g;
-  Mc8Shared f;
+  Mc8Shared<16> f16;
+  Mc8Shared<32> f32;
 };
 
 // Byte-wise rounding average of four packed samples: (a + b + 1) >> 1
@@ -122,14 +125,17 @@ DEV uint32_t four_samples(const uint32_t *w, int s, bool filter, uint32_t f0, ui
   return pack4(v[0], v[1], v[2], v[3]);
 }
 
-// The 8-bit tile of tw x th samples (multiples of 4, at most 16) whose first
-// sample is at (x0, y0) (1/16 sample, steps 16) from `ref` (bottom-right
-// sample (last_x, last_y)).  Returns the lane's output dword: row lane / 4,
-// columns 4 (lane % 4) .. + 3 (meaningful when inside the tile).  A footprint
-// wholly inside the plane is read as dwords; otherwise sample by sample at
-// clamped coordinates.  All 64 lanes call it (it holds barriers).
-DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int tw, int th,
-                      int filter, Mc8Shared &sh) {
+// The 8-bit tile of tw x th samples (multiples of 4; tw at most 16, th at
+// most MAXH, 16 or 32) whose first sample is at (x0, y0) (1/16 sample, steps
+// 16) from `ref` (bottom-right sample (last_x, last_y)).  res[h] is the
+// lane's output dword of row 16 h + lane / 4, columns 4 (lane % 4) .. + 3
+// (meaningful when inside the tile).  A footprint wholly inside the plane is
+// read as dwords; otherwise sample by sample at clamped coordinates.  All 64
+// lanes call it (it holds barriers).
+template <int MAXH>
+DEV void mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int tw, int th,
+                  int filter, Mc8Shared<MAXH> &sh, uint32_t (&res)[MAXH / 16]) {
+  constexpr int HP = Mc8Shared<MAXH>::HP, LR = (MAXH + 7 + 7) / 8;  // load rounds of 8 rows
   const int lane = threadIdx.x;
   const int px = x0 & 15, py = y0 & 15;
   const bool fx = px != 0, fy = py != 0;
@@ -141,15 +147,17 @@ DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_
   // footprint: lane (row lane / 8 + 8 i, dword lane % 8), all loads issued
   // before the first store; the horizontal pass's input is biased by -128
   const int dd = lane & 7, r0 = lane >> 3;
-  uint32_t w[3] = {0, 0, 0};
+  uint32_t w[LR];
+#pragma unroll
+  for (int i = 0; i < LR; i++) w[i] = 0;
   if (inside) {
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < LR; i++)
       if (dd < nd && r0 + 8 * i < ih)
         w[i] = gld((const uint32_t *)(ref + (size_t)(fy0 + r0 + 8 * i) * stride + ax + 4 * dd));
   } else {
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < LR; i++)
       if (dd < nd && r0 + 8 * i < ih) {
         const uint8_t *row = ref + (size_t)iclamp(fy0 + r0 + 8 * i, 0, last_y) * stride;
 #pragma unroll
@@ -158,34 +166,36 @@ DEV uint32_t mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_
   }
   const uint32_t flip = fx ? 0x80808080u : 0u;
 #pragma unroll
-  for (int i = 0; i < 3; i++)
+  for (int i = 0; i < LR; i++)
     if (dd < nd && r0 + 8 * i < ih) sh.fp[(r0 + 8 * i) * FP8 + dd] = w[i] ^ flip;
   __syncthreads();
   // horizontal: lane task t (column t % 16, rows 4 (t / 16) .. + 3) for
-  // t = lane, lane + 64.  Its results are biased when filtered (fx); the
+  // t = lane + 64 h.  Its results are biased when filtered (fx); the
   // vertical pass wants them biased when it filters (fy)
   const uint32_t f0 = c_taps8[filter][px][0], f1 = c_taps8[filter][px][1];
   const uint32_t vflip = fx != fy ? 0x80808080u : 0u;
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
+  for (int h = 0; h < (16 * HP + 63) / 64; h++) {
     const int t = lane + 64 * h, c = t & 15, rq = t >> 4;
-    if (t < 96 && c < tw && 4 * rq < ih) {
+    if (t < 16 * HP && c < tw && 4 * rq < ih) {
       const int o = off + c;
-      sh.hp[c * 6 + rq] = four_samples<FP8>(sh.fp + 4 * rq * FP8 + (o >> 2), o & 3, fx, f0, f1) ^ vflip;
+      sh.hp[c * HP + rq] = four_samples<FP8>(sh.fp + 4 * rq * FP8 + (o >> 2), o & 3, fx, f0, f1) ^ vflip;
     }
   }
   __syncthreads();
-  // vertical: lane task (row r, columns 4 q .. 4 q + 3)
-  const int r = lane >> 2, q = lane & 3;
-  uint32_t res = 0;
-  if (r < th && 4 * q < tw) {
-    const uint32_t g0 = c_taps8[filter][py][0], g1 = c_taps8[filter][py][1];
-    // four_samples reads columns 4 q .. 4 q + 3 as its "rows" (pitch 6);
+  // vertical: lane task (row 16 h + lane / 4, columns 4 q .. 4 q + 3)
+  const int q = lane & 3;
+  const uint32_t g0 = c_taps8[filter][py][0], g1 = c_taps8[filter][py][1];
+#pragma unroll
+  for (int h = 0; h < MAXH / 16; h++) {
+    const int r = 16 * h + (lane >> 2);
+    res[h] = 0;
+    // four_samples reads columns 4 q .. 4 q + 3 as its "rows" (pitch HP);
     // filtered results are biased
-    res = four_samples<6>(sh.hp + 4 * q * 6 + (r >> 2), r & 3, fy, g0, g1) ^ (fy ? 0x80808080u : 0u);
+    if (r < th && 4 * q < tw)
+      res[h] = four_samples<HP>(sh.hp + 4 * q * HP + (r >> 2), r & 3, fy, g0, g1) ^ (fy ? 0x80808080u : 0u);
   }
   __syncthreads();
-  return res;
 }
 
 // Predicts the tw x th (<= 16 x 16) tile whose first sample is at (x0, y0)
@@ -266,14 +276,17 @@ __global__ __launch_bounds__(64) void k_mc(const McArgs *fa, const Range *rg, ui
   const int r0 = b.ref_frame[0] - LAST_FRAME, r1 = compound ? b.ref_frame[1] - LAST_FRAME : r0;
   if (!HBD && ((t.tw | t.th | stride | (t.x + t.tx)) & 3) == 0 && rp0.xs == 16 && rp0.ys == 16 &&
       rp1.xs == 16 && rp1.ys == 16) {
-    uint32_t v = mc_tile8((const uint8_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
-                          rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f);
-    if (compound)
-      v = avg4(v, mc_tile8((const uint8_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x,
-                           rp1.last_y, rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f));
+    uint32_t v[1], v2[1];
+    mc_tile8<16>((const uint8_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
+                 rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f16, v);
+    if (compound) {
+      mc_tile8<16>((const uint8_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
+                   rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f16, v2);
+      v[0] = avg4(v[0], v2[0]);
+    }
     const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
     if (r < t.th && 4 * q < t.tw)
-      gst((uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q), v);
+      gst((uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q), v[0]);
     return;
   }
   int out[2][4];
@@ -302,6 +315,30 @@ struct DescTile {
   uint16_t tx, ty;
 };
 
+// Rows ty .. ty + th - 1 (th <= 16) of the tile at column tx of a
+// descriptor through the general path, written at dst (the tile's first
+// sample of those rows).
+template <class P>
+DEV void mc_desc_half(const vp9r_mc_desc &d, const void *const *refs, uint32_t ref_stride, int32_t last_x,
+                      int32_t last_y, P *dst, uint32_t out_stride, int tx, int ty, int tw, int th, int max,
+                      McShared &sh) {
+  const bool compound = d.ref2_idx >= 0;
+  int o0[4], o1[4];
+  mc_tile<P>((const P *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + d.xs * tx, d.y + d.ys * ty, d.xs, d.ys,
+             tw, th, d.filter, max, sh, o0);
+  if (compound)
+    mc_tile<P>((const P *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + d.xs * tx, d.y2 + d.ys * ty, d.xs,
+               d.ys, tw, th, d.filter, max, sh, o1);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int o = threadIdx.x + 64 * i;
+    if (o < th * tw) {
+      const int r = o / tw, c = o - r * tw;
+      dst[(size_t)r * out_stride + c] = (P)(compound ? (o0[i] + o1[i] + 1) >> 1 : o0[i]);
+    }
+  }
+}
+
 template <bool HBD>
 __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const DescTile *tiles,
                                                 const void *const *refs, uint32_t ref_stride,
@@ -312,34 +349,31 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
   const DescTile t = tiles[blockIdx.x];
   const vp9r_mc_desc d = descs[t.desc];
   const int max = (1 << bd) - 1;
-  const int th = d.h - t.ty < TILE ? d.h - t.ty : TILE, tw = d.w - t.tx < TILE ? d.w - t.tx : TILE;
+  const int th = d.h - t.ty < 2 * TILE ? d.h - t.ty : 2 * TILE, tw = d.w - t.tx < TILE ? d.w - t.tx : TILE;
   const bool compound = d.ref2_idx >= 0;
   P *dst = (P *)out + d.out_off + (size_t)t.ty * out_stride + t.tx;
   if (!HBD && d.xs == 16 && d.ys == 16 && ((tw | th | out_stride | (uint32_t)(uintptr_t)dst) & 3) == 0) {
-    uint32_t v = mc_tile8((const uint8_t *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + 16 * t.tx,
-                          d.y + 16 * t.ty, tw, th, d.filter, sh.f);
-    if (compound)
-      v = avg4(v, mc_tile8((const uint8_t *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + 16 * t.tx,
-                           d.y2 + 16 * t.ty, tw, th, d.filter, sh.f));
-    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-    if (r < th && 4 * q < tw) gst((uint32_t *)(dst + (size_t)r * out_stride + 4 * q), v);
+    uint32_t v[2], v2[2];
+    mc_tile8<32>((const uint8_t *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
+                 th, d.filter, sh.f32, v);
+    if (compound) {
+      mc_tile8<32>((const uint8_t *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + 16 * t.tx,
+                   d.y2 + 16 * t.ty, tw, th, d.filter, sh.f32, v2);
+      v[0] = avg4(v[0], v2[0]);
+      v[1] = avg4(v[1], v2[1]);
+    }
+    const int q = threadIdx.x & 3;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int r = 16 * h + (threadIdx.x >> 2);
+      if (r < th && 4 * q < tw) gst((uint32_t *)(dst + (size_t)r * out_stride + 4 * q), v[h]);
+    }
     return;
   }
-  int o0[4], o1[4];
-  mc_tile<P>((const P *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + d.xs * t.tx, d.y + d.ys * t.ty,
-             d.xs, d.ys, tw, th, d.filter, max, sh.g, o0);
-  if (compound)
-    mc_tile<P>((const P *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + d.xs * t.tx,
-               d.y2 + d.ys * t.ty, d.xs, d.ys, tw, th, d.filter, max, sh.g, o1);
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int o = threadIdx.x + 64 * i;
-    if (o < th * tw) {
-      const int r = o / tw, c = o - r * tw;
-      const int v = compound ? (o0[i] + o1[i] + 1) >> 1 : o0[i];
-      ((P *)out)[d.out_off + (size_t)(t.ty + r) * out_stride + t.tx + c] = (P)v;
-    }
-  }
+  // the general path: the tile's rows in halves of at most 16
+  for (int hh = 0; hh < th; hh += TILE)
+    mc_desc_half<P>(d, refs, ref_stride, last_x, last_y, dst + (size_t)hh * out_stride, out_stride, t.tx,
+                    t.ty + hh, tw, th - hh < TILE ? th - hh : TILE, max, sh.g);
 }
 
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {

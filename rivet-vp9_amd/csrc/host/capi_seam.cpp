@@ -130,7 +130,9 @@ int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const v
       set_error("motion compensation descriptor out of range");
       return VP9R_ERR_INVALID_INPUT;
     }
-    for (int ty = 0; ty < d.h; ty += 16)
+    // tiles of 16 columns by up to 32 rows (the kernel's 8-bit unscaled
+    // path filters the 32 rows in one pass; other paths take two halves)
+    for (int ty = 0; ty < d.h; ty += 32)
       for (int tx = 0; tx < d.w; tx += 16) tiles.push_back({i, (uint16_t)tx, (uint16_t)ty});
   }
   TRY(hipSetDevice(g->device), "hipSetDevice");
