@@ -405,7 +405,7 @@ class DevBuffer:
     def read(self, dtype, count):
         import numpy as np
         out = np.empty(count, dtype=dtype)
-        _check(library().vp9r_gpu_to_host(self.gpu.handle, out.ctypes.data_as(ctypes.c_void_p), self.ptr,
+        _check(library().vp9r_gpu_to_host(self._g, out.ctypes.data_as(ctypes.c_void_p), self.ptr,
                                           out.nbytes))
         return out
 
@@ -413,7 +413,7 @@ class DevBuffer:
         import numpy as np
         a = np.ascontiguousarray(arr)
         assert a.nbytes <= self.nbytes
-        _check(library().vp9r_gpu_to_device(self.gpu.handle, self.ptr, a.ctypes.data_as(ctypes.c_void_p),
+        _check(library().vp9r_gpu_to_device(self._g, self.ptr, a.ctypes.data_as(ctypes.c_void_p),
                                             a.nbytes))
 
 
@@ -444,6 +444,8 @@ class GpuFrame:
         self._f = ctypes.c_void_p()
         _check(library().vp9r_gpu_frame_new(gpu.handle, width, height, bit_depth, ss_x, ss_y,
                                             ctypes.byref(self._f)))
+        # the context stays valid while this frame store holds its reference
+        self._g = ctypes.c_void_p(gpu.handle.value)
 
     def __del__(self):
         if getattr(self, "_f", None):
@@ -465,7 +467,7 @@ class GpuFrame:
     def packed(self):
         n = library().vp9r_packed_size(self.width, self.height, self.bit_depth, self.ss_x, self.ss_y)
         buf = (ctypes.c_uint8 * n)()
-        _check(library().vp9r_gpu_download_packed(self.gpu.handle, self._f, buf))
+        _check(library().vp9r_gpu_download_packed(self._g, self._f, buf))
         return bytes(buf)
 
     def packed_async(self, host):
@@ -473,7 +475,7 @@ class GpuFrame:
         stream; wait() ends it."""
         n = library().vp9r_packed_size(self.width, self.height, self.bit_depth, self.ss_x, self.ss_y)
         assert host.nbytes >= n
-        _check(library().vp9r_gpu_download_packed_async(self.gpu.handle, self._f, host.ptr))
+        _check(library().vp9r_gpu_download_packed_async(self._g, self._f, host.ptr))
 
     def wait(self):
         _check(library().vp9r_gpu_frame_wait(self._f))

@@ -254,3 +254,30 @@ def test_lf_edges_matches_oracle(gpu, bd):
         assert st == 0
         got = d.read(dt, w * h).reshape(h, w).astype(np.uint16)
         assert np.array_equal(got, want), f"it={it} bd={bd}"
+
+
+def test_context_outlives_its_frames_and_buffers():
+    """Frame stores and device buffers keep their context alive: releasing
+    the context first, then the objects made from it, is safe, and they
+    still work in between."""
+    import gc
+    lib = vp.library()
+    g = vp.Gpu(0)
+    f = g.frame(64, 64, 8, 1, 1)
+    b = g.upload(np.arange(256, dtype=np.uint8))
+    handle = ctypes.c_void_p(g.handle.value)
+    g._g = None  # the wrapper forgets it; free the caller's reference now
+    lib.vp9r_gpu_free(handle)
+    assert np.array_equal(b.read(np.uint8, 256), np.arange(256, dtype=np.uint8))
+    s, r = f.geometry(0)
+    f.upload_plane(0, np.full((r, s), 7, dtype=np.uint16))
+    assert (f.download_plane(0) == 7).all()
+    assert len(f.packed()) == vp.library().vp9r_packed_size(64, 64, 8, 1, 1)
+    del f, b
+    gc.collect()
+
+
+def test_seam_time_needs_a_timed_call():
+    g = vp.Gpu(0)
+    with pytest.raises(vp.InvalidInputError):
+        g.seam_ms()
