@@ -33,20 +33,24 @@ namespace vp9g {
 __constant__ int16_t c_filters[4][16][8];
 // the taps as signed bytes, taps 0..3 then 4..7 (phase 0 unused)
 __constant__ uint32_t c_taps8[4][16][2];
+// the taps as signed 16-bit pairs (taps 2k, 2k + 1 in dword k)
+__constant__ uint32_t c_taps16[4][16][4];
 
 static bool filters_ready = false;
 static void upload_filters() {
   if (filters_ready) return;
   int16_t f[4][16][8];
-  uint32_t t8[4][16][2] = {};
+  uint32_t t8[4][16][2] = {}, t16[4][16][4] = {};
   for (int a = 0; a < 4; a++)
     for (int b = 0; b < 16; b++)
       for (int c = 0; c < 8; c++) {
         f[a][b][c] = (int16_t)VP9_SUBPEL_FILTERS[a][b][c];
         t8[a][b][c >> 2] |= (uint32_t)(uint8_t)(int8_t)VP9_SUBPEL_FILTERS[a][b][c] << (8 * (c & 3));
+        t16[a][b][c >> 1] |= (uint32_t)(uint16_t)(int16_t)VP9_SUBPEL_FILTERS[a][b][c] << (16 * (c & 1));
       }
   hipMemcpyToSymbol(HIP_SYMBOL(c_filters), f, sizeof f);
   hipMemcpyToSymbol(HIP_SYMBOL(c_taps8), t8, sizeof t8);
+  hipMemcpyToSymbol(HIP_SYMBOL(c_taps16), t16, sizeof t16);
   filters_ready = true;
 }
 
@@ -73,10 +77,23 @@ struct Mc8Shared {
   uint32_t hp[16 * HP];
 };
 
+// The 10/12-bit unscaled path's staging: footprint rows of FP16 dwords (two
+// samples each, from an even column), the horizontal pass column-major in
+// sample pairs (column c holds rows 0 .. MAXH + 7 in HW dwords).
+constexpr int FP16 = 13;
+template <int MAXH>
+struct Mc16Shared {
+  static constexpr int ROWS = MAXH + 8, HW = ROWS / 2;
+  uint32_t fp[ROWS * FP16];
+  uint32_t hp[16 * HW];
+};
+
 union McLds {
   McShared g;
   Mc8Shared<16> f16;
   Mc8Shared<32> f32;
+  Mc16Shared<16> h16;
+  Mc16Shared<32> h32;
 };
 
 // Byte-wise rounding average of four packed samples: (a + b + 1) >> 1
@@ -198,6 +215,115 @@ DEV void mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, i
   __syncthreads();
 }
 
+// Per-sample rounding average of two packed pairs: (a + b + 1) >> 1.
+DEV uint32_t avg2(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) >> 1) & 0x7fff7fffu); }
+
+typedef short McV2 __attribute__((ext_vector_type(2)));
+
+// One 8-tap sum over samples s .. s + 7 (s = 0 or 1) of the pairs
+// w0 .. w4, rounded by 7 and clamped to [0, max].
+DEV uint32_t tap8w(const uint32_t (&w)[5], int s, const uint32_t (&f)[4], int max) {
+  int sum = 64;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t d = __builtin_amdgcn_alignbyte(w[k + 1], w[k], 2 * s);
+    sum = __builtin_amdgcn_sdot2(__builtin_bit_cast(McV2, d), __builtin_bit_cast(McV2, f[k]), sum, false);
+  }
+  uint32_t v;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(v) : "v"(sum >> 7), "v"(max));
+  return v;
+}
+
+// Four samples as two dwords (first pair, second pair): filtered (taps f)
+// or plain, at sample offset s of rows w, w + pitch, ... (rows j: dwords
+// w[j * pitch] ..).
+template <int PITCH>
+DEV void four_samples16(const uint32_t *w, int s, bool filter, const uint32_t (&f)[4], int max, uint32_t (&out)[2]) {
+  uint32_t v[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (filter) {
+      uint32_t a[5];
+#pragma unroll
+      for (int m = 0; m < 5; m++) a[m] = w[j * PITCH + m];
+      v[j] = tap8w(a, s, f, max);
+    } else {
+      v[j] = (__builtin_amdgcn_alignbyte(w[j * PITCH + 1], w[j * PITCH], 2 * s)) & 0xffffu;
+    }
+  }
+  out[0] = v[0] | v[1] << 16;
+  out[1] = v[2] | v[3] << 16;
+}
+
+// The 10/12-bit tile of tw x th samples (multiples of 4; tw at most 16, th
+// at most MAXH) at (x0, y0) (1/16 sample, steps 16): the 8-bit path's
+// structure on sample pairs, each 8-tap sum four v_dot2_i32_i16 (samples
+// and taps fit signed 16 bits, the sums 32).  res[h][0..1] are the lane's
+// four samples of row 16 h + lane / 4, columns 4 (lane % 4) .. + 3.
+template <int MAXH>
+DEV void mc_tile16(const uint16_t *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int tw, int th,
+                   int filter, int max, Mc16Shared<MAXH> &sh, uint32_t (&res)[MAXH / 16][2]) {
+  constexpr int HW = Mc16Shared<MAXH>::HW, LR = (MAXH + 7 + 3) / 4;  // load rounds of 4 rows
+  const int lane = threadIdx.x;
+  const int px = x0 & 15, py = y0 & 15;
+  const bool fx = px != 0, fy = py != 0;
+  const int fx0 = (x0 >> 4) - (fx ? 3 : 0), fy0 = (y0 >> 4) - (fy ? 3 : 0);
+  const int iw = tw + (fx ? 7 : 0), ih = th + (fy ? 7 : 0);
+  const int ax = fx0 & ~1, off = fx0 - ax, nd = (off + iw + 1) >> 1;
+  const bool inside = fx0 >= 0 && fy0 >= 0 && ax + 2 * nd - 1 <= last_x && fy0 + ih - 1 <= last_y &&
+                      ((uint32_t)(uintptr_t)ref & 3) == 0 && (stride & 1) == 0;
+  // footprint: lane (row lane / 16 + 4 i, dword lane % 16)
+  const int dd = lane & 15, r0 = lane >> 4;
+  uint32_t w[LR];
+#pragma unroll
+  for (int i = 0; i < LR; i++) w[i] = 0;
+  if (inside) {
+#pragma unroll
+    for (int i = 0; i < LR; i++)
+      if (dd < nd && r0 + 4 * i < ih) w[i] = gld((const uint32_t *)(ref + (size_t)(fy0 + r0 + 4 * i) * stride + ax + 2 * dd));
+  } else {
+#pragma unroll
+    for (int i = 0; i < LR; i++)
+      if (dd < nd && r0 + 4 * i < ih) {
+        const uint16_t *row = ref + (size_t)iclamp(fy0 + r0 + 4 * i, 0, last_y) * stride;
+        w[i] = (uint32_t)gld(row + iclamp(ax + 2 * dd, 0, last_x)) |
+               (uint32_t)gld(row + iclamp(ax + 2 * dd + 1, 0, last_x)) << 16;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < LR; i++)
+    if (dd < nd && r0 + 4 * i < ih) sh.fp[(r0 + 4 * i) * FP16 + dd] = w[i];
+  __syncthreads();
+  uint32_t f[4], g[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    f[k] = c_taps16[filter][px][k];
+    g[k] = c_taps16[filter][py][k];
+  }
+  // horizontal: lane task t (column t % 16, rows 4 (t / 16) .. + 3)
+#pragma unroll
+  for (int h = 0; h < (16 * (HW / 2) + 63) / 64; h++) {
+    const int t = lane + 64 * h, c = t & 15, rq = t >> 4;
+    if (t < 16 * (HW / 2) && c < tw && 4 * rq < ih) {
+      const int o = off + c;
+      uint32_t two[2];
+      four_samples16<FP16>(sh.fp + 4 * rq * FP16 + (o >> 1), o & 1, fx, f, max, two);
+      sh.hp[c * HW + 2 * rq] = two[0];
+      sh.hp[c * HW + 2 * rq + 1] = two[1];
+    }
+  }
+  __syncthreads();
+  // vertical: lane task (row 16 h + lane / 4, columns 4 q .. 4 q + 3)
+  const int q = lane & 3;
+#pragma unroll
+  for (int h = 0; h < MAXH / 16; h++) {
+    const int r = 16 * h + (lane >> 2);
+    res[h][0] = res[h][1] = 0;
+    if (r < th && 4 * q < tw) four_samples16<HW>(sh.hp + 4 * q * HW + (r >> 1), r & 1, fy, g, max, res[h]);
+  }
+  __syncthreads();
+}
+
 // Predicts the tw x th (<= 16 x 16) tile whose first sample is at (x0, y0)
 // (1/16 sample) with steps (xs, ys) from `ref`, whose bottom-right sample is
 // (last_x, last_y).  Output o = lane + 64 i (row o / tw, column o % tw) goes
@@ -289,6 +415,25 @@ __global__ __launch_bounds__(64) void k_mc(const McArgs *fa, const Range *rg, ui
       gst((uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q), v[0]);
     return;
   }
+  if (HBD && ((t.tw | t.th | (t.x + t.tx)) & 3) == 0 && (stride & 1) == 0 && ((uintptr_t)dst & 3) == 0 &&
+      rp0.xs == 16 && rp0.ys == 16 && rp1.xs == 16 && rp1.ys == 16) {
+    uint32_t v[1][2], v2[1][2];
+    mc_tile16<16>((const uint16_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
+                  rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, b.interp_filter, max, sh.h16, v);
+    if (compound) {
+      mc_tile16<16>((const uint16_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
+                    rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, b.interp_filter, max, sh.h16, v2);
+      v[0][0] = avg2(v[0][0], v2[0][0]);
+      v[0][1] = avg2(v[0][1], v2[0][1]);
+    }
+    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+    if (r < t.th && 4 * q < t.tw) {
+      uint32_t *o = (uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q);
+      gst(o, v[0][0]);
+      gst(o + 1, v[0][1]);
+    }
+    return;
+  }
   int out[2][4];
   mc_tile<P>((const P *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
              rp0.x + rp0.xs * t.tx, rp0.y + rp0.ys * t.ty, rp0.xs, rp0.ys, t.tw, t.th, b.interp_filter, max, sh.g,
@@ -367,6 +512,32 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
     for (int h = 0; h < 2; h++) {
       const int r = 16 * h + (threadIdx.x >> 2);
       if (r < th && 4 * q < tw) gst((uint32_t *)(dst + (size_t)r * out_stride + 4 * q), v[h]);
+    }
+    return;
+  }
+  if (HBD && d.xs == 16 && d.ys == 16 && ((tw | th) & 3) == 0 && (out_stride & 1) == 0 &&
+      ((uintptr_t)dst & 3) == 0) {
+    uint32_t v[2][2], v2[2][2];
+    mc_tile16<32>((const uint16_t *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + 16 * t.tx, d.y + 16 * t.ty,
+                  tw, th, d.filter, max, sh.h32, v);
+    if (compound) {
+      mc_tile16<32>((const uint16_t *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + 16 * t.tx,
+                    d.y2 + 16 * t.ty, tw, th, d.filter, max, sh.h32, v2);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        v[h][0] = avg2(v[h][0], v2[h][0]);
+        v[h][1] = avg2(v[h][1], v2[h][1]);
+      }
+    }
+    const int q = threadIdx.x & 3;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int r = 16 * h + (threadIdx.x >> 2);
+      if (r < th && 4 * q < tw) {
+        uint32_t *o = (uint32_t *)(dst + (size_t)r * out_stride + 4 * q);
+        gst(o, v[h][0]);
+        gst(o + 1, v[h][1]);
+      }
     }
     return;
   }
