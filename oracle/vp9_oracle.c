@@ -16,7 +16,9 @@
  *     and the MD5s.
  *   - loop filter: flat-step monotone / strong edge untouched (lf.rs:570-593),
  *     and the MD5s for the decisions.  Sharpness > 0: parity unpinned.
- *   - frame process: the MD5s only. */
+ *   - frame process: the MD5s only.
+ *   - orc_itx_add_batch, orc_mc_batch: equal to the per-block calls above
+ *     (tests/test_microbench.py). */
 #include "vp9_oracle.h"
 
 #include <stdlib.h>
