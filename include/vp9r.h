@@ -134,6 +134,10 @@ void vp9r_parser_free(vp9r_parser *p);
 int vp9r_parser_parse_frame(vp9r_parser *p, const uint8_t *data, size_t len, vp9r_record *out);
 /* decoder/mod.rs:183-188 */
 void vp9r_parser_set_max_pixels(vp9r_parser *p, uint64_t pixels);
+/* Threads that parse a frame's tile columns at once (decoder/mod.rs:171-181
+ * set_threads, as decode_tiles_parallel uses them, block.rs:1769-1866);
+ * 0 or 1: the calling thread only.  The record is the same with any number. */
+void vp9r_parser_set_threads(vp9r_parser *p, uint32_t threads);
 
 /* ------------------------------------------------------------------------ */
 /* GPU reconstruction (one device per context).                              */
@@ -338,6 +342,9 @@ typedef struct vp9r_frame {
 
 int vp9r_decoder_new(int device, vp9r_decoder **out);
 void vp9r_decoder_free(vp9r_decoder *d);
+/* Threads for the CPU parse (tile columns in parallel, as the crate's
+ * decode_tiles_parallel); 0 (the default) is one per hardware thread.  The
+ * output is the same with any number (decoder/mod.rs:171-181). */
 void vp9r_decoder_set_threads(vp9r_decoder *d, uint32_t threads);
 uint32_t vp9r_decoder_threads(const vp9r_decoder *d);
 void vp9r_decoder_set_max_pixels(vp9r_decoder *d, uint64_t pixels);
@@ -346,6 +353,10 @@ void vp9r_decoder_set_max_pixels(vp9r_decoder *d, uint64_t pixels);
 int vp9r_decoder_decode(vp9r_decoder *d, const uint8_t *packet, size_t len, uint32_t *n_frames);
 int vp9r_decoder_frame(const vp9r_decoder *d, uint32_t i, vp9r_frame *out);
 uint64_t vp9r_decoder_frames_decoded(const vp9r_decoder *d);
+/* Test hook, no counterpart in the reference: the next n frame stores the
+ * decoder asks for are refused (VP9R_ERR_DEVICE), to test the recovery of
+ * a reconstruction that fails after its frame was parsed. */
+void vp9r_decoder_test_fail_stores(vp9r_decoder *d, uint32_t n);
 
 /* ------------------------------------------------------------------------ */
 /* Multi-stream server: many independent streams decoded on one GPU.        */

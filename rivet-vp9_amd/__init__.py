@@ -148,6 +148,7 @@ def library():
     lib.vp9r_parser_new.restype = P
     lib.vp9r_parser_free.argtypes = [P]
     lib.vp9r_parser_set_max_pixels.argtypes = [P, u64]
+    lib.vp9r_parser_set_threads.argtypes = [P, u32]
     lib.vp9r_parser_parse_frame.argtypes = [P, ctypes.c_char_p, sz, ctypes.POINTER(Record)]
     lib.vp9r_packed_size.restype = sz
     lib.vp9r_packed_size.argtypes = [u32, u32, u32, u32, u32]
@@ -198,6 +199,7 @@ def library():
     lib.vp9r_decoder_frame.argtypes = [P, u32, ctypes.POINTER(FrameDesc)]
     lib.vp9r_decoder_frames_decoded.restype = u64
     lib.vp9r_decoder_frames_decoded.argtypes = [P]
+    lib.vp9r_decoder_test_fail_stores.argtypes = [P, u32]
     lib.vp9r_server_new.argtypes = [ctypes.c_int, ctypes.POINTER(ServerConfig), PP]
     lib.vp9r_server_free.argtypes = [P]
     lib.vp9r_frame_store_bytes.restype = u64
@@ -242,6 +244,10 @@ class Parser:
 
     def set_max_pixels(self, pixels):
         library().vp9r_parser_set_max_pixels(self._p, pixels)
+
+    def set_threads(self, threads):
+        """Threads that parse tile columns at once (the record is the same)."""
+        library().vp9r_parser_set_threads(self._p, threads)
 
     def parse(self, frame):
         """Parses one frame; the returned Record is valid until the next call."""
@@ -555,6 +561,10 @@ class Decoder:
 
     def frames_decoded(self):
         return library().vp9r_decoder_frames_decoded(self._d)
+
+    def test_fail_stores(self, n):
+        """Test hook: refuse the next n frame stores (VP9R_ERR_DEVICE)."""
+        library().vp9r_decoder_test_fail_stores(self._d, n)
 
 
 class ServedFrame:

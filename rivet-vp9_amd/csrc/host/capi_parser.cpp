@@ -63,6 +63,8 @@ vp9r_parser *vp9r_parser_new(void) { return new (std::nothrow) vp9r_parser(); }
 void vp9r_parser_free(vp9r_parser *p) { delete p; }
 void vp9r_parser_set_max_pixels(vp9r_parser *p, uint64_t pixels) { p->p.max_pixels = pixels; }
 
+void vp9r_parser_set_threads(vp9r_parser *p, uint32_t threads) { p->p.tile_threads = threads ? threads : 1; }
+
 int vp9r_parser_parse_frame(vp9r_parser *p, const uint8_t *data, size_t len, vp9r_record *out) {
   try {
     p->p.parse_frame(data, len, out);

@@ -99,25 +99,68 @@ struct vp9r_decoder {
     if (st) return st;
     const vp9r_frame_hdr &h = rec.hdr;
     frames++;
-    if (h.show_existing_frame) return emit(slots[h.frame_to_show_map_idx]);
-    auto cur = take(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y, &st);
-    if (!cur) return st;
+    if (h.show_existing_frame) {
+      // the parser refuses slots it never filled; this one may also have
+      // been emptied by a failed reconstruction (forget_slots below)
+      if (!slots[h.frame_to_show_map_idx]) {
+        set_error("show_existing_frame names a slot that holds no frame");
+        return VP9R_ERR_BITSTREAM;
+      }
+      return emit(slots[h.frame_to_show_map_idx]);
+    }
+    st = reconstruct(rec);
+    if (st) {
+      // The parser has already refreshed its slots with this frame.  Empty
+      // them on both sides, so that no later frame predicts from or shows a
+      // store that does not hold this frame.
+      for (int i = 0; i < 8; i++)
+        if ((h.refresh_frame_flags >> i) & 1) slots[i].reset();
+      parser.p.forget_slots(h.refresh_frame_flags);
+      return st;
+    }
+    return h.show_frame ? emit(last) : VP9R_OK;
+  }
+
+  // Reconstruction and loop filter of a parsed frame into a store from the
+  // pool, then the slot refresh (decoder/mod.rs:398-415); `last` gets the
+  // store.
+  std::shared_ptr<FrameStore> last;
+  int reconstruct(const vp9r_record &rec) {
+    const vp9r_frame_hdr &h = rec.hdr;
+    int st = VP9R_OK;
+    vp9r_gpu_frame *refs[3] = {nullptr, nullptr, nullptr};
+    if (!h.frame_is_intra)
+      for (int i = 0; i < 3; i++) {
+        if (!slots[h.ref_frame_idx[i]]) {
+          set_error("active reference slot holds no frame");
+          return VP9R_ERR_BITSTREAM;
+        }
+        refs[i] = slots[h.ref_frame_idx[i]]->f;
+      }
+    auto cur = test_fail_stores > 0 ? nullptr : take(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y, &st);
+    if (!cur) {
+      if (test_fail_stores > 0) {
+        test_fail_stores--;
+        set_error("frame store allocation refused (test hook)");
+        return VP9R_ERR_DEVICE;
+      }
+      return st;
+    }
     cur->color_space = h.color_space;
     cur->color_range = h.color_range;
     cur->render_w = h.render_width;
     cur->render_h = h.render_height;
-    vp9r_gpu_frame *refs[3] = {nullptr, nullptr, nullptr};
-    if (!h.frame_is_intra)
-      for (int i = 0; i < 3; i++) refs[i] = slots[h.ref_frame_idx[i]] ? slots[h.ref_frame_idx[i]]->f : nullptr;
     st = vp9r_gpu_reconstruct(g, &rec, refs, cur->f);
     if (st) return st;
     st = vp9r_gpu_loop_filter(g, &rec, cur->f);
     if (st) return st;
     for (int i = 0; i < 8; i++)
       if ((h.refresh_frame_flags >> i) & 1) slots[i] = cur;
-    if (h.show_frame) return emit(cur);
+    last = cur;
     return VP9R_OK;
   }
+  // vp9r_decoder_test_fail_stores: frame store requests still to refuse
+  uint32_t test_fail_stores = 0;
 };
 
 extern "C" {
@@ -131,6 +174,7 @@ int vp9r_decoder_new(int device, vp9r_decoder **out) {
     delete d;
     return st;
   }
+  vp9r_decoder_set_threads(d, 0);
   *out = d;
   return VP9R_OK;
 }
@@ -144,12 +188,15 @@ void vp9r_decoder_free(vp9r_decoder *d) {
   delete d;
 }
 
-void vp9r_decoder_set_threads(vp9r_decoder *d, uint32_t threads) { d->threads = threads; }
-
 uint32_t vp9r_decoder_threads(const vp9r_decoder *d) {
   if (d->threads) return d->threads;
   unsigned n = std::thread::hardware_concurrency();
   return n ? n : 1;
+}
+
+void vp9r_decoder_set_threads(vp9r_decoder *d, uint32_t threads) {
+  d->threads = threads;
+  d->parser.p.tile_threads = vp9r_decoder_threads(d);
 }
 
 void vp9r_decoder_set_max_pixels(vp9r_decoder *d, uint64_t pixels) { d->parser.p.max_pixels = pixels; }
@@ -178,5 +225,7 @@ int vp9r_decoder_frame(const vp9r_decoder *d, uint32_t i, vp9r_frame *out) {
 }
 
 uint64_t vp9r_decoder_frames_decoded(const vp9r_decoder *d) { return d->frames; }
+
+void vp9r_decoder_test_fail_stores(vp9r_decoder *d, uint32_t n) { d->test_fail_stores = n; }
 
 }  // extern "C"

@@ -3,7 +3,9 @@
 #include "parser.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <thread>
 
 #include "vp9_consts.h"
 #include "vp9_tables.h"
@@ -21,7 +23,7 @@ class BitReader {
   BitReader(const uint8_t *d, size_t n) : d_(d), n_(n) {}
   uint32_t bit() {
     size_t byte = pos_ >> 3;
-    if (byte >= n_) bitstream("uncompressed header runs past the end of the frame");
+    if (byte >= n_) bitstream("frame data ends inside the uncompressed header");
     uint32_t b = (d_[byte] >> (7 - (pos_ & 7))) & 1;
     pos_++;
     return b;
@@ -54,9 +56,9 @@ class BoolDecoder {
  public:
   static constexpr bool kWrite = false;
   BoolDecoder(const uint8_t *d, size_t n) : p_(d), end_(d + n) {
-    if (n == 0) bitstream("boolean-coded partition of size 0");
+    if (n == 0) bitstream("bool-coded data has no bytes");
     refill();
-    if (read(128)) bitstream("boolean decoder marker bit is not 0");
+    if (read(128)) bitstream("bool decoder: the first decoded bit must be 0");
   }
   inline bool read(uint32_t prob) {
     uint32_t split = 1 + (((range_ - 1) * prob) >> 8);
@@ -236,7 +238,7 @@ enum { RESET_NONE = 0, RESET_ALL = 1, RESET_ONE = 2 };
 
 static void frame_sync_code(BitReader &r) {
   uint32_t a = r.f(8), b = r.f(8), c = r.f(8);
-  if (a != 0x49 || b != 0x83 || c != 0x42) bitstream("invalid frame sync code");
+  if (a != 0x49 || b != 0x83 || c != 0x42) bitstream("frame sync bytes are not 0x49 0x83 0x42");
 }
 
 static void color_config(BitReader &r, FrameHeader &h) {
@@ -247,8 +249,8 @@ static void color_config(BitReader &r, FrameHeader &h) {
     if (h.profile == 1 || h.profile == 3) {
       h.ss_x = r.f(1);
       h.ss_y = r.f(1);
-      if (r.flag()) bitstream("reserved_zero is not zero");
-      if (h.ss_x == 1 && h.ss_y == 1) bitstream("4:2:0 is not allowed in profiles 1 and 3");
+      if (r.flag()) bitstream("a reserved bit is set");
+      if (h.ss_x == 1 && h.ss_y == 1) bitstream("profiles 1 and 3 carry no 4:2:0 video");
     } else {
       h.ss_x = h.ss_y = 1;
     }
@@ -256,9 +258,9 @@ static void color_config(BitReader &r, FrameHeader &h) {
     h.color_range = true;
     if (h.profile == 1 || h.profile == 3) {
       h.ss_x = h.ss_y = 0;
-      if (r.flag()) bitstream("reserved_zero is not zero");
+      if (r.flag()) bitstream("a reserved bit is set");
     } else {
-      bitstream("sRGB is not allowed in profiles 0 and 2");
+      bitstream("profiles 0 and 2 carry no RGB video");
     }
   }
 }
@@ -287,10 +289,10 @@ void Parser::parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind
   FrameHeader &h = h_;
   h = FrameHeader();
   *reset_kind = RESET_NONE;
-  if (r.f(2) != 2) bitstream("frame_marker is not 2");
+  if (r.f(2) != 2) bitstream("the frame marker bits are not binary 10");
   uint32_t lo = r.f(1), hi = r.f(1);
   h.profile = (hi << 1) + lo;
-  if (h.profile == 3 && r.flag()) bitstream("reserved_zero is not zero");
+  if (h.profile == 3 && r.flag()) bitstream("a reserved bit is set");
   h.show_existing_frame = r.flag();
   if (h.show_existing_frame) {
     h.frame_to_show_map_idx = r.f(3);
@@ -341,7 +343,7 @@ void Parser::parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind
       for (int i = 0; i < 3; i++) {
         if (r.flag()) {
           uint32_t w = ref_sizes_[h.ref_frame_idx[i]][0], ht = ref_sizes_[h.ref_frame_idx[i]][1];
-          if (w == 0) bitstream("frame size taken from an empty reference slot");
+          if (w == 0) bitstream("size copied from a reference slot that holds no frame");
           h.width = w;
           h.height = ht;
           found = true;
@@ -711,7 +713,10 @@ class TileWalker {
  public:
   static constexpr bool W = Coder::kWrite;
   SyntaxPlan *plan = nullptr;
-  TileWalker(Parser &p) : P(p), h(p.h_), seg(p.seg_), probs(p.probs_), counts(p.counts_) {
+  // Records and symbol counts go to `out` (the parser's own record and
+  // counts, or one tile's part of them when tiles are parsed in parallel).
+  TileWalker(Parser &p, TileOut &out)
+      : P(p), h(p.h_), seg(p.seg_), probs(p.probs_), O(out), counts(out.counts) {
     uint32_t n4w = h.sb64_cols * 16 + 16, n4h = h.sb64_rows * 16 + 16;
     for (int i = 0; i < 3; i++) {
       above_nz[i].assign(n4w, 0);
@@ -752,6 +757,7 @@ class TileWalker {
   const FrameHeader &h;
   const SegParams &seg;
   const Probs &probs;
+  TileOut &O;
   Counts &counts;
   std::vector<uint8_t> above_nz[3], left_nz[3], above_part, left_part, above_segp, left_segp;
   uint32_t row_start = 0, row_end = 0, col_start = 0, col_end = 0;
@@ -793,7 +799,7 @@ class TileWalker {
       partition = PARTITION_SPLIT;
     if (!h.frame_is_intra) counts.partition[ctx][partition]++;
     int subsize = VP9_SUBSIZE_LOOKUP[partition][bsize];
-    if (subsize == BLOCK_INVALID) bitstream("invalid partition");
+    if (subsize == BLOCK_INVALID) bitstream("partition symbol out of range");
     if (subsize < BLOCK_8X8 || partition == PARTITION_NONE) {
       decode_block(d, r, c, subsize);
     } else if (partition == PARTITION_HORZ) {
@@ -821,7 +827,7 @@ class TileWalker {
   void decode_block(Coder &d, uint32_t r, uint32_t c, int subsize) {
     if (h.ss_x + h.ss_y > 0 && subsize >= BLOCK_8X8 &&
         VP9_SS_SIZE_LOOKUP[subsize][h.ss_x][h.ss_y] == BLOCK_INVALID)
-      bitstream("block size invalid for the chroma subsampling");
+      bitstream("this block size has no chroma block at this subsampling");
     memset(&b, 0, sizeof b);
     b.mi_row = r;
     b.mi_col = c;
@@ -842,7 +848,7 @@ class TileWalker {
       inter_frame_mode_info(d);
     eob_total = 0;
     // the record's block
-    uint32_t bi = (uint32_t)P.blocks_.size();
+    uint32_t bi = (uint32_t)O.blocks.size();
     vp9r_block rb;
     memset(&rb, 0, sizeof rb);
     rb.mi_row = (uint16_t)r;
@@ -861,7 +867,7 @@ class TileWalker {
     rb.avail_u = b.avail_u;
     rb.avail_l = b.avail_l;
     memcpy(rb.mv, b.block_mvs, sizeof rb.mv);
-    P.blocks_.push_back(rb);
+    O.blocks.push_back(rb);
     residual(d, bi);
     bool skip = b.skip;
     if (b.is_inter && subsize >= BLOCK_8X8 && eob_total == 0) skip = true;
@@ -1153,7 +1159,7 @@ class TileWalker {
   void read_ref_frames(Coder &d) {
     if (seg.active(b.segment_id, SEG_LVL_REF_FRAME)) {
       int rf = seg.feature_data[b.segment_id][SEG_LVL_REF_FRAME];
-      if (rf < 1 || rf > 3) bitstream("segment reference feature names no inter frame");
+      if (rf < 1 || rf > 3) bitstream("segment feature forces a reference on an intra-only frame");
       b.ref_frame[0] = (int8_t)rf;
       b.ref_frame[1] = REF_NONE;
       return;
@@ -1629,8 +1635,8 @@ class TileWalker {
     for (int i = 0; i < n; i++) {
       int rf = b.ref_frame[i];
       uint32_t ok = P.ref_ok_[rf - LAST_FRAME];
-      if (ok == 0) bitstream("missing reference");
-      if (ok == 2) bitstream("reference frame scaled beyond 2:1 / 1:16");
+      if (ok == 0) bitstream("inter block names a reference that is not loaded");
+      if (ok == 2) bitstream("reference size outside the 2x-down / 16x-up scaling range");
     }
   }
 
@@ -1641,7 +1647,7 @@ class TileWalker {
       int step = 1 << tx_sz;
       int sx = plane > 0 ? (int)h.ss_x : 0, sy = plane > 0 ? (int)h.ss_y : 0;
       int plane_sz = VP9_SS_SIZE_LOOKUP[bsize][sx][sy];
-      if (plane_sz == BLOCK_INVALID) bitstream("invalid chroma block size");
+      if (plane_sz == BLOCK_INVALID) bitstream("chroma plane has no block of this size");
       int n4w = VP9_NUM_4X4_WIDE[plane_sz], n4h = VP9_NUM_4X4_HIGH[plane_sz];
       uint32_t base_x = (b.mi_col * 8) >> sx, base_y = (b.mi_row * 8) >> sy;
       if (b.is_inter && plane == 0) check_refs();
@@ -1654,7 +1660,7 @@ class TileWalker {
           if (start_x < max_x && start_y < max_y) {
             int tt = tx_type(plane, tx_sz, block_idx);
             uint32_t eob = 0;
-            uint32_t cstart = (uint32_t)P.coef_pos_.size();
+            uint32_t cstart = (uint32_t)O.coef_pos.size();
             if (!b.skip) {
               eob = tokens(d, plane, start_x, start_y, tx_sz, tt);
               nonzero = eob > 0;
@@ -1663,8 +1669,8 @@ class TileWalker {
               for (uint32_t k = 0; k < eob; k++) {
                 int pos = sc[k];
                 if (coefs[pos]) {
-                  P.coef_pos_.push_back((uint16_t)pos);
-                  P.coef_val_.push_back(coefs[pos]);
+                  O.coef_pos.push_back((uint16_t)pos);
+                  O.coef_val.push_back(coefs[pos]);
                   coefs[pos] = 0;
                 }
               }
@@ -1678,10 +1684,10 @@ class TileWalker {
               t.tx_type = (uint8_t)tt;
               t._pad = 0;
               t.eob = (uint16_t)eob;
-              t.ncoef = (uint16_t)(P.coef_pos_.size() - cstart);
+              t.ncoef = (uint16_t)(O.coef_pos.size() - cstart);
               t.coef_start = cstart;
               t.block = bi;
-              P.txbs_.push_back(t);
+              O.txbs.push_back(t);
             }
           }
           uint8_t *an = above_nz[plane].data(), *ln = left_nz[plane].data();
@@ -1790,36 +1796,182 @@ Parser::Parser() {
 }
 
 bool Parser::decode_tiles(const uint8_t *data, size_t len) {
-  blocks_.clear();
-  txbs_.clear();
-  coef_pos_.clear();
-  coef_val_.clear();
-  memset(&counts_, 0, sizeof counts_);
   mi_.assign((size_t)h_.mi_rows * h_.mi_cols, MiInfo());
-  TileWalker<BoolDecoder> td(*this);
-  uint32_t tile_cols = 1u << h_.tile_cols_log2, tile_rows = 1u << h_.tile_rows_log2;
+  const uint32_t tile_cols = 1u << h_.tile_cols_log2, tile_rows = 1u << h_.tile_rows_log2;
+  // every tile's place in the data (each but the last is preceded by its
+  // 4-byte size), found before any tile is parsed
+  struct Span {
+    size_t pos, size;
+  };
+  std::vector<Span> span(tile_rows * tile_cols);
   size_t pos = 0;
-  for (uint32_t row = 0; row < tile_rows; row++)
-    for (uint32_t col = 0; col < tile_cols; col++) {
-      bool last = row == tile_rows - 1 && col == tile_cols - 1;
-      size_t size;
-      if (last) {
-        size = len - pos;
-      } else {
-        if (pos + 4 > len) bitstream("tile size runs past the end of the frame");
-        size = ((size_t)data[pos] << 24) | ((size_t)data[pos + 1] << 16) |
-               ((size_t)data[pos + 2] << 8) | data[pos + 3];
-        pos += 4;
-      }
-      if (pos + size > len) bitstream("tile runs past the end of the frame");
-      td.decode_tile(data + pos, size, row, col);
-      pos += size;
+  for (uint32_t i = 0; i < tile_rows * tile_cols; i++) {
+    size_t size;
+    if (i + 1 == tile_rows * tile_cols) {
+      size = len - pos;
+    } else {
+      if (pos + 4 > len) bitstream("tile length field reaches beyond the frame data");
+      size = ((size_t)data[pos] << 24) | ((size_t)data[pos + 1] << 16) | ((size_t)data[pos + 2] << 8) | data[pos + 3];
+      pos += 4;
     }
-  return td.padding_ok;
+    if (pos + size > len) bitstream("tile data reaches beyond the frame data");
+    span[i] = {pos, size};
+    pos += size;
+  }
+  const uint32_t workers = std::min(tile_threads ? tile_threads : 1u, tile_cols);
+  if (workers <= 1) {
+    // serial: straight into the record
+    TileOut out;
+    out.blocks.swap(blocks_);
+    out.txbs.swap(txbs_);
+    out.coef_pos.swap(coef_pos_);
+    out.coef_val.swap(coef_val_);
+    out.clear();
+    TileWalker<BoolDecoder> td(*this, out);
+    ParseError err{0, ""};
+    try {
+      for (uint32_t i = 0; i < tile_rows * tile_cols; i++)
+        td.decode_tile(data + span[i].pos, span[i].size, i / tile_cols, i % tile_cols);
+    } catch (const ParseError &e) {
+      err = e;
+    }
+    out.blocks.swap(blocks_);
+    out.txbs.swap(txbs_);
+    out.coef_pos.swap(coef_pos_);
+    out.coef_val.swap(coef_val_);
+    counts_ = out.counts;
+    if (err.code) throw err;
+    return td.padding_ok;
+  }
+  // Tile columns in parallel (the reference's decode_tiles_parallel,
+  // src/decoder/block.rs:1769-1866): a column's tiles read only mode info
+  // of their own columns (above context, left context reset per tile,
+  // motion vector candidates inside the tile's columns), so columns are
+  // independent; tile rows of one column stay in order.  Each tile writes
+  // its own part of the record; the parts are joined in the serial order
+  // (tile rows, then columns), and the symbol counts summed, so the record
+  // is the serial one.  The error reported is the first in that order.
+  tile_out_.resize(tile_rows * tile_cols);
+  if (!pool_ || pool_->threads() != workers) pool_.reset(new TaskPool(workers));
+  std::vector<ParseError> errs(tile_rows * tile_cols, ParseError{0, ""});
+  std::vector<uint8_t> pad(tile_cols, 1);
+  pool_->run(tile_cols, [&](uint32_t col) {
+    for (uint32_t row = 0; row < tile_rows; row++) {
+      const uint32_t i = row * tile_cols + col;
+      TileOut &o = tile_out_[i];
+      o.clear();
+      try {
+        TileWalker<BoolDecoder> td(*this, o);
+        td.decode_tile(data + span[i].pos, span[i].size, row, col);
+        pad[col] &= td.padding_ok;
+      } catch (const ParseError &e) {
+        errs[i] = e;
+        return;  // the later tiles of this column read this one's mode info
+      }
+    }
+  });
+  for (uint32_t i = 0; i < tile_rows * tile_cols; i++)
+    if (errs[i].code) throw errs[i];
+  // the parts' places in the record, then every part copied at once
+  const uint32_t n_tiles = tile_rows * tile_cols;
+  std::vector<size_t> b0(n_tiles + 1, 0), t0(n_tiles + 1, 0), c0(n_tiles + 1, 0);
+  for (uint32_t i = 0; i < n_tiles; i++) {
+    b0[i + 1] = b0[i] + tile_out_[i].blocks.size();
+    t0[i + 1] = t0[i] + tile_out_[i].txbs.size();
+    c0[i + 1] = c0[i] + tile_out_[i].coef_pos.size();
+  }
+  blocks_.resize(b0[n_tiles]);
+  txbs_.resize(t0[n_tiles]);
+  coef_pos_.resize(c0[n_tiles]);
+  coef_val_.resize(c0[n_tiles]);
+  pool_->run(n_tiles, [&](uint32_t i) {
+    const TileOut &o = tile_out_[i];
+    if (!o.blocks.empty()) memcpy(&blocks_[b0[i]], o.blocks.data(), sizeof(vp9r_block) * o.blocks.size());
+    for (size_t k = 0; k < o.txbs.size(); k++) {
+      vp9r_txb t = o.txbs[k];
+      t.block += (uint32_t)b0[i];
+      t.coef_start += (uint32_t)c0[i];
+      txbs_[t0[i] + k] = t;
+    }
+    if (!o.coef_pos.empty()) {
+      memcpy(&coef_pos_[c0[i]], o.coef_pos.data(), sizeof(uint16_t) * o.coef_pos.size());
+      memcpy(&coef_val_[c0[i]], o.coef_val.data(), sizeof(int32_t) * o.coef_val.size());
+    }
+  });
+  memset(&counts_, 0, sizeof counts_);
+  for (uint32_t i = 0; i < n_tiles; i++) {
+    uint32_t *dst = (uint32_t *)&counts_;
+    const uint32_t *src = (const uint32_t *)&tile_out_[i].counts;
+    for (size_t k = 0; k < sizeof(Counts) / sizeof(uint32_t); k++) dst[k] += src[k];
+  }
+  bool ok = true;
+  for (uint32_t c = 0; c < tile_cols; c++) ok = ok && pad[c];
+  return ok;
+}
+
+TaskPool::TaskPool(uint32_t threads) {
+  for (uint32_t t = 1; t < threads; t++) workers_.emplace_back([this] { work(); });
+}
+
+TaskPool::~TaskPool() {
+  {
+    std::lock_guard<std::mutex> l(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  for (auto &t : workers_) t.join();
+}
+
+void TaskPool::run(uint32_t n, const std::function<void(uint32_t)> &fn) {
+  std::unique_lock<std::mutex> l(mu_);
+  fn_ = &fn;
+  n_ = n;
+  next_ = 0;
+  gen_++;
+  cv_.notify_all();
+  // this thread takes indices too
+  while (next_ < n_) {
+    uint32_t i = next_++;
+    busy_++;
+    l.unlock();
+    fn(i);
+    l.lock();
+    busy_--;
+  }
+  done_cv_.wait(l, [&] { return busy_ == 0; });
+  fn_ = nullptr;
+}
+
+void TaskPool::work() {
+  std::unique_lock<std::mutex> l(mu_);
+  uint64_t seen = 0;
+  for (;;) {
+    cv_.wait(l, [&] { return stop_ || (gen_ != seen && fn_ && next_ < n_); });
+    if (stop_) return;
+    seen = gen_;
+    while (fn_ && next_ < n_) {
+      uint32_t i = next_++;
+      busy_++;
+      const std::function<void(uint32_t)> *fn = fn_;
+      l.unlock();
+      (*fn)(i);
+      l.lock();
+      busy_--;
+    }
+    if (busy_ == 0) done_cv_.notify_all();
+  }
+}
+
+void Parser::forget_slots(uint32_t mask) {
+  for (int i = 0; i < 8; i++)
+    if ((mask >> i) & 1) {
+      slots_[i] = SlotInfo();
+      ref_sizes_[i][0] = ref_sizes_[i][1] = 0;
+    }
 }
 
 void Parser::parse_frame(const uint8_t *data, size_t len, vp9r_record *out) {
-  if (len == 0) bitstream("empty frame");
+  if (len == 0) bitstream("zero-length frame");
   uint32_t saved_last_type = st_last_frame_type_;
   int reset_kind;
   parse_uncompressed(data, len, &reset_kind);
@@ -1828,7 +1980,7 @@ void Parser::parse_frame(const uint8_t *data, size_t len, vp9r_record *out) {
   if (h.show_existing_frame) {
     st_last_frame_type_ = saved_last_type;
     const SlotInfo &s = slots_[h.frame_to_show_map_idx];
-    if (!s.valid) bitstream("show_existing_frame of an empty slot");
+    if (!s.valid) bitstream("show_existing_frame names a slot that holds no frame");
     frames++;
     vp9r_frame_hdr &o = out->hdr;
     o.show_existing_frame = 1;
@@ -1845,7 +1997,7 @@ void Parser::parse_frame(const uint8_t *data, size_t len, vp9r_record *out) {
   size_t mi_len = (size_t)h.mi_rows * h.mi_cols;
   size_t comp_start = h.uncompressed_size;
   size_t comp_end = comp_start + h.header_size_in_bytes;
-  if (comp_end > len) bitstream("compressed header runs past the end of the frame");
+  if (comp_end > len) bitstream("frame data ends inside the compressed header");
   parse_compressed(data + comp_start, h.header_size_in_bytes);
   const uint8_t *tiles = data + comp_end;
   size_t tiles_len = len - comp_end;
@@ -1859,15 +2011,15 @@ void Parser::setup_frame(int reset_kind) {
   FrameHeader &h = h_;
   if ((uint64_t)h.width * h.height > max_pixels)
     fail(VP9R_ERR_UNSUPPORTED, "frame size is above the decoder's limit");
-  if (h.header_size_in_bytes == 0) bitstream("compressed header of size 0");
+  if (h.header_size_in_bytes == 0) bitstream("header_size_in_bytes is 0");
   // the active references
   for (int i = 0; i < 3; i++) ref_ok_[i] = 0;
   if (!h.frame_is_intra) {
     for (int i = 0; i < 3; i++) {
       const SlotInfo &s = slots_[h.ref_frame_idx[i]];
-      if (!s.valid) bitstream("reference to an empty slot");
+      if (!s.valid) bitstream("active reference slot holds no frame");
       if (s.ss_x != h.ss_x || s.ss_y != h.ss_y || s.bit_depth != h.bit_depth)
-        bitstream("reference frame of another format");
+        bitstream("active reference has another bit depth or subsampling");
       ref_ok_[i] = scale_ok(s.width, s.height, h.width, h.height) ? 1 : 2;
     }
   }
@@ -2152,7 +2304,13 @@ std::vector<uint8_t> Parser::write_frame(const uint8_t *uncompressed, size_t len
   coef_val_.clear();
   memset(&counts_, 0, sizeof counts_);
   mi_.assign((size_t)h_.mi_rows * h_.mi_cols, MiInfo());
-  TileWalker<BoolWriter> tw(*this);
+  TileOut wout;
+  wout.blocks.swap(blocks_);
+  wout.txbs.swap(txbs_);
+  wout.coef_pos.swap(coef_pos_);
+  wout.coef_val.swap(coef_val_);
+  wout.clear();
+  TileWalker<BoolWriter> tw(*this, wout);
   tw.plan = plan;
   std::vector<uint8_t> frame(uncompressed, uncompressed + len);
   // header_size_in_bytes (16 bits at header_size_pos)
@@ -2180,6 +2338,11 @@ std::vector<uint8_t> Parser::write_frame(const uint8_t *uncompressed, size_t len
       }
       frame.insert(frame.end(), tw_d.out.begin(), tw_d.out.end());
     }
+  wout.blocks.swap(blocks_);
+  wout.txbs.swap(txbs_);
+  wout.coef_pos.swap(coef_pos_);
+  wout.coef_val.swap(coef_val_);
+  counts_ = wout.counts;
   return frame;
 }
 

@@ -10,8 +10,14 @@
 // which the record hands to the GPU.
 #pragma once
 
+#include <condition_variable>
 #include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "vp9r.h"
@@ -75,6 +81,44 @@ struct Counts {
   uint32_t mv_hp[2][2];
   uint32_t token[4][2][2][6][6][3];
   uint32_t more_coefs[4][2][2][6][6][2];
+};
+
+// A fixed set of worker threads that run the indices of a task together
+// with the calling thread (for the tile columns of a frame).
+class TaskPool {
+ public:
+  explicit TaskPool(uint32_t threads);  // threads - 1 workers
+  ~TaskPool();
+  uint32_t threads() const { return (uint32_t)workers_.size() + 1; }
+  // fn(i) for every i < n, on the workers and this thread; returns when all
+  // are done.  fn must not throw.
+  void run(uint32_t n, const std::function<void(uint32_t)> &fn);
+
+ private:
+  void work();
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(uint32_t)> *fn_ = nullptr;
+  uint32_t n_ = 0, next_ = 0, busy_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+// One tile's part of the record and its symbol counts.
+struct TileOut {
+  std::vector<vp9r_block> blocks;
+  std::vector<vp9r_txb> txbs;
+  std::vector<uint16_t> coef_pos;
+  std::vector<int32_t> coef_val;
+  Counts counts;
+  void clear() {
+    blocks.clear();
+    txbs.clear();
+    coef_pos.clear();
+    coef_val.clear();
+    memset(&counts, 0, sizeof counts);
+  }
 };
 
 struct LoopFilterParams {
@@ -180,6 +224,15 @@ class Parser {
   void parse_frame(const uint8_t *data, size_t len, vp9r_record *out);
   uint64_t max_pixels = 8192ull * 8192ull;
   uint64_t frames = 0;
+  // Threads that parse tile columns at once (1: the calling thread only).
+  // The record is the same with any number.
+  uint32_t tile_threads = 1;
+  // Marks the slots of `mask` empty.  A caller whose reconstruction of a
+  // parsed frame failed calls it with the frame's refresh_frame_flags, so
+  // that the parser's view of the slots stays the decoder's: a later frame
+  // that names those slots is then a bitstream error, as with slots never
+  // filled.
+  void forget_slots(uint32_t mask);
   // Stream generation: given a frame's uncompressed header (its
   // header_size_in_bytes any nonzero placeholder), writes the compressed
   // header (tx_mode and reference_mode as given, no probability updates) and
@@ -222,6 +275,8 @@ class Parser {
   std::vector<uint16_t> coef_pos_;
   std::vector<int32_t> coef_val_;
   std::vector<vp9r_mi> lf_mi_;
+  std::vector<TileOut> tile_out_;  // per tile, when tile columns are parsed in parallel
+  std::unique_ptr<TaskPool> pool_;  // tile_threads threads, made on first use
 
   void parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind);
   void setup_frame(int reset_kind);

@@ -694,7 +694,7 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
           return VP9R_ERR_INVALID_INPUT;
         }
         if (!ref_ok[rf - 1]) {
-          set_error("reference frame scaled beyond 2:1 / 1:16");
+          set_error("reference size outside the 2x-down / 16x-up scaling range");
           return VP9R_ERR_BITSTREAM;
         }
       }

@@ -488,7 +488,7 @@ struct vp9r_server {
         Store *s = st->slots[j->show_idx];
         if (!s) {
           std::lock_guard<std::mutex> lk(m);
-          fail_stream(st, VP9R_ERR_BITSTREAM, "show_existing_frame of an empty slot");
+          fail_stream(st, VP9R_ERR_BITSTREAM, "show_existing_frame names a slot that holds no frame");
           continue;
         }
         if (!add_output(b, j, s)) return VP9R_ERR_DEVICE;
