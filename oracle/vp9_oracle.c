@@ -21,6 +21,8 @@
  *     (tests/test_microbench.py). */
 #include "vp9_oracle.h"
 
+#include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -919,49 +921,158 @@ static void reconstruct(recon_ctx *R, const vp9r_block *b, const vp9r_txb *t) {
               stride);
 }
 
-int orc_reconstruct(const vp9r_record *rec, const orc_frame *const refs[3], orc_frame *cur) {
+/* One block of the replay (block.rs:1405-1498): for each plane, inter
+ * prediction of the whole block (or each 4x4 below 8x8), then its transform
+ * blocks from *cursor on, in raster order: intra prediction, then the
+ * residual when eob > 0. */
+static int recon_block(recon_ctx *R, uint32_t bi, uint32_t *cursor) {
+  const vp9r_record *rec = R->rec;
+  const vp9r_frame_hdr *h = R->h;
+  const vp9r_block *b = &rec->blocks[bi];
+  int bsize = b->mi_size < BLOCK_8X8 ? BLOCK_8X8 : b->mi_size;
+  int status = 0;
+  for (int plane = 0; plane < 3 && !status; plane++) {
+    int sx = plane ? h->ss_x : 0, sy = plane ? h->ss_y : 0;
+    int tx_sz = plane ? uv_tx_size(h, b) : b->tx_size;
+    int step = 1 << tx_sz;
+    int plane_sz = VP9_SS_SIZE_LOOKUP[bsize][sx][sy];
+    int n4w = VP9_NUM_4X4_WIDE[plane_sz], n4h = VP9_NUM_4X4_HIGH[plane_sz];
+    int base_x = (b->mi_col * 8) >> sx, base_y = (b->mi_row * 8) >> sy;
+    if (b->is_inter) {
+      if (b->mi_size < BLOCK_8X8) {
+        for (int y = 0; y < n4h && !status; y++)
+          for (int x = 0; x < n4w && !status; x++)
+            status = predict_inter(R, b, plane, base_x + 4 * x, base_y + 4 * y, 4, 4, y * n4w + x);
+      } else {
+        status = predict_inter(R, b, plane, base_x, base_y, n4w * 4, n4h * 4, 0);
+      }
+    }
+    int per_row = (n4w + step - 1) / step;
+    while (!status && *cursor < rec->n_txbs && rec->txbs[*cursor].block == bi && rec->txbs[*cursor].plane == plane) {
+      const vp9r_txb *t = &rec->txbs[(*cursor)++];
+      int xi = (t->x - base_x) >> 2, yi = (t->y - base_y) >> 2;
+      int block_idx = (yi / step) * per_row + xi / step;
+      if (!b->is_inter)
+        predict_intra(R, b, plane, t->x, t->y, b->avail_l || xi > 0, b->avail_u || yi > 0, xi + step < n4w, tx_sz,
+                      block_idx);
+      if (t->eob > 0) reconstruct(R, b, t);
+    }
+  }
+  return status;
+}
+
+static recon_ctx *recon_new(const vp9r_record *rec, const orc_frame *const refs[3], orc_frame *cur) {
   recon_ctx *R = (recon_ctx *)malloc(sizeof *R);
   R->rec = rec;
   R->h = &rec->hdr;
   R->refs = refs;
   R->cur = cur;
-  const vp9r_frame_hdr *h = R->h;
+  return R;
+}
+
+int orc_reconstruct(const vp9r_record *rec, const orc_frame *const refs[3], orc_frame *cur) {
+  recon_ctx *R = recon_new(rec, refs, cur);
   uint32_t cursor = 0;
   int status = 0;
-  for (uint32_t bi = 0; bi < rec->n_blocks && !status; bi++) {
-    const vp9r_block *b = &rec->blocks[bi];
-    int bsize = b->mi_size < BLOCK_8X8 ? BLOCK_8X8 : b->mi_size;
-    for (int plane = 0; plane < 3 && !status; plane++) {
-      int sx = plane ? h->ss_x : 0, sy = plane ? h->ss_y : 0;
-      int tx_sz = plane ? uv_tx_size(h, b) : b->tx_size;
-      int step = 1 << tx_sz;
-      int plane_sz = VP9_SS_SIZE_LOOKUP[bsize][sx][sy];
-      int n4w = VP9_NUM_4X4_WIDE[plane_sz], n4h = VP9_NUM_4X4_HIGH[plane_sz];
-      int base_x = (b->mi_col * 8) >> sx, base_y = (b->mi_row * 8) >> sy;
-      if (b->is_inter) {
-        if (b->mi_size < BLOCK_8X8) {
-          for (int y = 0; y < n4h && !status; y++)
-            for (int x = 0; x < n4w && !status; x++)
-              status = predict_inter(R, b, plane, base_x + 4 * x, base_y + 4 * y, 4, 4, y * n4w + x);
-        } else {
-          status = predict_inter(R, b, plane, base_x, base_y, n4w * 4, n4h * 4, 0);
-        }
-      }
-      int per_row = (n4w + step - 1) / step;
-      while (!status && cursor < rec->n_txbs && rec->txbs[cursor].block == bi &&
-             rec->txbs[cursor].plane == plane) {
-        const vp9r_txb *t = &rec->txbs[cursor++];
-        int xi = (t->x - base_x) >> 2, yi = (t->y - base_y) >> 2;
-        int block_idx = (yi / step) * per_row + xi / step;
-        if (!b->is_inter)
-          predict_intra(R, b, plane, t->x, t->y, b->avail_l || xi > 0, b->avail_u || yi > 0,
-                        xi + step < n4w, tx_sz, block_idx);
-        if (t->eob > 0) reconstruct(R, b, t);
-      }
-    }
-  }
+  for (uint32_t bi = 0; bi < rec->n_blocks && !status; bi++) status = recon_block(R, bi, &cursor);
   free(R);
   return status;
+}
+
+/* ---- the threaded forms (the all-core CPU baseline) ----
+ *
+ * Reconstruction with the tile columns on several threads, as the
+ * reference's decode_tiles_parallel (src/decoder/block.rs:1769-1866): a tile
+ * column reads nothing of another (intra edges stop at the tile's left edge
+ * and never reach right of the block; inter prediction reads references
+ * only), so each column's blocks, in record order, are one task.  The loop
+ * filter as the reference's filter_bands (src/decoder/loopfilter.rs:195-265):
+ * superblock rows of each plane as tasks in order, superblock (r, c) after
+ * (r, c - 1) and after (r - 1, c + 1) of the same plane, whose regions are
+ * the only earlier ones in raster order that overlap it; the result is the
+ * serial filter's. */
+
+typedef struct mt_pool {
+  int n_tasks;
+  int next; /* atomic */
+  void (*run)(struct mt_pool *, int);
+} mt_pool;
+
+static void *mt_worker(void *arg) {
+  mt_pool *p = (mt_pool *)arg;
+  for (;;) {
+    int k = __atomic_fetch_add(&p->next, 1, __ATOMIC_RELAXED);
+    if (k >= p->n_tasks) return NULL;
+    p->run(p, k);
+  }
+}
+
+/* Tasks are taken in index order, so a task only ever waits for tasks that
+ * were taken before it. */
+static void mt_run(mt_pool *p, int threads) {
+  if (threads > 64) threads = 64;
+  if (threads > p->n_tasks) threads = p->n_tasks;
+  pthread_t tid[64];
+  int started = 0;
+  for (int i = 1; i < threads; i++)
+    if (pthread_create(&tid[started], NULL, mt_worker, p) == 0) started++;
+  mt_worker(p);
+  for (int i = 0; i < started; i++) pthread_join(tid[i], NULL);
+}
+
+typedef struct recon_mt {
+  mt_pool pool;
+  const vp9r_record *rec;
+  const orc_frame *const *refs;
+  orc_frame *cur;
+  const uint32_t *txb_first; /* per block: its first transform block */
+  const uint8_t *col_of;     /* per block: its tile column */
+  int status[64];
+} recon_mt;
+
+static void recon_column(mt_pool *p, int col) {
+  recon_mt *M = (recon_mt *)p;
+  recon_ctx *R = recon_new(M->rec, M->refs, M->cur);
+  int status = 0;
+  for (uint32_t bi = 0; bi < M->rec->n_blocks && !status; bi++) {
+    if (M->col_of[bi] != col) continue;
+    uint32_t cursor = M->txb_first[bi];
+    status = recon_block(R, bi, &cursor);
+  }
+  M->status[col] = status;
+  free(R);
+}
+
+int orc_reconstruct_mt(const vp9r_record *rec, const orc_frame *const refs[3], orc_frame *cur, int threads) {
+  const vp9r_frame_hdr *h = &rec->hdr;
+  const int cols = 1 << h->tile_cols_log2;
+  if (threads <= 1 || cols == 1 || cols > 64) return orc_reconstruct(rec, refs, cur);
+  recon_mt M;
+  memset(&M, 0, sizeof M);
+  uint32_t *txb_first = (uint32_t *)malloc(sizeof(uint32_t) * (rec->n_blocks + 1));
+  uint8_t *col_of = (uint8_t *)malloc(rec->n_blocks + 1);
+  uint32_t t = 0;
+  for (uint32_t bi = 0; bi < rec->n_blocks; bi++) {
+    while (t < rec->n_txbs && rec->txbs[t].block < bi) t++;
+    txb_first[bi] = t;
+    /* the tile column: MiColStarts (6.4), in whole superblocks */
+    int c = 0;
+    while (c + 1 < cols && (uint32_t)((((c + 1) * h->sb64_cols) >> h->tile_cols_log2) * 8) <= rec->blocks[bi].mi_col) c++;
+    col_of[bi] = (uint8_t)c;
+  }
+  M.pool.n_tasks = cols;
+  M.pool.run = recon_column;
+  M.rec = rec;
+  M.refs = refs;
+  M.cur = cur;
+  M.txb_first = txb_first;
+  M.col_of = col_of;
+  mt_run(&M.pool, threads);
+  free(txb_first);
+  free(col_of);
+  for (int c = 0; c < cols; c++)
+    if (M.status[c]) return M.status[c];
+  return 0;
 }
 
 /* ---- loop filter frame process (src/decoder/loopfilter.rs) ---- */
@@ -1105,4 +1216,55 @@ void orc_loop_filter(const vp9r_record *rec, orc_frame *cur) {
           orc_lf_filter_edges(cur->plane[plane], cur->stride[plane], x0, y0, pass == 0, 0, g & 0xff,
                               g >> 8, &e, h->bit_depth);
         }
+}
+
+typedef struct lf_mt {
+  mt_pool pool;
+  const vp9r_record *rec;
+  orc_frame *cur;
+  lf_ctx C;
+  uint32_t rows, cols;
+  int *done; /* per (plane, superblock row): superblocks finished (atomic) */
+} lf_mt;
+
+static void lf_band(mt_pool *p, int k) {
+  lf_mt *M = (lf_mt *)p;
+  const vp9r_frame_hdr *h = &M->rec->hdr;
+  const int plane = k / (int)M->rows, r = k % (int)M->rows;
+  int *above = r ? &M->done[plane * M->rows + r - 1] : NULL;
+  orc_edges e;
+  for (uint32_t c = 0; c < M->cols; c++) {
+    /* (r - 1, c + 1) done, or the whole row above */
+    const int need = (int)(c + 2 < M->cols ? c + 2 : M->cols);
+    if (above)
+      while (__atomic_load_n(above, __ATOMIC_ACQUIRE) < need) sched_yield();
+    for (int pass = 0; pass < 2; pass++) {
+      int x0, y0;
+      int g = lf_superblock(&M->C, plane, pass, (uint32_t)r * 8, c * 8, &e, &x0, &y0);
+      orc_lf_filter_edges(M->cur->plane[plane], M->cur->stride[plane], x0, y0, pass == 0, 0, g & 0xff, g >> 8, &e,
+                          h->bit_depth);
+    }
+    __atomic_store_n(&M->done[plane * M->rows + r], (int)c + 1, __ATOMIC_RELEASE);
+  }
+}
+
+void orc_loop_filter_mt(const vp9r_record *rec, orc_frame *cur, int threads) {
+  const vp9r_frame_hdr *h = &rec->hdr;
+  if (h->lf_level == 0) return;
+  if (threads <= 1) {
+    orc_loop_filter(rec, cur);
+    return;
+  }
+  lf_mt M;
+  memset(&M, 0, sizeof M);
+  lf_setup(rec, &M.C);
+  M.rec = rec;
+  M.cur = cur;
+  M.rows = (h->mi_rows + 7) / 8;
+  M.cols = (h->mi_cols + 7) / 8;
+  M.done = (int *)calloc(3 * M.rows, sizeof(int));
+  M.pool.n_tasks = (int)(3 * M.rows);
+  M.pool.run = lf_band;
+  mt_run(&M.pool, threads);
+  free(M.done);
 }

@@ -100,6 +100,13 @@ void orc_frame_free(orc_frame *f);
 int orc_reconstruct(const vp9r_record *rec, const orc_frame *const refs[3], orc_frame *cur);
 /* loopfilter::filter_frame (src/decoder/loopfilter.rs:51-81), serial order. */
 void orc_loop_filter(const vp9r_record *rec, orc_frame *cur);
+/* The same with the tile columns on up to `threads` threads (the
+ * reference's decode_tiles_parallel, block.rs:1769-1866), and the loop
+ * filter with the superblock rows of each plane as a wavefront on up to
+ * `threads` threads (filter_bands, loopfilter.rs:195-265).  Results equal
+ * the serial forms'.  The all-core CPU baseline. */
+int orc_reconstruct_mt(const vp9r_record *rec, const orc_frame *const refs[3], orc_frame *cur, int threads);
+void orc_loop_filter_mt(const vp9r_record *rec, orc_frame *cur, int threads);
 /* The edge decisions of one superblock pass (loopfilter.rs:466-620); returns
  * n_edges | n_runs << 8, writes the origin to x0/y0. */
 int orc_lf_superblock(const vp9r_record *rec, int plane, int pass, uint32_t row, uint32_t col,

@@ -218,6 +218,9 @@ struct vp9r_server {
 
   // ---- parse workers
   Stream *pick_for_parse() {
+    // after a device failure nothing more is parsed (the input slots of
+    // the failed batch may still be read by its queued work)
+    if (fatal != VP9R_OK) return nullptr;
     Stream *best = nullptr;
     for (auto &up : streams) {
       Stream *st = up.get();
@@ -575,8 +578,14 @@ struct vp9r_server {
       if (s) {
         fatal = s;
         fatal_msg = vp9r_last_error();
-        // the batch's work may be partly queued: the finisher still waits
-        // for it, but its frames are not handed out
+        // The batch's work may be partly queued, and its done event was not
+        // recorded (a recycled event reads as complete): wait for both
+        // streams here, so that the finisher releases the batch's input
+        // slots only once nothing reads them.  Its frames are not handed out.
+        (void)hipStreamSynchronize(g->stream);
+        (void)hipStreamSynchronize(g->copy_stream);
+        if (b->done) free_events.push_back(b->done);
+        b->done = nullptr;
         for (OutRec &o : b->outs) free_out.push_back(o.buf);
         b->outs.clear();
       }

@@ -43,6 +43,8 @@ def lib():
     L.orc_frame_free.argtypes = [FP]
     L.orc_reconstruct.argtypes = [ctypes.POINTER(vp.Record), ctypes.POINTER(FP), FP]
     L.orc_loop_filter.argtypes = [ctypes.POINTER(vp.Record), FP]
+    L.orc_reconstruct_mt.argtypes = [ctypes.POINTER(vp.Record), ctypes.POINTER(FP), FP, ctypes.c_int]
+    L.orc_loop_filter_mt.argtypes = [ctypes.POINTER(vp.Record), FP, ctypes.c_int]
     L.orc_packed_size.restype = ctypes.c_size_t
     L.orc_packed_size.argtypes = [FP]
     L.orc_pack.argtypes = [FP, P]
@@ -100,10 +102,16 @@ class Frame:
 
 class CpuDecoder:
     """The scalar CPU decoder: product parser + oracle reconstruction and loop
-    filter, with the reference's 8 slots (decoder/mod.rs:218-422)."""
+    filter, with the reference's 8 slots (decoder/mod.rs:218-422).  With
+    threads > 1: tile columns parsed and reconstructed in parallel and the
+    loop filter's superblock rows as a wavefront (the reference's threading;
+    the same frames)."""
 
-    def __init__(self):
+    def __init__(self, threads=1):
         self.parser = vp.Parser()
+        self.threads = threads
+        if threads > 1:
+            self.parser.set_threads(threads)
         self.slots = [None] * 8
         self.frames_decoded = 0
 
@@ -120,10 +128,16 @@ class CpuDecoder:
                 refs[i] = self.slots[h.ref_frame_idx[i]].f
         if on_record is not None:
             on_record(rec, [None if h.frame_is_intra else self.slots[h.ref_frame_idx[i]] for i in range(3)])
-        st = lib().orc_reconstruct(ctypes.byref(rec), refs, cur.f)
+        if self.threads > 1:
+            st = lib().orc_reconstruct_mt(ctypes.byref(rec), refs, cur.f, self.threads)
+        else:
+            st = lib().orc_reconstruct(ctypes.byref(rec), refs, cur.f)
         if st:
             raise vp.BitstreamError("oracle reconstruction failed")
-        lib().orc_loop_filter(ctypes.byref(rec), cur.f)
+        if self.threads > 1:
+            lib().orc_loop_filter_mt(ctypes.byref(rec), cur.f, self.threads)
+        else:
+            lib().orc_loop_filter(ctypes.byref(rec), cur.f)
         for i in range(8):
             if (h.refresh_frame_flags >> i) & 1:
                 self.slots[i] = cur
