@@ -17,9 +17,21 @@ over a second timed region of the same frames (the events cost the stream a
 few microseconds each, so the headline region has none; "stage_pass_fps" is
 that region's rate); the PCIe-inclusive rate (every frame also copied to
 pinned host memory on the copy stream); synchronous per-frame latency
-p50/p99; and the CPU baseline: the oracle's scalar reconstruction and loop
-filter of the same records on one host core, whose per-frame MD5s the GPU's
-output is checked against (bit_exact).
+p50/p99; and the CPU baseline: the oracle's reconstruction and loop filter
+of the same records on all host cores (tile columns in parallel, the loop
+filter's superblock rows as a wavefront: the reference's threading), with
+the one-core scalar figure inside, whose per-frame MD5s the GPU's output is
+checked against (bit_exact).  "roofline.traffic" is the dominant kernel's
+HBM bytes per launch from the committed counter profile.
+
+End to end ("end_to_end"): the same stream from packets to packed frames in
+host memory, through the decoder API (tile columns parsed in parallel) and
+through the server with this one stream (parse running ahead of the GPU),
+against the all-core CPU whole decode.
+
+Config 1 ("config1"): the 352x288 sinusoid stream of 30 frames from seed 1,
+decoded by the scalar CPU decoder on one thread (frames/s and per-frame
+MD5s), and by the GPU decoder API (its MD5s must equal them).
 
 Batched stages ("batched"): one inter frame of the stream reconstructed 64
 times in one batch (every launch covers all 64 copies), per-stage GB/s of
@@ -121,21 +133,29 @@ class Schedule:
         return cur if h.show_frame else None
 
 
-def cpu_baseline(frames_bytes, budget_s):
-    """The oracle's scalar reconstruction + loop filter (one core) of the
-    parsed records, best pass over the whole stream, repeated within the time
-    budget; also the per-frame MD5s of the shown frames."""
+def cpu_baseline(frames_bytes, budget_s, threads=1):
+    """The oracle's reconstruction + loop filter of the parsed records, best
+    pass over the whole stream, repeated within the time budget; also the
+    per-frame MD5s of the shown frames.  threads > 1: tile columns
+    reconstructed in parallel and the loop filter's superblock rows as a
+    wavefront (the reference's decode_tiles_parallel and filter_bands).
+    Returns (recon+LF frames/s, MD5s, passes, seconds, whole-decode frames/s
+    with the parse, tile columns parsed in parallel when threads > 1)."""
     vp = _package()
     if TESTS not in sys.path:
         sys.path.insert(0, TESTS)
     import oracle_lib as ol
-    best, md5s, passes, spent = None, None, 0, 0.0
+    best, best_whole, md5s, passes, spent = None, None, None, 0, 0.0
     while passes < 1 or (spent < budget_s and passes < 5):
         parser = vp.Parser()
+        if threads > 1:
+            parser.set_threads(threads)
         slots = [None] * 8
-        md5_pass, t_recon = [], 0.0
+        md5_pass, t_recon, t_parse = [], 0.0, 0.0
         for data in frames_bytes:
+            tp = time.perf_counter()
             rec = parser.parse(data)
+            t_parse += time.perf_counter() - tp
             h = rec.hdr
             if h.show_existing_frame:
                 md5_pass.append(ol.md5(slots[h.frame_to_show_map_idx].packed()))
@@ -146,9 +166,16 @@ def cpu_baseline(frames_bytes, budget_s):
                 for k in range(3):
                     refs[k] = slots[h.ref_frame_idx[k]].f
             t0 = time.perf_counter()
-            if ol.lib().orc_reconstruct(ctypes.byref(rec), refs, cur.f):
+            if threads > 1:
+                st = ol.lib().orc_reconstruct_mt(ctypes.byref(rec), refs, cur.f, threads)
+            else:
+                st = ol.lib().orc_reconstruct(ctypes.byref(rec), refs, cur.f)
+            if st:
                 raise RuntimeError("oracle reconstruction failed")
-            ol.lib().orc_loop_filter(ctypes.byref(rec), cur.f)
+            if threads > 1:
+                ol.lib().orc_loop_filter_mt(ctypes.byref(rec), cur.f, threads)
+            else:
+                ol.lib().orc_loop_filter(ctypes.byref(rec), cur.f)
             t_recon += time.perf_counter() - t0
             for k in range(8):
                 if (h.refresh_frame_flags >> k) & 1:
@@ -156,11 +183,165 @@ def cpu_baseline(frames_bytes, budget_s):
             if h.show_frame:
                 md5_pass.append(ol.md5(cur.packed()))
         passes += 1
-        spent += t_recon
+        spent += t_recon + t_parse
         fps = len(frames_bytes) / t_recon
+        whole = len(frames_bytes) / (t_recon + t_parse)
         best = fps if best is None else max(best, fps)
+        best_whole = whole if best_whole is None else max(best_whole, whole)
         md5s = md5_pass
-    return best, md5s, passes, spent
+    return best, md5s, passes, spent, best_whole
+
+
+def cpu_baselines(frames_bytes, budget_s, label):
+    """The one-core and all-core CPU baselines of a stream (recon + loop
+    filter of parsed records, the GPU's scope, and the whole decode with the
+    parse), as a cpu_baseline object: the all-core figure first (the
+    comparison the target names), the one-core one inside."""
+    cores = _host_cores()
+    one, md5_one, p1, s1, whole1 = cpu_baseline(frames_bytes, budget_s, 1)
+    alls, md5_all, pa, sa, whole_a = cpu_baseline(frames_bytes, budget_s / 2, cores)
+    n = len(frames_bytes)
+    out = {"value": round(alls, 3), "unit": "frames/s", "cores": cores, "kind": "port",
+           "sample": f"oracle reconstruction + loop filter of all {n} frames of {label} (records parsed outside "
+                     f"the timing), tile columns in parallel and the loop filter's superblock rows as a "
+                     f"wavefront on {cores} threads, best of {pa} passes, {sa:.1f}s",
+           "whole_decode_fps": round(whole_a, 3),
+           "one_core": {"value": round(one, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+                        "sample": f"the same, scalar on one core, best of {p1} passes, {s1:.1f}s",
+                        "whole_decode_fps": round(whole1, 3)}}
+    return out, md5_one, md5_all
+
+
+def profile_traffic(kernel, profile="recon_1080p_moving_rects.md"):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 summary
+    under profiles/: the mean FETCH_SIZE and WRITE_SIZE per dispatch (KiB),
+    each scaled by what the counter reads for a plain device copy of known
+    size in the same file (its calibration tables).  None when absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    tables = {}
+    for m in re.finditer(r"== \S*/(\w+)_(FETCH_SIZE|WRITE_SIZE)/\w+_results\.db\n(.*?)(?=\n==|\n```)", text, re.S):
+        tables[(m.group(1), m.group(2))] = m.group(3)
+
+    def mean(kind, counter, name):
+        body = tables.get((kind, counter))
+        if body is None:
+            return None
+        for line in body.splitlines():
+            line = line.strip()
+            if line.startswith(name) and line[len(name):][:1].isspace():
+                return float(line.split()[-1])
+        return None
+    copy = re.search(r"a (\d+) MiB device-to-device copy", text)
+    if not copy:
+        return None
+    copy_bytes = int(copy.group(1)) << 20
+    out = 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = mean("cal", counter, "__amd_rocclr_copyBuffer")
+        v = mean("pmc", counter, kernel)
+        if cal is None or v is None:
+            return None
+        out += v * (copy_bytes / (cal * 1024.0)) * 1024.0
+    return round(out)
+
+
+def config1_bench(vp, check, passes=5):
+    """The 352x288 8-bit 4:2:0 sinusoid stream of 30 frames from seed 1
+    (base_q_idx 120, golden refresh every 8 frames, one tile column): the
+    scalar CPU decoder (product parser + oracle, one thread, the
+    VP9_FORCE_SCALAR=1 equivalent) over every frame, best of `passes`, with
+    its per-frame MD5s (the oracle of this configuration); the same stream
+    through the GPU decoder API, whose MD5s must equal them."""
+    import streamgen
+    import oracle_lib as ol
+    frames = streamgen.generate(**streamgen.CONFIG1)
+    best, md5s = None, None
+    for _ in range(passes):
+        d = ol.CpuDecoder(1)
+        t0 = time.perf_counter()
+        got = [o for f in frames for o in d.decode(f)]
+        dt = time.perf_counter() - t0
+        md5s = [ol.md5(o.packed()) for o in got]
+        best = dt if best is None else min(best, dt)
+    dec = vp.Decoder(0)
+    best_gpu = None
+    for p in range(passes + 1):  # one decoder; the first pass untimed
+        t0 = time.perf_counter()
+        gpu_frames = [fr for f in frames for fr in dec.decode_all(f)]
+        dt = time.perf_counter() - t0
+        if p:
+            best_gpu = dt if best_gpu is None else min(best_gpu, dt)
+    gpu_md5 = [ol.md5(fr.packed()) for fr in gpu_frames]
+    out = {"workload": "352x288 8-bit 4:2:0 profile 0 sinusoid grid (periods 17 and 29 px) moving (+1.375, -0.625) "
+                       "px/frame with noise, 30 frames, seed 1, base_q_idx 120, golden every 8, 1 tile column",
+           "cpu_frames_per_s": round(len(frames) / best, 2), "cpu_threads": 1,
+           "cpu_note": "whole decode (parse, scalar reconstruction, loop filter, packed output), best of "
+                       f"{passes} passes",
+           "frame_md5": md5s,
+           "gpu_decoder_api_frames_per_s": round(len(frames) / best_gpu, 2),
+           "gpu_md5_equal": gpu_md5 == md5s}
+    _log(f"bench: config 1: CPU {out['cpu_frames_per_s']} frames/s (1 thread); GPU decoder API "
+         f"{out['gpu_decoder_api_frames_per_s']} frames/s; MD5s equal {out['gpu_md5_equal']}")
+    return out
+
+
+def end_to_end(vp, frames_bytes, want_md5, cpu_whole_all):
+    """Single-stream 1080p decode from packets to Frame bytes on the host:
+    (a) the decoder API, one frame at a time (parse with tile columns in
+    parallel on the host cores, upload, reconstruction, loop filter, packed
+    download); (b) the server with this one stream (parse workers run ahead
+    of the GPU, downloads overlap the next frames).  Best of 3 passes each;
+    MD5s against the oracle's."""
+    n = len(frames_bytes)
+    # one decoder and one server for every pass (the stream restarts at its
+    # key frame), the first pass untimed: context set-up and first
+    # allocations are not per-frame work
+    best_api = None
+    dec = vp.Decoder(0)
+    dec.set_threads(_host_cores())
+    for p in range(4):
+        t0 = time.perf_counter()
+        got = [fr for f in frames_bytes for fr in dec.decode_all(f)]
+        dt = time.perf_counter() - t0
+        if p:
+            best_api = dt if best_api is None else min(best_api, dt)
+    api_ok = [hashlib.md5(fr.packed()).hexdigest() for fr in got] == want_md5
+    del dec
+    best_srv, srv_ok = None, True
+    srv = vp.Server(0, parse_threads=4)
+    for p in range(4):
+        sid = srv.open(1920, 1080)
+        md5 = []
+        t0 = time.perf_counter()
+        for f in frames_bytes:
+            srv.submit(sid, f)
+        srv.close(sid)
+        while True:
+            fr = srv.poll(10000)
+            if fr is None:
+                raise RuntimeError("server stalled")
+            if fr.eos:
+                break
+            md5.append(hashlib.md5(fr.view()).hexdigest())
+            fr.release()
+        dt = time.perf_counter() - t0
+        srv_ok = srv_ok and md5 == want_md5
+        if p:
+            best_srv = dt if best_srv is None else min(best_srv, dt)
+    srv.close_server()
+    out = {"decoder_api_fps": round(n / best_api, 2), "decoder_api_threads": _host_cores(),
+           "server_one_stream_fps": round(n / best_srv, 2), "bit_exact": api_ok and srv_ok,
+           "cpu_all_core_whole_decode_fps": cpu_whole_all,
+           "scope": "packets in, parse, record upload, reconstruction, loop filter, packed Frame in host memory"}
+    if cpu_whole_all:
+        out["vs_cpu_all_core"] = round(max(out["decoder_api_fps"], out["server_one_stream_fps"]) / cpu_whole_all, 2)
+    _log(f"bench: end to end: decoder API {out['decoder_api_fps']} frames/s, server (one stream) "
+         f"{out['server_one_stream_fps']} frames/s, bit-exact {out['bit_exact']}")
+    return out
 
 
 # ---- the 4K 10-bit stream with scaled references
@@ -205,11 +386,10 @@ def uhd_bench(vp, gpu, check, passes=3):
            "frames_per_s": round(fps, 2), "ms_per_frame": round(1000.0 / fps, 4), "timed_frames": passes * n,
            "bit_exact": None}
     if check:
-        cpu_fps, cpu_md5, _, spent = cpu_baseline(frames_bytes, 0.0)
-        out["bit_exact"] = cpu_md5 == md5
-        out["cpu_baseline"] = {"value": round(cpu_fps, 3), "unit": "frames/s", "cores": 1, "kind": "port",
-                               "sample": f"oracle reconstruction + loop filter of the {n} frames, one pass, "
-                                         f"{spent:.1f}s"}
+        cpu, md5_one, md5_all = cpu_baselines(frames_bytes, 0.0, "the 4K 10-bit stream")
+        out["bit_exact"] = md5_one == md5 and md5_all == md5
+        out["cpu_baseline"] = cpu
+        out["vs_cpu_all_core"] = round(fps / cpu["value"], 2)
     _log(f"bench: 4K 10-bit {fps:.1f} frames/s, bit-exact {out['bit_exact']}")
     del sched, frames
     return out
@@ -356,7 +536,9 @@ def lf_bench(vp, gpu, check, copies=64, reps=5):
            "frames": copies, "kernel_ms": round(ms, 4), "frames_per_s": round(copies / (ms * 1e-3), 1),
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes},
-           "sharpness_levels": sorted({int(v) for v in sharp}), "bit_exact": None}
+           "sharpness_levels": sorted({int(v) for v in sharp}), "bit_exact": None,
+           "parity_unpinned": ["lf sharpness 1-7: the GPU equals the oracle's restatement of loopfilter.rs:143-160; "
+                               "no fixture pins it"]}
     if check:
         ctx = multiprocessing.get_context("spawn")
         with concurrent.futures.ProcessPoolExecutor(_host_cores(), mp_context=ctx) as ex:
@@ -727,9 +909,16 @@ def main():
     dom = max(range(4), key=lambda s: stage_ms[s])
     dname = STAGES[dom]
     achieved = stages[dname]["GB_per_s"] or 0.0
+    kern = {"loop_filter": "vp9g::k_lf<false, true>", "inter_pred": "vp9g::k_mc<false>",
+            "inter_residual": "vp9g::k_itx_inter<false>"}.get(dname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": STAGE_KERNELS[dom], "stage": dname}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profile_traffic(kern) if kern else None,
+                "kernel": STAGE_KERNELS[dom], "stage": dname,
+                "traffic_source": "profiles/recon_1080p_moving_rects.md: FETCH_SIZE and WRITE_SIZE per dispatch, "
+                                  "scaled by a device copy's counts; HBM bytes per launch"}
+    if roofline["traffic"] and dname in stages and stages[dname]["launches_per_frame"]:
+        roofline["algorithmic_bytes_per_launch"] = round(stages[dname]["bytes_per_frame"] /
+                                                         stages[dname]["launches_per_frame"])
 
     extras = {}
     if not args.no_extras:
@@ -777,16 +966,20 @@ def main():
 
     cpu = None
     bit_exact = None
+    e2e = None
+    config1 = None
     if not args.no_cpu:
-        cpu_fps, cpu_md5, passes, spent = cpu_baseline(frames_bytes, args.cpu_seconds)
-        bit_exact = cpu_md5 == gpu_md5
-        cpu = {"value": round(cpu_fps, 3), "unit": "frames/s", "cores": 1, "kind": "port",
-               "sample": f"oracle scalar reconstruction + loop filter of all {n} frames of the same stream "
-                         f"(records parsed outside the timing), best of {passes} passes, {spent:.1f}s of CPU work"}
-        _log(f"bench: CPU oracle {cpu_fps:.2f} frames/s; GPU MD5s {'==' if bit_exact else '!='} oracle MD5s")
+        cpu, cpu_md5, cpu_md5_all = cpu_baselines(frames_bytes, args.cpu_seconds, "the same stream")
+        bit_exact = cpu_md5 == gpu_md5 and cpu_md5_all == gpu_md5
+        cpu["vs_gpu"] = round(fps / cpu["value"], 2)
+        _log(f"bench: CPU oracle {cpu['one_core']['value']} frames/s on one core, {cpu['value']} on "
+             f"{cpu['cores']}; GPU MD5s {'==' if bit_exact else '!='} oracle MD5s")
         if not bit_exact:
             bad = [i for i, (a, b) in enumerate(zip(gpu_md5, cpu_md5)) if a != b]
             _log(f"bench: MD5 MISMATCH at shown frames {bad[:10]}")
+        if not args.no_extras:
+            e2e = end_to_end(vp, frames_bytes, cpu_md5, cpu["whole_decode_fps"])
+            config1 = config1_bench(vp, True)
 
     batched = None if args.no_batched else batched_stages(vp, gpu, frames)
     uhd = None if args.no_uhd else uhd_bench(vp, gpu, not args.no_cpu)
@@ -824,6 +1017,14 @@ def main():
         "parse_fps_one_core": round(n / t_parse, 2),
     }
     out.update(extras)
+    if e2e is not None:
+        out["end_to_end"] = e2e
+    if config1 is not None:
+        out["config1"] = config1
+    # parity that no fixture or closed form pins: the GPU equals the oracle's
+    # restatement of the reference there
+    out["parity_unpinned"] = ["loop filter sharpness 1-7 (no committed vector has sharpness > 0)",
+                              "more than one tile column (no committed vector has two)"]
     if batched is not None:
         out["batched"] = batched
     if uhd is not None:
@@ -834,6 +1035,7 @@ def main():
         out["serving"] = serving
     print(json.dumps(out), flush=True)
     if bit_exact is False or (serving is not None and not serving["bit_exact"]) or \
+            (e2e is not None and not e2e["bit_exact"]) or (config1 is not None and not config1["gpu_md5_equal"]) or \
             (batched is not None and not batched["bit_exact"]) or (uhd is not None and uhd["bit_exact"] is False) or \
             (micro is not None and micro["bit_exact"] is False):
         sys.exit(1)

@@ -22,6 +22,8 @@
 // dword of the output).
 #include <type_traits>
 
+#include <mutex>
+
 #include "dev_common.h"
 #include "launch.h"
 #include "mc_pos.h"
@@ -36,9 +38,17 @@ __constant__ uint32_t c_taps8[4][16][2];
 // the taps as signed 16-bit pairs (taps 2k, 2k + 1 in dword k)
 __constant__ uint32_t c_taps16[4][16][4];
 
-static bool filters_ready = false;
+// The tap tables go to each device's constant memory once, on the first MC
+// launch on that device (under a lock: contexts of several threads may launch
+// at once).  A failed copy leaves the device unmarked, and its error is the
+// one hipGetLastError() reports to the launch's caller.
+static std::mutex filters_mu;
+static uint64_t filters_ready = 0;  // bit d: device d holds the tables
 static void upload_filters() {
-  if (filters_ready) return;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  std::lock_guard<std::mutex> lock(filters_mu);
+  if ((filters_ready >> dev) & 1) return;
   int16_t f[4][16][8];
   uint32_t t8[4][16][2] = {}, t16[4][16][4] = {};
   for (int a = 0; a < 4; a++)
@@ -48,10 +58,11 @@ static void upload_filters() {
         t8[a][b][c >> 2] |= (uint32_t)(uint8_t)(int8_t)VP9_SUBPEL_FILTERS[a][b][c] << (8 * (c & 3));
         t16[a][b][c >> 1] |= (uint32_t)(uint16_t)(int16_t)VP9_SUBPEL_FILTERS[a][b][c] << (16 * (c & 1));
       }
-  hipMemcpyToSymbol(HIP_SYMBOL(c_filters), f, sizeof f);
-  hipMemcpyToSymbol(HIP_SYMBOL(c_taps8), t8, sizeof t8);
-  hipMemcpyToSymbol(HIP_SYMBOL(c_taps16), t16, sizeof t16);
-  filters_ready = true;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_filters), f, sizeof f) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_taps8), t8, sizeof t8) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_taps16), t16, sizeof t16) != hipSuccess)
+    return;
+  filters_ready |= 1ull << dev;
 }
 
 // A tile is at most 16 x 16 samples; its footprint at most 38 x 38 (a 2:1

@@ -23,9 +23,12 @@ struct FrameStore {
   ~FrameStore() { vp9r_gpu_frame_free(f); }
 };
 
+// A shown frame of the current packet: its description and a pinned host
+// buffer (reused from packet to packet) that its download lands in.
 struct OutFrame {
   vp9r_frame desc{};
-  std::vector<uint8_t> data;
+  vp9r::HostBuf *buf = nullptr;
+  vp9r_gpu_frame *f = nullptr;  // the store being downloaded
 };
 
 }  // namespace
@@ -36,6 +39,7 @@ struct vp9r_decoder {
   std::shared_ptr<FrameStore> slots[8];
   std::vector<std::shared_ptr<FrameStore>> pool;
   std::vector<OutFrame> out;
+  std::vector<std::unique_ptr<vp9r::HostBuf>> out_bufs;  // pinned, one per shown frame of a packet
   uint32_t threads = 0;
   uint64_t frames = 0;
 
@@ -64,13 +68,22 @@ struct vp9r_decoder {
     return fs;
   }
 
+  // Queues the packed download of a shown frame into a pinned buffer on
+  // the copy stream (behind the frame's reconstruction); vp9r_decoder_decode
+  // waits for all of a packet's downloads before it returns.
   int emit(const std::shared_ptr<FrameStore> &fs) {
     vp9r_gpu_frame *f = fs->f;
     OutFrame o;
     size_t n = vp9r_packed_size(f->width, f->height, f->bit_depth, f->ss_x, f->ss_y);
-    o.data.resize(n);
-    int st = vp9r_gpu_download_packed(g, f, o.data.data());
+    if (out_bufs.size() <= out.size()) out_bufs.emplace_back(new vp9r::HostBuf());
+    o.buf = out_bufs[out.size()].get();
+    if (o.buf->ensure(n) != hipSuccess) {
+      set_error("hipHostMalloc(frame output) failed");
+      return VP9R_ERR_DEVICE;
+    }
+    int st = vp9r_gpu_download_packed_async(g, f, (uint8_t *)o.buf->p);
     if (st) return st;
+    o.f = f;
     vp9r_frame &d = o.desc;
     d.width = f->width;
     d.height = f->height;
@@ -89,8 +102,18 @@ struct vp9r_decoder {
       d.plane_offset[p] = off;
       off += (size_t)d.plane_width[p] * d.plane_height[p] * bps;
     }
-    out.push_back(std::move(o));
+    out.push_back(o);
     return VP9R_OK;
+  }
+
+  // Waits for every download queued by emit() (each in its own buffer).
+  int finish_out() {
+    int st = VP9R_OK;
+    for (OutFrame &o : out) {
+      int s2 = vp9r_gpu_frame_wait(o.f);
+      if (s2 && !st) st = s2;
+    }
+    return st;
   }
 
   int decode_frame(const uint8_t *data, size_t len) {
@@ -181,7 +204,10 @@ int vp9r_decoder_new(int device, vp9r_decoder **out) {
 
 void vp9r_decoder_free(vp9r_decoder *d) {
   if (!d) return;
-  if (d->g) (void)hipStreamSynchronize(d->g->stream);
+  if (d->g) {
+    (void)hipStreamSynchronize(d->g->stream);
+    (void)hipStreamSynchronize(d->g->copy_stream);
+  }
   for (auto &s : d->slots) s.reset();
   d->pool.clear();
   vp9r_gpu_free(d->g);
@@ -208,7 +234,16 @@ int vp9r_decoder_decode(vp9r_decoder *d, const uint8_t *packet, size_t len, uint
   size_t n = vp9r_superframe_split(packet, len, offs, sizes, 8);
   for (size_t i = 0; i < n && i < 8; i++) {
     int st = d->decode_frame(packet + offs[i], sizes[i]);
-    if (st) return st;
+    if (st) {
+      (void)d->finish_out();
+      d->out.clear();
+      return st;
+    }
+  }
+  int st = d->finish_out();
+  if (st) {
+    d->out.clear();
+    return st;
   }
   *n_frames = (uint32_t)d->out.size();
   return VP9R_OK;
@@ -220,7 +255,7 @@ int vp9r_decoder_frame(const vp9r_decoder *d, uint32_t i, vp9r_frame *out) {
     return VP9R_ERR_INVALID_INPUT;
   }
   *out = d->out[i].desc;
-  out->data = const_cast<uint8_t *>(d->out[i].data.data());
+  out->data = (uint8_t *)d->out[i].buf->p;
   return VP9R_OK;
 }
 

@@ -1514,6 +1514,11 @@ int vp9r_gpu_prepared_stats(const vp9r_prepared *pr, uint64_t out[12]) {
 }
 
 int vp9r_gpu_download_packed(vp9r_gpu *g, vp9r_gpu_frame *f, uint8_t *dst) {
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (f->g != g) {
+    set_error("the frame store belongs to another context");
+    return VP9R_ERR_INVALID_INPUT;
+  }
   size_t bps = f->bit_depth > 8 ? 2 : 1, off = 0;
   for (int p = 0; p < 3; p++) {
     size_t w = p ? (f->width + f->ss_x) >> f->ss_x : f->width;

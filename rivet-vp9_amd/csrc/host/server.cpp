@@ -264,6 +264,15 @@ struct vp9r_server {
       auto pkt = std::move(st->packets.front());
       st->packets.pop_front();
       st->parsing = true;
+      // Streams with packets share the parse threads: with at most a
+      // quarter as many such streams as workers, a stream's tile columns
+      // (up to 4) are parsed in parallel by spare threads (the record is
+      // the same either way; two settings only, so that a stream's thread
+      // pool is rarely rebuilt).
+      size_t busy_streams = 0;
+      for (auto &up : streams)
+        busy_streams += up->admitted && !up->finished && !up->failed && (!up->packets.empty() || up->parsing);
+      st->parser.p.tile_threads = 4 * busy_streams <= cfg.parse_threads ? 4 : 1;
       lk.unlock();
       size_t offs[8], sizes[8];
       const std::vector<uint8_t> &data = pkt.first;

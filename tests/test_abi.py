@@ -46,14 +46,6 @@ def test_every_declared_function_resolves():
         assert getattr(lib, n) is not None
 
 
-def _cuda_visible():
-    try:
-        import torch  # noqa: F401
-    except Exception:
-        return None
-    return None
-
-
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a GPU is present")
 def test_no_gpu_fails_loudly():
     with pytest.raises(vp.DeviceError):
