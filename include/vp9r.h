@@ -266,13 +266,11 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
 /* Motion compensation (inter::predict, src/dsp/inter.rs:289-395, and
  * pixel::avg, src/dsp/pixel.rs:22-50), batched.  Block i predicts w x h
  * samples (w, h in 4..64) at 1/16-sample position (x, y) with steps
- * (xs, ys) in 1..32 from reference plane ref_idx of the `refs` array (device
- * plane pointers, all with row stride `ref_stride` and bottom-right sample
- * (last_x, last_y)), and, when ref2_idx >= 0, averages with a second
- * prediction at (x2, y2).  Writes to out + out_off[i] samples with stride
+ * (xs, ys) in 1..32 from reference plane ref_idx of the `refs` array, and,
+ * when ref2_idx >= 0, averages with a second prediction at (x2, y2) from
+ * plane ref2_idx.  Writes to out + out_off[i] samples with stride
  * out_stride.  `desc` is a host array (the wrapper expands it into tiles of
- * 16 columns by up to 32 rows); `refs` is a host array of device pointers;
- * `out` is a device pointer. */
+ * 16 columns by up to 32 rows); `out` is a device pointer. */
 typedef struct vp9r_mc_desc {
   int32_t x, y, x2, y2;
   int16_t xs, ys;
@@ -282,23 +280,77 @@ typedef struct vp9r_mc_desc {
   uint8_t _pad;
   uint64_t out_off;
 } vp9r_mc_desc;
+/* One reference plane (inter.rs:21-27 RefPlane: data, stride, last_x,
+ * last_y): a device pointer to samples (u8 at 8 bits, else u16), its row
+ * stride in samples, and its bottom-right sample (the reference's visible
+ * size minus one, recon.rs:169-170). */
+typedef struct vp9r_ref_plane {
+  const void *data;
+  uint32_t stride;
+  int32_t last_x, last_y;
+  uint32_t _pad;
+} vp9r_ref_plane;
+/* Each reference with its own plane geometry: `planes` is a host array of
+ * n_refs planes. */
+int vp9r_mc_batch_planes(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const vp9r_ref_plane *planes,
+                         uint32_t n_refs, void *out, uint32_t out_stride, uint32_t bit_depth);
+/* The same with every reference plane of one geometry: `refs` is a host
+ * array of n_refs device pointers, all of row stride ref_stride and
+ * bottom-right sample (last_x, last_y). */
 int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc,
                   const void *const *refs, uint32_t n_refs, uint32_t ref_stride, int32_t last_x,
                   int32_t last_y, void *out, uint32_t out_stride, uint32_t bit_depth);
 
+/* pixel::avg (src/dsp/pixel.rs:22-50), batched: block i writes
+ * (a + b + 1) >> 1 over w x h samples to dst + dst_off (row stride
+ * `stride`), a and b at a + a_off and b + b_off with row stride w.  `desc`
+ * is a host array; a, b and dst are device pointers to samples (u8 at 8
+ * bits, else u16). */
+typedef struct vp9r_avg_desc {
+  uint64_t a_off, b_off, dst_off;
+  uint8_t w, h, _pad[6];
+} vp9r_avg_desc;
+int vp9r_avg_batch(vp9r_gpu *g, uint32_t count, const vp9r_avg_desc *desc, const void *a, const void *b,
+                   void *dst, uint32_t stride, uint32_t bit_depth);
+/* pixel::add_const (src/dsp/pixel.rs:9-16), batched: v (clamped to
+ * +-65536) added to the n x n samples at dst + dst_off (u8, row stride
+ * `stride`), each clamped to 0..255; 8 bits only, as the reference. */
+typedef struct vp9r_add_const_desc {
+  uint64_t dst_off;
+  int32_t v;
+  uint8_t n, _pad[3];
+} vp9r_add_const_desc;
+int vp9r_add_const_batch(vp9r_gpu *g, uint32_t count, const vp9r_add_const_desc *desc, void *dst,
+                         uint32_t stride);
+
+/* intra::predict (src/dsp/intra.rs:19-29), batched: block i predicts mode
+ * `mode` at size 1 << log2 (log2 2..5) from the int32 edges at
+ * edges + edge_off (aboveRow[-1 .. 2 size - 1], then leftCol[0 .. size -
+ * 1]: intra.rs's above and left) into dst + dst_off (row stride `stride`).
+ * `desc` is a host array; edges and dst are device pointers (dst: u8 at 8
+ * bits, else u16). */
+typedef struct vp9r_intra_desc {
+  uint64_t dst_off;
+  uint32_t edge_off;
+  uint8_t mode, log2, have_left, have_above;
+} vp9r_intra_desc;
+int vp9r_intra_predict_batch(vp9r_gpu *g, uint32_t count, const vp9r_intra_desc *desc, const int32_t *edges,
+                             void *dst, uint32_t stride, uint32_t bit_depth);
+
 /* Loop filter sample kernels (lf::filter_edges, src/dsp/lf.rs:178-218) for a
  * sequence of superblock passes over one plane: pass i filters edges
- * [0, n_edges[i]) (n_runs[i] runs) of the region at (x0[i], y0[i]) with the
- * Edges tables at edges + 1024 * i (fs[16][16] int8, -1 = none, then
- * limit, blimit, thresh [16][16] uint8, unshifted).  Passes run in order, as
- * the reference calls them.  `buf` is a device plane of `rows` rows of
- * `stride` samples (u8 at 8 bits, else u16); every pass's reach (8 samples
- * before its first edge) must lie inside it.  The other arrays are host
- * arrays. */
+ * [first[i], n_edges[i]) (n_runs[i] runs) of the region at (x0[i], y0[i])
+ * with the Edges tables at edges + 1024 * i (fs[16][16] int8, -1 = none,
+ * then limit, blimit, thresh [16][16] uint8, unshifted).  Passes run in
+ * order, as the reference calls them, in one launch.  `buf` is a device
+ * plane of `rows` rows of `stride` samples (u8 at 8 bits, else u16); every
+ * pass's reach (8 samples past its last edge) must lie inside it.  The
+ * other arrays are host arrays; `first` may be NULL (every pass from edge
+ * 0). */
 int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const uint16_t *x0,
-                        const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
-                        const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t rows,
-                        uint32_t bit_depth);
+                        const uint16_t *y0, const uint8_t *vertical, const uint8_t *first,
+                        const uint8_t *n_edges, const uint8_t *n_runs, void *buf, uint32_t stride,
+                        uint32_t rows, uint32_t bit_depth);
 
 /* Device memory for callers of the batched forms (no counterpart in the
  * reference: a plain-pointer ABI needs its own allocation and copies).  The

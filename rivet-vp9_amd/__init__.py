@@ -129,6 +129,23 @@ class McDesc(ctypes.Structure):
                 ("_pad", u8), ("out_off", u64)]
 
 
+class RefPlane(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("stride", u32), ("last_x", i32), ("last_y", i32), ("_pad", u32)]
+
+
+class AvgDesc(ctypes.Structure):
+    _fields_ = [("a_off", u64), ("b_off", u64), ("dst_off", u64), ("w", u8), ("h", u8), ("_pad", u8 * 6)]
+
+
+class AddConstDesc(ctypes.Structure):
+    _fields_ = [("dst_off", u64), ("v", i32), ("n", u8), ("_pad", u8 * 3)]
+
+
+class IntraDesc(ctypes.Structure):
+    _fields_ = [("dst_off", u64), ("edge_off", u32), ("mode", u8), ("log2", u8), ("have_left", u8),
+                ("have_above", u8)]
+
+
 _lib = None
 
 
@@ -172,7 +189,11 @@ def library():
     lib.vp9r_gpu_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(u32)]
     lib.vp9r_itx_add_batch.argtypes = [P, u32, V, V, V, V, V, V, V, u32, u32, ctypes.c_int]
     lib.vp9r_mc_batch.argtypes = [P, u32, V, V, u32, u32, i32, i32, V, u32, u32]
-    lib.vp9r_lf_edges_batch.argtypes = [P, u32, V, V, V, V, V, V, V, u32, u32, u32]
+    lib.vp9r_lf_edges_batch.argtypes = [P, u32, V, V, V, V, V, V, V, V, u32, u32, u32]
+    lib.vp9r_mc_batch_planes.argtypes = [P, u32, V, V, u32, V, u32, u32]
+    lib.vp9r_avg_batch.argtypes = [P, u32, V, V, V, V, u32, u32]
+    lib.vp9r_add_const_batch.argtypes = [P, u32, V, V, u32]
+    lib.vp9r_intra_predict_batch.argtypes = [P, u32, V, V, V, u32, u32]
     lib.vp9r_itx_add_host.argtypes = [P, V, u32, u32, ctypes.c_int, u32, u32, V, u32]
     lib.vp9r_intra_predict_host.argtypes = [P, u32, u32, V, V, ctypes.c_int, ctypes.c_int, u32, V, u32]
     lib.vp9r_gpu_prepare.argtypes = [P, ctypes.POINTER(Record), PP]

@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "csrc/gpu/intra.hip",
     "csrc/gpu/lf.hip",
     "csrc/gpu/frame.hip",
+    "csrc/gpu/pixel.hip",
     "csrc/host/recon.cpp",
     "csrc/host/capi_seam.cpp",
     "csrc/host/decoder.cpp",

@@ -773,4 +773,38 @@ void launch_intra_pred(hipStream_t s, uint32_t mode, uint32_t log2, const int32_
                      have_above, (int)bit_depth, dst, stride);
 }
 
+// Per-kernel seam, batched: one intra::predict (intra.rs:19-29) per
+// workgroup, block i's edges at edges + desc[i].edge_off (aboveRow[-1 ..
+// 2 size - 1], then leftCol[0 .. size - 1]), its prediction written at
+// dst + desc[i].dst_off samples with row stride `stride`.
+template <bool HBD>
+__global__ __launch_bounds__(64) void k_intra_pred_batch(const vp9r_intra_desc *desc, const int32_t *edges,
+                                                         void *dst_, uint32_t stride, int bd) {
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  __shared__ int32_t e[65 + 32];
+  const vp9r_intra_desc d = desc[blockIdx.x];
+  const int log2 = d.log2, size = 1 << log2;
+  const int32_t *src = edges + d.edge_off;
+  for (int k = threadIdx.x; k <= 2 * size; k += 64) e[k] = src[k];
+  for (int k = threadIdx.x; k < size; k += 64) e[65 + k] = src[2 * size + 1 + k];
+  __syncthreads();
+  const int dc = dc_value(log2, e + 1, e + 65, d.have_left, d.have_above, bd);
+  P *dst = (P *)dst_ + d.dst_off;
+  for (int k = threadIdx.x; k < size * size; k += 64) {
+    const int i = k >> log2, j = k & (size - 1);
+    dst[(size_t)i * stride + j] = (P)intra_sample(d.mode, i, j, size, e + 1, e + 65, dc, (1 << bd) - 1);
+  }
+}
+
+void launch_intra_pred_batch(hipStream_t s, uint32_t count, const vp9r_intra_desc *desc, const int32_t *edges,
+                             void *dst, uint32_t stride, uint32_t bit_depth) {
+  if (!count) return;
+  if (bit_depth > 8)
+    hipLaunchKernelGGL(k_intra_pred_batch<true>, dim3(count), dim3(64), 0, s, desc, edges, dst, stride,
+                       (int)bit_depth);
+  else
+    hipLaunchKernelGGL(k_intra_pred_batch<false>, dim3(count), dim3(64), 0, s, desc, edges, dst, stride,
+                       (int)bit_depth);
+}
+
 }  // namespace vp9g

@@ -244,14 +244,23 @@ void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const
                       const uint8_t *tx, const uint8_t *tx_type, const uint16_t *eob,
                       const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
                       int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs);
-void launch_mc_desc(hipStream_t s, uint32_t count, const vp9r_mc_desc *desc, const void *const *refs,
-                    uint32_t ref_stride, int32_t last_x, int32_t last_y, void *out,
-                    uint32_t out_stride, uint32_t bit_depth);
-void launch_lf_edges(hipStream_t s, uint32_t count, const uint8_t *edges, const uint16_t *x0,
-                     const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
-                     const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t bit_depth);
+void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,
+                          const vp9r_ref_plane *planes, void *out, uint32_t out_stride, uint32_t bd);
+// one pass of lf::filter_edges: edges [first, n_edges) of the region at (x0, y0)
+struct LfPassDesc {
+  uint16_t x0, y0;
+  uint8_t vertical, first, n_edges, n_runs;
+};
+void launch_lf_edges(hipStream_t s, const uint8_t *tables, const LfPassDesc *passes, uint32_t count, void *buf,
+                     uint32_t stride, uint32_t bit_depth);
 void launch_intra_pred(hipStream_t s, uint32_t mode, uint32_t log2, const int32_t *above,
                        const int32_t *left, int have_left, int have_above, uint32_t bit_depth,
                        uint16_t *dst, uint32_t stride);
+void launch_intra_pred_batch(hipStream_t s, uint32_t count, const vp9r_intra_desc *desc, const int32_t *edges,
+                             void *dst, uint32_t stride, uint32_t bit_depth);
+void launch_avg_batch(hipStream_t s, uint32_t count, const vp9r_avg_desc *desc, const void *a, const void *b,
+                      void *dst, uint32_t stride, uint32_t bit_depth);
+void launch_add_const_batch(hipStream_t s, uint32_t count, const vp9r_add_const_desc *desc, void *dst,
+                            uint32_t stride);
 
 }  // namespace vp9g

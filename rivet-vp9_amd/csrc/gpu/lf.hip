@@ -168,60 +168,62 @@ void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n
   }
 }
 
-// Per-kernel seam: lf::filter_edges on global samples, one pass per
-// workgroup launch (passes are ordered by the stream).
+// Per-kernel seam: lf::filter_edges (lf.rs:178-218) over a sequence of
+// passes, in order, by one workgroup: each pass stages its region (edges
+// reach 8 samples back and 8 forward) in LDS, filters edges first ..
+// n_edges - 1 of it (one lane per line along the edges), and writes the
+// region back before the next pass reads the plane.
 template <bool HBD>
-__global__ __launch_bounds__(64) void k_lf_edges(const uint8_t *e, int x0, int y0, int vertical,
-                                                 int n_edges, int n_runs, void *buf_, uint32_t stride,
-                                                 int bd) {
+__global__ __launch_bounds__(64) void k_lf_edges(const uint8_t *tables, const LfPassDesc *passes, uint32_t count,
+                                                 void *buf_, uint32_t stride, int bd) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  // stage the pass's region (edges reach 8 back and 8 forward) in LDS
   __shared__ uint16_t buf[80 * 80];
   P *g = (P *)buf_;
-  const int8_t *fs = (const int8_t *)e;
-  const uint8_t *lim = e + 256, *blim = e + 512, *thr = e + 768;
-  const int ext_a = 4 * n_edges + 8, ext_r = 4 * n_runs;  // across, along
-  const int W = vertical ? ext_a + 8 : ext_r, H = vertical ? ext_r : ext_a + 8;
-  const int gx0 = vertical ? x0 - 8 : x0, gy0 = vertical ? y0 : y0 - 8;
-  for (int k = threadIdx.x; k < W * H; k += 64) {
-    int r = k / W, c = k - r * W;
-    if (gy0 + r >= 0 && gx0 + c >= 0) buf[r * 80 + c] = g[(size_t)(gy0 + r) * stride + gx0 + c];
-  }
-  __syncthreads();
-  const int lane = threadIdx.x;
-  if (lane < ext_r) {
-    const int rn = lane >> 2;
-    for (int ed = 0; ed < n_edges; ed++) {
-      const int f = fs[ed * 16 + rn];
-      if (f < 0) continue;
-      const int i = ed * 16 + rn;
-      if (vertical)
-        lf_filter(buf, lane * 80 + 8 + 4 * ed, 1, f, lim[i], blim[i], thr[i], bd);
-      else
-        lf_filter(buf, (8 + 4 * ed) * 80 + lane, 80, f, lim[i], blim[i], thr[i], bd);
+  for (uint32_t i = 0; i < count; i++) {
+    const LfPassDesc ps = passes[i];
+    const int8_t *fs = (const int8_t *)(tables + 1024 * (size_t)i);
+    const uint8_t *lim = tables + 1024 * (size_t)i + 256, *blim = lim + 256, *thr = lim + 512;
+    const int x0 = ps.x0, y0 = ps.y0, vertical = ps.vertical, n_edges = ps.n_edges, n_runs = ps.n_runs;
+    const int ext_a = 4 * n_edges + 8, ext_r = 4 * n_runs;  // across, along
+    const int W = vertical ? ext_a + 8 : ext_r, H = vertical ? ext_r : ext_a + 8;
+    const int gx0 = vertical ? x0 - 8 : x0, gy0 = vertical ? y0 : y0 - 8;
+    for (int k = threadIdx.x; k < W * H; k += 64) {
+      int r = k / W, c = k - r * W;
+      if (gy0 + r >= 0 && gx0 + c >= 0) buf[r * 80 + c] = g[(size_t)(gy0 + r) * stride + gx0 + c];
     }
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < W * H; k += 64) {
-    int r = k / W, c = k - r * W;
-    if (gy0 + r >= 0 && gx0 + c >= 0) g[(size_t)(gy0 + r) * stride + gx0 + c] = (P)buf[r * 80 + c];
+    __syncthreads();
+    const int lane = threadIdx.x;
+    if (lane < ext_r) {
+      const int rn = lane >> 2;
+      for (int ed = ps.first; ed < n_edges; ed++) {
+        const int f = fs[ed * 16 + rn];
+        if (f < 0) continue;
+        const int j = ed * 16 + rn;
+        if (vertical)
+          lf_filter(buf, lane * 80 + 8 + 4 * ed, 1, f, lim[j], blim[j], thr[j], bd);
+        else
+          lf_filter(buf, (8 + 4 * ed) * 80 + lane, 80, f, lim[j], blim[j], thr[j], bd);
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < W * H; k += 64) {
+      int r = k / W, c = k - r * W;
+      if (gy0 + r >= 0 && gx0 + c >= 0) g[(size_t)(gy0 + r) * stride + gx0 + c] = (P)buf[r * 80 + c];
+    }
+    __syncthreads();  // the next pass reads what this one wrote
   }
 }
 
-// lf_filter's limits are unshifted values; limit / blimit / thresh tables
-// per level are not used here: the Edges tables carry the values.
-void launch_lf_edges(hipStream_t s, uint32_t count, const uint8_t *edges, const uint16_t *x0,
-                     const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
-                     const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t bit_depth) {
-  // host arrays for the geometry, device pointer for the tables
-  for (uint32_t i = 0; i < count; i++) {
-    if (bit_depth > 8)
-      hipLaunchKernelGGL(k_lf_edges<true>, dim3(1), dim3(64), 0, s, edges + 1024 * (size_t)i, x0[i], y0[i],
-                         vertical[i], n_edges[i], n_runs[i], buf, stride, (int)bit_depth);
-    else
-      hipLaunchKernelGGL(k_lf_edges<false>, dim3(1), dim3(64), 0, s, edges + 1024 * (size_t)i, x0[i], y0[i],
-                         vertical[i], n_edges[i], n_runs[i], buf, stride, (int)bit_depth);
-  }
+// lf_filter's limits are unshifted values (the Edges tables carry them).
+void launch_lf_edges(hipStream_t s, const uint8_t *tables, const LfPassDesc *passes, uint32_t count, void *buf,
+                     uint32_t stride, uint32_t bit_depth) {
+  if (!count) return;
+  if (bit_depth > 8)
+    hipLaunchKernelGGL(k_lf_edges<true>, dim3(1), dim3(64), 0, s, tables, passes, count, buf, stride,
+                       (int)bit_depth);
+  else
+    hipLaunchKernelGGL(k_lf_edges<false>, dim3(1), dim3(64), 0, s, tables, passes, count, buf, stride,
+                       (int)bit_depth);
 }
 
 }  // namespace vp9g

@@ -475,16 +475,15 @@ struct DescTile {
 // descriptor through the general path, written at dst (the tile's first
 // sample of those rows).
 template <class P>
-DEV void mc_desc_half(const vp9r_mc_desc &d, const void *const *refs, uint32_t ref_stride, int32_t last_x,
-                      int32_t last_y, P *dst, uint32_t out_stride, int tx, int ty, int tw, int th, int max,
-                      McShared &sh) {
+DEV void mc_desc_half(const vp9r_mc_desc &d, const vp9r_ref_plane &r0, const vp9r_ref_plane &r1, P *dst,
+                      uint32_t out_stride, int tx, int ty, int tw, int th, int max, McShared &sh) {
   const bool compound = d.ref2_idx >= 0;
   int o0[4], o1[4];
-  mc_tile<P>((const P *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + d.xs * tx, d.y + d.ys * ty, d.xs, d.ys,
-             tw, th, d.filter, max, sh, o0);
+  mc_tile<P>((const P *)r0.data, r0.stride, r0.last_x, r0.last_y, d.x + d.xs * tx, d.y + d.ys * ty, d.xs, d.ys, tw,
+             th, d.filter, max, sh, o0);
   if (compound)
-    mc_tile<P>((const P *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + d.xs * tx, d.y2 + d.ys * ty, d.xs,
-               d.ys, tw, th, d.filter, max, sh, o1);
+    mc_tile<P>((const P *)r1.data, r1.stride, r1.last_x, r1.last_y, d.x2 + d.xs * tx, d.y2 + d.ys * ty, d.xs, d.ys,
+               tw, th, d.filter, max, sh, o1);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int o = threadIdx.x + 64 * i;
@@ -497,24 +496,25 @@ DEV void mc_desc_half(const vp9r_mc_desc &d, const void *const *refs, uint32_t r
 
 template <bool HBD>
 __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const DescTile *tiles,
-                                                const void *const *refs, uint32_t ref_stride,
-                                                int32_t last_x, int32_t last_y, void *out,
-                                                uint32_t out_stride, int bd) {
+                                                const vp9r_ref_plane *planes, void *out, uint32_t out_stride,
+                                                int bd) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ McLds sh;
   const DescTile t = tiles[blockIdx.x];
   const vp9r_mc_desc d = descs[t.desc];
+  // each reference its own plane (inter.rs:21-27 RefPlane)
+  const vp9r_ref_plane r0 = planes[d.ref_idx], r1 = planes[d.ref2_idx >= 0 ? d.ref2_idx : d.ref_idx];
   const int max = (1 << bd) - 1;
   const int th = d.h - t.ty < 2 * TILE ? d.h - t.ty : 2 * TILE, tw = d.w - t.tx < TILE ? d.w - t.tx : TILE;
   const bool compound = d.ref2_idx >= 0;
   P *dst = (P *)out + d.out_off + (size_t)t.ty * out_stride + t.tx;
   if (!HBD && d.xs == 16 && d.ys == 16 && ((tw | th | out_stride | (uint32_t)(uintptr_t)dst) & 3) == 0) {
     uint32_t v[2], v2[2];
-    mc_tile8<32>((const uint8_t *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
+    mc_tile8<32>((const uint8_t *)r0.data, r0.stride, r0.last_x, r0.last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
                  th, d.filter, sh.f32, v);
     if (compound) {
-      mc_tile8<32>((const uint8_t *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + 16 * t.tx,
-                   d.y2 + 16 * t.ty, tw, th, d.filter, sh.f32, v2);
+      mc_tile8<32>((const uint8_t *)r1.data, r1.stride, r1.last_x, r1.last_y, d.x2 + 16 * t.tx, d.y2 + 16 * t.ty,
+                   tw, th, d.filter, sh.f32, v2);
       v[0] = avg4(v[0], v2[0]);
       v[1] = avg4(v[1], v2[1]);
     }
@@ -529,11 +529,11 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
   if (HBD && d.xs == 16 && d.ys == 16 && ((tw | th) & 3) == 0 && (out_stride & 1) == 0 &&
       ((uintptr_t)dst & 3) == 0) {
     uint32_t v[2][2], v2[2][2];
-    mc_tile16<32>((const uint16_t *)refs[d.ref_idx], ref_stride, last_x, last_y, d.x + 16 * t.tx, d.y + 16 * t.ty,
-                  tw, th, d.filter, max, sh.h32, v);
+    mc_tile16<32>((const uint16_t *)r0.data, r0.stride, r0.last_x, r0.last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
+                  th, d.filter, max, sh.h32, v);
     if (compound) {
-      mc_tile16<32>((const uint16_t *)refs[d.ref2_idx], ref_stride, last_x, last_y, d.x2 + 16 * t.tx,
-                    d.y2 + 16 * t.ty, tw, th, d.filter, max, sh.h32, v2);
+      mc_tile16<32>((const uint16_t *)r1.data, r1.stride, r1.last_x, r1.last_y, d.x2 + 16 * t.tx, d.y2 + 16 * t.ty,
+                    tw, th, d.filter, max, sh.h32, v2);
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         v[h][0] = avg2(v[h][0], v2[h][0]);
@@ -554,8 +554,8 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
   }
   // the general path: the tile's rows in halves of at most 16
   for (int hh = 0; hh < th; hh += TILE)
-    mc_desc_half<P>(d, refs, ref_stride, last_x, last_y, dst + (size_t)hh * out_stride, out_stride, t.tx,
-                    t.ty + hh, tw, th - hh < TILE ? th - hh : TILE, max, sh.g);
+    mc_desc_half<P>(d, r0, r1, dst + (size_t)hh * out_stride, out_stride, t.tx, t.ty + hh, tw,
+                    th - hh < TILE ? th - hh : TILE, max, sh.g);
 }
 
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
@@ -568,16 +568,15 @@ void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint
 }
 
 void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,
-                          const void *const *refs, uint32_t ref_stride, int32_t last_x,
-                          int32_t last_y, void *out, uint32_t out_stride, uint32_t bd) {
+                          const vp9r_ref_plane *planes, void *out, uint32_t out_stride, uint32_t bd) {
   upload_filters();
   if (!n_tiles) return;
   if (bd > 8)
-    hipLaunchKernelGGL(k_mc_desc<true>, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles,
-                       refs, ref_stride, last_x, last_y, out, out_stride, (int)bd);
+    hipLaunchKernelGGL(k_mc_desc<true>, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles, planes, out,
+                       out_stride, (int)bd);
   else
-    hipLaunchKernelGGL(k_mc_desc<false>, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles,
-                       refs, ref_stride, last_x, last_y, out, out_stride, (int)bd);
+    hipLaunchKernelGGL(k_mc_desc<false>, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles, planes, out,
+                       out_stride, (int)bd);
 }
 
 }  // namespace vp9g

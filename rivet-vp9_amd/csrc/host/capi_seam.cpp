@@ -8,12 +8,6 @@
 
 using namespace vp9r;
 
-namespace vp9g {
-void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,
-                          const void *const *refs, uint32_t ref_stride, int32_t last_x, int32_t last_y,
-                          void *out, uint32_t out_stride, uint32_t bd);
-}
-
 static int dev_err(hipError_t e, const char *what) {
   set_error(std::string(what) + ": " + hipGetErrorString(e));
   return VP9R_ERR_DEVICE;
@@ -109,13 +103,18 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
   return VP9R_OK;
 }
 
-int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const void *const *refs,
-                  uint32_t n_refs, uint32_t ref_stride, int32_t last_x, int32_t last_y, void *out,
-                  uint32_t out_stride, uint32_t bit_depth) {
-  if (!bd_ok(bit_depth) || n_refs == 0 || n_refs > 64 || last_x < 0 || last_y < 0) {
+int vp9r_mc_batch_planes(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const vp9r_ref_plane *planes,
+                         uint32_t n_refs, void *out, uint32_t out_stride, uint32_t bit_depth) {
+  if (!bd_ok(bit_depth) || n_refs == 0 || n_refs > 64) {
     set_error("invalid motion compensation batch");
     return VP9R_ERR_INVALID_INPUT;
   }
+  for (uint32_t k = 0; k < n_refs; k++)
+    if (!planes[k].data || planes[k].last_x < 0 || planes[k].last_y < 0 ||
+        (uint32_t)planes[k].last_x >= planes[k].stride) {
+      set_error("invalid reference plane");
+      return VP9R_ERR_INVALID_INPUT;
+    }
   struct DescTile {
     uint32_t desc;
     uint16_t tx, ty;
@@ -139,14 +138,14 @@ int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const v
   size_t dbytes = sizeof(vp9r_mc_desc) * count, tbytes = sizeof(DescTile) * tiles.size();
   TRY(g->seam_a.ensure(dbytes + 256), "hipMalloc");
   TRY(g->seam_b.ensure(tbytes + 256), "hipMalloc");
-  TRY(g->seam_c.ensure(sizeof(void *) * n_refs), "hipMalloc");
+  TRY(g->seam_c.ensure(sizeof(vp9r_ref_plane) * n_refs), "hipMalloc");
   TRY(hipMemcpyAsync(g->seam_a.p, desc, dbytes, hipMemcpyHostToDevice, g->stream), "upload");
   TRY(hipMemcpyAsync(g->seam_b.p, tiles.data(), tbytes, hipMemcpyHostToDevice, g->stream), "upload");
-  TRY(hipMemcpyAsync(g->seam_c.p, refs, sizeof(void *) * n_refs, hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(hipMemcpyAsync(g->seam_c.p, planes, sizeof(vp9r_ref_plane) * n_refs, hipMemcpyHostToDevice, g->stream),
+      "upload");
   TRY(seam_begin(g), "hipEventRecord");
   vp9g::launch_mc_desc_tiles(g->stream, (const vp9r_mc_desc *)g->seam_a.p, g->seam_b.p, (uint32_t)tiles.size(),
-                             (const void *const *)g->seam_c.p, ref_stride, last_x, last_y, out, out_stride,
-                             bit_depth);
+                             (const vp9r_ref_plane *)g->seam_c.p, out, out_stride, bit_depth);
   TRY(hipGetLastError(), "mc launch");
   TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "mc sync");
@@ -154,31 +153,127 @@ int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const v
   return VP9R_OK;
 }
 
-int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const uint16_t *x0,
-                        const uint16_t *y0, const uint8_t *vertical, const uint8_t *n_edges,
-                        const uint8_t *n_runs, void *buf, uint32_t stride, uint32_t rows,
-                        uint32_t bit_depth) {
+int vp9r_mc_batch(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const void *const *refs,
+                  uint32_t n_refs, uint32_t ref_stride, int32_t last_x, int32_t last_y, void *out,
+                  uint32_t out_stride, uint32_t bit_depth) {
+  if (n_refs == 0 || n_refs > 64) {
+    set_error("invalid motion compensation batch");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  std::vector<vp9r_ref_plane> planes(n_refs);
+  for (uint32_t k = 0; k < n_refs; k++) planes[k] = {refs[k], ref_stride, last_x, last_y, 0};
+  return vp9r_mc_batch_planes(g, count, desc, planes.data(), n_refs, out, out_stride, bit_depth);
+}
+
+int vp9r_avg_batch(vp9r_gpu *g, uint32_t count, const vp9r_avg_desc *desc, const void *a, const void *b,
+                   void *dst, uint32_t stride, uint32_t bit_depth) {
   if (!bd_ok(bit_depth)) {
     set_error("bit depth must be 8, 10 or 12");
     return VP9R_ERR_INVALID_INPUT;
   }
+  for (uint32_t i = 0; i < count; i++)
+    if (desc[i].w < 1 || desc[i].w > 64 || desc[i].h < 1 || desc[i].h > 64 || desc[i].w > stride) {
+      set_error("average block out of range");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  TRY(g->seam_a.ensure(sizeof(vp9r_avg_desc) * std::max<uint32_t>(count, 1)), "hipMalloc");
+  if (count)
+    TRY(hipMemcpyAsync(g->seam_a.p, desc, sizeof(vp9r_avg_desc) * count, hipMemcpyHostToDevice, g->stream),
+        "upload");
+  TRY(seam_begin(g), "hipEventRecord");
+  vp9g::launch_avg_batch(g->stream, count, (const vp9r_avg_desc *)g->seam_a.p, a, b, dst, stride, bit_depth);
+  TRY(hipGetLastError(), "avg launch");
+  TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
+  TRY(hipStreamSynchronize(g->stream), "avg sync");
+  TRY(seam_end(g), "hipEventElapsedTime");
+  return VP9R_OK;
+}
+
+int vp9r_add_const_batch(vp9r_gpu *g, uint32_t count, const vp9r_add_const_desc *desc, void *dst,
+                         uint32_t stride) {
+  for (uint32_t i = 0; i < count; i++)
+    if (desc[i].n < 1 || desc[i].n > 64 || desc[i].n > stride) {
+      set_error("block out of range");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  TRY(g->seam_a.ensure(sizeof(vp9r_add_const_desc) * std::max<uint32_t>(count, 1)), "hipMalloc");
+  if (count)
+    TRY(hipMemcpyAsync(g->seam_a.p, desc, sizeof(vp9r_add_const_desc) * count, hipMemcpyHostToDevice, g->stream),
+        "upload");
+  TRY(seam_begin(g), "hipEventRecord");
+  vp9g::launch_add_const_batch(g->stream, count, (const vp9r_add_const_desc *)g->seam_a.p, dst, stride);
+  TRY(hipGetLastError(), "add_const launch");
+  TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
+  TRY(hipStreamSynchronize(g->stream), "add_const sync");
+  TRY(seam_end(g), "hipEventElapsedTime");
+  return VP9R_OK;
+}
+
+int vp9r_intra_predict_batch(vp9r_gpu *g, uint32_t count, const vp9r_intra_desc *desc, const int32_t *edges,
+                             void *dst, uint32_t stride, uint32_t bit_depth) {
+  if (!bd_ok(bit_depth)) {
+    set_error("bit depth must be 8, 10 or 12");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  for (uint32_t i = 0; i < count; i++)
+    if (desc[i].mode > 9 || desc[i].log2 < 2 || desc[i].log2 > 5 || (1u << desc[i].log2) > stride) {
+      set_error("invalid intra prediction");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  TRY(g->seam_a.ensure(sizeof(vp9r_intra_desc) * std::max<uint32_t>(count, 1)), "hipMalloc");
+  if (count)
+    TRY(hipMemcpyAsync(g->seam_a.p, desc, sizeof(vp9r_intra_desc) * count, hipMemcpyHostToDevice, g->stream),
+        "upload");
+  TRY(seam_begin(g), "hipEventRecord");
+  vp9g::launch_intra_pred_batch(g->stream, count, (const vp9r_intra_desc *)g->seam_a.p, edges, dst, stride,
+                                bit_depth);
+  TRY(hipGetLastError(), "intra launch");
+  TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
+  TRY(hipStreamSynchronize(g->stream), "intra sync");
+  TRY(seam_end(g), "hipEventElapsedTime");
+  return VP9R_OK;
+}
+
+int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const uint16_t *x0,
+                        const uint16_t *y0, const uint8_t *vertical, const uint8_t *first,
+                        const uint8_t *n_edges, const uint8_t *n_runs, void *buf, uint32_t stride,
+                        uint32_t rows, uint32_t bit_depth) {
+  if (!bd_ok(bit_depth)) {
+    set_error("bit depth must be 8, 10 or 12");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  std::vector<vp9g::LfPassDesc> passes(count);
   for (uint32_t i = 0; i < count; i++) {
     uint32_t across = vertical[i] ? x0[i] : y0[i], along = vertical[i] ? y0[i] : x0[i];
     uint32_t lim_a = vertical[i] ? stride : rows, lim_r = vertical[i] ? rows : stride;
-    bool ok = n_edges[i] >= 1 && n_edges[i] <= 16 && n_runs[i] >= 1 && n_runs[i] <= 16 &&
+    const uint32_t f = first ? first[i] : 0;
+    bool ok = n_edges[i] >= 1 && n_edges[i] <= 16 && n_runs[i] >= 1 && n_runs[i] <= 16 && f <= n_edges[i] &&
               across + 4u * n_edges[i] + 8 <= lim_a && along + 4u * n_runs[i] <= lim_r;
     if (!ok) {
       set_error("loop filter pass out of range of the plane");
       return VP9R_ERR_INVALID_INPUT;
     }
+    passes[i] = {x0[i], y0[i], (uint8_t)(vertical[i] ? 1 : 0), (uint8_t)f, n_edges[i], n_runs[i]};
   }
   TRY(hipSetDevice(g->device), "hipSetDevice");
   TRY(g->seam_a.ensure(1024 * (size_t)std::max<uint32_t>(count, 1)), "hipMalloc");
-  TRY(hipMemcpyAsync(g->seam_a.p, edges, 1024 * (size_t)count, hipMemcpyHostToDevice, g->stream), "upload");
-  vp9g::launch_lf_edges(g->stream, count, (const uint8_t *)g->seam_a.p, x0, y0, vertical, n_edges, n_runs, buf,
+  TRY(g->seam_b.ensure(sizeof(vp9g::LfPassDesc) * std::max<uint32_t>(count, 1)), "hipMalloc");
+  if (count) {
+    TRY(hipMemcpyAsync(g->seam_a.p, edges, 1024 * (size_t)count, hipMemcpyHostToDevice, g->stream), "upload");
+    TRY(hipMemcpyAsync(g->seam_b.p, passes.data(), sizeof(vp9g::LfPassDesc) * count, hipMemcpyHostToDevice,
+                       g->stream),
+        "upload");
+  }
+  TRY(seam_begin(g), "hipEventRecord");
+  vp9g::launch_lf_edges(g->stream, (const uint8_t *)g->seam_a.p, (const vp9g::LfPassDesc *)g->seam_b.p, count, buf,
                         stride, bit_depth);
   TRY(hipGetLastError(), "lf launch");
+  TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "lf sync");
+  TRY(seam_end(g), "hipEventElapsedTime");
   return VP9R_OK;
 }
 

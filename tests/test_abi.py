@@ -64,6 +64,8 @@ int main(void) {
          sizeof(vp9r_mi), sizeof(vp9r_record), sizeof(vp9r_frame), sizeof(vp9r_mc_desc));
   printf("%zu %zu %zu\n", offsetof(vp9r_frame_hdr, seg_feature_data), offsetof(vp9r_block, mv),
          offsetof(vp9r_record, mi));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(vp9r_ref_plane), sizeof(vp9r_avg_desc), sizeof(vp9r_add_const_desc),
+         sizeof(vp9r_intra_desc), offsetof(vp9r_intra_desc, mode), offsetof(vp9r_ref_plane, last_y));
   return 0;
 }
 """
@@ -77,7 +79,9 @@ int main(void) {
     sizes = [int(v) for v in out]
     want = [ctypes.sizeof(vp.FrameHdr), ctypes.sizeof(vp.Block), ctypes.sizeof(vp.Txb), ctypes.sizeof(vp.Mi),
             ctypes.sizeof(vp.Record), ctypes.sizeof(vp.FrameDesc), ctypes.sizeof(vp.McDesc),
-            vp.FrameHdr.seg_feature_data.offset, vp.Block.mv.offset, vp.Record.mi.offset]
+            vp.FrameHdr.seg_feature_data.offset, vp.Block.mv.offset, vp.Record.mi.offset,
+            ctypes.sizeof(vp.RefPlane), ctypes.sizeof(vp.AvgDesc), ctypes.sizeof(vp.AddConstDesc),
+            ctypes.sizeof(vp.IntraDesc), vp.IntraDesc.mode.offset, vp.RefPlane.last_y.offset]
     assert sizes == want
 
 

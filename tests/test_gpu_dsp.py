@@ -223,7 +223,7 @@ def test_lf_edges_matches_oracle(gpu, bd):
         pic = picture(rng, w, h, max_v, it % 5)
         n_passes = 4
         tables = np.zeros((n_passes, 1024), dtype=np.uint8)
-        x0s, y0s, verts, nes, nrs = [], [], [], [], []
+        x0s, y0s, verts, nes, nrs, firsts = [], [], [], [], [], []
         want = pic.copy()
         for p in range(n_passes):
             e = oracle_lib.Edges()
@@ -245,15 +245,171 @@ def test_lf_edges_matches_oracle(gpu, bd):
             n_runs = 8 if it % 3 == 0 else 16
             n_edges = 8 if it % 4 == 0 else 16
             x0, y0 = 16, 16
-            olib.orc_lf_filter_edges(ptr(want), w, x0, y0, int(vertical), 0, n_edges, n_runs, ctypes.byref(e), bd)
+            # edges from `first` on (lf.rs:178-190), as the reference's
+            # callers resume a pass
+            first = 0 if it % 2 == 0 else int(rng.integers(0, n_edges))
+            olib.orc_lf_filter_edges(ptr(want), w, x0, y0, int(vertical), first, n_edges, n_runs, ctypes.byref(e),
+                                     bd)
             x0s.append(x0), y0s.append(y0), verts.append(int(vertical)), nes.append(n_edges), nrs.append(n_runs)
+            firsts.append(first)
         d = gpu.upload(pic.astype(dt))
         st = lib.vp9r_lf_edges_batch(gpu.handle, n_passes, ptr(tables), ptr(np.array(x0s, np.uint16)),
                                      ptr(np.array(y0s, np.uint16)), ptr(np.array(verts, np.uint8)),
-                                     ptr(np.array(nes, np.uint8)), ptr(np.array(nrs, np.uint8)), d.ptr, w, h, bd)
+                                     ptr(np.array(firsts, np.uint8)), ptr(np.array(nes, np.uint8)),
+                                     ptr(np.array(nrs, np.uint8)), d.ptr, w, h, bd)
         assert st == 0
         got = d.read(dt, w * h).reshape(h, w).astype(np.uint16)
         assert np.array_equal(got, want), f"it={it} bd={bd}"
+
+
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_mc_batch_per_reference_planes(gpu, bd):
+    """inter::predict with each reference its own RefPlane (inter.rs:21-27):
+    three planes of different strides and visible sizes (as references of
+    another frame size are), single and compound across them."""
+    rng = np.random.default_rng(700 + bd)
+    lib = vp.library()
+    olib = oracle_lib.lib()
+    max_v = (1 << bd) - 1
+    dt = np.uint8 if bd == 8 else np.uint16
+    geo = [(150, 90, 160), (76, 45, 96), (300, 180, 320)]  # visible w, h, stride
+    refs, planes, keep = [], (vp.RefPlane * 3)(), []
+    for k, (pw, ph, stride) in enumerate(geo):
+        r = rng.integers(0, max_v + 1, size=(ph + 4, stride)).astype(np.uint16)
+        refs.append(r)
+        d = gpu.upload(r.astype(dt))
+        keep.append(d)
+        planes[k].data, planes[k].stride, planes[k].last_x, planes[k].last_y = d.ptr.value, stride, pw - 1, ph - 1
+    out_stride = 64 * 8
+    descs, expected = [], []
+    for i in range(72):
+        w, h = [(8, 8), (16, 8), (8, 16), (16, 16), (32, 32), (64, 64), (4, 4), (32, 16)][i % 8]
+        a_k, b_k = int(rng.integers(0, 3)), int(rng.integers(0, 3))
+        pw, ph, stride = geo[a_k]
+        x = int(rng.integers(-20 * 16, (pw + 10) * 16))
+        y = int(rng.integers(-20 * 16, (ph + 10) * 16))
+        xs, ys = (16, 16) if i % 3 else (int(rng.integers(8, 33)), int(rng.integers(8, 33)))
+        f = int(rng.integers(0, 4))
+        comp = i % 4 == 1
+        x2, y2 = x + int(rng.integers(-64, 64)), y + int(rng.integers(-64, 64))
+        d = vp.McDesc()
+        d.x, d.y, d.x2, d.y2, d.xs, d.ys, d.w, d.h, d.filter = x, y, x2, y2, xs, ys, w, h, f
+        d.ref_idx, d.ref2_idx = a_k, (b_k if comp else -1)
+        d.out_off = (i // 8) * 64 * out_stride + (i % 8) * 64
+        descs.append(d)
+        a = np.zeros((h, w), dtype=np.uint16)
+        olib.orc_inter_predict(ptr(refs[a_k]), stride, pw - 1, ph - 1, x, y, xs, ys, w, h, f, bd, ptr(a), w)
+        if comp:
+            bw, bh, bs = geo[b_k]
+            b = np.zeros((h, w), dtype=np.uint16)
+            olib.orc_inter_predict(ptr(refs[b_k]), bs, bw - 1, bh - 1, x2, y2, xs, ys, w, h, f, bd, ptr(b), w)
+            c = np.zeros((h, w), dtype=np.uint16)
+            olib.orc_avg(ptr(a), ptr(b), w, h, ptr(c), w)
+            a = c
+        expected.append(a)
+    rows = (len(descs) // 8 + 1) * 64
+    d_out = gpu.upload(np.zeros(rows * out_stride, dtype=dt))
+    arr = (vp.McDesc * len(descs))(*descs)
+    assert lib.vp9r_mc_batch_planes(gpu.handle, len(descs), ctypes.cast(arr, ctypes.c_void_p),
+                                    ctypes.cast(planes, ctypes.c_void_p), 3, d_out.ptr, out_stride, bd) == 0
+    out = d_out.read(dt, rows * out_stride).reshape(rows, out_stride).astype(np.uint16)
+    for i, d in enumerate(descs):
+        r0, c0 = divmod(d.out_off, out_stride)
+        assert np.array_equal(out[r0:r0 + d.h, c0:c0 + d.w], expected[i]), f"desc {i}"
+
+
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_avg_batch_matches_oracle(gpu, bd):
+    """pixel::avg (pixel.rs:22-50) as a batched launch."""
+    rng = np.random.default_rng(800 + bd)
+    lib = vp.library()
+    olib = oracle_lib.lib()
+    max_v = (1 << bd) - 1
+    dt = np.uint8 if bd == 8 else np.uint16
+    stride = 256
+    n = 40
+    descs = (vp.AvgDesc * n)()
+    a_all, b_all, want = [], [], np.zeros((n * 64, stride), dtype=np.uint16)
+    off = 0
+    for i in range(n):
+        w, h = [(4, 4), (8, 8), (16, 8), (64, 64), (32, 64), (8, 32), (64, 16), (4, 8)][i % 8]
+        a = rng.integers(0, max_v + 1, size=(h, w)).astype(np.uint16)
+        b = rng.integers(0, max_v + 1, size=(h, w)).astype(np.uint16)
+        descs[i].a_off = descs[i].b_off = off
+        descs[i].dst_off = i * 64 * stride + (i % 3) * 64
+        descs[i].w, descs[i].h = w, h
+        a_all.append(a.ravel()), b_all.append(b.ravel())
+        off += w * h
+        c = np.zeros((h, w), dtype=np.uint16)
+        olib.orc_avg(ptr(a), ptr(b), w, h, ptr(c), w)
+        r0, c0 = divmod(descs[i].dst_off, stride)
+        want[r0:r0 + h, c0:c0 + w] = c
+    da = gpu.upload(np.concatenate(a_all).astype(dt))
+    db = gpu.upload(np.concatenate(b_all).astype(dt))
+    dd = gpu.upload(np.zeros(n * 64 * stride, dtype=dt))
+    assert lib.vp9r_avg_batch(gpu.handle, n, ctypes.cast(descs, ctypes.c_void_p), da.ptr, db.ptr, dd.ptr, stride,
+                              bd) == 0
+    got = dd.read(dt, n * 64 * stride).reshape(n * 64, stride).astype(np.uint16)
+    assert np.array_equal(got, want)
+
+
+def test_add_const_batch_matches_oracle(gpu):
+    """pixel::add_const (pixel.rs:9-16): the clamp of v to +-65536 and of
+    the result to 0..255, at every block size."""
+    rng = np.random.default_rng(900)
+    lib = vp.library()
+    olib = oracle_lib.lib()
+    stride = 128
+    plane = rng.integers(0, 256, size=(64 * 8, stride)).astype(np.uint16)
+    want = plane.copy()
+    n = 32
+    descs = (vp.AddConstDesc * n)()
+    for i in range(n):
+        size = [4, 8, 16, 32][i % 4]
+        v = int([rng.integers(-300, 300), rng.integers(-10, 10), 70000, -70000][i % 4 if i % 8 < 4 else 1])
+        r0, c0 = (i // 4) * 64, (i % 4) * 32  # disjoint blocks
+        descs[i].dst_off, descs[i].v, descs[i].n = r0 * stride + c0, v, size
+        blk = np.ascontiguousarray(want[r0:r0 + size, c0:c0 + size])
+        olib.orc_add_const(ptr(blk), size, size, v)
+        want[r0:r0 + size, c0:c0 + size] = blk
+    dd = gpu.upload(plane.astype(np.uint8))
+    assert lib.vp9r_add_const_batch(gpu.handle, n, ctypes.cast(descs, ctypes.c_void_p), dd.ptr, stride) == 0
+    got = dd.read(np.uint8, plane.size).reshape(plane.shape).astype(np.uint16)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_intra_predict_batch_matches_oracle(gpu, bd):
+    """intra::predict (intra.rs:19-175) as a batched device launch: every
+    mode, size and edge availability in one call."""
+    rng = np.random.default_rng(1000 + bd)
+    lib = vp.library()
+    olib = oracle_lib.lib()
+    max_v = (1 << bd) - 1
+    dt = np.uint8 if bd == 8 else np.uint16
+    stride = 256
+    cases = [(m, l, hl, ha) for m in range(10) for l in (2, 3, 4, 5) for hl, ha in ((1, 1), (0, 1), (1, 0), (0, 0))]
+    descs = (vp.IntraDesc * len(cases))()
+    edges, want = [], np.zeros((len(cases) // 8 * 32 + 32, stride), dtype=np.uint16)
+    eoff = 0
+    for i, (m, l, hl, ha) in enumerate(cases):
+        size = 1 << l
+        above = rng.integers(0, max_v + 1, size=2 * size + 1).astype(np.int32)
+        left = rng.integers(0, max_v + 1, size=size).astype(np.int32)
+        r0, c0 = (i // 8) * 32, (i % 8) * 32
+        descs[i].dst_off, descs[i].edge_off = r0 * stride + c0, eoff
+        descs[i].mode, descs[i].log2, descs[i].have_left, descs[i].have_above = m, l, hl, ha
+        edges += [above, left]
+        eoff += 3 * size + 1
+        blk = np.zeros((size, size), dtype=np.uint16)
+        olib.orc_intra_predict(m, l, ptr(above), ptr(left), hl, ha, bd, ptr(blk), size)
+        want[r0:r0 + size, c0:c0 + size] = blk
+    de = gpu.upload(np.concatenate(edges))
+    dd = gpu.upload(np.zeros(want.size, dtype=dt))
+    assert lib.vp9r_intra_predict_batch(gpu.handle, len(cases), ctypes.cast(descs, ctypes.c_void_p), de.ptr, dd.ptr,
+                                        stride, bd) == 0
+    got = dd.read(dt, want.size).reshape(want.shape).astype(np.uint16)
+    assert np.array_equal(got, want)
 
 
 def test_context_outlives_its_frames_and_buffers():
