@@ -294,6 +294,18 @@ typedef struct vp9r_ref_plane {
  * n_refs planes. */
 int vp9r_mc_batch_planes(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, const vp9r_ref_plane *planes,
                          uint32_t n_refs, void *out, uint32_t out_stride, uint32_t bit_depth);
+/* vp9r_mc_batch_planes without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The descriptors and planes are
+ * staged in a fixed device array of the library (about 2 MB per GPU, held
+ * until the process ends): at most 65536 descriptors and 64 planes
+ * (VP9R_ERR_INVALID_INPUT beyond).  The calls on one GPU share that array:
+ * calls that overlap on one GPU race.  The same output as
+ * vp9r_mc_batch_planes; waits for completion, and is not timed
+ * (vp9r_gpu_seam_ms). */
+int vp9r_mc_batch_planes_noalloc(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc,
+                                 const vp9r_ref_plane *planes, uint32_t n_refs, void *out, uint32_t out_stride,
+                                 uint32_t bit_depth);
 /* The same with every reference plane of one geometry: `refs` is a host
  * array of n_refs device pointers, all of row stride ref_stride and
  * bottom-right sample (last_x, last_y). */

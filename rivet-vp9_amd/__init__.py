@@ -191,6 +191,7 @@ def library():
     lib.vp9r_mc_batch.argtypes = [P, u32, V, V, u32, u32, i32, i32, V, u32, u32]
     lib.vp9r_lf_edges_batch.argtypes = [P, u32, V, V, V, V, V, V, V, V, u32, u32, u32]
     lib.vp9r_mc_batch_planes.argtypes = [P, u32, V, V, u32, V, u32, u32]
+    lib.vp9r_mc_batch_planes_noalloc.argtypes = [P, u32, V, V, u32, V, u32, u32]
     lib.vp9r_avg_batch.argtypes = [P, u32, V, V, V, V, u32, u32]
     lib.vp9r_add_const_batch.argtypes = [P, u32, V, V, u32]
     lib.vp9r_intra_predict_batch.argtypes = [P, u32, V, V, V, u32, u32]

@@ -246,6 +246,8 @@ void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const
                       int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs);
 void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,
                           const vp9r_ref_plane *planes, void *out, uint32_t out_stride, uint32_t bd);
+void launch_mc_desc_expand(hipStream_t s, const vp9r_mc_desc *desc, uint32_t count, const vp9r_ref_plane *planes,
+                           void *out, uint32_t out_stride, uint32_t bd);
 // one pass of lf::filter_edges: edges [first, n_edges) of the region at (x0, y0)
 struct LfPassDesc {
   uint16_t x0, y0;

@@ -494,13 +494,25 @@ DEV void mc_desc_half(const vp9r_mc_desc &d, const vp9r_ref_plane &r0, const vp9
   }
 }
 
-template <bool HBD>
+// EXPAND: no tile list; workgroup 8 i + k is tile k of descriptor i (16
+// columns by 32 rows: column (k & 3), row half (k >> 2)), and tiles outside
+// the block exit at once (the whole workgroup, before any barrier).
+template <bool HBD, bool EXPAND>
 __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const DescTile *tiles,
                                                 const vp9r_ref_plane *planes, void *out, uint32_t out_stride,
                                                 int bd) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ McLds sh;
-  const DescTile t = tiles[blockIdx.x];
+  DescTile t;
+  if (EXPAND) {
+    t.desc = blockIdx.x >> 3;
+    t.tx = (uint16_t)(16 * (blockIdx.x & 3));
+    t.ty = (uint16_t)(32 * ((blockIdx.x >> 2) & 1));
+    const vp9r_mc_desc dd = descs[t.desc];
+    if (t.tx >= dd.w || t.ty >= dd.h) return;
+  } else {
+    t = tiles[blockIdx.x];
+  }
   const vp9r_mc_desc d = descs[t.desc];
   // each reference its own plane (inter.rs:21-27 RefPlane)
   const vp9r_ref_plane r0 = planes[d.ref_idx], r1 = planes[d.ref2_idx >= 0 ? d.ref2_idx : d.ref_idx];
@@ -572,10 +584,24 @@ void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *t
   upload_filters();
   if (!n_tiles) return;
   if (bd > 8)
-    hipLaunchKernelGGL(k_mc_desc<true>, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles, planes, out,
+    hipLaunchKernelGGL((k_mc_desc<true, false>), dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles,
+                       planes, out, out_stride, (int)bd);
+  else
+    hipLaunchKernelGGL((k_mc_desc<false, false>), dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles,
+                       planes, out, out_stride, (int)bd);
+}
+
+// The same without a tile list: 8 workgroups per descriptor (the tiles of a
+// 64 x 64 block), those outside a smaller block exit.
+void launch_mc_desc_expand(hipStream_t s, const vp9r_mc_desc *desc, uint32_t count, const vp9r_ref_plane *planes,
+                           void *out, uint32_t out_stride, uint32_t bd) {
+  upload_filters();
+  if (!count) return;
+  if (bd > 8)
+    hipLaunchKernelGGL((k_mc_desc<true, true>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
                        out_stride, (int)bd);
   else
-    hipLaunchKernelGGL(k_mc_desc<false>, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles, planes, out,
+    hipLaunchKernelGGL((k_mc_desc<false, true>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
                        out_stride, (int)bd);
 }
 

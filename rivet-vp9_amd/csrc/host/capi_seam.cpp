@@ -2,6 +2,7 @@
 // launches (for the kernel microbenchmarks and parity tests).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "recon.h"
@@ -19,6 +20,43 @@ static int dev_err(hipError_t e, const char *what) {
   } while (0)
 
 static bool bd_ok(uint32_t bd) { return bd == 8 || bd == 10 || bd == 12; }
+
+// ---- the allocation-free motion compensation entry point
+//
+// Its descriptors and reference planes are staged in `scratch`, a
+// file-scope device array of fixed size: each GPU holds its own copy until
+// the process ends, found once per device by hipGetSymbolAddress (the name
+// is a host-side stand-in, never a device address).  Calls that overlap on
+// one GPU share it and race.
+constexpr uint32_t kNoallocMaxDescs = 65536;
+constexpr uint32_t kNoallocMaxRefs = 64;
+constexpr size_t kNoallocDescBytes = sizeof(vp9r_mc_desc) * kNoallocMaxDescs;
+constexpr size_t kScratchBytes = kNoallocDescBytes + sizeof(vp9r_ref_plane) * kNoallocMaxRefs;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static uint8_t *scratch_ptr[kMaxDevices];  // per device, once found
+
+// The device address of this device's scratch (the device current), or
+// null with the error set.
+static uint8_t *scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    scratch_ptr[device] = (uint8_t *)p;
+  }
+  return scratch_ptr[device];
+}
 
 
 extern "C" {
@@ -150,6 +188,50 @@ int vp9r_mc_batch_planes(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, 
   TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "mc sync");
   TRY(seam_end(g), "hipEventElapsedTime");
+  return VP9R_OK;
+}
+
+int vp9r_mc_batch_planes_noalloc(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc,
+                                 const vp9r_ref_plane *planes, uint32_t n_refs, void *out, uint32_t out_stride,
+                                 uint32_t bit_depth) {
+  if (count > kNoallocMaxDescs || n_refs > kNoallocMaxRefs) {
+    set_error("vp9r_mc_batch_planes_noalloc: at most 65536 descriptors and 64 reference planes");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  if (!bd_ok(bit_depth) || n_refs == 0) {
+    set_error("invalid motion compensation batch");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  for (uint32_t k = 0; k < n_refs; k++)
+    if (!planes[k].data || planes[k].last_x < 0 || planes[k].last_y < 0 ||
+        (uint32_t)planes[k].last_x >= planes[k].stride) {
+      set_error("invalid reference plane");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  for (uint32_t i = 0; i < count; i++) {
+    const vp9r_mc_desc &d = desc[i];
+    bool ok = d.w >= 4 && d.w <= 64 && d.h >= 4 && d.h <= 64 && d.xs >= 1 && d.xs <= 32 && d.ys >= 1 &&
+              d.ys <= 32 && d.filter < 4 && d.ref_idx >= 0 && (uint32_t)d.ref_idx < n_refs &&
+              d.ref2_idx < (int)n_refs;
+    if (!ok) {
+      set_error("motion compensation descriptor out of range");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  }
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  uint8_t *base = scratch_on(g->device);
+  if (!base) return VP9R_ERR_DEVICE;
+  // every byte the kernels read is written here first (nothing needs to
+  // start at zero)
+  if (count)
+    TRY(hipMemcpyAsync(base, desc, sizeof(vp9r_mc_desc) * count, hipMemcpyHostToDevice, g->stream), "upload");
+  TRY(hipMemcpyAsync(base + kNoallocDescBytes, planes, sizeof(vp9r_ref_plane) * n_refs, hipMemcpyHostToDevice,
+                     g->stream),
+      "upload");
+  vp9g::launch_mc_desc_expand(g->stream, (const vp9r_mc_desc *)base, count,
+                              (const vp9r_ref_plane *)(base + kNoallocDescBytes), out, out_stride, bit_depth);
+  TRY(hipGetLastError(), "mc launch");
+  TRY(hipStreamSynchronize(g->stream), "mc sync");
   return VP9R_OK;
 }
 
