@@ -118,15 +118,26 @@ class Schedule:
         self.pool.append(f)
         return f
 
-    def step(self, i):
-        """Queues frame i; returns the frame store it shows (or None)."""
+    def step(self, i, graphs=None):
+        """Queues frame i; returns the frame store it shows (or None).  With
+        `graphs` (a dict): through frame i's HIP graph, captured on first
+        use and re-targeted when its frame stores differ."""
         fr = self.frames[i]
         h = fr["hdr"]
         if h.show_existing_frame:
             return self.slots[h.frame_to_show_map_idx]
         cur = self.store(h)
         refs = [None] * 3 if h.frame_is_intra else [self.slots[h.ref_frame_idx[k]] for k in range(3)]
-        self.gpu.run(fr["prep"], refs, cur)
+        if graphs is None:
+            self.gpu.run(fr["prep"], refs, cur)
+        else:
+            key = (tuple(id(r) for r in refs), id(cur))
+            if i not in graphs:
+                graphs[i] = [self.gpu.graph(fr["prep"], refs, cur), key]
+            elif graphs[i][1] != key:
+                graphs[i][0].update(refs, cur)
+                graphs[i][1] = key
+            graphs[i][0].run()
         for k in range(8):
             if (h.refresh_frame_flags >> k) & 1:
                 self.slots[k] = cur
@@ -381,16 +392,40 @@ def uhd_bench(vp, gpu, check, passes=3):
         sched.step(k % n)
     gpu.synchronize()
     fps = passes * n / (time.perf_counter() - t0)
+    # the same passes with each frame's launches replayed as a HIP graph
+    # (captured in an untimed pass; graphs re-targeted when the stores of a
+    # frame differ), and the MD5s of a graph pass
+    graphs = {}
+    for i in range(n):
+        sched.step(i, graphs)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    for k in range(passes * n):
+        sched.step(k % n, graphs)
+    gpu.synchronize()
+    gfps = passes * n / (time.perf_counter() - t0)
+    gmd5 = []
+    for i in range(n):
+        shown = sched.step(i, graphs)
+        if shown is not None:
+            gmd5.append(hashlib.md5(shown.packed()).hexdigest())
+    nodes = sum(g[0].nodes() for g in graphs.values()) / max(len(graphs), 1)
+    del graphs
     out = {"workload": "3840x2160 10-bit 4:2:0 profile 2, 48 frames, seed 4, every 12th frame at 1920x1080 "
                        "(2:1 / 1:2 scaled references), 4 tile columns, base_q_idx 80 (stream generator)",
            "frames_per_s": round(fps, 2), "ms_per_frame": round(1000.0 / fps, 4), "timed_frames": passes * n,
+           "launch": "direct (vp9r_gpu_run)",
+           "graph": {"frames_per_s": round(gfps, 2), "ms_per_frame": round(1000.0 / gfps, 4),
+                     "kernel_nodes_per_frame": round(nodes, 1), "md5_equal_direct": gmd5 == md5,
+                     "launch": "one HIP graph per frame (vp9r_gpu_graph_run), captured once"},
            "bit_exact": None}
     if check:
         cpu, md5_one, md5_all = cpu_baselines(frames_bytes, 0.0, "the 4K 10-bit stream")
-        out["bit_exact"] = md5_one == md5 and md5_all == md5
+        out["bit_exact"] = md5_one == md5 and md5_all == md5 and gmd5 == md5
         out["cpu_baseline"] = cpu
         out["vs_cpu_all_core"] = round(fps / cpu["value"], 2)
-    _log(f"bench: 4K 10-bit {fps:.1f} frames/s, bit-exact {out['bit_exact']}")
+    _log(f"bench: 4K 10-bit {fps:.1f} frames/s direct, {gfps:.1f} frames/s through graphs, "
+         f"bit-exact {out['bit_exact']}")
     del sched, frames
     return out
 
@@ -431,6 +466,14 @@ def micro_bench(vp, gpu, check, reps=5):
         gbs = nbytes / (ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(gbs / 8000.0, 4), "algorithmic_bytes": nbytes}
+
+    # the launch-shape autotuner first: its choices apply to the calls below
+    t0 = time.perf_counter()
+    choice, times = gpu.autotune()
+    out["autotune"] = {"choices": choice, "seconds": round(time.perf_counter() - t0, 2),
+                       "itx_us_by_class_size_waves_1_2_4": [round(t, 1) for t in times[:24]],
+                       "mc_us_by_class_rows_32_16": [round(t, 1) for t in times[24:26]]}
+    _log(f"bench: autotune {choice}")
 
     t0 = time.perf_counter()
     s = mb.ItxSet(2_000_000, 3)
@@ -859,6 +902,21 @@ def main():
     fps = args.steps / elapsed
     _log(f"bench: {args.steps} frames in {elapsed:.3f}s = {fps:.1f} frames/s")
 
+    # the same frames through one HIP graph per frame (captured in an
+    # untimed pass), reported beside the direct launches
+    graphs = {}
+    for i in range(n):
+        sched.step(i, graphs)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    for j in range(args.steps):
+        sched.step(j % n, graphs)
+    gpu.synchronize()
+    graph_fps = args.steps / (time.perf_counter() - t0)
+    pos = args.steps  # the next frame of the chain
+    del graphs
+    _log(f"bench: through graphs {graph_fps:.1f} frames/s")
+
     # the stage breakdown: the same number of frames again, with HIP events
     # between the stages on the kernel stream (each event record holds the
     # stream for a few microseconds, so the headline region above has none)
@@ -1014,6 +1072,7 @@ def main():
         "bit_exact": bit_exact,
         "stages": stages,
         "stage_pass_fps": round(staged_fps, 2),
+        "graph_fps": round(graph_fps, 2),
         "parse_fps_one_core": round(n / t_parse, 2),
     }
     out.update(extras)

@@ -207,6 +207,23 @@ int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *p,
  * Batches the single-frame vp9r_gpu_loop_filter. */
 int vp9r_gpu_filter_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *p,
                           vp9r_gpu_frame *const *cur);
+/* A prepared frame's launch sequence captured as a HIP graph, replayed by
+ * one hipGraphLaunch (no counterpart in the reference, whose decoder calls
+ * its kernels directly).  The graph is made for the frame stores given (its
+ * batch tables and kernel parameters name them); vp9r_gpu_graph_update
+ * re-targets it to other stores of the same geometry by capturing again and
+ * updating the instantiated graph's kernel parameters in place.  Runs on the
+ * context's stream like vp9r_gpu_run, with the same results; the prepared
+ * record and the stores must outlive the graph's last run.  Stage timing
+ * must be off. */
+typedef struct vp9r_graph vp9r_graph;
+int vp9r_gpu_graph_new(vp9r_gpu *g, const vp9r_prepared *p, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur,
+                       vp9r_graph **out);
+int vp9r_gpu_graph_update(vp9r_graph *gr, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur);
+int vp9r_gpu_graph_run(vp9r_graph *gr);
+/* The kernel launches the graph holds. */
+uint32_t vp9r_gpu_graph_nodes(const vp9r_graph *gr);
+void vp9r_gpu_graph_free(vp9r_graph *gr);
 /* Work of a prepared record: [0] inter prediction tiles (<= 16x16), [1..4]
  * inter transform blocks of 4x4 .. 32x32, [5] intra blocks, [6] intra
  * dependency levels (one launch each), [7] loop filter wavefront launches,
@@ -363,6 +380,24 @@ int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const
                         const uint16_t *y0, const uint8_t *vertical, const uint8_t *first,
                         const uint8_t *n_edges, const uint8_t *n_runs, void *buf, uint32_t stride,
                         uint32_t rows, uint32_t bit_depth);
+
+/* Autotuner of the batched forms' launch shapes (no counterpart in the
+ * reference: src/dsp/mod.rs:32-96 chooses its kernels by CPU feature).  It
+ * times, on synthetic batches, 1, 2 and 4 waves per workgroup for the
+ * inverse transforms of each size (4x4 .. 32x32) and bit depth class
+ * (8-bit, 10/12-bit), and tiles of 16 and 32 rows for motion compensation
+ * per bit depth class, and keeps the fastest of each for later calls on
+ * this context (the arithmetic, and so every output, is the same for every
+ * shape).  times_us (may be NULL) gets the 26 candidate times: transforms
+ * [class][size][waves 1, 2, 4], then motion compensation [class][rows 32,
+ * 16].  Takes a few hundred milliseconds; allocates its own inputs. */
+int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[26]);
+/* The shapes in use: [0..3] waves per transform workgroup at 8 bits for
+ * 4x4 .. 32x32, [4..7] the same at 10/12 bits, [8..9] motion compensation
+ * tile rows at 8 and at 10/12 bits, [10] 1 if vp9r_gpu_autotune has run.
+ * vp9r_gpu_set_tuning sets [0..9] (waves 1, 2 or 4; rows 16 or 32). */
+int vp9r_gpu_tuning(const vp9r_gpu *g, uint32_t out[11]);
+int vp9r_gpu_set_tuning(vp9r_gpu *g, const uint32_t in[10]);
 
 /* Device memory for callers of the batched forms (no counterpart in the
  * reference: a plain-pointer ABI needs its own allocation and copies).  The

@@ -200,6 +200,15 @@ def library():
     lib.vp9r_gpu_prepare.argtypes = [P, ctypes.POINTER(Record), PP]
     lib.vp9r_prepared_free.argtypes = [P]
     lib.vp9r_gpu_run.argtypes = [P, P, ctypes.c_void_p * 3, P]
+    lib.vp9r_gpu_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+    lib.vp9r_gpu_tuning.argtypes = [P, ctypes.POINTER(u32)]
+    lib.vp9r_gpu_set_tuning.argtypes = [P, ctypes.POINTER(u32)]
+    lib.vp9r_gpu_graph_new.argtypes = [P, P, ctypes.c_void_p * 3, P, PP]
+    lib.vp9r_gpu_graph_update.argtypes = [P, ctypes.c_void_p * 3, P]
+    lib.vp9r_gpu_graph_run.argtypes = [P]
+    lib.vp9r_gpu_graph_nodes.restype = u32
+    lib.vp9r_gpu_graph_nodes.argtypes = [P]
+    lib.vp9r_gpu_graph_free.argtypes = [P]
     lib.vp9r_gpu_run_batch.argtypes = [P, u32, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V)]
     lib.vp9r_gpu_filter_batch.argtypes = [P, u32, ctypes.POINTER(V), ctypes.POINTER(V)]
     lib.vp9r_gpu_prepared_stats.argtypes = [P, ctypes.POINTER(u64)]
@@ -313,6 +322,31 @@ class Gpu:
     def frame(self, width, height, bit_depth, ss_x, ss_y):
         return GpuFrame(self, width, height, bit_depth, ss_x, ss_y)
 
+    @staticmethod
+    def _refs(refs):
+        return (ctypes.c_void_p * 3)(*[(r.handle if r is not None else None) for r in refs])
+
+    def autotune(self):
+        """Runs the launch-shape autotuner; returns (choices, candidate times in us)."""
+        t = (ctypes.c_float * 26)()
+        _check(library().vp9r_gpu_autotune(self._g, t))
+        return self.tuning(), list(t)
+
+    def tuning(self):
+        out = (u32 * 11)()
+        _check(library().vp9r_gpu_tuning(self._g, out))
+        v = list(out)
+        return {"itx_waves_8bit": v[0:4], "itx_waves_hbd": v[4:8], "mc_tile_rows_8bit": v[8],
+                "mc_tile_rows_hbd": v[9], "tuned": bool(v[10])}
+
+    def set_tuning(self, itx_waves_8bit, itx_waves_hbd, mc_rows_8bit, mc_rows_hbd):
+        arr = (u32 * 10)(*(list(itx_waves_8bit) + list(itx_waves_hbd) + [mc_rows_8bit, mc_rows_hbd]))
+        _check(library().vp9r_gpu_set_tuning(self._g, arr))
+
+    def graph(self, prepared, refs, cur):
+        """The launches of run(prepared, refs, cur) captured as a Graph."""
+        return Graph(self, prepared, refs, cur)
+
     def reconstruct(self, rec, refs, cur):
         arr = (ctypes.c_void_p * 3)(*[(r.handle if r is not None else None) for r in refs])
         _check(library().vp9r_gpu_reconstruct(self._g, ctypes.byref(rec), arr, cur.handle))
@@ -411,6 +445,34 @@ class Prepared:
         library().vp9r_gpu_prepared_schedule(self._p, sched)
         d["wave_launches"], d["lf_tail_launches"] = sched[0], sched[1]
         return d
+
+
+class Graph:
+    """A prepared frame's launches captured as a HIP graph
+    (vp9r_gpu_graph_*): run() replays them for the frame stores it was made
+    for; update() re-targets it to other stores."""
+
+    def __init__(self, gpu, prepared, refs, cur):
+        self.gpu, self.prepared = gpu, prepared
+        self._keep = (list(refs), cur)
+        self._h = ctypes.c_void_p()
+        _check(library().vp9r_gpu_graph_new(gpu.handle, prepared.handle, Gpu._refs(refs), cur.handle,
+                                            ctypes.byref(self._h)))
+
+    def update(self, refs, cur):
+        _check(library().vp9r_gpu_graph_update(self._h, Gpu._refs(refs), cur.handle))
+        self._keep = (list(refs), cur)
+
+    def run(self):
+        _check(library().vp9r_gpu_graph_run(self._h))
+
+    def nodes(self):
+        return library().vp9r_gpu_graph_nodes(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            library().vp9r_gpu_graph_free(self._h)
+            self._h = None
 
 
 class DevBuffer:

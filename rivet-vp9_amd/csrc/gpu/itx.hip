@@ -179,8 +179,9 @@ __global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range
 // columns go through the LDS tile, the column residuals go back into the
 // tile, and lane k adds row k of them to the destination (one dword access
 // per 4 bytes of the row).
-template <int n, bool HBD>
-__global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const uint32_t *coef_off,
+// W waves per workgroup (the autotuner's choice per size and bit depth class).
+template <int n, bool HBD, int W>
+__global__ __launch_bounds__(64 * W) void k_itx_dense(const int32_t *coefs, const uint32_t *coef_off,
                                                   const uint8_t *tx_type, const uint16_t *eob,
                                                   const uint64_t *dst_off, void *dst_base,
                                                   uint32_t stride, int bd, int lossless,
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(64) void k_itx_dense(const int32_t *coefs, const ui
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  constexpr int n0 = 1 << n, BPW = 64 / n0, S = n0 + 1;
+  constexpr int n0 = 1 << n, BPW = 64 * W / n0, S = n0 + 1;
   __shared__ T tiles[BPW][n0 * S];
   const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
   const uint32_t job = blockIdx.x * BPW + team;
@@ -250,31 +251,44 @@ void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4],
     hipLaunchKernelGGL(k_itx_inter<false>, dim3(total), dim3(64), 0, s, fa, r[3], r[2], r[1], r[0], nf, b4, b8, b16);
 }
 
-template <int n>
-static void itx_dense_n(hipStream_t s, const int32_t *coefs, const uint32_t *coef_off,
-                        const uint8_t *tx_type, const uint16_t *eob, const uint64_t *dst_off,
-                        void *dst, uint32_t stride, uint32_t bd, int lossless, const uint32_t *jobs,
-                        uint32_t n_jobs) {
-  constexpr int BPW = 64 >> n;
+template <int n, int W>
+static void itx_dense_nw(hipStream_t s, const int32_t *coefs, const uint32_t *coef_off, const uint8_t *tx_type,
+                         const uint16_t *eob, const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bd,
+                         int lossless, const uint32_t *jobs, uint32_t n_jobs) {
+  constexpr int BPW = (64 * W) >> n;
   dim3 grid((n_jobs + BPW - 1) / BPW);
   if (bd > 8)
-    hipLaunchKernelGGL((k_itx_dense<n, true>), grid, dim3(64), 0, s, coefs, coef_off, tx_type, eob,
+    hipLaunchKernelGGL((k_itx_dense<n, true, W>), grid, dim3(64 * W), 0, s, coefs, coef_off, tx_type, eob,
                        dst_off, dst, stride, (int)bd, lossless, jobs, n_jobs);
   else
-    hipLaunchKernelGGL((k_itx_dense<n, false>), grid, dim3(64), 0, s, coefs, coef_off, tx_type, eob,
+    hipLaunchKernelGGL((k_itx_dense<n, false, W>), grid, dim3(64 * W), 0, s, coefs, coef_off, tx_type, eob,
                        dst_off, dst, stride, (int)bd, lossless, jobs, n_jobs);
+}
+
+// waves: 1, 2 or 4 per workgroup (4 only where the workgroup's tiles fit
+// 64 KB of LDS: 32 x 32 at 10/12 bits takes 2).
+template <int n>
+static void itx_dense_n(hipStream_t s, const int32_t *coefs, const uint32_t *coef_off, const uint8_t *tx_type,
+                        const uint16_t *eob, const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bd,
+                        int lossless, const uint32_t *jobs, uint32_t n_jobs, int waves) {
+  if (waves >= 4 && !(n == 5 && bd > 8))
+    itx_dense_nw<n, 4>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bd, lossless, jobs, n_jobs);
+  else if (waves >= 2)
+    itx_dense_nw<n, 2>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bd, lossless, jobs, n_jobs);
+  else
+    itx_dense_nw<n, 1>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bd, lossless, jobs, n_jobs);
 }
 
 void launch_itx_dense(hipStream_t s, uint32_t, const int32_t *coefs, const uint32_t *coef_off,
                       const uint8_t *, const uint8_t *tx_type, const uint16_t *eob,
                       const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
-                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs) {
+                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs, int waves) {
   if (!n_jobs) return;
   switch (tx_size) {
-    case 0: itx_dense_n<2>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
-    case 1: itx_dense_n<3>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
-    case 2: itx_dense_n<4>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
-    default: itx_dense_n<5>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs); break;
+    case 0: itx_dense_n<2>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
+    case 1: itx_dense_n<3>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
+    case 2: itx_dense_n<4>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
+    default: itx_dense_n<5>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
   }
 }
 

@@ -243,9 +243,13 @@ void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n
 void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,
                       const uint8_t *tx, const uint8_t *tx_type, const uint16_t *eob,
                       const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
-                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs);
+                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs, int waves = 1);
+// Copies the interpolation taps to the current device if not yet there (the
+// MC launches do it themselves; a graph capture calls it first).
+void mc_prepare();
+// tiles of 16 columns by tile_rows (16 or 32) rows
 void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,
-                          const vp9r_ref_plane *planes, void *out, uint32_t out_stride, uint32_t bd);
+                          const vp9r_ref_plane *planes, void *out, uint32_t out_stride, uint32_t bd, int tile_rows);
 void launch_mc_desc_expand(hipStream_t s, const vp9r_mc_desc *desc, uint32_t count, const vp9r_ref_plane *planes,
                            void *out, uint32_t out_stride, uint32_t bd);
 // one pass of lf::filter_edges: edges [first, n_edges) of the region at (x0, y0)

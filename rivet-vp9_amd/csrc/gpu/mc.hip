@@ -496,12 +496,29 @@ DEV void mc_desc_half(const vp9r_mc_desc &d, const vp9r_ref_plane &r0, const vp9
 
 // EXPAND: no tile list; workgroup 8 i + k is tile k of descriptor i (16
 // columns by 32 rows: column (k & 3), row half (k >> 2)), and tiles outside
-// the block exit at once (the whole workgroup, before any barrier).
-template <bool HBD, bool EXPAND>
+// the block exit at once (the whole workgroup, before any barrier).  TH:
+// the tile's rows, 16 or 32 (the autotuner's choice per bit depth class).
+template <int TH>
+DEV Mc8Shared<TH> &mc8_lds(McLds &sh) {
+  if constexpr (TH == 32)
+    return sh.f32;
+  else
+    return sh.f16;
+}
+template <int TH>
+DEV Mc16Shared<TH> &mc16_lds(McLds &sh) {
+  if constexpr (TH == 32)
+    return sh.h32;
+  else
+    return sh.h16;
+}
+
+template <bool HBD, bool EXPAND, int TH>
 __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const DescTile *tiles,
                                                 const vp9r_ref_plane *planes, void *out, uint32_t out_stride,
                                                 int bd) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  constexpr int NH = TH / 16;
   __shared__ McLds sh;
   DescTile t;
   if (EXPAND) {
@@ -517,22 +534,22 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
   // each reference its own plane (inter.rs:21-27 RefPlane)
   const vp9r_ref_plane r0 = planes[d.ref_idx], r1 = planes[d.ref2_idx >= 0 ? d.ref2_idx : d.ref_idx];
   const int max = (1 << bd) - 1;
-  const int th = d.h - t.ty < 2 * TILE ? d.h - t.ty : 2 * TILE, tw = d.w - t.tx < TILE ? d.w - t.tx : TILE;
+  const int th = d.h - t.ty < TH ? d.h - t.ty : TH, tw = d.w - t.tx < TILE ? d.w - t.tx : TILE;
   const bool compound = d.ref2_idx >= 0;
   P *dst = (P *)out + d.out_off + (size_t)t.ty * out_stride + t.tx;
   if (!HBD && d.xs == 16 && d.ys == 16 && ((tw | th | out_stride | (uint32_t)(uintptr_t)dst) & 3) == 0) {
-    uint32_t v[2], v2[2];
-    mc_tile8<32>((const uint8_t *)r0.data, r0.stride, r0.last_x, r0.last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
-                 th, d.filter, sh.f32, v);
+    uint32_t v[NH], v2[NH];
+    mc_tile8<TH>((const uint8_t *)r0.data, r0.stride, r0.last_x, r0.last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
+                 th, d.filter, mc8_lds<TH>(sh), v);
     if (compound) {
-      mc_tile8<32>((const uint8_t *)r1.data, r1.stride, r1.last_x, r1.last_y, d.x2 + 16 * t.tx, d.y2 + 16 * t.ty,
-                   tw, th, d.filter, sh.f32, v2);
-      v[0] = avg4(v[0], v2[0]);
-      v[1] = avg4(v[1], v2[1]);
+      mc_tile8<TH>((const uint8_t *)r1.data, r1.stride, r1.last_x, r1.last_y, d.x2 + 16 * t.tx, d.y2 + 16 * t.ty,
+                   tw, th, d.filter, mc8_lds<TH>(sh), v2);
+#pragma unroll
+      for (int h = 0; h < NH; h++) v[h] = avg4(v[h], v2[h]);
     }
     const int q = threadIdx.x & 3;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < NH; h++) {
       const int r = 16 * h + (threadIdx.x >> 2);
       if (r < th && 4 * q < tw) gst((uint32_t *)(dst + (size_t)r * out_stride + 4 * q), v[h]);
     }
@@ -540,21 +557,21 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
   }
   if (HBD && d.xs == 16 && d.ys == 16 && ((tw | th) & 3) == 0 && (out_stride & 1) == 0 &&
       ((uintptr_t)dst & 3) == 0) {
-    uint32_t v[2][2], v2[2][2];
-    mc_tile16<32>((const uint16_t *)r0.data, r0.stride, r0.last_x, r0.last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
-                  th, d.filter, max, sh.h32, v);
+    uint32_t v[NH][2], v2[NH][2];
+    mc_tile16<TH>((const uint16_t *)r0.data, r0.stride, r0.last_x, r0.last_y, d.x + 16 * t.tx, d.y + 16 * t.ty, tw,
+                  th, d.filter, max, mc16_lds<TH>(sh), v);
     if (compound) {
-      mc_tile16<32>((const uint16_t *)r1.data, r1.stride, r1.last_x, r1.last_y, d.x2 + 16 * t.tx, d.y2 + 16 * t.ty,
-                    tw, th, d.filter, max, sh.h32, v2);
+      mc_tile16<TH>((const uint16_t *)r1.data, r1.stride, r1.last_x, r1.last_y, d.x2 + 16 * t.tx, d.y2 + 16 * t.ty,
+                    tw, th, d.filter, max, mc16_lds<TH>(sh), v2);
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
+      for (int h = 0; h < NH; h++) {
         v[h][0] = avg2(v[h][0], v2[h][0]);
         v[h][1] = avg2(v[h][1], v2[h][1]);
       }
     }
     const int q = threadIdx.x & 3;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < NH; h++) {
       const int r = 16 * h + (threadIdx.x >> 2);
       if (r < th && 4 * q < tw) {
         uint32_t *o = (uint32_t *)(dst + (size_t)r * out_stride + 4 * q);
@@ -570,6 +587,8 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
                     th - hh < TILE ? th - hh : TILE, max, sh.g);
 }
 
+void mc_prepare() { upload_filters(); }
+
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
   upload_filters();
   if (!wgs) return;
@@ -580,15 +599,24 @@ void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint
 }
 
 void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,
-                          const vp9r_ref_plane *planes, void *out, uint32_t out_stride, uint32_t bd) {
+                          const vp9r_ref_plane *planes, void *out, uint32_t out_stride, uint32_t bd, int tile_rows) {
   upload_filters();
   if (!n_tiles) return;
-  if (bd > 8)
-    hipLaunchKernelGGL((k_mc_desc<true, false>), dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles,
-                       planes, out, out_stride, (int)bd);
-  else
-    hipLaunchKernelGGL((k_mc_desc<false, false>), dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles,
-                       planes, out, out_stride, (int)bd);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles, planes, out, out_stride,
+                       (int)bd);
+  };
+  if (bd > 8) {
+    if (tile_rows == 16)
+      go(k_mc_desc<true, false, 16>);
+    else
+      go(k_mc_desc<true, false, 32>);
+  } else {
+    if (tile_rows == 16)
+      go(k_mc_desc<false, false, 16>);
+    else
+      go(k_mc_desc<false, false, 32>);
+  }
 }
 
 // The same without a tile list: 8 workgroups per descriptor (the tiles of a
@@ -598,10 +626,10 @@ void launch_mc_desc_expand(hipStream_t s, const vp9r_mc_desc *desc, uint32_t cou
   upload_filters();
   if (!count) return;
   if (bd > 8)
-    hipLaunchKernelGGL((k_mc_desc<true, true>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
+    hipLaunchKernelGGL((k_mc_desc<true, true, 32>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
                        out_stride, (int)bd);
   else
-    hipLaunchKernelGGL((k_mc_desc<false, true>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
+    hipLaunchKernelGGL((k_mc_desc<false, true, 32>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
                        out_stride, (int)bd);
 }
 

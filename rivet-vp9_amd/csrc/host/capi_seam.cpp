@@ -133,7 +133,8 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
   TRY(seam_begin(g), "hipEventRecord");
   for (int k = 3; k >= 0; k--)
     vp9g::launch_itx_dense(g->stream, count, coefs, coef_off, nullptr, tx_type, eob, dst_off, dst, stride,
-                           bit_depth, lossless, k, dj + start[k], (uint32_t)jobs[k].size());
+                           bit_depth, lossless, k, dj + start[k], (uint32_t)jobs[k].size(),
+                           g->tune_itx_waves[bit_depth > 8][k]);
   TRY(hipGetLastError(), "itx launch");
   TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "itx sync");
@@ -157,6 +158,7 @@ int vp9r_mc_batch_planes(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, 
     uint32_t desc;
     uint16_t tx, ty;
   };
+  const int rows = g->tune_mc_rows[bit_depth > 8] == 16 ? 16 : 32;
   std::vector<DescTile> tiles;
   for (uint32_t i = 0; i < count; i++) {
     const vp9r_mc_desc &d = desc[i];
@@ -167,9 +169,10 @@ int vp9r_mc_batch_planes(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, 
       set_error("motion compensation descriptor out of range");
       return VP9R_ERR_INVALID_INPUT;
     }
-    // tiles of 16 columns by up to 32 rows (the kernel's 8-bit unscaled
-    // path filters the 32 rows in one pass; other paths take two halves)
-    for (int ty = 0; ty < d.h; ty += 32)
+    // tiles of 16 columns by up to `rows` rows (16 or 32, the tuned
+    // choice; the unscaled paths filter all rows in one pass, the general
+    // path in halves of 16)
+    for (int ty = 0; ty < d.h; ty += rows)
       for (int tx = 0; tx < d.w; tx += 16) tiles.push_back({i, (uint16_t)tx, (uint16_t)ty});
   }
   TRY(hipSetDevice(g->device), "hipSetDevice");
@@ -183,7 +186,7 @@ int vp9r_mc_batch_planes(vp9r_gpu *g, uint32_t count, const vp9r_mc_desc *desc, 
       "upload");
   TRY(seam_begin(g), "hipEventRecord");
   vp9g::launch_mc_desc_tiles(g->stream, (const vp9r_mc_desc *)g->seam_a.p, g->seam_b.p, (uint32_t)tiles.size(),
-                             (const vp9r_ref_plane *)g->seam_c.p, out, out_stride, bit_depth);
+                             (const vp9r_ref_plane *)g->seam_c.p, out, out_stride, bit_depth, rows);
   TRY(hipGetLastError(), "mc launch");
   TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "mc sync");
@@ -432,6 +435,183 @@ int vp9r_intra_predict_host(vp9r_gpu *g, uint32_t mode, uint32_t log2, const int
   TRY(hipStreamSynchronize(g->stream), "intra sync");
   for (uint32_t i = 0; i < size; i++)
     for (uint32_t j = 0; j < size; j++) dst[i * stride + j] = out[i * size + j];
+  return VP9R_OK;
+}
+
+// ---- the autotuner of the per-kernel seam's launch shapes
+//
+// For each class it times every candidate shape on a synthetic batch of that
+// class (best of three, device time between events around the launch) and
+// keeps the fastest; the arithmetic is the same for every shape.
+//   transforms: 1, 2 or 4 waves per workgroup, per size (4x4 .. 32x32) and
+//   bit depth class, on 32768 blocks of dense random coefficients;
+//   motion compensation: tiles of 16 or 32 rows per bit depth class, on 8192
+//   blocks of 16x16 .. 64x64 from a random 1920x1080 reference.
+// There is no counterpart in the reference (src/dsp/mod.rs:32-96 picks its
+// kernels by CPU feature, and src/dsp/bench.rs times them by hand).
+
+int vp9r_gpu_set_tuning(vp9r_gpu *g, const uint32_t in[10]) {
+  for (int i = 0; i < 8; i++)
+    if (in[i] != 1 && in[i] != 2 && in[i] != 4) {
+      set_error("transform waves per workgroup: 1, 2 or 4");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  for (int i = 8; i < 10; i++)
+    if (in[i] != 16 && in[i] != 32) {
+      set_error("motion compensation tile rows: 16 or 32");
+      return VP9R_ERR_INVALID_INPUT;
+    }
+  for (int c = 0; c < 2; c++) {
+    for (int k = 0; k < 4; k++) g->tune_itx_waves[c][k] = (uint8_t)in[4 * c + k];
+    g->tune_mc_rows[c] = (uint8_t)in[8 + c];
+  }
+  return VP9R_OK;
+}
+
+int vp9r_gpu_tuning(const vp9r_gpu *g, uint32_t out[11]) {
+  for (int c = 0; c < 2; c++) {
+    for (int k = 0; k < 4; k++) out[4 * c + k] = g->tune_itx_waves[c][k];
+    out[8 + c] = g->tune_mc_rows[c];
+  }
+  out[10] = g->tuned ? 1 : 0;
+  return VP9R_OK;
+}
+
+int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[26]) {
+  TRY(hipSetDevice(g->device), "hipSetDevice");
+  uint64_t rng = 0x9e3779b97f4a7c15ull;
+  auto next = [&]() {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
+  float tmp[26];
+  float *tu = times_us ? times_us : tmp;
+  int slot = 0;
+  // transforms
+  const uint32_t nb = 32768;
+  for (int c = 0; c < 2; c++) {
+    const uint32_t bd = c ? 10 : 8, bps = c ? 2 : 1;
+    for (int k = 0; k < 4; k++) {
+      const uint32_t n0 = 4u << k, stride = 1024;
+      const uint32_t per_row = stride / n0, rows_blk = (nb + per_row - 1) / per_row;
+      std::vector<int32_t> coefs((size_t)nb * n0 * n0);
+      for (auto &v : coefs) v = (int32_t)(next() % 97) - 48;
+      std::vector<uint32_t> coff(nb), jobs(nb);
+      std::vector<uint8_t> ttype(nb);
+      std::vector<uint16_t> eob(nb);
+      std::vector<uint64_t> doff(nb);
+      for (uint32_t i = 0; i < nb; i++) {
+        coff[i] = i * n0 * n0;
+        ttype[i] = k == 3 ? 0 : (uint8_t)(next() & 3);
+        eob[i] = (uint16_t)(n0 * n0);
+        doff[i] = (uint64_t)(i / per_row) * n0 * stride + (i % per_row) * n0;
+        jobs[i] = i;
+      }
+      void *d_coef = nullptr, *d_meta = nullptr, *d_dst = nullptr;
+      const size_t meta = 4 * nb + nb + 2 * nb + 8 * nb + 4 * nb + 64;
+      TRY(hipMalloc(&d_coef, 4 * coefs.size()), "hipMalloc(tune)");
+      TRY(hipMalloc(&d_meta, meta), "hipMalloc(tune)");
+      TRY(hipMalloc(&d_dst, (size_t)rows_blk * n0 * stride * bps), "hipMalloc(tune)");
+      uint8_t *m = (uint8_t *)d_meta;
+      uint32_t *d_coff = (uint32_t *)m;
+      uint64_t *d_doff = (uint64_t *)(((uintptr_t)(m + 4 * nb) + 7) & ~(uintptr_t)7);
+      uint32_t *d_jobs = (uint32_t *)(d_doff + nb);
+      uint16_t *d_eob = (uint16_t *)(d_jobs + nb);
+      uint8_t *d_tt = (uint8_t *)(d_eob + nb);
+      TRY(hipMemcpy(d_coef, coefs.data(), 4 * coefs.size(), hipMemcpyHostToDevice), "tune upload");
+      TRY(hipMemcpy(d_coff, coff.data(), 4 * nb, hipMemcpyHostToDevice), "tune upload");
+      TRY(hipMemcpy(d_doff, doff.data(), 8 * nb, hipMemcpyHostToDevice), "tune upload");
+      TRY(hipMemcpy(d_jobs, jobs.data(), 4 * nb, hipMemcpyHostToDevice), "tune upload");
+      TRY(hipMemcpy(d_eob, eob.data(), 2 * nb, hipMemcpyHostToDevice), "tune upload");
+      TRY(hipMemcpy(d_tt, ttype.data(), nb, hipMemcpyHostToDevice), "tune upload");
+      TRY(hipMemset(d_dst, 0x40, (size_t)rows_blk * n0 * stride * bps), "tune memset");
+      float best = 1e30f;
+      int best_w = 1;
+      const int cand[3] = {1, 2, 4};
+      for (int w : cand) {
+        float t_best = 1e30f;
+        for (int rep = 0; rep < 4; rep++) {
+          TRY(seam_begin(g), "hipEventRecord");
+          vp9g::launch_itx_dense(g->stream, nb, (const int32_t *)d_coef, d_coff, nullptr, d_tt, d_eob, d_doff, d_dst,
+                                 stride, bd, 0, k, d_jobs, nb, w);
+          TRY(hipGetLastError(), "tune itx launch");
+          TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
+          TRY(hipStreamSynchronize(g->stream), "tune sync");
+          TRY(seam_end(g), "hipEventElapsedTime");
+          if (rep) t_best = std::min(t_best, g->seam_ms);  // the first is a warm-up
+        }
+        if (slot < 26) tu[slot++] = 1000.0f * t_best;
+        if (t_best < best * 0.97f) {  // a candidate must win clearly to move off the default
+          best = t_best;
+          best_w = w;
+        }
+      }
+      g->tune_itx_waves[c][k] = (uint8_t)best_w;
+      (void)hipFree(d_coef);
+      (void)hipFree(d_meta);
+      (void)hipFree(d_dst);
+    }
+  }
+  // motion compensation
+  const uint32_t rw = 1920, rh = 1080, nd = 8192, ostride = 4096;
+  for (int c = 0; c < 2; c++) {
+    const uint32_t bd = c ? 10 : 8, bps = c ? 2 : 1;
+    std::vector<uint8_t> ref((size_t)rw * rh * bps);
+    for (auto &v : ref) v = (uint8_t)next();
+    if (c)
+      for (size_t i = 1; i < ref.size(); i += 2) ref[i] &= 3;  // 10-bit samples
+    std::vector<vp9r_mc_desc> desc(nd);
+    const int sizes[4] = {16, 32, 64, 32};
+    for (uint32_t i = 0; i < nd; i++) {
+      vp9r_mc_desc &d = desc[i];
+      d = vp9r_mc_desc{};
+      d.w = (uint8_t)sizes[i & 3];
+      d.h = (uint8_t)sizes[(i >> 2) & 3];
+      d.x = (int32_t)(next() % ((rw - 80) * 16)) + 128;
+      d.y = (int32_t)(next() % ((rh - 80) * 16)) + 128;
+      d.x2 = d.x + 40;
+      d.y2 = d.y - 24;
+      d.xs = d.ys = 16;
+      d.filter = (uint8_t)(next() % 3);
+      d.ref_idx = 0;
+      d.ref2_idx = (i % 10) < 3 ? 0 : -1;
+      d.out_off = (uint64_t)(i / 64) * 64 * ostride + (i % 64) * 64;
+    }
+    void *d_ref = nullptr, *d_out = nullptr;
+    const size_t out_bytes = (size_t)(nd / 64 + 1) * 64 * ostride * bps;
+    TRY(hipMalloc(&d_ref, ref.size()), "hipMalloc(tune)");
+    TRY(hipMalloc(&d_out, out_bytes), "hipMalloc(tune)");
+    TRY(hipMemcpy(d_ref, ref.data(), ref.size(), hipMemcpyHostToDevice), "tune upload");
+    vp9r_ref_plane plane{d_ref, rw, (int32_t)rw - 1, (int32_t)rh - 1, 0};
+    float best = 1e30f;
+    int best_rows = 32;
+    const int cand[2] = {32, 16};
+    for (int r : cand) {
+      g->tune_mc_rows[c] = (uint8_t)r;
+      float t_best = 1e30f;
+      for (int rep = 0; rep < 4; rep++) {
+        int st = vp9r_mc_batch_planes(g, nd, desc.data(), &plane, 1, d_out, ostride, bd);
+        if (st) {
+          (void)hipFree(d_ref);
+          (void)hipFree(d_out);
+          return st;
+        }
+        if (rep) t_best = std::min(t_best, g->seam_ms);
+      }
+      if (slot < 26) tu[slot++] = 1000.0f * t_best;
+      if (t_best < best * 0.97f) {
+        best = t_best;
+        best_rows = r;
+      }
+    }
+    g->tune_mc_rows[c] = (uint8_t)best_rows;
+    (void)hipFree(d_ref);
+    (void)hipFree(d_out);
+  }
+  g->tuned = true;
+  g->seam_ms = -1.0f;
   return VP9R_OK;
 }
 

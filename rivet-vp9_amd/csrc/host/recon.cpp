@@ -993,8 +993,12 @@ size_t wave_schedule(const Prepared &p, bool recon, bool lf, size_t &n_int, std:
   return std::max(n_int, n_lf ? (size_t)lf_at.back() + 1 : 0);
 }
 
-int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool lf) {
+int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool lf, GraphTarget *gt) {
   hipStream_t s = g->stream;
+  if (gt && (n_all != 1 || g->timing || !g->intra_blocks)) {
+    set_error("graph capture: one frame, stage timing off, leveled intra blocks");
+    return VP9R_ERR_INVALID_INPUT;
+  }
   for (int cls = 0; cls < 2; cls++) {
     const bool hbd = cls == 1;
     std::vector<const BatchFrame *> f;
@@ -1187,25 +1191,40 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       la.cur = cur->planes;
       la.p = p.lfp;
     }
-    // one transfer into the next device slot
-    BatchSlot &slot = g->slots[g->slot_next];
-    g->slot_next = (g->slot_next + 1) % kBatchSlots;
-    if (slot.pending) {
-      HIP_TRY(hipEventSynchronize(slot.ev), "batch slot reuse");
-      slot.pending = false;
+    const uint8_t *d = nullptr;
+    BatchSlot *slotp = nullptr;
+    if (gt) {
+      // the graph's own table copy, then the capture of the launches alone
+      vp9g::mc_prepare();  // no synchronous copy may happen under the capture
+      HIP_TRY(hipGetLastError(), "interpolation taps");
+      HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+      HIP_TRY(gt->tables.ensure(st.size()), "hipMalloc(graph tables)");
+      HIP_TRY(hipMemcpy(gt->tables.p, st.data(), st.size(), hipMemcpyHostToDevice), "upload graph tables");
+      d = (const uint8_t *)gt->tables.p;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    } else {
+      // one transfer into the next device slot
+      BatchSlot &slot = g->slots[g->slot_next];
+      slotp = &slot;
+      g->slot_next = (g->slot_next + 1) % kBatchSlots;
+      if (slot.pending) {
+        HIP_TRY(hipEventSynchronize(slot.ev), "batch slot reuse");
+        slot.pending = false;
+      }
+      HIP_TRY(slot.host.ensure(st.size()), "hipHostMalloc(batch)");
+      HIP_TRY(slot.dev.ensure(st.size()), "hipMalloc(batch)");
+      memcpy(slot.host.p, st.data(), st.size());
+      // on its own stream: the copy runs while earlier batches still filter
+      HIP_TRY(hipMemcpyAsync(slot.dev.p, slot.host.p, st.size(), hipMemcpyHostToDevice, g->up_stream),
+              "upload batch");
+      HIP_TRY(hipEventRecord(g->ev_up, g->up_stream), "hipEventRecord");
+      HIP_TRY(hipStreamWaitEvent(s, g->ev_up, 0), "hipStreamWaitEvent");
+      d = (const uint8_t *)slot.dev.p;
     }
-    HIP_TRY(slot.host.ensure(st.size()), "hipHostMalloc(batch)");
-    HIP_TRY(slot.dev.ensure(st.size()), "hipMalloc(batch)");
-    memcpy(slot.host.p, st.data(), st.size());
-    // on its own stream: the copy runs while earlier batches still filter
-    HIP_TRY(hipMemcpyAsync(slot.dev.p, slot.host.p, st.size(), hipMemcpyHostToDevice, g->up_stream),
-            "upload batch");
-    HIP_TRY(hipEventRecord(g->ev_up, g->up_stream), "hipEventRecord");
-    HIP_TRY(hipStreamWaitEvent(s, g->ev_up, 0), "hipStreamWaitEvent");
-    const uint8_t *d = (const uint8_t *)slot.dev.p;
     auto R = [&](const LaunchTable &t) { return (const vp9g::Range *)(d + t.off); };
     if (recon) {
-      for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
+      if (!gt)  // a graph's launch waits for the downloads before it
+        for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
       vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
       vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd);
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
@@ -1245,7 +1264,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     }
     if (fused) {
       // everything on the kernel stream: the decisions, then the fused levels
-      if (!recon)
+      if (!recon && !gt)
         for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
       if (lf) vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
       if (recon && g->timing && !n_wave_intra) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
@@ -1315,8 +1334,15 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       HIP_TRY(hipEventRecord(g->ev_lf_done, sl), "hipEventRecord");
       HIP_TRY(hipStreamWaitEvent(s, g->ev_lf_done, 0), "hipStreamWaitEvent");
     }
-    HIP_TRY(hipEventRecord(slot.ev, s), "hipEventRecord");
-    slot.pending = true;
+    if (gt) {
+      hipError_t le = hipGetLastError();
+      hipError_t ce = hipStreamEndCapture(s, &gt->graph);
+      HIP_TRY(le, "kernel launch (capture)");
+      HIP_TRY(ce, "hipStreamEndCapture");
+    } else {
+      HIP_TRY(hipEventRecord(slotp->ev, s), "hipEventRecord");
+      slotp->pending = true;
+    }
   }
   if (lf && g->timing) HIP_TRY(hipEventRecord(g->ev[4], s), "hipEventRecord");
   HIP_TRY(hipGetLastError(), "kernel launch");
@@ -1453,6 +1479,127 @@ int vp9r_gpu_run_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *pr, 
     HIP_TRY(hipEventRecord(g->ev[0], g->stream), "hipEventRecord");
   }
   return launch_batch(g, bf.data(), (int)n, true, true);
+}
+
+}  // extern "C"
+
+// A frame's launches captured once and replayed: the graph holds its own
+// batch tables and the frame stores it was made for.
+struct vp9r_graph {
+  vp9r_gpu *g = nullptr;
+  const vp9r_prepared *p = nullptr;
+  vp9r_gpu_frame *refs[3] = {nullptr, nullptr, nullptr};
+  vp9r_gpu_frame *cur = nullptr;
+  vp9r::GraphTarget target;
+  hipGraphExec_t exec = nullptr;
+  uint32_t nodes = 0;
+  ~vp9r_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (target.graph) (void)hipGraphDestroy(target.graph);
+  }
+};
+
+namespace {
+
+// Captures the launches of (p, refs, cur) into gr->target (tables and graph).
+int capture_frame(vp9r_gpu *g, const vp9r_prepared *pr, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur,
+                  vp9r_graph *gr) {
+  const Prepared &p = pr->p;
+  vp9r_gpu_frame *none[3] = {nullptr, nullptr, nullptr};
+  int st = check_frames(p, refs ? refs : none, cur);
+  if (st) return st;
+  BatchFrame bf{&p, (const uint8_t *)pr->dev.p, {nullptr, nullptr, nullptr}, cur};
+  if (!p.hdr.frame_is_intra)
+    for (int k = 0; k < 3; k++) bf.refs[k] = refs[k];
+  if (gr->target.graph) {
+    (void)hipGraphDestroy(gr->target.graph);
+    gr->target.graph = nullptr;
+  }
+  st = launch_batch(g, &bf, 1, true, true, &gr->target);
+  if (st) return st;
+  size_t n = 0;
+  HIP_TRY(hipGraphGetNodes(gr->target.graph, nullptr, &n), "hipGraphGetNodes");
+  gr->nodes = (uint32_t)n;
+  gr->p = pr;
+  for (int k = 0; k < 3; k++) gr->refs[k] = refs ? refs[k] : nullptr;
+  gr->cur = cur;
+  return VP9R_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vp9r_gpu_graph_new(vp9r_gpu *g, const vp9r_prepared *pr, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur,
+                       vp9r_graph **out) {
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (!pr) {
+    set_error("vp9r_gpu_graph_new: null prepared record");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  vp9r_graph *gr = new (std::nothrow) vp9r_graph();
+  if (!gr) return VP9R_ERR_INVALID_INPUT;
+  gr->g = g;
+  int st = capture_frame(g, pr, refs, cur, gr);
+  if (!st) {
+    hipError_t e = hipGraphInstantiate(&gr->exec, gr->target.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      st = VP9R_ERR_DEVICE;
+    }
+  }
+  if (st) {
+    delete gr;
+    return st;
+  }
+  gpu_retain(g);
+  *out = gr;
+  return VP9R_OK;
+}
+
+int vp9r_gpu_graph_update(vp9r_graph *gr, vp9r_gpu_frame *const refs[3], vp9r_gpu_frame *cur) {
+  vp9r_gpu *g = gr->g;
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  // a fresh capture with the new stores (tables re-uploaded), its kernel
+  // parameters moved into the instantiated graph; a fresh instantiation if
+  // the topology differs
+  int st = capture_frame(g, gr->p, refs, cur, gr);
+  if (st) return st;
+  hipGraphNode_t err_node = nullptr;
+  hipGraphExecUpdateResult res;
+  hipError_t e = hipGraphExecUpdate(gr->exec, gr->target.graph, &err_node, &res);
+  if (e != hipSuccess || res != hipGraphExecUpdateSuccess) {
+    (void)hipGetLastError();
+    (void)hipGraphExecDestroy(gr->exec);
+    gr->exec = nullptr;
+    e = hipGraphInstantiate(&gr->exec, gr->target.graph, nullptr, nullptr, 0);
+    HIP_TRY(e, "hipGraphInstantiate");
+  }
+  return VP9R_OK;
+}
+
+int vp9r_gpu_graph_run(vp9r_graph *gr) {
+  vp9r_gpu *g = gr->g;
+  HIP_TRY(hipSetDevice(g->device), "hipSetDevice");
+  if (g->timing) {
+    set_error("vp9r_gpu_graph_run: stage timing is on");
+    return VP9R_ERR_INVALID_INPUT;
+  }
+  // the downloads in flight of the store it writes, then the graph
+  HIP_TRY(order_after_copy(g->stream, gr->cur), "hipStreamWaitEvent");
+  HIP_TRY(hipGraphLaunch(gr->exec, g->stream), "hipGraphLaunch");
+  return VP9R_OK;
+}
+
+uint32_t vp9r_gpu_graph_nodes(const vp9r_graph *gr) { return gr->nodes; }
+
+void vp9r_gpu_graph_free(vp9r_graph *gr) {
+  if (!gr) return;
+  vp9r_gpu *g = gr->g;
+  (void)hipStreamSynchronize(g->stream);
+  delete gr;
+  gpu_release(g);
 }
 
 int vp9r_gpu_filter_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *pr, vp9r_gpu_frame *const *cur) {

@@ -95,9 +95,18 @@ struct BatchSlot {
 };
 constexpr int kBatchSlots = 4;
 
+// A capture of one frame's launches as a HIP graph: the batch tables go to
+// `tables` (owned by the graph, uploaded before the capture begins), and
+// only kernel launches are captured (no copies, events or waits).
+struct GraphTarget {
+  DevBuf tables;
+  hipGraph_t graph = nullptr;  // the capture's result
+};
+
 // Queues reconstruction (recon) and/or the loop filter (lf) of n frames on
-// the context's stream, each launch covering every frame.
-int launch_batch(vp9r_gpu *g, const BatchFrame *f, int n, bool recon, bool lf);
+// the context's stream, each launch covering every frame.  With gt (one
+// frame, timing off): captures the launches into gt->graph instead.
+int launch_batch(vp9r_gpu *g, const BatchFrame *f, int n, bool recon, bool lf, GraphTarget *gt = nullptr);
 
 // Takes / releases a reference to a context (the last release destroys it).
 void gpu_retain(vp9r_gpu *g);
@@ -166,4 +175,11 @@ struct vp9r_gpu {
   vp9r::DevBuf seam_a, seam_b, seam_c;
   hipEvent_t seam_ev[2] = {};
   float seam_ms = -1.0f;
+  // per-kernel seam launch shapes (vp9r_gpu_autotune / vp9r_gpu_set_tuning):
+  // waves per workgroup of the inverse transforms per [8-bit, 10/12-bit]
+  // [4x4 .. 32x32], and rows per motion compensation tile per [8-bit,
+  // 10/12-bit]
+  uint8_t tune_itx_waves[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
+  uint8_t tune_mc_rows[2] = {32, 32};
+  bool tuned = false;
 };

@@ -66,3 +66,42 @@ def test_mc_batch_every_block(gpu, n, bd):
     got = run_mc(gpu, m)
     want = mb.oracle_mc(m, THREADS)
     assert np.array_equal(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("bd", [8, 10])
+def test_itx_every_tuned_shape(bd, waves):
+    """Each launch shape the autotuner may choose (1, 2 or 4 waves per
+    transform workgroup) gives the oracle's output at every size."""
+    g = vp.Gpu(0)
+    g.set_tuning([waves] * 4, [waves] * 4, 32, 32)
+    s = mb.ItxSet(60_000, 30 + waves, bd)
+    got = run_itx(g, s)
+    want = mb.oracle_itx(s, THREADS)
+    assert np.array_equal(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("rows", [16, 32])
+@pytest.mark.parametrize("bd", [8, 10])
+def test_mc_every_tuned_shape(bd, rows):
+    """Tiles of 16 and of 32 rows give the oracle's output."""
+    g = vp.Gpu(0)
+    g.set_tuning([1] * 4, [1] * 4, rows, rows)
+    m = mb.McSet(30_000, 40 + rows, bd)
+    got = run_mc(g, m)
+    want = mb.oracle_mc(m, THREADS)
+    assert np.array_equal(got, want), first_diff(got, want)
+
+
+def test_autotune_records_its_choices():
+    g = vp.Gpu(0)
+    assert g.tuning()["tuned"] is False
+    choice, times = g.autotune()
+    print(choice, [round(t, 1) for t in times], flush=True)
+    assert choice["tuned"] is True
+    assert all(w in (1, 2, 4) for w in choice["itx_waves_8bit"] + choice["itx_waves_hbd"])
+    assert choice["mc_tile_rows_8bit"] in (16, 32) and choice["mc_tile_rows_hbd"] in (16, 32)
+    assert all(t > 0 for t in times)
+    # the tuned context still gives the oracle's output
+    s = mb.ItxSet(20_000, 5, 8)
+    assert np.array_equal(run_itx(g, s), mb.oracle_itx(s, THREADS))
