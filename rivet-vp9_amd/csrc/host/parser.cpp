@@ -1852,10 +1852,10 @@ bool Parser::decode_tiles(const uint8_t *data, size_t len) {
   // (tile rows, then columns), and the symbol counts summed, so the record
   // is the serial one.  The error reported is the first in that order.
   tile_out_.resize(tile_rows * tile_cols);
-  if (!pool_ || pool_->threads() != workers) pool_.reset(new TaskPool(workers));
+  if (!pool_.p || pool_.p->threads() != workers) pool_.p.reset(new TaskPool(workers));
   std::vector<ParseError> errs(tile_rows * tile_cols, ParseError{0, ""});
   std::vector<uint8_t> pad(tile_cols, 1);
-  pool_->run(tile_cols, [&](uint32_t col) {
+  pool_.p->run(tile_cols, [&](uint32_t col) {
     for (uint32_t row = 0; row < tile_rows; row++) {
       const uint32_t i = row * tile_cols + col;
       TileOut &o = tile_out_[i];
@@ -1884,7 +1884,7 @@ bool Parser::decode_tiles(const uint8_t *data, size_t len) {
   txbs_.resize(t0[n_tiles]);
   coef_pos_.resize(c0[n_tiles]);
   coef_val_.resize(c0[n_tiles]);
-  pool_->run(n_tiles, [&](uint32_t i) {
+  pool_.p->run(n_tiles, [&](uint32_t i) {
     const TileOut &o = tile_out_[i];
     if (!o.blocks.empty()) memcpy(&blocks_[b0[i]], o.blocks.data(), sizeof(vp9r_block) * o.blocks.size());
     for (size_t k = 0; k < o.txbs.size(); k++) {

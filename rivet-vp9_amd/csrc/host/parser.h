@@ -105,6 +105,18 @@ class TaskPool {
   bool stop_ = false;
 };
 
+// A parser's pool, which a copy of the parser does not share (a copy starts
+// without one and makes its own on first use).
+struct PoolHolder {
+  std::unique_ptr<TaskPool> p;
+  PoolHolder() = default;
+  PoolHolder(const PoolHolder &) {}
+  PoolHolder &operator=(const PoolHolder &) {
+    p.reset();
+    return *this;
+  }
+};
+
 // One tile's part of the record and its symbol counts.
 struct TileOut {
   std::vector<vp9r_block> blocks;
@@ -276,7 +288,7 @@ class Parser {
   std::vector<int32_t> coef_val_;
   std::vector<vp9r_mi> lf_mi_;
   std::vector<TileOut> tile_out_;  // per tile, when tile columns are parsed in parallel
-  std::unique_ptr<TaskPool> pool_;  // tile_threads threads, made on first use
+  PoolHolder pool_;  // tile_threads threads, made on first use
 
   void parse_uncompressed(const uint8_t *data, size_t len, int *reset_kind);
   void setup_frame(int reset_kind);

@@ -12,7 +12,8 @@ GEN_LIB = os.path.join(GEN_DIR, "_build", "libstreamgen.so")
 class Config(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "width", "height", "frames", "seed", "bit_depth", "ss_x", "ss_y", "base_q_idx", "tile_cols_log2",
-        "golden_interval", "content", "resize_every", "intra_percent", "allow_hp", "lf_level", "lf_sharpness")]
+        "golden_interval", "content", "resize_every", "intra_percent", "allow_hp", "lf_level", "lf_sharpness",
+        "segmentation")]
 
 
 _lib = None
@@ -21,8 +22,8 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(GEN_LIB):
-            subprocess.run(["make", "-s", "-C", GEN_DIR], check=True)
+        # make rebuilds when the generator's or the parser's sources are newer
+        subprocess.run(["make", "-s", "-C", GEN_DIR], check=True)
         L = ctypes.CDLL(GEN_LIB)
         L.sg_new.restype = ctypes.c_void_p
         L.sg_new.argtypes = [ctypes.POINTER(Config)]
@@ -36,7 +37,7 @@ def lib():
 
 def generate(width, height, frames, seed, bit_depth=8, ss_x=1, ss_y=1, base_q_idx=100, tile_cols_log2=0,
              golden_interval=8, content=1, resize_every=0, intra_percent=3, allow_hp=1, lf_level=None,
-             lf_sharpness=0):
+             lf_sharpness=0, segmentation=0):
     """The frames (bytes) of a generated stream.  lf_level: the loop filter
     level (0..63; None derives it from the quantiser); lf_sharpness 0..7."""
     L = lib()
@@ -44,7 +45,7 @@ def generate(width, height, frames, seed, bit_depth=8, ss_x=1, ss_y=1, base_q_id
         raise ValueError("loop filter level 0..63, sharpness 0..7")
     cfg = Config(width, height, frames, seed, bit_depth, ss_x, ss_y, base_q_idx, tile_cols_log2,
                  golden_interval, content, resize_every, intra_percent, allow_hp,
-                 0 if lf_level is None else lf_level + 1, lf_sharpness)
+                 0 if lf_level is None else lf_level + 1, lf_sharpness, segmentation)
     g = L.sg_new(ctypes.byref(cfg))
     out = []
     try:

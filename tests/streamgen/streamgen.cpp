@@ -18,9 +18,16 @@
 // Frame headers mirror the crate's encoder (src/encoder/mod.rs:558-697):
 // key frames, then error-resilient, frame-parallel inter frames without
 // context refresh, EIGHTTAP, single reference, TX_MODE_SELECT, loop filter
-// level (q * 10 + 32) / 64 unless set, sharpness as set (0 by default), no segmentation, LAST refreshed
+// level (q * 10 + 32) / 64 unless set, sharpness as set (0 by default), LAST refreshed
 // every frame and GOLDEN every golden_interval frames.  Optional resizing
 // codes every Nth frame at half size (scaled references both ways).
+// Optional segmentation (segmentation = 1): a segment map written every
+// frame but every third inter frame (which then keeps the predicted map,
+// all 0 after the error-resilient reset), temporally predicted on odd
+// frames, with the features of seg_features() in delta mode on even frames
+// and absolute mode on odd ones: quantiser (ALT_Q) and filter level
+// (ALT_L) changes, a forced LAST reference (REF_FRAME, inter frames) and
+// forced skip (SKIP).
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -51,6 +58,7 @@ typedef struct sg_config {
   uint32_t allow_hp;
   uint32_t lf_level;      // 0 derived from the quantiser, else the filter level + 1
   uint32_t lf_sharpness;  // 0..7
+  uint32_t segmentation;  // 0 off; 1 segment map and features (below)
 } sg_config;
 }
 
@@ -79,8 +87,18 @@ struct Rect {
 class Plan : public SyntaxPlan {
  public:
   Plan(const sg_config &c, uint32_t frame_w, uint32_t frame_h, int t, bool key, const std::vector<Rect> &rects,
-       double bg_vx, double bg_vy, std::mt19937_64 &rng)
-      : c_(c), w_(frame_w), h_(frame_h), t_(t), key_(key), rects_(rects), bgx_(bg_vx), bgy_(bg_vy), rng_(rng) {}
+       double bg_vx, double bg_vy, std::mt19937_64 &rng, bool seg_map)
+      : c_(c), w_(frame_w), h_(frame_h), t_(t), key_(key), rects_(rects), bgx_(bg_vx), bgy_(bg_vy), rng_(rng),
+        seg_map_(seg_map) {}
+
+  // The segment of a block: from its position and the frame when the map is
+  // written, else the predicted one (0: the error-resilient reset map).
+  uint8_t segment_of(uint32_t mi_row, uint32_t mi_col) const {
+    if (!c_.segmentation || !seg_map_) return 0;
+    const uint32_t v = (mi_row / 2) * 5 + (mi_col / 2) * 3 + (uint32_t)t_ * 7;
+    const uint8_t sid = (uint8_t)((v * 2654435761u) >> 29);  // 0..7
+    return key_ && sid == 4 ? 0 : sid;  // no forced reference on an intra frame
+  }
 
   // the object covering luma sample (x, y): -1 background, else a rectangle
   int object_at(double x, double y) const {
@@ -149,8 +167,11 @@ class Plan : public SyntaxPlan {
     int max_tx = VP9_MAX_TXSIZE_LOOKUP[bsize];
     o->tx_size = (uint8_t)(u(rng_) < 70 ? max_tx : (max_tx > 0 ? max_tx - 1 : 0));
     o->interp_filter = EIGHTTAP;
-    o->segment_id = 0;
-    bool intra = frame_is_intra || (int)u(rng_) < (int)c_.intra_percent || uncovered(cx, cy);
+    o->segment_id = segment_of(mi_row, mi_col);
+    // segment 4 forces the LAST reference (an inter block), segment 5 skip
+    const bool seg_ref = c_.segmentation && o->segment_id == 4;
+    const bool seg_skip = c_.segmentation && o->segment_id == 5;
+    bool intra = !seg_ref && (frame_is_intra || (int)u(rng_) < (int)c_.intra_percent || uncovered(cx, cy));
     cur_intra_ = intra;
     if (intra) {
       static const uint8_t modes[10] = {DC_PRED, DC_PRED, DC_PRED, TM_PRED, TM_PRED, V_PRED, H_PRED,
@@ -163,12 +184,12 @@ class Plan : public SyntaxPlan {
       o->is_inter = 0;
       o->ref_frame[0] = INTRA_FRAME;
       o->ref_frame[1] = REF_NONE;
-      o->skip = u(rng_) < 5;
+      o->skip = u(rng_) < 5 || seg_skip;
       cur_skip_ = o->skip;
       return;
     }
     o->is_inter = 1;
-    o->ref_frame[0] = (c_.golden_interval && u(rng_) < 8) ? GOLDEN_FRAME : LAST_FRAME;
+    o->ref_frame[0] = (!seg_ref && c_.golden_interval && u(rng_) < 8) ? GOLDEN_FRAME : LAST_FRAME;
     o->ref_frame[1] = REF_NONE;
     // motion: content moving by (vx, vy) per frame predicts from (-vx, -vy)
     double vx = bgx_, vy = bgy_;
@@ -183,7 +204,7 @@ class Plan : public SyntaxPlan {
     o->mv[0][0] = (int32_t)std::lround(-vy * 8.0 * frames_back);
     o->mv[0][1] = (int32_t)std::lround(-vx * 8.0 * frames_back);
     bool boundary = partition_boundary(mi_row, mi_col, bw, bh);
-    o->skip = !boundary && u(rng_) < (c_.content == 0 ? 20 : 60);
+    o->skip = (!boundary && u(rng_) < (c_.content == 0 ? 20 : 60)) || seg_skip;
     cur_skip_ = o->skip;
   }
 
@@ -230,6 +251,7 @@ class Plan : public SyntaxPlan {
   const std::vector<Rect> &rects_;
   double bgx_, bgy_;
   std::mt19937_64 &rng_;
+  bool seg_map_ = false;
   bool cur_intra_ = false, cur_skip_ = false;
 };
 
@@ -278,6 +300,40 @@ struct Gen {
       *w = (c.width + 1) / 2;
       *h = (c.height + 1) / 2;
     }
+  }
+
+  // Segmentation (6.2.11): the map is written on key frames and on two of
+  // every three inter frames.
+  bool seg_map(bool key) const { return key || t % 3 != 2; }
+  // Features per segment, delta or absolute: 1 and 7 quantiser only, 2 and
+  // 6 quantiser and filter level, 3 filter level only, 4 the LAST reference
+  // (inter frames), 5 skip, 0 none.
+  static void seg_features(bool key, bool abs, bool on[8][4], int16_t data[8][4]) {
+    memset(on, 0, sizeof(bool) * 32);
+    memset(data, 0, sizeof(int16_t) * 32);
+    auto set = [&](int s, int f, int v) {
+      on[s][f] = true;
+      data[s][f] = (int16_t)v;
+    };
+    if (abs) {
+      set(1, 0, 60);
+      set(2, 0, 140);
+      set(2, 1, 10);
+      set(3, 1, 36);
+      set(6, 0, 90);
+      set(6, 1, 0);
+      set(7, 0, 200);
+    } else {
+      set(1, 0, -24);
+      set(2, 0, 40);
+      set(2, 1, -12);
+      set(3, 1, 9);
+      set(6, 0, -8);
+      set(6, 1, 20);
+      set(7, 0, 60);
+    }
+    if (!key) set(4, 2, 1);  // LAST_FRAME
+    set(5, 3, 0);
   }
 
   std::vector<uint8_t> header(bool key, uint32_t w, uint32_t h, uint32_t refresh, uint32_t lf_level) {
@@ -340,7 +396,42 @@ struct Gen {
     b.f(1, 0);
     b.f(1, 0);
     b.f(1, 0);  // no delta_q
-    b.f(1, 0);  // segmentation_enabled
+    if (!c.segmentation) {
+      b.f(1, 0);  // segmentation_enabled
+    } else {
+      b.f(1, 1);
+      const bool map = seg_map(key);
+      b.f(1, map);  // update_map
+      if (map) {
+        for (int i = 0; i < 7; i++) {
+          const bool coded = (i + t) % 3 != 0;
+          b.f(1, coded);
+          if (coded) b.f(8, 64 + ((i * 37 + t * 11) % 128));
+        }
+        const bool temporal = !key && (t & 1);
+        b.f(1, temporal);  // temporal_update
+        if (temporal)
+          for (int i = 0; i < 3; i++) {
+            b.f(1, 1);
+            b.f(8, 160 + 20 * i);
+          }
+      }
+      b.f(1, 1);  // update_data (every frame: error resilience clears the features)
+      const bool abs = t & 1;
+      b.f(1, abs);
+      int16_t data[8][4];
+      bool on[8][4];
+      seg_features(key, abs, on, data);
+      static const int bits[4] = {8, 6, 2, 0}, sgn[4] = {1, 1, 0, 0};
+      for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 4; j++) {
+          b.f(1, on[i][j]);
+          if (!on[i][j]) continue;
+          const int v = data[i][j];
+          if (bits[j]) b.f(bits[j], (uint32_t)(v < 0 ? -v : v));
+          if (sgn[j]) b.f(1, v < 0);
+        }
+    }
     uint32_t sb64_cols = (((w + 7) >> 3) + 7) >> 3;
     uint32_t min_log2 = 0, max_log2 = 1;
     while ((64u << min_log2) < sb64_cols) min_log2++;
@@ -364,7 +455,7 @@ struct Gen {
     uint32_t refresh = key ? 0xff : (1u | ((c.golden_interval && t % c.golden_interval == 0) ? 2u : 0u));
     uint32_t lf_level = c.lf_level ? std::min(63u, c.lf_level - 1) : std::min(63u, (c.base_q_idx * 10 + 32) / 64);
     std::vector<uint8_t> hdr = header(key, w, h, refresh, lf_level);
-    Plan plan(c, w, h, (int)t, key, rects, bgx, bgy, rng);
+    Plan plan(c, w, h, (int)t, key, rects, bgx, bgy, rng, seg_map(key));
     Parser sim = dec;
     frame = sim.write_frame(hdr.data(), hdr.size(), TX_MODE_SELECT, SINGLE_REFERENCE, &plan);
     vp9r_record rec;

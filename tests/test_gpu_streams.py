@@ -33,6 +33,15 @@ CASES = {
                                   lf_sharpness=3),
     "level0-filter-off": dict(width=160, height=96, frames=3, seed=12, lf_level=0),
     "1080p-workload": dict(streamgen.CONFIG2, frames=3),
+    # segmentation: per-segment quantisers (ALT_Q) and filter levels (ALT_L),
+    # absolute and delta, forced reference and skip, written, temporally
+    # predicted and kept maps (recon.rs:219-230, loopfilter.rs:15-47)
+    "segmentation-352x288": dict(width=352, height=288, frames=6, seed=61, content=1, intra_percent=10,
+                                 segmentation=1),
+    "segmentation-640x360-2tiles-10bit": dict(width=640, height=360, frames=4, seed=62, tile_cols_log2=1,
+                                              bit_depth=10, content=1, intra_percent=10, segmentation=1),
+    "segmentation-12bit-444": dict(width=160, height=96, frames=4, seed=63, bit_depth=12, ss_x=0, ss_y=0,
+                                   segmentation=1),
     # the 4K 10-bit workload, resized every other frame: 2:1 and 1:2 scaled references
     "4k-10bit-scaled": dict(streamgen.CONFIG4, frames=4, resize_every=2),
 }

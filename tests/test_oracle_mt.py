@@ -24,7 +24,8 @@ def md5s(frames, threads):
          lf_sharpness=3),
     dict(width=352, height=288, frames=4, seed=53, content=0),
     dict(width=640, height=360, frames=3, seed=54, tile_cols_log2=1, bit_depth=10, content=1),
-], ids=["640x360-2tiles", "720p-4tiles-sharp3", "352x288-1tile", "640x360-10bit"])
+    dict(width=640, height=360, frames=4, seed=55, tile_cols_log2=1, content=1, intra_percent=10, segmentation=1),
+], ids=["640x360-2tiles", "720p-4tiles-sharp3", "352x288-1tile", "640x360-10bit", "640x360-segmentation"])
 def test_threaded_decoder_equals_serial(cfg):
     frames = streamgen.generate(**cfg)
     one = md5s(frames, 1)

@@ -38,9 +38,11 @@ def records(frames, threads):
     return out
 
 
-@pytest.mark.parametrize("w,h,tiles,frames", [(640, 360, 1, 5), (1280, 720, 2, 4), (1920, 1080, 2, 3)])
-def test_parallel_tile_parse_equals_serial(w, h, tiles, frames):
-    s = streamgen.generate(w, h, frames, seed=21, tile_cols_log2=tiles, content=1, intra_percent=10)
+@pytest.mark.parametrize("w,h,tiles,frames,seg", [(640, 360, 1, 5, 0), (1280, 720, 2, 4, 0), (1920, 1080, 2, 3, 0),
+                                                  (1280, 720, 2, 4, 1)])
+def test_parallel_tile_parse_equals_serial(w, h, tiles, frames, seg):
+    s = streamgen.generate(w, h, frames, seed=21, tile_cols_log2=tiles, content=1, intra_percent=10,
+                           segmentation=seg)
     one = records(s, 1)
     assert len(one) == frames
     for t in (2, 4, 8):

@@ -92,3 +92,32 @@ def test_benchmark_workload_headers():
             blocks = rec.blocks[:rec.n_blocks]
             inter = sum(1 for b in blocks if b.is_inter)
             assert 0 < inter < rec.n_blocks
+
+
+def test_segmentation_stream_reaches_every_segment_and_feature():
+    """The generator's segmentation: every segment id in the map, ALT_Q and
+    ALT_L in delta and absolute mode, the forced reference and skip, a map
+    kept from the prediction, and the oracle decodes it."""
+    frames = streamgen.generate(352, 288, 6, seed=61, content=1, intra_percent=10, segmentation=1)
+    p = vp.Parser()
+    seen, modes, kept = set(), set(), 0
+    for f in frames:
+        r = p.parse(f)
+        h = r.hdr
+        assert h.seg_enabled
+        ids = {r.blocks[i].segment_id for i in range(r.n_blocks)}
+        seen |= ids
+        kept += ids == {0}
+        modes.add(h.seg_abs_or_delta_update)
+        assert h.seg_feature_enabled[1][0] and h.seg_feature_enabled[3][1] and h.seg_feature_enabled[5][3]
+        if not h.frame_is_intra:
+            assert h.seg_feature_enabled[4][2] and h.seg_feature_data[4][2] == 1
+            for i in range(r.n_blocks):
+                b = r.blocks[i]
+                if b.segment_id == 4:
+                    assert b.is_inter and b.ref_frame[0] == 1
+                if b.segment_id == 5:
+                    assert b.skip
+    assert seen == set(range(8)) and modes == {0, 1} and kept >= 1
+    shown, _, _ = decode_all(frames)
+    assert len(shown) == 6
