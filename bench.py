@@ -714,7 +714,7 @@ def serve_run(vp, plan, streams, paced, parse_threads, fps=30.0):
     th = threading.Thread(target=submitter)
     th.start()
     pool = concurrent.futures.ThreadPoolExecutor(8)
-    futs, lat, status = [], [], {}
+    futs, lat, status, lat_at = [], [], {}, []
     last = start
     while len(status) < len(streams):
         if err:
@@ -729,6 +729,7 @@ def serve_run(vp, plan, streams, paced, parse_threads, fps=30.0):
             continue
         last = time.perf_counter()
         lat.append(fr.latency_ms)
+        lat_at.append((fr.stream, fr.index, fr.latency_ms))
         futs.append(pool.submit(hash_release, fr))
     th.join()
     if err:
@@ -742,6 +743,7 @@ def serve_run(vp, plan, streams, paced, parse_threads, fps=30.0):
     srv.close_server()
     md5 = [[m[k] for k in sorted(m)] for m in md5]
     st = [status.get(sid_of[i]) for i in range(len(streams))]
+    serve_run.latency_at = [(idx_of[sid], j, ms) for sid, j, ms in lat_at]
     return md5, st, lat, last - start, stats
 
 
@@ -781,13 +783,18 @@ def serving_bench(vp, n_streams, parse_threads):
                             "sample": "oracle decode (product parser + scalar reconstruction and loop filter) of "
                                       "every stream, one process per stream"}}
     exact = True
-    for mode, paced in (("paced", True), ("saturated", False)):
-        md5, st, lat, wall, stats = serve_run(vp, plan, streams, paced, parse_threads)
+    # paced: arrivals as planned; all_live: every stream arriving at once and
+    # paced at 30 frames/s, so that all of them are live together until the
+    # shortest ends; saturated: every packet at once (capacity)
+    plan_all = [(0.0, cfg) for _, cfg in plan]
+    for mode, paced, pl in (("paced", True, plan), ("paced_all_live", True, plan_all),
+                            ("saturated", False, plan)):
+        md5, st, lat, wall, stats = serve_run(vp, pl, streams, paced, parse_threads)
         ok = all(s == vp.OK for s in st) and md5 == expected
         exact &= ok
         conc = 0
         if paced:
-            iv = [(a, a + (len(s) - 1) / 30.0) for (a, _), s in zip(plan, streams)]
+            iv = [(a, a + (len(s) - 1) / 30.0) for (a, _), s in zip(pl, streams)]
             conc = max(sum(1 for a, b in iv if a <= t <= b) for t, _ in iv)
         out[mode] = {"aggregate_fps": round(len(lat) / wall, 2),
                      "latency_ms": {"p50": round(_pct(lat, 0.5), 3), "p99": round(_pct(lat, 0.99), 3),
@@ -798,6 +805,16 @@ def serving_bench(vp, n_streams, parse_threads):
         if paced:
             out[mode]["stream_fps"] = 30
             out[mode]["max_concurrent_streams"] = conc
+        if mode == "paced_all_live":
+            # the frames decoded while every stream was live (each stream's
+            # first min-length frames)
+            n_live = min(len(x) for x in streams)
+            live = [ms for i, j, ms in serve_run.latency_at if j < n_live]
+            out[mode]["all_live_window"] = {"streams": len(streams), "frames_per_stream": n_live,
+                                            "seconds": round(n_live / 30.0, 2),
+                                            "latency_ms": {"p50": round(_pct(live, 0.5), 3),
+                                                           "p99": round(_pct(live, 0.99), 3),
+                                                           "max": round(max(live), 3)}}
         _log(f"bench: serving {mode}: {out[mode]}")
     out["bit_exact"] = exact
     return out

@@ -694,7 +694,8 @@ __global__ __launch_bounds__(kIbThreads) void k_wave(uint32_t n_iw, uint32_t geo
   const uint32_t w = blockIdx.x - n_iw;
   // the wave's plane, uniform across the wave
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), plane = wave < 3 ? wave : 0;
-  LfShared &sh = ((LfShared *)ib_lds)[plane];
+  typedef LfShared<typename std::conditional<HBD, uint16_t, uint8_t>::type> LfS;
+  LfS &sh = ((LfS *)ib_lds)[plane];
   const int lane = (int)(threadIdx.x & 63);
   if (!ONE) {
     const uint32_t fi = range_frame(rg_l, nf, w);
@@ -709,7 +710,8 @@ __global__ __launch_bounds__(kIbThreads) void k_wave(uint32_t n_iw, uint32_t geo
 }
 
 uint32_t wave_lds_bytes(int ss_x, int ss_y) {
-  return max(ib_layout(ss_x, ss_y).bytes, (uint32_t)(3 * sizeof(LfShared)));
+  // the u16 form, the larger: one size for both sample types
+  return max(ib_layout(ss_x, ss_y).bytes, (uint32_t)(3 * sizeof(LfShared<uint16_t>)));
 }
 
 void launch_wave(hipStream_t s, const WaveLaunch &w, bool hbd) {

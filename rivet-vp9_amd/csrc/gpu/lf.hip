@@ -122,7 +122,7 @@ template <bool HBD, bool ONE>
 __global__ __launch_bounds__(64) void k_lf(uint8_t *p0, uint8_t *p1, uint8_t *p2, uint32_t s0, uint32_t s1,
                                            uint32_t s2, uint32_t *dq, uint32_t geo, uint32_t t1, const LfArgs *fa,
                                            const Range *rg, uint32_t nf) {
-  __shared__ LfShared sh;
+  __shared__ LfShared<typename std::conditional<HBD, uint16_t, uint8_t>::type> sh;
   // a launch of one frame (ONE) passes what the workgroups need as
   // arguments (preloaded, so the region loads need no load before them, and
   // known to be global memory: LfOne); a batch finds each frame's through

@@ -11,7 +11,13 @@
 
 namespace vp9g {
 
-constexpr int BP = 138;  // LDS pitch of a unit's region (8 + 128 columns + slack, odd in dwords)
+// LDS pitch of a unit's region in samples (8 + 128 columns + slack, an odd
+// number of dwords): samples are held in LDS as bytes at 8 bits, as u16
+// above.
+template <class T>
+struct LfPitch {
+  static constexpr int BP = sizeof(T) == 1 ? 140 : 138;
+};
 
 // |a - b| of two samples (< 2^16) in one instruction (v_sad_u16).
 DEV uint32_t adiff(int a, int b) { return __builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }
@@ -111,8 +117,10 @@ DEV void lf_filter(uint16_t *b, int pos, int step, int fs, int limit, int blimit
 // 8-sample window margin on its left.  The rows past the region are slack
 // for lf_stream's reads past a line's end.
 
+template <class T>
 struct __align__(16) LfShared {
-  uint16_t buf[80 * BP];
+  static constexpr int BP = LfPitch<T>::BP;
+  T buf[80 * BP];
   uint32_t dq[3][272];  // packed decisions of the item's passes (16 slack entries each)
 };
 
@@ -172,10 +180,12 @@ DEV void lf_region_load(const P *pl, uint32_t stride, const LfItem &it, LfRegion
   }
 }
 
-// Samples as 16-bit values, two to a dword store (BP and the column offsets
-// are even).
+// The region's 8-sample vectors into LDS (as bytes at 8 bits: two dword
+// stores; as u16 above: four), and back.  BP and the column offsets keep
+// every store dword-aligned.
 template <class P, int PART, int N>
-DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P, N> &rg, uint16_t *buf, int lane) {
+DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P, N> &rg, P *buf, int lane) {
+  constexpr int BP = LfPitch<P>::BP;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     int r, q;
@@ -183,10 +193,8 @@ DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P, N> &rg, uint16_t *
     uint32_t *l = (uint32_t *)(buf + r * BP + 8 + 8 * q);
     if (sizeof(P) == 1) {
       const uint2 w = *(const uint2 *)&rg.v[i];
-      l[0] = (w.x & 0xff) | (w.x & 0xff00) << 8;
-      l[1] = (w.x >> 16 & 0xff) | (w.x >> 8 & 0xff0000);
-      l[2] = (w.y & 0xff) | (w.y & 0xff00) << 8;
-      l[3] = (w.y >> 16 & 0xff) | (w.y >> 8 & 0xff0000);
+      l[0] = w.x;
+      l[1] = w.y;
     } else {
       const uint4 w = *(const uint4 *)&rg.v[i];
       l[0] = w.x;
@@ -200,8 +208,8 @@ DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P, N> &rg, uint16_t *
 // Stores the vectors of a part; skip_last leaves out the last vector of
 // each superblock row (part 0 before the second vertical pass).
 template <class P, int PART, int N>
-DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_t *buf, bool skip_last,
-                         int lane) {
+DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const P *buf, bool skip_last, int lane) {
+  constexpr int BP = LfPitch<P>::BP;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     int r, q;
@@ -209,14 +217,10 @@ DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_
     if (skip_last && r >= 8 && q == (1 << it.lnt) - 1) continue;
     const uint32_t *l = (const uint32_t *)(buf + r * BP + 8 + 8 * q);
     P *g = pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q;
-    if (sizeof(P) == 1) {
-      uint2 w;
-      w.x = (l[0] & 0xff) | (l[0] >> 8 & 0xff00) | (l[1] & 0xff) << 16 | (l[1] >> 16) << 24;
-      w.y = (l[2] & 0xff) | (l[2] >> 8 & 0xff00) | (l[3] & 0xff) << 16 | (l[3] >> 16) << 24;
-      gst((uint2 *)g, w);
-    } else {
+    if (sizeof(P) == 1)
+      gst((uint2 *)g, make_uint2(l[0], l[1]));
+    else
       gst((uint4 *)g, make_uint4(l[0], l[1], l[2], l[3]));
-    }
   }
 }
 
@@ -238,9 +242,9 @@ DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const uint16_
 // entering the window are fetched before the filter (the filter of edge e
 // writes at most up to position 4 e + 14).
 // Edge e = 4 m + K of lf_stream (K a constant: every window index is one).
-template <int K>
-DEV void lf_step(int (&w)[16], uint16_t *ln, int e, int step, int n_edges, const uint32_t *dq, int rn,
-                 uint32_t &d_next, int bd) {
+template <int K, class T>
+DEV void lf_step(int (&w)[16], T *ln, int e, int step, int n_edges, const uint32_t *dq, int rn, uint32_t &d_next,
+                 int bd) {
   // Reads past the line's end (positions >= len, at most 4 n_edges + 11)
   // stay inside LfShared and never reach a result: they enter the window
   // after the last edge that could read them (the last edge is 4 apart from
@@ -256,15 +260,15 @@ DEV void lf_step(int (&w)[16], uint16_t *ln, int e, int step, int n_edges, const
              (d >> 16) & 0xff, d >> 24, bd);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    ln[(4 * e + i) * step] = (uint16_t)w[4 * K + i];
+    ln[(4 * e + i) * step] = (T)w[4 * K + i];
     w[4 * K + i] = nw[i];
   }
 }
 
-DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, const uint32_t *dq, int rn,
-                   int bd) {
+template <class T>
+DEV void lf_stream(T *buf, int line_off, int step, int len, int n_edges, const uint32_t *dq, int rn, int bd) {
   int w[16];
-  uint16_t *ln = buf + line_off;
+  T *ln = buf + line_off;
 #pragma unroll
   for (int i = 0; i < 16; i++) w[i] = ln[i * step];
   uint32_t d_next = dq[rn];
@@ -279,7 +283,7 @@ DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, 
   const int base = 4 * n_edges;
 #pragma unroll
   for (int i = 0; i < 16; i++)
-    if (base + i < len) ln[(base + i) * step] = (uint16_t)w[i];
+    if (base + i < len) ln[(base + i) * step] = (T)w[i];
 }
 
 // ---- one plane of one work item of level f.t
@@ -291,8 +295,9 @@ DEV void lf_stream(uint16_t *buf, int line_off, int step, int len, int n_edges, 
 // depend on the item alone.
 template <bool HBD>
 DEV void lf_unit(uint8_t *plane_ptr, uint32_t stride, const uint32_t *dq, uint32_t geo, uint32_t t_level, int item,
-                 int plane, bool active, int lane, LfShared &sh) {
+                 int plane, bool active, int lane, LfShared<typename std::conditional<HBD, uint16_t, uint8_t>::type> &sh) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  constexpr int BP = LfPitch<P>::BP;
   // Level t: items below n_units are the units (r, t - r) from the first
   // row the level reaches; item n_units, when t + 1 < R, is the vertical
   // pass of superblock (t + 1, 0), which no later item of row t + 1
@@ -331,6 +336,11 @@ DEV void lf_unit(uint8_t *plane_ptr, uint32_t stride, const uint32_t *dq, uint32
     ((uint4 *)sh.dq[1])[lane] = d1;
     ((uint4 *)sh.dq[2])[lane] = d2;
     lf_region_to_lds<P, 0>(it, part0, sh.buf, lane);
+    // superblock c + 1 too: the horizontal pass works on superblock c's
+    // columns only (the first vertical pass reads past its line's end into
+    // these columns but never writes them), and its registers are then free
+    // for the passes
+    if (right_v) lf_region_to_lds<P, 1>(it, part1, sh.buf, lane);
   }
   __syncthreads();
   // pass 0: the vertical edges of superblock (r, c) (c = 0 only); pass 1 its
@@ -340,11 +350,8 @@ DEV void lf_unit(uint8_t *plane_ptr, uint32_t stride, const uint32_t *dq, uint32
   for (int pass = 0; pass < 3; pass++) {
     if (pass == 0 ? !first_v : pass == 1 ? lone_v : !right_v) continue;
     if (pass == 2) {
-      // superblock c + 1 into LDS, superblock c's finished samples out
-      if (active) {
-        lf_region_to_lds<P, 1>(it, part1, sh.buf, lane);
-        lf_region_store<P, 0, 9>(pl, stride, it, sh.buf, true, lane);
-      }
+      // superblock c's finished samples out
+      if (active) lf_region_store<P, 0, 9>(pl, stride, it, sh.buf, true, lane);
       __syncthreads();
     }
     const bool vert = pass != 1;
