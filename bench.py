@@ -357,7 +357,7 @@ def end_to_end(vp, frames_bytes, want_md5, cpu_whole_all):
 
 # ---- the 4K 10-bit stream with scaled references
 
-def uhd_bench(vp, gpu, check, passes=3):
+def uhd_bench(vp, gpu, check, passes=3, graphs_too=True):
     """The 3840x2160 10-bit 4:2:0 profile-2 stream of 48 frames from seed 4
     (moving rectangles at 10-bit range, 4 tile columns, q 80; every 12th
     frame coded at 1920x1080, so its neighbours predict through a 2:1 or 1:2
@@ -396,35 +396,38 @@ def uhd_bench(vp, gpu, check, passes=3):
     # (captured in an untimed pass; graphs re-targeted when the stores of a
     # frame differ), and the MD5s of a graph pass
     graphs = {}
-    for i in range(n):
-        sched.step(i, graphs)
-    gpu.synchronize()
-    t0 = time.perf_counter()
-    for k in range(passes * n):
-        sched.step(k % n, graphs)
-    gpu.synchronize()
-    gfps = passes * n / (time.perf_counter() - t0)
-    gmd5 = []
-    for i in range(n):
-        shown = sched.step(i, graphs)
-        if shown is not None:
-            gmd5.append(hashlib.md5(shown.packed()).hexdigest())
-    nodes = sum(g[0].nodes() for g in graphs.values()) / max(len(graphs), 1)
+    gfps, gmd5, nodes = None, md5, 0
+    if graphs_too:
+        for i in range(n):
+            sched.step(i, graphs)
+        gpu.synchronize()
+        t0 = time.perf_counter()
+        for k in range(passes * n):
+            sched.step(k % n, graphs)
+        gpu.synchronize()
+        gfps = passes * n / (time.perf_counter() - t0)
+        gmd5 = []
+        for i in range(n):
+            shown = sched.step(i, graphs)
+            if shown is not None:
+                gmd5.append(hashlib.md5(shown.packed()).hexdigest())
+        nodes = sum(g[0].nodes() for g in graphs.values()) / max(len(graphs), 1)
     del graphs
     out = {"workload": "3840x2160 10-bit 4:2:0 profile 2, 48 frames, seed 4, every 12th frame at 1920x1080 "
                        "(2:1 / 1:2 scaled references), 4 tile columns, base_q_idx 80 (stream generator)",
            "frames_per_s": round(fps, 2), "ms_per_frame": round(1000.0 / fps, 4), "timed_frames": passes * n,
            "launch": "direct (vp9r_gpu_run)",
-           "graph": {"frames_per_s": round(gfps, 2), "ms_per_frame": round(1000.0 / gfps, 4),
-                     "kernel_nodes_per_frame": round(nodes, 1), "md5_equal_direct": gmd5 == md5,
-                     "launch": "one HIP graph per frame (vp9r_gpu_graph_run), captured once"},
+           "graph": None if gfps is None else {
+               "frames_per_s": round(gfps, 2), "ms_per_frame": round(1000.0 / gfps, 4),
+               "kernel_nodes_per_frame": round(nodes, 1), "md5_equal_direct": gmd5 == md5,
+               "launch": "one HIP graph per frame (vp9r_gpu_graph_run), captured once"},
            "bit_exact": None}
     if check:
         cpu, md5_one, md5_all = cpu_baselines(frames_bytes, 0.0, "the 4K 10-bit stream")
         out["bit_exact"] = md5_one == md5 and md5_all == md5 and gmd5 == md5
         out["cpu_baseline"] = cpu
         out["vs_cpu_all_core"] = round(fps / cpu["value"], 2)
-    _log(f"bench: 4K 10-bit {fps:.1f} frames/s direct, {gfps:.1f} frames/s through graphs, "
+    _log(f"bench: 4K 10-bit {fps:.1f} frames/s direct, {gfps} frames/s through graphs, "
          f"bit-exact {out['bit_exact']}")
     del sched, frames
     return out
@@ -443,7 +446,7 @@ def _micro_cpu(kind, oracle_fn, obj, sample, threads):
     return cpu, oracle_fn(obj, threads)
 
 
-def micro_bench(vp, gpu, check, reps=5):
+def micro_bench(vp, gpu, check, reps=5, tune=True):
     """Kernel microbenchmark (tests/microbench.py): 2M inverse-transform
     blocks and 1M motion-compensation blocks from seed 3, 8-bit, each batch
     one vp9r_itx_add_batch / vp9r_mc_batch call with its inputs resident in
@@ -468,12 +471,13 @@ def micro_bench(vp, gpu, check, reps=5):
                 "frac": round(gbs / 8000.0, 4), "algorithmic_bytes": nbytes}
 
     # the launch-shape autotuner first: its choices apply to the calls below
-    t0 = time.perf_counter()
-    choice, times = gpu.autotune()
-    out["autotune"] = {"choices": choice, "seconds": round(time.perf_counter() - t0, 2),
-                       "itx_us_by_class_size_waves_1_2_4": [round(t, 1) for t in times[:24]],
-                       "mc_us_by_class_rows_32_16": [round(t, 1) for t in times[24:26]]}
-    _log(f"bench: autotune {choice}")
+    if tune:
+        t0 = time.perf_counter()
+        choice, times = gpu.autotune()
+        out["autotune"] = {"choices": choice, "seconds": round(time.perf_counter() - t0, 2),
+                           "itx_us_by_class_size_waves_1_2_4": [round(t, 1) for t in times[:24]],
+                           "mc_us_by_class_rows_32_16": [round(t, 1) for t in times[24:26]]}
+        _log(f"bench: autotune {choice}")
 
     t0 = time.perf_counter()
     s = mb.ItxSet(2_000_000, 3)
@@ -832,6 +836,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-stage measurement")
     ap.add_argument("--no-uhd", action="store_true", help="skip the 4K 10-bit scaled-reference stream")
     ap.add_argument("--no-micro", action="store_true", help="skip the transform / MC kernel microbenchmark")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP graph passes")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (default shapes)")
     ap.add_argument("--micro-only", action="store_true",
                     help="only run the kernel microbenchmark and print its JSON (a profiling aid)")
     ap.add_argument("--serve-streams", type=int, default=32, help="streams of the serving workload")
@@ -862,7 +868,7 @@ def main():
         sys.path.insert(0, TESTS)
     if args.micro_only:
         g = vp.Gpu(0)
-        micro = micro_bench(vp, g, not args.no_cpu)
+        micro = micro_bench(vp, g, not args.no_cpu, tune=not args.no_tune)
         micro["lf"] = lf_bench(vp, g, not args.no_cpu)
         if micro["lf"]["bit_exact"] is False:
             micro["bit_exact"] = False
@@ -921,18 +927,20 @@ def main():
 
     # the same frames through one HIP graph per frame (captured in an
     # untimed pass), reported beside the direct launches
-    graphs = {}
-    for i in range(n):
-        sched.step(i, graphs)
-    gpu.synchronize()
-    t0 = time.perf_counter()
-    for j in range(args.steps):
-        sched.step(j % n, graphs)
-    gpu.synchronize()
-    graph_fps = args.steps / (time.perf_counter() - t0)
-    pos = args.steps  # the next frame of the chain
-    del graphs
-    _log(f"bench: through graphs {graph_fps:.1f} frames/s")
+    graph_fps = None
+    if not args.no_graph:
+        graphs = {}
+        for i in range(n):
+            sched.step(i, graphs)
+        gpu.synchronize()
+        t0 = time.perf_counter()
+        for j in range(args.steps):
+            sched.step(j % n, graphs)
+        gpu.synchronize()
+        graph_fps = args.steps / (time.perf_counter() - t0)
+        pos = args.steps  # the next frame of the chain
+        del graphs
+        _log(f"bench: through graphs {graph_fps:.1f} frames/s")
 
     # the stage breakdown: the same number of frames again, with HIP events
     # between the stages on the kernel stream (each event record holds the
@@ -1057,8 +1065,8 @@ def main():
             config1 = config1_bench(vp, True)
 
     batched = None if args.no_batched else batched_stages(vp, gpu, frames)
-    uhd = None if args.no_uhd else uhd_bench(vp, gpu, not args.no_cpu)
-    micro = None if args.no_micro else micro_bench(vp, gpu, not args.no_cpu)
+    uhd = None if args.no_uhd else uhd_bench(vp, gpu, not args.no_cpu, graphs_too=not args.no_graph)
+    micro = None if args.no_micro else micro_bench(vp, gpu, not args.no_cpu, tune=not args.no_tune)
     if micro is not None:
         micro["lf"] = lf_bench(vp, gpu, not args.no_cpu)
         if micro["lf"]["bit_exact"] is False:
@@ -1089,7 +1097,7 @@ def main():
         "bit_exact": bit_exact,
         "stages": stages,
         "stage_pass_fps": round(staged_fps, 2),
-        "graph_fps": round(graph_fps, 2),
+        "graph_fps": None if graph_fps is None else round(graph_fps, 2),
         "parse_fps_one_core": round(n / t_parse, 2),
     }
     out.update(extras)
