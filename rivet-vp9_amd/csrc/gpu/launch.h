@@ -42,17 +42,30 @@ struct Range {
   uint32_t wg, first, count, _pad;
 };
 
-// The frame of workgroup w (r sorted by wg; n small).
+// The frame of workgroup w: the last i < n with r[i].wg <= w (r sorted by
+// wg).  Each lane of the wave reads one entry and a ballot counts those at
+// or below w, 64 entries per memory round trip (a binary search would wait
+// for one round trip per halving).  Called by whole waves in uniform control
+// flow.
 __device__ __forceinline__ uint32_t range_frame(const Range *r, uint32_t n, uint32_t w) {
-  uint32_t lo = 0, hi = n;  // r[lo].wg <= w < r[hi].wg
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (r[mid].wg <= w)
-      lo = mid;
-    else
-      hi = mid;
+  if (n <= 1) return 0;
+  const uint32_t lane = __lane_id();
+  for (uint32_t base = 0;; base += 64) {
+    const uint32_t i = base + lane;
+    const uint64_t le = __ballot(i < n && r[i].wg <= w);
+    const uint32_t c = (uint32_t)__popcll(le);
+    if (c < 64 || base + 64 >= n) return base + c > 0 ? base + c - 1 : 0;
   }
-  return lo;
+}
+
+// Workgroup b of a grid of g, renumbered so that each XCD takes a
+// contiguous run of the work: the hardware hands workgroups to the 8 XCDs in
+// turn (b % 8), each XCD has its own L2, and neighbouring tiles share
+// reference lines and output lines.  XCD x's workgroups take the x-th
+// eighth, in order (a bijection on [0, g)).
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t g) {
+  const uint32_t q = g >> 3, r = g & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
 }
 
 struct ClearArgs {
@@ -170,9 +183,9 @@ struct LfParams {
 };
 
 // k_lf_mask's ranges cover 6 entries (plane, pass) per superblock; k_lf's
-// ranges have first = the level t and count = 3 x its items (item j: plane
-// j % 3 of the level's (j / 3)-th unit (r, t - r), rows ascending, then the
-// lone vertical pass of superblock (t + 1, 0)).
+// ranges have first = the level t and count = 3 x its n items (workgroup j:
+// plane j / n of the level's (j % n)-th item: the units (r, t - r), rows
+// ascending, then the lone vertical pass of superblock (t + 1, 0)).
 struct LfArgs {
   FrameGeom g;
   const vp9r_mi *mi;

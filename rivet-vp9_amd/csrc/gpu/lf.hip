@@ -126,20 +126,21 @@ __global__ __launch_bounds__(64) void k_lf(uint8_t *p0, uint8_t *p1, uint8_t *p2
   // a launch of one frame (ONE) passes what the workgroups need as
   // arguments (preloaded, so the region loads need no load before them, and
   // known to be global memory: LfOne); a batch finds each frame's through
-  // the range table.  Workgroup j: plane j % 3 of item j / 3.
-  int j = (int)blockIdx.x;
+  // the range table.  A frame's 3 n workgroups are the luma planes of its n
+  // items, then the two chroma planes: the workgroups that take longest are
+  // dispatched first.
   if (!ONE) {
     const uint32_t fi = range_frame(rg, nf, blockIdx.x);
     const LfArgs &a = fa[fi];
-    j = (int)(blockIdx.x - rg[fi].wg);
-    const int plane = j % 3;
+    const int j = (int)(blockIdx.x - rg[fi].wg), items = (int)(rg[fi].count / 3);
+    const int plane = j / items;
     const LfOne o = lf_one(a, rg[fi].first);
-    lf_unit<HBD>((uint8_t *)a.cur.p[plane], a.cur.stride[plane], o.dq, o.geo, o.t, j / 3, plane, true,
+    lf_unit<HBD>((uint8_t *)a.cur.p[plane], a.cur.stride[plane], o.dq, o.geo, o.t, j - plane * items, plane, true,
                  (int)threadIdx.x, sh);
     return;
   }
-  const int plane = j % 3;
-  lf_unit<HBD>(plane == 0 ? p0 : plane == 1 ? p1 : p2, plane == 0 ? s0 : plane == 1 ? s1 : s2, dq, geo, t1, j / 3,
+  const int items = (int)(gridDim.x / 3), plane = (int)blockIdx.x / items, item = (int)blockIdx.x - plane * items;
+  lf_unit<HBD>(plane == 0 ? p0 : plane == 1 ? p1 : p2, plane == 0 ? s0 : plane == 1 ? s1 : s2, dq, geo, t1, item,
                plane, true, (int)threadIdx.x, sh);
 }
 

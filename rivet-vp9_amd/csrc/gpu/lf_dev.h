@@ -206,22 +206,29 @@ DEV void lf_region_to_lds(const LfItem &it, const LfRegion<P, N> &rg, P *buf, in
 }
 
 // Stores the vectors of a part; skip_last leaves out the last vector of
-// each superblock row (part 0 before the second vertical pass).
+// each superblock row (part 0 before the second vertical pass).  Every LDS
+// read is issued before the first store (an LDS address stays inside the
+// buffer for a vector the part does not have), so the reads' latency is paid
+// once.
 template <class P, int PART, int N>
 DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const P *buf, bool skip_last, int lane) {
   constexpr int BP = LfPitch<P>::BP;
+  typedef typename LfRegion<P, N>::V V;
+  V v[N];
+  int rr[N], qq[N];
+  bool ok[N];
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    int r, q;
-    if (!lf_vec<PART>(it, lane + 64 * i, r, q)) continue;
-    if (skip_last && r >= 8 && q == (1 << it.lnt) - 1) continue;
-    const uint32_t *l = (const uint32_t *)(buf + r * BP + 8 + 8 * q);
-    P *g = pl + (size_t)(it.y0 - 8 + r) * stride + it.x0 + 8 * q;
+    ok[i] = lf_vec<PART>(it, lane + 64 * i, rr[i], qq[i]) && !(skip_last && rr[i] >= 8 && qq[i] == (1 << it.lnt) - 1);
+    const uint32_t *l = (const uint32_t *)(buf + (ok[i] ? rr[i] * BP + 8 + 8 * qq[i] : 8));
     if (sizeof(P) == 1)
-      gst((uint2 *)g, make_uint2(l[0], l[1]));
+      *(uint2 *)&v[i] = make_uint2(l[0], l[1]);
     else
-      gst((uint4 *)g, make_uint4(l[0], l[1], l[2], l[3]));
+      *(uint4 *)&v[i] = make_uint4(l[0], l[1], l[2], l[3]);
   }
+#pragma unroll
+  for (int i = 0; i < N; i++)
+    if (ok[i]) gst((V *)(pl + (size_t)(it.y0 - 8 + rr[i]) * stride + it.x0 + 8 * qq[i]), v[i]);
 }
 
 // ---- the edge filters of one pass, streamed through a register window

@@ -400,9 +400,10 @@ template <bool HBD>
 __global__ __launch_bounds__(64) void k_mc(const McArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ McLds sh;
-  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+  const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
+  const uint32_t fi = range_frame(rg, nf, wg);
   const McArgs &a = fa[fi];
-  const McTile t = a.tiles[rg[fi].first + blockIdx.x - rg[fi].wg];
+  const McTile t = a.tiles[rg[fi].first + wg - rg[fi].wg];
   const vp9r_block &b = a.blocks[t.block];
   const int max = (1 << a.g.bit_depth) - 1;
   const bool compound = b.ref_frame[1] > INTRA_FRAME;
@@ -521,14 +522,15 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
   constexpr int NH = TH / 16;
   __shared__ McLds sh;
   DescTile t;
+  const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
   if (EXPAND) {
-    t.desc = blockIdx.x >> 3;
-    t.tx = (uint16_t)(16 * (blockIdx.x & 3));
-    t.ty = (uint16_t)(32 * ((blockIdx.x >> 2) & 1));
+    t.desc = wg >> 3;
+    t.tx = (uint16_t)(16 * (wg & 3));
+    t.ty = (uint16_t)(32 * ((wg >> 2) & 1));
     const vp9r_mc_desc dd = descs[t.desc];
     if (t.tx >= dd.w || t.ty >= dd.h) return;
   } else {
-    t = tiles[blockIdx.x];
+    t = tiles[wg];
   }
   const vp9r_mc_desc d = descs[t.desc];
   // each reference its own plane (inter.rs:21-27 RefPlane)
