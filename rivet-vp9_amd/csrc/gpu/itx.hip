@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64 * W) void k_itx_dense(const int32_t *coefs, cons
   constexpr int n0 = 1 << n, BPW = 64 * W / n0, S = n0 + 1;
   __shared__ T tiles[BPW][n0 * S];
   const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
-  const uint32_t job = blockIdx.x * BPW + team;
+  const uint32_t job = xcd_order(blockIdx.x, gridDim.x) * BPW + team;
   const bool active = job < n_jobs;
   T *tile = tiles[team];
   const uint32_t i = active ? jobs[job] : 0;
