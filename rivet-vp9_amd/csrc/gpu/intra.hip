@@ -398,10 +398,16 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   // planes and transform block records (kernel arguments: known to be
   // global memory)
   const IntraBlockItem it = ONE ? items0[wg] : a.items[rg[fi].first + wg - rg[fi].wg];
-  auto cur_plane = [&](int p) -> uint8_t * {
-    return ONE ? (p == 0 ? cp0 : p == 1 ? cp1 : cp2) : (uint8_t *)a.cur.p[p];
-  };
-  auto res_plane = [&](int p) -> const int16_t * { return ONE ? (p == 0 ? rp0 : p == 1 ? rp1 : rp2) : a.res[p]; };
+  // the planes, residual planes and strides as scalars, read once: the
+  // staging loops pick a plane per lane, and a per-lane read of the frame's
+  // arguments would hold up every load behind it
+  uint8_t *const pl_[3] = {ONE ? cp0 : (uint8_t *)a.cur.p[0], ONE ? cp1 : (uint8_t *)a.cur.p[1],
+                           ONE ? cp2 : (uint8_t *)a.cur.p[2]};
+  const int16_t *const rs_[3] = {ONE ? rp0 : a.res[0], ONE ? rp1 : a.res[1], ONE ? rp2 : a.res[2]};
+  const uint32_t st_[3] = {a.cur.stride[0], a.cur.stride[1], a.cur.stride[2]};
+  auto cur_plane = [&](int p) -> uint8_t * { return p == 0 ? pl_[0] : p == 1 ? pl_[1] : pl_[2]; };
+  auto res_plane = [&](int p) -> const int16_t * { return p == 0 ? rs_[0] : p == 1 ? rs_[1] : rs_[2]; };
+  auto stride_of = [&](int p) -> size_t { return p == 0 ? st_[0] : p == 1 ? st_[1] : st_[2]; };
   const vp9r_txb *txbs = ONE ? txbs1 : a.txbs;
   const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
   vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
@@ -438,18 +444,21 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
       const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
       if (local >= R.n) continue;
       const P *pl = (const P *)cur_plane(p);
-      const size_t stride = a.cur.stride[p];
+      const size_t stride = stride_of(p);
       if (local < R.nvec) {
         const int r = local >> R.lq, q = local & ((1 << R.lq) - 1), py = R.y0 - 1 + r;
         if (py < 0) continue;
-        const P *src = pl + (size_t)py * stride + R.x0 + q * R.V;
+        // one dword load per 4 bytes, each straight into its own component
+        // (loads of different widths into one vector would be merged with
+        // a wait for each)
+        const uint32_t *src = (const uint32_t *)(pl + (size_t)py * stride + R.x0 + q * R.V);
         const int nb = R.V * (int)sizeof(P);
-        if (nb == 16)
-          v[i] = *(const uint4 *)src;
-        else if (nb == 8)
-          v[i] = make_uint4(((const uint2 *)src)->x, ((const uint2 *)src)->y, 0, 0);
-        else
-          v[i] = make_uint4(*(const uint32_t *)src, 0, 0, 0);
+        v[i].x = src[0];
+        if (nb >= 8) v[i].y = src[1];
+        if (nb == 16) {
+          v[i].z = src[2];
+          v[i].w = src[3];
+        }
       } else {
         const int r = local - R.nvec, py = R.y0 - 1 + r;
         if (py < 0 || R.x0 == 0) continue;
@@ -467,11 +476,13 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
       if (local >= (p == 0 ? nr0 : p == 1 ? nr1 : nr2)) continue;
       const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int r = local >> R.lq, q = local & ((1 << R.lq) - 1);
-      const int16_t *src = res_plane(p) + (size_t)(R.y0 + r) * a.cur.stride[p] + R.x0 + q * R.V;
-      if (R.V == 8)
-        rv[i] = *(const uint4 *)src;
-      else
-        rv[i] = make_uint4(((const uint2 *)src)->x, ((const uint2 *)src)->y, 0, 0);
+      const uint32_t *src = (const uint32_t *)(res_plane(p) + (size_t)(R.y0 + r) * stride_of(p) + R.x0 + q * R.V);
+      rv[i].x = src[0];
+      rv[i].y = src[1];
+      if (R.V == 8) {
+        rv[i].z = src[2];
+        rv[i].w = src[3];
+      }
     }
 #pragma unroll
     for (int i = 0; i < kIbResSlots; i++) {
@@ -634,8 +645,8 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
     if (!R.n) continue;
     const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
     const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
-    const size_t stride = a.cur.stride[p];
-    P *pl = (P *)cur_plane(p);
+    const size_t stride = st_[p];
+    P *pl = (P *)pl_[p];
     for (int k = tid; k < (R.BH << R.lq); k += kIbThreads) {
       const int r = k >> R.lq, q = k & ((1 << R.lq) - 1);
       const uint16_t *src = tile + (r + 1) * tp + 1 + q * R.V;
