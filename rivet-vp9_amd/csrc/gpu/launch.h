@@ -182,7 +182,8 @@ struct LfParams {
   uint8_t limit[64], blimit[64], thresh[64];
 };
 
-// k_lf_mask's ranges cover 6 entries (plane, pass) per superblock; k_lf's
+// k_lf_mask's ranges cover 6 workgroups per superblock, or one (large
+// batches); k_lf's
 // ranges have first = the level t and count = 3 x its n items (workgroup j:
 // plane j / n of the level's (j % n)-th item: the units (r, t - r), rows
 // ascending, then the lone vertical pass of superblock (t + 1, 0)).
@@ -246,7 +247,9 @@ struct WaveLaunch {
 void launch_wave(hipStream_t s, const WaveLaunch &w, bool hbd);
 // dynamic LDS of a k_wave workgroup for a frame of subsampling (ss_x, ss_y)
 uint32_t wave_lds_bytes(int ss_x, int ss_y);
-void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs);
+// per_sb: 1 (a workgroup per superblock entry, 6 per superblock) or 6 (a
+// workgroup per superblock); the ranges count workgroups accordingly
+void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int per_sb);
 // one: for a launch of one frame, lf_one of its arguments and the level
 // (else p[0] null: each workgroup finds its frame through the range table)
 void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,

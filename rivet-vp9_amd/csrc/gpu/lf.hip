@@ -100,20 +100,28 @@ DEV void lf_decide(const LfArgs &a, int plane, int pass, uint32_t sb_row, uint32
   lvl_out = l;
 }
 
-// Per frame, workgroup j of 6 * n_sb: superblock j % n_sb, pass and plane
-// from j / n_sb.
+// Per frame, K = 1: workgroup j of 6 n_sb takes entry (plane, pass) = j /
+// n_sb of superblock j % n_sb; K = 6: workgroup j of n_sb takes the six
+// entries of superblock j in turn (for large batches, where a workgroup per
+// entry would spend more on dispatch than on the decisions).
+template <int K>
 __global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *rg, uint32_t nf) {
   const uint32_t fi = range_frame(rg, nf, blockIdx.x);
   const LfArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   const int n_sb = g.sb64_rows * g.sb64_cols;
-  const int j = blockIdx.x - rg[fi].wg, sb = j % n_sb, pp = j / n_sb, pass = pp & 1, plane = pp >> 1;
-  int fs, lvl;
-  lf_decide(a, plane, pass, sb / g.sb64_cols, sb % g.sb64_cols, threadIdx.x >> 4, threadIdx.x & 15, fs, lvl);
-  const size_t o = ((size_t)(plane * 2 + pass) * n_sb + sb) * 256 + threadIdx.x;
-  a.dq[o] = fs < 0 ? 0u
-                   : (uint32_t)(fs + 1) | (uint32_t)a.p.limit[lvl] << 8 | (uint32_t)a.p.blimit[lvl] << 16 |
-                         (uint32_t)a.p.thresh[lvl] << 24;
+  const int j = blockIdx.x - rg[fi].wg, sb = K == 1 ? j % n_sb : j;
+  const uint32_t sb_row = sb / g.sb64_cols, sb_col = sb % g.sb64_cols;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int pp = K == 1 ? j / n_sb : k, pass = pp & 1, plane = pp >> 1;
+    int fs, lvl;
+    lf_decide(a, plane, pass, sb_row, sb_col, threadIdx.x >> 4, threadIdx.x & 15, fs, lvl);
+    const size_t o = ((size_t)pp * n_sb + sb) * 256 + threadIdx.x;
+    a.dq[o] = fs < 0 ? 0u
+                     : (uint32_t)(fs + 1) | (uint32_t)a.p.limit[lvl] << 8 | (uint32_t)a.p.blimit[lvl] << 16 |
+                           (uint32_t)a.p.thresh[lvl] << 24;
+  }
 }
 
 // ---- one wavefront level per launch (decisions from k_lf_mask)
@@ -144,9 +152,12 @@ __global__ __launch_bounds__(64) void k_lf(uint8_t *p0, uint8_t *p1, uint8_t *p2
                plane, true, (int)threadIdx.x, sh);
 }
 
-void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs) {
+void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int per_sb) {
   if (!wgs) return;
-  hipLaunchKernelGGL(k_lf_mask, dim3(wgs), dim3(256), 0, s, fa, r, n);
+  if (per_sb == 6)
+    hipLaunchKernelGGL(k_lf_mask<6>, dim3(wgs), dim3(256), 0, s, fa, r, n);
+  else
+    hipLaunchKernelGGL(k_lf_mask<1>, dim3(wgs), dim3(256), 0, s, fa, r, n);
 }
 
 void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,

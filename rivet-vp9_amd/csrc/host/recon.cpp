@@ -942,6 +942,11 @@ static int check_frames(const Prepared &p, vp9r_gpu_frame *const refs[3], vp9r_g
 
 namespace {
 
+// Superblocks in a batch from which the loop filter decisions take one
+// workgroup per superblock (four 1080p frames): below it, a workgroup per
+// superblock entry finishes sooner.
+constexpr size_t kMaskPerSbMin = 2040;
+
 template <class T>
 size_t stage_take(std::vector<uint8_t> &buf, size_t n) {
   size_t o = (buf.size() + 255) & ~(size_t)255;
@@ -1048,6 +1053,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       return add_ranges(st, n, first.data(), count.data(), per_wg);
     };
     LaunchTable t_clear, t_mc, t_itx[4], t_mask;
+    int mask_per_sb = 1;
     std::vector<LaunchTable> t_intra, t_lf;
     bool groups = false;
     const bool fused = g->intra_blocks;
@@ -1121,10 +1127,15 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       }
     }
     if (lf) {
+      // decisions: a workgroup per superblock entry, or per superblock once
+      // the batch is large enough to keep the chip busy that way
+      size_t batch_sb = 0;
+      for (int i = 0; i < n; i++) batch_sb += (size_t)f[i]->p->hdr.sb64_rows * f[i]->p->hdr.sb64_cols;
+      mask_per_sb = batch_sb >= kMaskPerSbMin ? 6 : 1;
       t_mask = table([](int) { return 0u; },
                      [&](int i) {
                        const auto &h = f[i]->p->hdr;
-                       return f[i]->p->lf_level_start.empty() ? 0u : 6 * h.sb64_rows * h.sb64_cols;
+                       return f[i]->p->lf_level_start.empty() ? 0u : (6 / mask_per_sb) * h.sb64_rows * h.sb64_cols;
                      },
                      1);
       size_t levels = 0;
@@ -1266,7 +1277,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       // everything on the kernel stream: the decisions, then the fused levels
       if (!recon && !gt)
         for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
-      if (lf) vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
+      if (lf) vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs, mask_per_sb);
       if (recon && g->timing && !n_wave_intra) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
       uint32_t lds = 0;
       for (int i = 0; i < n; i++) lds = std::max(lds, vp9g::wave_lds_bytes(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y));
@@ -1309,7 +1320,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       // ev_pre follows, on the kernel stream, the batch tables, the frames'
       // records (uploaded on that stream) and the inter stages
       HIP_TRY(hipStreamWaitEvent(sl, g->ev_pre, 0), "hipStreamWaitEvent");
-      vp9g::launch_lf_mask(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs);
+      vp9g::launch_lf_mask(sl, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs, mask_per_sb);
       int waited = -1;
       for (size_t l = 0; l < t_lf.size(); l++) {
         int need = -1;

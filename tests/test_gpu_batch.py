@@ -151,3 +151,30 @@ def test_filter_batch_equals_the_oracle(gpu):
             gpu.filter_batch(jobs_lf)
     finally:
         gpu.set_timing(False)
+
+
+def test_filter_batch_large_equals_the_oracle(gpu):
+    """A batch of 2040 superblocks or more (four 1080p frames) takes the
+    loop filter decisions one workgroup per superblock: the same frames as
+    the oracle's decode."""
+    frames = streamgen.generate(width=1920, height=1080, frames=2, seed=34, lf_level=36, lf_sharpness=3)
+    ref = oracle_lib.CpuDecoder()
+    md5s = [oracle_lib.md5(o.packed()) for f in frames for o in ref.decode(f)]
+    p = vp.Parser()
+    rec0 = p.parse(frames[0])
+    h = rec0.hdr
+    assert h.sb64_rows * h.sb64_cols * 4 >= 2040
+    f0 = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+    gpu.run(gpu.prepare(rec0), [None] * 3, f0)
+    rec1 = p.parse(frames[1])
+    p_lf = gpu.prepare(rec1)
+    rec1.hdr.lf_level = 0
+    p_off = gpu.prepare(rec1)
+    jobs = []
+    for _ in range(4):
+        cur = gpu.frame(h.width, h.height, h.bit_depth, h.ss_x, h.ss_y)
+        gpu.run(p_off, [f0] * 3, cur)
+        jobs.append((p_lf, cur))
+    gpu.synchronize()
+    gpu.filter_batch(jobs)
+    assert [oracle_lib.md5(c.packed()) for _, c in jobs] == [md5s[1]] * 4
