@@ -286,7 +286,8 @@ __global__ __launch_bounds__(64) void k_intra(const IntraArgs *fa, const Range *
   }
 }
 
-// ---- intra blocks (k_intra_blocks): one workgroup of 256 lanes per block
+// ---- intra blocks (k_intra_blocks): one workgroup of kIbThreads (512) lanes
+// per block
 //
 // The block's three planes are independent (each reads only its own
 // plane), so they are processed together.  Each plane's region (the block
@@ -429,10 +430,10 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   // area are not reconstructed and keep the frame's).  All loads of the
   // three planes are in flight before the first LDS write: at most
   // kIbRegionSlots items per lane (4:4:4 64x64: 3 x (65 x 8 + 65) items).
-  constexpr int kIbRegionSlots = 7;
+  constexpr int kIbRegionSlots = (3 * (65 * 8 + 65) + kIbThreads - 1) / kIbThreads;
   const IbRegion rg0 = ib_region(it, g, 0, bw8, bh8, np0 > 0), rg1 = ib_region(it, g, 1, bw8, bh8, np1 > 0),
                  rg2 = ib_region(it, g, 2, bw8, bh8, np2 > 0);
-  constexpr int kIbResSlots = 6;
+  constexpr int kIbResSlots = (3 * 512 + kIbThreads - 1) / kIbThreads;
   const int nr0 = rg0.n ? rg0.BH << rg0.lq : 0, nr1 = rg1.n ? rg1.BH << rg1.lq : 0, nr2 = rg2.n ? rg2.BH << rg2.lq : 0;
   {
     uint4 v[kIbRegionSlots], rv[kIbResSlots];

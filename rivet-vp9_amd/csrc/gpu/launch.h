@@ -131,7 +131,7 @@ struct IntraBlockItem {
 // cell), DC values, plane geometry, and per plane the region tile ((64 >> sx) + 1 columns, u16) and
 // residuals ((64 >> sx) columns, int16).  The launch passes `bytes` (the
 // largest of its frames') as dynamic LDS.
-constexpr int kIbThreads = 256, kIbGeoBytes = 80;
+constexpr int kIbThreads = 512, kIbGeoBytes = 80;
 // The most transform blocks an intra block has: 64x64 at 4x4 in every plane.
 __host__ __device__ inline uint32_t ib_max_txbs(int ss_x, int ss_y) { return 256 + 2 * (256u >> (ss_x + ss_y)); }
 struct IbLayout {
