@@ -241,9 +241,17 @@ def profile_traffic(kernel, profile="recon_1080p_moving_rects.md"):
         body = tables.get((kind, counter))
         if body is None:
             return None
+        # the counter table only (a section may also hold a kernel trace
+        # table, whose lines start with the same names)
+        in_pmc = False
         for line in body.splitlines():
+            if line.startswith("pmc"):
+                in_pmc = True
+                continue
+            if not line.startswith(" "):
+                in_pmc = False
             line = line.strip()
-            if line.startswith(name) and line[len(name):][:1].isspace():
+            if in_pmc and line.startswith(name) and line[len(name):][:1].isspace():
                 return float(line.split()[-1])
         return None
     copy = re.search(r"a (\d+) MiB device-to-device copy", text)

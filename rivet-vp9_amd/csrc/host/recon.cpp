@@ -943,8 +943,10 @@ static int check_frames(const Prepared &p, vp9r_gpu_frame *const refs[3], vp9r_g
 namespace {
 
 // Superblocks in a batch from which the loop filter decisions take one
-// workgroup per superblock (four 1080p frames): below it, a workgroup per
-// superblock entry finishes sooner.
+// workgroup per superblock (four 1080p frames, or one 4K frame: 15.9
+// against 18.7 us; 64 1080p frames: 201 against 295 us): below it, a
+// workgroup per superblock entry finishes sooner (one 1080p frame: 7.5
+// against 9.8 us).
 constexpr size_t kMaskPerSbMin = 2040;
 
 template <class T>
