@@ -25,12 +25,20 @@ struct FrameGeom {
 
 // An inter prediction tile: a tw x th piece (at most 16 x 16) at offset
 // (tx, ty) of one prediction unit (a block's plane, or one 4x4 sub-block)
-// of size w x h at plane position (x, y).
+// of size w x h at plane position (x, y), with what its reference position
+// needs of its block, so that a workgroup reads its tile and then the
+// references: the unit's motion vector per reference list (selected, and
+// averaged for chroma sub-blocks, 8.5.2.1), the block's position, size,
+// reference frames and filter.
 struct McTile {
+  int32_t mv[2][2];
   uint32_t block;
   uint16_t x, y;
+  uint16_t mi_row, mi_col;
   uint8_t plane, sub, w, h;
   uint8_t tx, ty, tw, th;
+  uint8_t mi_size, interp_filter;
+  int8_t ref_frame[2];
 };
 
 // Cross-frame batching.  One launch covers the work items of several
@@ -223,7 +231,10 @@ __host__ __device__ inline LfOne lf_one(const LfArgs &a, uint32_t t) {
 // device memory; wgs = r[n].wg; hbd: samples are u16.
 uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g);
 void launch_clear_margins(hipStream_t s, const ClearArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
-void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
+// one_tiles: for a launch of one frame, its tiles (device memory), read
+// straight from a kernel argument
+void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
+               const McTile *one_tiles);
 // r[k], wgs[k]: the ranges and workgroups of transform size k (4x4 .. 32x32)
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4], uint32_t n, const uint32_t wgs[4],
                       bool hbd);

@@ -396,63 +396,119 @@ DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, 
   __syncthreads();
 }
 
-template <bool HBD>
-__global__ __launch_bounds__(64) void k_mc(const McArgs *fa, const Range *rg, uint32_t nf) {
+template <int TH>
+DEV Mc8Shared<TH> &mc8_lds(McLds &sh) {
+  if constexpr (TH == 32)
+    return sh.f32;
+  else
+    return sh.f16;
+}
+template <int TH>
+DEV Mc16Shared<TH> &mc16_lds(McLds &sh) {
+  if constexpr (TH == 32)
+    return sh.h32;
+  else
+    return sh.h16;
+}
+
+// The frame kernel's unscaled fast paths on a tile of TH rows (16, or 32
+// for the tiles the host makes tall: 8-bit or high bit depth, both
+// references unscaled), the same arithmetic as the per-kernel seam's.
+template <int TH>
+DEV void mc_frame8(const McArgs &a, const McTile &t, const RefPos &rp0, const RefPos &rp1, int r0, int r1,
+                   bool compound, int filter, uint8_t *dst, uint32_t stride, McLds &sh) {
+  constexpr int NH = TH / 16;
+  uint32_t v[NH], v2[NH];
+  mc_tile8<TH>((const uint8_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
+               rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, filter, mc8_lds<TH>(sh), v);
+  if (compound) {
+    mc_tile8<TH>((const uint8_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
+                 rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, filter, mc8_lds<TH>(sh), v2);
+#pragma unroll
+    for (int h = 0; h < NH; h++) v[h] = avg4(v[h], v2[h]);
+  }
+  const int q = threadIdx.x & 3;
+#pragma unroll
+  for (int h = 0; h < NH; h++) {
+    const int r = 16 * h + (threadIdx.x >> 2);
+    if (r < t.th && 4 * q < t.tw)
+      gst((uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q), v[h]);
+  }
+}
+
+template <int TH>
+DEV void mc_frame16(const McArgs &a, const McTile &t, const RefPos &rp0, const RefPos &rp1, int r0, int r1,
+                    bool compound, int filter, int max, uint16_t *dst, uint32_t stride, McLds &sh) {
+  constexpr int NH = TH / 16;
+  uint32_t v[NH][2], v2[NH][2];
+  mc_tile16<TH>((const uint16_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
+                rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, filter, max, mc16_lds<TH>(sh), v);
+  if (compound) {
+    mc_tile16<TH>((const uint16_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
+                  rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, filter, max, mc16_lds<TH>(sh), v2);
+#pragma unroll
+    for (int h = 0; h < NH; h++) {
+      v[h][0] = avg2(v[h][0], v2[h][0]);
+      v[h][1] = avg2(v[h][1], v2[h][1]);
+    }
+  }
+  const int q = threadIdx.x & 3;
+#pragma unroll
+  for (int h = 0; h < NH; h++) {
+    const int r = 16 * h + (threadIdx.x >> 2);
+    if (r < t.th && 4 * q < t.tw) {
+      uint32_t *o = (uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q);
+      gst(o, v[h][0]);
+      gst(o + 1, v[h][1]);
+    }
+  }
+}
+
+// A launch of one frame (ONE) reads its tile straight from the tiles
+// argument, beside the frame's arguments; a batch finds its frame and tile
+// through the range table.
+template <bool HBD, bool ONE>
+__global__ __launch_bounds__(64) void k_mc(const McTile *tiles1, const McArgs *fa, const Range *rg, uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   __shared__ McLds sh;
   const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
-  const uint32_t fi = range_frame(rg, nf, wg);
+  const uint32_t fi = ONE ? 0 : range_frame(rg, nf, wg);
   const McArgs &a = fa[fi];
-  const McTile t = a.tiles[rg[fi].first + wg - rg[fi].wg];
-  const vp9r_block &b = a.blocks[t.block];
+  const McTile t = ONE ? tiles1[wg] : a.tiles[rg[fi].first + wg - rg[fi].wg];
   const int max = (1 << a.g.bit_depth) - 1;
-  const bool compound = b.ref_frame[1] > INTRA_FRAME;
+  const bool compound = t.ref_frame[1] > INTRA_FRAME;
   P *dst = (P *)a.cur.p[t.plane];
   const uint32_t stride = a.cur.stride[t.plane];
-  const RefPos rp0 = mc_position(a.g, a.ref_w, a.ref_h, b, t.plane, t.x, t.y, t.sub, 0);
-  const RefPos rp1 = compound ? mc_position(a.g, a.ref_w, a.ref_h, b, t.plane, t.x, t.y, t.sub, 1) : rp0;
-  const int r0 = b.ref_frame[0] - LAST_FRAME, r1 = compound ? b.ref_frame[1] - LAST_FRAME : r0;
+  const RefPos rp0 = mc_position_mv(a.g, a.ref_w, a.ref_h, t.mv[0][0], t.mv[0][1], t.mi_row, t.mi_col, t.mi_size,
+                                    t.ref_frame[0], t.plane, t.x, t.y);
+  const RefPos rp1 = compound ? mc_position_mv(a.g, a.ref_w, a.ref_h, t.mv[1][0], t.mv[1][1], t.mi_row, t.mi_col,
+                                               t.mi_size, t.ref_frame[1], t.plane, t.x, t.y)
+                              : rp0;
+  const int r0 = t.ref_frame[0] - LAST_FRAME, r1 = compound ? t.ref_frame[1] - LAST_FRAME : r0;
+  const int filter = t.interp_filter;
   if (!HBD && ((t.tw | t.th | stride | (t.x + t.tx)) & 3) == 0 && rp0.xs == 16 && rp0.ys == 16 &&
       rp1.xs == 16 && rp1.ys == 16) {
-    uint32_t v[1], v2[1];
-    mc_tile8<16>((const uint8_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
-                 rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f16, v);
-    if (compound) {
-      mc_tile8<16>((const uint8_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
-                   rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, b.interp_filter, sh.f16, v2);
-      v[0] = avg4(v[0], v2[0]);
-    }
-    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-    if (r < t.th && 4 * q < t.tw)
-      gst((uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q), v[0]);
+    if (t.th > 16)
+      mc_frame8<32>(a, t, rp0, rp1, r0, r1, compound, filter, (uint8_t *)dst, stride, sh);
+    else
+      mc_frame8<16>(a, t, rp0, rp1, r0, r1, compound, filter, (uint8_t *)dst, stride, sh);
     return;
   }
   if (HBD && ((t.tw | t.th | (t.x + t.tx)) & 3) == 0 && (stride & 1) == 0 && ((uintptr_t)dst & 3) == 0 &&
       rp0.xs == 16 && rp0.ys == 16 && rp1.xs == 16 && rp1.ys == 16) {
-    uint32_t v[1][2], v2[1][2];
-    mc_tile16<16>((const uint16_t *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
-                  rp0.x + 16 * t.tx, rp0.y + 16 * t.ty, t.tw, t.th, b.interp_filter, max, sh.h16, v);
-    if (compound) {
-      mc_tile16<16>((const uint16_t *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
-                    rp1.x + 16 * t.tx, rp1.y + 16 * t.ty, t.tw, t.th, b.interp_filter, max, sh.h16, v2);
-      v[0][0] = avg2(v[0][0], v2[0][0]);
-      v[0][1] = avg2(v[0][1], v2[0][1]);
-    }
-    const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-    if (r < t.th && 4 * q < t.tw) {
-      uint32_t *o = (uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q);
-      gst(o, v[0][0]);
-      gst(o + 1, v[0][1]);
-    }
+    if (t.th > 16)
+      mc_frame16<32>(a, t, rp0, rp1, r0, r1, compound, filter, max, (uint16_t *)dst, stride, sh);
+    else
+      mc_frame16<16>(a, t, rp0, rp1, r0, r1, compound, filter, max, (uint16_t *)dst, stride, sh);
     return;
   }
   int out[2][4];
   mc_tile<P>((const P *)a.ref[r0].p[t.plane], a.ref[r0].stride[t.plane], rp0.last_x, rp0.last_y,
-             rp0.x + rp0.xs * t.tx, rp0.y + rp0.ys * t.ty, rp0.xs, rp0.ys, t.tw, t.th, b.interp_filter, max, sh.g,
+             rp0.x + rp0.xs * t.tx, rp0.y + rp0.ys * t.ty, rp0.xs, rp0.ys, t.tw, t.th, filter, max, sh.g,
              out[0]);
   if (compound)
     mc_tile<P>((const P *)a.ref[r1].p[t.plane], a.ref[r1].stride[t.plane], rp1.last_x, rp1.last_y,
-               rp1.x + rp1.xs * t.tx, rp1.y + rp1.ys * t.ty, rp1.xs, rp1.ys, t.tw, t.th, b.interp_filter, max,
+               rp1.x + rp1.xs * t.tx, rp1.y + rp1.ys * t.ty, rp1.xs, rp1.ys, t.tw, t.th, filter, max,
                sh.g, out[1]);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -499,21 +555,6 @@ DEV void mc_desc_half(const vp9r_mc_desc &d, const vp9r_ref_plane &r0, const vp9
 // columns by 32 rows: column (k & 3), row half (k >> 2)), and tiles outside
 // the block exit at once (the whole workgroup, before any barrier).  TH:
 // the tile's rows, 16 or 32 (the autotuner's choice per bit depth class).
-template <int TH>
-DEV Mc8Shared<TH> &mc8_lds(McLds &sh) {
-  if constexpr (TH == 32)
-    return sh.f32;
-  else
-    return sh.f16;
-}
-template <int TH>
-DEV Mc16Shared<TH> &mc16_lds(McLds &sh) {
-  if constexpr (TH == 32)
-    return sh.h32;
-  else
-    return sh.h16;
-}
-
 template <bool HBD, bool EXPAND, int TH>
 __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const DescTile *tiles,
                                                 const vp9r_ref_plane *planes, void *out, uint32_t out_stride,
@@ -591,13 +632,22 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
 
 void mc_prepare() { upload_filters(); }
 
-void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
+void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
+               const McTile *one_tiles) {
   upload_filters();
   if (!wgs) return;
-  if (hbd)
-    hipLaunchKernelGGL(k_mc<true>, dim3(wgs), dim3(64), 0, s, fa, r, n);
-  else
-    hipLaunchKernelGGL(k_mc<false>, dim3(wgs), dim3(64), 0, s, fa, r, n);
+  auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, dim3(wgs), dim3(64), 0, s, one_tiles, fa, r, n); };
+  if (one_tiles) {
+    if (hbd)
+      go(k_mc<true, true>);
+    else
+      go(k_mc<false, true>);
+  } else {
+    if (hbd)
+      go(k_mc<true, false>);
+    else
+      go(k_mc<false, false>);
+  }
 }
 
 void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *tiles, uint32_t n_tiles,

@@ -137,13 +137,25 @@ static vp9g::FrameGeom geom_of(const vp9r_frame_hdr &h) {
 // The inter prediction tiles of the record (block.rs:1425-1448): each
 // block's planes as one prediction unit, or one 4x4 unit per sub-block
 // below 8x8; units split into tiles of at most 16 x 16.
+// A prediction unit's tiles are 16 columns by 16 rows, or by 32 rows when
+// every reference it reads is unscaled (a step of one sample: the kernel's
+// fast paths, which take 32-row tiles; the general path takes 16).
 static void build_mc_tiles(Builder *g, const vp9r_record *rec) {
   const vp9r_frame_hdr &h = rec->hdr;
+  const vp9g::FrameGeom geo = geom_of(h);
+  bool unscaled[3];
+  for (int ri = 0; ri < 3; ri++) {
+    const int64_t xsc = ((int64_t)h.ref_width[ri] << 14) / h.width, ysc = ((int64_t)h.ref_height[ri] << 14) / h.height;
+    unscaled[ri] = ((16 * xsc) >> 14) == 16 && ((16 * ysc) >> 14) == 16;
+  }
   auto &tiles = g->tiles;
   tiles.clear();
   for (uint32_t bi = 0; bi < rec->n_blocks; bi++) {
     const vp9r_block &b = rec->blocks[bi];
     if (!b.is_inter) continue;
+    const bool compound = b.ref_frame[1] > INTRA_FRAME;
+    const int r0 = b.ref_frame[0] - LAST_FRAME, r1 = compound ? b.ref_frame[1] - LAST_FRAME : r0;
+    const int tile_rows = r0 >= 0 && r0 < 3 && r1 >= 0 && r1 < 3 && unscaled[r0] && unscaled[r1] ? 32 : 16;
     int bsize = b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size;
     for (int plane = 0; plane < 3; plane++) {
       int sx = plane ? h.ss_x : 0, sy = plane ? h.ss_y : 0;
@@ -151,12 +163,16 @@ static void build_mc_tiles(Builder *g, const vp9r_record *rec) {
       int n4w = VP9_NUM_4X4_WIDE[psz], n4h = VP9_NUM_4X4_HIGH[psz];
       int base_x = (b.mi_col * 8) >> sx, base_y = (b.mi_row * 8) >> sy;
       auto unit = [&](int x, int y, int w, int hh, int sub) {
-        for (int ty = 0; ty < hh; ty += 16)
+        for (int ty = 0; ty < hh; ty += tile_rows)
           for (int tx = 0; tx < w; tx += 16) {
-            vp9g::McTile t;
+            vp9g::McTile t = {};
+            for (int rl = 0; rl < (compound ? 2 : 1); rl++)
+              vp9g::mc_select_mv(geo, b, plane, sub, rl, t.mv[rl][0], t.mv[rl][1]);
             t.block = bi;
             t.x = (uint16_t)x;
             t.y = (uint16_t)y;
+            t.mi_row = (uint16_t)b.mi_row;
+            t.mi_col = (uint16_t)b.mi_col;
             t.plane = (uint8_t)plane;
             t.sub = (uint8_t)sub;
             t.w = (uint8_t)w;
@@ -164,7 +180,11 @@ static void build_mc_tiles(Builder *g, const vp9r_record *rec) {
             t.tx = (uint8_t)tx;
             t.ty = (uint8_t)ty;
             t.tw = (uint8_t)std::min(16, w - tx);
-            t.th = (uint8_t)std::min(16, hh - ty);
+            t.th = (uint8_t)std::min(tile_rows, hh - ty);
+            t.mi_size = (uint8_t)b.mi_size;
+            t.interp_filter = (uint8_t)b.interp_filter;
+            t.ref_frame[0] = (int8_t)b.ref_frame[0];
+            t.ref_frame[1] = (int8_t)b.ref_frame[1];
             tiles.push_back(t);
           }
       };
@@ -1239,7 +1259,8 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       if (!gt)  // a graph's launch waits for the downloads before it
         for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
       vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
-      vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd);
+      vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd,
+                      n == 1 ? (const vp9g::McTile *)(f[0]->dv + f[0]->p->o_tiles) : nullptr);
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
       {
         const vp9g::Range *rk[4] = {R(t_itx[0]), R(t_itx[1]), R(t_itx[2]), R(t_itx[3])};
