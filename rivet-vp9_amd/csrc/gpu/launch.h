@@ -231,10 +231,16 @@ __host__ __device__ inline LfOne lf_one(const LfArgs &a, uint32_t t) {
 // device memory; wgs = r[n].wg; hbd: samples are u16.
 uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g);
 void launch_clear_margins(hipStream_t s, const ClearArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
-// one_tiles: for a launch of one frame, its tiles (device memory), read
-// straight from a kernel argument
+// one_tiles: for a launch of one frame, its one_count tiles (device
+// memory), read straight from a kernel argument.  A launch of one frame
+// takes workgroups of kMcWavesOne waves, a tile each (a third fewer
+// workgroups to dispatch for the few thousand tiles of a frame: 12.3 to 11.5
+// us at 1080p); a batch one wave per workgroup (its ranges count
+// mc_tiles_per_wg(n) tiles per workgroup), as the per-kernel seam.
+constexpr int kMcWavesOne = 4;
+inline uint32_t mc_tiles_per_wg(uint32_t n_frames) { return n_frames == 1 ? kMcWavesOne : 1; }
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-               const McTile *one_tiles);
+               const McTile *one_tiles, uint32_t one_count);
 // r[k], wgs[k]: the ranges and workgroups of transform size k (4x4 .. 32x32)
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4], uint32_t n, const uint32_t wgs[4],
                       bool hbd);

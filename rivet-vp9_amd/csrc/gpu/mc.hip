@@ -80,6 +80,15 @@ struct McShared {
 // rows of FP8 dwords (bytes from a dword-aligned column; the pitch of 10
 // puts the row groups a read touches in distinct banks), and the horizontal
 // pass column-major (column c holds rows 0 .. MAXH + 7 in HP dwords).
+// Each wave of an MC workgroup works on a tile of its own, in LDS of its
+// own: the tile's LDS exchanges need the wave's own accesses ordered, not a
+// workgroup barrier (a wave's LDS instructions execute in program order; the
+// fence keeps the compiler from moving accesses across the point).
+DEV void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 constexpr int FP8 = 10;
 template <int MAXH>
 struct Mc8Shared {
@@ -164,7 +173,7 @@ template <int MAXH>
 DEV void mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int tw, int th,
                   int filter, Mc8Shared<MAXH> &sh, uint32_t (&res)[MAXH / 16]) {
   constexpr int HP = Mc8Shared<MAXH>::HP, LR = (MAXH + 7 + 7) / 8;  // load rounds of 8 rows
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int px = x0 & 15, py = y0 & 15;
   const bool fx = px != 0, fy = py != 0;
   const int fx0 = (x0 >> 4) - (fx ? 3 : 0), fy0 = (y0 >> 4) - (fy ? 3 : 0);
@@ -196,7 +205,7 @@ DEV void mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, i
 #pragma unroll
   for (int i = 0; i < LR; i++)
     if (dd < nd && r0 + 8 * i < ih) sh.fp[(r0 + 8 * i) * FP8 + dd] = w[i] ^ flip;
-  __syncthreads();
+  wave_lds_sync();
   // horizontal: lane task t (column t % 16, rows 4 (t / 16) .. + 3) for
   // t = lane + 64 h.  Its results are biased when filtered (fx); the
   // vertical pass wants them biased when it filters (fy)
@@ -210,7 +219,7 @@ DEV void mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, i
       sh.hp[c * HP + rq] = four_samples<FP8>(sh.fp + 4 * rq * FP8 + (o >> 2), o & 3, fx, f0, f1) ^ vflip;
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   // vertical: lane task (row 16 h + lane / 4, columns 4 q .. 4 q + 3)
   const int q = lane & 3;
   const uint32_t g0 = c_taps8[filter][py][0], g1 = c_taps8[filter][py][1];
@@ -223,7 +232,7 @@ DEV void mc_tile8(const uint8_t *ref, uint32_t stride, int last_x, int last_y, i
     if (r < th && 4 * q < tw)
       res[h] = four_samples<HP>(sh.hp + 4 * q * HP + (r >> 2), r & 3, fy, g0, g1) ^ (fy ? 0x80808080u : 0u);
   }
-  __syncthreads();
+  wave_lds_sync();
 }
 
 // Per-sample rounding average of two packed pairs: (a + b + 1) >> 1.
@@ -275,7 +284,7 @@ template <int MAXH>
 DEV void mc_tile16(const uint16_t *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int tw, int th,
                    int filter, int max, Mc16Shared<MAXH> &sh, uint32_t (&res)[MAXH / 16][2]) {
   constexpr int HW = Mc16Shared<MAXH>::HW, LR = (MAXH + 7 + 3) / 4;  // load rounds of 4 rows
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int px = x0 & 15, py = y0 & 15;
   const bool fx = px != 0, fy = py != 0;
   const int fx0 = (x0 >> 4) - (fx ? 3 : 0), fy0 = (y0 >> 4) - (fy ? 3 : 0);
@@ -304,7 +313,7 @@ DEV void mc_tile16(const uint16_t *ref, uint32_t stride, int last_x, int last_y,
 #pragma unroll
   for (int i = 0; i < LR; i++)
     if (dd < nd && r0 + 4 * i < ih) sh.fp[(r0 + 4 * i) * FP16 + dd] = w[i];
-  __syncthreads();
+  wave_lds_sync();
   uint32_t f[4], g[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -323,7 +332,7 @@ DEV void mc_tile16(const uint16_t *ref, uint32_t stride, int last_x, int last_y,
       sh.hp[c * HW + 2 * rq + 1] = two[1];
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   // vertical: lane task (row 16 h + lane / 4, columns 4 q .. 4 q + 3)
   const int q = lane & 3;
 #pragma unroll
@@ -332,7 +341,7 @@ DEV void mc_tile16(const uint16_t *ref, uint32_t stride, int last_x, int last_y,
     res[h][0] = res[h][1] = 0;
     if (r < th && 4 * q < tw) four_samples16<HW>(sh.hp + 4 * q * HW + (r >> 1), r & 1, fy, g, max, res[h]);
   }
-  __syncthreads();
+  wave_lds_sync();
 }
 
 // Predicts the tw x th (<= 16 x 16) tile whose first sample is at (x0, y0)
@@ -345,7 +354,7 @@ DEV void mc_tile16(const uint16_t *ref, uint32_t stride, int last_x, int last_y,
 template <class P>
 DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, int y0, int xs, int ys,
                  int tw, int th, int filter, int max, McShared &sh, int out[4]) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const bool fx = xs != 16 || (x0 & 15), fy = ys != 16 || (y0 & 15);
   const int fx0 = (x0 >> 4) - (fx ? 3 : 0), fy0 = (y0 >> 4) - (fy ? 3 : 0);
   const int iw = fx ? ((((tw - 1) * xs + (x0 & 15)) >> 4) + 8) : tw;
@@ -355,7 +364,7 @@ DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, 
     const int sy = iclamp(fy0 + rr, 0, last_y), sx = iclamp(fx0 + cc, 0, last_x);
     sh.fp[rr * FPW + cc] = ref[(size_t)sy * stride + sx];
   }
-  __syncthreads();
+  wave_lds_sync();
   for (int k = lane; k < ih * tw; k += 64) {
     const int rr = k / tw, cc = k - rr * tw;
     int v;
@@ -372,7 +381,7 @@ DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, 
     }
     sh.hp[rr * TILE + cc] = (int16_t)v;
   }
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int o = lane + 64 * i;
@@ -393,7 +402,7 @@ DEV void mc_tile(const P *ref, uint32_t stride, int last_x, int last_y, int x0, 
     }
     out[i] = v;
   }
-  __syncthreads();
+  wave_lds_sync();
 }
 
 template <int TH>
@@ -430,7 +439,7 @@ DEV void mc_frame8(const McArgs &a, const McTile &t, const RefPos &rp0, const Re
   const int q = threadIdx.x & 3;
 #pragma unroll
   for (int h = 0; h < NH; h++) {
-    const int r = 16 * h + (threadIdx.x >> 2);
+    const int r = 16 * h + ((threadIdx.x & 63) >> 2);
     if (r < t.th && 4 * q < t.tw)
       gst((uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q), v[h]);
   }
@@ -455,7 +464,7 @@ DEV void mc_frame16(const McArgs &a, const McTile &t, const RefPos &rp0, const R
   const int q = threadIdx.x & 3;
 #pragma unroll
   for (int h = 0; h < NH; h++) {
-    const int r = 16 * h + (threadIdx.x >> 2);
+    const int r = 16 * h + ((threadIdx.x & 63) >> 2);
     if (r < t.th && 4 * q < t.tw) {
       uint32_t *o = (uint32_t *)(dst + (size_t)(t.y + t.ty + r) * stride + t.x + t.tx + 4 * q);
       gst(o, v[h][0]);
@@ -464,17 +473,23 @@ DEV void mc_frame16(const McArgs &a, const McTile &t, const RefPos &rp0, const R
   }
 }
 
-// A launch of one frame (ONE) reads its tile straight from the tiles
-// argument, beside the frame's arguments; a batch finds its frame and tile
-// through the range table.
-template <bool HBD, bool ONE>
-__global__ __launch_bounds__(64) void k_mc(const McTile *tiles1, const McArgs *fa, const Range *rg, uint32_t nf) {
+// W waves per workgroup, each on a tile of its own (fewer workgroups to
+// dispatch).  A launch of one frame (ONE) reads its tiles straight from the
+// tiles argument (n1 of them), beside the frame's arguments; a batch finds
+// its frame and tiles through the range table (W tiles per workgroup).
+template <bool HBD, bool ONE, int W>
+__global__ __launch_bounds__(64 * W) void k_mc(const McTile *tiles1, uint32_t n1, const McArgs *fa, const Range *rg,
+                                               uint32_t nf) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  __shared__ McLds sh;
+  __shared__ McLds shw[W];
+  const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  McLds &sh = shw[wave];
   const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
   const uint32_t fi = ONE ? 0 : range_frame(rg, nf, wg);
+  const uint32_t local = (wg - (ONE ? 0 : rg[fi].wg)) * W + wave;
+  if (local >= (ONE ? n1 : rg[fi].count)) return;
   const McArgs &a = fa[fi];
-  const McTile t = ONE ? tiles1[wg] : a.tiles[rg[fi].first + wg - rg[fi].wg];
+  const McTile t = ONE ? tiles1[local] : a.tiles[rg[fi].first + local];
   const int max = (1 << a.g.bit_depth) - 1;
   const bool compound = t.ref_frame[1] > INTRA_FRAME;
   P *dst = (P *)a.cur.p[t.plane];
@@ -512,7 +527,7 @@ __global__ __launch_bounds__(64) void k_mc(const McTile *tiles1, const McArgs *f
                sh.g, out[1]);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    const int o = threadIdx.x + 64 * i;
+    const int o = (threadIdx.x & 63) + 64 * i;
     if (o < t.th * t.tw) {
       const int r = o / t.tw, c = o - r * t.tw;
       const int v = compound ? (out[0][i] + out[1][i] + 1) >> 1 : out[0][i];
@@ -543,7 +558,7 @@ DEV void mc_desc_half(const vp9r_mc_desc &d, const vp9r_ref_plane &r0, const vp9
                tw, th, d.filter, max, sh, o1);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    const int o = threadIdx.x + 64 * i;
+    const int o = (threadIdx.x & 63) + 64 * i;
     if (o < th * tw) {
       const int r = o / tw, c = o - r * tw;
       dst[(size_t)r * out_stride + c] = (P)(compound ? (o0[i] + o1[i] + 1) >> 1 : o0[i]);
@@ -551,27 +566,31 @@ DEV void mc_desc_half(const vp9r_mc_desc &d, const vp9r_ref_plane &r0, const vp9
   }
 }
 
-// EXPAND: no tile list; workgroup 8 i + k is tile k of descriptor i (16
-// columns by 32 rows: column (k & 3), row half (k >> 2)), and tiles outside
-// the block exit at once (the whole workgroup, before any barrier).  TH:
-// the tile's rows, 16 or 32 (the autotuner's choice per bit depth class).
-template <bool HBD, bool EXPAND, int TH>
-__global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const DescTile *tiles,
-                                                const vp9r_ref_plane *planes, void *out, uint32_t out_stride,
-                                                int bd) {
+// EXPAND: no tile list; tile 8 i + k is tile k of descriptor i (16 columns
+// by 32 rows: column (k & 3), row half (k >> 2)), and tiles outside the
+// block exit at once (their wave; no wave waits on another).  TH: the
+// tile's rows, 16 or 32 (the autotuner's choice per bit depth class).
+template <bool HBD, bool EXPAND, int TH, int W>
+__global__ __launch_bounds__(64 * W) void k_mc_desc(const vp9r_mc_desc *descs, const DescTile *tiles,
+                                                    uint32_t n_tiles, const vp9r_ref_plane *planes, void *out,
+                                                    uint32_t out_stride, int bd) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
   constexpr int NH = TH / 16;
-  __shared__ McLds sh;
+  __shared__ McLds shw[W];
+  const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  McLds &sh = shw[wave];
   DescTile t;
-  const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
+  // W consecutive tiles per workgroup, a wave each
+  const uint32_t id = xcd_order(blockIdx.x, gridDim.x) * W + wave;
+  if (id >= n_tiles) return;
   if (EXPAND) {
-    t.desc = wg >> 3;
-    t.tx = (uint16_t)(16 * (wg & 3));
-    t.ty = (uint16_t)(32 * ((wg >> 2) & 1));
+    t.desc = id >> 3;
+    t.tx = (uint16_t)(16 * (id & 3));
+    t.ty = (uint16_t)(32 * ((id >> 2) & 1));
     const vp9r_mc_desc dd = descs[t.desc];
     if (t.tx >= dd.w || t.ty >= dd.h) return;
   } else {
-    t = tiles[wg];
+    t = tiles[id];
   }
   const vp9r_mc_desc d = descs[t.desc];
   // each reference its own plane (inter.rs:21-27 RefPlane)
@@ -593,7 +612,7 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
     const int q = threadIdx.x & 3;
 #pragma unroll
     for (int h = 0; h < NH; h++) {
-      const int r = 16 * h + (threadIdx.x >> 2);
+      const int r = 16 * h + ((threadIdx.x & 63) >> 2);
       if (r < th && 4 * q < tw) gst((uint32_t *)(dst + (size_t)r * out_stride + 4 * q), v[h]);
     }
     return;
@@ -615,7 +634,7 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
     const int q = threadIdx.x & 3;
 #pragma unroll
     for (int h = 0; h < NH; h++) {
-      const int r = 16 * h + (threadIdx.x >> 2);
+      const int r = 16 * h + ((threadIdx.x & 63) >> 2);
       if (r < th && 4 * q < tw) {
         uint32_t *o = (uint32_t *)(dst + (size_t)r * out_stride + 4 * q);
         gst(o, v[h][0]);
@@ -633,20 +652,22 @@ __global__ __launch_bounds__(64) void k_mc_desc(const vp9r_mc_desc *descs, const
 void mc_prepare() { upload_filters(); }
 
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
-               const McTile *one_tiles) {
+               const McTile *one_tiles, uint32_t one_count) {
   upload_filters();
   if (!wgs) return;
-  auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, dim3(wgs), dim3(64), 0, s, one_tiles, fa, r, n); };
+  auto go = [&](auto kernel, int waves) {
+    hipLaunchKernelGGL(kernel, dim3(wgs), dim3(64 * waves), 0, s, one_tiles, one_count, fa, r, n);
+  };
   if (one_tiles) {
     if (hbd)
-      go(k_mc<true, true>);
+      go(k_mc<true, true, kMcWavesOne>, kMcWavesOne);
     else
-      go(k_mc<false, true>);
+      go(k_mc<false, true, kMcWavesOne>, kMcWavesOne);
   } else {
     if (hbd)
-      go(k_mc<true, false>);
+      go(k_mc<true, false, 1>, 1);
     else
-      go(k_mc<false, false>);
+      go(k_mc<false, false, 1>, 1);
   }
 }
 
@@ -655,34 +676,35 @@ void launch_mc_desc_tiles(hipStream_t s, const vp9r_mc_desc *desc, const void *t
   upload_filters();
   if (!n_tiles) return;
   auto go = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles, planes, out, out_stride,
-                       (int)bd);
+    hipLaunchKernelGGL(kernel, dim3(n_tiles), dim3(64), 0, s, desc, (const DescTile *)tiles, n_tiles, planes,
+                       out, out_stride, (int)bd);
   };
   if (bd > 8) {
     if (tile_rows == 16)
-      go(k_mc_desc<true, false, 16>);
+      go(k_mc_desc<true, false, 16, 1>);
     else
-      go(k_mc_desc<true, false, 32>);
+      go(k_mc_desc<true, false, 32, 1>);
   } else {
     if (tile_rows == 16)
-      go(k_mc_desc<false, false, 16>);
+      go(k_mc_desc<false, false, 16, 1>);
     else
-      go(k_mc_desc<false, false, 32>);
+      go(k_mc_desc<false, false, 32, 1>);
   }
 }
 
-// The same without a tile list: 8 workgroups per descriptor (the tiles of a
-// 64 x 64 block), those outside a smaller block exit.
+// The same without a tile list: 8 tiles per descriptor (those of a 64 x 64
+// block), those outside a smaller block exit.
 void launch_mc_desc_expand(hipStream_t s, const vp9r_mc_desc *desc, uint32_t count, const vp9r_ref_plane *planes,
                            void *out, uint32_t out_stride, uint32_t bd) {
   upload_filters();
   if (!count) return;
+  const uint32_t n_tiles = 8 * count;
   if (bd > 8)
-    hipLaunchKernelGGL((k_mc_desc<true, true, 32>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
-                       out_stride, (int)bd);
+    hipLaunchKernelGGL((k_mc_desc<true, true, 32, 1>), dim3(n_tiles), dim3(64), 0, s, desc, nullptr,
+                       n_tiles, planes, out, out_stride, (int)bd);
   else
-    hipLaunchKernelGGL((k_mc_desc<false, true, 32>), dim3(8 * count), dim3(64), 0, s, desc, nullptr, planes, out,
-                       out_stride, (int)bd);
+    hipLaunchKernelGGL((k_mc_desc<false, true, 32, 1>), dim3(n_tiles), dim3(64), 0, s, desc, nullptr,
+                       n_tiles, planes, out, out_stride, (int)bd);
 }
 
 }  // namespace vp9g

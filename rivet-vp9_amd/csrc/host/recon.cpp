@@ -1082,7 +1082,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     if (recon) {
       t_clear = table([](int) { return 0u; },
                       [&](int i) { return 3 * vp9g::clear_margin_rows(f[i]->cur->planes, f[i]->p->geo); }, 1);
-      t_mc = table([](int) { return 0u; }, [&](int i) { return f[i]->p->n_tiles; }, 1);
+      t_mc = table([](int) { return 0u; }, [&](int i) { return f[i]->p->n_tiles; }, vp9g::mc_tiles_per_wg(n));
       // intra: leveled blocks, whose residuals the inter residual launches
       // compute, or (superblock, plane) groups, which compute their own
       groups = !g->intra_blocks;
@@ -1260,7 +1260,8 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
         for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
       vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
       vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd,
-                      n == 1 ? (const vp9g::McTile *)(f[0]->dv + f[0]->p->o_tiles) : nullptr);
+                      n == 1 ? (const vp9g::McTile *)(f[0]->dv + f[0]->p->o_tiles) : nullptr,
+                      n == 1 ? f[0]->p->n_tiles : 0u);
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
       {
         const vp9g::Range *rk[4] = {R(t_itx[0]), R(t_itx[1]), R(t_itx[2]), R(t_itx[3])};
