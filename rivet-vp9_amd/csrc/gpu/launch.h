@@ -233,11 +233,12 @@ uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g);
 void launch_clear_margins(hipStream_t s, const ClearArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
 // one_tiles: for a launch of one frame, its one_count tiles (device
 // memory), read straight from a kernel argument.  A launch of one frame
-// takes workgroups of kMcWavesOne waves, a tile each (a third fewer
-// workgroups to dispatch for the few thousand tiles of a frame: 12.3 to 11.5
-// us at 1080p); a batch one wave per workgroup (its ranges count
-// mc_tiles_per_wg(n) tiles per workgroup), as the per-kernel seam.
-constexpr int kMcWavesOne = 4;
+// takes workgroups of kMcWavesOne waves, a tile each (half the workgroups
+// to dispatch for the few thousand tiles of a frame: 12.3 to 11.4 us at
+// 1080p; 4 waves 11.5-11.8, 8 waves 12.1); a batch one wave per workgroup
+// (its ranges count mc_tiles_per_wg(n) tiles per workgroup), as the
+// per-kernel seam (4 waves: 1.66 to 1.79 ms for the microbenchmark).
+constexpr int kMcWavesOne = 2;
 inline uint32_t mc_tiles_per_wg(uint32_t n_frames) { return n_frames == 1 ? kMcWavesOne : 1; }
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
                const McTile *one_tiles, uint32_t one_count);
