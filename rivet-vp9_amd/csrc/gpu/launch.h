@@ -272,6 +272,10 @@ void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n,
 // (else p[0] null: each workgroup finds its frame through the range table)
 void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
                      const LfOne &one);
+// Levels one.t and one.t + 1 of one frame in one launch (lf_pair.hip),
+// from the planes one.p into the planes `out` (same geometry, every sample
+// of the superblock grid written).
+void launch_lf_pair(hipStream_t s, const LfOne &one, const Planes &out, bool hbd);
 
 // Per-kernel seam (vp9r.h batched forms).
 void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,

@@ -469,6 +469,9 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   // VP9R_INTRA=groups: intra frames as (superblock, plane) groups
   const char *im = getenv("VP9R_INTRA");
   g->intra_blocks = !(im && strcmp(im, "groups") == 0);
+  // VP9R_LF=levels: one loop filter level per launch throughout
+  const char *lm = getenv("VP9R_LF");
+  g->lf_pairs = !(lm && strcmp(lm, "levels") == 0);
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->lf_stream, hipStreamNonBlocking);
@@ -1305,7 +1308,41 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       if (recon && g->timing && !n_wave_intra) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
       uint32_t lds = 0;
       for (int i = 0; i < n; i++) lds = std::max(lds, vp9g::wave_lds_bytes(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y));
+      // A single frame's levels after its intra levels: the first ones one
+      // per launch in place, the rest two per launch (k_lf_pair), from the
+      // frame store to lf_tmp and back, an even number of such launches so
+      // that the last writes the frame store.
+      size_t pair_from = n_wave;
+      vp9g::Planes tmp{};
+      if (n == 1 && !gt && lf && g->lf_pairs && n_wave > n_wave_intra) {
+        size_t m = n_wave - n_wave_intra, singles = m % 2, pairs = (m - singles) / 2;
+        if (pairs % 2) {
+          pairs--;
+          singles += 2;
+        }
+        if (pairs) {
+          const vp9r_gpu_frame *cf = f[0]->cur;
+          if (cf->bytes > g->lf_tmp.cap) {
+            HIP_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+            HIP_TRY(g->lf_tmp.ensure(cf->bytes), "hipMalloc(loop filter planes)");
+          }
+          tmp = cf->planes;
+          for (int p = 0; p < 3; p++)
+            tmp.p[p] = (uint8_t *)g->lf_tmp.p + ((const uint8_t *)cf->planes.p[p] - (const uint8_t *)cf->base);
+          pair_from = n_wave_intra + singles;
+        }
+      }
       for (size_t k = 0; k < n_wave; k++) {
+        if (k >= pair_from) {
+          const vp9g::LfArgs &la = ((const vp9g::LfArgs *)(st.data() + o_lf))[0];
+          vp9g::LfOne one = vp9g::lf_one(la, (uint32_t)lf_lvl[0][k]);
+          const bool back = (k - pair_from) % 4 == 2;  // lf_tmp to the frame store
+          if (back)
+            for (int p = 0; p < 3; p++) one.p[p] = (uint8_t *)tmp.p[p];
+          vp9g::launch_lf_pair(s, one, back ? la.cur : tmp, hbd);
+          k++;
+          continue;
+        }
         if (k >= n_wave_intra) {
           vp9g::launch_lf_level(s, (const vp9g::LfArgs *)(d + o_lf), R(tw_l[k]), n, tw_l[k].wgs, hbd,
                                 n == 1 && lf_lvl[0][k] >= 0

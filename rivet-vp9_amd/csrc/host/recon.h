@@ -166,6 +166,10 @@ struct vp9r_gpu {
   // host work lists, reused
   vp9r::Builder work;
   bool intra_blocks = true;  // intra as leveled blocks (else (superblock, plane) groups)
+  // a single frame's loop filter levels after its intra levels two per
+  // launch (else one), through a second copy of the frame's planes
+  bool lf_pairs = true;
+  vp9r::DevBuf lf_tmp;
   // batched launches
   std::vector<uint8_t> batch_stage;
   vp9r::BatchSlot slots[vp9r::kBatchSlots];

@@ -122,6 +122,30 @@ def test_stream_lockstep_intra_groups(gpu_groups, name):
     assert err is None, f"{name}: {n} frames equal, then {err}"
 
 
+@pytest.fixture(scope="module")
+def gpu_lf_levels():
+    old = os.environ.get("VP9R_LF")
+    os.environ["VP9R_LF"] = "levels"
+    try:
+        g = vp.Gpu(0)
+    finally:
+        if old is None:
+            del os.environ["VP9R_LF"]
+        else:
+            os.environ["VP9R_LF"] = old
+    return g
+
+
+@pytest.mark.parametrize("name", ["1080p-workload", "12bit-444", "8bit-422"])
+def test_stream_lockstep_lf_levels(gpu_lf_levels, name):
+    """One loop filter level per launch throughout (VP9R_LF=levels; by
+    default a single frame's last levels go two per launch)."""
+    print(f"stream {name} (loop filter levels)", flush=True)
+    frames = streamgen.generate(**CASES[name])
+    n, err = gpu_lockstep.run_frames(frames, gpu_lf_levels)
+    assert err is None, f"{name}: {n} frames equal, then {err}"
+
+
 def test_prepared_run_equals_reconstruct(gpu):
     """vp9r_gpu_prepare + vp9r_gpu_run (the benchmark's path) gives the same
     frames as vp9r_gpu_reconstruct + vp9r_gpu_loop_filter, run twice."""
