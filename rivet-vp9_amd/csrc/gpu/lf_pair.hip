@@ -14,9 +14,11 @@
 // the job where it is the left predecessor (else, in the last superblock
 // column, by the job where it is the upper one).  Jobs read the frame as it
 // was after level L - 1 while others write, so each launch reads one copy
-// of the planes (in) and writes the other (out); the remaining workgroups
-// copy the samples no level of the launch writes from in to out, so out
-// holds the whole frame after level L + 1.
+// of the planes (in) and writes the other (out), and the launches alternate
+// between the two.  The remaining workgroups copy from in to out the
+// samples that no level of the launch writes and that out lacks (all of
+// them in a frame's first such launch, else those that the previous launch
+// changed), so that out holds the whole frame after level L + 1.
 //
 // Superblock (r, c) of a plane (SW x SH samples) is written by the unit
 // levels r + c - 1 and r + c, and its bottom 8 rows also by level r + c + 1
@@ -44,77 +46,66 @@ struct __align__(16) LfPairShared {
   uint32_t dq[8][272];
 };
 
-// A rectangle of whole 8-sample vectors: plane rows [y0, y0 + rows), plane
-// columns [x0, x0 + 8 nq); `on` false for a rectangle the job lacks.
+// A rectangle of whole 16-byte vectors (16 samples at 8 bits, 8 above):
+// plane rows [y0, y0 + rows), plane columns from x0, 1 << lq vectors per
+// row (a superblock's width); `on` false for a rectangle the job lacks.
 struct LfRect {
-  int y0, rows, x0, lnq;  // lnq: log2 of the vectors per row
+  int y0, rows, x0;
   bool on;
 };
 
-template <class P>
-struct LfVec {
-  typedef typename std::conditional<sizeof(P) == 1, uint2, uint4>::type V;
-};
-
-// The vectors of rectangle `rc` taken by thread tid in slots i < N (vector
-// tid + 128 i): loads, then stores into LDS, and LDS back to a plane.
-template <class P, int N>
-DEV void pair_load(const P *pl, uint32_t stride, const LfRect &rc, typename LfVec<P>::V (&v)[N], int tid) {
+// Vectors k0 + t + NT i < kend (i < N) of rectangle `rc`, for thread t of
+// NT: loads, then stores into LDS, and LDS back to a plane.  LDS offsets
+// are dword-aligned (the pitch and the column offsets are multiples of 4
+// bytes), so each vector goes in and out as four dwords.
+template <class P, int NT, int N>
+DEV void pair_load(const P *pl, uint32_t stride, const LfRect &rc, int lq, uint4 (&v)[N], int t, int k0, int kend) {
+  constexpr int QB = 16 / (int)sizeof(P);
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    const int k = tid + 128 * i;
-    if (rc.on && k < (rc.rows << rc.lnq))
-      v[i] = gld((const typename LfVec<P>::V *)(pl + (size_t)(rc.y0 + (k >> rc.lnq)) * stride + rc.x0 +
-                                                   8 * (k & ((1 << rc.lnq) - 1))));
+    const int k = k0 + t + NT * i;
+    if (rc.on && k < kend)
+      v[i] = gld((const uint4 *)(pl + (size_t)(rc.y0 + (k >> lq)) * stride + rc.x0 + QB * (k & ((1 << lq) - 1))));
   }
 }
 
-template <class P, int N>
-DEV void pair_to_lds(const LfRect &rc, const typename LfVec<P>::V (&v)[N], P *buf, int ly0, int lx0, int tid) {
-  constexpr int BP = LfPairShared<P>::BP;
+template <class P, int NT, int N>
+DEV void pair_to_lds(const LfRect &rc, int lq, const uint4 (&v)[N], P *buf, int ly0, int lx0, int t, int k0,
+                     int kend) {
+  constexpr int BP = LfPairShared<P>::BP, QB = 16 / (int)sizeof(P);
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    const int k = tid + 128 * i;
-    if (rc.on && k < (rc.rows << rc.lnq)) {
-      uint32_t *l =
-          (uint32_t *)(buf + (rc.y0 - ly0 + (k >> rc.lnq)) * BP + rc.x0 - lx0 + 8 + 8 * (k & ((1 << rc.lnq) - 1)));
-      if (sizeof(P) == 1) {
-        const uint2 w = *(const uint2 *)&v[i];
-        l[0] = w.x;
-        l[1] = w.y;
-      } else {
-        const uint4 w = *(const uint4 *)&v[i];
-        l[0] = w.x;
-        l[1] = w.y;
-        l[2] = w.z;
-        l[3] = w.w;
-      }
+    const int k = k0 + t + NT * i;
+    if (rc.on && k < kend) {
+      uint32_t *l = (uint32_t *)(buf + (rc.y0 - ly0 + (k >> lq)) * BP + rc.x0 - lx0 + 8 + QB * (k & ((1 << lq) - 1)));
+      l[0] = v[i].x;
+      l[1] = v[i].y;
+      l[2] = v[i].z;
+      l[3] = v[i].w;
     }
   }
 }
 
-template <class P, int N>
-DEV void pair_store(P *pl, uint32_t stride, const LfRect &rc, const P *buf, int ly0, int lx0, int tid) {
-  constexpr int BP = LfPairShared<P>::BP;
-  typedef typename LfVec<P>::V V;
-  V v[N];
+// Every LDS read is issued before the first store.
+template <class P, int NT, int N>
+DEV void pair_store(P *pl, uint32_t stride, const LfRect &rc, int lq, const P *buf, int ly0, int lx0, int t) {
+  constexpr int BP = LfPairShared<P>::BP, QB = 16 / (int)sizeof(P);
+  uint4 v[N];
+  const int kend = rc.rows << lq;
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    const int k = tid + 128 * i;
-    const bool ok = rc.on && k < (rc.rows << rc.lnq);
-    const uint32_t *l = (const uint32_t *)(buf + (ok ? (rc.y0 - ly0 + (k >> rc.lnq)) * BP + rc.x0 - lx0 + 8 +
-                                                          8 * (k & ((1 << rc.lnq) - 1))
+    const int k = t + NT * i;
+    const bool ok = rc.on && k < kend;
+    const uint32_t *l = (const uint32_t *)(buf + (ok ? (rc.y0 - ly0 + (k >> lq)) * BP + rc.x0 - lx0 + 8 +
+                                                          QB * (k & ((1 << lq) - 1))
                                                     : 8));
-    if (sizeof(P) == 1)
-      *(uint2 *)&v[i] = make_uint2(l[0], l[1]);
-    else
-      *(uint4 *)&v[i] = make_uint4(l[0], l[1], l[2], l[3]);
+    v[i] = make_uint4(l[0], l[1], l[2], l[3]);
   }
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    const int k = tid + 128 * i;
-    if (rc.on && k < (rc.rows << rc.lnq))
-      gst((V *)(pl + (size_t)(rc.y0 + (k >> rc.lnq)) * stride + rc.x0 + 8 * (k & ((1 << rc.lnq) - 1))), v[i]);
+    const int k = t + NT * i;
+    if (rc.on && k < kend)
+      gst((uint4 *)(pl + (size_t)(rc.y0 + (k >> lq)) * stride + rc.x0 + QB * (k & ((1 << lq) - 1))), v[i]);
   }
 }
 
@@ -165,27 +156,49 @@ DEV void pair_unit(LfPairShared<P> &sh, int slot, bool lone, int ur, int uc, int
 template <bool HBD>
 __global__ __launch_bounds__(128) void k_lf_pair(uint8_t *i0, uint8_t *i1, uint8_t *i2, uint8_t *o0, uint8_t *o1,
                                                  uint8_t *o2, uint32_t s0, uint32_t s1, uint32_t s2, uint32_t *dq,
-                                                 uint32_t geo, uint32_t L, uint32_t n_jobs) {
+                                                 uint32_t geo, uint32_t L, uint32_t n_jobs, uint32_t copy_all) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  typedef typename LfVec<P>::V V;
   __shared__ LfPairShared<P> sh;
-  const int C = (int)(geo & 0xfff), R = (int)((geo >> 12) & 0xfff);
+  const int C = (int)(geo & 0xfff), R = (int)((geo >> 12) & 0xfff), M = min(R, C);
   const int bit_depth = (int)(geo >> 26), bd = HBD ? bit_depth : 8;
   const int tid = (int)threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t b = blockIdx.x;
-  const int plane = b < 3 * n_jobs ? (int)(b / n_jobs) : (int)((b - 3 * n_jobs) / (uint32_t)(R * C));
+  const uint32_t b = blockIdx.x, n_copy = copy_all ? (uint32_t)(R * C) : (uint32_t)(3 * M);
+  const int plane = b < 3 * n_jobs ? (int)(b / n_jobs) : (int)((b - 3 * n_jobs) / n_copy);
   const int sx = plane ? (int)((geo >> 24) & 1) : 0, sy = plane ? (int)((geo >> 25) & 1) : 0;
-  const int SW = 64 >> sx, SH = 64 >> sy, lnq = 3 - sx;
+  const int SW = 64 >> sx, SH = 64 >> sy;
   const P *in = (const P *)(plane == 0 ? i0 : plane == 1 ? i1 : i2);
   P *out = (P *)(plane == 0 ? o0 : plane == 1 ? o1 : o2);
   const uint32_t stride = plane == 0 ? s0 : plane == 1 ? s1 : s2;
   const int Li = (int)L;
   if (b >= 3 * n_jobs) {
-    // copy the parts of superblock j that no level of the launch writes
-    const int j = (int)((b - 3 * n_jobs) % (uint32_t)(R * C)), r = j / C, c = j % C, t = r + c;
-    const bool body = !(t - 1 == Li || t - 1 == Li + 1 || t == Li || t == Li + 1);
-    const bool bottom = body && !(r + 1 < R && (t + 1 == Li || t + 1 == Li + 1));
+    // The parts of superblock (r, c), t = r + c, whose samples in `out` are
+    // not those of the frame after level L + 1 and that no job writes.
+    // copy_all (out holds nothing yet): every part that the levels L and
+    // L + 1 leave alone, workgroup j taking superblock j.  Else out holds the
+    // frame after level L - 3, and the parts to copy are those the levels
+    // L - 2 and L - 1 wrote and L and L + 1 leave alone: the body (all but
+    // the bottom 8 rows) for t = L - 2 and L - 1, the bottom rows for t =
+    // L - 3 and L - 2 (L - 2 and L - 1 in the last superblock row), workgroup
+    // j taking the superblock of row max(0, t - C + 1) + j % M on diagonal
+    // t = L - 3 + j / M, if there is one.
+    const int j = (int)((b - 3 * n_jobs) % n_copy);
+    int r, c, t;
+    bool body, bottom;
+    if (copy_all) {
+      r = j / C;
+      c = j % C;
+      t = r + c;
+      body = !(t - 1 == Li || t - 1 == Li + 1 || t == Li || t == Li + 1);
+      bottom = body && !(r + 1 < R && (t + 1 == Li || t + 1 == Li + 1));
+    } else {
+      t = Li - 3 + j / M;
+      r = (t >= C ? t - C + 1 : 0) + j % M;
+      if (t < 0 || r > min(R - 1, t)) return;
+      c = t - r;
+      body = t == Li - 2 || t == Li - 1;
+      bottom = r + 1 < R ? t == Li - 3 || t == Li - 2 : body;
+    }
     // rows of 16-byte vectors: qs per row
     constexpr int QB = 16 / (int)sizeof(P);  // samples per vector
     const int lq = sizeof(P) == 1 ? 2 - sx : 3 - sx, y0 = r * SH, x0 = c * SW;
@@ -220,14 +233,17 @@ __global__ __launch_bounds__(128) void k_lf_pair(uint8_t *i0, uint8_t *i1, uint8
   // 8 is plane column (c - 1) SW
   const int ly0 = (r - 1) * SH - 8, lx0 = (c - 1) * SW;
   const bool right = c + 1 < C;
-  LfRect rc[7];
-  rc[0] = {(r - 1) * SH - 8, 8, c * SW, lnq, has_b && r >= 2};          // above U(r - 1, c)
-  rc[1] = {(r - 1) * SH, SH, c * SW, lnq, has_b};                      // superblock (r - 1, c)
-  rc[2] = {(r - 1) * SH, SH, (c + 1) * SW, lnq, has_b && right};       // superblock (r - 1, c + 1)
-  rc[3] = {r * SH - 8, 8, (c - 1) * SW, lnq, has_a && !a_lone && r >= 1};  // above U(r, c - 1)
-  rc[4] = {r * SH, SH, (c - 1) * SW, lnq, has_a && !a_lone};           // superblock (r, c - 1)
-  rc[5] = {r * SH, SH, c * SW, lnq, true};                             // superblock (r, c)
-  rc[6] = {r * SH, SH, (c + 1) * SW, lnq, !lone && right};            // superblock (r, c + 1)
+  // the rectangles of the region: wave 1 loads B's (0 .. 2), wave 0 A's
+  // (3 .. 5; for a lone job, its superblock), and each wave half of
+  // superblock (r, c + 1), which only V reads
+  const LfRect rc0 = {(r - 1) * SH - 8, 8, c * SW, has_b && r >= 2};          // above U(r - 1, c)
+  const LfRect rc1 = {(r - 1) * SH, SH, c * SW, has_b};                      // superblock (r - 1, c)
+  const LfRect rc2 = {(r - 1) * SH, SH, (c + 1) * SW, has_b && right};       // superblock (r - 1, c + 1)
+  const LfRect rc3 = {r * SH - 8, 8, (c - 1) * SW, has_a && !a_lone && r >= 1};  // above U(r, c - 1)
+  const LfRect rc4 = {r * SH, SH, (c - 1) * SW, has_a && !a_lone};           // superblock (r, c - 1)
+  const LfRect rc5 = {r * SH, SH, c * SW, true};                             // superblock (r, c)
+  const LfRect rc6 = {r * SH, SH, (c + 1) * SW, !lone && right};            // superblock (r, c + 1)
+  const LfRect &q0 = wave ? rc0 : rc3, &q1 = wave ? rc1 : rc4, &q2 = wave ? rc2 : rc5;
   // the passes' decisions: predecessor A in slots 0..2, B in 3..5, V in 6..7
   const size_t n_sb = (size_t)R * C;
   const uint32_t *dv = dq + (size_t)(plane * 2 + 0) * n_sb * 256, *dh = dq + (size_t)(plane * 2 + 1) * n_sb * 256;
@@ -245,50 +261,53 @@ __global__ __launch_bounds__(128) void k_lf_pair(uint8_t *i0, uint8_t *i1, uint8
     d[k] = make_uint4(0, 0, 0, 0);
     if (on && wave == (grp == 1 ? 1 : 0)) d[k] = gld((const uint4 *)((vert ? dv : dh) + (size_t)sb * 256) + lane);
   }
-  V v0[1], v1[4], v2[4], v3[1], v4[4], v5[4], v6[4];
-  pair_load<P, 1>(in, stride, rc[0], v0, tid);
-  pair_load<P, 4>(in, stride, rc[1], v1, tid);
-  pair_load<P, 4>(in, stride, rc[2], v2, tid);
-  pair_load<P, 1>(in, stride, rc[3], v3, tid);
-  pair_load<P, 4>(in, stride, rc[4], v4, tid);
-  pair_load<P, 4>(in, stride, rc[5], v5, tid);
-  pair_load<P, 4>(in, stride, rc[6], v6, tid);
-  // every slot its wave owns, used or not
+  // 16-byte vectors: 1 << lq per superblock row; NS per lane for a
+  // superblock, NS / 2 for half of one, one for 8 rows
+  const int lq = (sizeof(P) == 1 ? 2 : 3) - sx, nsb = SH << lq, nb8 = 8 << lq, half = nsb / 2;
+  constexpr int NS = sizeof(P) == 1 ? 4 : 8;
+  uint4 v0[1], v1[NS], v2[NS], v6[NS / 2];
+  pair_load<P, 64, 1>(in, stride, q0, lq, v0, lane, 0, nb8);
+  pair_load<P, 64, NS>(in, stride, q1, lq, v1, lane, 0, nsb);
+  pair_load<P, 64, NS>(in, stride, q2, lq, v2, lane, 0, nsb);
+  pair_load<P, 64, NS / 2>(in, stride, rc6, lq, v6, lane, wave * half, wave * half + half);
+  // every decision slot its wave owns, used or not
 #pragma unroll
   for (int k = 0; k < 8; k++)
     if (wave == (k >= 3 && k < 6 ? 1 : 0)) ((uint4 *)sh.dq[k])[lane] = d[k];
-  pair_to_lds<P, 1>(rc[0], v0, sh.buf, ly0, lx0, tid);
-  pair_to_lds<P, 4>(rc[1], v1, sh.buf, ly0, lx0, tid);
-  pair_to_lds<P, 4>(rc[2], v2, sh.buf, ly0, lx0, tid);
-  pair_to_lds<P, 1>(rc[3], v3, sh.buf, ly0, lx0, tid);
-  pair_to_lds<P, 4>(rc[4], v4, sh.buf, ly0, lx0, tid);
-  pair_to_lds<P, 4>(rc[5], v5, sh.buf, ly0, lx0, tid);
-  pair_to_lds<P, 4>(rc[6], v6, sh.buf, ly0, lx0, tid);
-  __syncthreads();
-  // the predecessors on the two waves, then V on wave 0
-  if (wave == 0 && has_a) pair_unit<P>(sh, 0, a_lone, a_r, a_c, r, c, C, SW, SH, lane, bd);
-  if (wave == 1 && has_b) pair_unit<P>(sh, 3, false, r - 1, c, r, c, C, SW, SH, lane, bd);
+  pair_to_lds<P, 64, 1>(q0, lq, v0, sh.buf, ly0, lx0, lane, 0, nb8);
+  pair_to_lds<P, 64, NS>(q1, lq, v1, sh.buf, ly0, lx0, lane, 0, nsb);
+  pair_to_lds<P, 64, NS>(q2, lq, v2, sh.buf, ly0, lx0, lane, 0, nsb);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // Each predecessor on the wave that staged its region: A's passes read
+  // rectangles 3 .. 5 only, B's 0 .. 2 (the reads past a line's end that
+  // reach the other wave's rectangles never reach a result).  Then the
+  // samples of the predecessor that this job writes (A's that no
+  // level-(L + 1) unit rewrites: V is A's right successor; B's likewise
+  // when B has no right successor, in the last superblock column), which V
+  // leaves alone, and the wave's half of superblock (r, c + 1).
+  if (wave == 0) {
+    if (has_a) pair_unit<P>(sh, 0, a_lone, a_r, a_c, r, c, C, SW, SH, lane, bd);
+    pair_store<P, 64, 1>(out, stride, LfRect{r * SH - 8, 8, (c - 1) * SW, has_a && !a_lone && r >= 1}, lq, sh.buf,
+                         ly0, lx0, lane);
+    pair_store<P, 64, NS>(out, stride, LfRect{r * SH, SH - (r + 1 < R ? 8 : 0), (c - 1) * SW, has_a && !a_lone}, lq,
+                          sh.buf, ly0, lx0, lane);
+  } else {
+    if (has_b) pair_unit<P>(sh, 3, false, r - 1, c, r, c, C, SW, SH, lane, bd);
+    pair_store<P, 64, 1>(out, stride, LfRect{(r - 1) * SH - 8, 8, c * SW, has_b && !right && r >= 2}, lq, sh.buf,
+                         ly0, lx0, lane);
+    pair_store<P, 64, NS>(out, stride, LfRect{(r - 1) * SH, SH - 8, c * SW, has_b && !right}, lq, sh.buf, ly0, lx0,
+                          lane);
+  }
+  pair_to_lds<P, 64, NS / 2>(rc6, lq, v6, sh.buf, ly0, lx0, lane, wave * half, wave * half + half);
   __syncthreads();
   if (wave == 0) pair_unit<P>(sh, 6, lone, r, c, r, c, C, SW, SH, lane, bd);
   __syncthreads();
-  // the samples this job writes: V's, A's that no level-(L + 1) unit
-  // rewrites (V is A's right successor), and B's likewise when B has no
-  // right successor (last superblock column)
-  LfRect w[7];
-  w[0] = {r * SH - 8, 8, c * SW, lnq, !lone && r >= 1};                                     // V: above (r, c)
-  w[1] = {r * SH, SH, c * SW, lnq, true};                                                     // V: (r, c)
-  w[2] = {r * SH, SH, (c + 1) * SW, lnq, !lone && right};                                      // V: (r, c + 1)
-  w[3] = {r * SH - 8, 8, (c - 1) * SW, lnq, has_a && !a_lone && r >= 1};                       // A: above
-  w[4] = {r * SH, SH - (r + 1 < R ? 8 : 0), (c - 1) * SW, lnq, has_a && !a_lone};              // A: (r, c - 1)
-  w[5] = {(r - 1) * SH - 8, 8, c * SW, lnq, has_b && !right && r >= 2};                        // B: above
-  w[6] = {(r - 1) * SH, SH - 8, c * SW, lnq, has_b && !right};                                 // B: (r - 1, c)
-  pair_store<P, 1>(out, stride, w[0], sh.buf, ly0, lx0, tid);
-  pair_store<P, 4>(out, stride, w[1], sh.buf, ly0, lx0, tid);
-  pair_store<P, 4>(out, stride, w[2], sh.buf, ly0, lx0, tid);
-  pair_store<P, 1>(out, stride, w[3], sh.buf, ly0, lx0, tid);
-  pair_store<P, 4>(out, stride, w[4], sh.buf, ly0, lx0, tid);
-  pair_store<P, 1>(out, stride, w[5], sh.buf, ly0, lx0, tid);
-  pair_store<P, 4>(out, stride, w[6], sh.buf, ly0, lx0, tid);
+  // V's samples: the 8 rows above superblock (r, c), superblocks (r, c)
+  // and (r, c + 1)
+  pair_store<P, 128, 1>(out, stride, LfRect{r * SH - 8, 8, c * SW, !lone && r >= 1}, lq, sh.buf, ly0, lx0, tid);
+  pair_store<P, 128, NS / 2>(out, stride, rc5, lq, sh.buf, ly0, lx0, tid);
+  pair_store<P, 128, NS / 2>(out, stride, rc6, lq, sh.buf, ly0, lx0, tid);
 }
 
 // level L + 1's items: its units, and the lone vertical pass of superblock
@@ -299,13 +318,13 @@ static uint32_t lf_pair_jobs(uint32_t rows, uint32_t cols, uint32_t L) {
   return (uint32_t)(n_units + (t + 1 < R ? 1 : 0));
 }
 
-void launch_lf_pair(hipStream_t s, const LfOne &in, const Planes &out, bool hbd) {
+void launch_lf_pair(hipStream_t s, const LfOne &in, const Planes &out, bool copy_all, bool hbd) {
   const uint32_t C = in.geo & 0xfff, R = (in.geo >> 12) & 0xfff, L = in.t;
-  const uint32_t n_jobs = lf_pair_jobs(R, C, L), wgs = 3 * n_jobs + 3 * R * C;
+  const uint32_t n_jobs = lf_pair_jobs(R, C, L), wgs = 3 * n_jobs + 3 * (copy_all ? R * C : 3 * std::min(R, C));
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(wgs), dim3(128), 0, s, in.p[0], in.p[1], in.p[2], (uint8_t *)out.p[0],
                        (uint8_t *)out.p[1], (uint8_t *)out.p[2], in.stride[0], in.stride[1], in.stride[2], in.dq,
-                       in.geo, L, n_jobs);
+                       in.geo, L, n_jobs, (uint32_t)copy_all);
   };
   if (hbd)
     go(k_lf_pair<true>);
