@@ -978,7 +978,7 @@ def main():
         st = frames[i]["stats"]
         if st is None:
             continue
-        tail = st["lf_tail_launches"] / st["lf_levels"] if st["lf_levels"] else 0.0
+        tail = st["lf_tail_levels"] / st["lf_levels"] if st["lf_levels"] else 0.0
         work["mc_bytes"] += st["mc_bytes"]
         work["itx_bytes"] += st["itx_bytes"]
         work["intra_bytes"] += st["intra_bytes"] + (1.0 - tail) * st["lf_bytes"]

@@ -235,9 +235,11 @@ int vp9r_gpu_prepared_stats(const vp9r_prepared *p, uint64_t out[12]);
 /* The launch schedule of a prepared record run alone (vp9r_gpu_run): [0]
  * fused wavefront launches holding an intra level (with the loop filter
  * levels placed beside them), [1] loop filter launches after the last intra
- * level, [2] the loop filter levels.  Measurement aid; no counterpart in the
+ * level (two levels per launch, the default; [3] of them under
+ * VP9R_LF=levels), [2] the loop filter levels, [3] the loop filter levels
+ * after the last intra level.  Measurement aid; no counterpart in the
  * reference (its filter runs after reconstruction, decoder/mod.rs:355-358). */
-int vp9r_gpu_prepared_schedule(const vp9r_prepared *p, uint32_t out[3]);
+int vp9r_gpu_prepared_schedule(const vp9r_prepared *p, uint32_t out[4]);
 /* Writes the visible frame in the crate's Frame::packed layout (Y, U, V; u8
  * at 8 bits, little-endian u16 above; frame.rs:112-229) to `dst`, which
  * holds vp9r_packed_size() bytes.  Waits for the frame. */

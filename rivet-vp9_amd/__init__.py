@@ -441,9 +441,9 @@ class Prepared:
         keys = ("mc_tiles", "itx4", "itx8", "itx16", "itx32", "intra_groups", "intra_levels", "lf_levels",
                 "mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
         d = dict(zip(keys, list(out)))
-        sched = (ctypes.c_uint32 * 3)()
+        sched = (ctypes.c_uint32 * 4)()
         library().vp9r_gpu_prepared_schedule(self._p, sched)
-        d["wave_launches"], d["lf_tail_launches"] = sched[0], sched[1]
+        d["wave_launches"], d["lf_tail_launches"], d["lf_tail_levels"] = sched[0], sched[1], sched[3]
         return d
 
 

@@ -1023,6 +1023,19 @@ size_t wave_schedule(const Prepared &p, bool recon, bool lf, size_t &n_int, std:
   return std::max(n_int, n_lf ? (size_t)lf_at.back() + 1 : 0);
 }
 
+// The m loop filter levels of a single frame after its intra levels: the
+// first `singles` one per launch in place, then `pairs` launches of two
+// levels each, an even number (they alternate between the frame store and
+// a second copy of its planes, and the last must write the frame store).
+void lf_tail_split(size_t m, size_t &singles, size_t &pairs) {
+  singles = m % 2;
+  pairs = (m - singles) / 2;
+  if (pairs % 2) {
+    pairs--;
+    singles += 2;
+  }
+}
+
 int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool lf, GraphTarget *gt) {
   hipStream_t s = g->stream;
   if (gt && (n_all != 1 || g->timing || !g->intra_blocks)) {
@@ -1315,11 +1328,8 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       size_t pair_from = n_wave;
       vp9g::Planes tmp{};
       if (n == 1 && !gt && lf && g->lf_pairs && n_wave > n_wave_intra) {
-        size_t m = n_wave - n_wave_intra, singles = m % 2, pairs = (m - singles) / 2;
-        if (pairs % 2) {
-          pairs--;
-          singles += 2;
-        }
+        size_t singles, pairs;
+        lf_tail_split(n_wave - n_wave_intra, singles, pairs);
         if (pairs) {
           const vp9r_gpu_frame *cf = f[0]->cur;
           if (cf->bytes > g->lf_tmp.cap) {
@@ -1339,7 +1349,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
           const bool back = (k - pair_from) % 4 == 2;  // lf_tmp to the frame store
           if (back)
             for (int p = 0; p < 3; p++) one.p[p] = (uint8_t *)tmp.p[p];
-          vp9g::launch_lf_pair(s, one, back ? la.cur : tmp, hbd);
+          vp9g::launch_lf_pair(s, one, back ? la.cur : tmp, k == pair_from, hbd);
           k++;
           continue;
         }
@@ -1709,13 +1719,15 @@ int vp9r_gpu_filter_batch(vp9r_gpu *g, uint32_t n, const vp9r_prepared *const *p
   return VP9R_OK;
 }
 
-int vp9r_gpu_prepared_schedule(const vp9r_prepared *pr, uint32_t out[3]) {
-  size_t n_int = 0;
+int vp9r_gpu_prepared_schedule(const vp9r_prepared *pr, uint32_t out[4]) {
+  size_t n_int = 0, singles = 0, pairs = 0;
   std::vector<int> lf_at;
   const size_t n_wave = vp9r::wave_schedule(pr->p, true, pr->p.hdr.lf_level != 0, n_int, lf_at);
+  vp9r::lf_tail_split(n_wave - n_int, singles, pairs);
   out[0] = (uint32_t)n_int;
-  out[1] = (uint32_t)(n_wave - n_int);
+  out[1] = (uint32_t)(singles + pairs);
   out[2] = (uint32_t)lf_at.size();
+  out[3] = (uint32_t)(n_wave - n_int);
   return VP9R_OK;
 }
 
