@@ -79,9 +79,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 STAGES = ("inter_pred", "inter_residual", "intra", "loop_filter")
 # intra: the loop filter decisions and the fused launches holding an intra
 # level (the loop filter levels placed beside them run there too);
-# loop_filter: the filter levels after the last intra level (two per
-# k_lf_pair launch, the first one or three one per k_lf launch)
-STAGE_KERNELS = ("k_mc", "k_itx_inter", "k_lf_mask + k_wave", "k_lf_pair + k_lf")
+# loop_filter: the filter levels after the last intra level (several per
+# k_lf_cone launch, a few one per k_lf launch)
+STAGE_KERNELS = ("k_mc", "k_itx_inter", "k_lf_mask + k_wave", "k_lf_cone + k_lf")
 STAGE_BYTES = ("mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
 
 
@@ -1003,7 +1003,7 @@ def main():
     dom = max(range(4), key=lambda s: stage_ms[s])
     dname = STAGES[dom]
     achieved = stages[dname]["GB_per_s"] or 0.0
-    kern = {"loop_filter": "vp9g::k_lf_pair<false>", "inter_pred": "vp9g::k_mc<false>",
+    kern = {"loop_filter": "vp9g::k_lf_cone<false, 3>", "inter_pred": "vp9g::k_mc<false>",
             "inter_residual": "vp9g::k_itx_inter<false>"}.get(dname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profile_traffic(kern) if kern else None,

@@ -27,7 +27,7 @@ HIP_SOURCES = [
     "csrc/gpu/mc.hip",
     "csrc/gpu/intra.hip",
     "csrc/gpu/lf.hip",
-    "csrc/gpu/lf_pair.hip",
+    "csrc/gpu/lf_cone.hip",
     "csrc/gpu/frame.hip",
     "csrc/gpu/pixel.hip",
     "csrc/host/recon.cpp",
@@ -53,7 +53,7 @@ GPU = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packe
 FILE_FLAGS = {
     "csrc/gpu/lf.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
     "csrc/gpu/intra.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],  # k_wave runs the loop filter too
-    "csrc/gpu/lf_pair.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
+    "csrc/gpu/lf_cone.hip": ["-mllvm", "-disable-promote-alloca-to-vector"],
 }
 
 

@@ -272,10 +272,13 @@ void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n,
 // (else p[0] null: each workgroup finds its frame through the range table)
 void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
                      const LfOne &one);
-// Levels one.t and one.t + 1 of one frame in one launch (lf_pair.hip),
-// from the planes one.p into the planes `out` (same geometry, every sample
-// of the superblock grid written).
-void launch_lf_pair(hipStream_t s, const LfOne &one, const Planes &out, bool hbd);
+// Levels one.t .. one.t + depth - 1 of one frame in one launch
+// (lf_cone.hip; depth 2 or 3, 4 at 8 bits), from the planes one.p (the
+// frame after level one.t - 1) into the planes `out` (same geometry), so
+// that `out` holds the frame after level one.t + depth - 1: prev = 0 when
+// `out` holds nothing of the frame yet, else the depth of the previous
+// launch, which read `out` (the frame after level one.t - prev - 1).
+void launch_lf_cone(hipStream_t s, const LfOne &one, const Planes &out, int depth, int prev, bool hbd);
 
 // Per-kernel seam (vp9r.h batched forms).
 void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,

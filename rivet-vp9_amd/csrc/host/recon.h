@@ -118,6 +118,16 @@ void gpu_release(vp9r_gpu *g);
 hipError_t seam_begin(vp9r_gpu *g);
 hipError_t seam_end(vp9r_gpu *g);
 
+// The launches of a single frame's m loop filter levels after its intra
+// levels: `singles` one-level launches, then launches of depths[i] levels
+// (an even number of them), at most max_depth each.
+constexpr int kLfDepthDefault = 3;
+struct LfPlan {
+  size_t singles = 0;
+  std::vector<int> depths;
+};
+LfPlan lf_tail_plan(size_t m, int max_depth);
+
 }  // namespace vp9r
 
 struct vp9r_gpu_frame {
@@ -166,9 +176,9 @@ struct vp9r_gpu {
   // host work lists, reused
   vp9r::Builder work;
   bool intra_blocks = true;  // intra as leveled blocks (else (superblock, plane) groups)
-  // a single frame's loop filter levels after its intra levels two per
-  // launch (else one), through a second copy of the frame's planes
-  bool lf_pairs = true;
+  // a single frame's loop filter levels after its intra levels up to
+  // lf_depth per launch, through a second copy of the frame's planes
+  int lf_depth = vp9r::kLfDepthDefault;
   vp9r::DevBuf lf_tmp;
   // batched launches
   std::vector<uint8_t> batch_stage;
