@@ -485,9 +485,19 @@ def micro_bench(vp, gpu, check, reps=5, tune=True):
     if tune:
         t0 = time.perf_counter()
         choice, times = gpu.autotune()
+        names = {f"{k}x{k}": None for k in (4, 8, 16, 32)}
+        itx_us = {}
+        for c, cls in enumerate(("8bit", "hbd")):
+            for k, size in enumerate(names):
+                row = times[(c * 4 + k) * 12:(c * 4 + k + 1) * 12]
+                itx_us[f"{cls} {size}"] = {vp.Gpu.shape_name(
+                    (1, 2, 4)[j % 3] | (vp.SHAPE_REG if (j // 3) % 2 else 0) | (vp.SHAPE_PERSISTENT if j // 6 else 0)):
+                    round(t, 1) for j, t in enumerate(row) if t >= 0}
         out["autotune"] = {"choices": choice, "seconds": round(time.perf_counter() - t0, 2),
-                           "itx_us_by_class_size_waves_1_2_4": [round(t, 1) for t in times[:24]],
-                           "mc_us_by_class_rows_32_16": [round(t, 1) for t in times[24:26]]}
+                           "chosen_itx_shapes": {"8bit": [vp.Gpu.shape_name(c) for c in choice["itx_shape_8bit"]],
+                                                 "hbd": [vp.Gpu.shape_name(c) for c in choice["itx_shape_hbd"]]},
+                           "itx_us_by_class_size_shape": itx_us,
+                           "mc_us_by_class_rows_32_16": [round(t, 1) for t in times[96:100]]}
         _log(f"bench: autotune {choice}")
 
     t0 = time.perf_counter()

@@ -385,19 +385,26 @@ int vp9r_lf_edges_batch(vp9r_gpu *g, uint32_t count, const uint8_t *edges, const
 
 /* Autotuner of the batched forms' launch shapes (no counterpart in the
  * reference: src/dsp/mod.rs:32-96 chooses its kernels by CPU feature).  It
- * times, on synthetic batches, 1, 2 and 4 waves per workgroup for the
- * inverse transforms of each size (4x4 .. 32x32) and bit depth class
- * (8-bit, 10/12-bit), and tiles of 16 and 32 rows for motion compensation
- * per bit depth class, and keeps the fastest of each for later calls on
- * this context (the arithmetic, and so every output, is the same for every
- * shape).  times_us (may be NULL) gets the 26 candidate times: transforms
- * [class][size][waves 1, 2, 4], then motion compensation [class][rows 32,
- * 16].  Takes a few hundred milliseconds; allocates its own inputs. */
-int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[26]);
-/* The shapes in use: [0..3] waves per transform workgroup at 8 bits for
- * 4x4 .. 32x32, [4..7] the same at 10/12 bits, [8..9] motion compensation
- * tile rows at 8 and at 10/12 bits, [10] 1 if vp9r_gpu_autotune has run.
- * vp9r_gpu_set_tuning sets [0..9] (waves 1, 2 or 4; rows 16 or 32). */
+ * times, on synthetic batches, every transform launch shape of each size
+ * (4x4 .. 32x32) and bit depth class (8-bit, 10/12-bit): 1, 2 or 4 waves per
+ * workgroup, the row-to-column transposes through LDS or in registers
+ * (DPP / ds_swizzle lane exchanges, 4x4 .. 16x16), and one workgroup per
+ * 64 W / n blocks or persistent (resident) workgroups looping over them;
+ * and tiles of 16 and 32 rows for motion compensation per bit depth class.
+ * It keeps the fastest of each for later calls on this context (the
+ * arithmetic, and so every output, is the same for every shape).  times_us
+ * (may be NULL) gets VP9R_TUNE_TIMES candidate times, -1 for a shape a size
+ * lacks: transforms [class][size][persistent 0/1][registers 0/1][waves 1, 2,
+ * 4] (96), then motion compensation [class][rows 32, 16].  Takes under a
+ * second; allocates its own inputs. */
+#define VP9R_TUNE_TIMES 100
+int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[VP9R_TUNE_TIMES]);
+/* The shapes in use: [0..3] the transform shape at 8 bits for 4x4 ..
+ * 32x32, [4..7] the same at 10/12 bits, [8..9] motion compensation tile
+ * rows at 8 and at 10/12 bits, [10] 1 if vp9r_gpu_autotune has run.  A
+ * transform shape is its waves per workgroup (1, 2 or 4) + 16 for register
+ * transposes (4x4 .. 16x16 only) + 32 for persistent workgroups.
+ * vp9r_gpu_set_tuning sets [0..9]. */
 int vp9r_gpu_tuning(const vp9r_gpu *g, uint32_t out[11]);
 int vp9r_gpu_set_tuning(vp9r_gpu *g, const uint32_t in[10]);
 

@@ -303,6 +303,11 @@ def ivf_frames(data):
     return out
 
 
+# transform launch shape flags (vp9r_gpu_set_tuning)
+SHAPE_REG = 16
+SHAPE_PERSISTENT = 32
+
+
 class Gpu:
     """A GPU reconstruction context on one HIP device (vp9r_gpu_*)."""
 
@@ -326,21 +331,34 @@ class Gpu:
     def _refs(refs):
         return (ctypes.c_void_p * 3)(*[(r.handle if r is not None else None) for r in refs])
 
+    TUNE_TIMES = 100
+
     def autotune(self):
-        """Runs the launch-shape autotuner; returns (choices, candidate times in us)."""
-        t = (ctypes.c_float * 26)()
+        """Runs the launch-shape autotuner; returns (choices, candidate times
+        in us: transforms [class][size][persistent][registers][waves 1, 2, 4],
+        then MC [class][rows 32, 16]; -1 where a size lacks the shape)."""
+        t = (ctypes.c_float * self.TUNE_TIMES)()
         _check(library().vp9r_gpu_autotune(self._g, t))
         return self.tuning(), list(t)
+
+    @staticmethod
+    def shape_name(code):
+        """A transform launch shape code as text: waves, transposes, grid."""
+        return (f"{code & 15}w" + ("+reg" if code & SHAPE_REG else "+lds") +
+                ("+persistent" if code & SHAPE_PERSISTENT else ""))
 
     def tuning(self):
         out = (u32 * 11)()
         _check(library().vp9r_gpu_tuning(self._g, out))
         v = list(out)
-        return {"itx_waves_8bit": v[0:4], "itx_waves_hbd": v[4:8], "mc_tile_rows_8bit": v[8],
+        return {"itx_shape_8bit": v[0:4], "itx_shape_hbd": v[4:8], "mc_tile_rows_8bit": v[8],
                 "mc_tile_rows_hbd": v[9], "tuned": bool(v[10])}
 
-    def set_tuning(self, itx_waves_8bit, itx_waves_hbd, mc_rows_8bit, mc_rows_hbd):
-        arr = (u32 * 10)(*(list(itx_waves_8bit) + list(itx_waves_hbd) + [mc_rows_8bit, mc_rows_hbd]))
+    def set_tuning(self, itx_shape_8bit, itx_shape_hbd, mc_rows_8bit, mc_rows_hbd):
+        """Transform shapes per size 4x4 .. 32x32: waves per workgroup (1, 2
+        or 4) | SHAPE_REG (register transposes, up to 16x16) |
+        SHAPE_PERSISTENT; MC tile rows 16 or 32."""
+        arr = (u32 * 10)(*(list(itx_shape_8bit) + list(itx_shape_hbd) + [mc_rows_8bit, mc_rows_hbd]))
         _check(library().vp9r_gpu_set_tuning(self._g, arr))
 
     def graph(self, prepared, refs, cur):

@@ -60,6 +60,15 @@ DEV void gst(T *p, const T &v) {
 
 namespace vp9g {
 
+// Orders a wave's own LDS accesses across lanes without a workgroup barrier,
+// for LDS that one wave alone uses: a wave's LDS instructions execute in
+// program order, and the fence keeps the compiler from moving accesses
+// across the point.
+DEV void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ---- integer lanes
 
 struct Lane32 {

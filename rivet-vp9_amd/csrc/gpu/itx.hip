@@ -3,7 +3,10 @@
 // n0 x n0, one lane per row then per column (src/dsp/itx.rs:701-749 with the
 // dequantisation of src/decoder/recon.rs:233-281).  Inter blocks' residuals
 // are independent of each other, so a whole frame's are one launch per size.
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "itx_dev.h"
 #include "launch.h"
@@ -174,58 +177,161 @@ __global__ __launch_bounds__(64) void k_itx_inter(const ItxArgs *fa, const Range
     itx_inter_wg<2, HBD>(fa, r4, nf, w - b16, tiles);
 }
 
-// Per-kernel seam: dense dequantised coefficients per block.  A team of n0
-// lanes takes a block: lane k transforms row k straight from memory, the
-// columns go through the LDS tile, the column residuals go back into the
-// tile, and lane k adds row k of them to the destination (one dword access
-// per 4 bytes of the row).
-// W waves per workgroup (the autotuner's choice per size and bit depth class).
-template <int n, bool HBD, int W>
-__global__ __launch_bounds__(64 * W) void k_itx_dense(const int32_t *coefs, const uint32_t *coef_off,
-                                                  const uint8_t *tx_type, const uint16_t *eob,
-                                                  const uint64_t *dst_off, void *dst_base,
-                                                  uint32_t stride, int bd, int lossless,
-                                                  const uint32_t *jobs, uint32_t n_jobs) {
+// ---- per-kernel seam: dense dequantised coefficients per block
+//
+// A team of n0 lanes takes a block: lane k transforms row k straight from
+// memory, the rows become columns, lane k transforms column k, the columns
+// become rows again, and lane k adds row k of the residuals to the
+// destination (one dword access per 4 bytes of the row).  Two ways to turn
+// rows into columns (the autotuner picks per size class):
+//   - through an LDS tile of the team's own (row stride n0 + 1), ordered by
+//     the wave alone (each team lies inside one wave);
+//   - in registers (team_transpose): log2(n0) rounds of lane exchanges,
+//     DPP quad permutations and row rotations or ds_swizzle, no LDS.
+
+// v from lane (this lane ^ K), K < 32: DPP within quads (1, 2) and rows (8:
+// a rotation by 8 of a 16-lane row), ds_swizzle's bit mode elsewhere.
+template <int K>
+DEV int32_t lane_xor32(int32_t v) {
+  if constexpr (K == 1)
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1, 0, 3, 2]
+  else if constexpr (K == 2)
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2, 3, 0, 1]
+  else if constexpr (K == 8)
+    return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, true);  // row_ror 8
+  else
+    return __builtin_amdgcn_ds_swizzle(v, 0x1F | (K << 10));  // and 0x1f, or 0, xor K
+}
+template <int K, class T>
+DEV T lane_xor(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return (T)lane_xor32<K>((int32_t)v);
+  } else {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)lane_xor32<K>((int32_t)(uint32_t)u);
+    const uint32_t hi = (uint32_t)lane_xor32<K>((int32_t)(uint32_t)(u >> 32));
+    return (T)((uint64_t)hi << 32 | lo);
+  }
+}
+
+// One round of team_transpose: element j and lane l trade bit K where they
+// differ, so lane l gives its elements whose bit K differs from its own to
+// lane l ^ K and takes that lane's in their place.  m is all ones on lanes
+// with bit K set, else zero: each choice is one bitfield select (v_bfi_b32)
+// per dword.
+template <class T>
+DEV T pick(uint32_t m, T a, T b) {  // m ? a : b
+  if constexpr (sizeof(T) == 4) {
+    return (T)((m & (uint32_t)a) | (~m & (uint32_t)b));
+  } else {
+    const uint64_t mm = (uint64_t)m << 32 | m;
+    return (T)((mm & (uint64_t)a) | (~mm & (uint64_t)b));
+  }
+}
+template <int K, int n0, class T>
+DEV void transpose_round(T (&t)[n0], uint32_t m) {
+  T send[n0 / 2], recv[n0 / 2];
+#pragma unroll
+  for (int j = 0, p = 0; j < n0; j++)
+    if (!(j & K)) send[p++] = pick(m, t[j], t[j | K]);
+#pragma unroll
+  for (int p = 0; p < n0 / 2; p++) recv[p] = lane_xor<K>(send[p]);
+#pragma unroll
+  for (int j = 0, p = 0; j < n0; j++)
+    if (!(j & K)) {
+      t[j] = pick(m, recv[p], t[j]);
+      t[j | K] = pick(m, t[j | K], recv[p]);
+      p++;
+    }
+}
+
+// Lane l of an aligned team of n0 lanes holds row l of an n0 x n0 matrix in
+// t; afterwards it holds column l.  Every lane of the wave must take part.
+template <int n0, class T>
+DEV void team_transpose(T (&t)[n0], int lane) {
+  auto mask = [&](int k) { return (lane & k) ? ~0u : 0u; };
+  transpose_round<1>(t, mask(1));
+  if constexpr (n0 > 2) transpose_round<2>(t, mask(2));
+  if constexpr (n0 > 4) transpose_round<4>(t, mask(4));
+  if constexpr (n0 > 8) transpose_round<8>(t, mask(8));
+  if constexpr (n0 > 16) transpose_round<16>(t, mask(16));
+}
+
+struct ItxDenseArgs {
+  const int32_t *coefs;
+  const uint32_t *coef_off;
+  const uint8_t *tx_type;
+  const uint16_t *eob;
+  const uint64_t *dst_off;
+  void *dst;
+  const uint32_t *jobs;
+  uint32_t stride, n_jobs;
+  int bd, lossless;
+};
+
+// Block jobs[job] by the team that holds `lane` (its index in the team).
+// REG: the transposes in registers (every lane of the wave calls this);
+// else through `tile`, the team's n0 x (n0 + 1) LDS tile.
+template <int n, bool HBD, bool REG>
+DEV void itx_dense_block(const ItxDenseArgs &a, uint32_t job, int lane,
+                         typename std::conditional<HBD, Lane64, Lane32>::type::T *tile) {
   typedef typename std::conditional<HBD, Lane64, Lane32>::type L;
   typedef typename L::T T;
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  constexpr int n0 = 1 << n, BPW = 64 * W / n0, S = n0 + 1;
-  __shared__ T tiles[BPW][n0 * S];
-  const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
-  const uint32_t job = xcd_order(blockIdx.x, gridDim.x) * BPW + team;
-  const bool active = job < n_jobs;
-  T *tile = tiles[team];
-  const uint32_t i = active ? jobs[job] : 0;
-  const int tt = active ? tx_type[i] : 0;
-  const int32_t *c = coefs + (active ? coef_off[i] : 0);
-  const bool dc_only = !HBD && active && eob[i] == 1 && tt == DCT_DCT && !lossless;
+  constexpr int n0 = 1 << n, S = n0 + 1;
+  const bool active = job < a.n_jobs;
+  const uint32_t i = active ? a.jobs[job] : 0;
+  const int tt = active ? a.tx_type[i] : 0;
+  const int32_t *c = a.coefs + (active ? a.coef_off[i] : 0);
+  const bool dc_only = !HBD && active && a.eob[i] == 1 && tt == DCT_DCT && !a.lossless;
+  const bool full = active && !dc_only;
   T t[n0];
-  if (active && !dc_only) {
+  if (full) {
     bool zero = true;
 #pragma unroll
     for (int k = 0; k < n0; k++) {
       t[k] = (T)c[lane * n0 + k];
       zero = zero && t[k] == 0;
     }
-    if (!zero || lossless) tx1d<L, n>(t, (tt & 2) == 0, lossless != 0, 2);
+    if (!zero || a.lossless) tx1d<L, n>(t, (tt & 2) == 0, a.lossless != 0, 2);
+  } else {
 #pragma unroll
-    for (int k = 0; k < n0; k++) tile[lane * S + k] = t[k];
+    for (int k = 0; k < n0; k++) t[k] = 0;
   }
-  __syncthreads();
-  const int max = (1 << bd) - 1;
-  if (active && !dc_only) {
-    // column `lane`; only this lane reads or writes it until the barrier
+  if constexpr (REG) {
+    team_transpose<n0>(t, lane);
+  } else {
+    if (full)
 #pragma unroll
-    for (int k = 0; k < n0; k++) t[k] = tile[k * S + lane];
-    tx1d<L, n>(t, (tt & 1) == 0, lossless != 0, 0);
+      for (int k = 0; k < n0; k++) tile[lane * S + k] = t[k];
+    wave_lds_sync();
+    if (full)
+#pragma unroll
+      for (int k = 0; k < n0; k++) t[k] = tile[k * S + lane];
+  }
+  if (full) {
+    tx1d<L, n>(t, (tt & 1) == 0, a.lossless != 0, 0);
     constexpr int shift = n + 2 < 6 ? n + 2 : 6;
 #pragma unroll
     for (int k = 0; k < n0; k++) {
-      const T v = lossless ? t[k] : L::rnd(t[k], shift);
-      tile[k * S + lane] = (T)lclamp((int64_t)v, -65536, 65536);
+      const T v = a.lossless ? t[k] : L::rnd(t[k], shift);
+      t[k] = (T)lclamp((int64_t)v, -65536, 65536);
     }
   }
-  __syncthreads();
+  if constexpr (REG) {
+    team_transpose<n0>(t, lane);
+  } else {
+    // column `lane` back into the tile (only this lane touched it since the
+    // last sync), then row `lane` out
+    if (full)
+#pragma unroll
+      for (int k = 0; k < n0; k++) tile[k * S + lane] = t[k];
+    wave_lds_sync();
+    if (full)
+#pragma unroll
+      for (int k = 0; k < n0; k++) t[k] = tile[lane * S + k];
+    wave_lds_sync();  // the tile's next use writes it
+  }
   if (active) {
     int32_t r[n0];
     if (dc_only) {
@@ -234,11 +340,46 @@ __global__ __launch_bounds__(64 * W) void k_itx_dense(const int32_t *coefs, cons
       for (int k = 0; k < n0; k++) r[k] = v;
     } else {
 #pragma unroll
-      for (int k = 0; k < n0; k++) r[k] = (int32_t)tile[lane * S + k];
+      for (int k = 0; k < n0; k++) r[k] = (int32_t)t[k];
     }
-    P *row = (P *)dst_base + dst_off[i] + (size_t)lane * stride;
-    add_row<P, n0>(row, r, HBD ? max : 255);
+    P *row = (P *)a.dst + a.dst_off[i] + (size_t)lane * a.stride;
+    add_row<P, n0>(row, r, HBD ? (1 << a.bd) - 1 : 255);
   }
+}
+
+template <int n, bool HBD, bool REG, int W>
+struct ItxDenseShared {
+  typedef typename std::conditional<HBD, Lane64, Lane32>::type::T T;
+  static constexpr int n0 = 1 << n, TEAMS = REG ? 1 : 64 * W / n0, TILE = REG ? 1 : n0 * (n0 + 1);
+  T tiles[TEAMS][TILE];
+};
+
+// One workgroup of W waves per 64 W / n0 blocks, in XCD-contiguous runs
+// (xcd_order).
+template <int n, bool HBD, int W, bool REG>
+__global__ __launch_bounds__(64 * W) void k_itx_dense(ItxDenseArgs a) {
+  constexpr int n0 = 1 << n, BPW = 64 * W / n0;
+  __shared__ ItxDenseShared<n, HBD, REG, W> sh;
+  const int team = threadIdx.x / n0, lane = threadIdx.x % n0;
+  const uint32_t job = xcd_order(blockIdx.x, gridDim.x) * BPW + team;
+  itx_dense_block<n, HBD, REG>(a, job, lane, sh.tiles[REG ? 0 : team]);
+}
+
+// Persistent form: as many workgroups as stay resident at once (a multiple
+// of 8), each wave looping over units of 64 / n0 blocks.  The units are cut
+// into 8 contiguous segments, segment x for the workgroups the hardware
+// places on XCD x (blockIdx % 8 == x), whose waves take its units in turn,
+// so that the blocks in flight on one XCD are neighbours (one L2).
+template <int n, bool HBD, int W, bool REG>
+__global__ __launch_bounds__(64 * W) void k_itx_dense_p(ItxDenseArgs a) {
+  constexpr int n0 = 1 << n, BPV = 64 / n0;
+  __shared__ ItxDenseShared<n, HBD, REG, W> sh;
+  const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63, team = l64 / n0, lane = l64 % n0;
+  const uint32_t units = (a.n_jobs + BPV - 1) / BPV;
+  const uint32_t x = blockIdx.x & 7, wx = (blockIdx.x >> 3) * W + wave, nw = (gridDim.x >> 3) * W;
+  const uint32_t lo = (uint32_t)((uint64_t)units * x / 8), hi = (uint32_t)((uint64_t)units * (x + 1) / 8);
+  for (uint32_t u = lo + wx; u < hi; u += nw)
+    itx_dense_block<n, HBD, REG>(a, u * BPV + team, lane, sh.tiles[REG ? 0 : wave * BPV + team]);
 }
 
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4], uint32_t nf,
@@ -251,44 +392,76 @@ void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4],
     hipLaunchKernelGGL(k_itx_inter<false>, dim3(total), dim3(64), 0, s, fa, r[3], r[2], r[1], r[0], nf, b4, b8, b16);
 }
 
-template <int n, int W>
-static void itx_dense_nw(hipStream_t s, const int32_t *coefs, const uint32_t *coef_off, const uint8_t *tx_type,
-                         const uint16_t *eob, const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bd,
-                         int lossless, const uint32_t *jobs, uint32_t n_jobs) {
-  constexpr int BPW = (64 * W) >> n;
-  dim3 grid((n_jobs + BPW - 1) / BPW);
-  if (bd > 8)
-    hipLaunchKernelGGL((k_itx_dense<n, true, W>), grid, dim3(64 * W), 0, s, coefs, coef_off, tx_type, eob,
-                       dst_off, dst, stride, (int)bd, lossless, jobs, n_jobs);
-  else
-    hipLaunchKernelGGL((k_itx_dense<n, false, W>), grid, dim3(64 * W), 0, s, coefs, coef_off, tx_type, eob,
-                       dst_off, dst, stride, (int)bd, lossless, jobs, n_jobs);
+// The resident workgroups of a kernel on this device (occupancy x CUs,
+// rounded up to a multiple of 8), kept per kernel and device.
+template <class K>
+static uint32_t resident_grid(K kern, int threads) {
+  static std::mutex mu;
+  static std::map<std::pair<const void *, int>, uint32_t> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_pair((const void *)kern, dev);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  const uint32_t g = ((uint32_t)(per_cu * cus) + 7) & ~7u;
+  cache[key] = g;
+  return g;
 }
 
-// waves: 1, 2 or 4 per workgroup (4 only where the workgroup's tiles fit
-// 64 KB of LDS: 32 x 32 at 10/12 bits takes 2).
-template <int n>
-static void itx_dense_n(hipStream_t s, const int32_t *coefs, const uint32_t *coef_off, const uint8_t *tx_type,
-                        const uint16_t *eob, const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bd,
-                        int lossless, const uint32_t *jobs, uint32_t n_jobs, int waves) {
-  if (waves >= 4 && !(n == 5 && bd > 8))
-    itx_dense_nw<n, 4>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bd, lossless, jobs, n_jobs);
+template <int n, bool HBD, int W, bool REG>
+static void itx_dense_go(hipStream_t s, const ItxDenseArgs &a, bool persistent) {
+  constexpr int BPW = (64 * W) >> n;
+  const uint32_t wgs = (a.n_jobs + BPW - 1) / BPW;
+  if (persistent) {
+    const uint32_t g = std::min(resident_grid(k_itx_dense_p<n, HBD, W, REG>, 64 * W), (wgs + 7) & ~7u);
+    hipLaunchKernelGGL((k_itx_dense_p<n, HBD, W, REG>), dim3(g), dim3(64 * W), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_itx_dense<n, HBD, W, REG>), dim3(wgs), dim3(64 * W), 0, s, a);
+  }
+}
+
+// shape: waves per workgroup (1, 2 or 4; 4 only where the workgroup's
+// tiles fit 64 KB of LDS: 32x32 at 10/12 bits takes 2) | kItxShapeReg
+// (transposes in registers: sizes 4x4 .. 16x16) | kItxShapePersistent.
+template <int n, bool HBD>
+static void itx_dense_n(hipStream_t s, const ItxDenseArgs &a, int shape) {
+  const int waves = shape & 7;
+  const bool reg = (shape & kItxShapeReg) != 0 && n <= 4, pers = (shape & kItxShapePersistent) != 0;
+  if (reg) {
+    if constexpr (n <= 4) {
+      if (waves >= 4)
+        itx_dense_go<n, HBD, 4, true>(s, a, pers);
+      else if (waves >= 2)
+        itx_dense_go<n, HBD, 2, true>(s, a, pers);
+      else
+        itx_dense_go<n, HBD, 1, true>(s, a, pers);
+    }
+    return;
+  }
+  if (waves >= 4 && !(n == 5 && HBD))
+    itx_dense_go<n, HBD, (n == 5 && HBD) ? 2 : 4, false>(s, a, pers);
   else if (waves >= 2)
-    itx_dense_nw<n, 2>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bd, lossless, jobs, n_jobs);
+    itx_dense_go<n, HBD, 2, false>(s, a, pers);
   else
-    itx_dense_nw<n, 1>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bd, lossless, jobs, n_jobs);
+    itx_dense_go<n, HBD, 1, false>(s, a, pers);
 }
 
 void launch_itx_dense(hipStream_t s, uint32_t, const int32_t *coefs, const uint32_t *coef_off,
                       const uint8_t *, const uint8_t *tx_type, const uint16_t *eob,
                       const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
-                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs, int waves) {
+                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs, int shape) {
   if (!n_jobs) return;
+  const ItxDenseArgs a{coefs, coef_off, tx_type, eob, dst_off, dst, jobs, stride, n_jobs, (int)bit_depth, lossless};
+  const bool hbd = bit_depth > 8;
   switch (tx_size) {
-    case 0: itx_dense_n<2>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
-    case 1: itx_dense_n<3>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
-    case 2: itx_dense_n<4>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
-    default: itx_dense_n<5>(s, coefs, coef_off, tx_type, eob, dst_off, dst, stride, bit_depth, lossless, jobs, n_jobs, waves); break;
+    case 0: hbd ? itx_dense_n<2, true>(s, a, shape) : itx_dense_n<2, false>(s, a, shape); break;
+    case 1: hbd ? itx_dense_n<3, true>(s, a, shape) : itx_dense_n<3, false>(s, a, shape); break;
+    case 2: hbd ? itx_dense_n<4, true>(s, a, shape) : itx_dense_n<4, false>(s, a, shape); break;
+    default: hbd ? itx_dense_n<5, true>(s, a, shape) : itx_dense_n<5, false>(s, a, shape); break;
   }
 }
 

@@ -281,10 +281,13 @@ void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n
 void launch_lf_cone(hipStream_t s, const LfOne &one, const Planes &out, int depth, int prev, bool hbd);
 
 // Per-kernel seam (vp9r.h batched forms).
+// A transform launch shape: waves per workgroup (1, 2 or 4) | the flags.
+constexpr int kItxShapeReg = 16;         // rows to columns in registers (4x4 .. 16x16), not LDS
+constexpr int kItxShapePersistent = 32;  // resident workgroups looping over the blocks
 void launch_itx_dense(hipStream_t s, uint32_t count, const int32_t *coefs, const uint32_t *coef_off,
                       const uint8_t *tx, const uint8_t *tx_type, const uint16_t *eob,
                       const uint64_t *dst_off, void *dst, uint32_t stride, uint32_t bit_depth,
-                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs, int waves = 1);
+                      int lossless, int tx_size, const uint32_t *jobs, uint32_t n_jobs, int shape = 1);
 // Copies the interpolation taps to the current device if not yet there (the
 // MC launches do it themselves; a graph capture calls it first).
 void mc_prepare();

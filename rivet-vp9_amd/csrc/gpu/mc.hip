@@ -82,12 +82,7 @@ struct McShared {
 // pass column-major (column c holds rows 0 .. MAXH + 7 in HP dwords).
 // Each wave of an MC workgroup works on a tile of its own, in LDS of its
 // own: the tile's LDS exchanges need the wave's own accesses ordered, not a
-// workgroup barrier (a wave's LDS instructions execute in program order; the
-// fence keeps the compiler from moving accesses across the point).
-DEV void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
+// workgroup barrier (wave_lds_sync).
 
 constexpr int FP8 = 10;
 template <int MAXH>

@@ -134,7 +134,7 @@ int vp9r_itx_add_batch(vp9r_gpu *g, uint32_t count, const int32_t *coefs, const 
   for (int k = 3; k >= 0; k--)
     vp9g::launch_itx_dense(g->stream, count, coefs, coef_off, nullptr, tx_type, eob, dst_off, dst, stride,
                            bit_depth, lossless, k, dj + start[k], (uint32_t)jobs[k].size(),
-                           g->tune_itx_waves[bit_depth > 8][k]);
+                           g->tune_itx_shape[bit_depth > 8][k]);
   TRY(hipGetLastError(), "itx launch");
   TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
   TRY(hipStreamSynchronize(g->stream), "itx sync");
@@ -443,17 +443,34 @@ int vp9r_intra_predict_host(vp9r_gpu *g, uint32_t mode, uint32_t log2, const int
 // For each class it times every candidate shape on a synthetic batch of that
 // class (best of three, device time between events around the launch) and
 // keeps the fastest; the arithmetic is the same for every shape.
-//   transforms: 1, 2 or 4 waves per workgroup, per size (4x4 .. 32x32) and
-//   bit depth class, on 32768 blocks of dense random coefficients;
+//   transforms, per size (4x4 .. 32x32) and bit depth class, on 262144
+//   (4x4) .. 32768 (32x32) blocks of dense random coefficients: 1, 2
+//   or 4 waves per workgroup, the transposes through LDS or in registers
+//   (4x4 .. 16x16), one workgroup per 64 W / n blocks or resident
+//   (persistent) workgroups;
 //   motion compensation: tiles of 16 or 32 rows per bit depth class, on 8192
 //   blocks of 16x16 .. 64x64 from a random 1920x1080 reference.
 // There is no counterpart in the reference (src/dsp/mod.rs:32-96 picks its
 // kernels by CPU feature, and src/dsp/bench.rs times them by hand).
 
+static bool itx_shape_ok(uint32_t v, int size) {
+  const uint32_t w = v & 15, flags = v & ~15u;
+  if (w != 1 && w != 2 && w != 4) return false;
+  if (flags & ~(uint32_t)(vp9g::kItxShapeReg | vp9g::kItxShapePersistent)) return false;
+  return !((flags & vp9g::kItxShapeReg) && size > 2);
+}
+
+// candidate c of the 12 per size: (persistent, registers, waves 1/2/4)
+static int itx_candidate(int c) {
+  static const int w[3] = {1, 2, 4};
+  return w[c % 3] | ((c / 3) % 2 ? vp9g::kItxShapeReg : 0) | (c / 6 ? vp9g::kItxShapePersistent : 0);
+}
+
 int vp9r_gpu_set_tuning(vp9r_gpu *g, const uint32_t in[10]) {
   for (int i = 0; i < 8; i++)
-    if (in[i] != 1 && in[i] != 2 && in[i] != 4) {
-      set_error("transform waves per workgroup: 1, 2 or 4");
+    if (!itx_shape_ok(in[i], i % 4)) {
+      set_error("transform shape: 1, 2 or 4 waves per workgroup, plus 16 (register transposes, 4x4 .. 16x16) "
+                "and 32 (persistent)");
       return VP9R_ERR_INVALID_INPUT;
     }
   for (int i = 8; i < 10; i++)
@@ -462,7 +479,7 @@ int vp9r_gpu_set_tuning(vp9r_gpu *g, const uint32_t in[10]) {
       return VP9R_ERR_INVALID_INPUT;
     }
   for (int c = 0; c < 2; c++) {
-    for (int k = 0; k < 4; k++) g->tune_itx_waves[c][k] = (uint8_t)in[4 * c + k];
+    for (int k = 0; k < 4; k++) g->tune_itx_shape[c][k] = (uint8_t)in[4 * c + k];
     g->tune_mc_rows[c] = (uint8_t)in[8 + c];
   }
   return VP9R_OK;
@@ -470,14 +487,14 @@ int vp9r_gpu_set_tuning(vp9r_gpu *g, const uint32_t in[10]) {
 
 int vp9r_gpu_tuning(const vp9r_gpu *g, uint32_t out[11]) {
   for (int c = 0; c < 2; c++) {
-    for (int k = 0; k < 4; k++) out[4 * c + k] = g->tune_itx_waves[c][k];
+    for (int k = 0; k < 4; k++) out[4 * c + k] = g->tune_itx_shape[c][k];
     out[8 + c] = g->tune_mc_rows[c];
   }
   out[10] = g->tuned ? 1 : 0;
   return VP9R_OK;
 }
 
-int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[26]) {
+int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[VP9R_TUNE_TIMES]) {
   TRY(hipSetDevice(g->device), "hipSetDevice");
   uint64_t rng = 0x9e3779b97f4a7c15ull;
   auto next = [&]() {
@@ -486,14 +503,14 @@ int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[26]) {
     rng ^= rng << 17;
     return rng;
   };
-  float tmp[26];
+  float tmp[VP9R_TUNE_TIMES];
   float *tu = times_us ? times_us : tmp;
-  int slot = 0;
+  for (int i = 0; i < VP9R_TUNE_TIMES; i++) tu[i] = -1.0f;
   // transforms
-  const uint32_t nb = 32768;
   for (int c = 0; c < 2; c++) {
     const uint32_t bd = c ? 10 : 8, bps = c ? 2 : 1;
     for (int k = 0; k < 4; k++) {
+      const uint32_t nb = 262144u >> k;  // 16 MB (4x4) .. 128 MB (32x32) of coefficients
       const uint32_t n0 = 4u << k, stride = 1024;
       const uint32_t per_row = stride / n0, rows_blk = (nb + per_row - 1) / per_row;
       std::vector<int32_t> coefs((size_t)nb * n0 * n0);
@@ -528,27 +545,28 @@ int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[26]) {
       TRY(hipMemcpy(d_tt, ttype.data(), nb, hipMemcpyHostToDevice), "tune upload");
       TRY(hipMemset(d_dst, 0x40, (size_t)rows_blk * n0 * stride * bps), "tune memset");
       float best = 1e30f;
-      int best_w = 1;
-      const int cand[3] = {1, 2, 4};
-      for (int w : cand) {
+      int best_shape = 1;
+      for (int cand = 0; cand < 12; cand++) {
+        const int shape = itx_candidate(cand);
+        if (!itx_shape_ok((uint32_t)shape, k)) continue;
         float t_best = 1e30f;
         for (int rep = 0; rep < 4; rep++) {
           TRY(seam_begin(g), "hipEventRecord");
           vp9g::launch_itx_dense(g->stream, nb, (const int32_t *)d_coef, d_coff, nullptr, d_tt, d_eob, d_doff, d_dst,
-                                 stride, bd, 0, k, d_jobs, nb, w);
+                                 stride, bd, 0, k, d_jobs, nb, shape);
           TRY(hipGetLastError(), "tune itx launch");
           TRY(hipEventRecord(g->seam_ev[1], g->stream), "hipEventRecord");
           TRY(hipStreamSynchronize(g->stream), "tune sync");
           TRY(seam_end(g), "hipEventElapsedTime");
           if (rep) t_best = std::min(t_best, g->seam_ms);  // the first is a warm-up
         }
-        if (slot < 26) tu[slot++] = 1000.0f * t_best;
+        tu[(c * 4 + k) * 12 + cand] = 1000.0f * t_best;
         if (t_best < best * 0.97f) {  // a candidate must win clearly to move off the default
           best = t_best;
-          best_w = w;
+          best_shape = shape;
         }
       }
-      g->tune_itx_waves[c][k] = (uint8_t)best_w;
+      g->tune_itx_shape[c][k] = (uint8_t)best_shape;
       (void)hipFree(d_coef);
       (void)hipFree(d_meta);
       (void)hipFree(d_dst);
@@ -600,7 +618,7 @@ int vp9r_gpu_autotune(vp9r_gpu *g, float times_us[26]) {
         }
         if (rep) t_best = std::min(t_best, g->seam_ms);
       }
-      if (slot < 26) tu[slot++] = 1000.0f * t_best;
+      tu[96 + 2 * c + (r == 32 ? 0 : 1)] = 1000.0f * t_best;
       if (t_best < best * 0.97f) {
         best = t_best;
         best_rows = r;

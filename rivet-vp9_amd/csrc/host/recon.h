@@ -193,7 +193,9 @@ struct vp9r_gpu {
   // waves per workgroup of the inverse transforms per [8-bit, 10/12-bit]
   // [4x4 .. 32x32], and rows per motion compensation tile per [8-bit,
   // 10/12-bit]
-  uint8_t tune_itx_waves[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
+  // transform launch shape per bit depth class and size: waves per
+  // workgroup | register transposes (16) | persistent (32)
+  uint8_t tune_itx_shape[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
   uint8_t tune_mc_rows[2] = {32, 32};
   bool tuned = false;
 };

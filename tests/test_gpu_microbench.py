@@ -68,17 +68,28 @@ def test_mc_batch_every_block(gpu, n, bd):
     assert np.array_equal(got, want), first_diff(got, want)
 
 
+@pytest.mark.parametrize("persistent", [0, vp.SHAPE_PERSISTENT])
+@pytest.mark.parametrize("reg", [0, vp.SHAPE_REG])
 @pytest.mark.parametrize("waves", [1, 2, 4])
 @pytest.mark.parametrize("bd", [8, 10])
-def test_itx_every_tuned_shape(bd, waves):
+def test_itx_every_tuned_shape(bd, waves, reg, persistent):
     """Each launch shape the autotuner may choose (1, 2 or 4 waves per
-    transform workgroup) gives the oracle's output at every size."""
+    transform workgroup; transposes through LDS or in registers, which
+    32x32 lacks; one workgroup per blocks or persistent workgroups) gives
+    the oracle's output at every size."""
     g = vp.Gpu(0)
-    g.set_tuning([waves] * 4, [waves] * 4, 32, 32)
-    s = mb.ItxSet(60_000, 30 + waves, bd)
+    small = waves | reg | persistent
+    g.set_tuning([small] * 3 + [waves | persistent], [small] * 3 + [waves | persistent], 32, 32)
+    s = mb.ItxSet(60_000, 30 + waves + reg + persistent, bd)
     got = run_itx(g, s)
     want = mb.oracle_itx(s, THREADS)
     assert np.array_equal(got, want), first_diff(got, want)
+
+
+def test_itx_register_shape_refused_at_32x32():
+    g = vp.Gpu(0)
+    with pytest.raises(vp.InvalidInputError):
+        g.set_tuning([1, 1, 1, 1 | vp.SHAPE_REG], [1] * 4, 32, 32)
 
 
 @pytest.mark.parametrize("rows", [16, 32])
@@ -99,9 +110,18 @@ def test_autotune_records_its_choices():
     choice, times = g.autotune()
     print(choice, [round(t, 1) for t in times], flush=True)
     assert choice["tuned"] is True
-    assert all(w in (1, 2, 4) for w in choice["itx_waves_8bit"] + choice["itx_waves_hbd"])
+    shapes = choice["itx_shape_8bit"] + choice["itx_shape_hbd"]
+    assert all((c & 15) in (1, 2, 4) for c in shapes)
+    assert not (choice["itx_shape_8bit"][3] & vp.SHAPE_REG or choice["itx_shape_hbd"][3] & vp.SHAPE_REG)
     assert choice["mc_tile_rows_8bit"] in (16, 32) and choice["mc_tile_rows_hbd"] in (16, 32)
-    assert all(t > 0 for t in times)
+    assert len(times) == vp.Gpu.TUNE_TIMES
+    # every shape timed except register transposes at 32x32
+    for c in range(2):
+        for k in range(4):
+            for cand in range(12):
+                t = times[(c * 4 + k) * 12 + cand]
+                assert (t < 0) == (k == 3 and (cand // 3) % 2 == 1), (c, k, cand, t)
+    assert all(t > 0 for t in times[96:])
     # the tuned context still gives the oracle's output
     s = mb.ItxSet(20_000, 5, 8)
     assert np.array_equal(run_itx(g, s), mb.oracle_itx(s, THREADS))
