@@ -384,55 +384,22 @@ struct IbTx {
   }
 };
 
-// Intra block wg of the launch (kIbThreads lanes).
-template <bool HBD, bool ONE>
-DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const IntraBlockItem *items0, uint8_t *cp0,
-                     uint8_t *cp1, uint8_t *cp2, const int16_t *rp0, const int16_t *rp1, const int16_t *rp2,
-                     const vp9r_txb *txbs1, uint32_t wg) {
+// Stages the regions of the three planes in LDS (layout ly): each region
+// rg (the block or superblock, the row above it from above-left and the
+// column left of it; samples outside the 8x8-aligned area are not
+// reconstructed and keep the frame's), and the residuals of its area.
+template <bool HBD>
+DEV void ib_stage(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, const IbRegion &rg1,
+                  const IbRegion &rg2, uint8_t *const (&pl_)[3], const int16_t *const (&rs_)[3],
+                  const uint32_t (&st_)[3], int tid) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
-  const int tid = threadIdx.x;
-  const uint32_t fi = range_frame(rg, nf, wg);
-  const IntraArgs &a = fa[fi];
-  const FrameGeom &g = a.g;
-  // a launch of one frame (ONE) passes its items (the item loads with the
-  // frame's arguments instead of after them) and its planes, residual
-  // planes and transform block records (kernel arguments: known to be
-  // global memory)
-  const IntraBlockItem it = ONE ? items0[wg] : a.items[rg[fi].first + wg - rg[fi].wg];
-  // the planes, residual planes and strides as scalars, read once: the
-  // staging loops pick a plane per lane, and a per-lane read of the frame's
-  // arguments would hold up every load behind it
-  uint8_t *const pl_[3] = {ONE ? cp0 : (uint8_t *)a.cur.p[0], ONE ? cp1 : (uint8_t *)a.cur.p[1],
-                           ONE ? cp2 : (uint8_t *)a.cur.p[2]};
-  const int16_t *const rs_[3] = {ONE ? rp0 : a.res[0], ONE ? rp1 : a.res[1], ONE ? rp2 : a.res[2]};
-  const uint32_t st_[3] = {a.cur.stride[0], a.cur.stride[1], a.cur.stride[2]};
   auto cur_plane = [&](int p) -> uint8_t * { return p == 0 ? pl_[0] : p == 1 ? pl_[1] : pl_[2]; };
   auto res_plane = [&](int p) -> const int16_t * { return p == 0 ? rs_[0] : p == 1 ? rs_[1] : rs_[2]; };
   auto stride_of = [&](int p) -> size_t { return p == 0 ? st_[0] : p == 1 ? st_[1] : st_[2]; };
-  const vp9r_txb *txbs = ONE ? txbs1 : a.txbs;
-  const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
-  vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
-  IbTx *txs = (IbTx *)(ib_lds + ly.txs);
-  int32_t *dcv = (int32_t *)(ib_lds + ly.dcv);
-  IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
-  // the transform block records, all in LDS (at most ib_max_txbs; the host
-  // checks each plane's grid, which bounds the count)
-  for (uint32_t k = tid; k < it.count; k += kIbThreads) stx[k] = txbs[it.first + k];
-  auto txb_at = [&](uint32_t k) -> vp9r_txb { return stx[k]; };
-  const int mi_size = it.mi_size;
-  const bool avail_u = (it.flags & 8) != 0, avail_l = (it.flags & 16) != 0;
-  const int bsize = mi_size < BLOCK_8X8 ? BLOCK_8X8 : mi_size;
-  const int bw8 = num8x8_wide(bsize), bh8 = num8x8_high(bsize);
-  const int bd = g.bit_depth, maxv = (1 << bd) - 1, base = 1 << (bd - 1);
-  const uint32_t np0 = it.n_y, np1 = it.n_u, np2 = it.count - it.n_y - it.n_u;
-  // every plane's region: the block, the row above it (from above-left) and
-  // the column left of it (samples of the block outside the 8x8-aligned
-  // area are not reconstructed and keep the frame's).  All loads of the
+  // All loads of the
   // three planes are in flight before the first LDS write: at most
   // kIbRegionSlots items per lane (4:4:4 64x64: 3 x (65 x 8 + 65) items).
   constexpr int kIbRegionSlots = (3 * (65 * 8 + 65) + kIbThreads - 1) / kIbThreads;
-  const IbRegion rg0 = ib_region(it, g, 0, bw8, bh8, np0 > 0), rg1 = ib_region(it, g, 1, bw8, bh8, np1 > 0),
-                 rg2 = ib_region(it, g, 2, bw8, bh8, np2 > 0);
   constexpr int kIbResSlots = (3 * 512 + kIbThreads - 1) / kIbThreads;
   const int nr0 = rg0.n ? rg0.BH << rg0.lq : 0, nr1 = rg1.n ? rg1.BH << rg1.lq : 0, nr2 = rg2.n ? rg2.BH << rg2.lq : 0;
   {
@@ -526,6 +493,173 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
       }
     }
   }
+}
+
+// Writes the regions' areas back from LDS to the frame.
+template <bool HBD>
+DEV void ib_store(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, const IbRegion &rg1,
+                  const IbRegion &rg2, uint8_t *const (&pl_)[3], const uint32_t (&st_)[3], int tid) {
+  typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
+  // the blocks back to the frame, in vectors of V samples
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
+    if (!R.n) continue;
+    const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
+    const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
+    const size_t stride = st_[p];
+    P *pl = (P *)pl_[p];
+    for (int k = tid; k < (R.BH << R.lq); k += kIbThreads) {
+      const int r = k >> R.lq, q = k & ((1 << R.lq) - 1);
+      const uint16_t *src = tile + (r + 1) * tp + 1 + q * R.V;
+      P *dst = pl + (size_t)(R.y0 + r) * stride + R.x0 + q * R.V;
+      uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if (j >= R.V) continue;
+        if (sizeof(P) == 1)
+          w[j >> 2] |= (uint32_t)src[j] << (8 * (j & 3));
+        else
+          w[j >> 1] |= (uint32_t)src[j] << (16 * (j & 1));
+      }
+      const int nb = R.V * (int)sizeof(P);
+      if (nb == 16)
+        *(uint4 *)dst = make_uint4(w[0], w[1], w[2], w[3]);
+      else if (nb == 8)
+        *(uint2 *)dst = make_uint2(w[0], w[1]);
+      else
+        *(uint32_t *)dst = w[0];
+    }
+  }
+}
+
+// A superblock item: every intra block of one superblock in one workgroup.
+// The superblock's region in all three planes (with the row above and the
+// column left of it) and its residuals are staged in LDS once; then the
+// item's levels run in order, each predicting and reconstructing its
+// transform blocks (whose edges come only from samples of earlier levels or
+// of earlier items) with a barrier between levels; the region goes back at
+// the end.  The host levels the transform blocks (intra_sb_items) and the
+// items by the superblocks left, above and above-left of them.
+template <bool HBD>
+DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (&pl_)[3],
+                  const int16_t *const (&rs_)[3], const uint32_t (&st_)[3]) {
+  const int tid = threadIdx.x;
+  const FrameGeom &g = a.g;
+  const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
+  const uint32_t M = ib_max_txbs(g.ss_x, g.ss_y);
+  IbTx *txs = (IbTx *)(ib_lds + ly.txs);
+  uint32_t *units = (uint32_t *)(ib_lds + ly.txb);
+  int32_t *dcv = (int32_t *)(ib_lds + ly.txb + 4 * M);
+  uint2 *levels = (uint2 *)(ib_lds + ly.txb + 8 * M);
+  const int bd = g.bit_depth, maxv = (1 << bd) - 1, base = 1 << (bd - 1);
+  const IbRegion rg0 = ib_region(it, g, 0, 8, 8, true), rg1 = ib_region(it, g, 1, 8, 8, true),
+                 rg2 = ib_region(it, g, 2, 8, 8, true);
+  // descriptors, units and levels first: their loads overlap the region's
+  for (uint32_t k = tid; k < it.count; k += kIbThreads) {
+    const IntraSbTx d = a.sb_tx[it.first + k];
+    const int p = d.plane, sx = p ? g.ss_x : 0;
+    const uint32_t tile_off = p == 0 ? ly.tile[0] : p == 1 ? ly.tile[1] : ly.tile[2];
+    const uint32_t res_off = p == 0 ? ly.res[0] : p == 1 ? ly.res[1] : ly.res[2];
+    IbTx t;
+    t.tp = (64 >> sx) + 1;
+    t.rp = 64 >> sx;
+    t.tile = (int)(tile_off >> 1) + (d.y + 1) * t.tp + d.x + 1;
+    t.res = (int)(res_off >> 1) + d.y * t.rp + d.x;
+    t.bits = d.mode | d.log2 << 4 | (d.flags & 1) << 8 | ((d.flags >> 1) & 1) << 9 | ((d.flags >> 2) & 1) << 10 | 1 << 11;
+    t.lim_a = d.lim_a;
+    t.lim_l = d.lim_l;
+    t._pad = 0;
+    txs[k] = t;
+  }
+  for (uint32_t k = tid; k < it.coef_count; k += kIbThreads) units[k] = a.sb_units[it.coef_first + k];
+  for (uint32_t k = tid; k < it.n_u; k += kIbThreads) levels[k] = a.sb_levels[it.sub_modes + k];
+  ib_stage<HBD>(g, ly, rg0, rg1, rg2, pl_, rs_, st_, tid);
+  __syncthreads();
+  const int16_t *res = (const int16_t *)ib_lds;
+  uint16_t *tile = (uint16_t *)ib_lds;
+  for (uint32_t lv = 0; lv < it.n_u; lv++) {
+    const uint2 L = levels[lv];
+    const int d0 = (int)(L.x & 0xffff), nd = (int)((L.x >> 16) & 0x7fff);
+    const int u0 = (int)(L.y & 0xffff), nu = (int)(L.y >> 16);
+    if (L.x >> 31) {
+      // DC values of the level's DC_PRED blocks, every edge read in flight at once
+      if (tid < nd) {
+        const IbTx d = txs[d0 + tid];
+        if (d.mode() == DC_PRED) {
+          int v;
+          switch ((d.bits >> 4) & 15) {
+            case 2: v = d.dc<2>(bd); break;
+            case 3: v = d.dc<3>(bd); break;
+            case 4: v = d.dc<4>(bd); break;
+            default: v = d.dc<5>(bd); break;
+          }
+          dcv[d0 + tid] = v;
+        }
+      }
+      __syncthreads();
+    }
+    for (int item = tid; item < nu * 16; item += kIbThreads) {
+      const uint32_t un = units[u0 + (item >> 4)];
+      const int di = (int)(un & 0xffff), sub = (int)(un >> 16), s16 = item & 15;
+      const IbTx d = txs[di];
+      const int l2 = (d.bits >> 4) & 15, size = 1 << l2;
+      const int i = ((sub >> (l2 - 2)) << 2) + (s16 >> 2), j = ((sub & ((size >> 2) - 1)) << 2) + (s16 & 3);
+      const int mode = d.mode();
+      const int pv = intra_sample_at(
+          mode, i, j, size, [&](int k) { return d.A(k, size, base); }, [&](int k) { return d.L(k, base); },
+          mode == DC_PRED ? dcv[di] : 0, maxv);
+      tile[d.tile + i * d.tp + j] = (uint16_t)iclamp(pv + res[d.res + i * d.rp + j], 0, maxv);
+    }
+    __syncthreads();
+  }
+  ib_store<HBD>(g, ly, rg0, rg1, rg2, pl_, st_, tid);
+}
+
+// Intra block wg of the launch (kIbThreads lanes).
+template <bool HBD, bool ONE>
+DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const IntraBlockItem *items0, uint8_t *cp0,
+                     uint8_t *cp1, uint8_t *cp2, const int16_t *rp0, const int16_t *rp1, const int16_t *rp2,
+                     const vp9r_txb *txbs1, uint32_t wg) {
+  const int tid = threadIdx.x;
+  const uint32_t fi = range_frame(rg, nf, wg);
+  const IntraArgs &a = fa[fi];
+  const FrameGeom &g = a.g;
+  // a launch of one frame (ONE) passes its items (the item loads with the
+  // frame's arguments instead of after them) and its planes, residual
+  // planes and transform block records (kernel arguments: known to be
+  // global memory)
+  const IntraBlockItem it = ONE ? items0[wg] : a.items[rg[fi].first + wg - rg[fi].wg];
+  // the planes, residual planes and strides as scalars, read once: the
+  // staging loops pick a plane per lane, and a per-lane read of the frame's
+  // arguments would hold up every load behind it
+  uint8_t *const pl_[3] = {ONE ? cp0 : (uint8_t *)a.cur.p[0], ONE ? cp1 : (uint8_t *)a.cur.p[1],
+                           ONE ? cp2 : (uint8_t *)a.cur.p[2]};
+  const int16_t *const rs_[3] = {ONE ? rp0 : a.res[0], ONE ? rp1 : a.res[1], ONE ? rp2 : a.res[2]};
+  const uint32_t st_[3] = {a.cur.stride[0], a.cur.stride[1], a.cur.stride[2]};
+  if (it.flags & kIbSuperblock) {
+    intra_sb<HBD>(a, it, pl_, rs_, st_);
+    return;
+  }
+  const vp9r_txb *txbs = ONE ? txbs1 : a.txbs;
+  const IbLayout ly = ib_layout(g.ss_x, g.ss_y);
+  vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
+  IbTx *txs = (IbTx *)(ib_lds + ly.txs);
+  int32_t *dcv = (int32_t *)(ib_lds + ly.dcv);
+  IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
+  // the transform block records, all in LDS (at most ib_max_txbs; the host
+  // checks each plane's grid, which bounds the count)
+  for (uint32_t k = tid; k < it.count; k += kIbThreads) stx[k] = txbs[it.first + k];
+  auto txb_at = [&](uint32_t k) -> vp9r_txb { return stx[k]; };
+  const int mi_size = it.mi_size;
+  const bool avail_u = (it.flags & 8) != 0, avail_l = (it.flags & 16) != 0;
+  const int bsize = mi_size < BLOCK_8X8 ? BLOCK_8X8 : mi_size;
+  const int bw8 = num8x8_wide(bsize), bh8 = num8x8_high(bsize);
+  const int bd = g.bit_depth, maxv = (1 << bd) - 1, base = 1 << (bd - 1);
+  const uint32_t np0 = it.n_y, np1 = it.n_u, np2 = it.count - it.n_y - it.n_u;
+  const IbRegion rg0 = ib_region(it, g, 0, bw8, bh8, np0 > 0), rg1 = ib_region(it, g, 1, bw8, bh8, np1 > 0),
+                 rg2 = ib_region(it, g, 2, bw8, bh8, np2 > 0);
+  ib_stage<HBD>(g, ly, rg0, rg1, rg2, pl_, rs_, st_, tid);
   for (int k = tid; k < 3 * 256; k += kIbThreads) txs[k].bits = 0;
   __syncthreads();
   if (tid < 3) {
@@ -639,37 +773,7 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
     }
     __syncthreads();
   }
-  // the blocks back to the frame, in vectors of V samples
-#pragma unroll
-  for (int p = 0; p < 3; p++) {
-    const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
-    if (!R.n) continue;
-    const int tp = (64 >> (p ? g.ss_x : 0)) + 1;
-    const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
-    const size_t stride = st_[p];
-    P *pl = (P *)pl_[p];
-    for (int k = tid; k < (R.BH << R.lq); k += kIbThreads) {
-      const int r = k >> R.lq, q = k & ((1 << R.lq) - 1);
-      const uint16_t *src = tile + (r + 1) * tp + 1 + q * R.V;
-      P *dst = pl + (size_t)(R.y0 + r) * stride + R.x0 + q * R.V;
-      uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        if (j >= R.V) continue;
-        if (sizeof(P) == 1)
-          w[j >> 2] |= (uint32_t)src[j] << (8 * (j & 3));
-        else
-          w[j >> 1] |= (uint32_t)src[j] << (16 * (j & 1));
-      }
-      const int nb = R.V * (int)sizeof(P);
-      if (nb == 16)
-        *(uint4 *)dst = make_uint4(w[0], w[1], w[2], w[3]);
-      else if (nb == 8)
-        *(uint2 *)dst = make_uint2(w[0], w[1]);
-      else
-        *(uint32_t *)dst = w[0];
-    }
-  }
+  ib_store<HBD>(g, ly, rg0, rg1, rg2, pl_, st_, tid);
 }
 
 void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {

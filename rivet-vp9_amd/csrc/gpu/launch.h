@@ -133,6 +133,31 @@ struct IntraBlockItem {
   uint32_t sub_modes;                       // 4 x 8 bits
 };
 
+// A superblock item (an IntraBlockItem with kIbSuperblock in flags) holds
+// every intra block of one superblock: its transform blocks (all planes)
+// as descriptors sb_tx[first .. first + count), sorted by their level
+// inside the superblock; units of 4x4 samples sb_units[coef_first ..
+// + coef_count) (transform block index | 4x4 index in it << 16, by level);
+// levels sb_levels[sub_modes .. sub_modes + n_u), each x = first
+// descriptor | count << 16 | any DC_PRED << 31, y = first unit | count <<
+// 16 (relative to the item's first); (mi_row, mi_col) the superblock's
+// origin.
+constexpr uint8_t kIbSuperblock = 0x80;
+// The most levels a superblock item may have (the host keeps frames with
+// more in the per-block form).
+constexpr uint32_t kIbSbMaxLevels = 128;
+// A transform block of a superblock item: plane, log2 of its size, its
+// prediction mode, flags (have_above 1, have_left 2, above-right read
+// inside its block 4: recon.rs:61-69), its plane position inside the
+// superblock and the clamps of its edge reads (the frame's last column
+// and row minus its position, recon.rs:42-43).
+struct IntraSbTx {
+  uint8_t plane, log2, mode, flags;
+  uint8_t x, y;
+  int16_t lim_a, lim_l;
+  uint16_t _pad[3];
+};
+
 // LDS of one k_intra_blocks workgroup (kIbThreads lanes) for a frame of
 // subsampling (ss_x, ss_y): byte offsets of the transform block records
 // (all of the block's), the three transform grids' descriptors (32 B per
@@ -181,6 +206,10 @@ struct IntraArgs {
   Planes cur;
   int32_t q[8][2][2];
   const int16_t *res[3];  // the intra transform blocks' residuals (k_itx_inter)
+  // superblock items (kIbSuperblock)
+  const IntraSbTx *sb_tx;
+  const uint32_t *sb_units;
+  const uint2 *sb_levels;
 };
 
 // Loop filter: per superblock, plane and pass, 16 x 16 entries of filter

@@ -315,6 +315,187 @@ static void build_intra(Builder *g, const vp9r_record *rec) {
   }
 }
 
+// Superblock items (kIbSuperblock): every intra block of a superblock in
+// one item.  An item is levelled one above the deepest of the items left,
+// above and above-left of it (an intra block reads samples of its own
+// superblock and of those three only: the row above from above-left to its
+// top-right, the column left of it, recon.rs:17-103).  Inside an item each
+// transform block is levelled one above the deepest transform block of the
+// item whose samples it reads (4x4 cells of each plane); inter samples are
+// complete before intra starts.  The frame takes this form when it has at
+// most half the levels of the per-block form (a key frame: 338 block
+// levels against 46 superblock diagonals at 1080p); false keeps the
+// per-block form, also for an item over kIbSbMaxLevels levels.  On success
+// block_items, item_level_start and item_level (per intra block, for
+// lf_need) describe the items.  The record's transform blocks were checked
+// by the per-block form's pass (one size per plane and block, no repeated
+// positions, inside the block).
+static bool build_intra_sb(Builder *g, const vp9r_record *rec) {
+  const vp9r_frame_hdr &h = rec->hdr;
+  const int C = (int)h.sb64_cols, R = (int)h.sb64_rows;
+  const size_t n_items = g->intra_items.size();
+  if (!n_items) return false;
+  std::vector<int32_t> &sb_level = g->sb_level;
+  sb_level.assign((size_t)R * C, -1);
+  auto sb_of = [&](const vp9r_block &b) { return (int)(b.mi_row / 8) * C + (int)(b.mi_col / 8); };
+  for (size_t k = 0; k < n_items; k++) sb_level[sb_of(rec->blocks[g->intra_items[k].block])] = 0;
+  int max_lv = -1;
+  for (int r = 0; r < R; r++)
+    for (int c = 0; c < C; c++) {
+      int32_t &lv = sb_level[(size_t)r * C + c];
+      if (lv < 0) continue;
+      auto dep = [&](int rr, int cc) {
+        if (rr >= 0 && cc >= 0 && sb_level[(size_t)rr * C + cc] >= 0) lv = std::max(lv, sb_level[(size_t)rr * C + cc] + 1);
+      };
+      dep(r, c - 1);
+      dep(r - 1, c);
+      dep(r - 1, c - 1);
+      max_lv = std::max(max_lv, lv);
+    }
+  const size_t block_levels = g->item_level_start.empty() ? 0 : g->item_level_start.size() - 1;
+  if ((size_t)(max_lv + 1) * 2 > block_levels) return false;
+  // items in superblock level order (raster order within a level)
+  std::vector<uint32_t> &order = g->sb_order;
+  order.clear();
+  std::vector<uint32_t> lstart(max_lv + 2, 0);
+  for (int32_t lv : sb_level)
+    if (lv >= 0) lstart[lv + 1]++;
+  for (int l = 0; l <= max_lv; l++) lstart[l + 1] += lstart[l];
+  order.resize(lstart[max_lv + 1]);
+  {
+    std::vector<uint32_t> fill(lstart.begin(), lstart.end());
+    for (int k = 0; k < R * C; k++)
+      if (sb_level[k] >= 0) order[fill[sb_level[k]]++] = (uint32_t)k;
+  }
+  // the intra blocks of each superblock (decode order keeps a superblock's
+  // blocks together)
+  std::vector<uint32_t> first_item((size_t)R * C, 0), n_item((size_t)R * C, 0);
+  for (size_t k = 0; k < n_items; k++) {
+    const int sb = sb_of(rec->blocks[g->intra_items[k].block]);
+    if (!n_item[sb]) first_item[sb] = (uint32_t)k;
+    if (first_item[sb] + n_item[sb] != k) return false;  // not contiguous: keep the per-block form
+    n_item[sb]++;
+  }
+  const uint32_t M = vp9g::ib_max_txbs(h.ss_x, h.ss_y);
+  g->sb_tx.clear();
+  g->sb_units.clear();
+  g->sb_levels.clear();
+  std::vector<vp9g::IntraBlockItem> items(order.size());
+  std::vector<int32_t> &cells = g->sb_cells;
+  std::vector<vp9g::IntraSbTx> descs;
+  std::vector<int32_t> &dlv = g->sb_tx_level;
+  for (size_t oi = 0; oi < order.size(); oi++) {
+    const int sb = (int)order[oi], sr = sb / C, sc = sb % C;
+    // 4x4 cells of the three planes: luma 16 x 16, chroma (16 >> ss_x) x (16 >> ss_y)
+    const int cw[3] = {16, 16 >> h.ss_x, 16 >> h.ss_x}, ch[3] = {16, 16 >> h.ss_y, 16 >> h.ss_y};
+    const int coff[3] = {0, 256, 256 + cw[1] * ch[1]};
+    cells.assign((size_t)coff[2] + cw[2] * ch[2], -1);
+    descs.clear();
+    dlv.clear();
+    int n_lv = 0;
+    for (uint32_t k = first_item[sb]; k < first_item[sb] + n_item[sb]; k++) {
+      const vp9g::IntraItem &it = g->intra_items[k];
+      const vp9r_block &b = rec->blocks[it.block];
+      const int bsize = b.mi_size < BLOCK_8X8 ? BLOCK_8X8 : b.mi_size;
+      for (uint32_t i = it.first; i < it.first + it.count; i++) {
+        const vp9r_txb &t = rec->txbs[i];
+        const int p = t.plane, sx = p ? h.ss_x : 0, sy = p ? h.ss_y : 0;
+        const int log2 = t.tx_size + 2, size = 1 << log2;
+        const int x0 = (sc * 64) >> sx, y0 = (sr * 64) >> sy;
+        const int x = (int)t.x - x0, y = (int)t.y - y0;
+        if (x < 0 || y < 0 || (x & 3) || (y & 3) || x + size > (64 >> sx) || y + size > (64 >> sy)) return false;
+        const int bx = (b.mi_col * 8) >> sx, by = (b.mi_row * 8) >> sy;
+        const int xi = ((int)t.x - bx) >> 2, yi = ((int)t.y - by) >> 2, step4 = 1 << t.tx_size;
+        const int n4w = vp9g::num4x4_wide(bsize) >> sx;
+        int mode;
+        if (p > 0)
+          mode = b.uv_mode;
+        else if (b.mi_size >= BLOCK_8X8)
+          mode = b.y_mode;
+        else
+          mode = b.sub_modes[((yi / step4) * ((n4w + step4 - 1) / step4) + xi / step4) & 3];
+        const bool have_left = b.avail_l || xi > 0, have_above = b.avail_u || yi > 0;
+        const bool a4 = log2 == 2 && xi + step4 < n4w;
+        const int max_x = (int)((h.mi_cols * 8) >> sx) - 1, max_y = (int)((h.mi_rows * 8) >> sy) - 1;
+        // the level: one above the deepest cell of this superblock read
+        int lv = 0;
+        auto dep = [&](int px, int py) {  // plane position relative to the superblock
+          if (px < 0 || py < 0) return;
+          const int l = cells[coff[p] + (py >> 2) * cw[p] + (px >> 2)];
+          if (l >= 0) lv = std::max(lv, l + 1);
+        };
+        if (have_above) {
+          const int n = a4 ? 2 * size : size;
+          for (int k2 = 0; k2 < n; k2++) dep(std::min(max_x - x0, x + k2), y - 1);
+          if (have_left) dep(x - 1, y - 1);
+        }
+        if (have_left)
+          for (int k2 = 0; k2 < size; k2++) dep(x - 1, std::min(max_y - y0, y + k2));
+        for (int cy = y >> 2; cy < (y + size) >> 2; cy++)
+          for (int cx = x >> 2; cx < (x + size) >> 2; cx++) {
+            int32_t &cell = cells[coff[p] + cy * cw[p] + cx];
+            if (cell >= 0) return false;  // repeated position
+            cell = lv;
+          }
+        vp9g::IntraSbTx d{};
+        d.plane = (uint8_t)p;
+        d.log2 = (uint8_t)log2;
+        d.mode = (uint8_t)mode;
+        d.flags = (uint8_t)((have_above ? 1 : 0) | (have_left ? 2 : 0) | (a4 ? 4 : 0));
+        d.x = (uint8_t)x;
+        d.y = (uint8_t)y;
+        d.lim_a = (int16_t)std::max(-32768, std::min(32767, max_x - (int)t.x));
+        d.lim_l = (int16_t)std::max(-32768, std::min(32767, max_y - (int)t.y));
+        descs.push_back(d);
+        dlv.push_back(lv);
+        n_lv = std::max(n_lv, lv + 1);
+      }
+    }
+    if (descs.size() > M || (uint32_t)n_lv > vp9g::kIbSbMaxLevels) return false;
+    // descriptors by level (stable), their units, the level table
+    std::vector<uint32_t> ls(n_lv + 1, 0);
+    for (int32_t l : dlv) ls[l + 1]++;
+    for (int l = 0; l < n_lv; l++) ls[l + 1] += ls[l];
+    std::vector<uint32_t> pos(descs.size());
+    {
+      std::vector<uint32_t> fill(ls.begin(), ls.end());
+      for (size_t j = 0; j < descs.size(); j++) pos[fill[dlv[j]]++] = (uint32_t)j;
+    }
+    vp9g::IntraBlockItem &o = items[oi];
+    o = vp9g::IntraBlockItem{};
+    o.first = (uint32_t)g->sb_tx.size();
+    o.count = (uint32_t)descs.size();
+    o.coef_first = (uint32_t)g->sb_units.size();
+    o.sub_modes = (uint32_t)g->sb_levels.size();
+    o.n_u = (uint16_t)n_lv;
+    o.mi_row = (uint16_t)(sr * 8);
+    o.mi_col = (uint16_t)(sc * 8);
+    o.mi_size = BLOCK_64X64;
+    o.flags = vp9g::kIbSuperblock;
+    uint32_t units = 0;
+    for (int l = 0; l < n_lv; l++) {
+      bool dc = false;
+      const uint32_t u0 = units;
+      for (uint32_t q = ls[l]; q < ls[l + 1]; q++) {
+        const vp9g::IntraSbTx &d = descs[pos[q]];
+        g->sb_tx.push_back(d);
+        dc = dc || d.mode == DC_PRED;
+        const uint32_t n4 = 1u << (2 * (d.log2 - 2));
+        for (uint32_t sub = 0; sub < n4; sub++) g->sb_units.push_back(q | sub << 16);
+        units += n4;
+      }
+      g->sb_levels.push_back(make_uint2(ls[l] | (ls[l + 1] - ls[l]) << 16 | (dc ? 1u << 31 : 0u),
+                                        u0 | (units - u0) << 16));
+    }
+    if (units > M) return false;
+    o.coef_count = units;
+  }
+  g->block_items = items;
+  g->item_level_start = lstart;
+  for (size_t k = 0; k < n_items; k++) g->item_level[k] = sb_level[sb_of(rec->blocks[g->intra_items[k].block])];
+  return true;
+}
+
 // The loop filter's wavefront levels (lf.hip): level t holds the units
 // (r, t - r) and, while t + 1 < rows, the lone vertical pass of superblock
 // (t + 1, 0); lf_level_start[t] = 3 x the items of the levels before t (one
@@ -826,6 +1007,10 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
       o.n_u = (uint16_t)n[1];
     }
   }
+  // a frame with many more intra levels than superblock diagonals (a key
+  // frame) takes superblock items
+  g->sb_mode = build_intra_sb(g, rec);
+  p.intra_sb = g->sb_mode;
   work_bytes(rec, p);
   const bool do_lf = h.lf_level != 0;
   if (do_lf) build_lf_levels(g, h.sb64_rows, h.sb64_cols);
@@ -846,6 +1031,11 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   p.o_iidx = take(sizeof(uint32_t) * g->intra_index.size());
   p.o_groups = take(sizeof(vp9g::IntraGroup) * g->groups.size());
   p.o_items = take(sizeof(vp9g::IntraBlockItem) * g->block_items.size());
+  const size_t n_sbtx = g->sb_mode ? g->sb_tx.size() : 0, n_sbunits = g->sb_mode ? g->sb_units.size() : 0,
+               n_sblevels = g->sb_mode ? g->sb_levels.size() : 0;
+  p.o_sbtx = take(sizeof(vp9g::IntraSbTx) * n_sbtx);
+  p.o_sbunits = take(sizeof(uint32_t) * n_sbunits);
+  p.o_sblevels = take(sizeof(uint2) * n_sblevels);
   p.bytes = off;
   p.n_tiles = (uint32_t)g->tiles.size();
   for (int k = 0; k < 4; k++) {
@@ -924,6 +1114,9 @@ int build_record(Builder &w, const vp9r_record *rec, Prepared &p, HostBuf &stage
   put(p.o_iidx, g->intra_index.data(), sizeof(uint32_t) * g->intra_index.size());
   put(p.o_groups, g->groups.data(), sizeof(vp9g::IntraGroup) * g->groups.size());
   put(p.o_items, g->block_items.data(), sizeof(vp9g::IntraBlockItem) * g->block_items.size());
+  put(p.o_sbtx, g->sb_tx.data(), sizeof(vp9g::IntraSbTx) * n_sbtx);
+  put(p.o_sbunits, g->sb_units.data(), sizeof(uint32_t) * n_sbunits);
+  put(p.o_sblevels, g->sb_levels.data(), sizeof(uint2) * n_sblevels);
   return VP9R_OK;
 }
 
@@ -1261,6 +1454,9 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       na.txb_index = (const uint32_t *)(dv + p.o_iidx);
       na.groups = (const vp9g::IntraGroup *)(dv + p.o_groups);
       na.items = (const vp9g::IntraBlockItem *)(dv + p.o_items);
+      na.sb_tx = (const vp9g::IntraSbTx *)(dv + p.o_sbtx);
+      na.sb_units = (const uint32_t *)(dv + p.o_sbunits);
+      na.sb_levels = (const uint2 *)(dv + p.o_sblevels);
       na.coef_pos = ia.coef_pos;
       na.coef_val = ia.coef_val;
       na.cur = cur->planes;

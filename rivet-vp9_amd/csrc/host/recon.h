@@ -40,7 +40,9 @@ struct Prepared {
   int32_t q[8][2][2];
   vp9g::LfParams lfp;
   size_t o_blocks = 0, o_txbs = 0, o_cpos = 0, o_cval = 0, o_mi = 0, o_tiles = 0, o_jobs[4] = {},
-         o_iidx = 0, o_groups = 0, o_items = 0;
+         o_iidx = 0, o_groups = 0, o_items = 0, o_sbtx = 0, o_sbunits = 0, o_sblevels = 0;
+  // the intra items are superblock items (kIbSuperblock), not blocks
+  bool intra_sb = false;
   // jobs[k] lists the inter transform blocks with coefficients (n_jobs[k]),
   // then every transform block of the intra blocks (n_ijobs[k], flagged
   // kItxIntra), whose residuals the intra block kernel reads
@@ -71,6 +73,13 @@ struct Builder {
   std::vector<uint32_t> lf_level_start;
   uint32_t lf_rows = 0, lf_cols = 0;
   std::vector<int32_t> sb_need;
+  // superblock intra items (kIbSuperblock), when the frame uses them
+  bool sb_mode = false;
+  std::vector<vp9g::IntraSbTx> sb_tx;
+  std::vector<uint32_t> sb_units;
+  std::vector<uint2> sb_levels;
+  std::vector<int32_t> sb_level, sb_cells, sb_tx_level;
+  std::vector<uint32_t> sb_order;
 };
 
 // Checks a record and writes its inputs and work lists to `stage` (pinned
