@@ -1013,7 +1013,7 @@ def main():
     dom = max(range(4), key=lambda s: stage_ms[s])
     dname = STAGES[dom]
     achieved = stages[dname]["GB_per_s"] or 0.0
-    kern = {"loop_filter": "vp9g::k_lf_cone<false, 3>", "inter_pred": "vp9g::k_mc<false>",
+    kern = {"loop_filter": "vp9g::k_lf_cone<false, 4>", "inter_pred": "vp9g::k_mc<false>",
             "inter_residual": "vp9g::k_itx_inter<false>"}.get(dname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profile_traffic(kern) if kern else None,

@@ -1228,7 +1228,7 @@ size_t wave_schedule(const Prepared &p, bool recon, bool lf, size_t &n_int, std:
 // from 1080p measurements (microseconds from one launch's start to the
 // next's, by depth 1 .. 4); max_depth 1 keeps one level per launch.
 LfPlan lf_tail_plan(size_t m, int max_depth) {
-  static const double cost[5] = {0.0, 10.3, 18.7, 23.6, 28.5};
+  static const double cost[5] = {0.0, 10.6, 15.7, 21.0, 26.5};
   LfPlan best;
   best.singles = m;
   if (max_depth < 2 || m < 4) return best;
