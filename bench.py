@@ -81,7 +81,7 @@ STAGES = ("inter_pred", "inter_residual", "intra", "loop_filter")
 # level (the loop filter levels placed beside them run there too);
 # loop_filter: the filter levels after the last intra level (several per
 # k_lf_cone launch, a few one per k_lf launch)
-STAGE_KERNELS = ("k_mc", "k_itx_inter", "k_lf_mask + k_wave", "k_lf_cone + k_lf")
+STAGE_KERNELS = ("k_prep + k_mc", "k_itx_inter", "k_wave", "k_lf_cone + k_lf")
 STAGE_BYTES = ("mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
 
 

@@ -1,41 +1,15 @@
-// Frame store set-up for a reconstruction.
-//
-// The reference reconstructs into a zero-filled frame (src/decoder/pool.rs:
-// buffers are handed out zero-filled) and some of its reads reach past the
-// 8x8-aligned area that reconstruction writes: the 4:2:0 loop filter's
-// internal 4x4 chroma edge in the last, odd, 8x8 row or column reads the 4
-// samples beyond it (loopfilter.rs:558-600).  Frame stores here are reused,
-// so every reconstruction first clears what lies outside that area.
+// Frame store set-up for a reconstruction: the margin clear (frame_dev.h).
 #include <type_traits>
 
 #include "dev_common.h"
+#include "frame_dev.h"
 #include "launch.h"
 
 namespace vp9g {
 
-// Per frame, workgroup j of count = 3 * rows: plane j / rows, and j % rows
-// counts first the rows below the 8x8-aligned area (whole rows), then the
-// rows of the area (their part right of it).
 template <class P>
 __global__ __launch_bounds__(256) void k_clear_margins(const ClearArgs *fa, const Range *rg, uint32_t nf) {
-  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
-  const ClearArgs &a = fa[fi];
-  const uint32_t j = blockIdx.x - rg[fi].wg, per = rg[fi].count / 3;
-  const int plane = j / per;
-  const uint32_t r = j - plane * per;
-  const FrameGeom &g = a.g;
-  const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
-  const uint32_t mi_w = (g.mi_cols * 8) >> sx, mi_h = (g.mi_rows * 8) >> sy;
-  const uint32_t rows = a.f.rows[plane], stride = a.f.stride[plane];
-  const uint32_t n_bottom = rows - mi_h;
-  P *pl = (P *)a.f.p[plane];
-  if (r < n_bottom) {
-    P *row = pl + (size_t)(mi_h + r) * stride;
-    for (uint32_t x = threadIdx.x; x < stride; x += 256) row[x] = 0;
-  } else if (r - n_bottom < mi_h && stride > mi_w) {
-    P *row = pl + (size_t)(r - n_bottom) * stride;
-    for (uint32_t x = mi_w + threadIdx.x; x < stride; x += 256) row[x] = 0;
-  }
+  clear_margins_wg<P>(fa, rg, nf, blockIdx.x);
 }
 
 // Workgroups per plane the margins of a frame need (count of its range).

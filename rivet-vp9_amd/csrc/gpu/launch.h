@@ -297,6 +297,11 @@ uint32_t wave_lds_bytes(int ss_x, int ss_y);
 // per_sb: 1 (a workgroup per superblock entry, 6 per superblock) or 6 (a
 // workgroup per superblock); the ranges count workgroups accordingly
 void launch_lf_mask(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, int per_sb);
+// The margin clear (n_clear workgroups, ranges rc over fc) and the loop
+// filter decisions (n_mask workgroups, ranges rl over fl, per_sb as
+// launch_lf_mask) of a batch of n frames as one launch.
+void launch_prep(hipStream_t s, const ClearArgs *fc, const Range *rc, uint32_t n_clear, const LfArgs *fl,
+                 const Range *rl, uint32_t n_mask, uint32_t n, int per_sb, bool hbd);
 // one: for a launch of one frame, lf_one of its arguments and the level
 // (else p[0] null: each workgroup finds its frame through the range table)
 void launch_lf_level(hipStream_t s, const LfArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,

@@ -25,6 +25,7 @@
 #include "dev_common.h"
 #include "launch.h"
 #include "vp9_consts.h"
+#include "frame_dev.h"
 #include "lf_dev.h"
 
 namespace vp9g {
@@ -105,12 +106,12 @@ DEV void lf_decide(const LfArgs &a, int plane, int pass, uint32_t sb_row, uint32
 // entries of superblock j in turn (for large batches, where a workgroup per
 // entry would spend more on dispatch than on the decisions).
 template <int K>
-__global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *rg, uint32_t nf) {
-  const uint32_t fi = range_frame(rg, nf, blockIdx.x);
+DEV void lf_mask_wg(const LfArgs *fa, const Range *rg, uint32_t nf, uint32_t wg) {
+  const uint32_t fi = range_frame(rg, nf, wg);
   const LfArgs &a = fa[fi];
   const FrameGeom &g = a.g;
   const int n_sb = g.sb64_rows * g.sb64_cols;
-  const int j = blockIdx.x - rg[fi].wg, sb = K == 1 ? j % n_sb : j;
+  const int j = wg - rg[fi].wg, sb = K == 1 ? j % n_sb : j;
   const uint32_t sb_row = sb / g.sb64_cols, sb_col = sb % g.sb64_cols;
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -122,6 +123,37 @@ __global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *
                      : (uint32_t)(fs + 1) | (uint32_t)a.p.limit[lvl] << 8 | (uint32_t)a.p.blimit[lvl] << 16 |
                            (uint32_t)a.p.thresh[lvl] << 24;
   }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_lf_mask(const LfArgs *fa, const Range *rg, uint32_t nf) {
+  lf_mask_wg<K>(fa, rg, nf, blockIdx.x);
+}
+
+// The margin clear and the loop filter decisions of a batch in one launch,
+// its first: workgroups below n_clear clear margins (frame_dev.h), the rest
+// decide.  The decisions read mode info only and write the decision
+// buffer, the clear writes frame samples no other part of the launch
+// touches.
+template <class P, int K>
+__global__ __launch_bounds__(256) void k_prep(const ClearArgs *fc, const Range *rc, uint32_t n_clear,
+                                              const LfArgs *fl, const Range *rl, uint32_t nf) {
+  if (blockIdx.x < n_clear)
+    clear_margins_wg<P>(fc, rc, nf, blockIdx.x);
+  else
+    lf_mask_wg<K>(fl, rl, nf, blockIdx.x - n_clear);
+}
+
+void launch_prep(hipStream_t s, const ClearArgs *fc, const Range *rc, uint32_t n_clear, const LfArgs *fl,
+                 const Range *rl, uint32_t n_mask, uint32_t n, int per_sb, bool hbd) {
+  if (!n_clear && !n_mask) return;
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(n_clear + n_mask), dim3(256), 0, s, fc, rc, n_clear, fl, rl, n);
+  };
+  if (hbd)
+    per_sb == 6 ? go(k_prep<uint16_t, 6>) : go(k_prep<uint16_t, 1>);
+  else
+    per_sb == 6 ? go(k_prep<uint8_t, 6>) : go(k_prep<uint8_t, 1>);
 }
 
 // ---- one wavefront level per launch (decisions from k_lf_mask)

@@ -1509,7 +1509,13 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
     if (recon) {
       if (!gt)  // a graph's launch waits for the downloads before it
         for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
-      vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
+      // the margin clear, with the loop filter decisions in the same launch
+      // when the filter follows in this call on this stream
+      if (fused && lf)
+        vp9g::launch_prep(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), t_clear.wgs,
+                          (const vp9g::LfArgs *)(d + o_lf), R(t_mask), t_mask.wgs, n, mask_per_sb, hbd);
+      else
+        vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
       vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd,
                       n == 1 ? (const vp9g::McTile *)(f[0]->dv + f[0]->p->o_tiles) : nullptr,
                       n == 1 ? f[0]->p->n_tiles : 0u);
@@ -1552,7 +1558,8 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       // everything on the kernel stream: the decisions, then the fused levels
       if (!recon && !gt)
         for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
-      if (lf) vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs, mask_per_sb);
+      if (lf && !recon)  // else in the prep launch
+        vp9g::launch_lf_mask(s, (const vp9g::LfArgs *)(d + o_lf), R(t_mask), n, t_mask.wgs, mask_per_sb);
       if (recon && g->timing && !n_wave_intra) HIP_TRY(hipEventRecord(g->ev[3], s), "hipEventRecord");
       uint32_t lds = 0;
       for (int i = 0; i < n; i++) lds = std::max(lds, vp9g::wave_lds_bytes(f[i]->p->hdr.ss_x, f[i]->p->hdr.ss_y));
