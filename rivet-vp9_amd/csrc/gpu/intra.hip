@@ -578,12 +578,20 @@ DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (
   __syncthreads();
   const int16_t *res = (const int16_t *)ib_lds;
   uint16_t *tile = (uint16_t *)ib_lds;
+  // Each wave takes chunks of the level in turn: up to four 4x4 units of
+  // one transform block, a 4x4 per 16 lanes.  A wave's lanes then predict
+  // one mode, so the mode's branch and its edge reads run once per wave (a
+  // wave spanning several blocks ran every block's branch in turn, their
+  // reads one after another).
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   for (uint32_t lv = 0; lv < it.n_u; lv++) {
     const uint2 L = levels[lv];
     const int d0 = (int)(L.x & 0xffff), nd = (int)((L.x >> 16) & 0x7fff);
-    const int u0 = (int)(L.y & 0xffff), nu = (int)(L.y >> 16);
-    if (L.x >> 31) {
-      // DC values of the level's DC_PRED blocks, every edge read in flight at once
+    const int c0 = (int)(L.y & 0xffff), nc = (int)(L.y >> 16);
+    const bool dc_pass = (L.x >> 31) != 0;
+    if (dc_pass) {
+      // DC values of the level's DC_PRED blocks (it has one of 16x16 or
+      // larger), every edge read in flight at once
       if (tid < nd) {
         const IbTx d = txs[d0 + tid];
         if (d.mode() == DC_PRED) {
@@ -599,16 +607,19 @@ DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (
       }
       __syncthreads();
     }
-    for (int item = tid; item < nu * 16; item += kIbThreads) {
-      const uint32_t un = units[u0 + (item >> 4)];
-      const int di = (int)(un & 0xffff), sub = (int)(un >> 16), s16 = item & 15;
+    for (int c = wave; c < nc; c += kIbThreads / 64) {
+      const uint32_t ch = units[c0 + c];
+      if (lane >= 16 * (int)(ch >> 18)) continue;
+      const int di = (int)(ch & 0x3ff), sub = (int)((ch >> 10) & 0xff) + (lane >> 4), s16 = lane & 15;
       const IbTx d = txs[di];
-      const int l2 = (d.bits >> 4) & 15, size = 1 << l2;
+      const int l2 = (d.bits >> 4) & 15, size = 1 << l2, mode = d.mode();
       const int i = ((sub >> (l2 - 2)) << 2) + (s16 >> 2), j = ((sub & ((size >> 2) - 1)) << 2) + (s16 & 3);
-      const int mode = d.mode();
+      // DC from the DC pass, else (4x4 and 8x8 only) computed here
+      int dc = 0;
+      if (mode == DC_PRED) dc = dc_pass ? dcv[di] : (l2 == 2 ? d.dc<2>(bd) : d.dc<3>(bd));
       const int pv = intra_sample_at(
-          mode, i, j, size, [&](int k) { return d.A(k, size, base); }, [&](int k) { return d.L(k, base); },
-          mode == DC_PRED ? dcv[di] : 0, maxv);
+          mode, i, j, size, [&](int k) { return d.A(k, size, base); }, [&](int k) { return d.L(k, base); }, dc,
+          maxv);
       tile[d.tile + i * d.tp + j] = (uint16_t)iclamp(pv + res[d.res + i * d.rp + j], 0, maxv);
     }
     __syncthreads();

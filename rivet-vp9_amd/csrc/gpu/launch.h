@@ -136,12 +136,14 @@ struct IntraBlockItem {
 // A superblock item (an IntraBlockItem with kIbSuperblock in flags) holds
 // every intra block of one superblock: its transform blocks (all planes)
 // as descriptors sb_tx[first .. first + count), sorted by their level
-// inside the superblock; units of 4x4 samples sb_units[coef_first ..
-// + coef_count) (transform block index | 4x4 index in it << 16, by level);
-// levels sb_levels[sub_modes .. sub_modes + n_u), each x = first
-// descriptor | count << 16 | any DC_PRED << 31, y = first unit | count <<
-// 16 (relative to the item's first); (mi_row, mi_col) the superblock's
-// origin.
+// inside the superblock; chunks sb_units[coef_first .. + coef_count), by
+// level, each up to four 4x4 units of one transform block (descriptor
+// index | first unit << 10 | units << 18, units in raster order inside the
+// block), so that a wave's lanes predict one mode; levels
+// sb_levels[sub_modes .. sub_modes + n_u), each x = first descriptor |
+// count << 16 | (a DC_PRED block of 16x16 or larger) << 31, y = first chunk
+// | count << 16 (relative to the item's first); (mi_row, mi_col) the
+// superblock's origin.
 constexpr uint8_t kIbSuperblock = 0x80;
 // The most levels a superblock item may have (the host keeps frames with
 // more in the per-block form).

@@ -472,23 +472,24 @@ static bool build_intra_sb(Builder *g, const vp9r_record *rec) {
     o.mi_col = (uint16_t)(sc * 8);
     o.mi_size = BLOCK_64X64;
     o.flags = vp9g::kIbSuperblock;
-    uint32_t units = 0;
+    uint32_t chunks = 0;
     for (int l = 0; l < n_lv; l++) {
-      bool dc = false;
-      const uint32_t u0 = units;
+      bool dc = false;  // a DC_PRED block of 16x16 or larger: a DC pass first
+      const uint32_t c0 = chunks;
       for (uint32_t q = ls[l]; q < ls[l + 1]; q++) {
         const vp9g::IntraSbTx &d = descs[pos[q]];
         g->sb_tx.push_back(d);
-        dc = dc || d.mode == DC_PRED;
+        dc = dc || (d.mode == DC_PRED && d.log2 >= 4);
+        // chunks of at most four 4x4 units of one transform block
         const uint32_t n4 = 1u << (2 * (d.log2 - 2));
-        for (uint32_t sub = 0; sub < n4; sub++) g->sb_units.push_back(q | sub << 16);
-        units += n4;
+        for (uint32_t sub = 0; sub < n4; sub += 4, chunks++)
+          g->sb_units.push_back(q | sub << 10 | std::min(4u, n4 - sub) << 18);
       }
       g->sb_levels.push_back(make_uint2(ls[l] | (ls[l + 1] - ls[l]) << 16 | (dc ? 1u << 31 : 0u),
-                                        u0 | (units - u0) << 16));
+                                        c0 | (chunks - c0) << 16));
     }
-    if (units > M) return false;
-    o.coef_count = units;
+    if (chunks > M) return false;
+    o.coef_count = chunks;
   }
   g->block_items = items;
   g->item_level_start = lstart;
