@@ -117,7 +117,8 @@ DEV void lf_mask_wg(const LfArgs *fa, const Range *rg, uint32_t nf, uint32_t wg)
   for (int k = 0; k < K; k++) {
     const int pp = K == 1 ? j / n_sb : k, pass = pp & 1, plane = pp >> 1;
     int fs, lvl;
-    lf_decide(a, plane, pass, sb_row, sb_col, threadIdx.x >> 4, threadIdx.x & 15, fs, lvl);
+    // entry rn * 16 + ed: a run's decisions along its line are contiguous
+    lf_decide(a, plane, pass, sb_row, sb_col, threadIdx.x & 15, threadIdx.x >> 4, fs, lvl);
     const size_t o = ((size_t)pp * n_sb + sb) * 256 + threadIdx.x;
     a.dq[o] = fs < 0 ? 0u
                      : (uint32_t)(fs + 1) | (uint32_t)a.p.limit[lvl] << 8 | (uint32_t)a.p.blimit[lvl] << 16 |

@@ -38,9 +38,11 @@ DEV void lf_wide(S s) {
   constexpr int n = (1 << (LOG2 - 1)) - 1;
   auto at = [&](int k) -> int { return s(k < -(n + 1) ? -(n + 1) : (k > n ? n : k)); };
   int t[2 * n];
-  int tt = 0;
+  // T[-n]: s[-n - 1] n times (the clamped positions) and s[-n] .. s[0]
+  // once; a 24-bit multiply (full rate) for the n copies
+  int tt = __mul24(at(-(n + 1)), n);
 #pragma unroll
-  for (int j = -n; j <= n; j++) tt += at(-n + j);
+  for (int k = -n; k <= 0; k++) tt += at(k);
 #pragma unroll
   for (int i = -n; i < n; i++) {
     t[i + n] = tt;
@@ -78,18 +80,21 @@ DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
     flat2 = f2 <= one;
   }
   if (on && !flat) {
-    // narrow filter (8.8.5.2)
-    const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1, off = 0x80 << sh;
-    const int ps1 = p1 - off, ps0 = p0 - off, qs0 = q0 - off, qs1 = q1 - off;
-    int f = hev ? clampi(ps1 - qs1, lo, hi) : 0;
-    f = clampi(f + 3 * (qs0 - ps0), lo, hi);
-    const int f1 = clampi(f + 4, lo, hi) >> 3, f2 = clampi(f + 3, lo, hi) >> 3;
-    s(0) = clampi(qs0 - f1, lo, hi) + off;
-    s(-1) = clampi(ps0 + f2, lo, hi) + off;
+    // narrow filter (8.8.5.2).  The reference works on samples offset by
+    // 0x80 << (bd - 8) into [lo, hi]; differences of two samples lose the
+    // offset, and clamp(x - off, lo, hi) + off = clamp(x, 0, max), so the
+    // outputs are formed on the samples themselves.
+    const int lo = -(1 << (bd - 1)), hi = (1 << (bd - 1)) - 1, mx = (1 << bd) - 1;
+    int f = hev ? clampi(p1 - q1, lo, hi) : 0;
+    f = clampi(f + __mul24(q0 - p0, 3), lo, hi);
+    // f >= lo, so f + 4 and f + 3 only need the upper clamp
+    const int f1 = min(f + 4, hi) >> 3, f2 = min(f + 3, hi) >> 3;
+    s(0) = clampi(q0 - f1, 0, mx);
+    s(-1) = clampi(p0 + f2, 0, mx);
     if (!hev) {
       const int g = (f1 + 1) >> 1;
-      s(1) = clampi(qs1 - g, lo, hi) + off;
-      s(-2) = clampi(ps1 + g, lo, hi) + off;
+      s(1) = clampi(q1 - g, 0, mx);
+      s(-2) = clampi(p1 + g, 0, mx);
     }
   }
   // wide filter (8.8.5.3): log2 3 (8 taps) or 4 (16 taps)
@@ -241,27 +246,26 @@ DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const P *buf,
 
 // One pass: lane-owned lines of `len` samples, line i at buf + i * line,
 // samples `step` apart; edges at 8 + 4 e for e < n_edges (a multiple of 4).
-// dq[e * 16 + rn] packs the edge's decision: (filter size + 1) | limit << 8
-// | blimit << 16 | thresh << 24, 0 for no filter.  The window holds the 16
+// dq[rn * 16 + e] packs the decision of edge e, run rn: (filter size + 1) |
+// limit << 8 | blimit << 16 | thresh << 24, 0 for no filter; the four of
+// edges 4 m .. 4 m + 3 are one 16-byte LDS read.  The window holds the 16
 // samples p7 .. q7 of the current edge, position x of the line in w[x & 15]:
 // with the edge loop unrolled by 4 every index is a constant, and sliding
-// the window is renaming.  The next edge's decision and the 4 samples
-// entering the window are fetched before the filter (the filter of edge e
-// writes at most up to position 4 e + 14).
+// the window is renaming.  The 4 samples entering the window are fetched
+// before the filter (the filter of edge e writes at most up to position
+// 4 e + 14).
 // Edge e = 4 m + K of lf_stream (K a constant: every window index is one).
 template <int K, class T>
-DEV void lf_step(int (&w)[16], T *ln, int e, int step, int n_edges, const uint32_t *dq, int rn, uint32_t &d_next,
-                 int bd) {
+DEV void lf_step(int (&w)[16], T *ln, int e, int step, uint32_t d, int bd) {
   // Reads past the line's end (positions >= len, at most 4 n_edges + 11)
-  // stay inside LfShared and never reach a result: they enter the window
-  // after the last edge that could read them (the last edge is 4 apart from
-  // the line's end, a narrow filter reading q0 .. q3 only) and are not
-  // written back.  Likewise the decision after the last edge.
-  const uint32_t d = d_next;
-  d_next = dq[(e + 1) * 16 + rn];
+  // stay inside the LDS buffer and never reach a result: they enter the
+  // window after the last edge that could read them (the last edge is 4
+  // apart from the line's end, a narrow filter reading q0 .. q3 only) and
+  // are not written back.
   int nw[4];
+  // zero-extended here, beside the load (a later block would mask it again)
 #pragma unroll
-  for (int i = 0; i < 4; i++) nw[i] = ln[(4 * e + 16 + i) * step];
+  for (int i = 0; i < 4; i++) nw[i] = (int)__builtin_amdgcn_ubfe((uint32_t)ln[(4 * e + 16 + i) * step], 0, 8 * sizeof(T));
   if (d)
     lf_apply([&](int kk) -> int & { return w[(4 * K + 8 + kk) & 15]; }, (int)(d & 0xff) - 1, (d >> 8) & 0xff,
              (d >> 16) & 0xff, d >> 24, bd);
@@ -277,14 +281,20 @@ DEV void lf_stream(T *buf, int line_off, int step, int len, int n_edges, const u
   int w[16];
   T *ln = buf + line_off;
 #pragma unroll
-  for (int i = 0; i < 16; i++) w[i] = ln[i * step];
-  uint32_t d_next = dq[rn];
+  for (int i = 0; i < 16; i++) w[i] = (int)__builtin_amdgcn_ubfe((uint32_t)ln[i * step], 0, 8 * sizeof(T));
+  // the next group's decisions are read before this group's filters; past
+  // the last group the read lands in the next run's entries or the 16-entry
+  // slack and is not used
+  const uint4 *dr = (const uint4 *)(dq + rn * 16);
+  uint4 dn = dr[0];
 #pragma unroll 1
   for (int e = 0; e < n_edges; e += 4) {
-    lf_step<0>(w, ln, e, step, n_edges, dq, rn, d_next, bd);
-    lf_step<1>(w, ln, e + 1, step, n_edges, dq, rn, d_next, bd);
-    lf_step<2>(w, ln, e + 2, step, n_edges, dq, rn, d_next, bd);
-    lf_step<3>(w, ln, e + 3, step, n_edges, dq, rn, d_next, bd);
+    const uint4 d = dn;
+    dn = dr[(e >> 2) + 1];
+    lf_step<0>(w, ln, e, step, d.x, bd);
+    lf_step<1>(w, ln, e + 1, step, d.y, bd);
+    lf_step<2>(w, ln, e + 2, step, d.z, bd);
+    lf_step<3>(w, ln, e + 3, step, d.w, bd);
   }
   // the last window: positions 4 n_edges + i at w[i]
   const int base = 4 * n_edges;
