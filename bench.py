@@ -77,11 +77,12 @@ TESTS = os.path.join(ROOT, "tests")
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 STAGES = ("inter_pred", "inter_residual", "intra", "loop_filter")
-# intra: the loop filter decisions and the fused launches holding an intra
+# inter_pred: the margin clear, the loop filter decisions and the inter
+# prediction (one launch); intra: the fused launches holding an intra
 # level (the loop filter levels placed beside them run there too);
 # loop_filter: the filter levels after the last intra level (several per
 # k_lf_cone launch, a few one per k_lf launch)
-STAGE_KERNELS = ("k_prep + k_mc", "k_itx_inter", "k_wave", "k_lf_cone + k_lf")
+STAGE_KERNELS = ("k_prep_mc", "k_itx_inter", "k_wave", "k_lf_cone + k_lf")
 STAGE_BYTES = ("mc_bytes", "itx_bytes", "intra_bytes", "lf_bytes")
 
 

@@ -9,10 +9,11 @@ namespace vp9g {
 
 template <class P>
 __global__ __launch_bounds__(256) void k_clear_margins(const ClearArgs *fa, const Range *rg, uint32_t nf) {
-  clear_margins_wg<P>(fa, rg, nf, blockIdx.x);
+  clear_margins_wg<P, 256>(fa, rg, nf, blockIdx.x);
 }
 
-// Workgroups per plane the margins of a frame need (count of its range).
+// Workgroups per plane the margins of a frame need (count of its range
+// over 3): its longest row list in groups of kClearRows.
 uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g) {
   uint32_t n = 0;
   for (int p = 0; p < 3; p++) {
@@ -20,6 +21,7 @@ uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g) {
     const uint32_t mi_w = (g.mi_cols * 8) >> sx, mi_h = (g.mi_rows * 8) >> sy;
     uint32_t np = f.rows[p] - mi_h;
     if (f.stride[p] > mi_w) np += mi_h;
+    np = (np + kClearRows - 1) / kClearRows;
     n = np > n ? np : n;
   }
   return n;

@@ -14,28 +14,39 @@
 
 namespace vp9g {
 
-// Per frame, workgroup j of count = 3 * rows (256 lanes): plane j / rows,
-// and j % rows counts first the rows below the 8x8-aligned area (whole
-// rows), then the rows of the area (their part right of it).
-template <class P>
+// Per frame, workgroup j of count = 3 * groups (NT lanes): plane j /
+// groups, and j % groups takes kClearRows rows of the plane's row list:
+// first the rows below the 8x8-aligned area (whole rows), then the rows of
+// the area (their part right of it).
+template <class P, int NT>
 DEV void clear_margins_wg(const ClearArgs *fa, const Range *rg, uint32_t nf, uint32_t wg) {
   const uint32_t fi = range_frame(rg, nf, wg);
   const ClearArgs &a = fa[fi];
   const uint32_t j = wg - rg[fi].wg, per = rg[fi].count / 3;
   const int plane = j / per;
-  const uint32_t r = j - plane * per;
+  const uint32_t r0 = (j - plane * per) * kClearRows;
   const FrameGeom &g = a.g;
   const int sx = plane ? g.ss_x : 0, sy = plane ? g.ss_y : 0;
   const uint32_t mi_w = (g.mi_cols * 8) >> sx, mi_h = (g.mi_rows * 8) >> sy;
   const uint32_t rows = a.f.rows[plane], stride = a.f.stride[plane];
-  const uint32_t n_bottom = rows - mi_h;
+  const uint32_t n_bottom = rows - mi_h, n_right = stride > mi_w ? mi_h : 0;
   P *pl = (P *)a.f.p[plane];
-  if (r < n_bottom) {
+  // whole rows below the area
+  const uint32_t b_end = min(r0 + kClearRows, n_bottom);
+  for (uint32_t r = r0; r < b_end; r++) {
     P *row = pl + (size_t)(mi_h + r) * stride;
-    for (uint32_t x = threadIdx.x; x < stride; x += 256) row[x] = 0;
-  } else if (r - n_bottom < mi_h && stride > mi_w) {
-    P *row = pl + (size_t)(r - n_bottom) * stride;
-    for (uint32_t x = mi_w + threadIdx.x; x < stride; x += 256) row[x] = 0;
+    for (uint32_t x = threadIdx.x; x < stride; x += NT) row[x] = 0;
+  }
+  // the part right of the area of the group's list entries lo .. hi - 1
+  // (area rows lo - n_bottom ..), flattened over the group
+  const uint32_t lo = max(r0, n_bottom), hi = min(r0 + kClearRows, n_bottom + n_right);
+  if (hi > lo) {
+    const uint32_t mw = stride - mi_w, n = (hi - lo) * mw;
+    P *first = pl + (size_t)(lo - n_bottom) * stride + mi_w;
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+      const uint32_t r = i / mw;
+      first[(size_t)r * stride + (i - r * mw)] = 0;
+    }
   }
 }
 

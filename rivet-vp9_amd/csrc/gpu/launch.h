@@ -260,6 +260,7 @@ __host__ __device__ inline LfOne lf_one(const LfArgs &a, uint32_t t) {
 
 // Launchers: fa = per-frame arguments, r = ranges (n + 1 entries), all in
 // device memory; wgs = r[n].wg; hbd: samples are u16.
+constexpr uint32_t kClearRows = 16;  // rows per margin clear workgroup
 uint32_t clear_margin_rows(const Planes &f, const FrameGeom &g);
 void launch_clear_margins(hipStream_t s, const ClearArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd);
 // one_tiles: for a launch of one frame, its one_count tiles (device
@@ -273,6 +274,13 @@ constexpr int kMcWavesOne = 2;
 inline uint32_t mc_tiles_per_wg(uint32_t n_frames) { return n_frames == 1 ? kMcWavesOne : 1; }
 void launch_mc(hipStream_t s, const McArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd,
                const McTile *one_tiles, uint32_t one_count);
+// A single frame's first launch: its margin clear (n_clear workgroups, as
+// launch_clear_margins), loop filter decisions (n_mask, as launch_lf_mask
+// with per_sb entries per workgroup; 0 for none) and inter prediction
+// (n_tiles tiles, as launch_mc of one frame).
+void launch_prep_mc(hipStream_t s, const ClearArgs *fc, const Range *rc, uint32_t n_clear, const LfArgs *fl,
+                    const Range *rl, uint32_t n_mask, int per_sb, const McArgs *fa, const Range *rg,
+                    const McTile *tiles, uint32_t n_tiles, bool hbd);
 // r[k], wgs[k]: the ranges and workgroups of transform size k (4x4 .. 32x32)
 void launch_itx_inter(hipStream_t s, const ItxArgs *fa, const Range *const r[4], uint32_t n, const uint32_t wgs[4],
                       bool hbd);

@@ -1511,15 +1511,22 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       if (!gt)  // a graph's launch waits for the downloads before it
         for (int i = 0; i < n; i++) HIP_TRY(order_after_copy(s, f[i]->cur), "hipStreamWaitEvent");
       // the margin clear, with the loop filter decisions in the same launch
-      // when the filter follows in this call on this stream
-      if (fused && lf)
-        vp9g::launch_prep(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), t_clear.wgs,
-                          (const vp9g::LfArgs *)(d + o_lf), R(t_mask), t_mask.wgs, n, mask_per_sb, hbd);
-      else
-        vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
-      vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd,
-                      n == 1 ? (const vp9g::McTile *)(f[0]->dv + f[0]->p->o_tiles) : nullptr,
-                      n == 1 ? f[0]->p->n_tiles : 0u);
+      // when the filter follows in this call on this stream; a single
+      // frame's inter prediction in that launch too
+      const bool masks = fused && lf;
+      if (n == 1) {
+        vp9g::launch_prep_mc(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), t_clear.wgs,
+                             (const vp9g::LfArgs *)(d + o_lf), R(t_mask), masks ? t_mask.wgs : 0u, mask_per_sb,
+                             (const vp9g::McArgs *)(d + o_mc), R(t_mc),
+                             (const vp9g::McTile *)(f[0]->dv + f[0]->p->o_tiles), f[0]->p->n_tiles, hbd);
+      } else {
+        if (masks)
+          vp9g::launch_prep(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), t_clear.wgs,
+                            (const vp9g::LfArgs *)(d + o_lf), R(t_mask), t_mask.wgs, n, mask_per_sb, hbd);
+        else
+          vp9g::launch_clear_margins(s, (const vp9g::ClearArgs *)(d + o_clear), R(t_clear), n, t_clear.wgs, hbd);
+        vp9g::launch_mc(s, (const vp9g::McArgs *)(d + o_mc), R(t_mc), n, t_mc.wgs, hbd, nullptr, 0u);
+      }
       if (g->timing) HIP_TRY(hipEventRecord(g->ev[1], s), "hipEventRecord");
       {
         const vp9g::Range *rk[4] = {R(t_itx[0]), R(t_itx[1]), R(t_itx[2]), R(t_itx[3])};
