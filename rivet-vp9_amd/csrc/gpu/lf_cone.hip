@@ -151,9 +151,11 @@ DEV void cone_unit(LfConeShared<P, D> &sh, int slot, bool lone, int ur, int uc, 
     const int rr = sb / C, cc = sb - rr * C;
     if (vert) {
       // lines: the superblock's rows; edges from 8 samples into the line
+      // (each line starts 4-byte aligned: the pitch and SW are multiples of
+      // 4 bytes)
       if (lane < SH)
-        lf_stream(sh.buf, ((rr - r + D - 1) * SH + 8 + lane) * BP + (cc - c + D - 1) * SW, 1, SW + 8, SW / 4,
-                  sh.dq[slot + k], lane >> 2, bd);
+        lf_stream<P, true>(sh.buf, ((rr - r + D - 1) * SH + 8 + lane) * BP + (cc - c + D - 1) * SW, 1, SW + 8,
+                           SW / 4, sh.dq[slot + k], lane >> 2, bd);
     } else {
       if (lane < SW)
         lf_stream(sh.buf, (rr - r + D - 1) * SH * BP + 8 + (cc - c + D - 1) * SW + lane, BP, SH + 8, SH / 4,

@@ -59,6 +59,7 @@ DEV void lf_wide(S s) {
 // three filters are exclusive per lane and each rewrites its samples in
 // place: three independent conditional blocks, so that no sample is copied
 // to merge the paths.
+// fs >= 0: callers filter only where the decision has a filter.
 template <class S>
 DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
   const int q0 = s(0), q1 = s(1), q2 = s(2), q3 = s(3);
@@ -67,11 +68,14 @@ DEV void lf_apply(S s, int fs, int limit, int blimit, int thresh, int bd) {
   const uint32_t d10 = adiff(p1, p0), e10 = adiff(q1, q0);
   const uint32_t m = max(max(max(adiff(p3, p2), adiff(p2, p1)), d10), max(max(e10, adiff(q2, q1)), adiff(q3, q2)));
   const uint32_t edge = adiff(p0, q0) * 2 + (adiff(p1, q1) >> 1);
-  const bool on = fs >= 0 && m <= (uint32_t)(limit << sh) && edge <= (uint32_t)(blimit << sh);
+  const bool on = (m <= (uint32_t)(limit << sh)) & (edge <= (uint32_t)(blimit << sh));
   const bool hev = max(d10, e10) > (uint32_t)(thresh << sh);
   const uint32_t one = 1u << sh;
-  const bool flat = on && fs >= 1 &&
-                    max(max(max(d10, e10), max(adiff(p2, p0), adiff(q2, q0))), max(adiff(p3, p0), adiff(q3, q0))) <= one;
+  // the flatness maximum for every lane (no branch around it: most waves
+  // have a lane that needs it)
+  const uint32_t fl =
+      max(max(max(d10, e10), max(adiff(p2, p0), adiff(q2, q0))), max(adiff(p3, p0), adiff(q3, q0)));
+  const bool flat = on & (fs >= 1) & (fl <= one);
   bool flat2 = false;
   if (flat && fs >= 2) {
     uint32_t f2 = 0;
@@ -254,8 +258,47 @@ DEV void lf_region_store(P *pl, uint32_t stride, const LfItem &it, const P *buf,
 // the window is renaming.  The 4 samples entering the window are fetched
 // before the filter (the filter of edge e writes at most up to position
 // 4 e + 14).
+// ROW: the line's samples are contiguous (step 1) and its start 4-byte
+// aligned, so the 4 samples entering and the 4 leaving the window at each
+// edge are one dword (8 bits) or two (above): one LDS access each way, with
+// the unpacking and packing in registers (every window value is a sample,
+// in [0, max]).
+template <class T, bool ROW>
+DEV void lf_load4(const T *p, int step, int (&v)[4]) {
+  if (ROW && sizeof(T) == 1) {
+    const uint32_t q = *(const uint32_t *)p;
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = (int)__builtin_amdgcn_ubfe(q, 8 * i, 8);
+  } else if (ROW) {
+    const uint32_t q0 = ((const uint32_t *)p)[0], q1 = ((const uint32_t *)p)[1];
+    v[0] = (int)(q0 & 0xffff);
+    v[1] = (int)(q0 >> 16);
+    v[2] = (int)(q1 & 0xffff);
+    v[3] = (int)(q1 >> 16);
+  } else {
+    // zero-extended here, beside the load (a later block would mask it again)
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = (int)__builtin_amdgcn_ubfe((uint32_t)p[i * step], 0, 8 * sizeof(T));
+  }
+}
+
+template <class T, bool ROW>
+DEV void lf_store4(T *p, int step, int a, int b, int c, int d) {
+  if (ROW && sizeof(T) == 1) {
+    *(uint32_t *)p = (uint32_t)a | (uint32_t)b << 8 | (uint32_t)c << 16 | (uint32_t)d << 24;
+  } else if (ROW) {
+    ((uint32_t *)p)[0] = (uint32_t)a | (uint32_t)b << 16;
+    ((uint32_t *)p)[1] = (uint32_t)c | (uint32_t)d << 16;
+  } else {
+    p[0] = (T)a;
+    p[step] = (T)b;
+    p[2 * step] = (T)c;
+    p[3 * step] = (T)d;
+  }
+}
+
 // Edge e = 4 m + K of lf_stream (K a constant: every window index is one).
-template <int K, class T>
+template <int K, bool ROW, class T>
 DEV void lf_step(int (&w)[16], T *ln, int e, int step, uint32_t d, int bd) {
   // Reads past the line's end (positions >= len, at most 4 n_edges + 11)
   // stay inside the LDS buffer and never reach a result: they enter the
@@ -263,25 +306,26 @@ DEV void lf_step(int (&w)[16], T *ln, int e, int step, uint32_t d, int bd) {
   // apart from the line's end, a narrow filter reading q0 .. q3 only) and
   // are not written back.
   int nw[4];
-  // zero-extended here, beside the load (a later block would mask it again)
-#pragma unroll
-  for (int i = 0; i < 4; i++) nw[i] = (int)__builtin_amdgcn_ubfe((uint32_t)ln[(4 * e + 16 + i) * step], 0, 8 * sizeof(T));
+  lf_load4<T, ROW>(ln + (4 * e + 16) * step, step, nw);
   if (d)
     lf_apply([&](int kk) -> int & { return w[(4 * K + 8 + kk) & 15]; }, (int)(d & 0xff) - 1, (d >> 8) & 0xff,
              (d >> 16) & 0xff, d >> 24, bd);
+  lf_store4<T, ROW>(ln + 4 * e * step, step, w[4 * K], w[4 * K + 1], w[4 * K + 2], w[4 * K + 3]);
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    ln[(4 * e + i) * step] = (T)w[4 * K + i];
-    w[4 * K + i] = nw[i];
-  }
+  for (int i = 0; i < 4; i++) w[4 * K + i] = nw[i];
 }
 
-template <class T>
+template <class T, bool ROW = false>
 DEV void lf_stream(T *buf, int line_off, int step, int len, int n_edges, const uint32_t *dq, int rn, int bd) {
   int w[16];
   T *ln = buf + line_off;
 #pragma unroll
-  for (int i = 0; i < 16; i++) w[i] = (int)__builtin_amdgcn_ubfe((uint32_t)ln[i * step], 0, 8 * sizeof(T));
+  for (int i = 0; i < 4; i++) {
+    int v[4];
+    lf_load4<T, ROW>(ln + 4 * i * step, step, v);
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[4 * i + k] = v[k];
+  }
   // the next group's decisions are read before this group's filters; past
   // the last group the read lands in the next run's entries or the 16-entry
   // slack and is not used
@@ -291,16 +335,16 @@ DEV void lf_stream(T *buf, int line_off, int step, int len, int n_edges, const u
   for (int e = 0; e < n_edges; e += 4) {
     const uint4 d = dn;
     dn = dr[(e >> 2) + 1];
-    lf_step<0>(w, ln, e, step, d.x, bd);
-    lf_step<1>(w, ln, e + 1, step, d.y, bd);
-    lf_step<2>(w, ln, e + 2, step, d.z, bd);
-    lf_step<3>(w, ln, e + 3, step, d.w, bd);
+    lf_step<0, ROW>(w, ln, e, step, d.x, bd);
+    lf_step<1, ROW>(w, ln, e + 1, step, d.y, bd);
+    lf_step<2, ROW>(w, ln, e + 2, step, d.z, bd);
+    lf_step<3, ROW>(w, ln, e + 3, step, d.w, bd);
   }
   // the last window: positions 4 n_edges + i at w[i]
   const int base = 4 * n_edges;
 #pragma unroll
-  for (int i = 0; i < 16; i++)
-    if (base + i < len) ln[(base + i) * step] = (T)w[i];
+  for (int i = 0; i < 16; i += 4)
+    if (base + i < len) lf_store4<T, ROW>(ln + (base + i) * step, step, w[i], w[i + 1], w[i + 2], w[i + 3]);
 }
 
 // ---- one plane of one work item of level f.t
