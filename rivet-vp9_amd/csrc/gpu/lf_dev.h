@@ -415,10 +415,13 @@ DEV void lf_unit(uint8_t *plane_ptr, uint32_t stride, const uint32_t *dq, uint32
       if (active) lf_region_store<P, 0, 9>(pl, stride, it, sh.buf, true, lane);
       __syncthreads();
     }
-    const bool vert = pass != 1;
-    if (active && lane < (vert ? SH : SW))
-      lf_stream(sh.buf, vert ? (8 + lane) * BP + (pass == 2 ? SW : 0) : 8 + lane, vert ? 1 : BP,
-                (vert ? SW : SH) + 8, (vert ? SW : SH) / 4, sh.dq[pass], lane >> 2, HBD ? bit_depth : 8);
+    // vertical passes: rows, 4-byte aligned (BP and SW are multiples of 4
+    // bytes), packed LDS accesses; the horizontal pass: columns
+    if (active && pass != 1 && lane < SH)
+      lf_stream<P, true>(sh.buf, (8 + lane) * BP + (pass == 2 ? SW : 0), 1, SW + 8, SW / 4, sh.dq[pass], lane >> 2,
+                         HBD ? bit_depth : 8);
+    if (active && pass == 1 && lane < SW)
+      lf_stream<P, false>(sh.buf, 8 + lane, BP, SH + 8, SH / 4, sh.dq[pass], lane >> 2, HBD ? bit_depth : 8);
     __syncthreads();
   }
   if (!active) return;
