@@ -732,9 +732,12 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   const int gh0 = geo[0].H, gh1 = geo[1].H, gh2 = geo[2].H;
   const int gn0 = geo[0].nlv, gn1 = geo[1].nlv, gn2 = geo[2].nlv;
   const int nlv = max(max(gn0, gn1), gn2);
-  bool any_dc = (np0 && geo[0].mode_u == DC_PRED) || ((np1 + np2) && it.uv_mode == DC_PRED);
-  if (np0 && geo[0].mode_u < 0)
-    for (int q = 0; q < 4; q++) any_dc = any_dc || ((it.sub_modes >> (8 * q)) & 0xff) == DC_PRED;
+  // DC values: a 4x4 or 8x8 block's (8 or 16 edge reads) by each of its
+  // lanes in the sample pass; a larger block's once, in a pass of its own
+  // before the samples, run only by an item with such a block (the luma
+  // modes of a block below 8x8 are 4x4 transforms)
+  const bool any_dc = (np0 && lg0 >= 4 && geo[0].mode_u == DC_PRED) ||
+                      (((np1 && lg1 >= 4) || (np2 && lg2 >= 4)) && it.uv_mode == DC_PRED);
   // level lv of plane p: rows r_lo .. of the grid, n_l blocks
   auto level = [&](int p, int lv, int &r_lo) -> int {
     const int W = sel3(p, gw0, gw1, gw2), H = sel3(p, gh0, gh1, gh2), k = sel3(p, kk0, kk1, kk2);
@@ -752,16 +755,9 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
         const int jt = tid - (p == 0 ? 0 : p == 1 ? n0 : n0 + n1);
         const int r = (p == 0 ? r0 : p == 1 ? r1 : r2) + jt, c = lv - sel3(p, kk0, kk1, kk2) * r;
         const IbTx d = txs[p * 256 + r * 16 + c];
-        if (d.valid() && d.mode() == DC_PRED) {
-          int v;
-          switch (sel3(p, lg0, lg1, lg2)) {  // every edge read in flight at once
-            case 2: v = d.dc<2>(bd); break;
-            case 3: v = d.dc<3>(bd); break;
-            case 4: v = d.dc<4>(bd); break;
-            default: v = d.dc<5>(bd); break;
-          }
-          dcv[p * 16 + jt] = v;
-        }
+        const int l2 = sel3(p, lg0, lg1, lg2);
+        if (d.valid() && d.mode() == DC_PRED && l2 >= 4)  // every edge read in flight at once
+          dcv[p * 16 + jt] = l2 == 4 ? d.dc<4>(bd) : d.dc<5>(bd);
       }
       __syncthreads();
     }
@@ -777,7 +773,7 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
       const int size = 1 << l2, i = s >> l2, j = s & (size - 1);
       const int pv = intra_sample_at(
           mode, i, j, size, [&](int k) { return d.A(k, size, base); }, [&](int k) { return d.L(k, base); },
-          mode == DC_PRED ? dcv[p * 16 + jt] : 0, maxv);
+          mode != DC_PRED ? 0 : l2 >= 4 ? dcv[p * 16 + jt] : l2 == 3 ? d.dc<3>(bd) : d.dc<2>(bd), maxv);
       const int16_t *res = (const int16_t *)ib_lds;
       uint16_t *tile = (uint16_t *)ib_lds;
       tile[d.tile + i * d.tp + j] = (uint16_t)iclamp(pv + res[d.res + i * d.rp + j], 0, maxv);
