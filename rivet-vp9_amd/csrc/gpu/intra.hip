@@ -286,7 +286,7 @@ __global__ __launch_bounds__(64) void k_intra(const IntraArgs *fa, const Range *
   }
 }
 
-// ---- intra blocks (k_intra_blocks): one workgroup of kIbThreads (512) lanes
+// ---- intra blocks (k_intra_blocks): one workgroup of kIbThreads (1024) lanes
 // per block
 //
 // The block's three planes are independent (each reads only its own
