@@ -286,7 +286,7 @@ __global__ __launch_bounds__(64) void k_intra(const IntraArgs *fa, const Range *
   }
 }
 
-// ---- intra blocks (k_intra_blocks): one workgroup of kIbThreads (1024) lanes
+// ---- intra blocks (k_intra_blocks): one workgroup of ib_threads lanes
 // per block
 //
 // The block's three planes are independent (each reads only its own
@@ -388,7 +388,7 @@ struct IbTx {
 // rg (the block or superblock, the row above it from above-left and the
 // column left of it; samples outside the 8x8-aligned area are not
 // reconstructed and keep the frame's), and the residuals of its area.
-template <bool HBD>
+template <bool HBD, int NT>
 DEV void ib_stage(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, const IbRegion &rg1,
                   const IbRegion &rg2, uint8_t *const (&pl_)[3], const int16_t *const (&rs_)[3],
                   const uint32_t (&st_)[3], int tid) {
@@ -399,14 +399,14 @@ DEV void ib_stage(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, c
   // All loads of the
   // three planes are in flight before the first LDS write: at most
   // kIbRegionSlots items per lane (4:4:4 64x64: 3 x (65 x 8 + 65) items).
-  constexpr int kIbRegionSlots = (3 * (65 * 8 + 65) + kIbThreads - 1) / kIbThreads;
-  constexpr int kIbResSlots = (3 * 512 + kIbThreads - 1) / kIbThreads;
+  constexpr int kIbRegionSlots = (3 * (65 * 8 + 65) + NT - 1) / NT;
+  constexpr int kIbResSlots = (3 * 512 + NT - 1) / NT;
   const int nr0 = rg0.n ? rg0.BH << rg0.lq : 0, nr1 = rg1.n ? rg1.BH << rg1.lq : 0, nr2 = rg2.n ? rg2.BH << rg2.lq : 0;
   {
     uint4 v[kIbRegionSlots], rv[kIbResSlots];
 #pragma unroll
     for (int i = 0; i < kIbRegionSlots; i++) {
-      const int k = tid + kIbThreads * i;
+      const int k = tid + NT * i;
       const int p = k < rg0.n ? 0 : k < rg0.n + rg1.n ? 1 : 2;
       const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
@@ -438,7 +438,7 @@ DEV void ib_stage(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, c
     // V samples: at most kIbResSlots per lane (4:4:4 64x64: 3 x 512)
 #pragma unroll
     for (int i = 0; i < kIbResSlots; i++) {
-      const int k = tid + kIbThreads * i;
+      const int k = tid + NT * i;
       const int p = k < nr0 ? 0 : k < nr0 + nr1 ? 1 : 2;
       const int local = k - (p == 0 ? 0 : p == 1 ? nr0 : nr0 + nr1);
       if (local >= (p == 0 ? nr0 : p == 1 ? nr1 : nr2)) continue;
@@ -454,7 +454,7 @@ DEV void ib_stage(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, c
     }
 #pragma unroll
     for (int i = 0; i < kIbResSlots; i++) {
-      const int k = tid + kIbThreads * i;
+      const int k = tid + NT * i;
       const int p = k < nr0 ? 0 : k < nr0 + nr1 ? 1 : 2;
       const int local = k - (p == 0 ? 0 : p == 1 ? nr0 : nr0 + nr1);
       if (local >= (p == 0 ? nr0 : p == 1 ? nr1 : nr2)) continue;
@@ -468,7 +468,7 @@ DEV void ib_stage(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, c
     }
 #pragma unroll
     for (int i = 0; i < kIbRegionSlots; i++) {
-      const int k = tid + kIbThreads * i;
+      const int k = tid + NT * i;
       const int p = k < rg0.n ? 0 : k < rg0.n + rg1.n ? 1 : 2;
       const IbRegion R = p == 0 ? rg0 : p == 1 ? rg1 : rg2;
       const int local = k - (p == 0 ? 0 : p == 1 ? rg0.n : rg0.n + rg1.n);
@@ -496,7 +496,7 @@ DEV void ib_stage(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, c
 }
 
 // Writes the regions' areas back from LDS to the frame.
-template <bool HBD>
+template <bool HBD, int NT>
 DEV void ib_store(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, const IbRegion &rg1,
                   const IbRegion &rg2, uint8_t *const (&pl_)[3], const uint32_t (&st_)[3], int tid) {
   typedef typename std::conditional<HBD, uint16_t, uint8_t>::type P;
@@ -509,7 +509,7 @@ DEV void ib_store(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, c
     const uint16_t *tile = (const uint16_t *)(ib_lds + ly.tile[p]);
     const size_t stride = st_[p];
     P *pl = (P *)pl_[p];
-    for (int k = tid; k < (R.BH << R.lq); k += kIbThreads) {
+    for (int k = tid; k < (R.BH << R.lq); k += NT) {
       const int r = k >> R.lq, q = k & ((1 << R.lq) - 1);
       const uint16_t *src = tile + (r + 1) * tp + 1 + q * R.V;
       P *dst = pl + (size_t)(R.y0 + r) * stride + R.x0 + q * R.V;
@@ -541,7 +541,7 @@ DEV void ib_store(const FrameGeom &g, const IbLayout &ly, const IbRegion &rg0, c
 // of earlier items) with a barrier between levels; the region goes back at
 // the end.  The host levels the transform blocks (intra_sb_items) and the
 // items by the superblocks left, above and above-left of them.
-template <bool HBD>
+template <bool HBD, int NT>
 DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (&pl_)[3],
                   const int16_t *const (&rs_)[3], const uint32_t (&st_)[3]) {
   const int tid = threadIdx.x;
@@ -556,7 +556,7 @@ DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (
   const IbRegion rg0 = ib_region(it, g, 0, 8, 8, true), rg1 = ib_region(it, g, 1, 8, 8, true),
                  rg2 = ib_region(it, g, 2, 8, 8, true);
   // descriptors, units and levels first: their loads overlap the region's
-  for (uint32_t k = tid; k < it.count; k += kIbThreads) {
+  for (uint32_t k = tid; k < it.count; k += NT) {
     const IntraSbTx d = a.sb_tx[it.first + k];
     const int p = d.plane, sx = p ? g.ss_x : 0;
     const uint32_t tile_off = p == 0 ? ly.tile[0] : p == 1 ? ly.tile[1] : ly.tile[2];
@@ -572,9 +572,9 @@ DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (
     t._pad = 0;
     txs[k] = t;
   }
-  for (uint32_t k = tid; k < it.coef_count; k += kIbThreads) units[k] = a.sb_units[it.coef_first + k];
-  for (uint32_t k = tid; k < it.n_u; k += kIbThreads) levels[k] = a.sb_levels[it.sub_modes + k];
-  ib_stage<HBD>(g, ly, rg0, rg1, rg2, pl_, rs_, st_, tid);
+  for (uint32_t k = tid; k < it.coef_count; k += NT) units[k] = a.sb_units[it.coef_first + k];
+  for (uint32_t k = tid; k < it.n_u; k += NT) levels[k] = a.sb_levels[it.sub_modes + k];
+  ib_stage<HBD, NT>(g, ly, rg0, rg1, rg2, pl_, rs_, st_, tid);
   __syncthreads();
   const int16_t *res = (const int16_t *)ib_lds;
   uint16_t *tile = (uint16_t *)ib_lds;
@@ -607,7 +607,7 @@ DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (
       }
       __syncthreads();
     }
-    for (int c = wave; c < nc; c += kIbThreads / 64) {
+    for (int c = wave; c < nc; c += NT / 64) {
       const uint32_t ch = units[c0 + c];
       if (lane >= 16 * (int)(ch >> 18)) continue;
       const int di = (int)(ch & 0x3ff), sub = (int)((ch >> 10) & 0xff) + (lane >> 4), s16 = lane & 15;
@@ -624,11 +624,11 @@ DEV void intra_sb(const IntraArgs &a, const IntraBlockItem &it, uint8_t *const (
     }
     __syncthreads();
   }
-  ib_store<HBD>(g, ly, rg0, rg1, rg2, pl_, st_, tid);
+  ib_store<HBD, NT>(g, ly, rg0, rg1, rg2, pl_, st_, tid);
 }
 
-// Intra block wg of the launch (kIbThreads lanes).
-template <bool HBD, bool ONE>
+// Intra block wg of the launch (NT lanes: ib_threads(ONE)).
+template <bool HBD, bool ONE, int NT = ib_threads(ONE)>
 DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const IntraBlockItem *items0, uint8_t *cp0,
                      uint8_t *cp1, uint8_t *cp2, const int16_t *rp0, const int16_t *rp1, const int16_t *rp2,
                      const vp9r_txb *txbs1, uint32_t wg) {
@@ -649,7 +649,7 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   const int16_t *const rs_[3] = {ONE ? rp0 : a.res[0], ONE ? rp1 : a.res[1], ONE ? rp2 : a.res[2]};
   const uint32_t st_[3] = {a.cur.stride[0], a.cur.stride[1], a.cur.stride[2]};
   if (it.flags & kIbSuperblock) {
-    intra_sb<HBD>(a, it, pl_, rs_, st_);
+    intra_sb<HBD, NT>(a, it, pl_, rs_, st_);
     return;
   }
   const vp9r_txb *txbs = ONE ? txbs1 : a.txbs;
@@ -660,7 +660,7 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
   // the transform block records, all in LDS (at most ib_max_txbs; the host
   // checks each plane's grid, which bounds the count)
-  for (uint32_t k = tid; k < it.count; k += kIbThreads) stx[k] = txbs[it.first + k];
+  for (uint32_t k = tid; k < it.count; k += NT) stx[k] = txbs[it.first + k];
   auto txb_at = [&](uint32_t k) -> vp9r_txb { return stx[k]; };
   const int mi_size = it.mi_size;
   const bool avail_u = (it.flags & 8) != 0, avail_l = (it.flags & 16) != 0;
@@ -670,8 +670,8 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   const uint32_t np0 = it.n_y, np1 = it.n_u, np2 = it.count - it.n_y - it.n_u;
   const IbRegion rg0 = ib_region(it, g, 0, bw8, bh8, np0 > 0), rg1 = ib_region(it, g, 1, bw8, bh8, np1 > 0),
                  rg2 = ib_region(it, g, 2, bw8, bh8, np2 > 0);
-  ib_stage<HBD>(g, ly, rg0, rg1, rg2, pl_, rs_, st_, tid);
-  for (int k = tid; k < 3 * 256; k += kIbThreads) txs[k].bits = 0;
+  ib_stage<HBD, NT>(g, ly, rg0, rg1, rg2, pl_, rs_, st_, tid);
+  for (int k = tid; k < 3 * 256; k += NT) txs[k].bits = 0;
   __syncthreads();
   if (tid < 3) {
     const int p = tid;
@@ -700,7 +700,7 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   }
   __syncthreads();
   // every transform block's descriptor in its grid cell
-  for (uint32_t j = tid; j < it.count; j += kIbThreads) {
+  for (uint32_t j = tid; j < it.count; j += NT) {
     const vp9r_txb t = txb_at(j);
     const IbGeo &G = geo[t.plane];
     const int tx = t.x - G.x0, ty = t.y - G.y0, xi = tx >> 2, yi = ty >> 2, step4 = 1 << (G.log2 - 2);
@@ -762,7 +762,7 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
       __syncthreads();
     }
     const int i0 = n0 << (2 * lg0), i1 = n1 << (2 * lg1), i2 = n2 << (2 * lg2);
-    for (int item = tid; item < i0 + i1 + i2; item += kIbThreads) {
+    for (int item = tid; item < i0 + i1 + i2; item += NT) {
       const int p = item < i0 ? 0 : item < i0 + i1 ? 1 : 2;
       const int local = item - (p == 0 ? 0 : p == 1 ? i0 : i0 + i1);
       const int l2 = sel3(p, lg0, lg1, lg2), jt = local >> (2 * l2), s = local & ((1 << (2 * l2)) - 1);
@@ -780,7 +780,7 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
     }
     __syncthreads();
   }
-  ib_store<HBD>(g, ly, rg0, rg1, rg2, pl_, st_, tid);
+  ib_store<HBD, NT>(g, ly, rg0, rg1, rg2, pl_, st_, tid);
 }
 
 void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n, uint32_t wgs, bool hbd) {
@@ -804,7 +804,7 @@ void launch_intra(hipStream_t s, const IntraArgs *fa, const Range *r, uint32_t n
 // as arguments (the first 16 dwords preloaded, all known to be global
 // memory); a batch goes through the range tables of both parts.
 template <bool HBD, bool ONE>
-__global__ __launch_bounds__(kIbThreads) void k_wave(uint32_t n_iw, uint32_t geo, uint32_t t1, uint8_t *p0,
+__global__ __launch_bounds__(ib_threads(ONE)) void k_wave(uint32_t n_iw, uint32_t geo, uint32_t t1, uint8_t *p0,
                                                      uint8_t *p1, uint8_t *p2, uint32_t *dq, uint32_t s0, uint32_t s1,
                                                      uint32_t s2, const IntraBlockItem *items0, const int16_t *rp0,
                                                      const int16_t *rp1, const int16_t *rp2, const vp9r_txb *txbs1,
@@ -858,7 +858,7 @@ void launch_wave(hipStream_t s, const WaveLaunch &w, bool hbd) {
       }
       txbs = w.one_i->txbs;
     }
-    hipLaunchKernelGGL(kernel, dim3(w.n_iw + w.n_lw), dim3(kIbThreads), w.lds_bytes, s, w.n_iw, lo.geo, lo.t, p[0],
+    hipLaunchKernelGGL(kernel, dim3(w.n_iw + w.n_lw), dim3(ib_threads(one)), w.lds_bytes, s, w.n_iw, lo.geo, lo.t, p[0],
                        p[1], p[2], lo.dq, lo.stride[0], lo.stride[1], lo.stride[2], w.items0, rp[0], rp[1], rp[2],
                        txbs, w.fa_i, w.rg_i, w.fa_l, w.rg_l, w.nf);
   };

@@ -160,17 +160,20 @@ struct IntraSbTx {
   uint16_t _pad[3];
 };
 
-// LDS of one k_intra_blocks workgroup (kIbThreads lanes) for a frame of
+// LDS of one k_intra_blocks workgroup for a frame of
 // subsampling (ss_x, ss_y): byte offsets of the transform block records
 // (all of the block's), the three transform grids' descriptors (32 B per
 // cell), DC values, plane geometry, and per plane the region tile ((64 >> sx) + 1 columns, u16) and
 // residuals ((64 >> sx) columns, int16).  The launch passes `bytes` (the
 // largest of its frames') as dynamic LDS.
-// 1024 lanes (16 waves, the most a workgroup takes): the large transform
-// blocks' levels (up to 3 x 1024 samples) are SIMD-bound at 8 waves and
-// their LDS reads wait less with 16 (1080p intra stage 0.147 -> 0.141 ms
-// per frame against 512 lanes).
-constexpr int kIbThreads = 1024, kIbGeoBytes = 80;
+// Lanes of an intra item's workgroup: a single frame's launches take 1024
+// (16 waves, the most a workgroup takes: the large transform blocks' levels,
+// up to 3 x 1024 samples, are SIMD-bound at 8 waves and their LDS reads wait
+// less with 16; 1080p intra stage 0.147 -> 0.141 ms per frame against 512),
+// a batch's 512 (with items of many frames on the chip, 1024-lane
+// workgroups made a batch of 64 frames 12% slower).
+constexpr int kIbThreadsOne = 1024, kIbThreadsBatch = 512, kIbGeoBytes = 80;
+constexpr int ib_threads(bool one) { return one ? kIbThreadsOne : kIbThreadsBatch; }
 // The most transform blocks an intra block has: 64x64 at 4x4 in every plane.
 __host__ __device__ inline uint32_t ib_max_txbs(int ss_x, int ss_y) { return 256 + 2 * (256u >> (ss_x + ss_y)); }
 struct IbLayout {
