@@ -657,7 +657,6 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   vp9r_txb *stx = (vp9r_txb *)(ib_lds + ly.txb);
   IbTx *txs = (IbTx *)(ib_lds + ly.txs);
   int32_t *dcv = (int32_t *)(ib_lds + ly.dcv);
-  IbGeo *geo = (IbGeo *)(ib_lds + ly.geo);
   // the transform block records, all in LDS (at most ib_max_txbs; the host
   // checks each plane's grid, which bounds the count)
   for (uint32_t k = tid; k < it.count; k += NT) stx[k] = txbs[it.first + k];
@@ -673,70 +672,55 @@ DEV void intra_block(const IntraArgs *fa, const Range *rg, uint32_t nf, const In
   ib_stage<HBD, NT>(g, ly, rg0, rg1, rg2, pl_, rs_, st_, tid);
   for (int k = tid; k < 3 * 256; k += NT) txs[k].bits = 0;
   __syncthreads();
-  if (tid < 3) {
-    const int p = tid;
-    const uint32_t np = p == 0 ? np0 : p == 1 ? np1 : np2, k0 = p == 0 ? 0 : p == 1 ? np0 : np0 + np1;
-    const int sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
-    IbGeo G;
-    G.x0 = (it.mi_col * 8) >> sx;
-    G.y0 = (it.mi_row * 8) >> sy;
-    G.BW = (bw8 * 8) >> sx;
-    G.BH = (bh8 * 8) >> sy;
-    const int tx = np ? txb_at(k0).tx_size : 0;
-    G.log2 = tx + 2;
-    G.kk = tx == TX_4X4 ? 2 : 1;
-    G.W = max(G.BW >> G.log2, 1);
-    G.H = max(G.BH >> G.log2, 1);
-    G.nlv = np ? (G.W - 1) + G.kk * (G.H - 1) + 1 : 0;
-    G.tpitch = (64 >> sx) + 1;
-    G.rpitch = 64 >> sx;
-    G.max_x = (int)((g.mi_cols * 8) >> sx) - 1;
-    G.max_y = (int)((g.mi_rows * 8) >> sy) - 1;
-    G.n4w = num4x4_wide(bsize) >> sx;
-    G.mode_u = p > 0 ? it.uv_mode : (mi_size >= BLOCK_8X8 ? it.y_mode : -1);
-    G.tile_off = p == 0 ? ly.tile[0] : p == 1 ? ly.tile[1] : ly.tile[2];
-    G.res_off = p == 0 ? ly.res[0] : p == 1 ? ly.res[1] : ly.res[2];
-    geo[p] = G;
-  }
-  __syncthreads();
+  // the planes' grid geometry, by every lane (a plane's transform size from
+  // its first record, a broadcast LDS read), in registers: no barrier for a
+  // table of it (selected per plane, not indexed: an indexed private array
+  // is memory)
+  auto sel3 = [](int p, int a, int b, int c) { return p == 0 ? a : p == 1 ? b : c; };
+  const int tx0 = np0 ? txb_at(0).tx_size : 0, tx1 = np1 ? txb_at(np0).tx_size : 0,
+            tx2 = np2 ? txb_at(np0 + np1).tx_size : 0;
+  const int lg0 = tx0 + 2, lg1 = tx1 + 2, lg2 = tx2 + 2;
+  const int kk0 = tx0 == TX_4X4 ? 2 : 1, kk1 = tx1 == TX_4X4 ? 2 : 1, kk2 = tx2 == TX_4X4 ? 2 : 1;
+  const int bw0 = bw8 * 8, bh0 = bh8 * 8, bwc = (bw8 * 8) >> g.ss_x, bhc = (bh8 * 8) >> g.ss_y;
+  const int gw0 = max(bw0 >> lg0, 1), gw1 = max(bwc >> lg1, 1), gw2 = max(bwc >> lg2, 1);
+  const int gh0 = max(bh0 >> lg0, 1), gh1 = max(bhc >> lg1, 1), gh2 = max(bhc >> lg2, 1);
+  const int gn0 = np0 ? (gw0 - 1) + kk0 * (gh0 - 1) + 1 : 0, gn1 = np1 ? (gw1 - 1) + kk1 * (gh1 - 1) + 1 : 0,
+            gn2 = np2 ? (gw2 - 1) + kk2 * (gh2 - 1) + 1 : 0;
+  const int mode_y = mi_size >= BLOCK_8X8 ? it.y_mode : -1;
   // every transform block's descriptor in its grid cell
   for (uint32_t j = tid; j < it.count; j += NT) {
     const vp9r_txb t = txb_at(j);
-    const IbGeo &G = geo[t.plane];
-    const int tx = t.x - G.x0, ty = t.y - G.y0, xi = tx >> 2, yi = ty >> 2, step4 = 1 << (G.log2 - 2);
-    int mode = G.mode_u;
+    const int p = t.plane, sx = p ? g.ss_x : 0, sy = p ? g.ss_y : 0;
+    const int x0 = (it.mi_col * 8) >> sx, y0 = (it.mi_row * 8) >> sy, log2 = sel3(p, lg0, lg1, lg2);
+    const int tpitch = (64 >> sx) + 1, rpitch = 64 >> sx, n4w = num4x4_wide(bsize) >> sx;
+    const uint32_t tile_off = p == 0 ? ly.tile[0] : p == 1 ? ly.tile[1] : ly.tile[2];
+    const uint32_t res_off = p == 0 ? ly.res[0] : p == 1 ? ly.res[1] : ly.res[2];
+    const int tx = t.x - x0, ty = t.y - y0, xi = tx >> 2, yi = ty >> 2, step4 = 1 << (log2 - 2);
+    int mode = p > 0 ? it.uv_mode : mode_y;
     if (mode < 0) {
-      const int block_idx = (yi / step4) * ((G.n4w + step4 - 1) / step4) + xi / step4;
+      const int block_idx = (yi / step4) * ((n4w + step4 - 1) / step4) + xi / step4;
       mode = (int)((it.sub_modes >> (8 * (block_idx & 3))) & 0xff);
     }
     IbTx d;
-    d.tp = G.tpitch;
-    d.tile = (int)(G.tile_off >> 1) + (ty + 1) * G.tpitch + tx + 1;
-    d.rp = G.rpitch;
-    d.res = (int)(G.res_off >> 1) + ty * G.rpitch + tx;
+    d.tp = tpitch;
+    d.tile = (int)(tile_off >> 1) + (ty + 1) * tpitch + tx + 1;
+    d.rp = rpitch;
+    d.res = (int)(res_off >> 1) + ty * rpitch + tx;
     const bool have_left = avail_l || xi > 0, have_above = avail_u || yi > 0;
-    const bool a4 = G.log2 == 2 && xi + step4 < G.n4w;
-    d.bits = mode | G.log2 << 4 | (int)have_above << 8 | (int)have_left << 9 | (int)a4 << 10 | 1 << 11;
-    d.lim_a = G.max_x - (int)t.x;
-    d.lim_l = G.max_y - (int)t.y;
+    const bool a4 = log2 == 2 && xi + step4 < n4w;
+    d.bits = mode | log2 << 4 | (int)have_above << 8 | (int)have_left << 9 | (int)a4 << 10 | 1 << 11;
+    d.lim_a = (int)((g.mi_cols * 8) >> sx) - 1 - (int)t.x;
+    d.lim_l = (int)((g.mi_rows * 8) >> sy) - 1 - (int)t.y;
     d._pad = 0;
-    txs[t.plane * 256 + (ty >> G.log2) * 16 + (tx >> G.log2)] = d;
+    txs[p * 256 + (ty >> log2) * 16 + (tx >> log2)] = d;
   }
   __syncthreads();
-  // the wavefront over the three grids; the planes' grid shapes in registers
-  // (selected per plane, not indexed: an indexed private array is memory)
-  auto sel3 = [](int p, int a, int b, int c) { return p == 0 ? a : p == 1 ? b : c; };
-  const int lg0 = geo[0].log2, lg1 = geo[1].log2, lg2 = geo[2].log2;
-  const int kk0 = geo[0].kk, kk1 = geo[1].kk, kk2 = geo[2].kk;
-  const int gw0 = geo[0].W, gw1 = geo[1].W, gw2 = geo[2].W;
-  const int gh0 = geo[0].H, gh1 = geo[1].H, gh2 = geo[2].H;
-  const int gn0 = geo[0].nlv, gn1 = geo[1].nlv, gn2 = geo[2].nlv;
   const int nlv = max(max(gn0, gn1), gn2);
   // DC values: a 4x4 or 8x8 block's (8 or 16 edge reads) by each of its
   // lanes in the sample pass; a larger block's once, in a pass of its own
   // before the samples, run only by an item with such a block (the luma
   // modes of a block below 8x8 are 4x4 transforms)
-  const bool any_dc = (np0 && lg0 >= 4 && geo[0].mode_u == DC_PRED) ||
+  const bool any_dc = (np0 && lg0 >= 4 && mode_y == DC_PRED) ||
                       (((np1 && lg1 >= 4) || (np2 && lg2 >= 4)) && it.uv_mode == DC_PRED);
   // level lv of plane p: rows r_lo .. of the grid, n_l blocks
   auto level = [&](int p, int lv, int &r_lo) -> int {
