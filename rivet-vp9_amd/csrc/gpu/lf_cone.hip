@@ -47,13 +47,16 @@ namespace vp9g {
 // to 11 samples past a line's end (never into a result): 4 slack rows
 // below, and the vertical lines of superblock c + 1 read up to LDS column
 // (D + 1) 64 + 11 (in the row at 8 bits, the next row's first samples
-// above).  Decisions: three passes per cone unit (only U(0, 0) has three).
+// above).  Decisions: two passes per cone unit (U(0, 0)'s third apart).
 template <class T, int D>
 struct __align__(16) LfConeShared {
   static constexpr int BP = sizeof(T) == 1 ? (D + 1) * 64 + 12 : (D + 1) * 64 + 10, ROWS = 8 + D * 64 + 4;
-  static constexpr int UNITS = D * (D + 1) / 2;
+  static constexpr int UNITS = D * (D + 1) / 2, EXTRA = 2 * UNITS;
   T buf[ROWS * BP];
-  uint32_t dq[3 * UNITS][272];
+  // two passes per unit (slots 2 u, 2 u + 1), and the first vertical pass
+  // of unit (0, 0), its third, in slot EXTRA; 256 entries and the 4 that
+  // lf_stream reads ahead past the last run's
+  uint32_t dq[2 * UNITS + 1][260];
 };
 
 // A rectangle of whole 16-byte vectors (16 samples at 8 bits, 8 above):
@@ -143,23 +146,25 @@ template <class P, int D>
 DEV void cone_unit(LfConeShared<P, D> &sh, int slot, bool lone, int ur, int uc, int r, int c, int C, int SW, int SH,
                    int lane, int bd) {
   constexpr int BP = LfConeShared<P, D>::BP;
+  const bool three = !lone && ur == 0 && uc == 0;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     int sb;
     bool vert;
     if (!cone_pass(lone, ur, uc, C, k, sb, vert)) break;
     const int rr = sb / C, cc = sb - rr * C;
+    const int q = three ? (k == 0 ? LfConeShared<P, D>::EXTRA : slot + k - 1) : slot + k;
     if (vert) {
       // lines: the superblock's rows; edges from 8 samples into the line
       // (each line starts 4-byte aligned: the pitch and SW are multiples of
       // 4 bytes)
       if (lane < SH)
         lf_stream<P, true>(sh.buf, ((rr - r + D - 1) * SH + 8 + lane) * BP + (cc - c + D - 1) * SW, 1, SW + 8,
-                           SW / 4, sh.dq[slot + k], lane >> 2, bd);
+                           SW / 4, sh.dq[q], lane >> 2, bd);
     } else {
       if (lane < SW)
         lf_stream(sh.buf, (rr - r + D - 1) * SH * BP + 8 + (cc - c + D - 1) * SW + lane, BP, SH + 8, SH / 4,
-                  sh.dq[slot + k], lane >> 2, bd);
+                  sh.dq[q], lane >> 2, bd);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -288,8 +293,9 @@ __global__ __launch_bounds__(64 * D) void k_lf_cone(uint8_t *i0, uint8_t *i1, ui
     cone_load<P, 64, NS>(in, stride, LfRect{rr * SH, SH, cc * SW, row_on && q <= D - k && cc >= 0 && cc < C}, lq,
                          vs[q], lane);
   }
-  // the decisions of the wave's units: unit m in slots 3 (idx(k, m)) ..
-  const int base = 3 * (k * D - k * (k - 1) / 2);
+  // the decisions of the wave's units: unit m in slots 2 (idx(k, m)) and
+  // the next (U(0, 0)'s first vertical pass in slot EXTRA)
+  const int base = 2 * (k * D - k * (k - 1) / 2);
   uint4 d[D][3];
 #pragma unroll
   for (int m = 0; m < D; m++) {
@@ -305,10 +311,16 @@ __global__ __launch_bounds__(64 * D) void k_lf_cone(uint8_t *i0, uint8_t *i1, ui
     }
   }
 #pragma unroll
-  for (int m = 0; m < D; m++)
+  for (int m = 0; m < D; m++) {
+    const bool three = rr == 0 && cl + m == 0;
 #pragma unroll
-    for (int p = 0; p < 3; p++)
-      if (m <= D - 1 - k) ((uint4 *)sh.dq[base + 3 * m + p])[lane] = d[m][p];
+    for (int p = 0; p < 3; p++) {
+      // a unit's passes beyond its second exist only for U(0, 0)
+      if (m > D - 1 - k || (p == 2 && !three)) continue;
+      const int q = three ? (p == 0 ? LfConeShared<P, D>::EXTRA : base + 2 * m + p - 1) : base + 2 * m + p;
+      ((uint4 *)sh.dq[q])[lane] = d[m][p];
+    }
+  }
   cone_to_lds<P, D, 64, 1>(top, lq, vt, sh.buf, ly0, lx0, lane);
 #pragma unroll
   for (int q = 0; q <= D; q++) {
@@ -325,7 +337,7 @@ __global__ __launch_bounds__(64 * D) void k_lf_cone(uint8_t *i0, uint8_t *i1, ui
   for (int m = 0; m < D; m++) {
     const int uc = cl + m;
     if (row_on && m <= D - 1 - k && uc >= -1 && !(uc == -1 && rr < 1)) {
-      cone_unit<P, D>(sh, base + 3 * m, uc == -1, rr, uc == -1 ? 0 : uc, r, c, C, SW, SH, lane, bd);
+      cone_unit<P, D>(sh, base + 2 * m, uc == -1, rr, uc == -1 ? 0 : uc, r, c, C, SW, SH, lane, bd);
       if (m < D - 1 && uc >= 0 && writer) {
         cone_store<P, D, 64, 1>(out, stride, LfRect{rr * SH - 8, 8, uc * SW, rr >= 1}, lq, sh.buf, ly0, lx0, lane);
         cone_store<P, D, 64, NS>(out, stride, LfRect{rr * SH, SH - (rr + 1 < R ? 8 : 0), uc * SW, true}, lq, sh.buf,
@@ -369,8 +381,10 @@ void launch_lf_cone(hipStream_t s, const LfOne &in, const Planes &out, int depth
       go(k_lf_cone<false, 2>, 2);
     else if (depth == 3)
       go(k_lf_cone<false, 3>, 3);
-    else
+    else if (depth == 4)
       go(k_lf_cone<false, 4>, 4);
+    else
+      go(k_lf_cone<false, 5>, 5);
   }
 }
 

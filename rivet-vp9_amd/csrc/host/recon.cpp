@@ -651,12 +651,12 @@ int vp9r_gpu_new(int device, vp9r_gpu **out) {
   // VP9R_INTRA=groups: intra frames as (superblock, plane) groups
   const char *im = getenv("VP9R_INTRA");
   g->intra_blocks = !(im && strcmp(im, "groups") == 0);
-  // VP9R_LF=levels: one loop filter level per launch throughout; =2, 3
-  // or 4: at most that many per launch
+  // VP9R_LF=levels: one loop filter level per launch throughout; =2 .. 5:
+  // at most that many per launch
   const char *lm = getenv("VP9R_LF");
   if (lm && strcmp(lm, "levels") == 0)
     g->lf_depth = 1;
-  else if (lm && lm[0] >= '2' && lm[0] <= '4' && !lm[1])
+  else if (lm && lm[0] >= '2' && lm[0] <= '5' && !lm[1])
     g->lf_depth = lm[0] - '0';
   e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->copy_stream, hipStreamNonBlocking);
@@ -1227,9 +1227,9 @@ size_t wave_schedule(const Prepared &p, bool recon, bool lf, size_t &n_int, std:
 // the frame store and a second copy of its planes, and the last must write
 // the frame store.  The split minimises the launches' summed time, taken
 // from 1080p measurements (microseconds from one launch's start to the
-// next's, by depth 1 .. 4); max_depth 1 keeps one level per launch.
+// next's, by depth 1 .. 5); max_depth 1 keeps one level per launch.
 LfPlan lf_tail_plan(size_t m, int max_depth) {
-  static const double cost[5] = {0.0, 10.6, 15.7, 21.0, 26.5};
+  static const double cost[6] = {0.0, 10.8, 15.6, 19.0, 23.3, 28.1};
   LfPlan best;
   best.singles = m;
   if (max_depth < 2 || m < 4) return best;
@@ -1578,7 +1578,7 @@ int launch_batch(vp9r_gpu *g, const BatchFrame *fr, int n_all, bool recon, bool 
       vp9g::Planes tmp{};
       LfPlan plan;
       if (n == 1 && !gt && lf && g->lf_depth > 1 && n_wave > n_wave_intra) {
-        plan = lf_tail_plan(n_wave - n_wave_intra, std::min(g->lf_depth, hbd ? 3 : 4));
+        plan = lf_tail_plan(n_wave - n_wave_intra, std::min(g->lf_depth, hbd ? 3 : 5));
         if (!plan.depths.empty()) {
           const vp9r_gpu_frame *cf = f[0]->cur;
           if (cf->bytes > g->lf_tmp.cap) {
@@ -1978,7 +1978,7 @@ int vp9r_gpu_prepared_schedule(const vp9r_prepared *pr, uint32_t out[4]) {
   std::vector<int> lf_at;
   const size_t n_wave = vp9r::wave_schedule(pr->p, true, pr->p.hdr.lf_level != 0, n_int, lf_at);
   const vp9r::LfPlan plan =
-      vp9r::lf_tail_plan(n_wave - n_int, std::min(vp9r::kLfDepthDefault, pr->p.hdr.bit_depth > 8 ? 3 : 4));
+      vp9r::lf_tail_plan(n_wave - n_int, std::min(vp9r::kLfDepthDefault, pr->p.hdr.bit_depth > 8 ? 3 : 5));
   out[0] = (uint32_t)n_int;
   out[1] = (uint32_t)(plan.singles + plan.depths.size());
   out[2] = (uint32_t)lf_at.size();

@@ -130,7 +130,7 @@ hipError_t seam_end(vp9r_gpu *g);
 // The launches of a single frame's m loop filter levels after its intra
 // levels: `singles` one-level launches, then launches of depths[i] levels
 // (an even number of them), at most max_depth each.
-constexpr int kLfDepthDefault = 4;  // 3 at 10/12 bits (the cone's LDS)
+constexpr int kLfDepthDefault = 5;  // 3 at 10/12 bits (the cone's LDS)
 struct LfPlan {
   size_t singles = 0;
   std::vector<int> depths;
