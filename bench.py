@@ -1014,10 +1014,15 @@ def main():
     dom = max(range(4), key=lambda s: stage_ms[s])
     dname = STAGES[dom]
     achieved = stages[dname]["GB_per_s"] or 0.0
-    kern = {"loop_filter": "vp9g::k_lf_cone<false, 4>", "inter_pred": "vp9g::k_mc<false>",
+    # the filter tail's deepest cone (five levels at 8 bits), else four
+    kern = {"loop_filter": "vp9g::k_lf_cone<false, 5>", "inter_pred": "vp9g::k_prep_mc<false, 1>",
             "inter_residual": "vp9g::k_itx_inter<false>"}.get(dname)
+    traffic = profile_traffic(kern) if kern else None
+    if traffic is None and dname == "loop_filter":
+        kern = "vp9g::k_lf_cone<false, 4>"
+        traffic = profile_traffic(kern)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profile_traffic(kern) if kern else None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": STAGE_KERNELS[dom], "stage": dname,
                 "traffic_source": "profiles/recon_1080p_moving_rects.md: FETCH_SIZE and WRITE_SIZE per dispatch, "
                                   "scaled by a device copy's counts; HBM bytes per launch"}

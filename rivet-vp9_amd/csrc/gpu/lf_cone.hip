@@ -1,4 +1,4 @@
-// The loop filter's unit levels D at a time (D = 2, 3 or 4; the levels
+// The loop filter's unit levels D at a time (D = 2 .. 5; the levels
 // after the last intra level of a frame).
 //
 // The unit wavefront of lf.hip runs one level per launch: unit U(r, c) =
